@@ -1,0 +1,45 @@
+# Recipe for oracle/_ref/: the reference's own steady-transport hot-path sources, compiled in
+# place from the read-only reference tree, linked with the one-rank harness of this directory.
+# Used only in the development container (the reference is not present anywhere else).
+#   make -f oracle/ref.mk REF=/root/reference
+# Strict IEEE flags (the reference Makefile uses -Ofast; fast-math would only perturb the last
+# bits of the golden vectors).
+REF     ?= /root/reference
+MPI     ?= /opt/conda
+HERE    := $(dir $(abspath $(lastword $(MAKEFILE_LIST))))
+OUT     := $(HERE)_ref
+CF      := $(REF)/00_CommonFiles
+ST      := $(REF)/01_SS_Transport_Equation/2D/01_CommonFiles
+CFLAGS  := -O2 -fno-fast-math -ffp-contract=off -w -DSSTranspEquation2D -I$(ST) -I$(MPI)/include
+
+ST_SRC := $(ST)/Fill_LM.c $(ST)/Permutation_of_LM.c $(ST)/csr_Initialization.c \
+          $(ST)/Build_K_F_SUPG.c $(ST)/F_assembly.c $(ST)/CAU_ShockCapture.c \
+          $(ST)/h_shock_2sqrtArea.c $(ST)/ebe_assembly.c $(ST)/csr_assembly.c $(ST)/setzeros.c
+CF_SRC := $(CF)/Allocation_Operations/mycalloc.c \
+          $(CF)/BLAS_Operations/daxpy.c $(CF)/BLAS_Operations/dcopy.c $(CF)/BLAS_Operations/ddot.c \
+          $(CF)/BLAS_Operations/dscal.c $(CF)/BLAS_Operations/dzero.c \
+          $(CF)/MatrixVector_Operations/EBE/ebemv.c $(CF)/MatrixVector_Operations/CSR/csrmv.c \
+          $(CF)/Reordering/reordering.c $(CF)/Reordering/symrcm.c \
+          $(CF)/Solvers_and_Preconditioners/NO_precond.c \
+          $(CF)/Solvers_and_Preconditioners/NO_precond_setup.c \
+          $(CF)/Solvers_and_Preconditioners/CSR/ILUp_precond.c \
+          $(CF)/Solvers_and_Preconditioners/CSR/ILUp_precond_setup.c \
+          $(CF)/Solvers_and_Preconditioners/pgmres.c
+
+all: $(OUT)/st_harness $(OUT)/meshGenerator
+
+# The structured mesh generator is a self-contained C program.
+$(OUT)/meshGenerator: $(REF)/Mesh_Generator/Structured_Mesh/meshGenerator.c
+	mkdir -p $(OUT)
+	gcc -O2 -w -o $@ $< -lm
+
+# mc73_fiedler (HSL_MC73, Spectral ordering only) is referenced by reordering.c but never
+# called with SYMRCM/NOT; its prebuilt objects are not used, so the symbol stays unresolved.
+$(OUT)/st_harness: $(HERE)ref_harness_st.c $(ST_SRC) $(CF_SRC)
+	mkdir -p $(OUT)
+	gcc $(CFLAGS) -o $@ $(HERE)ref_harness_st.c $(ST_SRC) $(CF_SRC) \
+	    -L$(MPI)/lib -Wl,-rpath,$(MPI)/lib -lmpi -lm \
+	    -Wl,--unresolved-symbols=ignore-in-object-files
+
+clean:
+	rm -rf $(OUT)

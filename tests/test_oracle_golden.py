@@ -1,0 +1,149 @@
+"""Pin the CPU oracle against fixtures written by the reference's own code (tests/golden).
+
+Both sides are strict-IEEE builds (no fast-math, no FMA contraction) that follow the same
+operation order, so everything -- integer structures, element matrices, ILU factors, GMRES
+iteration counts and solutions, the CAU fixed-point sequence -- must match bit for bit.
+"""
+import numpy as np
+import pytest
+
+import golden_cases as G
+
+CASES = G.names()
+
+
+def same(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def setup(O, d):
+    h = d["header"]
+    nn, nel, ebe, reorder = int(h[0]), int(h[1]), int(h[2]), int(h[3])
+    invP, neq = O.number_unknowns(d["type"])
+    lm = O.fill_lm(d["ien"], d["type"], invP, neq)
+    pa = G.problem_args(d)
+    pb = O.make_problem(k=pa["k"], bx=pa["bx"], by=pa["by"], shock=pa["shock"])
+    return nn, nel, ebe, reorder, invP, neq, lm, pb
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_mesh_matches_reference_generator(oracle, name):
+    d = G.load(name)
+    m = d["meta"]
+    x, y, ien = oracle.mesh_structured(m["M"], m["N"], right=(m["cut"] == "right"))
+    assert np.array_equal(ien, d["ien"])
+    # the generator prints coordinates with %.14lf
+    assert np.max(np.abs(x - d["x"])) < 1e-14 and np.max(np.abs(y - d["y"])) < 1e-14
+    assert np.array_equal(oracle.unit_square_types(d["x"], d["y"]), d["type"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_structure_bit_exact(oracle, name):
+    d = G.load(name)
+    nn, nel, ebe, reorder, invP, neq, lm, pb = setup(oracle, d)
+    assert np.array_equal(lm, d["ref_lm"])
+    ro = "SYMRCM" if reorder else "NOT"
+    if ebe:
+        Perm, invPerm = oracle.ebe_permutation(d["ien"], d["type"], invP, neq, reorder=ro)
+        assert np.array_equal(Perm, d["ref_Perm"])
+        assert np.array_equal(invPerm, d["ref_invPerm"])
+        assert np.array_equal(invP, d["ref_invP_id"])
+    else:
+        c = oracle.csr_init(lm, 3, neq, reorder=ro, type_=d["type"], invP=invP)
+        assert np.array_equal(c["IA"], d["ref_IA"])
+        assert np.array_equal(c["JA"], d["ref_JA"])
+        assert np.array_equal(c["perm"], d["ref_perm"])
+        assert np.array_equal(c["scheme"], d["ref_scheme"])
+        assert np.array_equal(c["invP"], d["ref_invP_id"])
+
+
+def _first_system(oracle, d):
+    nn, nel, ebe, reorder, invP, neq, lm, pb = setup(oracle, d)
+    ro = "SYMRCM" if reorder else "NOT"
+    out = dict(neq=neq, ebe=ebe, lm=lm, pb=pb)
+    if ebe:
+        Perm, invPerm = oracle.ebe_permutation(d["ien"], d["type"], invP, neq, reorder=ro)
+        out.update(invP=invP, Perm=Perm, invPerm=invPerm)
+    else:
+        c = oracle.csr_init(lm, 3, neq, reorder=ro, type_=d["type"], invP=invP)
+        out.update(invP=c["invP"], IA=c["IA"], JA=c["JA"], scheme=c["scheme"], nnz=c["nnz"])
+    Ke, F = oracle.build_transport_supg(d["x"], d["y"], d["type"], out["invP"], d["ien"], d["ubc"],
+                                        np.zeros(neq), pb, neq)
+    out.update(Ke=Ke, F=F)
+    if not ebe:
+        out["AA"] = oracle.csr_assemble(out["scheme"], Ke, out["nnz"])
+    return out
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_assembly_and_matvec(oracle, name):
+    d = G.load(name)
+    s = _first_system(oracle, d)
+    assert same(s["F"], d["ref_F"])
+    pv = np.zeros(s["neq"])
+    t1 = d["type"] == 1
+    pv[s["invP"][t1]] = d["probe"][t1]
+    if s["ebe"]:
+        assert same(s["Ke"], d["ref_A"])
+        q = oracle.ebemv(s["lm"], s["Perm"], s["invPerm"], s["Ke"], pv, s["neq"])
+    else:
+        assert same(s["AA"], d["ref_AA"])
+        q = oracle.csrmv(s["IA"], s["JA"], s["AA"], pv)
+    assert same(q, d["ref_mv"])
+
+
+@pytest.mark.parametrize("name", [n for n in CASES if "_csr_" in n])
+def test_ilu0_factors_and_apply(oracle, name):
+    d = G.load(name)
+    s = _first_system(oracle, d)
+    LU, Dinv, rc = oracle.ilu0_factor(s["IA"], s["JA"], s["AA"])
+    assert rc == 0
+    IA, JA = s["IA"], s["JA"]
+    row = np.repeat(np.arange(s["neq"]), np.diff(IA))
+    lower, upper = JA < row, JA > row
+    assert same(LU[:-1][lower], d["ref_ILU_L"])
+    assert same(LU[:-1][upper], d["ref_ILU_U"])
+    assert same(Dinv, d["ref_ILU_D"])
+    assert same(oracle.ilu0_apply(IA, JA, LU, Dinv, s["F"]), d["ref_MinvF"])
+
+
+def _solve_sequence(oracle, d):
+    """Process.c:29-86 (steady fixed point) driven through the oracle primitives."""
+    h = d["header"]
+    kmax, lmax, nl = int(h[5]), int(h[6]), int(h[7])
+    nltol = float(d["params"][10])
+    s = _first_system(oracle, d)
+    neq = s["neq"]
+    u = np.zeros(neq)
+    iters, X1 = [], None
+    normdiff, normu = nltol + 1, 1.0 / nltol
+    while normdiff > nltol * normu and len(iters) < nl:
+        Ke, F = oracle.build_transport_supg(d["x"], d["y"], d["type"], s["invP"], d["ien"], d["ubc"],
+                                            u, s["pb"], neq)
+        if s["ebe"]:
+            S = oracle.make_system(1, neq, lm=s["lm"], Perm=s["Perm"], invPerm=s["invPerm"], A=Ke,
+                                   nel=len(d["ien"]) // 3)
+            B = F
+        else:
+            AA = oracle.csr_assemble(s["scheme"], Ke, s["nnz"])
+            LU, Dinv, rc = oracle.ilu0_factor(s["IA"], s["JA"], AA)
+            B = oracle.ilu0_apply(s["IA"], s["JA"], LU, Dinv, F)
+            S = oracle.make_system(0, neq, IA=s["IA"], JA=s["JA"], AA=AA,
+                                   ilu=(s["IA"], s["JA"], LU, Dinv))
+        uold = u.copy()
+        u, it, rho, cyc = oracle.pgmres(S, B, kmax=kmax, lmax=lmax, tol=1e-8)
+        iters.append(it)
+        if X1 is None:
+            X1 = u.copy()
+        normdiff = np.sqrt(oracle.ddot(uold - u, uold - u))
+        normu = np.sqrt(oracle.ddot(u, u))
+    return iters, X1, u
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_gmres_iterations_and_solutions(oracle, name):
+    d = G.load(name)
+    iters, X1, u = _solve_sequence(oracle, d)
+    assert iters == d["ref_iters"].tolist()
+    assert same(X1, d["ref_X1"])
+    assert same(u, d["ref_u"])
