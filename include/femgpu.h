@@ -1,0 +1,239 @@
+/*
+ * femgpu.h -- C ABI of libfemgpu.so, the MI355X (gfx950) implementation of the linear-solve hot
+ * path of PARALLEL_FEM_CODES (steady/transient SUPG transport, NDOF = 1, plus block EBE).
+ *
+ * Plain C types only: ints, doubles, floats, raw host or device pointers and opaque handles.
+ * Every function returns FEMGPU_OK (0) or a negative status; femgpu_last_error() describes the
+ * last failure of the calling thread. Device pointers ("d_" prefix) are HIP device memory on the
+ * context's device; host pointers are ordinary memory. Indices are 0-based, one rank
+ * (neq_bef = 0, NEQ = neq): vectors have neq entries plus, where the reference uses it, the
+ * Dirichlet sink at index NEQ.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   ebemv / ebemvNDOF3 / ebemvNDOF4 / csrmv  00_CommonFiles/MatrixVector_Operations/matvec.h:3-18
+ *   pgmres                                   00_CommonFiles/Solvers_and_Preconditioners/solvers.h:14
+ *   ILUp_precond / ILUp_precond_setup        00_CommonFiles/Solvers_and_Preconditioners/preconditioners.h:30,37
+ *   ddot / daxpy / dcopy / dscal / dzero     00_CommonFiles/BLAS_Operations/ourBLAS.h:1-5
+ *   Fill_LM / csr_Initialization / Permutation_of_LM
+ *                                            01_SS_Transport_Equation/2D/01_CommonFiles/SSTranspEquation.h:170-221
+ *   Build_K_F_SUPG / Build_M_K_R_SUPG        SSTranspEquation.h:203, 02_Transport_Equation/.../TranspEquation.h
+ * The reference-signature shims that forward to this ABI live in parallel_fem_codes_amd/compat/.
+ */
+#ifndef FEMGPU_H
+#define FEMGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEMGPU_OK 0
+#define FEMGPU_ERR_ARG -1         /* bad argument (size, null pointer, unknown option) */
+#define FEMGPU_ERR_HIP -2         /* HIP runtime failure */
+#define FEMGPU_ERR_ZERO_PIVOT -3  /* ILU(0) met a zero pivot (the reference exits, ILUp_precond_setup.c:155-164) */
+#define FEMGPU_ERR_NOMEM -4       /* device allocation failed */
+#define FEMGPU_ERR_STATE -5       /* object used before it was set up (e.g. apply before factor) */
+#define FEMGPU_ERR_TIMEOUT -6     /* a bounded device-side wait expired (sweep scheduler) */
+
+const char *femgpu_last_error(void);
+const char *femgpu_version(void);
+
+/* =========================================================== integer structure work (host) */
+
+/* Unknown numbering of Parallel_Preprocess with reordering NOT and one part
+ * (Parallel_Preprocess/Preprocess.c:37-53): running count of Type-1 dofs, -1 for Dirichlet.
+ * type, invP_id: nnodes*ndof node-major. Returns neq (>= 0) or a negative status. */
+int femgpu_number_unknowns(int nnodes, int ndof, const int *type, int *invP_id);
+
+/* Fill_LM (01_SS_Transport_Equation/2D/01_CommonFiles/Fill_LM.c:3-27;
+ * 04_SS_NavierStokes_Equations/2D/01_CommonFiles/Fill_LM.c:3-31 for ndof > 1). */
+int femgpu_fill_lm(int nel, int ndof, int NEQ, const int *ien, const int *type,
+                   const int *invP_id, int *lm);
+
+/* Nonzero count of the CSR block csr_Initialization builds from lm (csr_Initialization.c:36-99).
+ * s = 3*ndof. Returns nnz or a negative status. */
+int femgpu_csr_nnz(int nel, int s, int neq, const int *lm);
+
+/* csr_Initialization for one rank (01_SS_Transport_Equation/2D/01_CommonFiles/csr_Initialization.c:5-571):
+ * IA[neq+1], JA[nnz], scheme[nel*s*s] (slot of each element entry, nnz = dummy), perm[neq]
+ * (new -> old). reorder: 0 = NOT, 1 = SYMRCM (Spectral needs HSL_MC73 and is not provided).
+ * invP_id of Type-1 dofs is renumbered; lm too when permute_lm != 0 (the transient driver's
+ * variant, 02_Transport_Equation/2D/01_CommonFiles/csr_Initialization.c:452-454). */
+int femgpu_csr_init(int nel, int s, int neq, int *lm, int reorder, int permute_lm,
+                    int ndofs_total, const int *type, int *invP_id, int *IA, int *JA, int *scheme,
+                    int *perm);
+
+/* REORDERING_SYMRCM (00_CommonFiles/Reordering/symrcm.c:4-237). perm[new] = old. */
+int femgpu_symrcm(int n, const int *IA, const int *JA, int *perm);
+
+/* Permutation_of_LM (01_SS_Transport_Equation/2D/01_CommonFiles/Permutation_of_LM.c:37-105):
+ * Perm/invPerm of NEQ+1 entries, Perm[NEQ] = invPerm[NEQ] = NEQ. */
+int femgpu_ebe_permutation(int nel, int NEQ, const int *ien, const int *type, const int *invP_id,
+                           int reorder, int *Perm, int *invPerm);
+
+/* Greedy element colouring (no reference counterpart: it replaces the serial scatter of
+ * ebemv.c:40-55): elements in index order take the smallest colour not used by an element that
+ * shares an unknown (lm entry < NEQ). Returns the number of colours, or a negative status. */
+int femgpu_ebe_colour(int nel, int s, int NEQ, const int *lm, int *colour);
+
+/* Dependency levels of the ILU(0) sweeps (no reference counterpart: the reference sweeps
+ * serially, ILUp_precond.c:26-44): lower != 0: level(i) = 1 + max level(j), j < i in row i,
+ * else 0; lower == 0: the same over j > i from the last row. Returns depth (max level + 1). */
+int femgpu_ilu_levels(int n, const int *IA, const int *JA, int lower, int *level);
+
+/* =========================================================== device context */
+
+typedef struct femgpu_ctx femgpu_ctx;
+
+/* stream: a hipStream_t (cast to void*) or NULL for the device's null stream. */
+int femgpu_create(int device, void *stream, femgpu_ctx **out);
+int femgpu_destroy(femgpu_ctx *ctx);
+int femgpu_sync(femgpu_ctx *ctx);
+
+/* =========================================================== CSR operator + ILU(0) */
+
+typedef struct femgpu_csr femgpu_csr;
+
+/* Upload a CSR structure (host IA/JA, rows sorted) and prepare the level-scheduled ILU(0)
+ * sweeps. Values are set separately. */
+int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA, const int *JA,
+                      femgpu_csr **out);
+int femgpu_csr_destroy(femgpu_csr *A);
+/* AA: nnz values (the reference's dummy slot AA[nnz] is not needed); on_device selects whether
+ * AA is a host or a device pointer. Also refreshes the FP32 copy. */
+int femgpu_csr_set_values(femgpu_csr *A, const double *AA, int on_device);
+/* Device pointer of the FP64 value array (nnz+1 entries, last = dummy slot), for in-place
+ * assembly; call femgpu_csr_values_changed afterwards to refresh the FP32 copy. */
+int femgpu_csr_values_ptr(femgpu_csr *A, double **d_AA);
+int femgpu_csr_values_changed(femgpu_csr *A);
+/* Sweep scheduler statistics: depth of the lower/upper dependency DAGs and row chunk count. */
+int femgpu_csr_sweep_info(femgpu_csr *A, int *depth_lower, int *depth_upper, int *nchunks);
+
+/* csrmv (CSR/csrmv.c:4-65), local block: Q = A P. FP64 and FP32 (FP32 copy of the values). */
+int femgpu_csrmv(femgpu_csr *A, const double *d_P, double *d_Q);
+int femgpu_csrmv_f32(femgpu_csr *A, const float *d_P, float *d_Q);
+
+/* ILU(0) numeric factorisation of the current values (ILUp_precond_setup.c:67-168 with p = 0),
+ * level-scheduled on the device; also builds the FP32 factors. */
+int femgpu_ilu0_factor(femgpu_csr *A);
+/* Factor on the host from the current values and upload (same arithmetic, serial order). */
+int femgpu_ilu0_factor_host(femgpu_csr *A);
+/* ILUp_precond (CSR/ILUp_precond.c:4-47): z = U^-1 L^-1 p; d_p == d_z allowed. */
+int femgpu_ilu0_apply(femgpu_csr *A, const double *d_p, double *d_z);
+int femgpu_ilu0_apply_f32(femgpu_csr *A, const float *d_p, float *d_z);
+/* Copy the factors to the host: LU[nnz] in A's layout (L multipliers below, U above the
+ * diagonal), Dinv[n] = 1/pivot. */
+int femgpu_ilu0_get(femgpu_csr *A, double *LU, double *Dinv);
+
+/* =========================================================== EBE operator */
+
+typedef struct femgpu_ebe femgpu_ebe;
+
+/* Element-by-element operator with s x s blocks (s = 3 transport, 9 Navier-Stokes, 12 Euler),
+ * lm[nel*s] host array with Dirichlet sink NEQ. Colours the elements for an atomics-free scatter. */
+int femgpu_ebe_create(femgpu_ctx *ctx, int nel, int s, int NEQ, const int *lm, femgpu_ebe **out);
+int femgpu_ebe_destroy(femgpu_ebe *E);
+/* A: nel*s*s row-major element matrices (ebe_assembly.c layout); host or device pointer. */
+int femgpu_ebe_set_values(femgpu_ebe *E, const double *A, int on_device);
+int femgpu_ebe_ncolours(femgpu_ebe *E);
+/* ebemv / ebemvNDOF3 / ebemvNDOF4: Q[0..NEQ) = sum_e A_e P|_e with P[NEQ] read as 0. */
+int femgpu_ebemv(femgpu_ebe *E, const double *d_P, double *d_Q);
+
+/* =========================================================== BLAS on device vectors */
+
+int femgpu_ddot(femgpu_ctx *ctx, int n, const double *d_x, const double *d_y, double *result);
+int femgpu_daxpy(femgpu_ctx *ctx, int n, double a, const double *d_x, double *d_y);
+int femgpu_dscal(femgpu_ctx *ctx, int n, double a, double *d_x);
+int femgpu_dcopy(femgpu_ctx *ctx, int n, const double *d_x, double *d_y);
+int femgpu_dzero(femgpu_ctx *ctx, int n, double *d_x);
+
+/* =========================================================== solvers */
+
+typedef struct {
+    femgpu_csr *csr;   /* operator, when non-null */
+    femgpu_ebe *ebe;   /* operator, when csr is null */
+    femgpu_csr *ilu;   /* factored ILU(0) preconditioner (left), or NULL for none */
+} femgpu_system;
+
+typedef struct {
+    int iterations;    /* Arnoldi steps (Parameters->iterations increment in pgmres.c:184) */
+    int cycles;        /* restart cycles */
+    int inner_iterations; /* mixed precision: total FP32 Arnoldi steps */
+    int status;        /* 0 converged, 1 iteration cap reached */
+    double rho;        /* final Givens estimate of the (preconditioned) residual norm */
+    double normb;      /* norm of the right-hand side the stopping test refers to */
+    double true_rel_residual;  /* ||F - A x|| / ||F|| in FP64 (mixed solver; 0 otherwise) */
+    double prec_rel_residual;  /* ||M^-1(F - A x)|| / ||M^-1 F|| in FP64 (mixed solver) */
+} femgpu_solve_info;
+
+/* pgmres (00_CommonFiles/Solvers_and_Preconditioners/pgmres.c:5-200): restarted, left
+ * preconditioned GMRES with modified Gram-Schmidt and Givens rotations; kmax-1 Arnoldi steps per
+ * cycle, at most lmax cycles, X zeroed on entry, stop when |e_{i+1}| <= tol*||B||. B is the
+ * already preconditioned right-hand side (as precond_setup leaves it). FP64 throughout. */
+int femgpu_pgmres(femgpu_ctx *ctx, const femgpu_system *sys, int kmax, int lmax, double tol,
+                  const double *d_B, double *d_X, femgpu_solve_info *info);
+
+typedef struct {
+    int inner_kmax;        /* basis size of the FP32 inner GMRES (kmax-1 steps per cycle) */
+    int inner_cycles;      /* inner restart cycles per refinement step */
+    double inner_tol;      /* relative tolerance of each inner solve (preconditioned norm) */
+    int max_refinements;   /* outer FP64 refinement steps */
+    int sweep_mode;        /* FP32 ILU application: 0 exact level-scheduled sweeps,
+                              k > 0: k Jacobi sweeps per triangular factor */
+    int pad;
+} femgpu_mixed_opts;
+
+/* Mixed precision: FP64 iterative refinement (residual F - A x, update x += d, Givens and least
+ * squares in FP64) around an FP32 left-ILU(0)-preconditioned inner GMRES. Stops when
+ * ||F - A x|| <= tol * ||F|| (FP64-verified). F is the raw (unpreconditioned) right-hand side.
+ * Requires sys->csr and sys->ilu. No reference counterpart. */
+int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *sys, const femgpu_mixed_opts *opt,
+                       double tol, const double *d_F, double *d_X, femgpu_solve_info *info);
+
+/* PCG following Solvers_and_Preconditioners/pcg.c:4-93 (dead code in the reference): stops when
+ * gamma < tol^2 gamma0. Returns the iteration count in info->iterations. */
+int femgpu_pcg(femgpu_ctx *ctx, const femgpu_system *sys, int itmax, double tol,
+               const double *d_B, double *d_X, femgpu_solve_info *info);
+
+/* =========================================================== transport assembly (NDOF = 1) */
+
+typedef struct {
+    double k;        /* Condutivity() (kx = ky) */
+    double gamma;    /* Reaction() */
+    double f0;       /* Font(): constant source */
+    double bx[3];    /* Velocity(): Be_x = bx[0] + bx[1]*XB + bx[2]*YB at the element centroid */
+    double by[3];    /*             Be_y = by[0] + by[1]*XB + by[2]*YB */
+    int shock;       /* 0: no shock capture (Eu = 0); 1: CAU with h_shock = sqrt|2A| */
+    int pad;
+} femgpu_transport_problem;
+
+typedef struct femgpu_mesh femgpu_mesh;
+
+/* Device-resident mesh for the assembly kernels: node coordinates, Type, invP_id (unknown id in
+ * the solve numbering), Dirichlet values ubc (upresc at the node), connectivity ien[nel*3] and
+ * element map lm[nel*3] (sink NEQ). Builds the gather maps of the RHS and, when scheme is
+ * non-null, of the CSR values (scheme[nel*9] from femgpu_csr_init, nnz slots). */
+int femgpu_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, const double *x,
+                       const double *y, const int *type, const int *invP_id, const double *ubc,
+                       const int *ien, const int *lm, const int *scheme, int nnz,
+                       femgpu_mesh **out);
+int femgpu_mesh_destroy(femgpu_mesh *M);
+
+/* Build_K_F_SUPG (01_SS_Transport_Equation/2D/01_CommonFiles/Build_K_F_SUPG.c:4-187 with
+ * F_assembly.c, CAU_ShockCapture.c, h_shock_2sqrtArea.c): element matrices into d_Ke[nel*9]
+ * (may be NULL when only CSR values are wanted), RHS into d_F[neq] (F_assembly order), CSR values
+ * into d_AA[nnz+1] (csr_assembly order) when d_AA is non-null. d_u: current iterate (neq). */
+int femgpu_assemble_transport(femgpu_mesh *M, const femgpu_transport_problem *pb,
+                              const double *d_u, double *d_Ke, double *d_F, double *d_AA);
+
+/* Build_M_K_R_SUPG (02_Transport_Equation/2D/01_CommonFiles/Build_M_K_R_SUPG.c:6-250 with
+ * eval_U_dU.c and its CAU): Ae = Me + alpha*dt*Ke into d_Ae (nullable) / d_AA (nullable),
+ * residual into d_R[NEQ+1] (R[NEQ] = 0). d_u, d_du: current u and du (neq). */
+int femgpu_assemble_transport_transient(femgpu_mesh *M, const femgpu_transport_problem *pb,
+                                        double alpha, double dt, const double *d_u,
+                                        const double *d_du, double *d_Ae, double *d_R,
+                                        double *d_AA);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

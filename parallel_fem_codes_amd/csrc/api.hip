@@ -1,0 +1,474 @@
+// C ABI of libfemgpu.so: contexts, device-resident CSR/EBE operators, ILU(0), BLAS and PCG.
+// Solvers are in krylov.hip, assembly in assembly.hip, integer structure work in structure.cpp.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "device_util.h"
+
+namespace fg {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line)
+{
+    set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    return FEMGPU_ERR_HIP;
+}
+
+int check_device_error(femgpu_ctx *ctx)
+{
+    int e = 0;
+    FG_HIP(hipMemcpy(&e, ctx->err.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (e == 0) return FEMGPU_OK;
+    FG_HIP(hipMemset(ctx->err.p, 0, sizeof(int)));
+    if (e == 2) {
+        set_error("ILU(0): zero pivot");
+        return FEMGPU_ERR_ZERO_PIVOT;
+    }
+    if (e == 3) {
+        set_error("sweep scheduler: ticket out of range (a previous sweep was aborted)");
+        return FEMGPU_ERR_STATE;
+    }
+    set_error("sweep scheduler: a bounded wait expired");
+    return FEMGPU_ERR_TIMEOUT;
+}
+
+__global__ void k_axpy(int n, double a, const double *__restrict__ x, double *__restrict__ y)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] += a * x[i];
+}
+
+__global__ void k_scal(int n, double a, double *__restrict__ x)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = a * x[i];
+}
+
+static int vgrid(int n) { return std::max(1, std::min(4096, (n + 255) / 256)); }
+
+// Host ILU(0) with the serial order of ILUp (ILUp_precond_setup.c:84-166), p = 0.
+static int host_ilu0(int n, const std::vector<int> &IA, const std::vector<int> &JA, const std::vector<int> &diag,
+                     std::vector<double> &LU, std::vector<double> &Dinv)
+{
+    std::vector<int> jw(n, -1);
+    for (int i = 0; i < n; ++i) {
+        for (int p = IA[i]; p < IA[i + 1]; ++p) jw[JA[p]] = p;
+        double d = LU[diag[i]];
+        for (int j = IA[i]; j < diag[i]; ++j) {
+            const int jrow = JA[j];
+            LU[j] *= Dinv[jrow];
+            const double l = LU[j];
+            for (int k = diag[jrow] + 1; k < IA[jrow + 1]; ++k) {
+                const int col = JA[k];
+                if (col == i) {
+                    d -= l * LU[k];
+                } else if (jw[col] >= 0) {
+                    LU[jw[col]] -= l * LU[k];
+                }
+            }
+        }
+        for (int p = IA[i]; p < IA[i + 1]; ++p) jw[JA[p]] = -1;
+        if (d == 0.0) {
+            set_error("ILU(0): zero pivot in row %d", i);
+            return FEMGPU_ERR_ZERO_PIVOT;
+        }
+        LU[diag[i]] = d;
+        Dinv[i] = 1.0 / d;
+    }
+    return FEMGPU_OK;
+}
+
+}  // namespace fg
+
+using namespace fg;
+
+extern "C" const char *femgpu_last_error(void) { return g_err; }
+extern "C" const char *femgpu_version(void) { return "femgpu 0.1 (gfx950)"; }
+
+extern "C" int femgpu_create(int device, void *stream, femgpu_ctx **out)
+{
+    FG_CHECK(out, FEMGPU_ERR_ARG, "femgpu_create: null output");
+    int count = 0;
+    FG_HIP(hipGetDeviceCount(&count));
+    FG_CHECK(device >= 0 && device < count, FEMGPU_ERR_ARG, "femgpu_create: device %d of %d", device, count);
+    FG_HIP(hipSetDevice(device));
+    auto *ctx = new femgpu_ctx();
+    ctx->device = device;
+    ctx->stream = static_cast<hipStream_t>(stream);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    int rc = ctx->partials.alloc(kReduceBlocks);
+    if (!rc) rc = ctx->counter.alloc(4);
+    if (!rc) rc = ctx->scalars.alloc(16);
+    if (!rc) rc = ctx->err.alloc(4);
+    if (!rc && hipMemset(ctx->counter.p, 0, 4 * sizeof(unsigned)) != hipSuccess) rc = FEMGPU_ERR_HIP;
+    if (!rc && hipMemset(ctx->err.p, 0, 4 * sizeof(int)) != hipSuccess) rc = FEMGPU_ERR_HIP;
+    if (!rc && hipHostMalloc((void **)&ctx->h_pinned, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        rc = FEMGPU_ERR_HIP;
+    // per-step solver status written by the device: 256 ints + 256 doubles
+    void *st = nullptr;
+    if (!rc && hipHostMalloc(&st, 256 * sizeof(int) + 256 * sizeof(double), hipHostMallocMapped) != hipSuccess)
+        rc = FEMGPU_ERR_HIP;
+    if (!rc) {
+        ctx->h_status = static_cast<volatile int *>(st);
+        memset(st, 0, 256 * sizeof(int) + 256 * sizeof(double));
+        void *dp = nullptr;
+        if (hipHostGetDevicePointer(&dp, st, 0) != hipSuccess) rc = FEMGPU_ERR_HIP;
+        ctx->d_status = static_cast<int *>(dp);
+    }
+    if (rc) {
+        set_error("femgpu_create: device allocations failed");
+        femgpu_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_destroy(femgpu_ctx *ctx)
+{
+    if (!ctx) return FEMGPU_OK;
+    if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+    if (ctx->h_status) (void)hipHostFree(const_cast<int *>(ctx->h_status));
+    delete ctx;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_sync(femgpu_ctx *ctx)
+{
+    FG_CHECK(ctx, FEMGPU_ERR_ARG, "femgpu_sync: null context");
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    return check_device_error(ctx);
+}
+
+// ------------------------------------------------------------------------------- CSR
+
+extern "C" int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA, const int *JA, femgpu_csr **out)
+{
+    FG_CHECK(ctx && out && n >= 0 && nnz >= 0 && IA && JA, FEMGPU_ERR_ARG, "femgpu_csr_create: bad arguments");
+    FG_CHECK(IA[0] == 0 && IA[n] == nnz, FEMGPU_ERR_ARG, "femgpu_csr_create: IA[0]=%d IA[n]=%d nnz=%d", IA[0], IA[n], nnz);
+    std::vector<int> diag(n);
+    for (int i = 0; i < n; ++i) {
+        FG_CHECK(IA[i] <= IA[i + 1], FEMGPU_ERR_ARG, "femgpu_csr_create: IA not monotone at row %d", i);
+        diag[i] = -1;
+        for (int p = IA[i]; p < IA[i + 1]; ++p) {
+            FG_CHECK(JA[p] >= 0 && JA[p] < n, FEMGPU_ERR_ARG, "femgpu_csr_create: column %d out of range", JA[p]);
+            FG_CHECK(p == IA[i] || JA[p] > JA[p - 1], FEMGPU_ERR_ARG, "femgpu_csr_create: row %d not sorted", i);
+            if (JA[p] == i) diag[i] = p;
+        }
+        FG_CHECK(diag[i] >= 0, FEMGPU_ERR_ARG, "femgpu_csr_create: row %d has no diagonal entry", i);
+    }
+    auto *A = new femgpu_csr();
+    A->ctx = ctx;
+    A->n = n;
+    A->nnz = nnz;
+    A->hIA.assign(IA, IA + n + 1);
+    A->hJA.assign(JA, JA + nnz);
+    hipStream_t s = ctx->stream;
+    int rc = A->IA.upload(IA, n + 1, s);
+    if (!rc) rc = A->JA.upload(JA, nnz, s);
+    if (!rc) rc = A->diag.upload(diag.data(), n, s);
+    if (!rc) rc = A->AA.alloc(static_cast<size_t>(nnz) + 1);
+    if (!rc) rc = A->AA32.alloc(static_cast<size_t>(nnz) + 1);
+    if (!rc) rc = A->LU.alloc(static_cast<size_t>(nnz) + 1);
+    if (!rc) rc = A->Dinv.alloc(static_cast<size_t>(n) + 1);
+    if (!rc) rc = A->LU32.alloc(static_cast<size_t>(nnz) + 1);
+    if (!rc) rc = A->Dinv32.alloc(static_cast<size_t>(n) + 1);
+    // Rows per chunk: large enough that a chunk covers a few dependency levels of a banded
+    // (reordered) mesh matrix, small enough to spread over the CUs.
+    const int R = n > 200000 ? 1024 : 256;
+    if (!rc) rc = build_sweep_plan(ctx, n, IA, JA, true, R, A->lower);
+    if (!rc) rc = build_sweep_plan(ctx, n, IA, JA, false, R, A->upper);
+    if (!rc && hipMemsetAsync(A->AA.p, 0, sizeof(double) * (nnz + 1), s) != hipSuccess) rc = FEMGPU_ERR_HIP;
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
+    if (rc) {
+        delete A;
+        return rc;
+    }
+    *out = A;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_csr_destroy(femgpu_csr *A)
+{
+    delete A;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_csr_set_values(femgpu_csr *A, const double *AA, int on_device)
+{
+    FG_CHECK(A && AA, FEMGPU_ERR_ARG, "femgpu_csr_set_values: bad arguments");
+    femgpu_ctx *ctx = A->ctx;
+    FG_HIP(hipMemcpyAsync(A->AA.p, AA, sizeof(double) * A->nnz,
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+    A->factored = false;
+    return femgpu_csr_values_changed(A);
+}
+
+extern "C" int femgpu_csr_values_ptr(femgpu_csr *A, double **d_AA)
+{
+    FG_CHECK(A && d_AA, FEMGPU_ERR_ARG, "femgpu_csr_values_ptr: bad arguments");
+    *d_AA = A->AA.p;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_csr_values_changed(femgpu_csr *A)
+{
+    FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_csr_values_changed: null");
+    A->factored = false;
+    return launch_to_f32(A->ctx, A->AA.p, A->AA32.p, A->nnz);
+}
+
+extern "C" int femgpu_csr_sweep_info(femgpu_csr *A, int *dl, int *du, int *nchunks)
+{
+    FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_csr_sweep_info: null");
+    if (dl) *dl = A->lower.depth;
+    if (du) *du = A->upper.depth;
+    if (nchunks) *nchunks = A->lower.nchunks;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_csrmv(femgpu_csr *A, const double *d_P, double *d_Q)
+{
+    FG_CHECK(A && d_P && d_Q && d_P != d_Q, FEMGPU_ERR_ARG, "femgpu_csrmv: bad arguments");
+    return launch_spmv64(A->ctx, A, d_P, nullptr, nullptr, d_Q, nullptr);
+}
+
+extern "C" int femgpu_csrmv_f32(femgpu_csr *A, const float *d_P, float *d_Q)
+{
+    FG_CHECK(A && d_P && d_Q && d_P != d_Q, FEMGPU_ERR_ARG, "femgpu_csrmv_f32: bad arguments");
+    return launch_spmv32(A->ctx, A, d_P, nullptr, nullptr, d_Q, nullptr);
+}
+
+extern "C" int femgpu_ilu0_factor(femgpu_csr *A)
+{
+    FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_ilu0_factor: null");
+    FG_TRY(launch_ilu_factor(A->ctx, A));
+    FG_HIP(hipStreamSynchronize(A->ctx->stream));
+    FG_TRY(check_device_error(A->ctx));
+    A->factored = true;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_ilu0_factor_host(femgpu_csr *A)
+{
+    FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_ilu0_factor_host: null");
+    femgpu_ctx *ctx = A->ctx;
+    std::vector<double> LU(A->nnz), Dinv(A->n);
+    std::vector<int> diag(A->n);
+    FG_HIP(hipMemcpyAsync(LU.data(), A->AA.p, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, ctx->stream));
+    FG_HIP(hipMemcpyAsync(diag.data(), A->diag.p, sizeof(int) * A->n, hipMemcpyDeviceToHost, ctx->stream));
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    FG_TRY(host_ilu0(A->n, A->hIA, A->hJA, diag, LU, Dinv));
+    FG_HIP(hipMemcpyAsync(A->LU.p, LU.data(), sizeof(double) * A->nnz, hipMemcpyHostToDevice, ctx->stream));
+    FG_HIP(hipMemcpyAsync(A->Dinv.p, Dinv.data(), sizeof(double) * A->n, hipMemcpyHostToDevice, ctx->stream));
+    FG_TRY(launch_to_f32(ctx, A->LU.p, A->LU32.p, A->nnz));
+    FG_TRY(launch_to_f32(ctx, A->Dinv.p, A->Dinv32.p, A->n));
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    A->factored = true;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_ilu0_apply(femgpu_csr *A, const double *d_p, double *d_z)
+{
+    FG_CHECK(A && d_p && d_z, FEMGPU_ERR_ARG, "femgpu_ilu0_apply: bad arguments");
+    FG_CHECK(A->factored, FEMGPU_ERR_STATE, "femgpu_ilu0_apply: not factored");
+    return launch_sweep64(A->ctx, A, d_p, d_z, nullptr);
+}
+
+extern "C" int femgpu_ilu0_apply_f32(femgpu_csr *A, const float *d_p, float *d_z)
+{
+    FG_CHECK(A && d_p && d_z, FEMGPU_ERR_ARG, "femgpu_ilu0_apply_f32: bad arguments");
+    FG_CHECK(A->factored, FEMGPU_ERR_STATE, "femgpu_ilu0_apply_f32: not factored");
+    return launch_sweep32(A->ctx, A, d_p, d_z, nullptr);
+}
+
+extern "C" int femgpu_ilu0_get(femgpu_csr *A, double *LU, double *Dinv)
+{
+    FG_CHECK(A && LU && Dinv, FEMGPU_ERR_ARG, "femgpu_ilu0_get: bad arguments");
+    FG_CHECK(A->factored, FEMGPU_ERR_STATE, "femgpu_ilu0_get: not factored");
+    FG_HIP(hipMemcpyAsync(LU, A->LU.p, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, A->ctx->stream));
+    FG_HIP(hipMemcpyAsync(Dinv, A->Dinv.p, sizeof(double) * A->n, hipMemcpyDeviceToHost, A->ctx->stream));
+    FG_HIP(hipStreamSynchronize(A->ctx->stream));
+    return FEMGPU_OK;
+}
+
+// ------------------------------------------------------------------------------- EBE
+
+extern "C" int femgpu_ebe_create(femgpu_ctx *ctx, int nel, int s, int NEQ, const int *lm, femgpu_ebe **out)
+{
+    FG_CHECK(ctx && out && nel >= 0 && NEQ >= 0 && lm && (s == 3 || s == 9 || s == 12), FEMGPU_ERR_ARG,
+             "femgpu_ebe_create: bad arguments (s must be 3, 9 or 12)");
+    for (size_t q = 0; q < static_cast<size_t>(nel) * s; ++q)
+        FG_CHECK(lm[q] >= 0 && lm[q] <= NEQ, FEMGPU_ERR_ARG, "femgpu_ebe_create: lm entry %d out of [0, NEQ]", lm[q]);
+    std::vector<int> colour(nel);
+    const int nc = femgpu_ebe_colour(nel, s, NEQ, lm, colour.data());
+    if (nc < 0) return nc;
+    auto *E = new femgpu_ebe();
+    E->ctx = ctx;
+    E->nel = nel;
+    E->s = s;
+    E->NEQ = NEQ;
+    E->ncolours = nc;
+    E->colour_ptr.assign(nc + 1, 0);
+    for (int e = 0; e < nel; ++e) E->colour_ptr[colour[e] + 1]++;
+    for (int c = 0; c < nc; ++c) E->colour_ptr[c + 1] += E->colour_ptr[c];
+    std::vector<int> fill(E->colour_ptr.begin(), E->colour_ptr.end() - 1), elem(nel);
+    for (int e = 0; e < nel; ++e) elem[fill[colour[e]]++] = e;
+    std::vector<int> lmc(static_cast<size_t>(nel) * s);
+    for (int k = 0; k < nel; ++k)
+        for (int a = 0; a < s; ++a) lmc[static_cast<size_t>(k) * s + a] = lm[static_cast<size_t>(elem[k]) * s + a];
+    int rc = E->elem.upload(elem.data(), nel, ctx->stream);
+    if (!rc) rc = E->lm.upload(lmc.data(), lmc.size(), ctx->stream);
+    if (!rc) rc = E->A.alloc(static_cast<size_t>(nel) * s * s);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = FEMGPU_ERR_HIP;
+    if (rc) {
+        delete E;
+        return rc;
+    }
+    *out = E;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_ebe_destroy(femgpu_ebe *E)
+{
+    delete E;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_ebe_ncolours(femgpu_ebe *E) { return E ? E->ncolours : FEMGPU_ERR_ARG; }
+
+extern "C" int femgpu_ebe_set_values(femgpu_ebe *E, const double *A, int on_device)
+{
+    FG_CHECK(E && A, FEMGPU_ERR_ARG, "femgpu_ebe_set_values: bad arguments");
+    femgpu_ctx *ctx = E->ctx;
+    const size_t total = static_cast<size_t>(E->nel) * E->s * E->s;
+    if (on_device) return launch_ebe_gather_values(ctx, E, A);
+    DevBuf<double> tmp;
+    FG_TRY(tmp.upload(A, total, ctx->stream));
+    FG_TRY(launch_ebe_gather_values(ctx, E, tmp.p));
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_ebemv(femgpu_ebe *E, const double *d_P, double *d_Q)
+{
+    FG_CHECK(E && d_P && d_Q && d_P != d_Q, FEMGPU_ERR_ARG, "femgpu_ebemv: bad arguments");
+    return launch_ebemv(E->ctx, E, d_P, d_Q);
+}
+
+// ------------------------------------------------------------------------------- BLAS
+
+extern "C" int femgpu_ddot(femgpu_ctx *ctx, int n, const double *x, const double *y, double *result)
+{
+    FG_CHECK(ctx && n >= 0 && x && y && result, FEMGPU_ERR_ARG, "femgpu_ddot: bad arguments");
+    FG_TRY(dot64(ctx, n, x, y, ctx->scalars.p));
+    FG_HIP(hipMemcpyAsync(result, ctx->scalars.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_daxpy(femgpu_ctx *ctx, int n, double a, const double *x, double *y)
+{
+    FG_CHECK(ctx && n >= 0 && x && y, FEMGPU_ERR_ARG, "femgpu_daxpy: bad arguments");
+    if (n) k_axpy<<<vgrid(n), 256, 0, ctx->stream>>>(n, a, x, y);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_dscal(femgpu_ctx *ctx, int n, double a, double *x)
+{
+    FG_CHECK(ctx && n >= 0 && x, FEMGPU_ERR_ARG, "femgpu_dscal: bad arguments");
+    if (n) k_scal<<<vgrid(n), 256, 0, ctx->stream>>>(n, a, x);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_dcopy(femgpu_ctx *ctx, int n, const double *x, double *y)
+{
+    FG_CHECK(ctx && n >= 0 && x && y, FEMGPU_ERR_ARG, "femgpu_dcopy: bad arguments");
+    if (n) FG_HIP(hipMemcpyAsync(y, x, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_dzero(femgpu_ctx *ctx, int n, double *x)
+{
+    FG_CHECK(ctx && n >= 0 && x, FEMGPU_ERR_ARG, "femgpu_dzero: bad arguments");
+    if (n) k_scal<<<vgrid(n), 256, 0, ctx->stream>>>(n, 0.0, x);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+// ------------------------------------------------------------------------------- PCG
+
+extern "C" int femgpu_pcg(femgpu_ctx *ctx, const femgpu_system *S, int itmax, double tol, const double *d_B,
+                          double *d_X, femgpu_solve_info *info)
+{
+    FG_CHECK(ctx && S && S->csr && d_B && d_X, FEMGPU_ERR_ARG, "femgpu_pcg: needs a CSR operator");
+    FG_CHECK(!S->ilu || S->ilu->factored, FEMGPU_ERR_STATE, "femgpu_pcg: ILU(0) not factored");
+    const int n = S->csr->n;
+    DevBuf<double> P, Q, R, Z;
+    FG_TRY(P.alloc(n));
+    FG_TRY(Q.alloc(n));
+    FG_TRY(R.alloc(n));
+    FG_TRY(Z.alloc(n));
+    auto dot = [&](const double *a, const double *b, double &out) -> int {
+        FG_TRY(dot64(ctx, n, a, b, ctx->scalars.p));
+        FG_HIP(hipMemcpyAsync(&out, ctx->scalars.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        FG_HIP(hipStreamSynchronize(ctx->stream));
+        return FEMGPU_OK;
+    };
+    auto precond = [&](double *z) -> int {
+        if (S->ilu) FG_TRY(launch_sweep64(ctx, S->ilu, z, z, nullptr));
+        return FEMGPU_OK;
+    };
+    // pcg.c:31-47
+    FG_TRY(femgpu_dzero(ctx, n, d_X));
+    FG_TRY(femgpu_dcopy(ctx, n, d_B, R.p));
+    FG_TRY(femgpu_dcopy(ctx, n, R.p, Z.p));
+    FG_TRY(precond(Z.p));
+    FG_TRY(femgpu_dcopy(ctx, n, Z.p, P.p));
+    double gnew = 0;
+    FG_TRY(dot(R.p, Z.p, gnew));
+    const double r0 = gnew;
+    int iter = 0;
+    if (!(gnew < tol * tol * r0)) {
+        for (iter = 0; iter < itmax; ++iter) {
+            // pcg.c:49-80
+            FG_TRY(launch_spmv64(ctx, S->csr, P.p, nullptr, nullptr, Q.p, nullptr));
+            double tau = 0;
+            FG_TRY(dot(P.p, Q.p, tau));
+            const double alpha = gnew / tau;
+            FG_TRY(femgpu_daxpy(ctx, n, alpha, P.p, d_X));
+            FG_TRY(femgpu_daxpy(ctx, n, -alpha, Q.p, R.p));
+            FG_TRY(femgpu_dcopy(ctx, n, R.p, Z.p));
+            FG_TRY(precond(Z.p));
+            const double gold = gnew;
+            FG_TRY(dot(R.p, Z.p, gnew));
+            if (gnew < tol * tol * r0) break;
+            FG_TRY(femgpu_dscal(ctx, n, gnew / gold, P.p));
+            FG_TRY(femgpu_daxpy(ctx, n, 1.0, Z.p, P.p));
+        }
+    }
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    FG_TRY(check_device_error(ctx));
+    if (info) {
+        memset(info, 0, sizeof(*info));
+        info->iterations = iter;
+        info->rho = std::sqrt(std::fabs(gnew));
+        info->normb = std::sqrt(std::fabs(r0));
+        info->status = (gnew < tol * tol * r0) ? 0 : 1;
+    }
+    return FEMGPU_OK;
+}

@@ -1,0 +1,428 @@
+// Per-element SUPG assembly of the transport problems on the device.
+//
+// One thread per element evaluates the closed-form 3x3 SUPG(+CAU) element matrix and loads
+// (Build_K_F_SUPG.c:47-168 steady, Build_M_K_R_SUPG.c:47-184 transient). Global accumulation is a
+// gather instead of the reference's serial scatter: every CSR slot and every right-hand-side row
+// owns the list of its element contributions in element order, so each sum is formed in exactly
+// the reference's order (csr_assembly.c:24-33, F_assembly.c:5-27, Build_M_K_R_SUPG.c:173-184)
+// with no atomics. Contraction is off in this file: the element arithmetic follows the
+// reference's expression order operation for operation.
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+
+namespace fg {
+
+struct ElemIn {
+    const double *x, *y, *ubc, *u, *du;
+    const int *type, *invP, *ien;
+    femgpu_transport_problem pb;
+    double alpha_dt;
+};
+
+struct SupgElem {
+    double y23, y31, y12, x32, x13, x21, Area, invArea, tau, Be_x, Be_y, fe, feb;
+    double ke[3][3];
+};
+
+__device__ __forceinline__ void supg_geometry(SupgElem &E, const double X[3], const double Y[3],
+                                              const femgpu_transport_problem &pb)
+{
+#pragma clang fp contract(off)
+    const double third = 1.0 / 3.0;
+    E.y23 = Y[1] - Y[2];
+    E.y31 = Y[2] - Y[0];
+    E.y12 = Y[0] - Y[1];
+    E.x32 = X[2] - X[1];
+    E.x13 = X[0] - X[2];
+    E.x21 = X[1] - X[0];
+    const double TwoA = E.x21 * E.y31 - E.x13 * E.y12;
+    E.Area = 0.5 * TwoA;
+    E.invArea = 1.0 / E.Area;
+    const double h = sqrt(fabs(TwoA));
+    const double XB = third * (X[0] + X[1] + X[2]);
+    const double YB = third * (Y[0] + Y[1] + Y[2]);
+    E.Be_x = pb.bx[0] + pb.bx[1] * XB + pb.bx[2] * YB;
+    E.Be_y = pb.by[0] + pb.by[1] * XB + pb.by[2] * YB;
+    const double norm = sqrt(E.Be_x * E.Be_x + E.Be_y * E.Be_y);
+    E.tau = (norm > 0) ? 0.5 * h / norm : 0;
+    E.fe = pb.f0;
+    E.feb = third * (E.fe + E.fe + E.fe);
+}
+
+// CAU shock-capture coefficient (CAU_ShockCapture.c:3-31; the transient form adds dueb).
+__device__ __forceinline__ double cau_eu(const SupgElem &E, const femgpu_transport_problem &pb, double ue1,
+                                         double ue2, double ue3, double ueb, double dueb)
+{
+#pragma clang fp contract(off)
+    const double third = 1. / 3.;
+    const double kx = pb.k, ky = pb.k, gamma = pb.gamma, invArea = E.invArea;
+    const double Be_x = E.Be_x, Be_y = E.Be_y;
+    const double BetaGradu = 0.5 * invArea *
+                             (ue1 * (Be_x * E.y23 + Be_y * E.x32) + ue2 * (Be_x * E.y31 + Be_y * E.x13) +
+                              ue3 * (Be_x * E.y12 + Be_y * E.x21));
+    const double Ru = dueb + BetaGradu + gamma * ueb - E.feb;
+    const double g0 = 0.5 * invArea * (E.y23 * ue1 + E.y31 * ue2 + E.y12 * ue3);
+    const double g1 = 0.5 * invArea * (E.x32 * ue1 + E.x13 * ue2 + E.x21 * ue3);
+    const double normGradu = sqrt(g0 * g0 + g1 * g1);
+    if (!(normGradu > 0)) return 0.0;
+    const double h = sqrt(fabs(2 * E.Area));
+    const double Be_xp = BetaGradu * g0 / (normGradu * normGradu);
+    const double Be_yp = BetaGradu * g1 / (normGradu * normGradu);
+    const double ktil_p = Be_xp * Be_xp * kx + Be_yp * Be_yp * ky;
+    const double normBe_p = sqrt(Be_xp * Be_xp + Be_yp * Be_yp);
+    const double Pee = 0.5 * h * normBe_p * normBe_p * normBe_p / ktil_p;
+    if (third * Pee < 0.7) return 0.5 * h * third * Pee * fabs(Ru) / normGradu;
+    return 0.5 * h * 0.7 * fabs(Ru) / normGradu;
+}
+
+__device__ __forceinline__ void supg_ke(SupgElem &E, const femgpu_transport_problem &pb, double Eu)
+{
+#pragma clang fp contract(off)
+    const double sixth = 1.0 / 6.0, twelfth = 1.0 / 12.0;
+    const double kx = pb.k, ky = pb.k, gamma = pb.gamma;
+    const double Be_x = E.Be_x, Be_y = E.Be_y, tau = E.tau, invArea = E.invArea, Area = E.Area;
+    const double y23 = E.y23, y31 = E.y31, y12 = E.y12, x32 = E.x32, x13 = E.x13, x21 = E.x21;
+    const double D12 = 0.25 * invArea * ((kx * y23 * y31) + (ky * x32 * x13));
+    const double D13 = 0.25 * invArea * ((kx * y23 * y12) + (ky * x32 * x21));
+    const double D23 = 0.25 * invArea * ((kx * y31 * y12) + (ky * x13 * x21));
+    const double C11 = sixth * (Be_x * y23 + Be_y * x32);
+    const double C12 = sixth * (Be_x * y31 + Be_y * x13);
+    const double C13 = sixth * (Be_x * y12 + Be_y * x21);
+    const double R11 = sixth * (gamma * Area);
+    const double R12 = twelfth * (gamma * Area);
+    const double Cpg12 = 0.25 * invArea * tau * (y23 * (Be_x * Be_x * y31 + Be_x * Be_y * x13) + x32 * (Be_x * Be_y * y31 + Be_y * Be_y * x13));
+    const double Cpg13 = 0.25 * invArea * tau * (y23 * (Be_x * Be_x * y12 + Be_x * Be_y * x21) + x32 * (Be_x * Be_y * y12 + Be_y * Be_y * x21));
+    const double Cpg23 = 0.25 * invArea * tau * (y31 * (Be_x * Be_x * y12 + Be_x * Be_y * x21) + x13 * (Be_x * Be_y * y12 + Be_y * Be_y * x21));
+    const double Rpg11 = sixth * tau * gamma * (Be_x * y23 + Be_y * x32);
+    const double Rpg21 = sixth * tau * gamma * (Be_x * y31 + Be_y * x13);
+    const double Rpg31 = sixth * tau * gamma * (Be_x * y12 + Be_y * x21);
+    const double Csc12 = Eu * 0.25 * invArea * (y23 * y31 + x32 * x13);
+    const double Csc13 = Eu * 0.25 * invArea * (y23 * y12 + x32 * x21);
+    const double Csc23 = Eu * 0.25 * invArea * (y31 * y12 + x13 * x21);
+    E.ke[0][0] = -(D12 + D13) + C11 - (Cpg12 + Cpg13) + R11 + Rpg11 - (Csc12 + Csc13);
+    E.ke[0][1] = D12 + C12 + Cpg12 + R12 + Rpg11 + Csc12;
+    E.ke[0][2] = D13 + C13 + Cpg13 + R12 + Rpg11 + Csc13;
+    E.ke[1][0] = D12 + C11 + Cpg12 + R12 + Rpg21 + Csc12;
+    E.ke[1][1] = -(D12 + D23) + C12 - (Cpg12 + Cpg23) + R11 + Rpg21 - (Csc12 + Csc23);
+    E.ke[1][2] = D23 + C13 + Cpg23 + R12 + Rpg21 + Csc23;
+    E.ke[2][0] = D13 + C11 + Cpg13 + R12 + Rpg31 + Csc13;
+    E.ke[2][1] = D23 + C12 + Cpg23 + R12 + Rpg31 + Csc23;
+    E.ke[2][2] = -(D13 + D23) + C13 - (Cpg13 + Cpg23) + R11 + Rpg31 - (Csc13 + Csc23);
+}
+
+__device__ __forceinline__ void elem_loads(const SupgElem &E, double Fe[3])
+{
+#pragma clang fp contract(off)
+    const double sixth = 1.0 / 6.0, twelfth = 1.0 / 12.0;
+    const double f = E.fe, fs = f + f + f;
+    Fe[0] = E.Area * twelfth * (2 * f + f + f) + sixth * E.tau * fs * (E.Be_x * E.y23 + E.Be_y * E.x32);
+    Fe[1] = E.Area * twelfth * (f + 2 * f + f) + sixth * E.tau * fs * (E.Be_x * E.y31 + E.Be_y * E.x13);
+    Fe[2] = E.Area * twelfth * (f + f + 2 * f) + sixth * E.tau * fs * (E.Be_x * E.y12 + E.Be_y * E.x21);
+}
+
+// Steady element kernel: Ke (9), loads fe (3), lifting terms lift[2a+t] = ke[a][b_t]*ubc[v_bt] for the
+// two other vertices b_t of a (ascending), mask = Dirichlet vertices.
+__global__ __launch_bounds__(256) void k_elem_steady(int nel, ElemIn in, double *__restrict__ Ke,
+                                                     double *__restrict__ fe, double *__restrict__ lift,
+                                                     int *__restrict__ mask)
+{
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= nel) return;
+    int v[3];
+    double X[3], Y[3], U[3];
+    int m = 0;
+    for (int a = 0; a < 3; ++a) {
+        v[a] = in.ien[3 * e + a];
+        X[a] = in.x[v[a]];
+        Y[a] = in.y[v[a]];
+        const int t = in.type[v[a]];
+        U[a] = (t == 0) ? in.ubc[v[a]] : in.u[in.invP[v[a]]];
+        if (t == 0) m |= 1 << a;
+    }
+    SupgElem E;
+    supg_geometry(E, X, Y, in.pb);
+    const double third = 1.0 / 3.0;
+    const double ueb = third * (U[0] + U[1] + U[2]);
+    const double Eu = in.pb.shock == 1 ? cau_eu(E, in.pb, U[0], U[1], U[2], ueb, 0.0) : 0.0;
+    supg_ke(E, in.pb, Eu);
+    double Fe[3];
+    elem_loads(E, Fe);
+    for (int a = 0; a < 3; ++a) {
+        fe[3 * e + a] = Fe[a];
+        int t = 0;
+        for (int b = 0; b < 3; ++b) {
+            if (b == a) continue;
+            lift[6 * e + 2 * a + t] = E.ke[a][b] * in.ubc[v[b]];
+            ++t;
+        }
+        for (int b = 0; b < 3; ++b) Ke[9 * static_cast<size_t>(e) + 3 * a + b] = E.ke[a][b];
+    }
+    mask[e] = m;
+}
+
+// Transient element kernel: Ae = Me + alpha*dt*Ke (9), loads fe (3), coupling terms
+// lift[3a+b] = me[a][b]*due[b] + ke[a][b]*ue[b] (Build_M_K_R_SUPG.c:148-184).
+__global__ __launch_bounds__(256) void k_elem_transient(int nel, ElemIn in, double *__restrict__ Ae,
+                                                        double *__restrict__ fe, double *__restrict__ lift)
+{
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= nel) return;
+    double X[3], Y[3], ue[3], due[3];
+    for (int a = 0; a < 3; ++a) {
+        const int v = in.ien[3 * e + a];
+        X[a] = in.x[v];
+        Y[a] = in.y[v];
+        if (in.type[v] == 0) {
+            ue[a] = in.ubc[v];
+            due[a] = 0.0;
+        } else {
+            ue[a] = in.u[in.invP[v]];
+            due[a] = in.du[in.invP[v]];
+        }
+    }
+    SupgElem E;
+    supg_geometry(E, X, Y, in.pb);
+    const double third = 1.0 / 3.0, sixth = 1.0 / 6.0, twelfth = 1.0 / 12.0;
+    const double ueb = third * (ue[0] + ue[1] + ue[2]);
+    const double dueb = third * (due[0] + due[1] + due[2]);
+    const double Eu = in.pb.shock == 1 ? cau_eu(E, in.pb, ue[0], ue[1], ue[2], ueb, dueb) : 0.0;
+    supg_ke(E, in.pb, Eu);
+    const double C[3] = {sixth * (E.Be_x * E.y23 + E.Be_y * E.x32), sixth * (E.Be_x * E.y31 + E.Be_y * E.x13),
+                         sixth * (E.Be_x * E.y12 + E.Be_y * E.x21)};
+    const double sixA = sixth * E.Area, twA = twelfth * E.Area;
+    double Fe[3];
+    elem_loads(E, Fe);
+    for (int a = 0; a < 3; ++a) {
+        fe[3 * e + a] = Fe[a];
+        for (int b = 0; b < 3; ++b) {
+            const double me = ((a == b) ? sixA : twA) + E.tau * C[a];
+            Ae[9 * static_cast<size_t>(e) + 3 * a + b] = me + in.alpha_dt * E.ke[a][b];
+            lift[9 * static_cast<size_t>(e) + 3 * a + b] = me * due[b] + E.ke[a][b] * ue[b];
+        }
+    }
+}
+
+// RHS gathers. rhs_src entries are e*3+a in element order.
+__global__ void k_rhs_steady(int neq, const int *__restrict__ ptr, const int *__restrict__ src,
+                             const double *__restrict__ fe, const double *__restrict__ lift,
+                             const int *__restrict__ mask, double *__restrict__ F)
+{
+#pragma clang fp contract(off)
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= neq) return;
+    double f = 0;
+    for (int q = ptr[I]; q < ptr[I + 1]; ++q) {
+        const int ea = src[q], e = ea / 3, a = ea % 3;
+        f += fe[ea];
+        const int m = mask[e];
+        int t = 0;
+        for (int b = 0; b < 3; ++b) {
+            if (b == a) continue;
+            if (m & (1 << b)) f -= lift[6 * e + 2 * a + t];
+            ++t;
+        }
+    }
+    F[I] = f;
+}
+
+__global__ void k_rhs_transient(int NEQ, const int *__restrict__ ptr, const int *__restrict__ src,
+                                const double *__restrict__ fe, const double *__restrict__ lift,
+                                double *__restrict__ R)
+{
+#pragma clang fp contract(off)
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I > NEQ) return;
+    if (I == NEQ) {
+        R[NEQ] = 0.0;
+        return;
+    }
+    double r = 0;
+    for (int q = ptr[I]; q < ptr[I + 1]; ++q) {
+        const int ea = src[q];
+        r += fe[ea];
+        const double *t = lift + 9 * static_cast<size_t>(ea / 3) + 3 * (ea % 3);
+        r -= t[0];
+        r -= t[1];
+        r -= t[2];
+    }
+    R[I] = r;
+}
+
+// CSR values: AA[slot] = sum of its element entries in element order; AA[nnz] = 0.
+__global__ void k_aa_gather(int nnz, const int *__restrict__ ptr, const int *__restrict__ src,
+                            const double *__restrict__ Ke, double *__restrict__ AA)
+{
+#pragma clang fp contract(off)
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nnz) return;
+    double v = 0;
+    if (s < nnz)
+        for (int q = ptr[s]; q < ptr[s + 1]; ++q) v += Ke[src[q]];
+    AA[s] = v;
+}
+
+static ElemIn elem_in(femgpu_mesh *M, const femgpu_transport_problem *pb, const double *u, const double *du,
+                      double alpha_dt)
+{
+    ElemIn in;
+    in.x = M->x.p;
+    in.y = M->y.p;
+    in.ubc = M->ubc.p;
+    in.u = u;
+    in.du = du;
+    in.type = M->type.p;
+    in.invP = M->invP.p;
+    in.ien = M->ien.p;
+    in.pb = *pb;
+    in.alpha_dt = alpha_dt;
+    return in;
+}
+
+// Counting sort of (key, source) pairs by key, keeping source order inside a key.
+static void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vector<int> &ptr,
+                    std::vector<int> &src)
+{
+    ptr.assign(static_cast<size_t>(nkeys) + 1, 0);
+    for (auto &p : pairs) ptr[p.first + 1]++;
+    for (int k = 0; k < nkeys; ++k) ptr[k + 1] += ptr[k];
+    src.resize(pairs.size());
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (auto &p : pairs) src[fill[p.first]++] = p.second;
+}
+
+}  // namespace fg
+
+using namespace fg;
+
+extern "C" int femgpu_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, const double *x, const double *y,
+                                  const int *type, const int *invP_id, const double *ubc, const int *ien,
+                                  const int *lm, const int *scheme, int nnz, femgpu_mesh **out)
+{
+    FG_CHECK(ctx && nnodes > 0 && nel >= 0 && neq >= 0 && x && y && type && invP_id && ubc && ien && out,
+             FEMGPU_ERR_ARG, "femgpu_mesh_create: bad arguments");
+    for (int v = 0; v < nnodes; ++v)
+        FG_CHECK(type[v] == 0 || type[v] == 1, FEMGPU_ERR_ARG, "femgpu_mesh_create: node %d has Type %d (one rank: 0 or 1)",
+                 v, type[v]);
+    for (size_t q = 0; q < static_cast<size_t>(nel) * 3; ++q)
+        FG_CHECK(ien[q] >= 0 && ien[q] < nnodes, FEMGPU_ERR_ARG, "femgpu_mesh_create: vertex %d out of range", ien[q]);
+    auto *M = new femgpu_mesh();
+    M->ctx = ctx;
+    M->nnodes = nnodes;
+    M->nel = nel;
+    M->neq = neq;
+    M->nnz = scheme ? nnz : 0;
+    hipStream_t s = ctx->stream;
+    int rc = FEMGPU_OK;
+    auto up = [&](auto &buf, auto *h, size_t n) {
+        if (rc == FEMGPU_OK) rc = buf.upload(h, n, s);
+    };
+    up(M->x, x, nnodes);
+    up(M->y, y, nnodes);
+    up(M->ubc, ubc, nnodes);
+    up(M->type, type, nnodes);
+    up(M->invP, invP_id, nnodes);
+    up(M->ien, ien, static_cast<size_t>(nel) * 3);
+    if (lm) up(M->lm, lm, static_cast<size_t>(nel) * 3);
+    // unknown -> element-vertex incidences in element order (Type-1 vertices only)
+    std::vector<std::pair<int, int>> pairs;
+    pairs.reserve(static_cast<size_t>(nel) * 3);
+    for (int e = 0; e < nel; ++e)
+        for (int a = 0; a < 3; ++a) {
+            const int v = ien[3 * e + a];
+            if (type[v] == 1) {
+                if (invP_id[v] < 0 || invP_id[v] >= neq) rc = FEMGPU_ERR_ARG;
+                else pairs.emplace_back(invP_id[v], 3 * e + a);
+            }
+        }
+    if (rc == FEMGPU_ERR_ARG) set_error("femgpu_mesh_create: invP_id out of range");
+    std::vector<int> ptr, src;
+    if (rc == FEMGPU_OK) {
+        csr_map(neq, pairs, ptr, src);
+        up(M->rhs_ptr, ptr.data(), ptr.size());
+        up(M->rhs_src, src.data(), src.size());
+    }
+    if (rc == FEMGPU_OK && scheme) {
+        pairs.clear();
+        for (int e = 0; e < nel; ++e)
+            for (int q = 0; q < 9; ++q) {
+                const int slot = scheme[9 * static_cast<size_t>(e) + q];
+                if (slot < 0 || slot > nnz) rc = FEMGPU_ERR_ARG;
+                else if (slot < nnz) pairs.emplace_back(slot, 9 * e + q);
+            }
+        if (rc == FEMGPU_ERR_ARG) set_error("femgpu_mesh_create: scheme slot out of range");
+        if (rc == FEMGPU_OK) {
+            csr_map(nnz, pairs, ptr, src);
+            up(M->aa_ptr, ptr.data(), ptr.size());
+            up(M->aa_src, src.data(), src.size());
+        }
+    }
+    if (rc == FEMGPU_OK) rc = M->fe.alloc(static_cast<size_t>(nel) * 3);
+    if (rc == FEMGPU_OK) rc = M->lift.alloc(static_cast<size_t>(nel) * 9);
+    if (rc == FEMGPU_OK) rc = M->mask.alloc(static_cast<size_t>(nel));
+    if (rc == FEMGPU_OK) rc = M->ke.alloc(static_cast<size_t>(nel) * 9);
+    if (rc == FEMGPU_OK && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
+    if (rc != FEMGPU_OK) {
+        delete M;
+        return rc;
+    }
+    *out = M;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_mesh_destroy(femgpu_mesh *M)
+{
+    delete M;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_assemble_transport(femgpu_mesh *M, const femgpu_transport_problem *pb, const double *d_u,
+                                         double *d_Ke, double *d_F, double *d_AA)
+{
+    FG_CHECK(M && pb && d_u && d_F, FEMGPU_ERR_ARG, "femgpu_assemble_transport: bad arguments");
+    FG_CHECK(!d_AA || M->nnz > 0 || M->aa_ptr.p, FEMGPU_ERR_STATE,
+             "femgpu_assemble_transport: mesh created without a CSR slot table");
+    femgpu_ctx *ctx = M->ctx;
+    double *Ke = d_Ke ? d_Ke : M->ke.p;
+    const ElemIn in = elem_in(M, pb, d_u, nullptr, 0.0);
+    if (M->nel > 0) {
+        k_elem_steady<<<(M->nel + 255) / 256, 256, 0, ctx->stream>>>(M->nel, in, Ke, M->fe.p, M->lift.p, M->mask.p);
+        FG_HIP(hipGetLastError());
+    }
+    if (M->neq > 0) {
+        k_rhs_steady<<<(M->neq + 255) / 256, 256, 0, ctx->stream>>>(M->neq, M->rhs_ptr.p, M->rhs_src.p, M->fe.p,
+                                                                    M->lift.p, M->mask.p, d_F);
+        FG_HIP(hipGetLastError());
+    }
+    if (d_AA) {
+        k_aa_gather<<<(M->nnz + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->nnz, M->aa_ptr.p, M->aa_src.p, Ke, d_AA);
+        FG_HIP(hipGetLastError());
+    }
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_assemble_transport_transient(femgpu_mesh *M, const femgpu_transport_problem *pb, double alpha,
+                                                   double dt, const double *d_u, const double *d_du, double *d_Ae,
+                                                   double *d_R, double *d_AA)
+{
+    FG_CHECK(M && pb && d_u && d_du && d_R, FEMGPU_ERR_ARG, "femgpu_assemble_transport_transient: bad arguments");
+    femgpu_ctx *ctx = M->ctx;
+    double *Ae = d_Ae ? d_Ae : M->ke.p;
+    const ElemIn in = elem_in(M, pb, d_u, d_du, alpha * dt);
+    if (M->nel > 0) {
+        k_elem_transient<<<(M->nel + 255) / 256, 256, 0, ctx->stream>>>(M->nel, in, Ae, M->fe.p, M->lift.p);
+        FG_HIP(hipGetLastError());
+    }
+    k_rhs_transient<<<(M->neq + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->neq, M->rhs_ptr.p, M->rhs_src.p, M->fe.p,
+                                                                       M->lift.p, d_R);
+    FG_HIP(hipGetLastError());
+    if (d_AA) {
+        FG_CHECK(M->aa_ptr.p, FEMGPU_ERR_STATE, "femgpu_assemble_transport_transient: no CSR slot table");
+        k_aa_gather<<<(M->nnz + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->nnz, M->aa_ptr.p, M->aa_src.p, Ae, d_AA);
+        FG_HIP(hipGetLastError());
+    }
+    return FEMGPU_OK;
+}

@@ -1,0 +1,163 @@
+// Internal declarations of libfemgpu.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/femgpu.h"
+
+namespace fg {
+
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define FG_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) return fg::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define FG_CHECK(cond, code, ...)         \
+    do {                                          \
+        if (!(cond)) {                            \
+            fg::set_error(__VA_ARGS__);           \
+            return (code);                        \
+        }                                         \
+    } while (0)
+
+#define FG_TRY(call)                  \
+    do {                              \
+        int r_ = (call);              \
+        if (r_ != FEMGPU_OK) return r_; \
+    } while (0)
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kReduceBlocks = 1024; // upper bound of blocks of the reduction kernels
+
+// Typed device allocation; frees on destruction.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        if (count <= n && p) return FEMGPU_OK;
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error("hipMalloc(%zu bytes) failed: %s", count * sizeof(T), hipGetErrorString(e));
+            return FEMGPU_ERR_NOMEM;
+        }
+        n = count;
+        return FEMGPU_OK;
+    }
+    int upload(const T *h, size_t count, hipStream_t s) {
+        FG_TRY(alloc(count));
+        if (count) FG_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+        return FEMGPU_OK;
+    }
+};
+
+// Level-scheduled sweep plan over contiguous row chunks (one workgroup per chunk, chunks taken
+// in order from a ticket counter, rows of a chunk grouped by dependency level).
+struct SweepPlan {
+    int n = 0, rows_per_chunk = 0, nchunks = 0, ngroups = 0, depth = 0;
+    DevBuf<int> order;      // rows of each chunk sorted by (level, row); n entries
+    DevBuf<int> grp_ptr;    // chunk -> first group (nchunks+1)
+    DevBuf<int> grp_start;  // group -> first position in order (ngroups+1)
+    DevBuf<int> grp_level;  // group -> level
+    DevBuf<int> dep;        // chunk -> farthest chunk it reads from (lower: lowest, upper: highest)
+    DevBuf<int> progress;   // chunk -> highest level completed (-1 before start); self-resetting
+    DevBuf<unsigned> ticket; // [0] next chunk ticket, [1] finished chunks; self-resetting
+};
+
+}  // namespace fg
+
+struct femgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 256;
+    fg::DevBuf<double> partials;   // per-block partial sums
+    fg::DevBuf<unsigned> counter;  // last-block detection of the reductions
+    fg::DevBuf<double> scalars;    // small device scalars (dot results)
+    fg::DevBuf<int> err;           // sticky device error flag (bounded waits)
+    double *h_pinned = nullptr;    // pinned host mirror for scalar results
+    // Solver workspaces (grown on demand, kept across calls).
+    fg::DevBuf<double> gm_basis, gm_small, gm_w, gm_r;
+    fg::DevBuf<float> gm_basis32, gm_w32;
+    fg::DevBuf<int> gm_flags;
+    volatile int *h_status = nullptr;  // pinned, device-visible solver status words
+    int *d_status = nullptr;
+};
+
+struct femgpu_csr {
+    femgpu_ctx *ctx = nullptr;
+    int n = 0, nnz = 0;
+    std::vector<int> hIA, hJA;
+    fg::DevBuf<int> IA, JA, diag;   // diag[i] = position of the diagonal entry of row i
+    fg::DevBuf<double> AA;          // nnz+1 (dummy slot nnz)
+    fg::DevBuf<float> AA32;
+    fg::DevBuf<double> LU, Dinv;    // ILU(0) factors in A's layout
+    fg::DevBuf<float> LU32, Dinv32;
+    bool factored = false;
+    fg::SweepPlan lower, upper;
+    int spmv_lanes = 8;             // lanes per row of the SpMV kernel
+};
+
+struct femgpu_ebe {
+    femgpu_ctx *ctx = nullptr;
+    int nel = 0, s = 0, NEQ = 0, ncolours = 0;
+    std::vector<int> colour_ptr;    // elements of colour c: [colour_ptr[c], colour_ptr[c+1])
+    fg::DevBuf<int> elem;           // colour-sorted position -> element id
+    fg::DevBuf<int> lm;             // lm in colour-sorted order
+    fg::DevBuf<double> A;           // element matrices in colour-sorted order
+};
+
+struct femgpu_mesh {
+    femgpu_ctx *ctx = nullptr;
+    int nnodes = 0, nel = 0, neq = 0, nnz = 0;
+    fg::DevBuf<double> x, y, ubc;
+    fg::DevBuf<int> type, invP, ien, lm;
+    fg::DevBuf<int> rhs_ptr, rhs_src;   // unknown -> (element*3 + a) contributions, element order
+    fg::DevBuf<int> aa_ptr, aa_src;     // CSR slot -> (element*9 + ab) contributions, element order
+    fg::DevBuf<double> fe, lift;        // per element: 3 loads, 9 coupling terms
+    fg::DevBuf<int> mask;               // per element: Dirichlet-vertex bit mask
+    fg::DevBuf<double> ke;              // element matrices when the caller passes none
+};
+
+namespace fg {
+
+// Host-side integer work (structure.cpp).
+int build_sweep_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, bool lower, int rows_per_chunk,
+                     SweepPlan &plan);
+
+// Launchers (kernels_*.hip).
+int launch_spmv64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *xs, double *xout,
+                  double *y, const int *skip);
+int launch_spmv32(femgpu_ctx *ctx, const femgpu_csr *A, const float *x, const double *xs, float *xout,
+                  float *y, const int *skip);
+int launch_residual64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *f, double *r);
+int launch_sweep64(femgpu_ctx *ctx, femgpu_csr *A, const double *b, double *x, const int *skip);
+int launch_sweep32(femgpu_ctx *ctx, femgpu_csr *A, const float *b, float *x, const int *skip);
+int launch_ilu_factor(femgpu_ctx *ctx, femgpu_csr *A);
+int launch_to_f32(femgpu_ctx *ctx, const double *src, float *dst, size_t n);
+int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *Q);
+int launch_ebe_gather_values(femgpu_ctx *ctx, femgpu_ebe *E, const double *A_elem_order);
+
+int dot64(femgpu_ctx *ctx, int n, const double *x, const double *y, double *d_out);
+int check_device_error(femgpu_ctx *ctx);
+
+}  // namespace fg

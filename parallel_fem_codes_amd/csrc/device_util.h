@@ -1,0 +1,50 @@
+// Device helpers shared by the kernels: 64-wide wave reductions, block reductions and the
+// last-block detection of single-pass grid reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fg {
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum over the block; the result is valid in every thread of wave 0 (and returned to all lanes
+// of wave 0). red must hold BS/64 entries.
+template <int BS, typename T>
+__device__ __forceinline__ T block_sum(T v, T *red)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        v = (l < BS / 64) ? red[l] : T(0);
+        v = wave_sum(v);
+    }
+    return v;
+}
+
+// Called by all threads after each block stored its partial: true in the block that finished
+// last (it then sees every partial). Release before the count, acquire after it.
+__device__ __forceinline__ bool last_block(unsigned *counter)
+{
+    __shared__ int s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return s_last != 0;
+}
+
+}  // namespace fg

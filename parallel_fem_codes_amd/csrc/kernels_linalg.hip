@@ -1,0 +1,501 @@
+// Sparse kernels of the hot path for gfx950: CSR SpMV (FP64/FP32), residual, the chunked
+// level-scheduled ILU(0) sweeps and numeric factorisation, the coloured EBE matvec, and the
+// reduction / vector kernels. All memory-bound: no MFMA (3x3..12x12 blocks and CSR rows are not
+// dense contractions).
+#include <climits>
+
+#include "common.h"
+#include "device_util.h"
+
+namespace fg {
+
+// ------------------------------------------------------------------------------- CSR SpMV
+// LPR lanes per row (a 64-wide wave covers 64/LPR consecutive rows, so the AA/JA reads of a wave
+// are one contiguous span); partial row sums combined with xor-shuffles inside the lane group.
+// The x operand is read as x[col]*xs with xs = *xs_ptr (1 when null): with xs = 1/h this fuses the
+// Arnoldi normalisation of pgmres.c:113 into the product, and when xout is set lane 0 also stores
+// the normalised basis entry xout[row] = x[row]*xs. A non-zero *skip makes the launch a no-op
+// (used by the solvers to drain steps enqueued after convergence).
+template <typename T, int LPR>
+__global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ IA, const int *__restrict__ JA,
+                                              const T *__restrict__ AA, const T *__restrict__ x,
+                                              const double *__restrict__ xs_ptr, T *__restrict__ xout,
+                                              T *__restrict__ y, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int gt = blockIdx.x * 256 + threadIdx.x;
+    const int row = gt / LPR, lane = gt % LPR;
+    if (row >= n) return;
+    const T xs = xs_ptr ? static_cast<T>(*xs_ptr) : T(1);
+    const int p0 = IA[row], p1 = IA[row + 1];
+    T acc = 0;
+    for (int p = p0 + lane; p < p1; p += LPR) acc += AA[p] * (x[JA[p]] * xs);
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+    if (lane == 0) {
+        y[row] = acc;
+        if (xout) xout[row] = x[row] * xs;
+    }
+}
+
+// r = f - A x (FP64 outer residual of the mixed solver).
+template <int LPR>
+__global__ __launch_bounds__(256) void k_residual(int n, const int *__restrict__ IA, const int *__restrict__ JA,
+                                                  const double *__restrict__ AA, const double *__restrict__ x,
+                                                  const double *__restrict__ f, double *__restrict__ r)
+{
+    const int gt = blockIdx.x * 256 + threadIdx.x;
+    const int row = gt / LPR, lane = gt % LPR;
+    if (row >= n) return;
+    const int p0 = IA[row], p1 = IA[row + 1];
+    double acc = 0;
+    for (int p = p0 + lane; p < p1; p += LPR) acc += AA[p] * x[JA[p]];
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+    if (lane == 0) r[row] = f[row] - acc;
+}
+
+template <typename T>
+static int spmv_dispatch(femgpu_ctx *ctx, const femgpu_csr *A, const T *AA, const T *x, const double *xs,
+                         T *xout, T *y, const int *skip)
+{
+    const int L = A->spmv_lanes;
+    const long threads = static_cast<long>(A->n) * L;
+    const int grid = static_cast<int>((threads + 255) / 256);
+    if (A->n == 0) return FEMGPU_OK;
+    switch (L) {
+    case 2: k_spmv<T, 2><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, AA, x, xs, xout, y, skip); break;
+    case 4: k_spmv<T, 4><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, AA, x, xs, xout, y, skip); break;
+    case 16: k_spmv<T, 16><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, AA, x, xs, xout, y, skip); break;
+    default: k_spmv<T, 8><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, AA, x, xs, xout, y, skip); break;
+    }
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+int launch_spmv64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *xs, double *xout,
+                  double *y, const int *skip)
+{
+    return spmv_dispatch<double>(ctx, A, A->AA.p, x, xs, xout, y, skip);
+}
+
+int launch_spmv32(femgpu_ctx *ctx, const femgpu_csr *A, const float *x, const double *xs, float *xout,
+                  float *y, const int *skip)
+{
+    return spmv_dispatch<float>(ctx, A, A->AA32.p, x, xs, xout, y, skip);
+}
+
+int launch_residual64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *f, double *r)
+{
+    if (A->n == 0) return FEMGPU_OK;
+    const int grid = static_cast<int>((static_cast<long>(A->n) * 8 + 255) / 256);
+    k_residual<8><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, A->AA.p, x, f, r);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+// ------------------------------------------------------------------------------- conversions
+__global__ void k_to_f32(const double *__restrict__ s, float *__restrict__ d, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = static_cast<float>(s[i]);
+}
+
+int launch_to_f32(femgpu_ctx *ctx, const double *src, float *dst, size_t n)
+{
+    if (!n) return FEMGPU_OK;
+    const int grid = static_cast<int>(std::min<size_t>((n + 255) / 256, 4096));
+    k_to_f32<<<grid, 256, 0, ctx->stream>>>(src, dst, n);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+// ------------------------------------------------------------------------------- sweep scheduler
+// One workgroup per row chunk. Chunks are claimed in sweep order from a ticket counter, so every
+// chunk a workgroup waits on was claimed earlier by a running (or finished) workgroup, and the
+// lowest unfinished chunk never waits: the waits always drain. Each chunk walks its level groups
+// in increasing level; before a group of level L it waits until every chunk it reads from has
+// published progress >= L-1 (all their rows of level < L done), and after the group it publishes
+// the level below its next group. The last chunk to finish restores ticket/progress for the next
+// launch, so no reset launch or memset is needed between sweeps (graph friendly).
+// Every wait is bounded (spin_limit); on expiry the sticky error flag is raised and the workgroup
+// stops waiting, so a scheduling bug reports FEMGPU_ERR_TIMEOUT instead of hanging the device.
+struct SweepArgs {
+    int n, R, nchunks;
+    const int *order, *grp_ptr, *grp_start, *grp_level, *dep;
+    int *progress;
+    unsigned *ticket;
+    int *err;
+    unsigned spin_limit;
+};
+
+__device__ __forceinline__ void sweep_wait(const SweepArgs &S, int c, bool lower, int need)
+{
+    // chunks read from: lower: [dep[c], c-1]; upper: [c+1, dep[c]]
+    const int far = S.dep[c];
+    const int ndep = lower ? (c - far) : (far - c);
+    if (threadIdx.x < ndep) {
+        for (int k = threadIdx.x; k < ndep; k += blockDim.x) {
+            const int dc = lower ? (c - 1 - k) : (c + 1 + k);
+            unsigned spins = 0;
+            while (__hip_atomic_load(&S.progress[dc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                if (++spins > S.spin_limit) {
+                    __hip_atomic_store(S.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                if (__hip_atomic_load(S.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        // the published rows of the chunks just observed become visible to this workgroup
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void sweep_publish(const SweepArgs &S, int c, int value)
+{
+    // every wave makes its own row stores visible at agent scope, then one store publishes them
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&S.progress[c], value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__device__ __forceinline__ void sweep_finish(const SweepArgs &S)
+{
+    __shared__ int s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned done = __hip_atomic_fetch_add(&S.ticket[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (done == static_cast<unsigned>(S.nchunks) - 1u);
+    }
+    __syncthreads();
+    if (s_last) {
+        // every other chunk has finished all its waits and stores: restore the initial state
+        for (int c = threadIdx.x; c < S.nchunks; c += blockDim.x)
+            __hip_atomic_store(&S.progress[c], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) {
+            __hip_atomic_store(&S.ticket[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&S.ticket[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Returns the chunk of this workgroup, or -1 if the ticket is out of range (a launch after an
+// aborted one): then the sticky error flag is raised and the workgroup does nothing.
+__device__ __forceinline__ int sweep_claim(const SweepArgs &S, bool lower)
+{
+    __shared__ int s_chunk;
+    if (threadIdx.x == 0) {
+        const unsigned t = __hip_atomic_fetch_add(&S.ticket[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t >= static_cast<unsigned>(S.nchunks)) {
+            __hip_atomic_store(S.err, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_chunk = -1;
+        } else {
+            s_chunk = lower ? static_cast<int>(t) : (S.nchunks - 1 - static_cast<int>(t));
+        }
+    }
+    __syncthreads();
+    return s_chunk;
+}
+
+constexpr int kSweepThreads = 256;
+
+// Triangular sweeps of ILUp_precond.c:26-44 over the ILU(0) factors stored in A's layout:
+//   lower: x[i] = b[i] - sum_{j<i} L_ij x[j]      (unit lower factor, ascending columns)
+//   upper: x[i] = (x[i] - sum_{j>i} U_ij x[j]) * Dinv[i]
+// Each row accumulates in the reference's order with contraction off, so the result is the
+// serial sweep's bit for bit. Rows of the own chunk are read back from LDS.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs S, const int *__restrict__ IA,
+                                                         const int *__restrict__ JA, const int *__restrict__ diag,
+                                                         const T *__restrict__ LU, const T *__restrict__ Dinv,
+                                                         const T *b, T *x, const int *__restrict__ skip)
+{
+#pragma clang fp contract(off)
+    if (skip && *skip) return;   // uniform: the flag was written by an earlier launch
+    extern __shared__ unsigned char smem_raw[];
+    T *xs = reinterpret_cast<T *>(smem_raw);
+    const int c = sweep_claim(S, LOWER);
+    if (c < 0) return;
+    const int r0 = c * S.R, r1 = min(S.n, r0 + S.R);
+    const int g0 = S.grp_ptr[c], g1 = S.grp_ptr[c + 1];
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&S.progress[c], S.grp_level[g0] - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int g = g0; g < g1; ++g) {
+        const int lev = S.grp_level[g];
+        if (lev > 0) sweep_wait(S, c, LOWER, lev - 1);
+        const int k1 = S.grp_start[g + 1];
+        for (int k = S.grp_start[g] + threadIdx.x; k < k1; k += blockDim.x) {
+            const int i = S.order[k];
+            const int d = diag[i];
+            T acc;
+            if (LOWER) {
+                acc = b[i];
+                for (int p = IA[i]; p < d; ++p) {
+                    const int col = JA[p];
+                    const T xv = (col >= r0) ? xs[col - r0] : x[col];
+                    acc -= xv * LU[p];
+                }
+            } else {
+                acc = x[i];
+                const int pe = IA[i + 1];
+                for (int p = d + 1; p < pe; ++p) {
+                    const int col = JA[p];
+                    const T xv = (col < r1) ? xs[col - r0] : x[col];
+                    acc -= xv * LU[p];
+                }
+                acc *= Dinv[i];
+            }
+            xs[i - r0] = acc;
+            x[i] = acc;
+        }
+        sweep_publish(S, c, (g + 1 < g1) ? S.grp_level[g + 1] - 1 : INT_MAX);
+    }
+    sweep_finish(S);
+}
+
+// Numeric ILU(0) of ILUp (ILUp_precond_setup.c:84-166) for p = 0, one thread per row, rows
+// scheduled like the lower sweep (row i reads the finished U rows and pivots of its L columns).
+// LU holds A's values on entry. Same update order as the serial code, contraction off.
+__global__ __launch_bounds__(kSweepThreads) void k_ilu0_factor(SweepArgs S, const int *__restrict__ IA,
+                                                               const int *__restrict__ JA,
+                                                               const int *__restrict__ diag, double *LU,
+                                                               double *Dinv)
+{
+#pragma clang fp contract(off)
+    const int c = sweep_claim(S, true);
+    if (c < 0) return;
+    const int g0 = S.grp_ptr[c], g1 = S.grp_ptr[c + 1];
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&S.progress[c], S.grp_level[g0] - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int g = g0; g < g1; ++g) {
+        const int lev = S.grp_level[g];
+        if (lev > 0) sweep_wait(S, c, true, lev - 1);
+        const int k1 = S.grp_start[g + 1];
+        for (int k = S.grp_start[g] + threadIdx.x; k < k1; k += blockDim.x) {
+            const int i = S.order[k];
+            const int p0 = IA[i], d = diag[i], p1 = IA[i + 1];
+            double dv = LU[d];
+            for (int j = p0; j < d; ++j) {
+                const int jrow = JA[j];
+                const double lij = LU[j] * Dinv[jrow];
+                LU[j] = lij;
+                const int ke = IA[jrow + 1];
+                for (int kk = diag[jrow] + 1; kk < ke; ++kk) {
+                    const int col = JA[kk];
+                    if (col == i) {
+                        dv -= lij * LU[kk];
+                        continue;
+                    }
+                    // position of col in row i (sorted): binary search
+                    int lo = p0, hi = p1 - 1, pos = -1;
+                    while (lo <= hi) {
+                        const int m = (lo + hi) >> 1;
+                        const int cm = JA[m];
+                        if (cm == col) { pos = m; break; }
+                        if (cm < col) lo = m + 1; else hi = m - 1;
+                    }
+                    if (pos >= 0) LU[pos] -= lij * LU[kk];
+                }
+            }
+            LU[d] = dv;
+            if (dv == 0.0) {
+                __hip_atomic_store(S.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                Dinv[i] = 0.0;
+            } else {
+                Dinv[i] = 1.0 / dv;
+            }
+        }
+        sweep_publish(S, c, (g + 1 < g1) ? S.grp_level[g + 1] - 1 : INT_MAX);
+    }
+    sweep_finish(S);
+}
+
+static SweepArgs make_args(femgpu_ctx *ctx, SweepPlan &P)
+{
+    SweepArgs a;
+    a.n = P.n;
+    a.R = P.rows_per_chunk;
+    a.nchunks = P.nchunks;
+    a.order = P.order.p;
+    a.grp_ptr = P.grp_ptr.p;
+    a.grp_start = P.grp_start.p;
+    a.grp_level = P.grp_level.p;
+    a.dep = P.dep.p;
+    a.progress = P.progress.p;
+    a.ticket = P.ticket.p;
+    a.err = ctx->err.p;
+    a.spin_limit = 1u << 22;   // a few seconds of polling per wait at most
+    return a;
+}
+
+template <typename T>
+static int sweep_pair(femgpu_ctx *ctx, femgpu_csr *A, const T *LU, const T *Dinv, const T *b, T *x,
+                      const int *skip)
+{
+    if (A->n == 0) return FEMGPU_OK;
+    const size_t lds = sizeof(T) * A->lower.rows_per_chunk;
+    k_sweep<T, true><<<A->lower.nchunks, kSweepThreads, lds, ctx->stream>>>(make_args(ctx, A->lower), A->IA.p,
+                                                                              A->JA.p, A->diag.p, LU, Dinv, b, x, skip);
+    FG_HIP(hipGetLastError());
+    k_sweep<T, false><<<A->upper.nchunks, kSweepThreads, lds, ctx->stream>>>(make_args(ctx, A->upper), A->IA.p,
+                                                                               A->JA.p, A->diag.p, LU, Dinv, x, x, skip);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+int launch_sweep64(femgpu_ctx *ctx, femgpu_csr *A, const double *b, double *x, const int *skip)
+{
+    return sweep_pair<double>(ctx, A, A->LU.p, A->Dinv.p, b, x, skip);
+}
+
+int launch_sweep32(femgpu_ctx *ctx, femgpu_csr *A, const float *b, float *x, const int *skip)
+{
+    return sweep_pair<float>(ctx, A, A->LU32.p, A->Dinv32.p, b, x, skip);
+}
+
+int launch_ilu_factor(femgpu_ctx *ctx, femgpu_csr *A)
+{
+    if (A->n == 0) return FEMGPU_OK;
+    FG_HIP(hipMemcpyAsync(A->LU.p, A->AA.p, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice, ctx->stream));
+    k_ilu0_factor<<<A->lower.nchunks, kSweepThreads, 0, ctx->stream>>>(make_args(ctx, A->lower), A->IA.p, A->JA.p,
+                                                                         A->diag.p, A->LU.p, A->Dinv.p);
+    FG_HIP(hipGetLastError());
+    FG_TRY(launch_to_f32(ctx, A->LU.p, A->LU32.p, A->nnz));
+    FG_TRY(launch_to_f32(ctx, A->Dinv.p, A->Dinv32.p, A->n));
+    return FEMGPU_OK;
+}
+
+// ------------------------------------------------------------------------------- EBE matvec
+// Elements are stored colour by colour; inside one colour no two elements share an unknown, so
+// each colour is one launch whose read-modify-write of Q needs no atomics. s = 3: one thread per
+// element; s = 9/12: one thread per element row, the element's lm staged in LDS.
+__global__ __launch_bounds__(256) void k_ebe3(int e0, int e1, int NEQ, const int *__restrict__ lm,
+                                              const double *__restrict__ A, const double *__restrict__ P,
+                                              double *__restrict__ Q)
+{
+    const int k = e0 + blockIdx.x * 256 + threadIdx.x;
+    if (k >= e1) return;
+    const int i1 = lm[3 * k], i2 = lm[3 * k + 1], i3 = lm[3 * k + 2];
+    const double p1 = (i1 < NEQ) ? P[i1] : 0.0, p2 = (i2 < NEQ) ? P[i2] : 0.0, p3 = (i3 < NEQ) ? P[i3] : 0.0;
+    const double *a = A + 9 * static_cast<size_t>(k);
+    if (i1 < NEQ) Q[i1] += a[0] * p1 + a[1] * p2 + a[2] * p3;
+    if (i2 < NEQ) Q[i2] += a[3] * p1 + a[4] * p2 + a[5] * p3;
+    if (i3 < NEQ) Q[i3] += a[6] * p1 + a[7] * p2 + a[8] * p3;
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_ebe_block(int e0, int e1, int NEQ, const int *__restrict__ lm,
+                                                   const double *__restrict__ A, const double *__restrict__ P,
+                                                   double *__restrict__ Q)
+{
+    constexpr int EPB = 256 / S;   // elements per block
+    __shared__ int sl[EPB * S];
+    __shared__ double sp[EPB * S];
+    const int ebase = e0 + blockIdx.x * EPB;
+    for (int t = threadIdx.x; t < EPB * S; t += 256) {
+        const int e = ebase + t / S;
+        int v = NEQ;
+        if (e < e1) v = lm[static_cast<size_t>(e) * S + t % S];
+        sl[t] = v;
+        sp[t] = (v < NEQ) ? P[v] : 0.0;
+    }
+    __syncthreads();
+    const int le = threadIdx.x / S, a = threadIdx.x % S;
+    const int e = ebase + le;
+    if (le >= EPB || e >= e1) return;
+    const int row = sl[le * S + a];
+    if (row >= NEQ) return;
+    const double *ar = A + (static_cast<size_t>(e) * S + a) * S;
+    double acc = 0;
+#pragma unroll
+    for (int b = 0; b < S; ++b) acc += ar[b] * sp[le * S + b];
+    Q[row] += acc;
+}
+
+__global__ void k_zero(double *__restrict__ q, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) q[i] = 0.0;
+}
+
+int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *Q)
+{
+    if (E->NEQ > 0) {
+        k_zero<<<std::min(4096, (E->NEQ + 255) / 256), 256, 0, ctx->stream>>>(Q, E->NEQ);
+        FG_HIP(hipGetLastError());
+    }
+    for (int c = 0; c < E->ncolours; ++c) {
+        const int e0 = E->colour_ptr[c], e1 = E->colour_ptr[c + 1];
+        if (e1 <= e0) continue;
+        if (E->s == 3) {
+            k_ebe3<<<(e1 - e0 + 255) / 256, 256, 0, ctx->stream>>>(e0, e1, E->NEQ, E->lm.p, E->A.p, P, Q);
+        } else if (E->s == 9) {
+            constexpr int EPB = 256 / 9;
+            k_ebe_block<9><<<(e1 - e0 + EPB - 1) / EPB, 256, 0, ctx->stream>>>(e0, e1, E->NEQ, E->lm.p, E->A.p, P, Q);
+        } else if (E->s == 12) {
+            constexpr int EPB = 256 / 12;
+            k_ebe_block<12><<<(e1 - e0 + EPB - 1) / EPB, 256, 0, ctx->stream>>>(e0, e1, E->NEQ, E->lm.p, E->A.p, P, Q);
+        } else {
+            set_error("femgpu_ebemv: unsupported block size %d", E->s);
+            return FEMGPU_ERR_ARG;
+        }
+        FG_HIP(hipGetLastError());
+    }
+    return FEMGPU_OK;
+}
+
+// Element matrices from element order into colour order.
+__global__ void k_gather_blocks(const int *__restrict__ elem, int nel, int ss, const double *__restrict__ src,
+                                double *__restrict__ dst)
+{
+    const size_t total = static_cast<size_t>(nel) * ss;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = t / ss, q = t % ss;
+        dst[t] = src[static_cast<size_t>(elem[k]) * ss + q];
+    }
+}
+
+int launch_ebe_gather_values(femgpu_ctx *ctx, femgpu_ebe *E, const double *A_elem_order)
+{
+    const size_t total = static_cast<size_t>(E->nel) * E->s * E->s;
+    if (!total) return FEMGPU_OK;
+    const int grid = static_cast<int>(std::min<size_t>((total + 255) / 256, 8192));
+    k_gather_blocks<<<grid, 256, 0, ctx->stream>>>(E->elem.p, E->nel, E->s * E->s, A_elem_order, E->A.p);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+// ------------------------------------------------------------------------------- dot product
+__global__ __launch_bounds__(256) void k_dot(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                             double *partials, unsigned *counter, double *out)
+{
+    __shared__ double red[256 / kWave];
+    double acc = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) acc += x[i] * y[i];
+    acc = block_sum<256>(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+    if (last_block(counter)) {
+        double v = 0;
+        for (int b = threadIdx.x; b < gridDim.x; b += 256) v += partials[b];
+        v = block_sum<256>(v, red);
+        if (threadIdx.x == 0) {
+            *out = v;
+            *counter = 0;
+        }
+    }
+}
+
+int dot64(femgpu_ctx *ctx, int n, const double *x, const double *y, double *d_out)
+{
+    const int grid = std::max(1, std::min(kReduceBlocks, (n + 1023) / 1024));
+    k_dot<<<grid, 256, 0, ctx->stream>>>(n, x, y, ctx->partials.p, ctx->counter.p, d_out);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+}  // namespace fg

@@ -1,0 +1,511 @@
+// Krylov solvers on the device: pgmres (FP64, the reference's semantics), the mixed FP32-inner /
+// FP64-refinement GMRES, PCG, and the fused Arnoldi kernels they share.
+//
+// One Arnoldi step of pgmres.c:87-156 is, on the device:
+//   spmv   : w' = A (w * 1/h), storing the normalised basis vector V[i] = w * 1/h on the way
+//   sweeps : w' = U^-1 L^-1 w' (level-scheduled ILU(0), in place)
+//   mgs j  : w' -= h_{j-1,i} V[j-1] fused with h_{j,i} = <V[j], w'>, j = 0..i (one pass each)
+//   norm   : w' -= h_{i,i} V[i] fused with ||w'||; the last block finishes the column: Givens
+//            rotations, residual estimate, convergence flag (pgmres.c:118-156)
+// Every launch of a step returns immediately once the cycle-done flag is set, so the host can
+// enqueue one step ahead of the convergence check it reads back.
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+#include "device_util.h"
+
+namespace fg {
+
+struct GDev {
+    double *H;        // (ld x ld) Hessenberg, H[r*ld + c]
+    double *c, *s, *e, *y, *hnorm, *invh;   // ld each
+    double *scal;     // [0] eps  [1] rho  [2] normb
+    int *flags;       // [0] cycle done  [1] steps in cycle
+    int ld, kmax;
+    double *partials;
+    unsigned *counter;
+    int *h_status;    // host-mapped: per step {done} ; h_rho host-mapped rho per step
+    double *h_rho;
+};
+
+__device__ __forceinline__ double &Hat(const GDev &G, int r, int c) { return G.H[r * G.ld + c]; }
+
+// Givens update of column i and the convergence test (pgmres.c:118-159), one thread.
+__device__ void givens_column(const GDev &G, int i)
+{
+#pragma clang fp contract(off)
+    for (int j = 0; j <= i - 1; ++j) {
+        const double h1 = G.c[j] * Hat(G, j, i) + G.s[j] * Hat(G, j + 1, i);
+        const double h2 = -G.s[j] * Hat(G, j, i) + G.c[j] * Hat(G, j + 1, i);
+        Hat(G, j, i) = h1;
+        Hat(G, j + 1, i) = h2;
+    }
+    const double a = Hat(G, i, i), b = Hat(G, i + 1, i);
+    const double r = sqrt(a * a + b * b);
+    G.c[i] = a / r;
+    G.s[i] = b / r;
+    Hat(G, i, i) = r;
+    Hat(G, i + 1, i) = 0.0;
+    G.e[i + 1] = -G.s[i] * G.e[i];
+    G.e[i] = G.c[i] * G.e[i];
+    const double rho = fabs(G.e[i + 1]);
+    const int steps = i + 1;
+    const int done = !(rho > G.scal[0]) || (steps >= G.kmax - 1);
+    G.scal[1] = rho;
+    G.flags[1] = steps;
+    G.flags[0] = done;
+    __hip_atomic_store(&G.h_rho[i], rho, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&G.h_status[i], done ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// MGS pass j of step i on w (vector type T, accumulation in FP64):
+//   uprev != null : w -= h_{j-1,i} * uprev
+//   unext != null : h_{j,i} = <unext, w>         (unext == null: norm pass, then Givens)
+template <typename T>
+__global__ __launch_bounds__(256) void k_mgs(GDev G, int n, int i, int j, T *__restrict__ w,
+                                             const T *__restrict__ uprev, const T *__restrict__ unext)
+{
+    if (G.flags[0]) return;
+    __shared__ double red[4];
+    const double hp = uprev ? Hat(G, j - 1, i) : 0.0;
+    const T h = static_cast<T>(hp);
+    double acc = 0;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        T wv = w[k];
+        if (uprev) {
+            wv = wv - h * uprev[k];
+            w[k] = wv;
+        }
+        const T u = unext ? unext[k] : wv;
+        acc += static_cast<double>(u) * static_cast<double>(wv);
+    }
+    acc = block_sum<256>(acc, red);
+    if (threadIdx.x == 0) G.partials[blockIdx.x] = acc;
+    if (last_block(G.counter)) {
+        double v = 0;
+        for (int b = threadIdx.x; b < gridDim.x; b += 256) v += G.partials[b];
+        v = block_sum<256>(v, red);
+        if (threadIdx.x == 0) {
+            *G.counter = 0;
+            if (unext) {
+                Hat(G, j, i) = v;
+            } else {
+                const double hn = sqrt(v);
+                Hat(G, i + 1, i) = hn;
+                G.hnorm[i + 1] = hn;
+                G.invh[i + 1] = 1.0 / hn;
+                givens_column(G, i);
+            }
+        }
+    }
+}
+
+// Start of a cycle from the residual vector w (already M^-1 (B - A X) or B itself):
+//   negate = 1: w = B - w (pgmres.c:73-74), then e0 = ||w||, invh[0] = 1/e0, rho = e0, flags reset
+template <typename T>
+__global__ __launch_bounds__(256) void k_cycle_start(GDev G, int n, T *__restrict__ w, const T *__restrict__ B,
+                                                     int negate, int first, double tol)
+{
+    __shared__ double red[4];
+    double acc = 0;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        T v = w[k];
+        if (negate) {
+            v = B[k] - v;
+            w[k] = v;
+        }
+        acc += static_cast<double>(v) * static_cast<double>(v);
+    }
+    acc = block_sum<256>(acc, red);
+    if (threadIdx.x == 0) G.partials[blockIdx.x] = acc;
+    if (last_block(G.counter)) {
+        double v = 0;
+        for (int b = threadIdx.x; b < gridDim.x; b += 256) v += G.partials[b];
+        v = block_sum<256>(v, red);
+        if (threadIdx.x == 0) {
+            *G.counter = 0;
+            const double e0 = sqrt(v);
+            if (first) {   // pgmres.c:42-46: ||B|| and eps
+                G.scal[2] = e0;
+                G.scal[0] = tol * e0;
+            }
+            G.e[0] = e0;
+            G.hnorm[0] = e0;
+            G.invh[0] = 1.0 / e0;
+            G.scal[1] = e0;
+            G.flags[0] = 0;
+            G.flags[1] = 0;
+        }
+    }
+}
+
+// Back substitution (pgmres.c:159-170) by every block into LDS, then X += scale * sum_j V[j] y[j]
+// in the reference's per-entry order (pgmres.c:173-175).
+template <typename T, typename TX>
+__global__ __launch_bounds__(256) void k_cycle_finish(GDev G, int n, const T *__restrict__ V, size_t ldv,
+                                                      TX *__restrict__ X, double scale)
+{
+#pragma clang fp contract(off)
+    __shared__ double ys[160];
+    const int i = G.flags[1] - 1;
+    if (threadIdx.x == 0 && i >= 0) {
+        ys[i] = G.e[i] / Hat(G, i, i);
+        for (int j = i - 1; j >= 0; --j) {
+            double soma = 0.0;
+            for (int k = j + 1; k <= i; ++k) soma = soma + Hat(G, j, k) * ys[k];
+            ys[j] = (G.e[j] - soma) / Hat(G, j, j);
+        }
+        if (blockIdx.x == 0)
+            for (int j = 0; j <= i; ++j) G.y[j] = ys[j];
+    }
+    __syncthreads();
+    if (i < 0) return;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        if (scale == 1.0) {
+            TX xv = X[k];
+            for (int j = 0; j <= i; ++j) xv = xv + static_cast<TX>(V[j * ldv + k]) * static_cast<TX>(ys[j]);
+            X[k] = xv;
+        } else {
+            double d = 0;
+            for (int j = 0; j <= i; ++j) d = d + static_cast<double>(V[j * ldv + k]) * ys[j];
+            X[k] = static_cast<TX>(static_cast<double>(X[k]) + scale * d);
+        }
+    }
+}
+
+// Scaled FP32 copy of the FP64 residual: r32 = r * (1/rn).
+__global__ void k_scale_to_f32(int n, const double *__restrict__ r, const double *__restrict__ rn,
+                               float *__restrict__ r32)
+{
+    const double inv = 1.0 / *rn;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        r32[k] = static_cast<float>(r[k] * inv);
+}
+
+__global__ void k_sqrt_inplace(double *v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *v = sqrt(*v);
+}
+
+static int grid_for(int n) { return std::max(1, std::min(kReduceBlocks, (n + 1023) / 1024)); }
+
+// ------------------------------------------------------------------------------- workspace
+template <typename T>
+static int ensure_ws(femgpu_ctx *ctx, int n, int kmax, DevBuf<T> &basis, DevBuf<T> &wbuf, size_t &ldv)
+{
+    ldv = (static_cast<size_t>(n) + 63) / 64 * 64;
+    FG_TRY(basis.alloc(ldv * (kmax + 1)));
+    FG_TRY(wbuf.alloc(ldv * 3));
+    return FEMGPU_OK;
+}
+
+static int make_gdev(femgpu_ctx *ctx, int kmax, GDev &G)
+{
+    const int ld = kmax + 1;
+    FG_CHECK(kmax >= 2 && kmax <= 150, FEMGPU_ERR_ARG, "GMRES basis size %d outside [2, 150]", kmax);
+    FG_TRY(ctx->gm_small.alloc(static_cast<size_t>(ld) * ld + 6 * ld + 8));
+    FG_TRY(ctx->gm_flags.alloc(8));
+    double *base = ctx->gm_small.p;
+    G.H = base;
+    G.c = base + ld * ld;
+    G.s = G.c + ld;
+    G.e = G.s + ld;
+    G.y = G.e + ld;
+    G.hnorm = G.y + ld;
+    G.invh = G.hnorm + ld;
+    G.scal = G.invh + ld;
+    G.flags = ctx->gm_flags.p;
+    G.ld = ld;
+    G.kmax = kmax;
+    G.partials = ctx->partials.p;
+    G.counter = ctx->counter.p;
+    G.h_status = ctx->d_status;
+    G.h_rho = reinterpret_cast<double *>(ctx->d_status + 256);
+    FG_HIP(hipMemsetAsync(G.H, 0, sizeof(double) * ld * ld, ctx->stream));
+    return FEMGPU_OK;
+}
+
+// Operator + preconditioner application w_out = M^-1 A (x * xs), storing x*xs into xout.
+static int apply_op64(femgpu_ctx *ctx, const femgpu_system *S, const double *x, const double *xs, double *xout,
+                      double *w_out, const int *skip);
+
+// One Arnoldi cycle (steps until the device flag says done), FP64 or FP32 vectors. Returns the
+// steps the device took in the cycle.
+template <typename T>
+static int run_cycle(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, T *V, size_t ldv, T *wA, T *wB,
+                     int kmax, int *steps_out);
+
+template <>
+int run_cycle<double>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, double *V, size_t ldv, double *wA,
+                      double *wB, int kmax, int *steps_out)
+{
+    hipEvent_t ev[2];
+    FG_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    FG_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    const int grid = grid_for(n);
+    double *wc = wA, *wn = wB;
+    int rc = FEMGPU_OK, steps = 0;
+    for (int i = 0; i < kmax - 1; ++i) {
+        ctx->h_status[i] = 0;
+        rc = apply_op64(ctx, S, wc, G.invh + i, V + i * ldv, wn, G.flags);
+        if (rc) break;
+        for (int j = 0; j <= i; ++j)
+            k_mgs<double><<<grid, 256, 0, ctx->stream>>>(G, n, i, j, wn, j ? V + (j - 1) * ldv : nullptr, V + j * ldv);
+        k_mgs<double><<<grid, 256, 0, ctx->stream>>>(G, n, i, i + 1, wn, V + i * ldv, nullptr);
+        if (hipGetLastError() != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
+        if (hipEventRecord(ev[i & 1], ctx->stream) != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
+        std::swap(wc, wn);
+        steps = i + 1;
+        if (i >= 1) {
+            if (hipEventSynchronize(ev[(i - 1) & 1]) != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
+            if (ctx->h_status[i - 1]) break;   // step i was enqueued speculatively and drains
+        }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    *steps_out = steps;
+    if (rc) set_error("pgmres: launch failed");
+    return rc;
+}
+
+template <>
+int run_cycle<float>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, float *V, size_t ldv, float *wA,
+                     float *wB, int kmax, int *steps_out)
+{
+    hipEvent_t ev[2];
+    FG_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    FG_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    const int grid = grid_for(n);
+    float *wc = wA, *wn = wB;
+    int rc = FEMGPU_OK, steps = 0;
+    for (int i = 0; i < kmax - 1; ++i) {
+        ctx->h_status[i] = 0;
+        rc = launch_spmv32(ctx, S->csr, wc, G.invh + i, V + i * ldv, wn, G.flags);
+        if (!rc) rc = launch_sweep32(ctx, S->ilu, wn, wn, G.flags);
+        if (rc) break;
+        for (int j = 0; j <= i; ++j)
+            k_mgs<float><<<grid, 256, 0, ctx->stream>>>(G, n, i, j, wn, j ? V + (j - 1) * ldv : nullptr, V + j * ldv);
+        k_mgs<float><<<grid, 256, 0, ctx->stream>>>(G, n, i, i + 1, wn, V + i * ldv, nullptr);
+        if (hipGetLastError() != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
+        if (hipEventRecord(ev[i & 1], ctx->stream) != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
+        std::swap(wc, wn);
+        steps = i + 1;
+        if (i >= 1) {
+            if (hipEventSynchronize(ev[(i - 1) & 1]) != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
+            if (ctx->h_status[i - 1]) break;
+        }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    *steps_out = steps;
+    if (rc) set_error("mixed GMRES: launch failed");
+    return rc;
+}
+
+static int apply_op64(femgpu_ctx *ctx, const femgpu_system *S, const double *x, const double *xs, double *xout,
+                      double *w_out, const int *skip)
+{
+    if (S->csr) {
+        FG_TRY(launch_spmv64(ctx, S->csr, x, xs, xout, w_out, skip));
+    } else {
+        FG_CHECK(xs == nullptr && xout == nullptr, FEMGPU_ERR_ARG, "EBE operator: internal scaling unsupported");
+        FG_TRY(launch_ebemv(ctx, S->ebe, x, w_out));
+    }
+    if (S->ilu) FG_TRY(launch_sweep64(ctx, S->ilu, w_out, w_out, skip));
+    return FEMGPU_OK;
+}
+
+__global__ void k_scale_copy(int n, const double *__restrict__ x, const double *__restrict__ xs,
+                             double *__restrict__ out, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const double s = *xs;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) out[k] = x[k] * s;
+}
+
+// EBE operators take unscaled input: scale into the basis first, then apply to the basis vector.
+static int apply_step_input(femgpu_ctx *ctx, const femgpu_system *S, int n, const double *wc, const double *xs,
+                            double *Vi, double *wn, const int *skip)
+{
+    if (S->csr) return apply_op64(ctx, S, wc, xs, Vi, wn, skip);
+    k_scale_copy<<<grid_for(n) * 4, 256, 0, ctx->stream>>>(n, wc, xs, Vi, skip);
+    FG_HIP(hipGetLastError());
+    FG_TRY(launch_ebemv(ctx, S->ebe, Vi, wn));
+    if (S->ilu) FG_TRY(launch_sweep64(ctx, S->ilu, wn, wn, skip));
+    return FEMGPU_OK;
+}
+
+static int sys_n(const femgpu_system *S) { return S->csr ? S->csr->n : S->ebe->NEQ; }
+
+}  // namespace fg
+
+using namespace fg;
+
+extern "C" int femgpu_pgmres(femgpu_ctx *ctx, const femgpu_system *S, int kmax, int lmax, double tol,
+                             const double *d_B, double *d_X, femgpu_solve_info *info)
+{
+    FG_CHECK(ctx && S && (S->csr || S->ebe) && d_B && d_X, FEMGPU_ERR_ARG, "femgpu_pgmres: bad arguments");
+    FG_CHECK(!S->ilu || S->ilu->factored, FEMGPU_ERR_STATE, "femgpu_pgmres: ILU(0) preconditioner not factored");
+    const int n = sys_n(S);
+    FG_CHECK(!S->ilu || S->ilu->n == n, FEMGPU_ERR_ARG, "femgpu_pgmres: preconditioner size mismatch");
+    GDev G;
+    FG_TRY(make_gdev(ctx, kmax, G));
+    size_t ldv;
+    FG_TRY(ensure_ws<double>(ctx, n, kmax, ctx->gm_basis, ctx->gm_w, ldv));
+    double *V = ctx->gm_basis.p, *wA = ctx->gm_w.p, *wB = wA + ldv, *wT = wB + ldv;
+    const int grid = grid_for(n);
+    FG_HIP(hipMemsetAsync(d_X, 0, sizeof(double) * n, ctx->stream));   // pgmres.c:39
+    // first cycle: the residual is B itself (X = 0)
+    FG_HIP(hipMemcpyAsync(wA, d_B, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    k_cycle_start<double><<<grid, 256, 0, ctx->stream>>>(G, n, wA, d_B, 0, 1, tol);
+    FG_HIP(hipGetLastError());
+    double normb = 0;
+    FG_HIP(hipMemcpyAsync(&ctx->h_pinned[0], G.scal + 2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    normb = ctx->h_pinned[0];
+    if (info) memset(info, 0, sizeof(*info));
+    if (normb == 0.0) {   // zero right-hand side: X = 0 is exact (the reference divides by zero here)
+        if (info) info->normb = 0.0;
+        return FEMGPU_OK;
+    }
+    int total = 0, cycles = 0;
+    double rho = normb;
+    const double eps = tol * normb;
+    do {
+        if (cycles > 0) {
+            // pgmres.c:59-80: u0 = -(M^-1 A X - B)
+            if (S->csr) {
+                FG_TRY(launch_spmv64(ctx, S->csr, d_X, nullptr, nullptr, wT, nullptr));
+            } else {
+                FG_TRY(launch_ebemv(ctx, S->ebe, d_X, wT));
+            }
+            if (S->ilu) FG_TRY(launch_sweep64(ctx, S->ilu, wT, wT, nullptr));
+            k_cycle_start<double><<<grid, 256, 0, ctx->stream>>>(G, n, wT, d_B, 1, 0, tol);
+            FG_HIP(hipGetLastError());
+            std::swap(wA, wT);
+        }
+        int steps = 0;
+        if (S->csr) {
+            FG_TRY(run_cycle<double>(ctx, S, G, n, V, ldv, wA, wB, kmax, &steps));
+        } else {
+            // EBE: same cycle with an explicit scaling pass into the basis
+            hipEvent_t ev[2];
+            FG_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+            FG_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+            double *wc = wA, *wn = wB;
+            for (int i = 0; i < kmax - 1; ++i) {
+                ctx->h_status[i] = 0;
+                FG_TRY(apply_step_input(ctx, S, n, wc, G.invh + i, V + i * ldv, wn, G.flags));
+                for (int j = 0; j <= i; ++j)
+                    k_mgs<double><<<grid, 256, 0, ctx->stream>>>(G, n, i, j, wn, j ? V + (j - 1) * ldv : nullptr,
+                                                                V + j * ldv);
+                k_mgs<double><<<grid, 256, 0, ctx->stream>>>(G, n, i, i + 1, wn, V + i * ldv, nullptr);
+                FG_HIP(hipGetLastError());
+                FG_HIP(hipEventRecord(ev[i & 1], ctx->stream));
+                std::swap(wc, wn);
+                steps = i + 1;
+                if (i >= 1) {
+                    FG_HIP(hipEventSynchronize(ev[(i - 1) & 1]));
+                    if (ctx->h_status[i - 1]) break;
+                }
+            }
+            (void)hipEventDestroy(ev[0]);
+            (void)hipEventDestroy(ev[1]);
+        }
+        k_cycle_finish<double, double><<<grid, 256, 0, ctx->stream>>>(G, n, V, ldv, d_X, 1.0);
+        FG_HIP(hipGetLastError());
+        // device truth: steps and rho of this cycle
+        FG_HIP(hipMemcpyAsync(&ctx->h_pinned[0], G.scal + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        FG_HIP(hipMemcpyAsync(&ctx->h_pinned[1], G.flags + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        FG_HIP(hipStreamSynchronize(ctx->stream));
+        FG_TRY(check_device_error(ctx));
+        rho = ctx->h_pinned[0];
+        int dsteps;
+        memcpy(&dsteps, &ctx->h_pinned[1], sizeof(int));
+        total += dsteps;
+        ++cycles;
+        (void)steps;
+    } while (rho > eps && cycles < lmax);
+    if (info) {
+        info->iterations = total;
+        info->cycles = cycles;
+        info->rho = rho;
+        info->normb = normb;
+        info->status = (rho > eps) ? 1 : 0;
+    }
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const femgpu_mixed_opts *opt,
+                                  double tol, const double *d_F, double *d_X, femgpu_solve_info *info)
+{
+    FG_CHECK(ctx && S && S->csr && S->ilu && opt && d_F && d_X, FEMGPU_ERR_ARG,
+             "femgpu_gmres_mixed: needs a CSR operator, a factored ILU(0) and options");
+    FG_CHECK(S->ilu->factored, FEMGPU_ERR_STATE, "femgpu_gmres_mixed: ILU(0) not factored");
+    FG_CHECK(opt->sweep_mode == 0, FEMGPU_ERR_ARG, "femgpu_gmres_mixed: sweep_mode %d not available", opt->sweep_mode);
+    const int n = S->csr->n;
+    const int kin = opt->inner_kmax;
+    GDev G;
+    FG_TRY(make_gdev(ctx, kin, G));
+    size_t ldv;
+    FG_TRY(ensure_ws<float>(ctx, n, kin, ctx->gm_basis32, ctx->gm_w32, ldv));
+    FG_TRY(ctx->gm_r.alloc(static_cast<size_t>(n) + 64));
+    float *V = ctx->gm_basis32.p, *wA = ctx->gm_w32.p, *wB = wA + ldv;
+    double *r = ctx->gm_r.p;
+    double *d_rn = ctx->scalars.p, *d_fn = ctx->scalars.p + 1;
+    const int grid = grid_for(n);
+    FG_HIP(hipMemsetAsync(d_X, 0, sizeof(double) * n, ctx->stream));
+    FG_TRY(dot64(ctx, n, d_F, d_F, d_fn));
+    FG_HIP(hipMemcpyAsync(&ctx->h_pinned[0], d_fn, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    const double normF = sqrt(ctx->h_pinned[0]);
+    if (info) memset(info, 0, sizeof(*info));
+    if (normF == 0.0) return FEMGPU_OK;
+    int refinements = 0, inner_total = 0;
+    double rn = normF;
+    for (;;) {
+        // outer FP64 residual r = F - A X (X = 0 on the first pass: r = F)
+        if (refinements == 0) {
+            FG_HIP(hipMemcpyAsync(r, d_F, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        } else {
+            FG_TRY(launch_residual64(ctx, S->csr, d_X, d_F, r));
+        }
+        FG_TRY(dot64(ctx, n, r, r, d_rn));
+        k_sqrt_inplace<<<1, 64, 0, ctx->stream>>>(d_rn);
+        FG_HIP(hipMemcpyAsync(&ctx->h_pinned[2], d_rn, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        FG_HIP(hipStreamSynchronize(ctx->stream));
+        rn = ctx->h_pinned[2];
+        if (rn <= tol * normF || refinements >= opt->max_refinements) break;
+        // inner FP32 solve of M^-1 A d = M^-1 r / ||r||
+        k_scale_to_f32<<<grid * 4, 256, 0, ctx->stream>>>(n, r, d_rn, wA);
+        FG_HIP(hipGetLastError());
+        FG_TRY(launch_sweep32(ctx, S->ilu, wA, wA, nullptr));
+        for (int cyc = 0; cyc < std::max(1, opt->inner_cycles); ++cyc) {
+            if (cyc > 0) break;   // one inner cycle per refinement step
+            k_cycle_start<float><<<grid, 256, 0, ctx->stream>>>(G, n, wA, nullptr, 0, 1, opt->inner_tol);
+            FG_HIP(hipGetLastError());
+            int steps = 0;
+            FG_TRY(run_cycle<float>(ctx, S, G, n, V, ldv, wA, wB, kin, &steps));
+            k_cycle_finish<float, double><<<grid, 256, 0, ctx->stream>>>(G, n, V, ldv, d_X, rn);
+            FG_HIP(hipGetLastError());
+            FG_HIP(hipMemcpyAsync(&ctx->h_pinned[1], G.flags + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            FG_HIP(hipStreamSynchronize(ctx->stream));
+            FG_TRY(check_device_error(ctx));
+            int dsteps;
+            memcpy(&dsteps, &ctx->h_pinned[1], sizeof(int));
+            inner_total += dsteps;
+        }
+        ++refinements;
+    }
+    if (info) {
+        info->iterations = inner_total;
+        info->inner_iterations = inner_total;
+        info->cycles = refinements;
+        info->normb = normF;
+        info->rho = rn;
+        info->true_rel_residual = rn / normF;
+        info->status = (rn <= tol * normF) ? 0 : 1;
+    }
+    return FEMGPU_OK;
+}

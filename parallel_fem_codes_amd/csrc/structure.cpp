@@ -1,0 +1,457 @@
+// Host-side integer work of the hot path: unknown numbering, element maps, the CSR structure with
+// its per-element slot table and reordering, the EBE permutation, element colouring, ILU(0)
+// dependency levels and the chunked sweep schedules. Everything here must be bit-exact with the
+// reference's integer results (see the citations at each entry point in include/femgpu.h).
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "common.h"
+
+namespace {
+
+// Sorted, de-duplicated adjacency of the rows [0, nrows) induced by element maps: every entry
+// lm[e*s+a] in range couples with every entry lm[e*s+b] in range of the same element.
+// Row lists come out ascending, which is the order csr_List_insertA keeps its lists in.
+void element_pattern(int nel, int s, int nrows, const int *lm, std::vector<int> &IA,
+                     std::vector<int> &JA)
+{
+    std::vector<int> cnt(static_cast<size_t>(nrows) + 1, 0);
+    for (int e = 0; e < nel; ++e) {
+        const int *L = lm + static_cast<size_t>(e) * s;
+        int inrange = 0;
+        for (int b = 0; b < s; ++b) inrange += (L[b] >= 0 && L[b] < nrows);
+        for (int a = 0; a < s; ++a)
+            if (L[a] >= 0 && L[a] < nrows) cnt[L[a] + 1] += inrange;
+    }
+    for (int i = 0; i < nrows; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int> buf(cnt[nrows]);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < nel; ++e) {
+        const int *L = lm + static_cast<size_t>(e) * s;
+        for (int a = 0; a < s; ++a) {
+            if (L[a] < 0 || L[a] >= nrows) continue;
+            for (int b = 0; b < s; ++b)
+                if (L[b] >= 0 && L[b] < nrows) buf[fill[L[a]]++] = L[b];
+        }
+    }
+    IA.assign(static_cast<size_t>(nrows) + 1, 0);
+    JA.clear();
+    JA.reserve(buf.size() / 2 + 1);
+    for (int i = 0; i < nrows; ++i) {
+        auto first = buf.begin() + cnt[i], last = buf.begin() + cnt[i + 1];
+        std::sort(first, last);
+        auto end = std::unique(first, last);
+        JA.insert(JA.end(), first, end);
+        IA[i + 1] = static_cast<int>(JA.size());
+    }
+}
+
+// ---- Reverse Cuthill-McKee with the exact tie-breaking of symrcm.c ----------------------------
+// BFS records carry (id, degree, distance). The neighbour heap is a binary min-heap on degree whose
+// insert sifts up on strict '<' and whose removal moves the last record to the root and sifts down
+// preferring the left child, then the right one, on strict '<' (symrcm.c:505-567). These rules fix
+// the order of equal-degree neighbours and therefore the permutation.
+struct Rec {
+    int id, deg, dist;
+};
+
+struct DegreeHeap {
+    std::vector<Rec> h;
+    int size = 0;
+    explicit DegreeHeap(int cap) : h(static_cast<size_t>(cap) + 1) {}
+    void push(const Rec &r)
+    {
+        int i = size++;
+        h[i] = r;
+        while (i > 0) {
+            int p = (i - 1) / 2;
+            if (!(h[i].deg < h[p].deg)) break;
+            std::swap(h[i], h[p]);
+            i = p;
+        }
+    }
+    Rec pop()
+    {
+        Rec top = h[0];
+        h[0] = h[--size];
+        for (int j = 0;;) {
+            int l = 2 * j + 1, r = l + 1, m = j;
+            if (l < size && h[l].deg < h[m].deg) m = l;
+            if (r < size && h[r].deg < h[m].deg) m = r;
+            if (m == j) break;
+            std::swap(h[j], h[m]);
+            j = m;
+        }
+        return top;
+    }
+};
+
+// Pattern plus its transpose; neighbours of i are the union of row i of both, ascending, an index
+// present in both visited once (symrcm.c:116-176, :386-459).
+struct SymGraph {
+    const int *IA, *JA;
+    std::vector<int> TP, TJ;
+    int n;
+    SymGraph(int n_, const int *IA_, const int *JA_) : IA(IA_), JA(JA_), n(n_)
+    {
+        const int nnz = IA[n];
+        TP.assign(static_cast<size_t>(n) + 1, 0);
+        TJ.assign(static_cast<size_t>(nnz) + 1, 0);
+        for (int k = 0; k < nnz; ++k) TP[JA[k] + 1]++;
+        for (int i = 0; i < n; ++i) TP[i + 1] += TP[i];
+        std::vector<int> pos(TP.begin(), TP.end() - 1);
+        for (int r = 0; r < n; ++r)
+            for (int k = IA[r]; k < IA[r + 1]; ++k) TJ[pos[JA[k]]++] = r;
+    }
+    template <typename F>
+    void for_each_neighbour(int i, F &&f) const
+    {
+        int a = IA[i], ae = IA[i + 1], b = TP[i], be = TP[i + 1];
+        while (a < ae || b < be) {
+            if (a == ae) { f(TJ[b++]); continue; }
+            if (b == be) { f(JA[a++]); continue; }
+            int u = JA[a], v = TJ[b];
+            if (u <= v) {
+                f(u);
+                ++a;
+                if (u == v) ++b;
+            } else {
+                f(v);
+                ++b;
+            }
+        }
+    }
+};
+
+// Degrees as calc_degrees counts them (symrcm.c:305-350): every stored entry counts once for its
+// row index, and an entry without a mirrored partner also counts for its column.
+int rcm_degrees(int n, const int *IA, const int *JA, std::vector<int> &D)
+{
+    D.assign(n, 0);
+    int maxd = 0;
+    for (int j = 0; j < n; ++j)
+        for (int p = IA[j]; p < IA[j + 1]; ++p) {
+            const int k = JA[p];
+            maxd = std::max(maxd, ++D[k]);
+            if (k == j) continue;
+            bool mirrored = false;
+            for (int q = IA[k]; q < IA[k + 1]; ++q) {
+                if (JA[q] == j) { mirrored = true; break; }
+                if (JA[q] > j) break;
+            }
+            if (!mirrored) maxd = std::max(maxd, ++D[j]);
+        }
+    return maxd;
+}
+
+// Pseudo-peripheral start (find_starting_node, symrcm.c:352-483): repeated BFS; the candidate is
+// replaced by any dequeued record that is farther, or (not itself and) of larger degree.
+int rcm_start_node(const SymGraph &G, const std::vector<int> &D, int start)
+{
+    const int n = G.n;
+    std::vector<Rec> Q(static_cast<size_t>(n) + 1);
+    std::vector<char> seen(n, 0);
+    int qh = 0, qt = 0;
+    Rec x{start, D[start], 0};
+    auto enq = [&](const Rec &r) { Q[qt] = r; qt = (qt + 1) % (n + 1); };
+    enq(x);
+    seen[start] = 1;
+    int max_dist = 0;
+    for (;;) {
+        while (qh != qt) {
+            Rec v = Q[qh];
+            qh = (qh + 1) % (n + 1);
+            if (v.dist > x.dist || (v.id != x.id && v.deg > x.deg)) x = v;
+            G.for_each_neighbour(v.id, [&](int r) {
+                if (!seen[r]) {
+                    enq(Rec{r, D[r], v.dist + 1});
+                    seen[r] = 1;
+                }
+            });
+        }
+        if (max_dist < x.dist) {
+            max_dist = x.dist;
+            std::fill(seen.begin(), seen.end(), 0);
+            seen[x.id] = 1;
+            x.dist = 0;
+            qh = qt = 0;
+            enq(x);
+        } else {
+            break;
+        }
+    }
+    return x.id;
+}
+
+}  // namespace
+
+extern "C" int femgpu_symrcm(int n, const int *IA, const int *JA, int *perm)
+{
+    FG_CHECK(n >= 0 && (n == 0 || (IA && JA && perm)), FEMGPU_ERR_ARG, "femgpu_symrcm: bad arguments");
+    if (n == 0) return FEMGPU_OK;
+    std::vector<int> D;
+    const int maxd = rcm_degrees(n, IA, JA, D);
+    if (maxd == 0) {
+        std::iota(perm, perm + n, 0);
+        return FEMGPU_OK;
+    }
+    SymGraph G(n, IA, JA);
+    DegreeHeap heap(2 * maxd + 2);
+    std::vector<Rec> Q(static_cast<size_t>(n) + 1);
+    std::vector<char> visited(n, 0);
+    int qh = 0, qt = 0, c = -1, scan = 0;
+    do {
+        while (scan < n && visited[scan]) ++scan;   // first unvisited index (symrcm.c:75-77)
+        Rec v{rcm_start_node(G, D, scan), 0, 0};
+        v.deg = D[v.id];
+        visited[v.id] = 1;
+        Q[qt] = v;
+        qt = (qt + 1) % (n + 1);
+        while (qh != qt) {
+            v = Q[qh];
+            qh = (qh + 1) % (n + 1);
+            perm[++c] = v.id;
+            G.for_each_neighbour(v.id, [&](int r) {
+                if (!visited[r]) {
+                    heap.push(Rec{r, D[r], v.dist + 1});
+                    visited[r] = 1;
+                }
+            });
+            while (heap.size) {
+                Q[qt] = heap.pop();
+                qt = (qt + 1) % (n + 1);
+            }
+        }
+        scan = 0;
+    } while (c + 1 < n);
+    std::reverse(perm, perm + n);   // symrcm.c:231-235
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_number_unknowns(int nnodes, int ndof, const int *type, int *invP_id)
+{
+    FG_CHECK(nnodes >= 0 && ndof > 0 && type && invP_id, FEMGPU_ERR_ARG,
+             "femgpu_number_unknowns: bad arguments");
+    int neq = 0;
+    const size_t total = static_cast<size_t>(nnodes) * ndof;
+    for (size_t k = 0; k < total; ++k) invP_id[k] = (type[k] == 1) ? neq++ : -1;
+    return neq;
+}
+
+extern "C" int femgpu_fill_lm(int nel, int ndof, int NEQ, const int *ien, const int *type,
+                              const int *invP_id, int *lm)
+{
+    FG_CHECK(nel >= 0 && ndof > 0 && ien && type && invP_id && lm, FEMGPU_ERR_ARG,
+             "femgpu_fill_lm: bad arguments");
+    const int s = 3 * ndof;
+    for (int e = 0; e < nel; ++e)
+        for (int a = 0; a < 3; ++a) {
+            const size_t base = static_cast<size_t>(ien[3 * e + a]) * ndof;
+            for (int d = 0; d < ndof; ++d)
+                lm[static_cast<size_t>(e) * s + a * ndof + d] = type[base + d] ? invP_id[base + d] : NEQ;
+        }
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_csr_nnz(int nel, int s, int neq, const int *lm)
+{
+    FG_CHECK(nel >= 0 && s > 0 && neq >= 0 && lm, FEMGPU_ERR_ARG, "femgpu_csr_nnz: bad arguments");
+    std::vector<int> IA, JA;
+    element_pattern(nel, s, neq, lm, IA, JA);
+    return static_cast<int>(JA.size());
+}
+
+extern "C" int femgpu_csr_init(int nel, int s, int neq, int *lm, int reorder, int permute_lm,
+                               int ndofs_total, const int *type, int *invP_id, int *IA, int *JA,
+                               int *scheme, int *perm)
+{
+    FG_CHECK(nel >= 0 && s > 0 && neq >= 0 && lm && IA && JA && scheme && perm, FEMGPU_ERR_ARG,
+             "femgpu_csr_init: bad arguments");
+    FG_CHECK(reorder == 0 || reorder == 1, FEMGPU_ERR_ARG,
+             "femgpu_csr_init: reorder must be 0 (NOT) or 1 (SYMRCM)");
+    std::vector<int> ia, ja;
+    element_pattern(nel, s, neq, lm, ia, ja);
+    const int nnz = static_cast<int>(ja.size());
+    // Slot of every element entry in the unpermuted structure (csr_Initialization.c:101-191).
+    const size_t ss = static_cast<size_t>(s) * s;
+    for (int e = 0; e < nel; ++e) {
+        const int *L = lm + static_cast<size_t>(e) * s;
+        for (int a = 0; a < s; ++a)
+            for (int b = 0; b < s; ++b) {
+                int slot = nnz;
+                if (L[a] >= 0 && L[a] < neq && L[b] >= 0 && L[b] < neq) {
+                    auto first = ja.begin() + ia[L[a]], last = ja.begin() + ia[L[a] + 1];
+                    slot = static_cast<int>(std::lower_bound(first, last, L[b]) - ja.begin());
+                }
+                scheme[e * ss + a * s + b] = slot;
+            }
+    }
+    // Reordering (reordering.c:3-22), then rows and columns permuted with per-row re-sort,
+    // carrying the original slot of every entry (reordering.c:101-167).
+    std::iota(perm, perm + neq, 0);
+    if (reorder == 1) FG_TRY(femgpu_symrcm(neq, ia.data(), ja.data(), perm));
+    std::vector<int> inv(static_cast<size_t>(neq) + 1);
+    for (int i = 0; i < neq; ++i) inv[perm[i]] = i;
+    inv[neq] = neq;
+    std::vector<int> newslot(static_cast<size_t>(nnz) + 1);
+    newslot[nnz] = nnz;
+    IA[0] = 0;
+    std::vector<std::pair<int, int>> row;   // (new column, old slot)
+    int k = 0;
+    for (int i = 0; i < neq; ++i) {
+        const int old = perm[i];
+        row.clear();
+        for (int p = ia[old]; p < ia[old + 1]; ++p) row.emplace_back(inv[ja[p]], p);
+        std::sort(row.begin(), row.end());
+        for (auto &cs : row) {
+            JA[k] = cs.first;
+            newslot[cs.second] = k;
+            ++k;
+        }
+        IA[i + 1] = k;
+    }
+    for (size_t q = 0; q < static_cast<size_t>(nel) * ss; ++q) scheme[q] = newslot[scheme[q]];
+    if (permute_lm)
+        for (size_t q = 0; q < static_cast<size_t>(nel) * s; ++q)
+            if (lm[q] >= 0 && lm[q] < neq) lm[q] = inv[lm[q]];
+    if (invP_id && type)
+        for (int q = 0; q < ndofs_total; ++q)
+            if (type[q] == 1) invP_id[q] = inv[invP_id[q]];
+    return nnz;
+}
+
+extern "C" int femgpu_ebe_permutation(int nel, int NEQ, const int *ien, const int *type,
+                                      const int *invP_id, int reorder, int *Perm, int *invPerm)
+{
+    FG_CHECK(nel >= 0 && NEQ >= 0 && ien && type && invP_id && Perm && invPerm, FEMGPU_ERR_ARG,
+             "femgpu_ebe_permutation: bad arguments");
+    FG_CHECK(reorder == 0 || reorder == 1, FEMGPU_ERR_ARG, "femgpu_ebe_permutation: bad reorder");
+    std::vector<int> lmu(static_cast<size_t>(nel) * 3);
+    for (size_t q = 0; q < lmu.size(); ++q) lmu[q] = type[ien[q]] ? invP_id[ien[q]] : -1;
+    std::iota(Perm, Perm + NEQ, 0);
+    Perm[NEQ] = NEQ;
+    if (reorder == 1) {
+        std::vector<int> IA, JA;
+        element_pattern(nel, 3, NEQ, lmu.data(), IA, JA);
+        FG_TRY(femgpu_symrcm(NEQ, IA.data(), JA.data(), Perm));
+    }
+    for (int i = 0; i < NEQ; ++i) invPerm[Perm[i]] = i;
+    invPerm[NEQ] = NEQ;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_ebe_colour(int nel, int s, int NEQ, const int *lm, int *colour)
+{
+    FG_CHECK(nel >= 0 && s > 0 && lm && colour, FEMGPU_ERR_ARG, "femgpu_ebe_colour: bad arguments");
+    // Colours used so far at each unknown, as a bit set per 64 colours (extended on demand).
+    int words = 1;
+    std::vector<uint64_t> used(static_cast<size_t>(NEQ) + 1, 0);
+    int ncol = 0;
+    for (int e = 0; e < nel; ++e) {
+        const int *L = lm + static_cast<size_t>(e) * s;
+        int c = 0;
+        for (;; ++c) {
+            if (c / 64 >= words) {
+                // widen the bit sets
+                std::vector<uint64_t> wider(static_cast<size_t>(NEQ + 1) * (words + 1), 0);
+                for (int u = 0; u <= NEQ; ++u)
+                    for (int w = 0; w < words; ++w) wider[static_cast<size_t>(u) * (words + 1) + w] = used[static_cast<size_t>(u) * words + w];
+                used.swap(wider);
+                ++words;
+            }
+            bool clash = false;
+            for (int a = 0; a < s && !clash; ++a)
+                if (L[a] >= 0 && L[a] < NEQ)
+                    clash = (used[static_cast<size_t>(L[a]) * words + c / 64] >> (c % 64)) & 1u;
+            if (!clash) break;
+        }
+        colour[e] = c;
+        ncol = std::max(ncol, c + 1);
+        for (int a = 0; a < s; ++a)
+            if (L[a] >= 0 && L[a] < NEQ) used[static_cast<size_t>(L[a]) * words + c / 64] |= (uint64_t(1) << (c % 64));
+    }
+    return ncol;
+}
+
+extern "C" int femgpu_ilu_levels(int n, const int *IA, const int *JA, int lower, int *level)
+{
+    FG_CHECK(n >= 0 && IA && JA && level, FEMGPU_ERR_ARG, "femgpu_ilu_levels: bad arguments");
+    int depth = 0;
+    if (lower) {
+        for (int i = 0; i < n; ++i) {
+            int lv = 0;
+            for (int p = IA[i]; p < IA[i + 1] && JA[p] < i; ++p) lv = std::max(lv, level[JA[p]] + 1);
+            level[i] = lv;
+            depth = std::max(depth, lv + 1);
+        }
+    } else {
+        for (int i = n - 1; i >= 0; --i) {
+            int lv = 0;
+            for (int p = IA[i + 1] - 1; p >= IA[i] && JA[p] > i; --p) lv = std::max(lv, level[JA[p]] + 1);
+            level[i] = lv;
+            depth = std::max(depth, lv + 1);
+        }
+    }
+    return depth;
+}
+
+namespace fg {
+
+// Chunked schedule: rows split into contiguous chunks of rows_per_chunk (in sweep order: ascending
+// for the lower factor, descending for the upper one); inside a chunk rows are grouped by level so
+// a workgroup can walk its groups in increasing level, waiting only on the chunks it reads from.
+int build_sweep_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, bool lower, int R, SweepPlan &P)
+{
+    std::vector<int> level(static_cast<size_t>(n) + 1);
+    P.depth = femgpu_ilu_levels(n, IA, JA, lower ? 1 : 0, level.data());
+    P.n = n;
+    P.rows_per_chunk = R;
+    P.nchunks = (n + R - 1) / R;
+    std::vector<int> order(n), grp_ptr(static_cast<size_t>(P.nchunks) + 1), grp_start, grp_level,
+        dep(P.nchunks);
+    grp_start.reserve(static_cast<size_t>(P.nchunks) * 8);
+    std::vector<std::pair<int, int>> rows;
+    int pos = 0;
+    for (int c = 0; c < P.nchunks; ++c) {
+        const int r0 = c * R, r1 = std::min(n, r0 + R);
+        rows.clear();
+        int far = c;
+        for (int r = r0; r < r1; ++r) {
+            rows.emplace_back(level[r], r);
+            if (lower) {
+                if (IA[r] < IA[r + 1] && JA[IA[r]] < r) far = std::min(far, JA[IA[r]] / R);
+            } else {
+                if (IA[r] < IA[r + 1] && JA[IA[r + 1] - 1] > r) far = std::max(far, JA[IA[r + 1] - 1] / R);
+            }
+        }
+        std::sort(rows.begin(), rows.end());
+        grp_ptr[c] = static_cast<int>(grp_level.size());
+        for (size_t q = 0; q < rows.size(); ++q) {
+            if (q == 0 || rows[q].first != rows[q - 1].first) {
+                grp_start.push_back(pos);
+                grp_level.push_back(rows[q].first);
+            }
+            order[pos++] = rows[q].second;
+        }
+        dep[c] = far;
+    }
+    grp_ptr[P.nchunks] = static_cast<int>(grp_level.size());
+    grp_start.push_back(pos);
+    P.ngroups = static_cast<int>(grp_level.size());
+    hipStream_t s = ctx->stream;
+    FG_TRY(P.order.upload(order.data(), order.size(), s));
+    FG_TRY(P.grp_ptr.upload(grp_ptr.data(), grp_ptr.size(), s));
+    FG_TRY(P.grp_start.upload(grp_start.data(), grp_start.size(), s));
+    FG_TRY(P.grp_level.upload(grp_level.data(), grp_level.size(), s));
+    FG_TRY(P.dep.upload(dep.data(), dep.size(), s));
+    FG_TRY(P.progress.alloc(static_cast<size_t>(P.nchunks)));
+    FG_TRY(P.ticket.alloc(2));
+    FG_HIP(hipMemsetAsync(P.progress.p, 0xFF, sizeof(int) * P.nchunks, s));
+    FG_HIP(hipMemsetAsync(P.ticket.p, 0, sizeof(unsigned) * 2, s));
+    return FEMGPU_OK;
+}
+
+}  // namespace fg

@@ -1,0 +1,358 @@
+"""Host-side mirror of the reference hot-path interface, over the libfemgpu C ABI.
+
+Names follow the reference (``Fill_LM``, ``csr_Initialization``, ``Permutation_of_LM``,
+``pgmres``, ``ILUp_precond_setup`` ...) with the argument meaning of the one-rank case. Device
+vectors are torch tensors on the GPU (PyTorch is used only to own device memory and streams);
+host arrays are numpy. Integer preprocessing runs in the library's C++ host code; every numerical
+operation runs in the library's HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import native as N
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError("device vectors must be contiguous GPU tensors")
+    return t.data_ptr()
+
+
+# ------------------------------------------------------------------------------ mesh generation
+
+def structured_mesh(M, N, box=(0.0, 0.0, 1.0, 1.0), right=True):
+    """M x N squares cut on one diagonal, numbered like MG/Structured_Mesh/meshGenerator.c
+    (nodes row by row, squares row by row, two triangles each; "right" cut from the lower-left to
+    the upper-right corner, "left" cut the other diagonal). Returns x, y, ien (int32, nel*3)."""
+    x0, y0, x1, y1 = box
+    nosx = M + 1
+    hx, hy = (x1 - x0) / M, (y1 - y0) / N
+    i = np.arange(nosx, dtype=np.float64)
+    j = np.arange(N + 1, dtype=np.float64)
+    x = np.tile(x0 + i * hx, N + 1)
+    y = np.repeat(y0 + j * hy, nosx)
+    jj, ii = np.meshgrid(np.arange(N), np.arange(M), indexing="ij")
+    a = (jj * nosx + ii).ravel().astype(np.int64)
+    ien = np.empty((M * N, 2, 3), np.int64)
+    if right:
+        ien[:, 0] = np.stack([a, a + 1, a + 1 + nosx], 1)
+        ien[:, 1] = np.stack([a, a + 1 + nosx, a + nosx], 1)
+    else:
+        ien[:, 0] = np.stack([a, a + 1, a + nosx], 1)
+        ien[:, 1] = np.stack([a + 1, a + 1 + nosx, a + nosx], 1)
+    return x, y, ien.reshape(-1).astype(np.int32)
+
+
+def boundary_types(x, y, box=(0.0, 0.0, 1.0, 1.0), tol=1e-15):
+    """Type per node: 0 (Dirichlet) on the boundary of the box, 1 inside (mark 15 / mark 0)."""
+    x0, y0, x1, y1 = box
+    b = (np.abs(x - x1) <= tol) | (np.abs(x - x0) <= tol) | (np.abs(y - y1) <= tol) | (np.abs(y - y0) <= tol)
+    return np.where(b, 0, 1).astype(np.int32)
+
+
+# ------------------------------------------------------------------------------ integer work
+
+def number_unknowns(type_, ndof=1):
+    t = _i32(type_)
+    inv = np.empty_like(t)
+    neq = N.check(N.lib().femgpu_number_unknowns(len(t) // ndof, ndof, t, inv))
+    return inv, neq
+
+
+def Fill_LM(ien, type_, invP_id, NEQ, ndof=1):
+    nel = len(ien) // 3
+    lm = np.empty(nel * 3 * ndof, np.int32)
+    N.check(N.lib().femgpu_fill_lm(nel, ndof, NEQ, _i32(ien), _i32(type_), _i32(invP_id), lm))
+    return lm
+
+
+@dataclass
+class CsrStructure:
+    IA: np.ndarray
+    JA: np.ndarray
+    scheme: np.ndarray
+    perm: np.ndarray
+    lm: np.ndarray
+    invP_id: np.ndarray
+    nnz: int
+
+
+def csr_Initialization(lm, s, neq, reordering="SYMRCM", permute_lm=False, type_=None, invP_id=None):
+    """csr_Initialization for one rank: sorted CSR from lm, slot table, reordering and remaps."""
+    code = {"NOT": 0, "SYMRCM": 1}[reordering.upper()]
+    L = N.lib()
+    lm = _i32(lm).copy()
+    nel = len(lm) // s
+    nnz = N.check(L.femgpu_csr_nnz(nel, s, neq, lm))
+    IA = np.empty(neq + 1, np.int32)
+    JA = np.empty(max(nnz, 1), np.int32)
+    scheme = np.empty(max(nel * s * s, 1), np.int32)
+    perm = np.empty(max(neq, 1), np.int32)
+    t = _i32(type_) if type_ is not None else np.zeros(1, np.int32)
+    inv = _i32(invP_id).copy() if invP_id is not None else np.zeros(1, np.int32)
+    nt = len(t) if type_ is not None else 0
+    r = N.check(L.femgpu_csr_init(nel, s, neq, lm, code, int(permute_lm), nt, t, inv, IA, JA, scheme, perm))
+    assert r == nnz
+    return CsrStructure(IA, JA[:nnz], scheme[:nel * s * s], perm[:neq], lm,
+                        inv if invP_id is not None else None, nnz)
+
+
+def symrcm(IA, JA):
+    n = len(IA) - 1
+    p = np.empty(max(n, 1), np.int32)
+    N.check(N.lib().femgpu_symrcm(n, _i32(IA), _i32(JA), p))
+    return p[:n]
+
+
+def Permutation_of_LM(ien, type_, invP_id, NEQ, reordering="SYMRCM"):
+    code = {"NOT": 0, "SYMRCM": 1}[reordering.upper()]
+    Perm, invPerm = np.empty(NEQ + 1, np.int32), np.empty(NEQ + 1, np.int32)
+    N.check(N.lib().femgpu_ebe_permutation(len(ien) // 3, NEQ, _i32(ien), _i32(type_), _i32(invP_id),
+                                           code, Perm, invPerm))
+    return Perm, invPerm
+
+
+def ebe_colour(lm, s, NEQ):
+    lm = _i32(lm)
+    col = np.empty(max(len(lm) // s, 1), np.int32)
+    nc = N.check(N.lib().femgpu_ebe_colour(len(lm) // s, s, NEQ, lm, col))
+    return col[:len(lm) // s], nc
+
+
+def ilu_levels(IA, JA, lower=True):
+    n = len(IA) - 1
+    lev = np.empty(max(n, 1), np.int32)
+    depth = N.check(N.lib().femgpu_ilu_levels(n, _i32(IA), _i32(JA), int(lower), lev))
+    return lev[:n], depth
+
+
+# ------------------------------------------------------------------------------ device objects
+
+class Device:
+    """A libfemgpu context on one GPU, issuing work on the given HIP stream (torch's current
+    stream by default, so torch tensors produced before a call are ready for it)."""
+
+    def __init__(self, device=0, stream=None):
+        import torch
+        torch.cuda.init()
+        if stream is None:
+            stream = torch.cuda.current_stream(device).cuda_stream
+        h = C.c_void_p()
+        N.check(N.lib().femgpu_create(device, C.c_void_p(stream), C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            N.lib().femgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        N.check(N.lib().femgpu_sync(self.h))
+
+    # BLAS (00_CommonFiles/BLAS_Operations) on device vectors
+    def ddot(self, x, y):
+        r = C.c_double()
+        N.check(N.lib().femgpu_ddot(self.h, x.numel(), _ptr(x), _ptr(y), C.byref(r)))
+        return r.value
+
+    def daxpy(self, a, x, y):
+        N.check(N.lib().femgpu_daxpy(self.h, x.numel(), float(a), _ptr(x), _ptr(y)))
+
+    def dscal(self, a, x):
+        N.check(N.lib().femgpu_dscal(self.h, x.numel(), float(a), _ptr(x)))
+
+    def dcopy(self, x, y):
+        N.check(N.lib().femgpu_dcopy(self.h, x.numel(), _ptr(x), _ptr(y)))
+
+    def dzero(self, x):
+        N.check(N.lib().femgpu_dzero(self.h, x.numel(), _ptr(x)))
+
+
+class CsrMatrix:
+    """Device CSR operator (csrmv) with its ILU(0) preconditioner (ILUp_precond[_setup])."""
+
+    def __init__(self, dev: Device, IA, JA):
+        self.dev, self.n, self.nnz = dev, len(IA) - 1, len(JA)
+        h = C.c_void_p()
+        N.check(N.lib().femgpu_csr_create(dev.h, self.n, self.nnz, _i32(IA), _i32(JA), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            N.lib().femgpu_csr_destroy(self.h)
+            self.h = None
+
+    def set_values(self, AA):
+        if hasattr(AA, "is_cuda"):
+            N.check(N.lib().femgpu_csr_set_values(self.h, C.c_void_p(_ptr(AA)), 1))
+        else:
+            a = _f64(AA)
+            N.check(N.lib().femgpu_csr_set_values(self.h, a.ctypes.data_as(C.c_void_p), 0))
+            self.dev.sync()
+
+    def values_ptr(self):
+        p = C.c_void_p()
+        N.check(N.lib().femgpu_csr_values_ptr(self.h, C.byref(p)))
+        return p.value
+
+    def values_changed(self):
+        N.check(N.lib().femgpu_csr_values_changed(self.h))
+
+    def sweep_info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        N.check(N.lib().femgpu_csr_sweep_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(depth_lower=a.value, depth_upper=b.value, nchunks=c.value)
+
+    def csrmv(self, P, Q):
+        fn = N.lib().femgpu_csrmv_f32 if P.dtype.itemsize == 4 else N.lib().femgpu_csrmv
+        N.check(fn(self.h, _ptr(P), _ptr(Q)))
+
+    def ilu0_factor(self, on_host=False):
+        N.check((N.lib().femgpu_ilu0_factor_host if on_host else N.lib().femgpu_ilu0_factor)(self.h))
+
+    def ilu0_apply(self, p, z):
+        fn = N.lib().femgpu_ilu0_apply_f32 if p.dtype.itemsize == 4 else N.lib().femgpu_ilu0_apply
+        N.check(fn(self.h, _ptr(p), _ptr(z)))
+
+    def ilu0_factors(self):
+        LU, Dinv = np.empty(max(self.nnz, 1)), np.empty(max(self.n, 1))
+        N.check(N.lib().femgpu_ilu0_get(self.h, LU, Dinv))
+        return LU[:self.nnz], Dinv[:self.n]
+
+    def ILUp_precond_setup(self, F):
+        """ILUp_precond_setup with p = 0: factor the current values, then F := M^-1 F in place."""
+        self.ilu0_factor()
+        self.ilu0_apply(F, F)
+
+
+class EbeMatrix:
+    """Device element-by-element operator (ebemv / ebemvNDOF3 / ebemvNDOF4)."""
+
+    def __init__(self, dev: Device, lm, s, NEQ):
+        self.dev, self.s, self.NEQ = dev, s, NEQ
+        self.nel = len(lm) // s
+        h = C.c_void_p()
+        N.check(N.lib().femgpu_ebe_create(dev.h, self.nel, s, NEQ, _i32(lm), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            N.lib().femgpu_ebe_destroy(self.h)
+            self.h = None
+
+    @property
+    def ncolours(self):
+        return N.check(N.lib().femgpu_ebe_ncolours(self.h))
+
+    def set_values(self, A):
+        if hasattr(A, "is_cuda"):
+            N.check(N.lib().femgpu_ebe_set_values(self.h, C.c_void_p(_ptr(A)), 1))
+        else:
+            a = _f64(A)
+            N.check(N.lib().femgpu_ebe_set_values(self.h, a.ctypes.data_as(C.c_void_p), 0))
+
+    def ebemv(self, P, Q):
+        N.check(N.lib().femgpu_ebemv(self.h, _ptr(P), _ptr(Q)))
+
+
+def make_problem(k=1e-4, gamma=0.0, f0=0.0, bx=(0.5, 0.0, -1.0), by=(-0.5, 1.0, 0.0), shock=0):
+    """Problem callbacks of the transport drivers as data (Condutivity, Reaction, Font, Velocity;
+    CAU shock capture on/off). The default velocity is the rotating field (-y+0.5, x-0.5)."""
+    p = N.TransportProblem()
+    p.k, p.gamma, p.f0 = k, gamma, f0
+    p.bx[:] = list(bx)
+    p.by[:] = list(by)
+    p.shock = shock
+    return p
+
+
+class Mesh:
+    """Device-resident mesh + gather maps for Build_K_F_SUPG / Build_M_K_R_SUPG."""
+
+    def __init__(self, dev: Device, x, y, type_, invP_id, ubc, ien, neq, lm=None, scheme=None, nnz=0):
+        self.dev, self.neq, self.nel = dev, neq, len(ien) // 3
+        h = C.c_void_p()
+        lm_a = _i32(lm) if lm is not None else None
+        sc = _i32(scheme) if scheme is not None else None
+        N.check(N.lib().femgpu_mesh_create(
+            dev.h, len(x), self.nel, neq, _f64(x), _f64(y), _i32(type_), _i32(invP_id), _f64(ubc), _i32(ien),
+            lm_a.ctypes.data_as(C.c_void_p) if lm_a is not None else None,
+            sc.ctypes.data_as(C.c_void_p) if sc is not None else None, int(nnz), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            N.lib().femgpu_mesh_destroy(self.h)
+            self.h = None
+
+    def Build_K_F_SUPG(self, pb, u, F, Ke=None, AA_ptr=None):
+        N.check(N.lib().femgpu_assemble_transport(self.h, C.byref(pb), _ptr(u), _ptr(Ke), _ptr(F),
+                                                  C.c_void_p(AA_ptr) if AA_ptr else None))
+
+    def Build_M_K_R_SUPG(self, pb, alpha, dt, u, du, R, Ae=None, AA_ptr=None):
+        N.check(N.lib().femgpu_assemble_transport_transient(self.h, C.byref(pb), float(alpha), float(dt),
+                                                            _ptr(u), _ptr(du), _ptr(Ae), _ptr(R),
+                                                            C.c_void_p(AA_ptr) if AA_ptr else None))
+
+
+def _system(op, ilu):
+    S = N.System()
+    if isinstance(op, CsrMatrix):
+        S.csr = op.h
+    else:
+        S.ebe = op.h
+    S.ilu = ilu.h if ilu is not None else None
+    return S
+
+
+def pgmres(dev: Device, op, B, X, KrylovBasisVectorsQuantity=30, LinearMaxIter=1000,
+           SolverTolerance=1e-8, precond: CsrMatrix | None = None):
+    """pgmres (pgmres.c:5-200) on device vectors: B already preconditioned, X overwritten.
+    Returns the SolveInfo (iterations = Arnoldi steps, as added to Parameters->iterations)."""
+    info = N.SolveInfo()
+    S = _system(op, precond)
+    N.check(N.lib().femgpu_pgmres(dev.h, C.byref(S), int(KrylovBasisVectorsQuantity), int(LinearMaxIter),
+                                  float(SolverTolerance), _ptr(B), _ptr(X), C.byref(info)))
+    return info
+
+
+def gmres_mixed(dev: Device, A: CsrMatrix, F, X, tol=1e-8, inner_kmax=30, inner_tol=1e-4,
+                max_refinements=50, sweep_mode=0):
+    """FP32 ILU(0)-GMRES inside FP64 iterative refinement; F is the raw right-hand side."""
+    info = N.SolveInfo()
+    o = N.MixedOpts()
+    o.inner_kmax, o.inner_cycles, o.inner_tol = inner_kmax, 1, inner_tol
+    o.max_refinements, o.sweep_mode = max_refinements, sweep_mode
+    S = _system(A, A)
+    N.check(N.lib().femgpu_gmres_mixed(dev.h, C.byref(S), C.byref(o), float(tol), _ptr(F), _ptr(X),
+                                       C.byref(info)))
+    return info
+
+
+def pcg(dev: Device, A: CsrMatrix, B, X, itmax=1000, tol=1e-8, precond: CsrMatrix | None = None):
+    info = N.SolveInfo()
+    S = _system(A, precond)
+    N.check(N.lib().femgpu_pcg(dev.h, C.byref(S), int(itmax), float(tol), _ptr(B), _ptr(X), C.byref(info)))
+    return info

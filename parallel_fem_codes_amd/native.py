@@ -1,0 +1,142 @@
+"""ctypes binding of libfemgpu.so (the C ABI declared in include/femgpu.h).
+
+The library is built in-tree (``parallel_fem_codes_amd/libfemgpu.so``) by ``build()`` or
+``make -C parallel_fem_codes_amd/csrc``. There is no fallback: if the library is missing,
+``lib()`` raises, and every failing call raises ``FemGpuError`` with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+SO_PATH = os.path.join(PKG, "libfemgpu.so")
+CSRC = os.path.join(PKG, "csrc")
+
+OK, ERR_ARG, ERR_HIP, ERR_ZERO_PIVOT, ERR_NOMEM, ERR_STATE, ERR_TIMEOUT = 0, -1, -2, -3, -4, -5, -6
+
+
+class FemGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"femgpu error {code}: {msg}")
+        self.code = code
+
+
+class TransportProblem(C.Structure):
+    """femgpu_transport_problem."""
+    _fields_ = [("k", C.c_double), ("gamma", C.c_double), ("f0", C.c_double),
+                ("bx", C.c_double * 3), ("by", C.c_double * 3), ("shock", C.c_int), ("pad", C.c_int)]
+
+
+class System(C.Structure):
+    """femgpu_system."""
+    _fields_ = [("csr", C.c_void_p), ("ebe", C.c_void_p), ("ilu", C.c_void_p)]
+
+
+class SolveInfo(C.Structure):
+    """femgpu_solve_info."""
+    _fields_ = [("iterations", C.c_int), ("cycles", C.c_int), ("inner_iterations", C.c_int),
+                ("status", C.c_int), ("rho", C.c_double), ("normb", C.c_double),
+                ("true_rel_residual", C.c_double), ("prec_rel_residual", C.c_double)]
+
+
+class MixedOpts(C.Structure):
+    """femgpu_mixed_opts."""
+    _fields_ = [("inner_kmax", C.c_int), ("inner_cycles", C.c_int), ("inner_tol", C.c_double),
+                ("max_refinements", C.c_int), ("sweep_mode", C.c_int), ("pad", C.c_int)]
+
+
+_LIB = None
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+vp = C.c_void_p
+I, D = C.c_int, C.c_double
+
+# name: (restype, argtypes)
+_SIGS = {
+    "femgpu_last_error": (C.c_char_p, []),
+    "femgpu_version": (C.c_char_p, []),
+    "femgpu_number_unknowns": (I, [I, I, i32p, i32p]),
+    "femgpu_fill_lm": (I, [I, I, I, i32p, i32p, i32p, i32p]),
+    "femgpu_csr_nnz": (I, [I, I, I, i32p]),
+    "femgpu_csr_init": (I, [I, I, I, i32p, I, I, I, i32p, i32p, i32p, i32p, i32p, i32p]),
+    "femgpu_symrcm": (I, [I, i32p, i32p, i32p]),
+    "femgpu_ebe_permutation": (I, [I, I, i32p, i32p, i32p, I, i32p, i32p]),
+    "femgpu_ebe_colour": (I, [I, I, I, i32p, i32p]),
+    "femgpu_ilu_levels": (I, [I, i32p, i32p, I, i32p]),
+    "femgpu_create": (I, [I, vp, C.POINTER(vp)]),
+    "femgpu_destroy": (I, [vp]),
+    "femgpu_sync": (I, [vp]),
+    "femgpu_csr_create": (I, [vp, I, I, i32p, i32p, C.POINTER(vp)]),
+    "femgpu_csr_destroy": (I, [vp]),
+    "femgpu_csr_set_values": (I, [vp, vp, I]),
+    "femgpu_csr_values_ptr": (I, [vp, C.POINTER(vp)]),
+    "femgpu_csr_values_changed": (I, [vp]),
+    "femgpu_csr_sweep_info": (I, [vp, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+    "femgpu_csrmv": (I, [vp, vp, vp]),
+    "femgpu_csrmv_f32": (I, [vp, vp, vp]),
+    "femgpu_ilu0_factor": (I, [vp]),
+    "femgpu_ilu0_factor_host": (I, [vp]),
+    "femgpu_ilu0_apply": (I, [vp, vp, vp]),
+    "femgpu_ilu0_apply_f32": (I, [vp, vp, vp]),
+    "femgpu_ilu0_get": (I, [vp, f64p, f64p]),
+    "femgpu_ebe_create": (I, [vp, I, I, I, i32p, C.POINTER(vp)]),
+    "femgpu_ebe_destroy": (I, [vp]),
+    "femgpu_ebe_set_values": (I, [vp, vp, I]),
+    "femgpu_ebe_ncolours": (I, [vp]),
+    "femgpu_ebemv": (I, [vp, vp, vp]),
+    "femgpu_ddot": (I, [vp, I, vp, vp, C.POINTER(D)]),
+    "femgpu_daxpy": (I, [vp, I, D, vp, vp]),
+    "femgpu_dscal": (I, [vp, I, D, vp]),
+    "femgpu_dcopy": (I, [vp, I, vp, vp]),
+    "femgpu_dzero": (I, [vp, I, vp]),
+    "femgpu_pgmres": (I, [vp, C.POINTER(System), I, I, D, vp, vp, C.POINTER(SolveInfo)]),
+    "femgpu_gmres_mixed": (I, [vp, C.POINTER(System), C.POINTER(MixedOpts), D, vp, vp,
+                              C.POINTER(SolveInfo)]),
+    "femgpu_pcg": (I, [vp, C.POINTER(System), I, D, vp, vp, C.POINTER(SolveInfo)]),
+    "femgpu_mesh_create": (I, [vp, I, I, I, f64p, f64p, i32p, i32p, f64p, i32p, vp, vp, I,
+                              C.POINTER(vp)]),
+    "femgpu_mesh_destroy": (I, [vp]),
+    "femgpu_assemble_transport": (I, [vp, C.POINTER(TransportProblem), vp, vp, vp, vp]),
+    "femgpu_assemble_transport_transient": (I, [vp, C.POINTER(TransportProblem), D, D, vp, vp,
+                                                vp, vp, vp]),
+}
+
+
+def declared_symbols():
+    """Every entry point of include/femgpu.h, parsed from the header itself."""
+    import re
+    hdr = open(os.path.join(os.path.dirname(PKG), "include", "femgpu.h")).read()
+    return sorted(set(re.findall(r"\b(femgpu_[a-z0-9_]+)\s*\(", hdr)))
+
+
+def build(force: bool = False):
+    """Compile libfemgpu.so for gfx950 in-tree with hipcc (cross-compiles without a GPU)."""
+    args = ["make", "-C", CSRC, "-j8"]
+    if force:
+        args.insert(1, "-B")
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL)
+    return SO_PATH
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(SO_PATH):
+            raise FemGpuError(ERR_STATE, f"{SO_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(SO_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc < 0:
+        raise FemGpuError(rc, lib().femgpu_last_error().decode())
+    return rc

@@ -1,0 +1,143 @@
+"""CPU tests of libfemgpu's host code: the C ABI loads and exports every declared symbol, and the
+integer work (mesh numbering, lm, CSR structure + SYMRCM, EBE permutation, colouring, ILU levels)
+is bit-exact against the reference-generated fixtures and the oracle."""
+import numpy as np
+import pytest
+
+import golden_cases as G
+from parallel_fem_codes_amd import fem, native
+
+CASES = G.names()
+
+
+@pytest.fixture(scope="module")
+def L():
+    native.build()
+    return native.lib()
+
+
+def test_library_exports_every_declared_symbol(L):
+    import ctypes
+    declared = native.declared_symbols()
+    assert len(declared) >= 40
+    for name in declared:
+        assert hasattr(L, name), name
+    assert set(declared) == set(native._SIGS), "binding table out of sync with include/femgpu.h"
+    assert b"gfx950" in L.femgpu_version()
+    assert isinstance(L.femgpu_last_error(), bytes)
+    del ctypes
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_mesh_generator_matches_reference(L, name):
+    d = G.load(name)
+    m = d["meta"]
+    x, y, ien = fem.structured_mesh(m["M"], m["N"], right=(m["cut"] == "right"))
+    assert np.array_equal(ien, d["ien"])
+    assert np.abs(x - d["x"]).max() < 1e-14 and np.abs(y - d["y"]).max() < 1e-14
+    assert np.array_equal(fem.boundary_types(d["x"], d["y"]), d["type"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_structure_matches_reference(L, name):
+    d = G.load(name)
+    h = d["header"]
+    ebe, reorder = int(h[2]), ("SYMRCM" if int(h[3]) else "NOT")
+    invP, neq = fem.number_unknowns(d["type"])
+    lm = fem.Fill_LM(d["ien"], d["type"], invP, neq)
+    assert np.array_equal(lm, d["ref_lm"])
+    if ebe:
+        Perm, invPerm = fem.Permutation_of_LM(d["ien"], d["type"], invP, neq, reorder)
+        assert np.array_equal(Perm, d["ref_Perm"]) and np.array_equal(invPerm, d["ref_invPerm"])
+    else:
+        c = fem.csr_Initialization(lm, 3, neq, reorder, type_=d["type"], invP_id=invP)
+        for k, ref in (("IA", "ref_IA"), ("JA", "ref_JA"), ("perm", "ref_perm"), ("scheme", "ref_scheme"),
+                       ("invP_id", "ref_invP_id")):
+            assert np.array_equal(getattr(c, k), d[ref]), k
+
+
+@pytest.mark.parametrize("m,right,reorder", [(200, True, "SYMRCM"), (97, False, "SYMRCM"), (60, True, "NOT")])
+def test_structure_matches_oracle_at_scale(L, oracle, m, right, reorder):
+    """Config-0 size (201x201 nodes) and others, including the transient variant that renumbers lm."""
+    x, y, ien = fem.structured_mesh(m, m, right=right)
+    t = fem.boundary_types(x, y)
+    invP, neq = fem.number_unknowns(t)
+    oinv, oneq = oracle.number_unknowns(t)
+    assert neq == oneq and np.array_equal(invP, oinv)
+    lm = fem.Fill_LM(ien, t, invP, neq)
+    assert np.array_equal(lm, oracle.fill_lm(ien, t, invP, neq))
+    for permute in (False, True):
+        c = fem.csr_Initialization(lm, 3, neq, reorder, permute_lm=permute, type_=t, invP_id=invP)
+        o = oracle.csr_init(lm, 3, neq, reorder=reorder, permute_lm=permute, type_=t, invP=invP)
+        assert c.nnz == o["nnz"]
+        for k, ok in (("IA", "IA"), ("JA", "JA"), ("perm", "perm"), ("scheme", "scheme"), ("lm", "lm"),
+                      ("invP_id", "invP")):
+            assert np.array_equal(getattr(c, k), o[ok]), (k, permute)
+    if m == 200:
+        assert c.nnz == 275617   # the survey's count for the 201x201-node mesh
+    Perm, invPerm = fem.Permutation_of_LM(ien, t, invP, neq, reorder)
+    oP, oiP = oracle.ebe_permutation(ien, t, invP, neq, reorder=reorder)
+    assert np.array_equal(Perm, oP) and np.array_equal(invPerm, oiP)
+
+
+def _random_symmetric(n, p, rng, components=1):
+    A = rng.random((n, n)) < p
+    A = A | A.T
+    if components > 1:   # block-diagonal: disconnected graph
+        blk = np.repeat(np.arange(components), -(-n // components))[:n]
+        A &= blk[:, None] == blk[None, :]
+    np.fill_diagonal(A, True)
+    IA = np.concatenate([[0], np.cumsum(A.sum(1))]).astype(np.int32)
+    JA = np.concatenate([np.nonzero(r)[0] for r in A]).astype(np.int32)
+    return IA, JA
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_symrcm_matches_oracle_on_random_graphs(L, oracle, seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 120))
+    IA, JA = _random_symmetric(n, float(rng.uniform(0.01, 0.2)), rng, components=1 + seed % 3)
+    p = fem.symrcm(IA, JA)
+    assert np.array_equal(np.sort(p), np.arange(n))
+    assert np.array_equal(p, oracle.symrcm(IA, JA))
+
+
+@pytest.mark.parametrize("m,s", [(30, 3), (17, 3)])
+def test_colouring_is_conflict_free_and_greedy(L, m, s):
+    x, y, ien = fem.structured_mesh(m, m)
+    t = fem.boundary_types(x, y)
+    invP, neq = fem.number_unknowns(t)
+    lm = fem.Fill_LM(ien, t, invP, neq)
+    col, nc = fem.ebe_colour(lm, s, neq)
+    assert nc == col.max() + 1
+    lmr = lm.reshape(-1, s)
+    # conflict-free: inside a colour every unknown appears at most once
+    for c in range(nc):
+        u = lmr[col == c].ravel()
+        u = u[u < neq]
+        assert len(u) == len(np.unique(u))
+    # greedy in element order: restate and compare
+    used = [set() for _ in range(neq)]
+    for e, row in enumerate(lmr):
+        taken = set().union(*[used[v] for v in row if v < neq])
+        c = 0
+        while c in taken:
+            c += 1
+        assert col[e] == c
+        for v in row:
+            if v < neq:
+                used[v].add(c)
+
+
+def test_ilu_levels_are_valid_topological_levels(L):
+    x, y, ien = fem.structured_mesh(40, 40)
+    t = fem.boundary_types(x, y)
+    invP, neq = fem.number_unknowns(t)
+    c = fem.csr_Initialization(fem.Fill_LM(ien, t, invP, neq), 3, neq, "SYMRCM", type_=t, invP_id=invP)
+    for lower in (True, False):
+        lev, depth = fem.ilu_levels(c.IA, c.JA, lower)
+        assert depth == lev.max() + 1
+        for i in range(neq):
+            cols = c.JA[c.IA[i]:c.IA[i + 1]]
+            deps = cols[cols < i] if lower else cols[cols > i]
+            assert lev[i] == (lev[deps].max() + 1 if len(deps) else 0)
