@@ -105,8 +105,10 @@ int femgpu_csr_set_values(femgpu_csr *A, const double *AA, int on_device);
  * assembly; call femgpu_csr_values_changed afterwards to refresh the FP32 copy. */
 int femgpu_csr_values_ptr(femgpu_csr *A, double **d_AA);
 int femgpu_csr_values_changed(femgpu_csr *A);
-/* Sweep scheduler statistics: depth of the lower/upper dependency DAGs and row chunk count. */
-int femgpu_csr_sweep_info(femgpu_csr *A, int *depth_lower, int *depth_upper, int *nchunks);
+/* Sweep scheduler statistics: depth of the lower/upper dependency DAGs, chunk count and row bound
+ * per chunk of the lower sweep, and the LDS bytes available to one workgroup. */
+int femgpu_csr_sweep_info(femgpu_csr *A, int *depth_lower, int *depth_upper, int *nchunks,
+                          int *rows_per_chunk, int *lds_limit);
 
 /* csrmv (CSR/csrmv.c:4-65), local block: Q = A P. FP64 and FP32 (FP32 copy of the values). */
 int femgpu_csrmv(femgpu_csr *A, const double *d_P, double *d_Q);

@@ -107,7 +107,10 @@ extern "C" int femgpu_create(int device, void *stream, femgpu_ctx **out)
     ctx->device = device;
     ctx->stream = static_cast<hipStream_t>(stream);
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        ctx->num_cus = prop.multiProcessorCount;
+        ctx->lds_limit = prop.sharedMemPerBlock;
+    }
     int rc = ctx->partials.alloc(kReduceBlocks);
     if (!rc) rc = ctx->counter.alloc(4);
     if (!rc) rc = ctx->scalars.alloc(16);
@@ -183,13 +186,13 @@ extern "C" int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA,
     if (!rc) rc = A->AA32.alloc(static_cast<size_t>(nnz) + 1);
     if (!rc) rc = A->LU.alloc(static_cast<size_t>(nnz) + 1);
     if (!rc) rc = A->Dinv.alloc(static_cast<size_t>(n) + 1);
-    if (!rc) rc = A->LU32.alloc(static_cast<size_t>(nnz) + 1);
-    if (!rc) rc = A->Dinv32.alloc(static_cast<size_t>(n) + 1);
     // Rows per chunk: large enough that a chunk covers a few dependency levels of a banded
     // (reordered) mesh matrix, small enough to spread over the CUs.
     const int R = n > 200000 ? 1024 : 256;
     if (!rc) rc = build_sweep_plan(ctx, n, IA, JA, true, R, A->lower);
-    if (!rc) rc = build_sweep_plan(ctx, n, IA, JA, false, R, A->upper);
+    if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), false, A->plo);
+    if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), true, A->pup);
+    if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data());
     if (!rc && hipMemsetAsync(A->AA.p, 0, sizeof(double) * (nnz + 1), s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (rc) {
@@ -230,12 +233,14 @@ extern "C" int femgpu_csr_values_changed(femgpu_csr *A)
     return launch_to_f32(A->ctx, A->AA.p, A->AA32.p, A->nnz);
 }
 
-extern "C" int femgpu_csr_sweep_info(femgpu_csr *A, int *dl, int *du, int *nchunks)
+extern "C" int femgpu_csr_sweep_info(femgpu_csr *A, int *dl, int *du, int *nchunks, int *rows_per_chunk, int *lds_limit)
 {
     FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_csr_sweep_info: null");
-    if (dl) *dl = A->lower.depth;
-    if (du) *du = A->upper.depth;
-    if (nchunks) *nchunks = A->lower.nchunks;
+    if (dl) *dl = A->plo.depth;
+    if (du) *du = A->pup.depth;
+    if (nchunks) *nchunks = A->plo.nchunks;
+    if (rows_per_chunk) *rows_per_chunk = A->plo.R;
+    if (lds_limit) *lds_limit = static_cast<int>(A->ctx->lds_limit);
     return FEMGPU_OK;
 }
 
@@ -273,8 +278,7 @@ extern "C" int femgpu_ilu0_factor_host(femgpu_csr *A)
     FG_TRY(host_ilu0(A->n, A->hIA, A->hJA, diag, LU, Dinv));
     FG_HIP(hipMemcpyAsync(A->LU.p, LU.data(), sizeof(double) * A->nnz, hipMemcpyHostToDevice, ctx->stream));
     FG_HIP(hipMemcpyAsync(A->Dinv.p, Dinv.data(), sizeof(double) * A->n, hipMemcpyHostToDevice, ctx->stream));
-    FG_TRY(launch_to_f32(ctx, A->LU.p, A->LU32.p, A->nnz));
-    FG_TRY(launch_to_f32(ctx, A->Dinv.p, A->Dinv32.p, A->n));
+    FG_TRY(packed_refresh(ctx, A));
     FG_HIP(hipStreamSynchronize(ctx->stream));
     A->factored = true;
     return FEMGPU_OK;

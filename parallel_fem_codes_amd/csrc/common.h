@@ -84,12 +84,44 @@ struct SweepPlan {
     DevBuf<unsigned> ticket; // [0] next chunk ticket, [1] finished chunks; self-resetting
 };
 
+// Packed sweep plan (sweep.hip): chunks of R rows in sweep order, rows of a chunk level-sorted,
+// factor entries packed in schedule order with their dependency pre-encoded (>= 0: LDS slot of a
+// row of the same chunk; < 0: -(row+1) of a row of an earlier chunk, read from its mailbox).
+struct PackedSweep {
+    int n = 0, R = 0, nchunks = 0, ngroups = 0, depth = 0, E = 0, emax = 0;
+    DevBuf<int> chunk_ptr;   // chunk -> first schedule position (nchunks+1)
+    DevBuf<int> lvl_ptr;     // level -> first schedule position (depth+1); the schedule is level-sorted
+    DevBuf<int> row_of;      // schedule position -> row (n)
+    DevBuf<int> ent_ptr;     // schedule position -> first packed entry (n+1)
+    DevBuf<int> ent_code;    // packed entry -> dependency code (E)
+    DevBuf<int> ent_src;     // packed entry -> position in A's layout, to refresh values (E)
+    DevBuf<int> grp_ptr;     // chunk -> first group (nchunks+1)
+    DevBuf<int> grp_start;   // group -> first in-chunk position (ngroups+1)
+    DevBuf<int> grp_flags;   // group -> bit 0: has a remote dependency
+    DevBuf<unsigned char> exported;   // schedule position -> row read by a later chunk
+    DevBuf<double> val64, dinv64;     // packed factor values / inverse pivots in schedule order
+    DevBuf<float> val32, dinv32;
+    DevBuf<unsigned> ctl;             // ticket, finished count, epoch
+    DevBuf<unsigned long long> mbox;    // FP64: two alternating arrays of one word per row
+    DevBuf<unsigned long long> mbox32;  // FP32: {epoch, value} word per row
+};
+
+// Numeric ILU(0) plan: for every row in the lower sweep's level schedule, its CSR position, length,
+// number of L entries, and its update list -- (U entry position k of a dependency row, target slot
+// in this row, index t of the L entry it belongs to) -- in the serial factorisation's order.
+struct FactorPlan {
+    int maxrow = 0, nupd = 0;
+    DevBuf<int4> rowinfo;   // per schedule position: {p0, len, nL, first update}
+    DevBuf<int2> upd;       // {k, dest | t << 16}; dest == nL marks the pivot
+};
+
 }  // namespace fg
 
 struct femgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cus = 256;
+    size_t lds_limit = 65536;      // dynamic LDS available to one workgroup
     fg::DevBuf<double> partials;   // per-block partial sums
     fg::DevBuf<unsigned> counter;  // last-block detection of the reductions
     fg::DevBuf<double> scalars;    // small device scalars (dot results)
@@ -110,10 +142,11 @@ struct femgpu_csr {
     fg::DevBuf<int> IA, JA, diag;   // diag[i] = position of the diagonal entry of row i
     fg::DevBuf<double> AA;          // nnz+1 (dummy slot nnz)
     fg::DevBuf<float> AA32;
-    fg::DevBuf<double> LU, Dinv;    // ILU(0) factors in A's layout
-    fg::DevBuf<float> LU32, Dinv32;
+    fg::DevBuf<double> LU, Dinv;    // ILU(0) factors in A's layout (FP32 copies live in the packed plans)
     bool factored = false;
-    fg::SweepPlan lower, upper;
+    fg::SweepPlan lower;            // scheduler of the numeric factorisation
+    fg::PackedSweep plo, pup;       // packed lower / upper sweeps
+    fg::FactorPlan fplan;           // numeric factorisation
     int spmv_lanes = 8;             // lanes per row of the SpMV kernel
 };
 
@@ -143,6 +176,12 @@ namespace fg {
 // Host-side integer work (structure.cpp).
 int build_sweep_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, bool lower, int rows_per_chunk,
                      SweepPlan &plan);
+int build_packed_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, const int *diag, bool upper,
+                      PackedSweep &plan);
+int packed_refresh(femgpu_ctx *ctx, femgpu_csr *A);
+int build_factor_plan(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
+template <typename T>
+size_t packed_lds_bytes(const PackedSweep &P);
 
 // Launchers (kernels_*.hip).
 int launch_spmv64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *xs, double *xout,

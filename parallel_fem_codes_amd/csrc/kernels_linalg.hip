@@ -205,60 +205,6 @@ __device__ __forceinline__ int sweep_claim(const SweepArgs &S, bool lower)
 
 constexpr int kSweepThreads = 256;
 
-// Triangular sweeps of ILUp_precond.c:26-44 over the ILU(0) factors stored in A's layout:
-//   lower: x[i] = b[i] - sum_{j<i} L_ij x[j]      (unit lower factor, ascending columns)
-//   upper: x[i] = (x[i] - sum_{j>i} U_ij x[j]) * Dinv[i]
-// Each row accumulates in the reference's order with contraction off, so the result is the
-// serial sweep's bit for bit. Rows of the own chunk are read back from LDS.
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs S, const int *__restrict__ IA,
-                                                         const int *__restrict__ JA, const int *__restrict__ diag,
-                                                         const T *__restrict__ LU, const T *__restrict__ Dinv,
-                                                         const T *b, T *x, const int *__restrict__ skip)
-{
-#pragma clang fp contract(off)
-    if (skip && *skip) return;   // uniform: the flag was written by an earlier launch
-    extern __shared__ unsigned char smem_raw[];
-    T *xs = reinterpret_cast<T *>(smem_raw);
-    const int c = sweep_claim(S, LOWER);
-    if (c < 0) return;
-    const int r0 = c * S.R, r1 = min(S.n, r0 + S.R);
-    const int g0 = S.grp_ptr[c], g1 = S.grp_ptr[c + 1];
-    if (threadIdx.x == 0)
-        __hip_atomic_store(&S.progress[c], S.grp_level[g0] - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int g = g0; g < g1; ++g) {
-        const int lev = S.grp_level[g];
-        if (lev > 0) sweep_wait(S, c, LOWER, lev - 1);
-        const int k1 = S.grp_start[g + 1];
-        for (int k = S.grp_start[g] + threadIdx.x; k < k1; k += blockDim.x) {
-            const int i = S.order[k];
-            const int d = diag[i];
-            T acc;
-            if (LOWER) {
-                acc = b[i];
-                for (int p = IA[i]; p < d; ++p) {
-                    const int col = JA[p];
-                    const T xv = (col >= r0) ? xs[col - r0] : x[col];
-                    acc -= xv * LU[p];
-                }
-            } else {
-                acc = x[i];
-                const int pe = IA[i + 1];
-                for (int p = d + 1; p < pe; ++p) {
-                    const int col = JA[p];
-                    const T xv = (col < r1) ? xs[col - r0] : x[col];
-                    acc -= xv * LU[p];
-                }
-                acc *= Dinv[i];
-            }
-            xs[i - r0] = acc;
-            x[i] = acc;
-        }
-        sweep_publish(S, c, (g + 1 < g1) ? S.grp_level[g + 1] - 1 : INT_MAX);
-    }
-    sweep_finish(S);
-}
-
 // Numeric ILU(0) of ILUp (ILUp_precond_setup.c:84-166) for p = 0, one thread per row, rows
 // scheduled like the lower sweep (row i reads the finished U rows and pivots of its L columns).
 // LU holds A's values on entry. Same update order as the serial code, contraction off.
@@ -334,41 +280,160 @@ static SweepArgs make_args(femgpu_ctx *ctx, SweepPlan &P)
     return a;
 }
 
-template <typename T>
-static int sweep_pair(femgpu_ctx *ctx, femgpu_csr *A, const T *LU, const T *Dinv, const T *b, T *x,
-                      const int *skip)
+// Numeric ILU(0) by one persistent workgroup walking the level schedule of the lower sweep: rows of
+// a level in parallel, a barrier between levels. Inside one workgroup the barrier alone makes the
+// finished rows visible (one CU, one L1), so no agent-scope synchronisation is needed.
+constexpr int kFactorThreads = 1024;
+
+__global__ __launch_bounds__(kFactorThreads) void k_ilu0_factor_levels(int depth, const int *__restrict__ lvl_ptr,
+                                                                       const int *__restrict__ row_of,
+                                                                       const int *__restrict__ IA, const int *__restrict__ JA,
+                                                                       const int *__restrict__ diag, double *LU, double *Dinv,
+                                                                       int *err)
 {
-    if (A->n == 0) return FEMGPU_OK;
-    const size_t lds = sizeof(T) * A->lower.rows_per_chunk;
-    k_sweep<T, true><<<A->lower.nchunks, kSweepThreads, lds, ctx->stream>>>(make_args(ctx, A->lower), A->IA.p,
-                                                                              A->JA.p, A->diag.p, LU, Dinv, b, x, skip);
-    FG_HIP(hipGetLastError());
-    k_sweep<T, false><<<A->upper.nchunks, kSweepThreads, lds, ctx->stream>>>(make_args(ctx, A->upper), A->IA.p,
-                                                                               A->JA.p, A->diag.p, LU, Dinv, x, x, skip);
-    FG_HIP(hipGetLastError());
-    return FEMGPU_OK;
+#pragma clang fp contract(off)
+    for (int l = 0; l < depth; ++l) {
+        const int q1 = lvl_ptr[l + 1];
+        for (int q = lvl_ptr[l] + threadIdx.x; q < q1; q += kFactorThreads) {
+            const int i = row_of[q];
+            const int p0 = IA[i], d = diag[i], p1 = IA[i + 1];
+            double dv = LU[d];
+            for (int j = p0; j < d; ++j) {
+                const int jrow = JA[j];
+                const double lij = LU[j] * Dinv[jrow];
+                LU[j] = lij;
+                const int ke = IA[jrow + 1];
+                for (int kk = diag[jrow] + 1; kk < ke; ++kk) {
+                    const int col = JA[kk];
+                    if (col == i) {
+                        dv -= lij * LU[kk];
+                        continue;
+                    }
+                    int lo = p0, hi = p1 - 1, pos = -1;
+                    while (lo <= hi) {
+                        const int m = (lo + hi) >> 1;
+                        const int cm = JA[m];
+                        if (cm == col) { pos = m; break; }
+                        if (cm < col) lo = m + 1; else hi = m - 1;
+                    }
+                    if (pos >= 0) LU[pos] -= lij * LU[kk];
+                }
+            }
+            LU[d] = dv;
+            if (dv == 0.0) {
+                __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                Dinv[i] = 0.0;
+            } else {
+                Dinv[i] = 1.0 / dv;
+            }
+        }
+        __syncthreads();
+    }
 }
 
-int launch_sweep64(femgpu_ctx *ctx, femgpu_csr *A, const double *b, double *x, const int *skip)
+// Numeric ILU(0) from the update lists: one persistent workgroup, rows of a level in parallel (one
+// thread per row), a barrier between levels. A row first issues all of its loads at once -- its
+// update list, the finished U values and pivots it reads, its own values -- so a level costs a few
+// memory round trips instead of one per update; the row's values and multipliers live in a private
+// LDS slice. Same operations in the same order as the serial factorisation, contraction off.
+// Rows beyond the compile-time bounds (MAXROW values, MAXL multipliers, MAXU updates) take a
+// sequential path with the same arithmetic.
+template <int MAXROW, int MAXL, int MAXU, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_ilu0_factor_lists(int depth, const int *__restrict__ lvl_ptr,
+                                                               const int4 *__restrict__ rowinfo,
+                                                               const int2 *__restrict__ upd, const int *__restrict__ JA,
+                                                               const double *__restrict__ AA, double *LU, double *Dinv,
+                                                               int *err)
 {
-    return sweep_pair<double>(ctx, A, A->LU.p, A->Dinv.p, b, x, skip);
-}
-
-int launch_sweep32(femgpu_ctx *ctx, femgpu_csr *A, const float *b, float *x, const int *skip)
-{
-    return sweep_pair<float>(ctx, A, A->LU32.p, A->Dinv32.p, b, x, skip);
+#pragma clang fp contract(off)
+    __shared__ double scratch[(MAXROW + MAXL) * THREADS];
+    double *v = scratch + threadIdx.x;                 // row value e at v[e * THREADS]
+    double *dj = scratch + MAXROW * THREADS + threadIdx.x;   // pivot inverse of L entry t at dj[t * THREADS]
+    for (int l = 0; l < depth; ++l) {
+        const int q1 = lvl_ptr[l + 1];
+        for (int q = lvl_ptr[l] + threadIdx.x; q < q1; q += THREADS) {
+            const int4 ri = rowinfo[q];
+            const int p0 = ri.x, len = ri.y, nL = ri.z, ub = ri.w;
+            const int nu = rowinfo[q + 1].w - ub;   // rowinfo has a sentinel entry after the last row
+            const int i = JA[p0 + nL];
+            double piv;
+            if (len <= MAXROW && nL <= MAXL && nu <= MAXU) {
+                int2 U[MAXU];
+                double Uv[MAXU];
+                int jr[MAXL];
+#pragma unroll
+                for (int u = 0; u < MAXU; ++u)
+                    if (u < nu) U[u] = upd[ub + u];
+#pragma unroll
+                for (int t = 0; t < MAXL; ++t)
+                    if (t < nL) jr[t] = JA[p0 + t];
+#pragma unroll
+                for (int e = 0; e < MAXROW; ++e)
+                    if (e < len) v[e * THREADS] = AA[p0 + e];
+#pragma unroll
+                for (int t = 0; t < MAXL; ++t)
+                    if (t < nL) dj[t * THREADS] = Dinv[jr[t]];
+#pragma unroll
+                for (int u = 0; u < MAXU; ++u)
+                    if (u < nu) Uv[u] = LU[U[u].x];
+                int sc = 0;   // L entries [0, sc) already turned into multipliers
+#pragma unroll
+                for (int u = 0; u < MAXU; ++u) {
+                    if (u < nu) {
+                        const int t = U[u].y >> 16, dest = U[u].y & 0xffff;
+                        for (; sc <= t; ++sc) v[sc * THREADS] = v[sc * THREADS] * dj[sc * THREADS];
+                        v[dest * THREADS] -= v[t * THREADS] * Uv[u];
+                    }
+                }
+                for (; sc < nL; ++sc) v[sc * THREADS] = v[sc * THREADS] * dj[sc * THREADS];
+#pragma unroll
+                for (int e = 0; e < MAXROW; ++e)
+                    if (e < len) LU[p0 + e] = v[e * THREADS];
+                piv = (nL < MAXROW) ? v[nL * THREADS] : LU[p0 + nL];
+            } else {
+                // sequential path directly on the factors in global memory
+                for (int e = 0; e < len; ++e) LU[p0 + e] = AA[p0 + e];
+                int u = ub;
+                const int ue = ub + nu;
+                for (int t = 0; t < nL; ++t) {
+                    const double lij = LU[p0 + t] * Dinv[JA[p0 + t]];
+                    LU[p0 + t] = lij;
+                    for (; u < ue && (upd[u].y >> 16) == t; ++u)
+                        LU[p0 + (upd[u].y & 0xffff)] -= lij * LU[upd[u].x];
+                }
+                piv = LU[p0 + nL];
+            }
+            if (piv == 0.0) {
+                __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                Dinv[i] = 0.0;
+            } else {
+                Dinv[i] = 1.0 / piv;
+            }
+        }
+        __syncthreads();
+    }
 }
 
 int launch_ilu_factor(femgpu_ctx *ctx, femgpu_csr *A)
 {
     if (A->n == 0) return FEMGPU_OK;
-    FG_HIP(hipMemcpyAsync(A->LU.p, A->AA.p, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice, ctx->stream));
-    k_ilu0_factor<<<A->lower.nchunks, kSweepThreads, 0, ctx->stream>>>(make_args(ctx, A->lower), A->IA.p, A->JA.p,
-                                                                         A->diag.p, A->LU.p, A->Dinv.p);
+    const FactorPlan &F = A->fplan;
+    if (F.maxrow <= 8) {
+        k_ilu0_factor_lists<8, 8, 24, 512><<<1, 512, 0, ctx->stream>>>(A->plo.depth, A->plo.lvl_ptr.p, F.rowinfo.p,
+                                                                        F.upd.p, A->JA.p, A->AA.p, A->LU.p, A->Dinv.p,
+                                                                        ctx->err.p);
+    } else if (F.maxrow <= 32) {
+        k_ilu0_factor_lists<32, 16, 32, 128><<<1, 128, 0, ctx->stream>>>(A->plo.depth, A->plo.lvl_ptr.p, F.rowinfo.p,
+                                                                          F.upd.p, A->JA.p, A->AA.p, A->LU.p, A->Dinv.p,
+                                                                          ctx->err.p);
+    } else {
+        FG_HIP(hipMemcpyAsync(A->LU.p, A->AA.p, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice, ctx->stream));
+        k_ilu0_factor_levels<<<1, kFactorThreads, 0, ctx->stream>>>(A->plo.depth, A->plo.lvl_ptr.p, A->plo.row_of.p,
+                                                                    A->IA.p, A->JA.p, A->diag.p, A->LU.p, A->Dinv.p,
+                                                                    ctx->err.p);
+    }
     FG_HIP(hipGetLastError());
-    FG_TRY(launch_to_f32(ctx, A->LU.p, A->LU32.p, A->nnz));
-    FG_TRY(launch_to_f32(ctx, A->Dinv.p, A->Dinv32.p, A->n));
-    return FEMGPU_OK;
+    return packed_refresh(ctx, A);
 }
 
 // ------------------------------------------------------------------------------- EBE matvec

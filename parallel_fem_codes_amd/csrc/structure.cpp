@@ -455,3 +455,207 @@ int build_sweep_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, bool 
 }
 
 }  // namespace fg
+
+namespace fg {
+
+// Packed sweep plan. Sweep index s runs over rows in elimination order (row s for the lower
+// factor, row n-1-s for the upper one), so both directions are "lower" in s. Rows are scheduled by
+// (level, s) and cut into chunks along level bands: a chunk holds whole consecutive levels (a level
+// wider than a chunk is split), so only the first level of a chunk reads rows of earlier chunks.
+int build_packed_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, const int *diag, bool upper,
+                      PackedSweep &P)
+{
+    auto row_at = [&](int s) { return upper ? n - 1 - s : s; };
+    auto sidx = [&](int row) { return upper ? n - 1 - row : row; };
+    auto ent_begin = [&](int row) { return upper ? diag[row] + 1 : IA[row]; };
+    auto ent_end = [&](int row) { return upper ? IA[row + 1] : diag[row]; };
+    auto nent = [&](int s) { return ent_end(row_at(s)) - ent_begin(row_at(s)); };
+    std::vector<int> lev(n);
+    int depth = 0;
+    for (int s = 0; s < n; ++s) {
+        const int r = row_at(s);
+        int l = 0;
+        for (int p = ent_begin(r); p < ent_end(r); ++p) l = std::max(l, lev[sidx(JA[p])] + 1);
+        lev[s] = l;
+        depth = std::max(depth, l + 1);
+    }
+    P.n = n;
+    P.depth = depth;
+    // global schedule by (level, s): counting sort on the level
+    std::vector<int> lptr(static_cast<size_t>(depth) + 1, 0), sched(n);
+    for (int s = 0; s < n; ++s) lptr[lev[s] + 1]++;
+    for (int l = 0; l < depth; ++l) lptr[l + 1] += lptr[l];
+    {
+        std::vector<int> fill(lptr.begin(), lptr.end() - 1);
+        for (int s = 0; s < n; ++s) sched[fill[lev[s]]++] = s;
+    }
+    // chunking for a rows-per-chunk bound Rb: whole levels while they fit, wide levels split
+    auto make_chunks = [&](int Rb, std::vector<int> &cptr, int &emax) {
+        cptr.assign(1, 0);
+        emax = 0;
+        int start = 0, ents = 0;
+        for (int l = 0; l < depth; ++l) {
+            const int a = lptr[l], b = lptr[l + 1];
+            if (b - start > Rb && a > start) {   // this level does not fit: close the chunk
+                cptr.push_back(a);
+                emax = std::max(emax, ents);
+                start = a;
+                ents = 0;
+            }
+            for (int q = a; q < b; ++q) {
+                if (q - start == Rb) {   // a level wider than a chunk
+                    cptr.push_back(q);
+                    emax = std::max(emax, ents);
+                    start = q;
+                    ents = 0;
+                }
+                ents += nent(sched[q]);
+            }
+        }
+        if (n > start || n == 0) cptr.push_back(n);
+        emax = std::max(emax, ents);
+    };
+    static const int cand[] = {6144, 4096, 3072, 2048, 1536, 1024, 768, 512, 256, 128, 64};
+    const size_t limit = ctx->lds_limit > 1024 ? ctx->lds_limit - 1024 : ctx->lds_limit;
+    std::vector<int> cptr;
+    for (int Rc : cand) {
+        int em = 0;
+        make_chunks(Rc, cptr, em);
+        PackedSweep probe;
+        probe.R = Rc;
+        probe.emax = std::max(em, 1);
+        if (packed_lds_bytes<double>(probe) <= limit || Rc == 64) {
+            P.R = Rc;
+            P.emax = probe.emax;
+            break;
+        }
+    }
+    P.nchunks = static_cast<int>(cptr.size()) - 1;
+    std::vector<int> row_of(n), pos(n), chunk_of(n), ent_ptr(static_cast<size_t>(n) + 1),
+        grp_ptr(static_cast<size_t>(P.nchunks) + 1);
+    std::vector<int> grp_range, grp_flags, code, src;   // grp_range: (start, end) pairs in chunk positions
+    std::vector<unsigned char> exported(n, 0);
+    for (int c = 0; c < P.nchunks; ++c)
+        for (int q = cptr[c]; q < cptr[c + 1]; ++q) {
+            row_of[q] = row_at(sched[q]);
+            pos[sched[q]] = q - cptr[c];
+            chunk_of[sched[q]] = c;
+        }
+    code.reserve(static_cast<size_t>(IA[n]) / 2 + 1);
+    src.reserve(static_cast<size_t>(IA[n]) / 2 + 1);
+    for (int c = 0; c < P.nchunks; ++c) {
+        const int q0 = cptr[c], q1 = cptr[c + 1];
+        grp_ptr[c] = static_cast<int>(grp_flags.size());
+        int cur = -1;
+        for (int q = q0; q < q1; ++q) {
+            const int k = q - q0, r = row_of[q], l = lev[sidx(r)];
+            if (l != cur) {
+                if (cur >= 0) grp_range.push_back(k);
+                grp_range.push_back(k);
+                grp_flags.push_back(0);
+                cur = l;
+            }
+            ent_ptr[q] = static_cast<int>(code.size());
+            // the reference's per-row order: ascending columns (ILUp_precond.c:26-44)
+            for (int p = ent_begin(r); p < ent_end(r); ++p) {
+                const int dr = JA[p], ds = sidx(dr);
+                if (chunk_of[ds] == c) {
+                    code.push_back(pos[ds]);
+                } else {
+                    code.push_back(-(dr + 1));
+                    grp_flags.back() |= 1;
+                    exported[cptr[chunk_of[ds]] + pos[ds]] = 1;
+                }
+                src.push_back(p);
+            }
+        }
+        grp_range.push_back(q1 - q0);
+    }
+    ent_ptr[n] = static_cast<int>(code.size());
+    grp_ptr[P.nchunks] = static_cast<int>(grp_flags.size());
+    // Self-check of the order the kernel relies on to make progress: a local dependency sits in
+    // an earlier level group of the same chunk, a remote one in an earlier chunk.
+    for (int c = 0; c < P.nchunks; ++c)
+        for (int g = grp_ptr[c]; g < grp_ptr[c + 1]; ++g)
+            for (int k = grp_range[2 * g]; k < grp_range[2 * g + 1]; ++k)
+                for (int e = ent_ptr[cptr[c] + k]; e < ent_ptr[cptr[c] + k + 1]; ++e) {
+                    const int cd = code[e];
+                    const bool ok = cd >= 0 ? cd < grp_range[2 * g] : chunk_of[sidx(-cd - 1)] < c;
+                    FG_CHECK(ok, FEMGPU_ERR_STATE, "packed sweep plan: dependency order violated (chunk %d)", c);
+                }
+    FG_CHECK(P.nchunks == 0 || *std::max_element(cptr.begin() + 1, cptr.end()) <= n, FEMGPU_ERR_STATE,
+             "packed sweep plan: bad chunks");
+    for (int c = 0; c < P.nchunks; ++c)
+        FG_CHECK(cptr[c + 1] - cptr[c] <= P.R, FEMGPU_ERR_STATE, "packed sweep plan: chunk %d too large", c);
+    P.ngroups = static_cast<int>(grp_flags.size());
+    P.E = static_cast<int>(code.size());
+    hipStream_t st = ctx->stream;
+    FG_TRY(P.chunk_ptr.upload(cptr.data(), cptr.size(), st));
+    FG_TRY(P.lvl_ptr.upload(lptr.data(), lptr.size(), st));
+    FG_TRY(P.row_of.upload(row_of.data(), n, st));
+    FG_TRY(P.ent_ptr.upload(ent_ptr.data(), ent_ptr.size(), st));
+    FG_TRY(P.ent_code.upload(code.data(), code.size(), st));
+    FG_TRY(P.ent_src.upload(src.data(), src.size(), st));
+    FG_TRY(P.grp_ptr.upload(grp_ptr.data(), grp_ptr.size(), st));
+    FG_TRY(P.grp_start.upload(grp_range.data(), grp_range.size(), st));
+    FG_TRY(P.grp_flags.upload(grp_flags.data(), grp_flags.size(), st));
+    FG_TRY(P.exported.upload(exported.data(), exported.size(), st));
+    FG_TRY(P.val64.alloc(static_cast<size_t>(P.E) + 1));
+    FG_TRY(P.val32.alloc(static_cast<size_t>(P.E) + 1));
+    FG_TRY(P.dinv64.alloc(static_cast<size_t>(n) + 1));
+    FG_TRY(P.dinv32.alloc(static_cast<size_t>(n) + 1));
+    FG_TRY(P.ctl.alloc(4));
+    FG_TRY(P.mbox.alloc(2 * static_cast<size_t>(n) + 2));
+    FG_TRY(P.mbox32.alloc(static_cast<size_t>(n) + 1));
+    const unsigned ctl0[4] = {0u, 0u, 1u, 0u};   // ticket, finished, epoch (FP32 mailboxes start at epoch 0)
+    FG_HIP(hipMemcpyAsync(P.ctl.p, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, st));
+    {
+        std::vector<unsigned long long> sentinel(2 * static_cast<size_t>(n) + 2, 0xfff8f00df00df00dull);
+        FG_HIP(hipMemcpyAsync(P.mbox.p, sentinel.data(), sizeof(unsigned long long) * sentinel.size(),
+                              hipMemcpyHostToDevice, st));
+        FG_HIP(hipStreamSynchronize(st));
+    }
+    FG_HIP(hipMemsetAsync(P.mbox32.p, 0, sizeof(unsigned long long) * (static_cast<size_t>(n) + 1), st));
+    FG_HIP(hipStreamSynchronize(st));
+    return FEMGPU_OK;
+}
+
+// Update lists of the numeric ILU(0) (ILUp_precond_setup.c:84-166 with p = 0): for row i and each
+// L entry t (column jrow, ascending), every U entry k of row jrow whose column is in row i's
+// pattern updates that slot (or the pivot when the column is i), in U order.
+int build_factor_plan(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag)
+{
+    const int n = A->n;
+    std::vector<int> row_of(n);
+    FG_HIP(hipMemcpy(row_of.data(), A->plo.row_of.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::vector<int4> info(static_cast<size_t>(n) + 1);
+    std::vector<int2> upd;
+    upd.reserve(static_cast<size_t>(IA[n]) * 2);
+    std::vector<int> slot(n, -1);
+    int maxrow = 0;
+    for (int q = 0; q < n; ++q) {
+        const int i = row_of[q], p0 = IA[i], len = IA[i + 1] - p0, nL = diag[i] - p0;
+        maxrow = std::max(maxrow, len);
+        info[q] = make_int4(p0, len, nL, static_cast<int>(upd.size()));
+        for (int p = p0; p < p0 + len; ++p) slot[JA[p]] = p - p0;
+        for (int t = 0; t < nL; ++t) {
+            const int jrow = JA[p0 + t];
+            for (int k = diag[jrow] + 1; k < IA[jrow + 1]; ++k) {
+                const int dest = slot[JA[k]];
+                if (dest < 0) continue;
+                upd.push_back(make_int2(k, dest | (t << 16)));
+            }
+        }
+        for (int p = p0; p < p0 + len; ++p) slot[JA[p]] = -1;
+        FG_CHECK(len < 65536 && nL < 32768, FEMGPU_ERR_ARG, "row %d too long for the factorisation plan", i);
+    }
+    info[n] = make_int4(0, 0, 0, static_cast<int>(upd.size()));   // sentinel: end of the last list
+    A->fplan.maxrow = maxrow;
+    A->fplan.nupd = static_cast<int>(upd.size());
+    FG_TRY(A->fplan.rowinfo.upload(info.data(), info.size(), ctx->stream));
+    FG_TRY(A->fplan.upd.upload(upd.data(), upd.size(), ctx->stream));
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    return FEMGPU_OK;
+}
+
+}  // namespace fg

@@ -1,0 +1,342 @@
+// Level-scheduled ILU(0) triangular sweeps, second design: LDS-staged row chunks with mailbox hand-off.
+//
+// The sweep order (ascending rows for L, descending for U) is cut into chunks of R consecutive rows;
+// one workgroup per chunk, chunks claimed in sweep order from a ticket. Inside a chunk, rows are
+// scheduled by dependency level (groups of equal level, a barrier between groups). At plan time
+// every dependency of a row is encoded either as the LDS slot of a row of the same chunk, or as a
+// remote row. A workgroup first stages everything its chunk needs (right-hand side, packed factor
+// entries, inverse pivots) into LDS -- without waiting on anybody, so the staging of all resident
+// chunks overlaps the sweep front -- and only then walks its groups.
+//
+// Remote values travel through per-row mailboxes of 64-bit words {epoch:32 | payload:32} (one word
+// for FP32 rows, two for FP64 rows), written and polled with relaxed agent-scope atomics. Because
+// the data word carries its own validity (the epoch of the current launch), no fences, flags or
+// resets are needed, and every dependency is waited on individually. The epoch advances once per
+// launch (by the last chunk to finish, which also rewinds the ticket), so launches need no reset
+// kernel or memset in between. Waits are bounded: on expiry the sticky error flag is raised.
+//
+// Arithmetic per row follows ILUp_precond.c:26-44 exactly (contraction off): the result is the
+// serial sweep's, bit for bit.
+#include <climits>
+
+#include "common.h"
+
+namespace fg {
+
+struct PackedArgs {
+    int n, R, nchunks, emax;
+    const int *chunk_ptr, *row_of, *ent_ptr, *ent_code, *grp_ptr, *grp_start, *grp_flags;   // grp_start: (start, end) pairs
+    const unsigned char *exported;
+    unsigned *ctl;                 // [0] ticket [1] done [2] FP32 epoch [3] FP64 launch parity counter
+    unsigned long long *mbox;      // per row: 1 (FP32) or 2 (FP64) words
+    int *err;
+    unsigned spin_limit;
+};
+
+template <typename T>
+struct MailboxIO;
+
+template <>
+struct MailboxIO<float> {
+    static __device__ __forceinline__ void put(unsigned long long *mb, int row, unsigned epoch, float v)
+    {
+        const unsigned long long w = (static_cast<unsigned long long>(epoch) << 32) | __float_as_uint(v);
+        __hip_atomic_store(mb + row, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+// FP64 rows: one word holding the double itself; "not yet written" is a sentinel NaN pattern that
+// no arithmetic produces. Two mailbox arrays alternate between launches (epoch parity): a launch
+// reads and writes the active one and returns its own exported rows of the idle one to the
+// sentinel, so the next launch finds them reset without a separate pass.
+constexpr unsigned long long kSentinel = 0xfff8f00df00df00dull;
+
+template <>
+struct MailboxIO<double> {
+    static __device__ __forceinline__ unsigned long long *active(const PackedArgs &S, unsigned epoch)
+    {
+        return S.mbox + (epoch & 1u) * static_cast<size_t>(S.n);
+    }
+    static __device__ __forceinline__ void put(unsigned long long *mb, int row, unsigned epoch, double v)
+    {
+        (void)epoch;
+        __hip_atomic_store(mb + row, static_cast<unsigned long long>(__double_as_longlong(v)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+// Poll the mailboxes of up to N remote dependencies together (one memory round trip per try).
+template <typename T, int N>
+struct RemoteGather;
+
+template <int N>
+struct RemoteGather<double, N> {
+    static __device__ __forceinline__ void run(const PackedArgs &S, unsigned epoch, const int *rows, int cnt, double *out)
+    {
+        const unsigned long long *mb = MailboxIO<double>::active(S, epoch);
+        unsigned long long w[N];
+        unsigned spins = 0;
+        unsigned pending = (cnt >= 32) ? 0xffffffffu : ((1u << cnt) - 1u);
+        while (pending) {
+#pragma unroll
+            for (int t = 0; t < N; ++t)
+                if (pending & (1u << t)) w[t] = __hip_atomic_load(mb + rows[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int t = 0; t < N; ++t)
+                if ((pending & (1u << t)) && w[t] != kSentinel) {
+                    out[t] = __longlong_as_double(static_cast<long long>(w[t]));
+                    pending &= ~(1u << t);
+                }
+            if (!pending) break;
+            if (++spins > S.spin_limit ||
+                ((spins & 255u) == 0u && __hip_atomic_load(S.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                __hip_atomic_store(S.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+                for (int t = 0; t < N; ++t) if (pending & (1u << t)) out[t] = 0.0;
+                return;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+};
+
+template <int N>
+struct RemoteGather<float, N> {
+    static __device__ __forceinline__ void run(const PackedArgs &S, unsigned epoch, const int *rows, int cnt, float *out)
+    {
+        unsigned long long w[N];
+        unsigned spins = 0;
+        unsigned pending = (cnt >= 32) ? 0xffffffffu : ((1u << cnt) - 1u);
+        while (pending) {
+#pragma unroll
+            for (int t = 0; t < N; ++t)
+                if (pending & (1u << t)) w[t] = __hip_atomic_load(S.mbox + rows[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int t = 0; t < N; ++t)
+                if ((pending & (1u << t)) && static_cast<unsigned>(w[t] >> 32) == epoch) {
+                    out[t] = __uint_as_float(static_cast<unsigned>(w[t]));
+                    pending &= ~(1u << t);
+                }
+            if (!pending) break;
+            if (++spins > S.spin_limit ||
+                ((spins & 255u) == 0u && __hip_atomic_load(S.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                __hip_atomic_store(S.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+                for (int t = 0; t < N; ++t) if (pending & (1u << t)) out[t] = 0.0f;
+                return;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+};
+
+constexpr int kPackedThreads = 512;
+
+// Barrier that orders LDS only: global stores (results, mailboxes) stay in flight across it, so a
+// level step never waits for a memory round trip.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// UPPER: x = (b - U x) * Dinv; else x = b - L x. b may alias x (each row reads its own b during
+// staging, before any row of the launch is written, and other chunks never read x: they read the
+// mailboxes).
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(kPackedThreads) void k_sweep_packed(PackedArgs S, const T *__restrict__ val,
+                                                                 const T *__restrict__ dinv, const T *b, T *x,
+                                                                 const int *__restrict__ skip)
+{
+#pragma clang fp contract(off)
+    if (skip && *skip) return;   // uniform: written by an earlier launch
+    extern __shared__ unsigned char smem_raw[];
+    __shared__ int s_chunk;
+    __shared__ unsigned s_epoch;
+    const int R = S.R;
+    T *xs = reinterpret_cast<T *>(smem_raw);   // results of the chunk's rows
+    T *bs = xs + R;                            // right-hand side
+    T *ds = bs + R;                            // inverse pivots (upper)
+    T *vs = ds + R;                            // packed factor values
+    int *cs = reinterpret_cast<int *>(vs + S.emax);   // packed dependency codes
+    int *os = cs + S.emax;                             // R+1 entry offsets
+    int *rs = os + R + 1;                              // row index, bit 31: exported
+    int *gs = rs + R;                                  // group ranges of the chunk (2 per group)
+    int *gf = gs + 2 * R;                              // group flags (bit 0: reads earlier chunks)
+    if (threadIdx.x == 0) {
+        const unsigned t = __hip_atomic_fetch_add(&S.ctl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_epoch = __hip_atomic_load(&S.ctl[sizeof(T) == 8 ? 3 : 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t >= static_cast<unsigned>(S.nchunks)) {
+            __hip_atomic_store(S.err, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_chunk = -1;
+        } else {
+            s_chunk = static_cast<int>(t);
+        }
+    }
+    __syncthreads();
+    const int c = s_chunk;
+    if (c < 0) return;
+    const unsigned epoch = s_epoch;
+    unsigned long long *mbw = sizeof(T) == 8 ? S.mbox + (epoch & 1u) * static_cast<size_t>(S.n) : S.mbox;
+    const int q0 = S.chunk_ptr[c], q1 = S.chunk_ptr[c + 1], rows = q1 - q0;
+    const int e0 = S.ent_ptr[q0], e1 = S.ent_ptr[q1];
+    // ---- staging (no waiting on other chunks)
+    for (int k = threadIdx.x; k < rows; k += blockDim.x) {
+        const int row = S.row_of[q0 + k];
+        bs[k] = b[row];
+        if (UPPER) ds[k] = dinv[q0 + k];
+        os[k] = S.ent_ptr[q0 + k] - e0;
+        rs[k] = row | (S.exported[q0 + k] ? int(0x80000000u) : 0);
+    }
+    if (threadIdx.x == 0) os[rows] = e1 - e0;
+    const int g0 = S.grp_ptr[c], ng = S.grp_ptr[c + 1] - g0;
+    for (int g = threadIdx.x; g < 2 * ng; g += blockDim.x) gs[g] = S.grp_start[2 * g0 + g];
+    for (int g = threadIdx.x; g < ng; g += blockDim.x) gf[g] = S.grp_flags[g0 + g];
+    for (int e = threadIdx.x; e < e1 - e0; e += blockDim.x) {
+        cs[e] = S.ent_code[e0 + e];
+        vs[e] = val[e0 + e];
+    }
+    __syncthreads();
+    // ---- level groups: LDS reads, global stores left in flight
+    for (int g = 0; g < ng; ++g) {
+        const int k0 = gs[2 * g], k1 = gs[2 * g + 1];
+        for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+            T acc = bs[k];
+            const int pb = os[k], pe = os[k + 1];
+            if (gf[g]) {
+                // rows that read earlier chunks: fetch every remote value in one polling loop,
+                // then accumulate in the reference's order
+                constexpr int N = 8;
+                for (int p = pb; p < pe; p += N) {
+                    const int cnt = min(N, pe - p);
+                    int rrows[N];
+                    int nr = 0;
+                    T rv[N];
+#pragma unroll
+                    for (int t = 0; t < N; ++t)
+                        if (t < cnt && cs[p + t] < 0) rrows[nr++] = -cs[p + t] - 1;
+                    if (nr) RemoteGather<T, N>::run(S, epoch, rrows, nr, rv);
+                    int r = 0;
+#pragma unroll
+                    for (int t = 0; t < N; ++t) {
+                        if (t >= cnt) break;
+                        const int code = cs[p + t];
+                        const T xv = code >= 0 ? xs[code] : rv[r++];
+                        acc -= xv * vs[p + t];
+                    }
+                }
+            } else {
+                for (int p = pb; p < pe; ++p) acc -= xs[cs[p]] * vs[p];
+            }
+            if (UPPER) acc *= ds[k];
+            xs[k] = acc;
+            const int rw = rs[k];
+            const int row = rw & 0x7fffffff;
+            x[row] = acc;
+            if (rw < 0) MailboxIO<T>::put(mbw, row, epoch, acc);
+        }
+        lds_barrier();
+    }
+    if (sizeof(T) == 8) {
+        unsigned long long *idle = S.mbox + ((epoch + 1u) & 1u) * static_cast<size_t>(S.n);
+        for (int k = threadIdx.x; k < rows; k += blockDim.x)
+            if (rs[k] < 0) idle[rs[k] & 0x7fffffff] = kSentinel;
+    }
+    // ---- last chunk out rewinds the ticket and advances the epoch for the next launch
+    if (threadIdx.x == 0) {
+        const unsigned done = __hip_atomic_fetch_add(&S.ctl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (done == static_cast<unsigned>(S.nchunks) - 1u) {
+            __hip_atomic_store(&S.ctl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&S.ctl[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // FP32 tags skip 0 (the mailboxes' initial content); FP64 only uses the parity
+            unsigned ne = epoch + 1u;
+            if (sizeof(T) == 4 && ne == 0u) ne = 1u;
+            __hip_atomic_store(&S.ctl[sizeof(T) == 8 ? 3 : 2], ne, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Packed factor values (and inverse pivots in sweep order) from the factors in A's layout.
+template <typename T>
+__global__ void k_pack_values(int E, const int *__restrict__ src, const double *__restrict__ LU, T *__restrict__ val,
+                              int n, const int *__restrict__ row_of, const double *__restrict__ Dinv, T *__restrict__ dinv)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x)
+        val[e] = static_cast<T>(LU[src[e]]);
+    if (dinv)
+        for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+            dinv[q] = static_cast<T>(Dinv[row_of[q]]);
+}
+
+static PackedArgs packed_args(femgpu_ctx *ctx, PackedSweep &P, bool fp32)
+{
+    PackedArgs a;
+    a.n = P.n;
+    a.R = P.R;
+    a.nchunks = P.nchunks;
+    a.emax = P.emax;
+    a.chunk_ptr = P.chunk_ptr.p;
+    a.row_of = P.row_of.p;
+    a.ent_ptr = P.ent_ptr.p;
+    a.ent_code = P.ent_code.p;
+    a.grp_ptr = P.grp_ptr.p;
+    a.grp_flags = P.grp_flags.p;
+    a.grp_start = P.grp_start.p;
+    a.exported = P.exported.p;
+    a.ctl = P.ctl.p;
+    a.mbox = fp32 ? P.mbox32.p : P.mbox.p;
+    a.err = ctx->err.p;
+    a.spin_limit = 1u << 22;
+    return a;
+}
+
+template <typename T>
+size_t packed_lds_bytes(const PackedSweep &P)
+{
+    return sizeof(T) * (3 * static_cast<size_t>(P.R) + P.emax) + sizeof(int) * (static_cast<size_t>(P.emax) + 5 * P.R + 1);
+}
+
+template size_t packed_lds_bytes<double>(const PackedSweep &);
+template size_t packed_lds_bytes<float>(const PackedSweep &);
+
+int packed_refresh(femgpu_ctx *ctx, femgpu_csr *A)
+{
+    for (PackedSweep *P : {&A->plo, &A->pup}) {
+        const bool up = (P == &A->pup);
+        const int grid = std::max(1, std::min(4096, (P->E + 255) / 256));
+        k_pack_values<double><<<grid, 256, 0, ctx->stream>>>(P->E, P->ent_src.p, A->LU.p, P->val64.p, P->n, P->row_of.p,
+                                                             A->Dinv.p, up ? P->dinv64.p : nullptr);
+        k_pack_values<float><<<grid, 256, 0, ctx->stream>>>(P->E, P->ent_src.p, A->LU.p, P->val32.p, P->n, P->row_of.p,
+                                                            A->Dinv.p, up ? P->dinv32.p : nullptr);
+        FG_HIP(hipGetLastError());
+    }
+    return FEMGPU_OK;
+}
+
+template <typename T>
+static int packed_pair(femgpu_ctx *ctx, femgpu_csr *A, const T *b, T *x, const int *skip, const T *vlo,
+                       const T *vup, const T *dup)
+{
+    if (A->n == 0) return FEMGPU_OK;
+    const size_t llo = packed_lds_bytes<T>(A->plo), lup = packed_lds_bytes<T>(A->pup);
+    k_sweep_packed<T, false><<<A->plo.nchunks, kPackedThreads, llo, ctx->stream>>>(packed_args(ctx, A->plo, sizeof(T) == 4), vlo,
+                                                                                     nullptr, b, x, skip);
+    FG_HIP(hipGetLastError());
+    k_sweep_packed<T, true><<<A->pup.nchunks, kPackedThreads, lup, ctx->stream>>>(packed_args(ctx, A->pup, sizeof(T) == 4), vup, dup,
+                                                                                    x, x, skip);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+int launch_sweep64(femgpu_ctx *ctx, femgpu_csr *A, const double *b, double *x, const int *skip)
+{
+    return packed_pair<double>(ctx, A, b, x, skip, A->plo.val64.p, A->pup.val64.p, A->pup.dinv64.p);
+}
+
+int launch_sweep32(femgpu_ctx *ctx, femgpu_csr *A, const float *b, float *x, const int *skip)
+{
+    return packed_pair<float>(ctx, A, b, x, skip, A->plo.val32.p, A->pup.val32.p, A->pup.dinv32.p);
+}
+
+}  // namespace fg

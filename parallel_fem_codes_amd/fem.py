@@ -221,9 +221,10 @@ class CsrMatrix:
         N.check(N.lib().femgpu_csr_values_changed(self.h))
 
     def sweep_info(self):
-        a, b, c = C.c_int(), C.c_int(), C.c_int()
-        N.check(N.lib().femgpu_csr_sweep_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
-        return dict(depth_lower=a.value, depth_upper=b.value, nchunks=c.value)
+        a, b, c, d, e = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        N.check(N.lib().femgpu_csr_sweep_info(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d), C.byref(e)))
+        return dict(depth_lower=a.value, depth_upper=b.value, nchunks=c.value, rows_per_chunk=d.value,
+                    lds_limit=e.value)
 
     def csrmv(self, P, Q):
         fn = N.lib().femgpu_csrmv_f32 if P.dtype.itemsize == 4 else N.lib().femgpu_csrmv

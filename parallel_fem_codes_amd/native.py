@@ -75,7 +75,7 @@ _SIGS = {
     "femgpu_csr_set_values": (I, [vp, vp, I]),
     "femgpu_csr_values_ptr": (I, [vp, C.POINTER(vp)]),
     "femgpu_csr_values_changed": (I, [vp]),
-    "femgpu_csr_sweep_info": (I, [vp, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+    "femgpu_csr_sweep_info": (I, [vp, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
     "femgpu_csrmv": (I, [vp, vp, vp]),
     "femgpu_csrmv_f32": (I, [vp, vp, vp]),
     "femgpu_ilu0_factor": (I, [vp]),

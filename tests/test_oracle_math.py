@@ -1,0 +1,193 @@
+"""Pins of the oracle by mathematics, for what the reference fixtures leave open: the transient
+assembly (its reference sources are not buildable here: TranspEquation.h includes
+/usr/include/mpi/mpi.h), block EBE products, ILU(0) on elimination-closed patterns, GMRES
+termination, PCG, and the P1 reproduction of constants. Each check is chosen so that a dropped
+term, a wrong sign or a transposed operand fails it."""
+import numpy as np
+import pytest
+
+
+def csr_from_dense(A):
+    IA = np.concatenate([[0], np.cumsum((A != 0).sum(1))]).astype(np.int32)
+    JA = np.concatenate([np.nonzero(r)[0] for r in A]).astype(np.int32)
+    AA = np.concatenate([r[r != 0] for r in A] + [[0.0]])
+    return IA, JA, AA
+
+
+def dense_ilu0(A):
+    """Textbook IKJ ILU(0) on the dense pattern of A (Saad, Iterative Methods, Alg. 10.4)."""
+    n = len(A)
+    LU = A.copy()
+    P = A != 0
+    for i in range(1, n):
+        for k in range(i):
+            if not P[i, k]:
+                continue
+            LU[i, k] /= LU[k, k]
+            for j in range(k + 1, n):
+                if P[i, j]:
+                    LU[i, j] -= LU[i, k] * LU[k, j]
+    return LU
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_ilu0_equals_textbook_and_matches_A_on_pattern(oracle, seed):
+    rng = np.random.default_rng(seed)
+    n = 40
+    P = rng.random((n, n)) < 0.12
+    P = P | P.T | np.eye(n, dtype=bool)
+    A = np.where(P, rng.standard_normal((n, n)), 0.0) + np.eye(n) * n
+    IA, JA, AA = csr_from_dense(A)
+    LU, Dinv, rc = oracle.ilu0_factor(IA, JA, AA)
+    assert rc == 0
+    ref = dense_ilu0(A)
+    dense = np.zeros_like(A)
+    for i in range(n):
+        dense[i, JA[IA[i]:IA[i + 1]]] = LU[IA[i]:IA[i + 1]]
+    assert np.allclose(dense[P], ref[P], rtol=1e-12, atol=1e-12)
+    assert np.allclose(Dinv, 1.0 / np.diag(ref), rtol=1e-14)
+    L = np.tril(dense, -1) + np.eye(n)
+    U = np.triu(dense)
+    assert np.allclose((L @ U)[P], A[P], rtol=1e-11, atol=1e-11)   # ILU(0) is exact on A's pattern
+
+
+def test_ilu0_is_exact_lu_on_tridiagonal_and_banded(oracle):
+    rng = np.random.default_rng(7)
+    for bw in (1, 3):
+        n = 60
+        A = np.zeros((n, n))
+        for d in range(-bw, bw + 1):
+            A += np.diag(rng.standard_normal(n - abs(d)), d)
+        A += np.eye(n) * 8
+        IA, JA, AA = csr_from_dense(A)
+        LU, Dinv, rc = oracle.ilu0_factor(IA, JA, AA)
+        b = rng.standard_normal(n)
+        x = oracle.ilu0_apply(IA, JA, LU, Dinv, b)
+        assert np.allclose(A @ x, b, rtol=0, atol=1e-12)   # banded pattern is closed under elimination
+
+
+def test_gmres_terminates_in_d_steps_for_d_distinct_eigenvalues(oracle):
+    n, d = 300, 7
+    ev = np.repeat(np.arange(1, d + 1, dtype=float), n // d + 1)[:n]
+    IA = np.arange(n + 1, dtype=np.int32)
+    JA = np.arange(n, dtype=np.int32)
+    AA = np.append(ev, 0.0)
+    S = oracle.make_system(0, n, IA=IA, JA=JA, AA=AA)
+    b = np.random.default_rng(1).standard_normal(n)
+    x, it, rho, cyc = oracle.pgmres(S, b, kmax=30, lmax=5, tol=1e-12)
+    assert it == d and cyc == 1
+    assert np.allclose(x, b / ev, rtol=1e-10)
+
+
+def test_gmres_estimate_equals_true_residual_and_restart_semantics(oracle):
+    rng = np.random.default_rng(2)
+    n = 120
+    A = np.eye(n) * 4 + rng.standard_normal((n, n)) * 0.3
+    IA, JA, AA = csr_from_dense(A)
+    S = oracle.make_system(0, n, IA=IA, JA=JA, AA=AA)
+    b = rng.standard_normal(n)
+    for kmax in (5, 12):
+        x, it, rho, cyc = oracle.pgmres(S, b, kmax=kmax, lmax=1, tol=1e-30)
+        assert it == kmax - 1 and cyc == 1           # kmax-1 Arnoldi steps per cycle
+        assert abs(np.linalg.norm(b - A @ x) - rho) < 1e-10 * np.linalg.norm(b)
+
+
+def _mesh(oracle, m, right=True):
+    x, y, ien = oracle.mesh_structured(m, m, right=right)
+    t = oracle.unit_square_types(x, y)
+    invP, neq = oracle.number_unknowns(t)
+    lm = oracle.fill_lm(ien, t, invP, neq)
+    return x, y, ien, t, invP, neq, lm
+
+
+@pytest.mark.parametrize("shock", [0, 1])
+def test_element_rows_annihilate_constants(oracle, shock):
+    """Diffusion, convection, SUPG and CAU blocks all vanish on constants (gamma = 0), so every
+    element row sums to zero -- a dropped or sign-flipped term breaks this."""
+    x, y, ien, t, invP, neq, lm = _mesh(oracle, 13, right=False)
+    pb = oracle.make_problem(k=3e-3, bx=(0.3, -1.2, 0.7), by=(-0.4, 0.9, 1.1), shock=shock)
+    u = np.random.default_rng(3).standard_normal(neq)
+    Ke, F = oracle.build_transport_supg(x, y, t, invP, ien, oracle.inflow_ubc(x, y), u, pb, neq)
+    K = Ke.reshape(-1, 3, 3)
+    assert np.abs(K.sum(2)).max() < 1e-12 * np.abs(K).max()
+
+
+def test_teste_known_answer(oracle):
+    """TESTE (k = 1, beta = 0, f = 0, u = 10 on the boundary): the discrete solution is u = 10."""
+    x, y, ien, t, invP, neq, lm = _mesh(oracle, 20)
+    c = oracle.csr_init(lm, 3, neq, reorder="SYMRCM", type_=t, invP=invP)
+    pb = oracle.make_problem(k=1.0, bx=(0, 0, 0), by=(0, 0, 0))
+    Ke, F = oracle.build_transport_supg(x, y, t, c["invP"], ien, np.full(len(x), 10.0), np.zeros(neq), pb, neq)
+    AA = oracle.csr_assemble(c["scheme"], Ke, c["nnz"])
+    LU, Dinv, rc = oracle.ilu0_factor(c["IA"], c["JA"], AA)
+    B = oracle.ilu0_apply(c["IA"], c["JA"], LU, Dinv, F)
+    S = oracle.make_system(0, neq, IA=c["IA"], JA=c["JA"], AA=AA, ilu=(c["IA"], c["JA"], LU, Dinv))
+    X, it, rho, cyc = oracle.pgmres(S, B, kmax=30, lmax=100, tol=1e-12)
+    assert np.abs(X - 10).max() < 1e-9
+    Xc, itc = oracle.pcg(oracle.make_system(0, neq, IA=c["IA"], JA=c["JA"], AA=AA), F, itmax=500, tol=1e-12)
+    assert np.abs(Xc - 10).max() < 1e-9 and 0 < itc < 500
+
+
+def test_transient_assembly_against_steady_build(oracle):
+    """Build_M_K_R_SUPG vs the reference-pinned Build_K_F_SUPG: with du = 0 the SUPG/CAU stiffness
+    is the steady one, Ae - alpha*dt*Ke is the SUPG mass (consistent part sums to the element area,
+    upwind part tau*C_a per row), and R = F_steady - K u on the unknowns."""
+    x, y, ien, t, invP, neq, lm = _mesh(oracle, 17)
+    c = oracle.csr_init(lm, 3, neq, reorder="SYMRCM", permute_lm=True, type_=t, invP=invP)
+    ubc = oracle.inflow_ubc(x, y)
+    rng = np.random.default_rng(4)
+    u = rng.standard_normal(neq) * 0.2
+    alpha, dt = 0.5, 1e-3
+    for shock in (0, 1):
+        pb = oracle.make_problem(k=1e-3, shock=shock)
+        Ae, R = oracle.build_transport_transient(x, y, t, c["invP"], ien, c["lm"], ubc, u, np.zeros(neq), pb,
+                                                 alpha, dt, neq)
+        Ke, F = oracle.build_transport_supg(x, y, t, c["invP"], ien, ubc, u, pb, neq)
+        Me = (Ae - alpha * dt * Ke).reshape(-1, 3, 3)
+        X, Y = x[ien.reshape(-1, 3)], y[ien.reshape(-1, 3)]
+        area = 0.5 * ((X[:, 1] - X[:, 0]) * (Y[:, 2] - Y[:, 0]) - (X[:, 0] - X[:, 2]) * (Y[:, 0] - Y[:, 1]))
+        consistent = np.array([[2, 1, 1], [1, 2, 1], [1, 1, 2]]) / 12.0
+        upw = Me - area[:, None, None] * consistent
+        assert np.allclose(upw, upw[:, :, :1], rtol=0, atol=1e-14)         # tau*C_a constant along row a
+        assert np.allclose(upw[:, :, 0].sum(1), 0, atol=1e-14)             # sum_a C_a = 0
+        AA = oracle.csr_assemble(c["scheme"], Ke, c["nnz"])
+        assert np.allclose(R[:neq], F - oracle.csrmv(c["IA"], c["JA"], AA, u), rtol=0, atol=1e-13)
+        assert R[neq] == 0.0
+
+
+@pytest.mark.parametrize("s", [3, 9, 12])
+def test_block_ebemv_against_dense_assembly(oracle, s):
+    ndof = s // 3
+    x, y, ien = oracle.mesh_structured(6, 5)
+    t = np.repeat(oracle.unit_square_types(x, y), ndof).astype(np.int32)
+    invP, neq = oracle.number_unknowns(t, ndof)
+    lm = oracle.fill_lm(ien, t, invP, neq, ndof)
+    nel = len(ien) // 3
+    A = np.random.default_rng(s).standard_normal(nel * s * s)
+    D = np.zeros((neq, neq))
+    for e in range(nel):
+        L = lm[e * s:(e + 1) * s]
+        blk = A[e * s * s:(e + 1) * s * s].reshape(s, s)
+        for a in range(s):
+            for b in range(s):
+                if L[a] < neq and L[b] < neq:
+                    D[L[a], L[b]] += blk[a, b]
+    p = np.random.default_rng(1).standard_normal(neq)
+    assert np.allclose(oracle.ebemv_block(lm, A, p, neq, s), D @ p, rtol=0, atol=1e-12)
+    if s == 3:
+        Perm, invPerm = np.arange(neq + 1, dtype=np.int32), np.arange(neq + 1, dtype=np.int32)
+        assert np.allclose(oracle.ebemv(lm, Perm, invPerm, A, p, neq), D @ p, rtol=0, atol=1e-12)
+
+
+def test_pcg_matches_direct_solve(oracle):
+    rng = np.random.default_rng(5)
+    n = 80
+    B = rng.standard_normal((n, n))
+    A = B @ B.T + n * np.eye(n)
+    A[np.abs(A) < 2.0] = 0.0
+    A = (A + A.T) / 2 + n * np.eye(n)
+    IA, JA, AA = csr_from_dense(A)
+    S = oracle.make_system(0, n, IA=IA, JA=JA, AA=AA)
+    b = rng.standard_normal(n)
+    x, it = oracle.pcg(S, b, itmax=500, tol=1e-13)
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
