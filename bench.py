@@ -178,8 +178,7 @@ def main():
     roofline = dict(bound="hbm", kernel="ILU(0) apply: lower+upper level-scheduled sweeps (FP64)",
                     achieved=round(dom["GBs"], 2), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(dom["GBs"] / HBM_PEAK_GBS, 5), traffic=None,
-                    bytes_per_launch=bytes_ilu, dependency_depth=[si["depth_lower"], si["depth_upper"]],
-                    chunks=si["nchunks"], rows_per_chunk=si["rows_per_chunk"], lds_limit=si["lds_limit"])
+                    bytes_per_launch=bytes_ilu, sweep=si)
 
     cpu = None
     if not a.no_cpu:

@@ -105,10 +105,22 @@ int femgpu_csr_set_values(femgpu_csr *A, const double *AA, int on_device);
  * assembly; call femgpu_csr_values_changed afterwards to refresh the FP32 copy. */
 int femgpu_csr_values_ptr(femgpu_csr *A, double **d_AA);
 int femgpu_csr_values_changed(femgpu_csr *A);
-/* Sweep scheduler statistics: depth of the lower/upper dependency DAGs, chunk count and row bound
- * per chunk of the lower sweep, and the LDS bytes available to one workgroup. */
-int femgpu_csr_sweep_info(femgpu_csr *A, int *depth_lower, int *depth_upper, int *nchunks,
-                          int *rows_per_chunk, int *lds_limit);
+/* ILU(0) sweep scheduler description (no reference counterpart: the reference sweeps serially). */
+typedef struct femgpu_sweep_info {
+    int kernel;            /* 0: streamed single-workgroup sweep, 1: packed multi-workgroup sweep */
+    int depth_lower;       /* dependency levels of the L sweep */
+    int depth_upper;       /* dependency levels of the U sweep */
+    int steps_lower;       /* streamed: barrier steps of the L sweep (levels split to <= 512 rows) */
+    int steps_upper;
+    int ring;              /* streamed: LDS ring entries holding recent results */
+    int entries_per_row;   /* streamed: padded dependency entries per row */
+    int chunks_lower;      /* packed: row chunks of the L sweep */
+    int rows_per_chunk;    /* packed: row bound per chunk */
+    int lds_limit;         /* LDS bytes available to one workgroup */
+} femgpu_sweep_info;
+int femgpu_csr_sweep_info(femgpu_csr *A, femgpu_sweep_info *info);
+/* Select the sweep kernel: 0 = streamed when the plan allows it (default), 1 = packed. */
+int femgpu_csr_set_sweep_kernel(femgpu_csr *A, int kernel);
 
 /* csrmv (CSR/csrmv.c:4-65), local block: Q = A P. FP64 and FP32 (FP32 copy of the values). */
 int femgpu_csrmv(femgpu_csr *A, const double *d_P, double *d_Q);

@@ -192,6 +192,8 @@ extern "C" int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA,
     if (!rc) rc = build_sweep_plan(ctx, n, IA, JA, true, R, A->lower);
     if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), false, A->plo);
     if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), true, A->pup);
+    if (!rc) rc = build_stream_plan(ctx, n, IA, JA, diag.data(), false, A->slo);
+    if (!rc) rc = build_stream_plan(ctx, n, IA, JA, diag.data(), true, A->sup);
     if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data());
     if (!rc && hipMemsetAsync(A->AA.p, 0, sizeof(double) * (nnz + 1), s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
@@ -233,14 +235,28 @@ extern "C" int femgpu_csr_values_changed(femgpu_csr *A)
     return launch_to_f32(A->ctx, A->AA.p, A->AA32.p, A->nnz);
 }
 
-extern "C" int femgpu_csr_sweep_info(femgpu_csr *A, int *dl, int *du, int *nchunks, int *rows_per_chunk, int *lds_limit)
+extern "C" int femgpu_csr_sweep_info(femgpu_csr *A, femgpu_sweep_info *info)
 {
-    FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_csr_sweep_info: null");
-    if (dl) *dl = A->plo.depth;
-    if (du) *du = A->pup.depth;
-    if (nchunks) *nchunks = A->plo.nchunks;
-    if (rows_per_chunk) *rows_per_chunk = A->plo.R;
-    if (lds_limit) *lds_limit = static_cast<int>(A->ctx->lds_limit);
+    FG_CHECK(A && info, FEMGPU_ERR_ARG, "femgpu_csr_sweep_info: null");
+    memset(info, 0, sizeof(*info));
+    const bool stream = A->sweep_kernel == 0 && A->slo.ok && A->sup.ok;
+    info->kernel = stream ? 0 : 1;
+    info->depth_lower = A->plo.depth;
+    info->depth_upper = A->pup.depth;
+    info->steps_lower = A->slo.steps;
+    info->steps_upper = A->sup.steps;
+    info->ring = A->slo.W;
+    info->entries_per_row = A->slo.K;
+    info->chunks_lower = A->plo.nchunks;
+    info->rows_per_chunk = A->plo.R;
+    info->lds_limit = static_cast<int>(A->ctx->lds_limit);
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_csr_set_sweep_kernel(femgpu_csr *A, int kernel)
+{
+    FG_CHECK(A && (kernel == 0 || kernel == 1), FEMGPU_ERR_ARG, "femgpu_csr_set_sweep_kernel: bad arguments");
+    A->sweep_kernel = kernel;
     return FEMGPU_OK;
 }
 

@@ -106,6 +106,27 @@ struct PackedSweep {
     DevBuf<unsigned long long> mbox32;  // FP32: {epoch, value} word per row
 };
 
+// Single-workgroup streamed sweep (sweep.hip, k_sweep_stream): the level schedule cut into steps of
+// at most kStreamThreads rows (a level wider than that is split), row data padded to K dependency
+// entries per row in schedule order, and each dependency named by its slot in an LDS ring of W
+// results (schedule position mod W; slot W holds a zero for padding). The plan exists only when
+// every dependency is still in the ring when read (W >= end of the reading step - position of the
+// dependency) and the ring fits in LDS; otherwise the packed multi-workgroup sweep is used.
+constexpr int kStreamThreads = 512;
+constexpr int kStreamPad = 16;     // step count padded to a multiple of this (twice the largest lookahead)
+
+struct StreamSweep {
+    bool ok = false;
+    int n = 0, nsteps = 0, steps = 0, K = 0, W = 0, depth = 0;   // nsteps: padded, steps: real
+    DevBuf<int> stepq;              // step -> first schedule position (nsteps + 2*kStreamPad + 1, tail = n)
+    DevBuf<int> row_of;             // schedule position -> row (n)
+    DevBuf<unsigned short> slot;    // n*K ring slots
+    DevBuf<int> src;                // n*K positions in A's layout, -1 for padding (value refresh)
+    DevBuf<double> val64, dinv64;   // n*K factor values / n inverse pivots (upper sweep) in schedule order
+    DevBuf<float> val32, dinv32;
+    DevBuf<double> sink;            // kStreamThreads idle-thread results
+};
+
 // Numeric ILU(0) plan: for every row in the lower sweep's level schedule, its CSR position, length,
 // number of L entries, and its update list -- (U entry position k of a dependency row, target slot
 // in this row, index t of the L entry it belongs to) -- in the serial factorisation's order.
@@ -146,6 +167,8 @@ struct femgpu_csr {
     bool factored = false;
     fg::SweepPlan lower;            // scheduler of the numeric factorisation
     fg::PackedSweep plo, pup;       // packed lower / upper sweeps
+    fg::StreamSweep slo, sup;       // streamed single-workgroup sweeps (used when both exist)
+    int sweep_kernel = 0;           // 0: streamed when available, 1: packed
     fg::FactorPlan fplan;           // numeric factorisation
     int spmv_lanes = 8;             // lanes per row of the SpMV kernel
 };
@@ -179,6 +202,9 @@ int build_sweep_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, bool 
 int build_packed_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, const int *diag, bool upper,
                       PackedSweep &plan);
 int packed_refresh(femgpu_ctx *ctx, femgpu_csr *A);
+int build_stream_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, const int *diag, bool upper,
+                      StreamSweep &plan);
+size_t stream_lds_bytes(const StreamSweep &P, size_t elem);
 int build_factor_plan(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
 template <typename T>
 size_t packed_lds_bytes(const PackedSweep &P);

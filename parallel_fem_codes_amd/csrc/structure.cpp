@@ -623,6 +623,111 @@ int build_packed_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, cons
 // Update lists of the numeric ILU(0) (ILUp_precond_setup.c:84-166 with p = 0): for row i and each
 // L entry t (column jrow, ascending), every U entry k of row jrow whose column is in row i's
 // pattern updates that slot (or the pivot when the column is i), in U order.
+size_t stream_lds_bytes(const StreamSweep &P, size_t elem)
+{
+    return elem * (static_cast<size_t>(P.W) + 2) + sizeof(int) * (static_cast<size_t>(P.nsteps) + 2 * kStreamPad + 1);
+}
+
+int build_stream_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, const int *diag, bool upper,
+                      StreamSweep &P)
+{
+    P.ok = false;
+    P.n = n;
+    if (n == 0) return FEMGPU_OK;
+    auto ent_begin = [&](int row) { return upper ? diag[row] + 1 : IA[row]; };
+    auto ent_end = [&](int row) { return upper ? IA[row + 1] : diag[row]; };
+    // dependency levels in sweep order (ascending rows for L, descending for U)
+    std::vector<int> lev(n);
+    int depth = 0, maxdeg = 0;
+    for (int s = 0; s < n; ++s) {
+        const int r = upper ? n - 1 - s : s;
+        int l = 0;
+        for (int p = ent_begin(r); p < ent_end(r); ++p) l = std::max(l, lev[JA[p]] + 1);
+        lev[r] = l;
+        depth = std::max(depth, l + 1);
+        maxdeg = std::max(maxdeg, ent_end(r) - ent_begin(r));
+    }
+    P.depth = depth;
+    int K = 0;
+    for (int k : {4, 8})
+        if (maxdeg <= k) { K = k; break; }
+    if (K == 0) return FEMGPU_OK;   // rows too long for the register-staged kernel
+    // schedule: rows by (level, sweep order); levels cut into steps of at most kStreamThreads rows
+    std::vector<int> lptr(static_cast<size_t>(depth) + 1, 0), row_of(n), pos(n);
+    for (int r = 0; r < n; ++r) lptr[lev[r] + 1]++;
+    for (int l = 0; l < depth; ++l) lptr[l + 1] += lptr[l];
+    {
+        std::vector<int> fill(lptr.begin(), lptr.end() - 1);
+        for (int s = 0; s < n; ++s) {
+            const int r = upper ? n - 1 - s : s;
+            row_of[fill[lev[r]]++] = r;
+        }
+    }
+    for (int q = 0; q < n; ++q) pos[row_of[q]] = q;
+    std::vector<int> stepq(1, 0), step_beg(n), step_end(n);
+    for (int l = 0; l < depth; ++l)
+        for (int a = lptr[l]; a < lptr[l + 1]; a += kStreamThreads) {
+            const int b = std::min(lptr[l + 1], a + kStreamThreads);
+            stepq.push_back(b);
+            for (int q = a; q < b; ++q) {
+                step_beg[q] = a;
+                step_end[q] = b;
+            }
+        }
+    // kStreamPad empty steps in front (the kernel's pipeline fills on them), empty steps at the tail
+    const int nsteps_real = static_cast<int>(stepq.size()) - 1;
+    const int nsteps = kStreamPad + (nsteps_real + kStreamPad - 1) / kStreamPad * kStreamPad;
+    stepq.insert(stepq.begin(), kStreamPad, 0);
+    stepq.resize(static_cast<size_t>(nsteps) + 2 * kStreamPad + 1, n);
+    // ring size: a dependency at position qj read by a row of a step ending at e must not be
+    // overwritten before e, i.e. qj + W >= e
+    long long wreq = 1;
+    for (int q = 0; q < n; ++q) {
+        const int r = row_of[q];
+        for (int p = ent_begin(r); p < ent_end(r); ++p) wreq = std::max<long long>(wreq, step_end[q] - pos[JA[p]]);
+    }
+    int W = 64;
+    while (W < wreq && W < (1 << 15)) W <<= 1;
+    if (W < wreq) return FEMGPU_OK;
+    P.K = K;
+    P.W = W;
+    P.nsteps = nsteps;
+    P.steps = nsteps_real;
+    const size_t limit = ctx->lds_limit > 1024 ? ctx->lds_limit - 1024 : ctx->lds_limit;
+    if (stream_lds_bytes(P, sizeof(double)) > limit) return FEMGPU_OK;
+    // padded row data in schedule order, entries in the serial sweep's order
+    std::vector<unsigned short> slot(static_cast<size_t>(n) * K, static_cast<unsigned short>(W));
+    std::vector<int> src(static_cast<size_t>(n) * K, -1);
+    for (int q = 0; q < n; ++q) {
+        const int r = row_of[q];
+        int k = 0;
+        for (int p = ent_begin(r); p < ent_end(r); ++p, ++k) {
+            const int qj = pos[JA[p]];
+            if (qj >= step_beg[q]) {
+                set_error("stream plan: dependency of row %d is not scheduled before it", r);
+                return FEMGPU_ERR_ARG;
+            }
+            slot[static_cast<size_t>(q) * K + k] = static_cast<unsigned short>(qj & (W - 1));
+            src[static_cast<size_t>(q) * K + k] = p;
+        }
+    }
+    hipStream_t st = ctx->stream;
+    FG_TRY(P.stepq.upload(stepq.data(), stepq.size(), st));
+    FG_TRY(P.row_of.upload(row_of.data(), n, st));
+    FG_TRY(P.slot.upload(slot.data(), slot.size(), st));
+    FG_TRY(P.src.upload(src.data(), src.size(), st));
+    FG_TRY(P.val64.alloc(slot.size()));
+    FG_TRY(P.val32.alloc(slot.size()));
+    FG_TRY(P.sink.alloc(kStreamThreads));
+    if (upper) {
+        FG_TRY(P.dinv64.alloc(n));
+        FG_TRY(P.dinv32.alloc(n));
+    }
+    FG_HIP(hipStreamSynchronize(st));
+    P.ok = true;
+    return FEMGPU_OK;
+}
+
 int build_factor_plan(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag)
 {
     const int n = A->n;

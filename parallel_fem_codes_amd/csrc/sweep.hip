@@ -257,6 +257,201 @@ __global__ __launch_bounds__(kPackedThreads) void k_sweep_packed(PackedArgs S, c
     }
 }
 
+
+// ------------------------------------------------------------------------------------------------
+// Streamed sweep: one workgroup walks the whole level schedule, one step (at most kStreamThreads
+// rows of one level) per LDS barrier, thread t taking row t of the step. Results of recent rows live
+// in an LDS ring, so a row reads its dependencies from LDS only and the per-level cost is one LDS
+// round trip plus one barrier -- no cross-workgroup hand-off. The row data of a step (padded factor
+// entries, ring slots, inverse pivot, right-hand side) does not depend on any result, so it is
+// fetched into registers D steps ahead (the row index, which the right-hand-side gather needs, 2D
+// steps ahead); those loads and the result stores stay in flight across the LDS-only barriers.
+// Every thread issues the same loads and stores on every step (idle threads use row 0 and store to
+// a sink), so the compiler's wait counts see one path and wait only for the step being computed.
+
+struct StreamArgs {
+    int nsteps;          // kStreamPad empty steps, the schedule, empty steps up to a multiple of
+                         // kStreamPad; stepq has nsteps + 2*kStreamPad + 1 entries
+    int W;
+    const int *stepq, *row_of;
+    const unsigned short *slot;
+    void *sink;          // kStreamThreads results of idle threads
+};
+
+template <typename T, int K>
+struct StreamStage {
+    T v[K];
+    unsigned short s[K];
+    T b, d;
+    int q;               // schedule position of this thread's row in the staged step, -1: idle
+};
+
+template <int BYTES>
+__device__ __forceinline__ void load_bytes(const void *__restrict__ src, void *dst)
+{
+    static_assert(BYTES % 8 == 0, "8-byte granules");
+    if constexpr (BYTES % 16 == 0) {
+        const float4 *p = reinterpret_cast<const float4 *>(src);
+#pragma unroll
+        for (int i = 0; i < BYTES / 16; ++i) {
+            const float4 w = p[i];
+            __builtin_memcpy(static_cast<char *>(dst) + 16 * i, &w, 16);
+        }
+    } else {
+        const float2 *p = reinterpret_cast<const float2 *>(src);
+#pragma unroll
+        for (int i = 0; i < BYTES / 8; ++i) {
+            const float2 w = p[i];
+            __builtin_memcpy(static_cast<char *>(dst) + 8 * i, &w, 8);
+        }
+    }
+}
+
+// Schedule position of this thread's row in step s, or -1.
+__device__ __forceinline__ int stream_pos(const int *sq, int s)
+{
+    const int q = sq[s] + static_cast<int>(threadIdx.x);
+    return q < sq[s + 1] ? q : -1;
+}
+
+// Stage step s into g: entries, slots and pivot by position, right-hand side through the row index
+// fetched D steps earlier.
+template <typename T, bool UPPER, int K>
+__device__ __forceinline__ void stream_fetch(StreamStage<T, K> &g, const StreamArgs &a, const int *sq, int s,
+                                             unsigned row, const T *__restrict__ val, const T *__restrict__ dinv,
+                                             const T *b)
+{
+    const int q = stream_pos(sq, s);
+    const int qq = q < 0 ? 0 : q;
+    load_bytes<K * sizeof(T)>(val + static_cast<size_t>(qq) * K, g.v);
+    load_bytes<K * 2>(a.slot + static_cast<size_t>(qq) * K, g.s);
+    if (UPPER) g.d = dinv[qq];
+    g.b = b[static_cast<size_t>(row)];
+    g.q = q;
+}
+
+// Row indices are handled as unsigned so that no instruction touches a freshly loaded index (a sign
+// extension) before the step that uses it.
+__device__ __forceinline__ unsigned stream_row(const StreamArgs &a, const int *sq, int s)
+{
+    const int q = stream_pos(sq, s);
+    return reinterpret_cast<const unsigned *>(a.row_of)[q < 0 ? 0 : q];
+}
+
+// One step: compute the staged rows (cur = their row indices), then restage for step s + D, whose
+// row indices are in nxt, and fetch the row indices of step s + 2D into cur.
+template <typename T, bool UPPER, int K, int D>
+__device__ __forceinline__ void stream_step(StreamStage<T, K> &g, unsigned &cur, unsigned nxt, const StreamArgs &a,
+                                            const int *sq, int s, T *ring, T *sink, const T *__restrict__ val,
+                                            const T *__restrict__ dinv, const T *b, T *x)
+{
+#pragma clang fp contract(off)
+    // keep each step's work in its own region: hoisting a later step's slot decoding would make
+    // this step wait for that step's loads
+    __builtin_amdgcn_sched_barrier(0);
+    T acc = g.b;
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc -= g.v[k] * ring[g.s[k]];
+    if (UPPER) acc *= g.d;
+    const bool on = g.q >= 0;
+    ring[on ? (g.q & (a.W - 1)) : a.W + 1] = acc;
+    *(on ? x + static_cast<size_t>(cur) : sink) = acc;
+    stream_fetch<T, UPPER, K>(g, a, sq, s + D, nxt, val, dinv, b);
+    cur = stream_row(a, sq, s + 2 * D);
+    lds_barrier();
+}
+
+// UPPER: x = (b - U x) * Dinv; else x = b - L x (ILUp_precond.c:26-44, same operation order,
+// contraction off; padding entries subtract 0 * 0). b may alias x: a row's b is read D steps before
+// the row's own result is stored, and no other row reads it.
+template <typename T, bool UPPER, int K, int D>
+__global__ __launch_bounds__(kStreamThreads) void k_sweep_stream(StreamArgs a, const T *__restrict__ val,
+                                                                 const T *__restrict__ dinv, const T *b, T *x,
+                                                                 const int *__restrict__ skip)
+{
+    static_assert(2 * D <= kStreamPad, "step padding too small for the lookahead");
+    if (skip && *skip) return;
+    extern __shared__ unsigned char smem_raw[];
+    T *ring = reinterpret_cast<T *>(smem_raw);                  // W results, a zero, an idle slot
+    int *sq = reinterpret_cast<int *>(ring + a.W + 2);
+    const int nsq = a.nsteps + 2 * kStreamPad + 1;
+    for (int i = threadIdx.x; i < nsq; i += blockDim.x) sq[i] = a.stepq[i];
+    if (threadIdx.x == 0) ring[a.W] = T(0);
+    __syncthreads();
+    T *sink = static_cast<T *>(a.sink) + threadIdx.x;
+    // The schedule starts with kStreamPad empty steps: the loop itself stages the first real
+    // steps, so there is no separate prologue (whose different order of loads would make the wait
+    // counts at the loop head conservative).
+    StreamStage<T, K> st[D];
+    unsigned ra[D], rb[D];   // row indices; the roles (current / next) swap every D steps
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            st[u].v[k] = T(0);
+            st[u].s[k] = static_cast<unsigned short>(a.W);
+        }
+        st[u].b = st[u].d = T(0);
+        st[u].q = -1;
+        ra[u] = rb[u] = 0u;
+    }
+    for (int s0 = 0; s0 < a.nsteps; s0 += 2 * D) {
+#pragma unroll
+        for (int u = 0; u < D; ++u)
+            stream_step<T, UPPER, K, D>(st[u], ra[u], rb[u], a, sq, s0 + u, ring, sink, val, dinv, b, x);
+#pragma unroll
+        for (int u = 0; u < D; ++u)
+            stream_step<T, UPPER, K, D>(st[u], rb[u], ra[u], a, sq, s0 + D + u, ring, sink, val, dinv, b, x);
+    }
+}
+
+// Streamed-plan values (padding 0) and inverse pivots in schedule order from the factors.
+template <typename T>
+__global__ void k_stream_values(size_t E, const int *__restrict__ src, const double *__restrict__ LU,
+                                T *__restrict__ val, int n, const int *__restrict__ row_of,
+                                const double *__restrict__ Dinv, T *__restrict__ dinv)
+{
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < E; e += stride) {
+        const int p = src[e];
+        val[e] = p >= 0 ? static_cast<T>(LU[p]) : T(0);
+    }
+    if (dinv)
+        for (size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; q < static_cast<size_t>(n); q += stride)
+            dinv[q] = static_cast<T>(Dinv[row_of[q]]);
+}
+
+static StreamArgs stream_args(const StreamSweep &P)
+{
+    StreamArgs a;
+    a.nsteps = P.nsteps;
+    a.W = P.W;
+    a.stepq = P.stepq.p;
+    a.row_of = P.row_of.p;
+    a.slot = P.slot.p;
+    a.sink = P.sink.p;
+    return a;
+}
+
+template <typename T, bool UPPER>
+static int stream_launch(femgpu_ctx *ctx, const StreamSweep &P, const T *val, const T *dinv, const T *b, T *x,
+                         const int *skip)
+{
+    const size_t lds = stream_lds_bytes(P, sizeof(T));
+    const StreamArgs a = stream_args(P);
+    if (P.K == 4)
+        k_sweep_stream<T, UPPER, 4, 8><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
+    else
+        k_sweep_stream<T, UPPER, 8, 4><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+static bool use_stream(const femgpu_csr *A)
+{
+    return A->sweep_kernel == 0 && A->slo.ok && A->sup.ok;
+}
+
 // Packed factor values (and inverse pivots in sweep order) from the factors in A's layout.
 template <typename T>
 __global__ void k_pack_values(int E, const int *__restrict__ src, const double *__restrict__ LU, T *__restrict__ val,
@@ -302,6 +497,17 @@ template size_t packed_lds_bytes<float>(const PackedSweep &);
 
 int packed_refresh(femgpu_ctx *ctx, femgpu_csr *A)
 {
+    for (StreamSweep *P : {&A->slo, &A->sup}) {
+        if (!P->ok) continue;
+        const bool up = (P == &A->sup);
+        const size_t E = static_cast<size_t>(P->n) * P->K;
+        const int grid = static_cast<int>(std::max<size_t>(1, std::min<size_t>(4096, (E + 255) / 256)));
+        k_stream_values<double><<<grid, 256, 0, ctx->stream>>>(E, P->src.p, A->LU.p, P->val64.p, P->n, P->row_of.p,
+                                                               A->Dinv.p, up ? P->dinv64.p : nullptr);
+        k_stream_values<float><<<grid, 256, 0, ctx->stream>>>(E, P->src.p, A->LU.p, P->val32.p, P->n, P->row_of.p,
+                                                              A->Dinv.p, up ? P->dinv32.p : nullptr);
+        FG_HIP(hipGetLastError());
+    }
     for (PackedSweep *P : {&A->plo, &A->pup}) {
         const bool up = (P == &A->pup);
         const int grid = std::max(1, std::min(4096, (P->E + 255) / 256));
@@ -331,11 +537,21 @@ static int packed_pair(femgpu_ctx *ctx, femgpu_csr *A, const T *b, T *x, const i
 
 int launch_sweep64(femgpu_ctx *ctx, femgpu_csr *A, const double *b, double *x, const int *skip)
 {
+    if (A->n == 0) return FEMGPU_OK;
+    if (use_stream(A)) {
+        FG_TRY((stream_launch<double, false>(ctx, A->slo, A->slo.val64.p, nullptr, b, x, skip)));
+        return stream_launch<double, true>(ctx, A->sup, A->sup.val64.p, A->sup.dinv64.p, x, x, skip);
+    }
     return packed_pair<double>(ctx, A, b, x, skip, A->plo.val64.p, A->pup.val64.p, A->pup.dinv64.p);
 }
 
 int launch_sweep32(femgpu_ctx *ctx, femgpu_csr *A, const float *b, float *x, const int *skip)
 {
+    if (A->n == 0) return FEMGPU_OK;
+    if (use_stream(A)) {
+        FG_TRY((stream_launch<float, false>(ctx, A->slo, A->slo.val32.p, nullptr, b, x, skip)));
+        return stream_launch<float, true>(ctx, A->sup, A->sup.val32.p, A->sup.dinv32.p, x, x, skip);
+    }
     return packed_pair<float>(ctx, A, b, x, skip, A->plo.val32.p, A->pup.val32.p, A->pup.dinv32.p);
 }
 

@@ -221,10 +221,13 @@ class CsrMatrix:
         N.check(N.lib().femgpu_csr_values_changed(self.h))
 
     def sweep_info(self):
-        a, b, c, d, e = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        N.check(N.lib().femgpu_csr_sweep_info(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d), C.byref(e)))
-        return dict(depth_lower=a.value, depth_upper=b.value, nchunks=c.value, rows_per_chunk=d.value,
-                    lds_limit=e.value)
+        si = N.SweepInfo()
+        N.check(N.lib().femgpu_csr_sweep_info(self.h, C.byref(si)))
+        return {f: getattr(si, f) for f, _ in N.SweepInfo._fields_}
+
+    def set_sweep_kernel(self, kind):
+        """"stream" (single-workgroup streamed sweep, the default where its plan exists) or "packed"."""
+        N.check(N.lib().femgpu_csr_set_sweep_kernel(self.h, {"stream": 0, "packed": 1}[kind]))
 
     def csrmv(self, P, Q):
         fn = N.lib().femgpu_csrmv_f32 if P.dtype.itemsize == 4 else N.lib().femgpu_csrmv

@@ -43,6 +43,12 @@ class SolveInfo(C.Structure):
                 ("true_rel_residual", C.c_double), ("prec_rel_residual", C.c_double)]
 
 
+class SweepInfo(C.Structure):
+    """femgpu_sweep_info."""
+    _fields_ = [(f, C.c_int) for f in ("kernel", "depth_lower", "depth_upper", "steps_lower", "steps_upper",
+                                        "ring", "entries_per_row", "chunks_lower", "rows_per_chunk", "lds_limit")]
+
+
 class MixedOpts(C.Structure):
     """femgpu_mixed_opts."""
     _fields_ = [("inner_kmax", C.c_int), ("inner_cycles", C.c_int), ("inner_tol", C.c_double),
@@ -75,7 +81,8 @@ _SIGS = {
     "femgpu_csr_set_values": (I, [vp, vp, I]),
     "femgpu_csr_values_ptr": (I, [vp, C.POINTER(vp)]),
     "femgpu_csr_values_changed": (I, [vp]),
-    "femgpu_csr_sweep_info": (I, [vp, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+    "femgpu_csr_sweep_info": (I, [vp, C.POINTER(SweepInfo)]),
+    "femgpu_csr_set_sweep_kernel": (I, [vp, I]),
     "femgpu_csrmv": (I, [vp, vp, vp]),
     "femgpu_csrmv_f32": (I, [vp, vp, vp]),
     "femgpu_ilu0_factor": (I, [vp]),

@@ -144,17 +144,54 @@ def test_block_ebemv_matches_oracle(dev, torch_cuda, oracle, s_blk):
 
 # ----------------------------------------------------------------------------- ILU(0)
 
-@pytest.mark.parametrize("name", CSR_CASES + ["m200", "m200cau"])
-def test_ilu0_factor_and_sweeps_bit_exact(dev, torch_cuda, oracle, name):
-    torch = torch_cuda
+def synthetic_csr(n, maxdeg, seed):
+    """Diagonally dominant random CSR (sorted rows, full diagonal) whose rows reach up to maxdeg
+    entries on each side of the diagonal -- longer than mesh rows, so the streamed sweep's padded
+    row format does not apply and the packed sweep is used."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(n):
+        lo = rng.choice(np.arange(max(0, i - 300), i), size=min(i, rng.integers(0, maxdeg + 1)), replace=False) if i else []
+        hi = rng.choice(np.arange(i + 1, min(n, i + 301)), size=min(n - 1 - i, rng.integers(0, maxdeg + 1)),
+                        replace=False) if i < n - 1 else []
+        rows.append(np.sort(np.concatenate([np.asarray(lo, int), [i], np.asarray(hi, int)])))
+    IA = np.zeros(n + 1, np.int32)
+    IA[1:] = np.cumsum([len(r) for r in rows])
+    JA = np.concatenate(rows).astype(np.int32)
+    AA = rng.uniform(-1, 1, len(JA))
+    for i in range(n):
+        p = IA[i] + int(np.searchsorted(rows[i], i))
+        AA[p] = 2.0 + np.abs(AA[IA[i]:IA[i + 1]]).sum()
+    return dict(IA=IA, JA=JA, AA=np.append(AA, 0.0), neq=n)
+
+
+def ilu_case(oracle, name):
     if name == "m200":
-        s = steady_system(oracle, m=200)
-    elif name == "m200cau":
+        return steady_system(oracle, m=200)
+    if name == "m200cau":
         s0 = steady_system(oracle, m=200)
-        s = steady_system(oracle, m=200, shock=1, u=np.sin(np.arange(s0["neq"])) * 0.3)
-    else:
-        s = steady_system(oracle, d=G.load(name))
+        return steady_system(oracle, m=200, shock=1, u=np.sin(np.arange(s0["neq"])) * 0.3)
+    if name == "m600not":   # natural ordering: levels up to 599 rows wide, split into two steps
+        return steady_system(oracle, m=600, reorder="NOT")
+    if name == "synthetic12":
+        return synthetic_csr(20000, 12, 3)
+    return steady_system(oracle, d=G.load(name))
+
+
+@pytest.mark.parametrize("kernel", ["stream", "packed"])
+@pytest.mark.parametrize("name", CSR_CASES + ["m200", "m200cau", "m600not", "synthetic12"])
+def test_ilu0_factor_and_sweeps_bit_exact(dev, torch_cuda, oracle, name, kernel):
+    torch = torch_cuda
+    s = ilu_case(oracle, name)
     A = fem.CsrMatrix(dev, s["IA"], s["JA"])
+    A.set_sweep_kernel(kernel)
+    info = A.sweep_info()
+    if kernel == "packed" or name == "synthetic12":
+        assert info["kernel"] == 1
+    else:
+        assert info["kernel"] == 0 and info["entries_per_row"] == 4
+        if name == "m600not":
+            assert info["steps_lower"] > info["depth_lower"]
     A.set_values(s["AA"][:-1])
     LUo, Dio, rc = oracle.ilu0_factor(s["IA"], s["JA"], s["AA"])
     assert rc == 0
@@ -177,11 +214,14 @@ def test_ilu0_factor_and_sweeps_bit_exact(dev, torch_cuda, oracle, name):
     assert rel(H(z32), ref) < 1e-4
 
 
-def test_sweeps_repeat_consistently(dev, torch_cuda, oracle):
-    """The self-resetting scheduler: many back-to-back sweeps give identical results."""
+@pytest.mark.parametrize("kernel", ["stream", "packed"])
+def test_sweeps_repeat_consistently(dev, torch_cuda, oracle, kernel):
+    """Many back-to-back sweeps (the packed scheduler resets itself between launches) give
+    identical results."""
     torch = torch_cuda
     s = steady_system(oracle, m=120)
     A = fem.CsrMatrix(dev, s["IA"], s["JA"])
+    A.set_sweep_kernel(kernel)
     A.set_values(s["AA"][:-1])
     A.ilu0_factor()
     rhs = T(torch, np.random.default_rng(6).standard_normal(s["neq"]))
