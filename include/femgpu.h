@@ -138,6 +138,19 @@ int femgpu_ilu0_apply_f32(femgpu_csr *A, const float *d_p, float *d_z);
  * diagonal), Dinv[n] = 1/pivot. */
 int femgpu_ilu0_get(femgpu_csr *A, double *LU, double *Dinv);
 
+/* FP32 preconditioner pieces of the mixed solver (no reference counterpart).
+ * Parallel ILU(0): FP32 factors in A's layout by `sweeps` fixed-point sweeps of the Chow-Patel
+ * iteration from the current values (converges to the ILU(0) factors; exact after as many sweeps
+ * as the factorisation has dependency levels). */
+int femgpu_ilu0_factor_parallel_f32(femgpu_csr *A, int sweeps);
+/* FP32 copy of the factors of femgpu_ilu0_factor into the same FP32 factor storage. */
+int femgpu_ilu0_factor_to_f32(femgpu_csr *A);
+/* z ~ U^-1 L^-1 p with `iterations` (1..1024) Jacobi iterations per triangle on the FP32 factors
+ * (exact after as many iterations as the sweep has dependency levels); d_p == d_z allowed. */
+int femgpu_ilu0_apply_jacobi_f32(femgpu_csr *A, int iterations, const float *d_p, float *d_z);
+/* Copy the FP32 factors to the host (layout as femgpu_ilu0_get). */
+int femgpu_ilu0_get_f32(femgpu_csr *A, float *LU, float *Dinv);
+
 /* =========================================================== EBE operator */
 
 typedef struct femgpu_ebe femgpu_ebe;
@@ -192,14 +205,18 @@ typedef struct {
     double inner_tol;      /* relative tolerance of each inner solve (preconditioned norm) */
     int max_refinements;   /* outer FP64 refinement steps */
     int sweep_mode;        /* FP32 ILU application: 0 exact level-scheduled sweeps,
-                              k > 0: k Jacobi sweeps per triangular factor */
-    int pad;
+                              k > 0: k Jacobi iterations per triangular factor */
+    int factor_sweeps;     /* FP32 factors: 0 = the FP32 copy of the factorisation done by
+                              femgpu_ilu0_factor; k > 0 = k sweeps of the parallel (fixed-point)
+                              ILU(0) from the current values, no FP64 factorisation needed
+                              (requires sweep_mode > 0) */
 } femgpu_mixed_opts;
 
 /* Mixed precision: FP64 iterative refinement (residual F - A x, update x += d, Givens and least
  * squares in FP64) around an FP32 left-ILU(0)-preconditioned inner GMRES. Stops when
  * ||F - A x|| <= tol * ||F|| (FP64-verified). F is the raw (unpreconditioned) right-hand side.
- * Requires sys->csr and sys->ilu. No reference counterpart. */
+ * Requires sys->csr and sys->ilu (factored unless opt->factor_sweeps > 0). No reference
+ * counterpart. */
 int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *sys, const femgpu_mixed_opts *opt,
                        double tol, const double *d_F, double *d_X, femgpu_solve_info *info);
 

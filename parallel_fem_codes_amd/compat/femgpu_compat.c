@@ -1,0 +1,239 @@
+/*
+ * femgpu_compat.c -- the reference's hot-path entry points, by name and signature, forwarding to
+ * libfemgpu.so. Compiled once per equation driver with that driver's macro and include path, like
+ * the reference compiles 00_CommonFiles (MatrixVector_Operations/matvec.h:1-16), e.g.
+ *
+ *   gcc -c -DSSTranspEquation2D -I<reference>/00_CommonFiles -I<repo>/include -I/opt/rocm/include \
+ *       -D__HIP_PLATFORM_AMD__ femgpu_compat.c
+ *
+ * and linked in place of the objects of MatrixVector_Operations/EBE/ebemv.c, .../CSR/csrmv.c,
+ * Solvers_and_Preconditioners/pgmres.c and CSR/ILUp_precond(_setup).c (with -lfemgpu -lamdhip64).
+ * The driver keeps its host arrays; the shim mirrors what the path needs on the device:
+ *
+ *   ILUp_precond_setup  uploads AA, factors on the device and applies F := M^-1 F (ILUp_precond_setup.c:8-61)
+ *   ILUp_precond        z = U^-1 L^-1 p through the device sweeps (ILUp_precond.c:4-47)
+ *   csrmv / ebemv       one device product (csrmv.c:4-65, ebemv.c:4-64)
+ *   pgmres              the whole solve on the device: B up, X down, iterations added
+ *                       (pgmres.c:5-200); matrix values are uploaded once per solve, never per step
+ *
+ *   SPARILU_clean / SPARMAT_clean  release the device objects (Postprocess.c:137-141 calls them
+ *                       for the ILU preconditioners; the reference structures are never allocated)
+ *
+ * One rank only (the GPU holds the whole mesh): NPROC != 1 stops with a message. Failures print the
+ * library's message and exit(1), as the reference does on fatal errors (mycalloc.c:10-13).
+ * ILU(p) with p > 0 and preconditioners other than NO/ILU0 are not provided by the library.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "MatrixVector_Operations/matvec.h"
+#include "Solvers_and_Preconditioners/solvers.h"
+#include "Solvers_and_Preconditioners/preconditioners.h"
+
+#include "femgpu.h"
+
+#if defined(SSNavierStokesEquations2D)
+#define COMPAT_S 9
+#elif defined(EulerEquations2D)
+#define COMPAT_S 12
+#else
+#define COMPAT_S 3
+#endif
+
+static struct {
+    femgpu_ctx *ctx;
+    femgpu_csr *csr;
+    femgpu_ebe *ebe;
+    const MatrixDataType *csr_owner;   /* the MatrixData whose pattern csr holds */
+    int factored;                      /* csr holds ILU(0) factors of the current AA */
+    double *dP, *dQ;                   /* NEQ + 1 doubles each */
+    int NEQ;
+} G;
+
+static void fail(const char *where)
+{
+    printf("libfemgpu: %s: %s\n", where, femgpu_last_error());
+    exit(1);
+}
+
+#define CALL(expr)                  \
+    do {                            \
+        if ((expr) != FEMGPU_OK)    \
+            fail(#expr);            \
+    } while (0)
+
+static void compat_init(ParametersType *Parameters)
+{
+    if (G.ctx)
+        return;
+    if (Parameters->NPROC != 1) {
+        printf("libfemgpu: one GPU holds the whole mesh; run with one MPI rank (NPROC = %d)\n",
+               Parameters->NPROC);
+        exit(1);
+    }
+    CALL(femgpu_create(0, NULL, &G.ctx));
+    G.NEQ = Parameters->NEQ;
+    if (hipMalloc((void **)&G.dP, sizeof(double) * (G.NEQ + 1)) != hipSuccess ||
+        hipMalloc((void **)&G.dQ, sizeof(double) * (G.NEQ + 1)) != hipSuccess) {
+        printf("libfemgpu: device allocation failed\n");
+        exit(1);
+    }
+}
+
+static void up(double *d, const double *h, int n)
+{
+    if (hipMemcpy(d, h, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+        fail("upload");
+}
+
+static void down(double *h, const double *d, int n)
+{
+    if (hipMemcpy(h, d, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        fail("download");
+}
+
+/* The CSR operator of MatrixData (pattern uploaded on first use), with its current values. */
+static femgpu_csr *csr_of(ParametersType *Parameters, MatrixDataType *MatrixData, int upload_values)
+{
+    if (!G.csr || G.csr_owner != MatrixData) {
+        if (G.csr)
+            femgpu_csr_destroy(G.csr);
+        CALL(femgpu_csr_create(G.ctx, Parameters->neq, Parameters->nnzero, MatrixData->IA, MatrixData->JA, &G.csr));
+        G.csr_owner = MatrixData;
+        G.factored = 0;
+    }
+    if (upload_values) {
+        CALL(femgpu_csr_set_values(G.csr, MatrixData->AA, 0));
+        G.factored = 0;
+    }
+    return G.csr;
+}
+
+/* The EBE operator (element lists on first use) with the current element matrices. */
+static femgpu_ebe *ebe_of(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs)
+{
+    if (!G.ebe)
+        CALL(femgpu_ebe_create(G.ctx, Parameters->nel, COMPAT_S, Parameters->NEQ, FemStructs->lmaux, &G.ebe));
+    CALL(femgpu_ebe_set_values(G.ebe, MatrixData->Aaux, 0));
+    return G.ebe;
+}
+
+int csrmv(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs, double *P, double *Q)
+{
+    (void)FemStructs;
+    compat_init(Parameters);
+    femgpu_csr *A = csr_of(Parameters, MatrixData, 1);
+    up(G.dP, P, Parameters->neq);
+    CALL(femgpu_csrmv(A, G.dP, G.dQ));
+    down(Q, G.dQ, Parameters->neq);
+    return 0;
+}
+
+#if COMPAT_S == 3
+int ebemv(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs, double *P, double *Q)
+#elif COMPAT_S == 9
+int ebemvNDOF3(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs, double *P, double *Q)
+#else
+int ebemvNDOF4(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs, double *P, double *Q)
+#endif
+{
+    compat_init(Parameters);
+    femgpu_ebe *E = ebe_of(Parameters, MatrixData, FemStructs);
+    up(G.dP, P, Parameters->NEQ);
+    CALL(femgpu_ebemv(E, G.dP, G.dQ));
+    down(Q, G.dQ, Parameters->NEQ);
+    return 0;
+}
+
+int ILUp_precond_setup(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs, int tag,
+                       double *F)
+{
+    (void)FemStructs;
+    (void)tag;   /* the pattern is uploaded once per MatrixData; values are refreshed on every call */
+    compat_init(Parameters);
+    if (atoi(Parameters->Preconditioner + 3) != 0) {
+        printf("libfemgpu: only ILU(0) is provided (Preconditioner = %s)\n", Parameters->Preconditioner);
+        exit(1);
+    }
+    femgpu_csr *A = csr_of(Parameters, MatrixData, 1);
+    CALL(femgpu_ilu0_factor(A));
+    G.factored = 1;
+    up(G.dP, F, Parameters->neq);
+    CALL(femgpu_ilu0_apply(A, G.dP, G.dP));
+    down(F, G.dP, Parameters->neq);
+    return 0;
+}
+
+int ILUp_precond(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs, double *p,
+                 double *z)
+{
+    (void)FemStructs;
+    compat_init(Parameters);
+    femgpu_csr *A = csr_of(Parameters, MatrixData, 0);
+    if (!G.factored)
+        fail("ILUp_precond before ILUp_precond_setup");
+    up(G.dP, p, Parameters->neq);
+    CALL(femgpu_ilu0_apply(A, G.dP, G.dQ));
+    down(z, G.dQ, Parameters->neq);
+    return 0;
+}
+
+int pgmres(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs,
+           FemFunctionsType *FemFunctions, double *B, double *X)
+{
+    femgpu_system S = {NULL, NULL, NULL};
+    femgpu_solve_info info;
+    compat_init(Parameters);
+    if (FemFunctions->precond == ILUp_precond) {
+        S.ilu = csr_of(Parameters, MatrixData, 0);
+        if (!G.factored)
+            fail("pgmres: ILU(0) preconditioner used before ILUp_precond_setup");
+    } else if (FemFunctions->precond != NO_precond) {
+        printf("libfemgpu: preconditioner %s is not provided; use NOT or ILU0\n", Parameters->Preconditioner);
+        exit(1);
+    }
+    if (FemFunctions->mv == csrmv)
+        S.csr = csr_of(Parameters, MatrixData, S.ilu == NULL);   /* factored values are current */
+#if COMPAT_S == 3
+    else if (FemFunctions->mv == ebemv)
+#elif COMPAT_S == 9
+    else if (FemFunctions->mv == ebemvNDOF3)
+#else
+    else if (FemFunctions->mv == ebemvNDOF4)
+#endif
+        S.ebe = ebe_of(Parameters, MatrixData, FemStructs);
+    else {
+        printf("libfemgpu: matrix-vector product %s is not provided; use EBE or CSR\n",
+               Parameters->MatrixVectorProductScheme);
+        exit(1);
+    }
+    up(G.dP, B, Parameters->neq);
+    CALL(femgpu_pgmres(G.ctx, &S, Parameters->KrylovBasisVectorsQuantity, Parameters->LinearMaxIter,
+                       Parameters->SolverTolerance, G.dP, G.dQ, &info));
+    down(X, G.dQ, Parameters->neq);
+    Parameters->iterations += info.iterations;
+    return 0;
+}
+
+/* The ILU(0) data lives on the device: Postprocess's clean-up of the reference's ILU structures
+ * (which this shim never allocates) releases the library objects instead. */
+void SPARILU_clean(SparILU *lu)
+{
+    (void)lu;
+    if (G.csr)
+        femgpu_csr_destroy(G.csr);
+    if (G.ebe)
+        femgpu_ebe_destroy(G.ebe);
+    G.csr = NULL;
+    G.ebe = NULL;
+    G.csr_owner = NULL;
+    G.factored = 0;
+}
+
+void SPARMAT_clean(SparMAT *mat)
+{
+    (void)mat;
+}

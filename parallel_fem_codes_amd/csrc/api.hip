@@ -232,6 +232,7 @@ extern "C" int femgpu_csr_values_changed(femgpu_csr *A)
 {
     FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_csr_values_changed: null");
     A->factored = false;
+    A->f32_factored = false;
     return launch_to_f32(A->ctx, A->AA.p, A->AA32.p, A->nnz);
 }
 
@@ -297,6 +298,41 @@ extern "C" int femgpu_ilu0_factor_host(femgpu_csr *A)
     FG_TRY(packed_refresh(ctx, A));
     FG_HIP(hipStreamSynchronize(ctx->stream));
     A->factored = true;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_ilu0_factor_parallel_f32(femgpu_csr *A, int sweeps)
+{
+    FG_CHECK(A && sweeps >= 0 && sweeps <= 1024, FEMGPU_ERR_ARG, "femgpu_ilu0_factor_parallel_f32: bad arguments");
+    FG_TRY(launch_parilu32(A->ctx, A, sweeps));
+    FG_HIP(hipStreamSynchronize(A->ctx->stream));
+    A->f32_factored = true;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_ilu0_factor_to_f32(femgpu_csr *A)
+{
+    FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_ilu0_factor_to_f32: null");
+    FG_CHECK(A->factored, FEMGPU_ERR_STATE, "femgpu_ilu0_factor_to_f32: not factored");
+    FG_TRY(launch_factor_to_f32(A->ctx, A));
+    A->f32_factored = true;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_ilu0_apply_jacobi_f32(femgpu_csr *A, int iterations, const float *d_p, float *d_z)
+{
+    FG_CHECK(A && d_p && d_z, FEMGPU_ERR_ARG, "femgpu_ilu0_apply_jacobi_f32: bad arguments");
+    FG_CHECK(A->f32_factored, FEMGPU_ERR_STATE, "femgpu_ilu0_apply_jacobi_f32: no FP32 factors");
+    return launch_jacobi_apply32(A->ctx, A, iterations, d_p, d_z, nullptr);
+}
+
+extern "C" int femgpu_ilu0_get_f32(femgpu_csr *A, float *LU, float *Dinv)
+{
+    FG_CHECK(A && LU && Dinv, FEMGPU_ERR_ARG, "femgpu_ilu0_get_f32: bad arguments");
+    FG_CHECK(A->f32_factored, FEMGPU_ERR_STATE, "femgpu_ilu0_get_f32: no FP32 factors");
+    FG_HIP(hipMemcpyAsync(LU, A->LU32.p, sizeof(float) * A->nnz, hipMemcpyDeviceToHost, A->ctx->stream));
+    FG_HIP(hipMemcpyAsync(Dinv, A->Dinv32.p, sizeof(float) * A->n, hipMemcpyDeviceToHost, A->ctx->stream));
+    FG_HIP(hipStreamSynchronize(A->ctx->stream));
     return FEMGPU_OK;
 }
 

@@ -131,7 +131,7 @@ struct StreamSweep {
 // number of L entries, and its update list -- (U entry position k of a dependency row, target slot
 // in this row, index t of the L entry it belongs to) -- in the serial factorisation's order.
 struct FactorPlan {
-    int maxrow = 0, nupd = 0;
+    int maxrow = 0, nupd = 0, maxupd = 0;
     DevBuf<int4> rowinfo;   // per schedule position: {p0, len, nL, first update}
     DevBuf<int2> upd;       // {k, dest | t << 16}; dest == nL marks the pivot
 };
@@ -169,6 +169,10 @@ struct femgpu_csr {
     fg::PackedSweep plo, pup;       // packed lower / upper sweeps
     fg::StreamSweep slo, sup;       // streamed single-workgroup sweeps (used when both exist)
     int sweep_kernel = 0;           // 0: streamed when available, 1: packed
+    bool f32_factored = false;          // LU32/Dinv32 hold factors of the current values
+    fg::DevBuf<float> LU32, Dinv32;     // FP32 factors in A's layout (mixed solver's Jacobi sweeps)
+    fg::DevBuf<float> LU32b, Dinv32b;   // second iterate of the parallel factorisation
+    fg::DevBuf<float> jac_a, jac_b;     // Jacobi sweep iterates
     fg::FactorPlan fplan;           // numeric factorisation
     int spmv_lanes = 8;             // lanes per row of the SpMV kernel
 };
@@ -218,6 +222,9 @@ int launch_residual64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, con
 int launch_sweep64(femgpu_ctx *ctx, femgpu_csr *A, const double *b, double *x, const int *skip);
 int launch_sweep32(femgpu_ctx *ctx, femgpu_csr *A, const float *b, float *x, const int *skip);
 int launch_ilu_factor(femgpu_ctx *ctx, femgpu_csr *A);
+int launch_parilu32(femgpu_ctx *ctx, femgpu_csr *A, int sweeps);
+int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A);
+int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip);
 int launch_to_f32(femgpu_ctx *ctx, const double *src, float *dst, size_t n);
 int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *Q);
 int launch_ebe_gather_values(femgpu_ctx *ctx, femgpu_ebe *E, const double *A_elem_order);

@@ -27,6 +27,7 @@ struct GDev {
     unsigned *counter;
     int *h_status;    // host-mapped: per step {done} ; h_rho host-mapped rho per step
     double *h_rho;
+    int prec_mode;    // FP32 preconditioner (host side only): 0 exact sweeps, k > 0 Jacobi iterations
 };
 
 __device__ __forceinline__ double &Hat(const GDev &G, int r, int c) { return G.H[r * G.ld + c]; }
@@ -232,6 +233,13 @@ static int apply_op64(femgpu_ctx *ctx, const femgpu_system *S, const double *x, 
 
 // One Arnoldi cycle (steps until the device flag says done), FP64 or FP32 vectors. Returns the
 // steps the device took in the cycle.
+// FP32 preconditioner of the inner solver: the exact level-scheduled sweeps on the FP32 copy of the
+// ILU(0) factors, or mode Jacobi iterations per triangle on the FP32 factors in A's layout.
+static int prec32(femgpu_ctx *ctx, femgpu_csr *A, int mode, const float *b, float *x, const int *skip)
+{
+    return mode == 0 ? launch_sweep32(ctx, A, b, x, skip) : launch_jacobi_apply32(ctx, A, mode, b, x, skip);
+}
+
 template <typename T>
 static int run_cycle(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, T *V, size_t ldv, T *wA, T *wB,
                      int kmax, int *steps_out);
@@ -282,7 +290,7 @@ int run_cycle<float>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, fl
     for (int i = 0; i < kmax - 1; ++i) {
         ctx->h_status[i] = 0;
         rc = launch_spmv32(ctx, S->csr, wc, G.invh + i, V + i * ldv, wn, G.flags);
-        if (!rc) rc = launch_sweep32(ctx, S->ilu, wn, wn, G.flags);
+        if (!rc) rc = prec32(ctx, S->ilu, G.prec_mode, wn, wn, G.flags);
         if (rc) break;
         for (int j = 0; j <= i; ++j)
             k_mgs<float><<<grid, 256, 0, ctx->stream>>>(G, n, i, j, wn, j ? V + (j - 1) * ldv : nullptr, V + j * ldv);
@@ -442,12 +450,24 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
 {
     FG_CHECK(ctx && S && S->csr && S->ilu && opt && d_F && d_X, FEMGPU_ERR_ARG,
              "femgpu_gmres_mixed: needs a CSR operator, a factored ILU(0) and options");
-    FG_CHECK(S->ilu->factored, FEMGPU_ERR_STATE, "femgpu_gmres_mixed: ILU(0) not factored");
-    FG_CHECK(opt->sweep_mode == 0, FEMGPU_ERR_ARG, "femgpu_gmres_mixed: sweep_mode %d not available", opt->sweep_mode);
+    FG_CHECK(opt->sweep_mode >= 0 && opt->sweep_mode <= 16 && opt->factor_sweeps >= 0 && opt->factor_sweeps <= 16,
+             FEMGPU_ERR_ARG, "femgpu_gmres_mixed: sweep_mode %d / factor_sweeps %d out of range", opt->sweep_mode,
+             opt->factor_sweeps);
+    FG_CHECK(opt->factor_sweeps == 0 || opt->sweep_mode > 0, FEMGPU_ERR_ARG,
+             "femgpu_gmres_mixed: the parallel factorisation is applied with Jacobi sweeps (sweep_mode > 0)");
+    FG_CHECK(opt->factor_sweeps > 0 || S->ilu->factored, FEMGPU_ERR_STATE, "femgpu_gmres_mixed: ILU(0) not factored");
     const int n = S->csr->n;
     const int kin = opt->inner_kmax;
     GDev G;
     FG_TRY(make_gdev(ctx, kin, G));
+    G.prec_mode = opt->sweep_mode;
+    if (opt->factor_sweeps > 0) {
+        FG_TRY(launch_parilu32(ctx, S->ilu, opt->factor_sweeps));
+        S->ilu->f32_factored = true;
+    } else if (opt->sweep_mode > 0) {
+        FG_TRY(launch_factor_to_f32(ctx, S->ilu));
+        S->ilu->f32_factored = true;
+    }
     size_t ldv;
     FG_TRY(ensure_ws<float>(ctx, n, kin, ctx->gm_basis32, ctx->gm_w32, ldv));
     FG_TRY(ctx->gm_r.alloc(static_cast<size_t>(n) + 64));
@@ -480,7 +500,7 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
         // inner FP32 solve of M^-1 A d = M^-1 r / ||r||
         k_scale_to_f32<<<grid * 4, 256, 0, ctx->stream>>>(n, r, d_rn, wA);
         FG_HIP(hipGetLastError());
-        FG_TRY(launch_sweep32(ctx, S->ilu, wA, wA, nullptr));
+        FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wA, wA, nullptr));
         for (int cyc = 0; cyc < std::max(1, opt->inner_cycles); ++cyc) {
             if (cyc > 0) break;   // one inner cycle per refinement step
             k_cycle_start<float><<<grid, 256, 0, ctx->stream>>>(G, n, wA, nullptr, 0, 1, opt->inner_tol);

@@ -1,0 +1,203 @@
+// FP32 preconditioner of the mixed-precision solver: a fine-grained parallel ILU(0) and Jacobi
+// iterations for its triangular solves. Neither has a counterpart in the reference (whose ILU(0)
+// is the serial IKJ elimination of ILUp_precond_setup.c:67-168 and whose sweeps are the serial
+// loops of ILUp_precond.c:26-44); both approximate those operators with work that is parallel over
+// all rows, which is what a latency-bound dependency chain of a few thousand levels is not.
+//
+// Parallel ILU(0) (Chow and Patel, "Fine-grained parallel incomplete LU factorization", SIAM J.
+// Sci. Comput. 37(2), 2015): the factors are the fixed point of, for every (i, j) in the pattern,
+//     l_ij = (a_ij - sum_{k<j} l_ik u_kj) / u_jj      (i > j)
+//     u_ij =  a_ij - sum_{k<i} l_ik u_kj              (i <= j)
+// Each sweep evaluates all right-hand sides from the previous iterate (two buffers, so the result
+// does not depend on thread timing), starting from l_ij = a_ij / a_jj, u_ij = a_ij. The products of
+// row i are exactly the IKJ update list of row i (the numeric factorisation plan), so the same list
+// drives both the exact and the parallel factorisation.
+//
+// Jacobi triangular solves (Anzt, Chow and Dongarra, "Iterative sparse triangular solves for
+// preconditioning", Euro-Par 2015): y = L^-1 b by y_{k+1} = b - L_s y_k from y_0 = b, and
+// z = U^-1 y by z_{k+1} = Dinv (y - U_s z_k) from z_0 = Dinv y, a fixed number of iterations, so
+// the preconditioner is one fixed linear operator for the Krylov method.
+#include "common.h"
+
+namespace fg {
+
+// Initial iterate, and the inverse pivots a parallel sweep divides by.
+__global__ void k_parilu_init(int n, const int *__restrict__ IA, const int *__restrict__ JA,
+                              const int *__restrict__ diag, const float *__restrict__ AA, float *__restrict__ LU,
+                              float *__restrict__ Dinv)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int d = diag[i];
+    for (int p = IA[i]; p < d; ++p) LU[p] = AA[p] / AA[diag[JA[p]]];
+    for (int p = d; p < IA[i + 1]; ++p) LU[p] = AA[p];
+    Dinv[i] = 1.0f / AA[d];
+}
+
+// One Jacobi-style sweep of the parallel ILU(0), one row per thread (rows in the plan's order; any
+// order gives the same result). Row entries are kept in registers when the row has at most MAXROW
+// entries and its list at most MAXU updates; other rows take the in-memory path.
+template <int MAXROW, int MAXU>
+__global__ __launch_bounds__(256) void k_parilu_sweep(int n, const int4 *__restrict__ rowinfo, const int2 *__restrict__ upd,
+                                                      const int *__restrict__ JA, const float *__restrict__ AA,
+                                                      const float *__restrict__ LUo, const float *__restrict__ Dio,
+                                                      float *__restrict__ LUn, float *__restrict__ Din,
+                                                      const int *__restrict__ row_of)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int4 ri = rowinfo[q];
+    const int p0 = ri.x, len = ri.y, nL = ri.z, ub = ri.w, nu = rowinfo[q + 1].w - ub;
+    const int i = row_of[q];
+    float piv;
+    if (len <= MAXROW && nu <= MAXU) {
+        float v[MAXROW];
+#pragma unroll
+        for (int e = 0; e < MAXROW; ++e) v[e] = (e < len) ? AA[p0 + e] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < MAXU; ++u) {
+            if (u < nu) {
+                const int2 w = upd[ub + u];
+                const int t = w.y >> 16, dest = w.y & 0xffff;
+                const float c = LUo[p0 + t] * LUo[w.x];
+#pragma unroll
+                for (int e = 0; e < MAXROW; ++e)
+                    if (e == dest) v[e] -= c;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < MAXROW; ++e) {
+            if (e < nL) LUn[p0 + e] = v[e] * Dio[JA[p0 + e]];
+            else if (e < len) LUn[p0 + e] = v[e];
+        }
+        piv = LUn[p0 + nL];
+    } else {
+        for (int e = 0; e < len; ++e) LUn[p0 + e] = AA[p0 + e];
+        for (int u = ub; u < ub + nu; ++u) {
+            const int2 w = upd[u];
+            const int t = w.y >> 16, dest = w.y & 0xffff;
+            LUn[p0 + dest] -= LUo[p0 + t] * LUo[w.x];
+        }
+        for (int e = 0; e < nL; ++e) LUn[p0 + e] *= Dio[JA[p0 + e]];
+        piv = LUn[p0 + nL];
+    }
+    Din[i] = 1.0f / piv;
+}
+
+// y_out = b - L_s y_in (y_in == nullptr: y_in = b).
+template <int LPR>
+__global__ __launch_bounds__(256) void k_jacobi_lower(int n, const int *__restrict__ IA, const int *__restrict__ JA,
+                                                      const int *__restrict__ diag, const float *__restrict__ LU,
+                                                      const float *__restrict__ b, const float *__restrict__ yin,
+                                                      float *__restrict__ yout, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int gt = blockIdx.x * 256 + threadIdx.x;
+    const int row = gt / LPR, lane = gt % LPR;
+    if (row >= n) return;
+    const float *src = yin ? yin : b;
+    float acc = 0.0f;
+    for (int p = IA[row] + lane; p < diag[row]; p += LPR) acc += LU[p] * src[JA[p]];
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+    if (lane == 0) yout[row] = b[row] - acc;
+}
+
+// z_out = Dinv (y - U_s z_in) (z_in == nullptr: z_in = Dinv y).
+template <int LPR>
+__global__ __launch_bounds__(256) void k_jacobi_upper(int n, const int *__restrict__ IA, const int *__restrict__ JA,
+                                                      const int *__restrict__ diag, const float *__restrict__ LU,
+                                                      const float *__restrict__ Dinv, const float *__restrict__ y,
+                                                      const float *__restrict__ zin, float *__restrict__ zout,
+                                                      const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int gt = blockIdx.x * 256 + threadIdx.x;
+    const int row = gt / LPR, lane = gt % LPR;
+    if (row >= n) return;
+    float acc = 0.0f;
+    for (int p = diag[row] + 1 + lane; p < IA[row + 1]; p += LPR) {
+        const int j = JA[p];
+        acc += LU[p] * (zin ? zin[j] : Dinv[j] * y[j]);
+    }
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+    if (lane == 0) zout[row] = Dinv[row] * (y[row] - acc);
+}
+
+static int ensure_par(femgpu_csr *A)
+{
+    const size_t nnz = static_cast<size_t>(A->nnz) + 1, n = static_cast<size_t>(A->n) + 1;
+    FG_TRY(A->LU32.alloc(nnz));
+    FG_TRY(A->LU32b.alloc(nnz));
+    FG_TRY(A->Dinv32.alloc(n));
+    FG_TRY(A->Dinv32b.alloc(n));
+    FG_TRY(A->jac_a.alloc(n));
+    FG_TRY(A->jac_b.alloc(n));
+    return FEMGPU_OK;
+}
+
+int launch_parilu32(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
+{
+    if (A->n == 0) return FEMGPU_OK;
+    FG_TRY(ensure_par(A));
+    const FactorPlan &F = A->fplan;
+    const int grid = (A->n + 255) / 256;
+    // the final sweep must land in LU32/Dinv32: start in whichever buffer makes the parity right
+    float *L0 = (sweeps % 2 == 0) ? A->LU32.p : A->LU32b.p, *L1 = (sweeps % 2 == 0) ? A->LU32b.p : A->LU32.p;
+    float *D0 = (sweeps % 2 == 0) ? A->Dinv32.p : A->Dinv32b.p, *D1 = (sweeps % 2 == 0) ? A->Dinv32b.p : A->Dinv32.p;
+    k_parilu_init<<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, A->diag.p, A->AA32.p, L0, D0);
+    FG_HIP(hipGetLastError());
+    for (int s = 0; s < sweeps; ++s) {
+        if (F.maxrow <= 8 && F.maxupd <= 16)
+            k_parilu_sweep<8, 16><<<grid, 256, 0, ctx->stream>>>(A->n, F.rowinfo.p, F.upd.p, A->JA.p, A->AA32.p, L0, D0,
+                                                                 L1, D1, A->plo.row_of.p);
+        else
+            k_parilu_sweep<1, 1><<<grid, 256, 0, ctx->stream>>>(A->n, F.rowinfo.p, F.upd.p, A->JA.p, A->AA32.p, L0, D0,
+                                                                L1, D1, A->plo.row_of.p);
+        FG_HIP(hipGetLastError());
+        std::swap(L0, L1);
+        std::swap(D0, D1);
+    }
+    return FEMGPU_OK;
+}
+
+int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A)
+{
+    FG_TRY(ensure_par(A));
+    FG_TRY(launch_to_f32(ctx, A->LU.p, A->LU32.p, A->nnz));
+    return launch_to_f32(ctx, A->Dinv.p, A->Dinv32.p, A->n);
+}
+
+// x = (U~ L~)^-1 b approximated with it Jacobi iterations per triangle on the FP32 factors in
+// LU32/Dinv32. b may alias x.
+int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip)
+{
+    const int n = A->n;
+    if (n == 0) return FEMGPU_OK;
+    FG_CHECK(it >= 1 && it <= 1024, FEMGPU_ERR_ARG, "Jacobi sweeps: %d iterations out of range", it);
+    constexpr int LPR = 4;
+    const int grid = static_cast<int>((static_cast<long>(n) * LPR + 255) / 256);
+    float *t[2] = {A->jac_a.p, A->jac_b.p};
+    // lower: y_1 .. y_it alternate t[0], t[1]
+    const float *prev = nullptr;
+    for (int k = 0; k < it; ++k) {
+        float *out = t[k & 1];
+        k_jacobi_lower<LPR><<<grid, 256, 0, ctx->stream>>>(n, A->IA.p, A->JA.p, A->diag.p, A->LU32.p, b, prev, out, skip);
+        prev = out;
+    }
+    const float *y = prev;
+    float *other = (y == t[0]) ? t[1] : t[0];
+    // upper: the last iterate goes to x; earlier ones alternate between x and the free buffer
+    prev = nullptr;
+    for (int k = 1; k <= it; ++k) {
+        float *out = ((it - k) % 2 == 0) ? x : other;
+        k_jacobi_upper<LPR><<<grid, 256, 0, ctx->stream>>>(n, A->IA.p, A->JA.p, A->diag.p, A->LU32.p, A->Dinv32.p, y,
+                                                           prev, out, skip);
+        prev = out;
+    }
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+}  // namespace fg

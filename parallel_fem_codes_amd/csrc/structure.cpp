@@ -737,9 +737,10 @@ int build_factor_plan(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *
     std::vector<int2> upd;
     upd.reserve(static_cast<size_t>(IA[n]) * 2);
     std::vector<int> slot(n, -1);
-    int maxrow = 0;
+    int maxrow = 0, maxupd = 0;
     for (int q = 0; q < n; ++q) {
         const int i = row_of[q], p0 = IA[i], len = IA[i + 1] - p0, nL = diag[i] - p0;
+        const size_t u0 = upd.size();
         maxrow = std::max(maxrow, len);
         info[q] = make_int4(p0, len, nL, static_cast<int>(upd.size()));
         for (int p = p0; p < p0 + len; ++p) slot[JA[p]] = p - p0;
@@ -752,11 +753,13 @@ int build_factor_plan(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *
             }
         }
         for (int p = p0; p < p0 + len; ++p) slot[JA[p]] = -1;
+        maxupd = std::max(maxupd, static_cast<int>(upd.size() - u0));
         FG_CHECK(len < 65536 && nL < 32768, FEMGPU_ERR_ARG, "row %d too long for the factorisation plan", i);
     }
     info[n] = make_int4(0, 0, 0, static_cast<int>(upd.size()));   // sentinel: end of the last list
     A->fplan.maxrow = maxrow;
     A->fplan.nupd = static_cast<int>(upd.size());
+    A->fplan.maxupd = maxupd;
     FG_TRY(A->fplan.rowinfo.upload(info.data(), info.size(), ctx->stream));
     FG_TRY(A->fplan.upd.upload(upd.data(), upd.size(), ctx->stream));
     FG_HIP(hipStreamSynchronize(ctx->stream));

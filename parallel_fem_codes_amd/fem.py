@@ -245,6 +245,21 @@ class CsrMatrix:
         N.check(N.lib().femgpu_ilu0_get(self.h, LU, Dinv))
         return LU[:self.nnz], Dinv[:self.n]
 
+    # FP32 preconditioner of the mixed solver
+    def ilu0_factor_parallel_f32(self, sweeps):
+        N.check(N.lib().femgpu_ilu0_factor_parallel_f32(self.h, int(sweeps)))
+
+    def ilu0_factor_to_f32(self):
+        N.check(N.lib().femgpu_ilu0_factor_to_f32(self.h))
+
+    def ilu0_apply_jacobi_f32(self, iterations, p, z):
+        N.check(N.lib().femgpu_ilu0_apply_jacobi_f32(self.h, int(iterations), _ptr(p), _ptr(z)))
+
+    def ilu0_factors_f32(self):
+        LU, Dinv = np.empty(max(self.nnz, 1), np.float32), np.empty(max(self.n, 1), np.float32)
+        N.check(N.lib().femgpu_ilu0_get_f32(self.h, LU, Dinv))
+        return LU[:self.nnz], Dinv[:self.n]
+
     def ILUp_precond_setup(self, F):
         """ILUp_precond_setup with p = 0: factor the current values, then F := M^-1 F in place."""
         self.ilu0_factor()
@@ -343,12 +358,16 @@ def pgmres(dev: Device, op, B, X, KrylovBasisVectorsQuantity=30, LinearMaxIter=1
 
 
 def gmres_mixed(dev: Device, A: CsrMatrix, F, X, tol=1e-8, inner_kmax=30, inner_tol=1e-4,
-                max_refinements=50, sweep_mode=0):
-    """FP32 ILU(0)-GMRES inside FP64 iterative refinement; F is the raw right-hand side."""
+                max_refinements=50, sweep_mode=0, factor_sweeps=0):
+    """FP32 ILU(0)-GMRES inside FP64 iterative refinement; F is the raw right-hand side.
+
+    sweep_mode 0 applies the exact FP32 sweeps of A's FP64 factorisation (A.ilu0_factor() first);
+    sweep_mode k > 0 applies k Jacobi iterations per triangle. factor_sweeps k > 0 builds the FP32
+    factors by k parallel ILU(0) sweeps instead, with no FP64 factorisation."""
     info = N.SolveInfo()
     o = N.MixedOpts()
     o.inner_kmax, o.inner_cycles, o.inner_tol = inner_kmax, 1, inner_tol
-    o.max_refinements, o.sweep_mode = max_refinements, sweep_mode
+    o.max_refinements, o.sweep_mode, o.factor_sweeps = max_refinements, sweep_mode, factor_sweeps
     S = _system(A, A)
     N.check(N.lib().femgpu_gmres_mixed(dev.h, C.byref(S), C.byref(o), float(tol), _ptr(F), _ptr(X),
                                        C.byref(info)))

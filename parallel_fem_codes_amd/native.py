@@ -52,12 +52,13 @@ class SweepInfo(C.Structure):
 class MixedOpts(C.Structure):
     """femgpu_mixed_opts."""
     _fields_ = [("inner_kmax", C.c_int), ("inner_cycles", C.c_int), ("inner_tol", C.c_double),
-                ("max_refinements", C.c_int), ("sweep_mode", C.c_int), ("pad", C.c_int)]
+                ("max_refinements", C.c_int), ("sweep_mode", C.c_int), ("factor_sweeps", C.c_int)]
 
 
 _LIB = None
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 vp = C.c_void_p
 I, D = C.c_int, C.c_double
 
@@ -90,6 +91,10 @@ _SIGS = {
     "femgpu_ilu0_apply": (I, [vp, vp, vp]),
     "femgpu_ilu0_apply_f32": (I, [vp, vp, vp]),
     "femgpu_ilu0_get": (I, [vp, f64p, f64p]),
+    "femgpu_ilu0_factor_parallel_f32": (I, [vp, I]),
+    "femgpu_ilu0_factor_to_f32": (I, [vp]),
+    "femgpu_ilu0_apply_jacobi_f32": (I, [vp, I, vp, vp]),
+    "femgpu_ilu0_get_f32": (I, [vp, f32p, f32p]),
     "femgpu_ebe_create": (I, [vp, I, I, I, i32p, C.POINTER(vp)]),
     "femgpu_ebe_destroy": (I, [vp]),
     "femgpu_ebe_set_values": (I, [vp, vp, I]),

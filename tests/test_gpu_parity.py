@@ -377,6 +377,111 @@ def test_mixed_precision_reaches_fp64_residual(dev, torch_cuda, oracle, m):
     assert rel(x, Xo) < 1e-4
 
 
+# ----------------------------------------------------------------------------- FP32 preconditioner
+
+def parilu_reference(IA, JA, AA, sweeps):
+    """Chow-Patel fixed-point sweeps written out entry by entry (FP64), for comparison."""
+    n = len(IA) - 1
+    cols = [dict((int(JA[p]), p) for p in range(IA[i], IA[i + 1])) for i in range(n)]
+    diag = np.array([cols[i][i] for i in range(n)])
+    LU = np.array(AA[:IA[n]], dtype=np.float64)
+    for i in range(n):
+        for p in range(IA[i], diag[i]):
+            LU[p] = AA[p] / AA[diag[JA[p]]]
+    for _ in range(sweeps):
+        new = np.empty_like(LU)
+        for i in range(n):
+            for p in range(IA[i], IA[i + 1]):
+                j = int(JA[p])
+                acc = AA[p]
+                for q in range(IA[i], diag[i]):          # l_ik, k < min(i, j)
+                    k = int(JA[q])
+                    if k >= j:
+                        break
+                    r = cols[k].get(j)
+                    if r is not None and r > diag[k]:
+                        acc -= LU[q] * LU[r]
+                new[p] = acc / LU[diag[j]] if j < i else acc
+        LU = new
+    return LU, 1.0 / LU[diag]
+
+
+@pytest.mark.parametrize("name", ["st_csr_rcm_teste_r8", "st_csr_rcm_cau_r24"])
+def test_parallel_ilu_fixed_point_is_ilu0(dev, torch_cuda, oracle, name):
+    """After twice as many sweeps as the factorisation has dependency levels (the entries of one
+    row depend on each other in sequence), the parallel iteration has reached its fixed point: the
+    ILU(0) factors (FP32 rounding)."""
+    s = ilu_case(oracle, name)
+    depth = fem.ilu_levels(s["IA"], s["JA"], True)[1]
+    A = fem.CsrMatrix(dev, s["IA"], s["JA"])
+    A.set_values(s["AA"][:-1])
+    LUo, Dio, rc = oracle.ilu0_factor(s["IA"], s["JA"], s["AA"])
+    A.ilu0_factor_parallel_f32(2 * depth + 1)
+    LU, Di = A.ilu0_factors_f32()
+    assert np.abs(LU - LUo[:-1]).max() <= 1e-5 * np.abs(LUo).max()
+    assert np.abs(Di - Dio).max() <= 1e-5 * np.abs(Dio).max()
+    # a few sweeps: the iteration itself, against the entry-by-entry restatement
+    for sweeps in (0, 1, 3):
+        A.ilu0_factor_parallel_f32(sweeps)
+        LU, Di = A.ilu0_factors_f32()
+        LUr, Dir = parilu_reference(s["IA"], s["JA"], s["AA"], sweeps)
+        assert np.abs(LU - LUr).max() <= 1e-5 * np.abs(LUr).max(), sweeps
+        assert np.abs(Di - Dir).max() <= 1e-5 * np.abs(Dir).max(), sweeps
+
+
+@pytest.mark.parametrize("name", ["st_csr_rcm_teste_r8", "st_csr_rcm_cau_r24"])
+def test_jacobi_sweeps(dev, torch_cuda, oracle, name):
+    """Jacobi iterations on the triangular factors: iteration by iteration against the sparse
+    restatement, and exact (= the level-scheduled sweeps) after depth iterations."""
+    torch = torch_cuda
+    s = ilu_case(oracle, name)
+    depth = fem.ilu_levels(s["IA"], s["JA"], True)[1]
+    A = fem.CsrMatrix(dev, s["IA"], s["JA"])
+    A.set_values(s["AA"][:-1])
+    A.ilu0_factor()
+    A.ilu0_factor_to_f32()
+    LUo, Dio, rc = oracle.ilu0_factor(s["IA"], s["JA"], s["AA"])
+    n, IA, JA = s["neq"], s["IA"], s["JA"]
+    rows = np.repeat(np.arange(n), np.diff(IA))
+    Ls = np.zeros((n, n))
+    Us = np.zeros((n, n))
+    low, up = JA < rows, JA > rows
+    Ls[rows[low], JA[low]] = LUo[:-1][low]
+    Us[rows[up], JA[up]] = LUo[:-1][up]
+    b = np.random.default_rng(9).standard_normal(n)
+    exact = oracle.ilu0_apply(IA, JA, LUo, Dio, b)
+    for it in (1, 2, depth):
+        y = b.copy()
+        for _ in range(it):
+            y = b - Ls @ y
+        z = Dio * y
+        for _ in range(it):
+            z = Dio * (y - Us @ z)
+        zg = T(torch, b, torch.float32)
+        A.ilu0_apply_jacobi_f32(it, zg, zg)
+        assert rel(H(zg), z) < 1e-5, it
+    assert rel(H(zg), exact) < 1e-5
+
+
+@pytest.mark.parametrize("factor_sweeps,sweep_mode", [(0, 2), (3, 2), (2, 3)])
+def test_mixed_parallel_preconditioner(dev, torch_cuda, oracle, factor_sweeps, sweep_mode):
+    """The mixed solver with the parallel FP32 preconditioner still reaches 1e-8 (FP64-verified)."""
+    torch = torch_cuda
+    s = steady_system(oracle, m=200, k=1e-2)
+    A = fem.CsrMatrix(dev, s["IA"], s["JA"])
+    A.set_values(s["AA"][:-1])
+    if factor_sweeps == 0:
+        A.ilu0_factor()
+    X = torch.empty(s["neq"], dtype=torch.float64, device="cuda")
+    info = fem.gmres_mixed(dev, A, T(torch, s["F"]), X, tol=1e-8, inner_kmax=30, inner_tol=1e-4,
+                           sweep_mode=sweep_mode, factor_sweeps=factor_sweeps)
+    x = H(X)
+    res = np.linalg.norm(s["F"] - oracle.csrmv(s["IA"], s["JA"], s["AA"], x)) / np.linalg.norm(s["F"])
+    print(f"factor_sweeps={factor_sweeps} sweep_mode={sweep_mode}: refinements {info.cycles} "
+          f"inner steps {info.inner_iterations} true rel residual {res:.2e}")
+    assert info.status == 0 and res <= 1e-8
+
+
 def test_pcg_on_spd_diffusion(dev, torch_cuda, oracle):
     torch = torch_cuda
     x, y, ien = oracle.mesh_structured(40, 40)
