@@ -186,10 +186,6 @@ extern "C" int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA,
     if (!rc) rc = A->AA32.alloc(static_cast<size_t>(nnz) + 1);
     if (!rc) rc = A->LU.alloc(static_cast<size_t>(nnz) + 1);
     if (!rc) rc = A->Dinv.alloc(static_cast<size_t>(n) + 1);
-    // Rows per chunk: large enough that a chunk covers a few dependency levels of a banded
-    // (reordered) mesh matrix, small enough to spread over the CUs.
-    const int R = n > 200000 ? 1024 : 256;
-    if (!rc) rc = build_sweep_plan(ctx, n, IA, JA, true, R, A->lower);
     if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), false, A->plo);
     if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), true, A->pup);
     if (!rc) rc = build_stream_plan(ctx, n, IA, JA, diag.data(), false, A->slo);

@@ -71,19 +71,6 @@ struct DevBuf {
     }
 };
 
-// Level-scheduled sweep plan over contiguous row chunks (one workgroup per chunk, chunks taken
-// in order from a ticket counter, rows of a chunk grouped by dependency level).
-struct SweepPlan {
-    int n = 0, rows_per_chunk = 0, nchunks = 0, ngroups = 0, depth = 0;
-    DevBuf<int> order;      // rows of each chunk sorted by (level, row); n entries
-    DevBuf<int> grp_ptr;    // chunk -> first group (nchunks+1)
-    DevBuf<int> grp_start;  // group -> first position in order (ngroups+1)
-    DevBuf<int> grp_level;  // group -> level
-    DevBuf<int> dep;        // chunk -> farthest chunk it reads from (lower: lowest, upper: highest)
-    DevBuf<int> progress;   // chunk -> highest level completed (-1 before start); self-resetting
-    DevBuf<unsigned> ticket; // [0] next chunk ticket, [1] finished chunks; self-resetting
-};
-
 // Packed sweep plan (sweep.hip): chunks of R rows in sweep order, rows of a chunk level-sorted,
 // factor entries packed in schedule order with their dependency pre-encoded (>= 0: LDS slot of a
 // row of the same chunk; < 0: -(row+1) of a row of an earlier chunk, read from its mailbox).
@@ -165,7 +152,6 @@ struct femgpu_csr {
     fg::DevBuf<float> AA32;
     fg::DevBuf<double> LU, Dinv;    // ILU(0) factors in A's layout (FP32 copies live in the packed plans)
     bool factored = false;
-    fg::SweepPlan lower;            // scheduler of the numeric factorisation
     fg::PackedSweep plo, pup;       // packed lower / upper sweeps
     fg::StreamSweep slo, sup;       // streamed single-workgroup sweeps (used when both exist)
     int sweep_kernel = 0;           // 0: streamed when available, 1: packed
@@ -201,8 +187,6 @@ struct femgpu_mesh {
 namespace fg {
 
 // Host-side integer work (structure.cpp).
-int build_sweep_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, bool lower, int rows_per_chunk,
-                     SweepPlan &plan);
 int build_packed_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, const int *diag, bool upper,
                       PackedSweep &plan);
 int packed_refresh(femgpu_ctx *ctx, femgpu_csr *A);

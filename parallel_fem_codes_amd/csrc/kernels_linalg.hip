@@ -110,176 +110,7 @@ int launch_to_f32(femgpu_ctx *ctx, const double *src, float *dst, size_t n)
     return FEMGPU_OK;
 }
 
-// ------------------------------------------------------------------------------- sweep scheduler
-// One workgroup per row chunk. Chunks are claimed in sweep order from a ticket counter, so every
-// chunk a workgroup waits on was claimed earlier by a running (or finished) workgroup, and the
-// lowest unfinished chunk never waits: the waits always drain. Each chunk walks its level groups
-// in increasing level; before a group of level L it waits until every chunk it reads from has
-// published progress >= L-1 (all their rows of level < L done), and after the group it publishes
-// the level below its next group. The last chunk to finish restores ticket/progress for the next
-// launch, so no reset launch or memset is needed between sweeps (graph friendly).
-// Every wait is bounded (spin_limit); on expiry the sticky error flag is raised and the workgroup
-// stops waiting, so a scheduling bug reports FEMGPU_ERR_TIMEOUT instead of hanging the device.
-struct SweepArgs {
-    int n, R, nchunks;
-    const int *order, *grp_ptr, *grp_start, *grp_level, *dep;
-    int *progress;
-    unsigned *ticket;
-    int *err;
-    unsigned spin_limit;
-};
-
-__device__ __forceinline__ void sweep_wait(const SweepArgs &S, int c, bool lower, int need)
-{
-    // chunks read from: lower: [dep[c], c-1]; upper: [c+1, dep[c]]
-    const int far = S.dep[c];
-    const int ndep = lower ? (c - far) : (far - c);
-    if (threadIdx.x < ndep) {
-        for (int k = threadIdx.x; k < ndep; k += blockDim.x) {
-            const int dc = lower ? (c - 1 - k) : (c + 1 + k);
-            unsigned spins = 0;
-            while (__hip_atomic_load(&S.progress[dc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-                if (++spins > S.spin_limit) {
-                    __hip_atomic_store(S.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-                if (__hip_atomic_load(S.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        // the published rows of the chunks just observed become visible to this workgroup
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ void sweep_publish(const SweepArgs &S, int c, int value)
-{
-    // every wave makes its own row stores visible at agent scope, then one store publishes them
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(&S.progress[c], value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-
-__device__ __forceinline__ void sweep_finish(const SweepArgs &S)
-{
-    __shared__ int s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const unsigned done = __hip_atomic_fetch_add(&S.ticket[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (done == static_cast<unsigned>(S.nchunks) - 1u);
-    }
-    __syncthreads();
-    if (s_last) {
-        // every other chunk has finished all its waits and stores: restore the initial state
-        for (int c = threadIdx.x; c < S.nchunks; c += blockDim.x)
-            __hip_atomic_store(&S.progress[c], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (threadIdx.x == 0) {
-            __hip_atomic_store(&S.ticket[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&S.ticket[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-}
-
-// Returns the chunk of this workgroup, or -1 if the ticket is out of range (a launch after an
-// aborted one): then the sticky error flag is raised and the workgroup does nothing.
-__device__ __forceinline__ int sweep_claim(const SweepArgs &S, bool lower)
-{
-    __shared__ int s_chunk;
-    if (threadIdx.x == 0) {
-        const unsigned t = __hip_atomic_fetch_add(&S.ticket[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (t >= static_cast<unsigned>(S.nchunks)) {
-            __hip_atomic_store(S.err, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_chunk = -1;
-        } else {
-            s_chunk = lower ? static_cast<int>(t) : (S.nchunks - 1 - static_cast<int>(t));
-        }
-    }
-    __syncthreads();
-    return s_chunk;
-}
-
-constexpr int kSweepThreads = 256;
-
-// Numeric ILU(0) of ILUp (ILUp_precond_setup.c:84-166) for p = 0, one thread per row, rows
-// scheduled like the lower sweep (row i reads the finished U rows and pivots of its L columns).
-// LU holds A's values on entry. Same update order as the serial code, contraction off.
-__global__ __launch_bounds__(kSweepThreads) void k_ilu0_factor(SweepArgs S, const int *__restrict__ IA,
-                                                               const int *__restrict__ JA,
-                                                               const int *__restrict__ diag, double *LU,
-                                                               double *Dinv)
-{
-#pragma clang fp contract(off)
-    const int c = sweep_claim(S, true);
-    if (c < 0) return;
-    const int g0 = S.grp_ptr[c], g1 = S.grp_ptr[c + 1];
-    if (threadIdx.x == 0)
-        __hip_atomic_store(&S.progress[c], S.grp_level[g0] - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int g = g0; g < g1; ++g) {
-        const int lev = S.grp_level[g];
-        if (lev > 0) sweep_wait(S, c, true, lev - 1);
-        const int k1 = S.grp_start[g + 1];
-        for (int k = S.grp_start[g] + threadIdx.x; k < k1; k += blockDim.x) {
-            const int i = S.order[k];
-            const int p0 = IA[i], d = diag[i], p1 = IA[i + 1];
-            double dv = LU[d];
-            for (int j = p0; j < d; ++j) {
-                const int jrow = JA[j];
-                const double lij = LU[j] * Dinv[jrow];
-                LU[j] = lij;
-                const int ke = IA[jrow + 1];
-                for (int kk = diag[jrow] + 1; kk < ke; ++kk) {
-                    const int col = JA[kk];
-                    if (col == i) {
-                        dv -= lij * LU[kk];
-                        continue;
-                    }
-                    // position of col in row i (sorted): binary search
-                    int lo = p0, hi = p1 - 1, pos = -1;
-                    while (lo <= hi) {
-                        const int m = (lo + hi) >> 1;
-                        const int cm = JA[m];
-                        if (cm == col) { pos = m; break; }
-                        if (cm < col) lo = m + 1; else hi = m - 1;
-                    }
-                    if (pos >= 0) LU[pos] -= lij * LU[kk];
-                }
-            }
-            LU[d] = dv;
-            if (dv == 0.0) {
-                __hip_atomic_store(S.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                Dinv[i] = 0.0;
-            } else {
-                Dinv[i] = 1.0 / dv;
-            }
-        }
-        sweep_publish(S, c, (g + 1 < g1) ? S.grp_level[g + 1] - 1 : INT_MAX);
-    }
-    sweep_finish(S);
-}
-
-static SweepArgs make_args(femgpu_ctx *ctx, SweepPlan &P)
-{
-    SweepArgs a;
-    a.n = P.n;
-    a.R = P.rows_per_chunk;
-    a.nchunks = P.nchunks;
-    a.order = P.order.p;
-    a.grp_ptr = P.grp_ptr.p;
-    a.grp_start = P.grp_start.p;
-    a.grp_level = P.grp_level.p;
-    a.dep = P.dep.p;
-    a.progress = P.progress.p;
-    a.ticket = P.ticket.p;
-    a.err = ctx->err.p;
-    a.spin_limit = 1u << 22;   // a few seconds of polling per wait at most
-    return a;
-}
-
+// ------------------------------------------------------------------------------- ILU(0) factorisation
 // Numeric ILU(0) by one persistent workgroup walking the level schedule of the lower sweep: rows of
 // a level in parallel, a barrier between levels. Inside one workgroup the barrier alone makes the
 // finished rows visible (one CU, one L1), so no agent-scope synchronisation is needed.

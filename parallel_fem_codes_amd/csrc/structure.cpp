@@ -400,60 +400,6 @@ extern "C" int femgpu_ilu_levels(int n, const int *IA, const int *JA, int lower,
 
 namespace fg {
 
-// Chunked schedule: rows split into contiguous chunks of rows_per_chunk (in sweep order: ascending
-// for the lower factor, descending for the upper one); inside a chunk rows are grouped by level so
-// a workgroup can walk its groups in increasing level, waiting only on the chunks it reads from.
-int build_sweep_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, bool lower, int R, SweepPlan &P)
-{
-    std::vector<int> level(static_cast<size_t>(n) + 1);
-    P.depth = femgpu_ilu_levels(n, IA, JA, lower ? 1 : 0, level.data());
-    P.n = n;
-    P.rows_per_chunk = R;
-    P.nchunks = (n + R - 1) / R;
-    std::vector<int> order(n), grp_ptr(static_cast<size_t>(P.nchunks) + 1), grp_start, grp_level,
-        dep(P.nchunks);
-    grp_start.reserve(static_cast<size_t>(P.nchunks) * 8);
-    std::vector<std::pair<int, int>> rows;
-    int pos = 0;
-    for (int c = 0; c < P.nchunks; ++c) {
-        const int r0 = c * R, r1 = std::min(n, r0 + R);
-        rows.clear();
-        int far = c;
-        for (int r = r0; r < r1; ++r) {
-            rows.emplace_back(level[r], r);
-            if (lower) {
-                if (IA[r] < IA[r + 1] && JA[IA[r]] < r) far = std::min(far, JA[IA[r]] / R);
-            } else {
-                if (IA[r] < IA[r + 1] && JA[IA[r + 1] - 1] > r) far = std::max(far, JA[IA[r + 1] - 1] / R);
-            }
-        }
-        std::sort(rows.begin(), rows.end());
-        grp_ptr[c] = static_cast<int>(grp_level.size());
-        for (size_t q = 0; q < rows.size(); ++q) {
-            if (q == 0 || rows[q].first != rows[q - 1].first) {
-                grp_start.push_back(pos);
-                grp_level.push_back(rows[q].first);
-            }
-            order[pos++] = rows[q].second;
-        }
-        dep[c] = far;
-    }
-    grp_ptr[P.nchunks] = static_cast<int>(grp_level.size());
-    grp_start.push_back(pos);
-    P.ngroups = static_cast<int>(grp_level.size());
-    hipStream_t s = ctx->stream;
-    FG_TRY(P.order.upload(order.data(), order.size(), s));
-    FG_TRY(P.grp_ptr.upload(grp_ptr.data(), grp_ptr.size(), s));
-    FG_TRY(P.grp_start.upload(grp_start.data(), grp_start.size(), s));
-    FG_TRY(P.grp_level.upload(grp_level.data(), grp_level.size(), s));
-    FG_TRY(P.dep.upload(dep.data(), dep.size(), s));
-    FG_TRY(P.progress.alloc(static_cast<size_t>(P.nchunks)));
-    FG_TRY(P.ticket.alloc(2));
-    FG_HIP(hipMemsetAsync(P.progress.p, 0xFF, sizeof(int) * P.nchunks, s));
-    FG_HIP(hipMemsetAsync(P.ticket.p, 0, sizeof(unsigned) * 2, s));
-    return FEMGPU_OK;
-}
-
 }  // namespace fg
 
 namespace fg {
