@@ -641,6 +641,124 @@ void orc_build_transport_transient(int nnodes, int nel, int NEQ, const double *x
     free(U); free(dU);
 }
 
+/* ====================================================================== Navier-Stokes (NDOF = 3) */
+
+void orc_ns_nodal_state(int nnodes, const int *type3, const int *invP3, const double *u,
+                        const double *presc3, double *U3)
+{
+    for (int k = 0; k < 3 * nnodes; k++) U3[k] = (type3[k] > 0) ? u[invP3[k]] : presc3[k];
+}
+
+/* Per-element restatement of Build_K_F_SUPG_PSPG.c:41-466. Vertex a has the edge vectors
+ * b[a] = (y23, y31, y12)[a] and c[a] = (x32, x13, x21)[a]; every term below is the reference's
+ * expression for vertex pair (a, b) evaluated in the reference's operation order (for the
+ * symmetric terms the reference evaluates the (min, max) pair and reuses it, so the loops do too). */
+void orc_build_ns_supg_pspg(int nnodes, int nel, int NEQ, const double *x, const double *y,
+                            const int *ien, const int *lm, const double *U3, const double *fx,
+                            const double *fy, double Re, double *Ke, double *F)
+{
+    const double third = 1.0 / 3.0, sixth = 1.0 / 6.0;
+    (void)nnodes;
+    for (int I = 0; I <= NEQ; I++) F[I] = 0;
+    for (int e = 0; e < nel; e++) {
+        const int *v = ien + 3 * e;
+        const double X0 = x[v[0]], X1 = x[v[1]], X2 = x[v[2]];
+        const double Y0 = y[v[0]], Y1 = y[v[1]], Y2 = y[v[2]];
+        const double b[3] = {Y1 - Y2, Y2 - Y0, Y0 - Y1};   /* y23, y31, y12 */
+        const double c[3] = {X2 - X1, X0 - X2, X1 - X0};   /* x32, x13, x21 */
+        const double TwoA = fabs(c[2] * b[1] - c[1] * b[2]);
+        const double Area = 0.5 * TwoA, invArea = 1.0 / Area;
+        double ux_[3], uy_[3], p_[3];
+        for (int a = 0; a < 3; a++) {
+            ux_[a] = U3[3 * v[a]];
+            uy_[a] = U3[3 * v[a] + 1];
+            p_[a] = U3[3 * v[a] + 2];
+        }
+        const double ux = third * (ux_[0] + ux_[1] + ux_[2]);
+        const double uy = third * (uy_[0] + uy_[1] + uy_[2]);
+        const double duxdx = 0.5 * invArea * (ux_[0] * b[0] + ux_[1] * b[1] + ux_[2] * b[2]);
+        const double duxdy = 0.5 * invArea * (ux_[0] * c[0] + ux_[1] * c[1] + ux_[2] * c[2]);
+        const double duydx = 0.5 * invArea * (uy_[0] * b[0] + uy_[1] * b[1] + uy_[2] * b[2]);
+        const double duydy = 0.5 * invArea * (uy_[0] * c[0] + uy_[1] * c[1] + uy_[2] * c[2]);
+        const double duxduy = ux_[0] * b[0] + uy_[0] * c[0] + ux_[1] * b[1] + uy_[1] * c[1] + ux_[2] * b[2] +
+                              uy_[2] * c[2];
+        const double p = third * (p_[0] + p_[1] + p_[2]);
+        const double dpdx = 0.5 * invArea * (p_[0] * b[0] + p_[1] * b[1] + p_[2] * b[2]);
+        const double dpdy = 0.5 * invArea * (p_[0] * c[0] + p_[1] * c[1] + p_[2] * c[2]);
+        double C[3];
+        for (int a = 0; a < 3; a++) C[a] = 0.5 * invArea * (ux * b[a] + uy * c[a]);
+        /* stabilisation parameters (:118-131) */
+        const double h = sqrt(TwoA), visc = 1. / Re, rho = 1.;
+        const double invrho = 1.0 / rho, nu = visc / rho;
+        const double unorm = sqrt(ux * ux + uy * uy);
+        double auxtau = (2.0 * unorm / h) * (2.0 * unorm / h) + 9.0 * (4.0 * nu / (h * h)) * (4.0 * nu / (h * h));
+        auxtau = sqrt(auxtau);
+        const double tau = 1.0 / auxtau;
+        const double tau_l = sqrt(nu * nu + (0.5 * h * unorm) * (0.5 * h * unorm));
+        /* dof-interleaved edge vector P = (y23, x32, y31, x13, y12, x21) of the LSIC terms */
+        const double P6[6] = {b[0], c[0], b[1], c[1], b[2], c[2]};
+        double *K = Ke + (size_t)81 * e;
+        for (int a = 0; a < 3; a++) {
+            for (int bb = 0; bb < 3; bb++) {
+                const int lo = a < bb ? a : bb, hi = a < bb ? bb : a;
+                const double Nb = rho * third * Area * C[bb];                       /* N11/N13/N15 */
+                const double Nd = rho * tau * Area * C[lo] * C[hi];                 /* Ndelta */
+                const double Kuu = visc * 0.25 * invArea * (2 * b[lo] * b[hi] + c[lo] * c[hi]);
+                const double Kvv = visc * 0.25 * invArea * (b[lo] * b[hi] + 2 * c[lo] * c[hi]);
+                const double Kuv = visc * 0.25 * invArea * (c[a] * b[bb]);
+                const double Kvu = visc * 0.25 * invArea * (b[a] * c[bb]);
+                const double ks = tau_l * rho * 0.25 * invArea;
+                const double Ks_uu = ks * P6[2 * bb] * P6[2 * a], Ks_uv = ks * P6[2 * bb + 1] * P6[2 * a];
+                const double Ks_vu = ks * P6[2 * bb] * P6[2 * a + 1], Ks_vv = ks * P6[2 * bb + 1] * P6[2 * a + 1];
+                const double GTu_a = sixth * b[a], GTv_a = sixth * c[a];
+                const double GTu_b = sixth * b[bb], GTv_b = sixth * c[bb];
+                const double Gdu_ab = 0.5 * tau * C[a] * b[bb], Gdv_ab = 0.5 * tau * C[a] * c[bb];
+                const double Gdu_ba = 0.5 * tau * C[bb] * b[a], Gdv_ba = 0.5 * tau * C[bb] * c[a];
+                const double Gphi = invrho * 0.25 * tau * invArea * (b[lo] * b[hi] + c[lo] * c[hi]);
+                double *r0 = K + 27 * a + 3 * bb;   /* row 3a (u), columns 3bb .. 3bb+2 */
+                r0[0] = Nb + Nd + Kuu + Ks_uu;
+                r0[1] = Kuv + Ks_uv;
+                r0[2] = -(GTu_a + Gdu_ab);
+                double *r1 = r0 + 9;                /* row 3a+1 (v) */
+                r1[0] = Kvu + Ks_vu;
+                r1[1] = Nb + Nd + Kvv + Ks_vv;
+                r1[2] = -(GTv_a + Gdv_ab);
+                double *r2 = r0 + 18;               /* row 3a+2 (p) */
+                r2[0] = GTu_b + Gdu_ba;
+                r2[1] = GTv_b + Gdv_ba;
+                r2[2] = Gphi;
+            }
+        }
+        /* loads (:392-430) */
+        const double fxB = third * (fx[v[0]] + fx[v[1]] + fx[v[2]]);
+        const double fyB = third * (fy[v[0]] + fy[v[1]] + fy[v[2]]);
+        const double f1 = rho * third * Area * fxB, f2 = rho * third * Area * fyB;
+        /* residual (:432-464) */
+        const double cvx = duxdx * ux + duxdy * uy, cvy = duydx * ux + duydy * uy;
+        const double Nvx = rho * third * Area * cvx, Nvy = rho * third * Area * cvy;
+        const double GTv = third * Area * (duxdx + duydy);
+        double Res[9];
+        for (int a = 0; a < 3; a++) {
+            const double fdx = rho * tau * Area * C[a] * fxB, fdy = rho * tau * Area * C[a] * fyB;
+            const double fphi = tau * Area * (b[a] * fxB + c[a] * fyB);
+            const double Ndx = rho * tau * Area * C[a] * cvx, Ndy = rho * tau * Area * C[a] * cvy;
+            const double Kvx = 0.5 * visc * (2 * b[a] * duxdx + c[a] * (duxdy + duydx));
+            const double Kvy = 0.5 * visc * (2 * c[a] * duydy + b[a] * (duxdy + duydx));
+            const double Ksx = tau_l * rho * 0.25 * invArea * b[a] * duxduy;
+            const double Ksy = tau_l * rho * 0.25 * invArea * c[a] * duxduy;
+            const double Gvx = 0.5 * p * b[a], Gvy = 0.5 * p * c[a];
+            const double Gdx = tau * Area * C[a] * dpdx, Gdy = tau * Area * C[a] * dpdy;
+            const double Nphi = 0.5 * tau * ((duxdx * ux + duxdy * uy) * b[a] + (duydx * ux + duydy * uy) * c[a]);
+            const double Gphiv = 0.5 * tau * invrho * (b[a] * dpdx + c[a] * dpdy);
+            Res[3 * a] = (f1 + fdx) - (Nvx + Ndx + Kvx + Ksx - (Gvx + Gdx));
+            Res[3 * a + 1] = (f2 + fdy) - (Nvy + Ndy + Kvy + Ksy - (Gvy + Gdy));
+            Res[3 * a + 2] = fphi - (GTv + Nphi + Gphiv);
+        }
+        for (int i = 0; i < 9; i++) F[lm[9 * e + i]] += Res[i];
+        F[NEQ] = 0;
+    }
+}
+
 void orc_csr_assemble(int nel, int s, int nnz, const int *scheme, const double *Ke, double *AA)
 {
     const size_t ss = (size_t)s * s;

@@ -101,6 +101,21 @@ void orc_build_transport_transient(int nnodes, int nel, int NEQ, const double *x
                                    const double *du, const orc_transport_problem *pb,
                                    double alpha, double dt, double *Ae, double *R);
 
+/* eval_U (04_SS_NavierStokes_Equations/2D/01_CommonFiles/eval_U.c:3-38): nodal state
+ * U3[3I+d] (d = u, v, p) = u[invP3[3I+d]] when type3[3I+d] > 0, else the prescribed presc3[3I+d]. */
+void orc_ns_nodal_state(int nnodes, const int *type3, const int *invP3, const double *u,
+                        const double *presc3, double *U3);
+
+/* Build_K_F_SUPG_PSPG (04_SS_NavierStokes_Equations/2D/01_CommonFiles/Build_K_F_SUPG_PSPG.c:8-671),
+ * the Picard tangent (the branch taken while its call counter is below 50000, :466-566): per element
+ * the 9x9 SUPG/PSPG/LSIC matrix (dofs u, v, p per vertex) into Ke[nel*81] row-major
+ * (ebe_assembly.c:5-17) and the residual Res added to F[lm] in element order, F[NEQ] = 0
+ * (:655-658). U3: nodal state (orc_ns_nodal_state); fx, fy: f1ext/f2ext at the nodes; rho = 1,
+ * viscosity 1/Re (:120-125). */
+void orc_build_ns_supg_pspg(int nnodes, int nel, int NEQ, const double *x, const double *y,
+                            const int *ien, const int *lm, const double *U3, const double *fx,
+                            const double *fy, double Re, double *Ke, double *F);
+
 /* csr_assembly (csr_assembly.c:3-54, one rank): AA[nnz+1] zeroed then AA[scheme] += Ke per
  * element in element order; AA[nnz] = 0. */
 void orc_csr_assemble(int nel, int s, int nnz, const int *scheme, const double *Ke, double *AA);

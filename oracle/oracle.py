@@ -81,6 +81,9 @@ def lib():
                                                     C.POINTER(TransportProblem), C.c_double,
                                                     C.c_double, f64p, f64p]
         L.orc_csr_assemble.argtypes = [C.c_int, C.c_int, C.c_int, i32p, f64p, f64p]
+        L.orc_ns_nodal_state.argtypes = [C.c_int, i32p, i32p, f64p, f64p, f64p]
+        L.orc_build_ns_supg_pspg.argtypes = [C.c_int, C.c_int, C.c_int, f64p, f64p, i32p, i32p, f64p,
+                                             f64p, f64p, C.c_double, f64p, f64p]
         L.orc_ebemv.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, f64p, f64p, f64p]
         L.orc_ebemv_block.argtypes = [C.c_int, C.c_int, C.c_int, i32p, f64p, f64p, f64p]
         L.orc_csrmv.argtypes = [C.c_int, i32p, i32p, f64p, f64p, f64p]
@@ -188,6 +191,22 @@ def build_transport_transient(x, y, type_, invP, ien, lm, ubc, u, du, pb, alpha,
                                         _i(ien), _i(lm), _f(ubc), _f(u), _f(du), C.byref(pb),
                                         alpha, dt, Ae, R)
     return Ae, R
+
+
+def ns_nodal_state(type3, invP3, u, presc3):
+    """eval_U: nodal (u, v, p) from the unknowns and the prescribed values."""
+    U3 = np.empty(len(type3))
+    lib().orc_ns_nodal_state(len(type3) // 3, _i(type3), _i(invP3), _f(np.append(u, 0.0)), _f(presc3), U3)
+    return U3
+
+
+def build_ns_supg_pspg(x, y, ien, lm, U3, fx, fy, Re, NEQ):
+    """Build_K_F_SUPG_PSPG (Picard tangent): Ke[nel*81], F[NEQ+1]."""
+    nel = len(ien) // 3
+    Ke, F = np.empty(81 * nel), np.empty(NEQ + 1)
+    lib().orc_build_ns_supg_pspg(len(x), nel, NEQ, _f(x), _f(y), _i(ien), _i(lm), _f(U3), _f(fx), _f(fy),
+                                 float(Re), Ke, F)
+    return Ke, F
 
 
 def csr_assemble(scheme, Ke, nnz, s=3):

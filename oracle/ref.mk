@@ -26,7 +26,13 @@ CF_SRC := $(CF)/Allocation_Operations/mycalloc.c \
           $(CF)/Solvers_and_Preconditioners/CSR/ILUp_precond_setup.c \
           $(CF)/Solvers_and_Preconditioners/pgmres.c
 
-all: $(OUT)/st_harness $(OUT)/meshGenerator
+NS      := $(REF)/04_SS_NavierStokes_Equations/2D/01_CommonFiles
+NS_SRC  := $(NS)/Build_K_F_SUPG_PSPG.c $(NS)/eval_U.c $(NS)/setzeros.c $(NS)/ebe_assembly.c \
+           $(NS)/csr_assembly.c $(NS)/Fill_LM.c $(NS)/csr_Initialization.c
+NS_CF   := $(CF)/Allocation_Operations/mycalloc.c $(CF)/BLAS_Operations/dzero.c \
+           $(CF)/BLAS_Operations/ddot.c $(CF)/Reordering/reordering.c $(CF)/Reordering/symrcm.c
+
+all: $(OUT)/st_harness $(OUT)/ns_harness $(OUT)/meshGenerator
 
 # The structured mesh generator is a self-contained C program.
 $(OUT)/meshGenerator: $(REF)/Mesh_Generator/Structured_Mesh/meshGenerator.c
@@ -40,6 +46,12 @@ $(OUT)/st_harness: $(HERE)ref_harness_st.c $(ST_SRC) $(CF_SRC)
 	gcc $(CFLAGS) -o $@ $(HERE)ref_harness_st.c $(ST_SRC) $(CF_SRC) \
 	    -L$(MPI)/lib -Wl,-rpath,$(MPI)/lib -lmpi -lm \
 	    -Wl,--unresolved-symbols=ignore-in-object-files
+
+$(OUT)/ns_harness: $(HERE)ref_harness_ns.c $(NS_SRC) $(NS_CF)
+	mkdir -p $(OUT)
+	gcc -O2 -fno-fast-math -ffp-contract=off -w -DSSNavierStokesEquations2D -I$(NS) -I$(MPI)/include \
+	    -o $@ $(HERE)ref_harness_ns.c $(NS_SRC) $(NS_CF) \
+	    -L$(MPI)/lib -Wl,-rpath,$(MPI)/lib -lmpi -lm -Wl,--unresolved-symbols=ignore-in-object-files
 
 clean:
 	rm -rf $(OUT)

@@ -75,6 +75,13 @@ def run_mesh_generator(M, N, cut, workdir):
     return x, y, t, ien
 
 
+NS_CASES = [
+    dict(name="ns_ebe_r8", M=8, N=8, cut="right", ebe=1, reorder=0, Re=1000.0),
+    dict(name="ns_csr_rcm_l10", M=10, N=10, cut="left", ebe=0, reorder=1, Re=100.0),
+    dict(name="ns_csr_not_r6x5", M=6, N=5, cut="right", ebe=0, reorder=0, Re=1.0),
+]
+
+
 def read_records(path):
     out = {}
     with open(path, "rb") as f:
@@ -114,6 +121,38 @@ def run_case(c, workdir):
     return data
 
 
+def run_ns_case(c, workdir):
+    """Navier-Stokes assembly on a cavity-like problem: velocity prescribed on the boundary (lid
+    u = 1 at y = 1), pressure pinned at the first interior node, random smooth sources and state."""
+    x, y, t1, ien = run_mesh_generator(c["M"], c["N"], c["cut"], workdir)
+    nn = len(x)
+    bnd = t1 == 0
+    type3 = np.ones((nn, 3), np.int32)
+    type3[bnd, 0] = 0
+    type3[bnd, 1] = 0
+    type3[np.flatnonzero(~bnd)[0], 2] = 0
+    presc = np.zeros((nn, 3))
+    presc[bnd & (np.abs(y - 1.0) <= 1e-15), 0] = 1.0
+    rng = np.random.default_rng(zlib.crc32(c["name"].encode()))
+    fsrc = np.stack([np.sin(3 * x + y), np.cos(2 * y - x)], 1) * 0.1
+    state = rng.uniform(-1, 1, (nn, 3))
+    hd = np.zeros(16, np.int32)
+    hd[:4] = [nn, len(ien) // 3, c["ebe"], c["reorder"]]
+    pr = np.zeros(16)
+    pr[0] = c["Re"]
+    inp, outp = os.path.join(workdir, "in.bin"), os.path.join(workdir, "out.bin")
+    with open(inp, "wb") as f:
+        for a in (hd, pr, x, y, type3, ien, presc, fsrc, state):
+            f.write(np.ascontiguousarray(a).tobytes())
+    subprocess.run([MPIEXEC, "-n", "1", os.path.join(REFBIN, "ns_harness"), inp, outp], check=True,
+                   stdout=subprocess.DEVNULL, timeout=600)
+    rec = read_records(outp)
+    data = dict(x=x, y=y, type3=type3.ravel(), ien=ien, presc=presc.ravel(), fsrc=fsrc.ravel(),
+                state=state.ravel(), header=hd, params=pr)
+    data.update({"ref_" + k: v for k, v in rec.items()})
+    return data
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -132,6 +171,14 @@ def main():
             meta["iters"] = d["ref_iters"].tolist()
             manifest[c["name"]] = meta
             print(c["name"], "iters", meta["iters"], "bytes", os.path.getsize(fn))
+        for c in NS_CASES:
+            d = run_ns_case(c, work)
+            fn = os.path.join(HERE, c["name"] + ".npz")
+            np.savez_compressed(fn, **d)
+            meta = {k: v for k, v in c.items() if k != "name"}
+            meta["sha256"] = hashlib.sha256(open(fn, "rb").read()).hexdigest()
+            manifest[c["name"]] = meta
+            print(c["name"], "bytes", os.path.getsize(fn))
     finally:
         shutil.rmtree(work)
     with open(os.path.join(HERE, "manifest.json"), "w") as f:

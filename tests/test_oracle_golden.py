@@ -147,3 +147,48 @@ def test_gmres_iterations_and_solutions(oracle, name):
     assert iters == d["ref_iters"].tolist()
     assert same(X1, d["ref_X1"])
     assert same(u, d["ref_u"])
+
+
+# ----------------------------------------------------------------------------- Navier-Stokes (NDOF = 3)
+
+NS_CASES = G.names("ns_")
+
+
+def ns_setup(oracle, d):
+    """Numbering, lm and (CSR) structure of a Navier-Stokes fixture, as the reference builds them:
+    per-dof running numbering, Fill_LM with NDOF = 3, csr_Initialization with s = 9 that also
+    renumbers lm and invP_id (04_SS_NavierStokes_Equations/.../csr_Initialization.c:330-345)."""
+    h = d["header"]
+    ebe, reorder = int(h[2]), int(h[3])
+    type3 = d["type3"]
+    invP, neq = oracle.number_unknowns(type3, 3)
+    lm = oracle.fill_lm(d["ien"], type3, invP, neq, 3)
+    assert np.array_equal(lm, d["ref_lm"])      # Fill_LM output, before any renumbering
+    c = None
+    if not ebe:
+        c = oracle.csr_init(lm, 9, neq, reorder="SYMRCM" if reorder else "NOT", permute_lm=True,
+                            type_=type3, invP=invP)
+        lm, invP = c["lm"], c["invP"]
+    u = np.zeros(neq)
+    unk = type3 != 0
+    u[invP[unk]] = d["state"][unk]
+    return ebe, neq, lm, invP, u, c
+
+
+@pytest.mark.parametrize("name", NS_CASES)
+def test_ns_structure_and_assembly_bit_exact(oracle, name):
+    d = G.load(name)
+    ebe, neq, lm, invP, u, c = ns_setup(oracle, d)
+    assert np.array_equal(invP, d["ref_invP_id"])
+    if c is not None:
+        for k in ("IA", "JA", "perm", "scheme"):
+            assert np.array_equal(c[k], d["ref_" + k]), k
+    U3 = oracle.ns_nodal_state(d["type3"], invP, u, d["presc"])
+    f = d["fsrc"].reshape(-1, 2)
+    Ke, F = oracle.build_ns_supg_pspg(d["x"], d["y"], d["ien"], lm, U3, f[:, 0].copy(), f[:, 1].copy(),
+                                      float(d["params"][0]), neq)
+    assert np.array_equal(F, d["ref_F"])
+    if ebe:
+        assert np.array_equal(Ke, d["ref_A"])
+    else:
+        assert np.array_equal(oracle.csr_assemble(c["scheme"], Ke, c["nnz"], s=9), d["ref_AA"])
