@@ -264,6 +264,26 @@ int femgpu_assemble_transport_transient(femgpu_mesh *M, const femgpu_transport_p
                                         const double *d_du, double *d_Ae, double *d_R,
                                         double *d_AA);
 
+/* =========================================================== Navier-Stokes assembly (NDOF = 3) */
+
+/* Device-resident Navier-Stokes mesh: node coordinates, per-dof Type and invP_id (type3, invP3:
+ * 3*nnodes, node-major u, v, p; invP_id in the solve numbering), prescribed values presc3
+ * (v1presc, v2presc, ppresc at the node), connectivity ien[nel*3], lm[nel*9] (sink NEQ = neq) and,
+ * when scheme is non-null, the CSR slot table scheme[nel*81] (femgpu_csr_init with s = 9).
+ * Destroy with femgpu_mesh_destroy. */
+int femgpu_ns_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, const double *x,
+                          const double *y, const int *type3, const int *invP3, const double *presc3,
+                          const int *ien, const int *lm, const int *scheme, int nnz,
+                          femgpu_mesh **out);
+
+/* Build_K_F_SUPG_PSPG (04_SS_NavierStokes_Equations/2D/01_CommonFiles/Build_K_F_SUPG_PSPG.c:8-671,
+ * the Picard tangent of :466-566, with eval_U.c:3-38): 9x9 SUPG/PSPG/LSIC element matrices into
+ * d_Ke[nel*81] (nullable), residual into d_F[NEQ+1] (F[lm] += Res in element order, F[NEQ] = 0),
+ * CSR values into d_AA[nnz+1] (nullable; csr_assembly order). d_u: current iterate (neq);
+ * d_fx, d_fy: f1ext/f2ext at the nodes (nullable: zero source); Re: Reynolds number (rho = 1). */
+int femgpu_assemble_ns(femgpu_mesh *M, double Re, const double *d_u, const double *d_fx,
+                       const double *d_fy, double *d_Ke, double *d_F, double *d_AA);
+
 #ifdef __cplusplus
 }
 #endif

@@ -282,8 +282,7 @@ static ElemIn elem_in(femgpu_mesh *M, const femgpu_transport_problem *pb, const 
     return in;
 }
 
-// Counting sort of (key, source) pairs by key, keeping source order inside a key.
-static void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vector<int> &ptr,
+void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vector<int> &ptr,
                     std::vector<int> &src)
 {
     ptr.assign(static_cast<size_t>(nkeys) + 1, 0);
@@ -383,6 +382,7 @@ extern "C" int femgpu_assemble_transport(femgpu_mesh *M, const femgpu_transport_
                                          double *d_Ke, double *d_F, double *d_AA)
 {
     FG_CHECK(M && pb && d_u && d_F, FEMGPU_ERR_ARG, "femgpu_assemble_transport: bad arguments");
+    FG_CHECK(M->ndof == 1, FEMGPU_ERR_STATE, "femgpu_assemble_transport: not a transport mesh");
     FG_CHECK(!d_AA || M->nnz > 0 || M->aa_ptr.p, FEMGPU_ERR_STATE,
              "femgpu_assemble_transport: mesh created without a CSR slot table");
     femgpu_ctx *ctx = M->ctx;
@@ -409,6 +409,7 @@ extern "C" int femgpu_assemble_transport_transient(femgpu_mesh *M, const femgpu_
                                                    double *d_R, double *d_AA)
 {
     FG_CHECK(M && pb && d_u && d_du && d_R, FEMGPU_ERR_ARG, "femgpu_assemble_transport_transient: bad arguments");
+    FG_CHECK(M->ndof == 1, FEMGPU_ERR_STATE, "femgpu_assemble_transport_transient: not a transport mesh");
     femgpu_ctx *ctx = M->ctx;
     double *Ae = d_Ae ? d_Ae : M->ke.p;
     const ElemIn in = elem_in(M, pb, d_u, d_du, alpha * dt);

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "../../include/femgpu.h"
@@ -174,12 +175,13 @@ struct femgpu_ebe {
 
 struct femgpu_mesh {
     femgpu_ctx *ctx = nullptr;
+    int ndof = 1;                       // 1: transport; 3: Navier-Stokes (u, v, p per node)
     int nnodes = 0, nel = 0, neq = 0, nnz = 0;
-    fg::DevBuf<double> x, y, ubc;
-    fg::DevBuf<int> type, invP, ien, lm;
-    fg::DevBuf<int> rhs_ptr, rhs_src;   // unknown -> (element*3 + a) contributions, element order
-    fg::DevBuf<int> aa_ptr, aa_src;     // CSR slot -> (element*9 + ab) contributions, element order
-    fg::DevBuf<double> fe, lift;        // per element: 3 loads, 9 coupling terms
+    fg::DevBuf<double> x, y, ubc;       // ubc: prescribed value per dof (ndof per node)
+    fg::DevBuf<int> type, invP, ien, lm;   // type/invP per dof, lm[nel*3*ndof]
+    fg::DevBuf<int> rhs_ptr, rhs_src;   // unknown -> (element*s + i) contributions, element order
+    fg::DevBuf<int> aa_ptr, aa_src;     // CSR slot -> (element*s*s + ij) contributions, element order
+    fg::DevBuf<double> fe, lift;        // per element: s loads / residual entries, 9 coupling terms
     fg::DevBuf<int> mask;               // per element: Dirichlet-vertex bit mask
     fg::DevBuf<double> ke;              // element matrices when the caller passes none
 };
@@ -214,6 +216,8 @@ int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *
 int launch_ebe_gather_values(femgpu_ctx *ctx, femgpu_ebe *E, const double *A_elem_order);
 
 int dot64(femgpu_ctx *ctx, int n, const double *x, const double *y, double *d_out);
+// Counting sort of (key, source) pairs by key, keeping source order inside a key (assembly maps).
+void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vector<int> &ptr, std::vector<int> &src);
 int check_device_error(femgpu_ctx *ctx);
 
 }  // namespace fg

@@ -336,6 +336,29 @@ class Mesh:
                                                             C.c_void_p(AA_ptr) if AA_ptr else None))
 
 
+class NsMesh:
+    """Device-resident Navier-Stokes mesh (u, v, p per node) for Build_K_F_SUPG_PSPG."""
+
+    def __init__(self, dev: Device, x, y, type3, invP3, presc3, ien, lm, neq, scheme=None, nnz=0):
+        self.dev, self.neq, self.nel = dev, neq, len(ien) // 3
+        h = C.c_void_p()
+        sc = _i32(scheme) if scheme is not None else None
+        N.check(N.lib().femgpu_ns_mesh_create(
+            dev.h, len(x), self.nel, neq, _f64(x), _f64(y), _i32(type3), _i32(invP3), _f64(presc3), _i32(ien),
+            _i32(lm), sc.ctypes.data_as(C.c_void_p) if sc is not None else None, int(nnz), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            N.lib().femgpu_mesh_destroy(self.h)
+            self.h = None
+
+    def Build_K_F_SUPG_PSPG(self, Re, u, F, fx=None, fy=None, Ke=None, AA_ptr=None):
+        """Picard tangent and residual; F has NEQ+1 entries (F[NEQ] = 0)."""
+        N.check(N.lib().femgpu_assemble_ns(self.h, float(Re), _ptr(u), _ptr(fx), _ptr(fy), _ptr(Ke), _ptr(F),
+                                           C.c_void_p(AA_ptr) if AA_ptr else None))
+
+
 def _system(op, ilu):
     S = N.System()
     if isinstance(op, CsrMatrix):

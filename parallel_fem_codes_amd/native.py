@@ -112,6 +112,9 @@ _SIGS = {
     "femgpu_mesh_create": (I, [vp, I, I, I, f64p, f64p, i32p, i32p, f64p, i32p, vp, vp, I,
                               C.POINTER(vp)]),
     "femgpu_mesh_destroy": (I, [vp]),
+    "femgpu_ns_mesh_create": (I, [vp, I, I, I, f64p, f64p, i32p, i32p, f64p, i32p, i32p, vp, I,
+                                  C.POINTER(vp)]),
+    "femgpu_assemble_ns": (I, [vp, D, vp, vp, vp, vp, vp, vp]),
     "femgpu_assemble_transport": (I, [vp, C.POINTER(TransportProblem), vp, vp, vp, vp]),
     "femgpu_assemble_transport_transient": (I, [vp, C.POINTER(TransportProblem), D, D, vp, vp,
                                                 vp, vp, vp]),

@@ -518,3 +518,64 @@ def test_blas(dev, torch_cuda, oracle):
     dev.dzero(tb)
     dev.sync()
     assert np.array_equal(H(ta), 2.0 * (b + 0.5 * a)) and not H(tb).any()
+
+
+# ----------------------------------------------------------------------------- Navier-Stokes assembly
+
+def ns_case(oracle, d=None, m=None, reorder="SYMRCM", Re=1000.0):
+    """Navier-Stokes system on a golden fixture (as the reference numbered it) or on an m x m
+    cavity-like mesh: numbering, lm, (CSR) structure and the oracle's Ke / AA / F."""
+    if d is not None:
+        x, y, ien, type3 = d["x"], d["y"], d["ien"], d["type3"]
+        presc, fsrc, state = d["presc"], d["fsrc"].reshape(-1, 2), d["state"]
+        ebe, reorder, Re = int(d["header"][2]), ("SYMRCM" if int(d["header"][3]) else "NOT"), float(d["params"][0])
+    else:
+        x, y, ien = oracle.mesh_structured(m, m)
+        nn = len(x)
+        bnd = oracle.unit_square_types(x, y) == 0
+        type3 = np.ones((nn, 3), np.int32)
+        type3[bnd, :2] = 0
+        type3[np.flatnonzero(~bnd)[0], 2] = 0
+        type3 = type3.ravel()
+        presc = np.zeros((nn, 3))
+        presc[bnd & (np.abs(y - 1.0) <= 1e-15), 0] = 1.0
+        presc = presc.ravel()
+        fsrc = np.stack([np.sin(3 * x + y), np.cos(2 * y - x)], 1) * 0.1
+        state = np.random.default_rng(m).uniform(-1, 1, 3 * nn)
+        ebe = 0
+    invP, neq = oracle.number_unknowns(type3, 3)
+    lm = oracle.fill_lm(ien, type3, invP, neq, 3)
+    c = None
+    if not ebe:
+        c = oracle.csr_init(lm, 9, neq, reorder=reorder, permute_lm=True, type_=type3, invP=invP)
+        lm, invP = c["lm"], c["invP"]
+    u = np.zeros(neq)
+    unk = type3 != 0
+    u[invP[unk]] = state[unk]
+    U3 = oracle.ns_nodal_state(type3, invP, u, presc)
+    Ke, F = oracle.build_ns_supg_pspg(x, y, ien, lm, U3, fsrc[:, 0].copy(), fsrc[:, 1].copy(), Re, neq)
+    AA = oracle.csr_assemble(c["scheme"], Ke, c["nnz"], s=9) if c is not None else None
+    return dict(x=x, y=y, ien=ien, type3=type3, invP=invP, presc=presc, fx=fsrc[:, 0].copy(), fy=fsrc[:, 1].copy(),
+                neq=neq, lm=lm, c=c, u=u, Re=Re, Ke=Ke, F=F, AA=AA)
+
+
+@pytest.mark.parametrize("name", G.names("ns_") + ["m64"])
+def test_ns_assembly_bit_exact(dev, torch_cuda, oracle, name):
+    torch = torch_cuda
+    s = ns_case(oracle, m=64) if name == "m64" else ns_case(oracle, d=G.load(name))
+    if name != "m64":
+        d = G.load(name)
+        assert np.array_equal(s["F"], d["ref_F"])      # the oracle side, as pinned by the fixture
+    c = s["c"]
+    M = fem.NsMesh(dev, s["x"], s["y"], s["type3"], s["invP"], s["presc"], s["ien"], s["lm"], s["neq"],
+                   scheme=c["scheme"] if c is not None else None, nnz=c["nnz"] if c is not None else 0)
+    nel = len(s["ien"]) // 3
+    Ke = torch.empty(81 * nel, dtype=torch.float64, device="cuda")
+    F = torch.empty(s["neq"] + 1, dtype=torch.float64, device="cuda")
+    AA = torch.empty(c["nnz"] + 1, dtype=torch.float64, device="cuda") if c is not None else None
+    M.Build_K_F_SUPG_PSPG(s["Re"], T(torch, s["u"]), F, fx=T(torch, s["fx"]), fy=T(torch, s["fy"]), Ke=Ke,
+                          AA_ptr=AA.data_ptr() if AA is not None else None)
+    assert np.array_equal(H(Ke), s["Ke"])
+    assert np.array_equal(H(F), s["F"])
+    if AA is not None:
+        assert np.array_equal(H(AA), s["AA"])
