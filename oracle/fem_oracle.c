@@ -759,6 +759,241 @@ void orc_build_ns_supg_pspg(int nnodes, int nel, int NEQ, const double *x, const
     }
 }
 
+/* ====================================================================== Euler (NDOF = 4) */
+
+void orc_euler_nodal_state(int nnodes, const int *type4, const int *invP4, const double *u,
+                           const double *du, const double *presc4, double *U4, double *dU4)
+{
+    for (int k = 0; k < 4 * nnodes; k++) {
+        if (type4[k] > 0) { U4[k] = u[invP4[k]]; dU4[k] = du[invP4[k]]; }
+        else { U4[k] = presc4[k]; dU4[k] = 0.0; }
+    }
+}
+
+/* Flux Jacobians at the barycentric state (Ax_Ay_calculations.c:3-53 dimensionless, :55-99
+ * dimensional). Entries the reference leaves structurally zero are exact zeros here, including
+ * Ay[0][3], which the reference's hand-expanded products treat as zero. */
+static void euler_jacobians(const orc_euler_params *p, const double Ub[4], double Ax[4][4], double Ay[4][4])
+{
+    const double g = p->gamma;
+    const double v1 = Ub[1] / Ub[0], v2 = Ub[2] / Ub[0], E = Ub[3] / Ub[0];
+    memset(Ax, 0, sizeof(double) * 16);
+    memset(Ay, 0, sizeof(double) * 16);
+    Ax[0][1] = 1.0;
+    Ay[0][2] = 1.0;
+    Ax[2][0] = -v1 * v2;
+    Ax[2][1] = v2;
+    Ax[2][2] = v1;
+    Ax[1][1] = (3.0 - g) * v1;
+    Ax[1][2] = -(g - 1.0) * v2;
+    Ax[3][3] = g * v1;
+    Ay[1][1] = v2;
+    Ay[1][2] = v1;
+    Ay[2][1] = -(g - 1.0) * v1;
+    Ay[2][2] = (3.0 - g) * v2;
+    Ay[3][3] = g * v2;
+    if (p->dimensionless) {
+        const double nv = v1 * v1 + v2 * v2, eps2 = p->Mach * p->Mach;
+        Ax[1][0] = -(v1 * v1) + ((g - 1) * nv * 0.5);
+        Ax[1][3] = (g - 1.0) / eps2;
+        Ax[3][0] = -v1 * (g * E - ((g - 1.0) * eps2 * nv));
+        Ax[3][1] = g * E - (g - 1.0) * eps2 * (0.5 * nv + v1 * v1);
+        Ax[3][2] = -(g - 1.0) * eps2 * v1 * v2;
+        Ay[2][0] = -(v2 * v2) + (((g - 1.0) * nv * 0.5));
+        Ay[3][0] = -v2 * (g * E - (g - 1.0) * eps2 * nv);
+        Ay[3][2] = g * E - (g - 1.0) * eps2 * (0.5 * nv + v2 * v2);
+    } else {
+        const double rho = Ub[0], nU = (Ub[1] * Ub[1]) + (Ub[2] * Ub[2]);
+        Ax[1][0] = -(v1 * v1) + (((g - 1) * nU * 0.5) / (rho * rho));
+        Ax[1][3] = g - 1.0;
+        Ax[3][0] = (-g * v1 * E) + ((g - 1.0) * v1 * (nU / (rho * rho)));
+        Ax[3][1] = (g * E) - ((g - 1.0) * 0.5 * ((nU / (rho * rho)) + (2.0 * v1 * v1)));
+        Ax[3][2] = -((g - 1.0) * v1 * v2);
+        Ay[2][0] = -(v2 * v2) + (((g - 1.0) * nU * 0.5) / (rho * rho));
+        Ay[3][0] = (-g * v2 * E) + (((g - 1.0) * v2 * nU) / (rho * rho));
+        Ay[3][2] = (g * E) - ((g - 1.0) * 0.5 * ((nU / (rho * rho)) + (2.0 * v2 * v2)));
+    }
+    Ay[1][0] = Ax[2][0];
+    Ay[2][3] = Ax[1][3];
+    Ay[3][1] = Ax[3][2];
+}
+
+/* C = A B, each entry accumulated over k = 0..3 from the k = 0 term. The reference's hand-expanded
+ * products (Build_M_K_F_SUPG.c:163-240) sum the same terms in the same order without the terms
+ * with a structural zero factor (and with factors 1 written out), which leaves every value equal. */
+static void mul4(const double A[4][4], const double B[4][4], double Cm[4][4])
+{
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            double acc = A[i][0] * B[0][j];
+            for (int k = 1; k < 4; k++) acc += A[i][k] * B[k][j];
+            Cm[i][j] = acc;
+        }
+}
+
+/* w^T A0 w in the reference's grouping (Delta_CAU.c:39-73). */
+static double a0_form(const double A0[4][4], const double *w)
+{
+    return w[0] * (A0[0][0] * w[0] + 2.0 * A0[0][1] * w[1] + 2.0 * A0[0][2] * w[2] + 2.0 * A0[0][3] * w[3]) +
+           w[1] * (A0[1][1] * w[1] + 2.0 * A0[1][2] * w[2] + 2.0 * A0[1][3] * w[3]) +
+           w[2] * (A0[2][2] * w[2] + 2.0 * A0[2][3] * w[3]) + w[3] * A0[3][3] * w[3];
+}
+
+/* Delta_CAU (03_Euler_Equations/2D/01_CommonFiles/Delta_CAU.c:4-83). */
+static double euler_cau(double tolerance, const double gUx[4], const double gUy[4], const double Ax[4][4],
+                        const double Ay[4][4], const double A0[4][4], const double dUb[4], const double b[3],
+                        const double c[3], double twoArea)
+{
+    double AgUx[4], AgUy[4], gkx[4], gky[4], LUh[4];
+    AgUx[0] = gUx[1];
+    AgUx[1] = Ax[1][0] * gUx[0] + Ax[1][1] * gUx[1] + Ax[1][2] * gUx[2] + Ax[1][3] * gUx[3];
+    AgUx[2] = Ax[2][0] * gUx[0] + Ax[2][1] * gUx[1] + Ax[2][2] * gUx[2];
+    AgUx[3] = Ax[3][0] * gUx[0] + Ax[3][1] * gUx[1] + Ax[3][2] * gUx[2] + Ax[3][3] * gUx[3];
+    AgUy[0] = gUy[2];
+    AgUy[1] = Ay[1][0] * gUy[0] + Ay[1][1] * gUy[1] + Ay[1][2] * gUy[2];
+    AgUy[2] = Ay[2][0] * gUy[0] + Ay[2][1] * gUy[1] + Ay[2][2] * gUy[2] + Ay[2][3] * gUy[3];
+    AgUy[3] = Ay[3][0] * gUy[0] + Ay[3][1] * gUy[1] + Ay[3][2] * gUy[2] + Ay[3][3] * gUy[3];
+    for (int i = 0; i < 4; i++) {
+        gkx[i] = (b[0] * gUx[i] + c[0] * gUy[i]) / twoArea;
+        gky[i] = (b[1] * gUx[i] + c[1] * gUy[i]) / twoArea;
+    }
+    const double n_gk = sqrt(a0_form(A0, gkx) + a0_form(A0, gky));
+    for (int i = 0; i < 4; i++) LUh[i] = dUb[i] + AgUx[i] + AgUy[i];
+    const double n_LUh = sqrt(a0_form(A0, LUh));
+    const double n_gU = sqrt(a0_form(A0, gUx) + a0_form(A0, gUy));
+    return (n_gU >= tolerance) ? n_LUh / n_gk : 0.0;
+}
+
+void orc_build_euler_supg(int nnodes, int nel, int NEQ, const double *x, const double *y,
+                          const int *ien, const int *lm, const double *U4, const double *dU4,
+                          const orc_euler_params *p, double *Ae, double *R)
+{
+    const double third = 1.0 / 3.0, sixth = 1.0 / 6.0;
+    const double gamma = p->gamma, alpha = p->alpha, delta_t = p->dt;
+    (void)nnodes;
+    for (int I = 0; I <= NEQ; I++) R[I] = 0;
+    for (int e = 0; e < nel; e++) {
+        const int *v = ien + 3 * e;
+        const double X0 = x[v[0]], X1 = x[v[1]], X2 = x[v[2]];
+        const double Y0 = y[v[0]], Y1 = y[v[1]], Y2 = y[v[2]];
+        const double b[3] = {Y1 - Y2, Y2 - Y0, Y0 - Y1};   /* y23, y31, y12 */
+        const double c[3] = {X2 - X1, X0 - X2, X1 - X0};   /* x32, x13, x21 */
+        const double twoArea = fabs(c[2] * b[1] - c[1] * b[2]);
+        const double Area = 0.5 * twoArea;
+        double Ue[12], dUe[12], Ub[4], dUb[4], gUx[4], gUy[4];
+        for (int a = 0; a < 3; a++)
+            for (int k = 0; k < 4; k++) {
+                Ue[4 * a + k] = U4[4 * v[a] + k];
+                dUe[4 * a + k] = dU4[4 * v[a] + k];
+            }
+        for (int i = 0; i < 4; i++) {
+            Ub[i] = (Ue[i] + Ue[i + 4] + Ue[i + 8]) * third;
+            dUb[i] = (dUe[i] + dUe[i + 4] + dUe[i + 8]) * third;
+        }
+        for (int i = 0; i < 4; i++) {
+            gUx[i] = (Ue[i] * b[0] + Ue[i + 4] * b[1] + Ue[i + 8] * b[2]) / twoArea;
+            gUy[i] = (Ue[i] * c[0] + Ue[i + 4] * c[1] + Ue[i + 8] * c[2]) / twoArea;
+        }
+        const double v1 = Ub[1] / Ub[0], v2 = Ub[2] / Ub[0], E = Ub[3] / Ub[0];
+        const double nU = (Ub[1] * Ub[1]) + (Ub[2] * Ub[2]);
+        double Ax[4][4], Ay[4][4];
+        euler_jacobians(p, Ub, Ax, Ay);
+        /* inverse Riemannian metric A0^-1 in entropy variables (:134-160) */
+        const double rho_i = Ub[3] - ((nU * 0.5) / Ub[0]);
+        const double V2 = Ub[1] / rho_i, V3 = Ub[2] / rho_i, V4 = (-Ub[0]) / rho_i;
+        const double rhoi_e = -1.0 / (rho_i * V4);
+        const double k1 = (V2 * V2 + V3 * V3) * 0.5 / V4;
+        double A0[4][4];
+        A0[0][0] = rhoi_e * (k1 * k1 + gamma);
+        A0[0][1] = rhoi_e * (k1 * V2);
+        A0[0][2] = rhoi_e * (k1 * V3);
+        A0[0][3] = rhoi_e * ((k1 + 1.0) * V4);
+        A0[1][1] = rhoi_e * (V2 * V2 - V4);
+        A0[1][2] = rhoi_e * (V2 * V3);
+        A0[1][3] = rhoi_e * (V2 * V4);
+        A0[2][2] = rhoi_e * (V3 * V3 - V4);
+        A0[2][3] = rhoi_e * (V3 * V4);
+        A0[3][3] = rhoi_e * V4 * V4;
+        A0[1][0] = A0[0][1]; A0[2][0] = A0[0][2]; A0[2][1] = A0[1][2];
+        A0[3][0] = A0[0][3]; A0[3][1] = A0[1][3]; A0[3][2] = A0[2][3];
+        double Axx[4][4], Axy[4][4], Ayx[4][4], Ayy[4][4];
+        mul4(Ax, Ax, Axx);
+        mul4(Ax, Ay, Axy);
+        mul4(Ay, Ax, Ayx);
+        mul4(Ay, Ay, Ayy);
+        const double delta = p->shock ? euler_cau(p->tolerance, gUx, gUy, Ax, Ay, A0, dUb, b, c, twoArea) : 0.0;
+        /* SUPG parameter (:246-271) */
+        double betax = 2.0 * (gUx[0] * Ub[0] + gUx[1] * Ub[1] + gUx[2] * Ub[2] + gUx[3] * Ub[3]);
+        double betay = 2.0 * (gUy[0] * Ub[0] + gUy[1] * Ub[1] + gUy[2] * Ub[2] + gUy[3] * Ub[3]);
+        const double betaxy = sqrt(betax * betax + betay * betay);
+        if (betaxy != 0.0) {
+            betax = betax / betaxy;
+            betay = betay / betaxy;
+        }
+        const double h = sqrt(twoArea);
+        const double abs_vbeta = fabs(v1 * betax + v2 * betay);
+        const double cs = sqrt(gamma * (gamma - 1.0) * (E - 0.5 * (v1 * v1 + v2 * v2)));
+        const double CFL = ((cs + abs_vbeta) * delta_t) / h;
+        const double psi = (2.0 * alpha * CFL) * (1.0 / (1.0 + 2.0 * alpha * CFL));
+        const double tau_d = delta / ((cs + abs_vbeta) * (cs + abs_vbeta));
+        const double tau_a = h / (2.0 * (cs + abs_vbeta));
+        const double tau_t = (2.0 / (3.0 * (1.0 + 2.0 * alpha * CFL))) * tau_a;
+        double tau = tau_t + psi * (tau_a - tau_d);
+        if (tau < 0.0) tau = 0.0;
+        /* per-vertex blocks: C_a = (b_a Ax + c_a Ay)/6, its SUPG products, mass/convection (:276-720) */
+        double Cv[3][4][4], AX[3][4][4], AY[3][4][4];
+        for (int a = 0; a < 3; a++)
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                    Cv[a][i][j] = (b[a] * Ax[i][j] + c[a] * Ay[i][j]) * sixth;
+                    AX[a][i][j] = b[a] * Axx[i][j] + c[a] * Ayx[i][j];
+                    AY[a][i][j] = b[a] * Axy[i][j] + c[a] * Ayy[i][j];
+                }
+        const double Mg1 = Area * sixth, Mg2 = Area / 12.0;
+        const double tau4area = tau / (4 * Area), delta4area = delta / (4.0 * Area);
+        double Kcau[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int bb = a + 1; bb < 3; bb++)
+                Kcau[a][bb] = Kcau[bb][a] = delta4area * (b[a] * b[bb] + c[a] * c[bb]);
+        double Me[12][12], Ke[12][12];
+        for (int a = 0; a < 3; a++) {
+            const int o1 = a == 0 ? 1 : 0, o2 = a == 2 ? 1 : 2;   /* the other vertices, ascending */
+            for (int bb = 0; bb < 3; bb++)
+                for (int i = 0; i < 4; i++)
+                    for (int j = 0; j < 4; j++) {
+                        const double Mpg = Cv[a][i][j] * tau;
+                        double m, k;
+                        if (i == j) m = (a == bb ? Mg1 : Mg2) + Mpg;
+                        else m = Mpg;
+                        if (a == bb) {
+                            const double k1v = tau4area * (b[o1] * AX[a][i][j] + c[o1] * AY[a][i][j]);
+                            const double k2v = tau4area * (b[o2] * AX[a][i][j] + c[o2] * AY[a][i][j]);
+                            k = Cv[a][i][j] - (k1v + k2v);
+                            if (i == j) k = k - (Kcau[a][o1] + Kcau[a][o2]);
+                        } else {
+                            const double kp = tau4area * (b[bb] * AX[a][i][j] + c[bb] * AY[a][i][j]);
+                            k = Cv[bb][i][j] + kp;
+                            if (i == j) k = k + Kcau[a][bb];
+                        }
+                        Me[4 * a + i][4 * bb + j] = m;
+                        Ke[4 * a + i][4 * bb + j] = k;
+                    }
+        }
+        /* Ae and the residual (:1025-1047) */
+        double *A = Ae + (size_t)144 * e;
+        for (int i = 0; i < 12; i++) {
+            double MdU = 0, KU = 0;
+            for (int j = 0; j < 12; j++) {
+                MdU += Me[i][j] * dUe[j];
+                KU += Ke[i][j] * Ue[j];
+                A[12 * i + j] = Me[i][j] + alpha * delta_t * Ke[i][j];
+            }
+            R[lm[12 * e + i]] += -MdU - KU;
+        }
+        R[NEQ] = 0;
+    }
+}
+
 void orc_csr_assemble(int nel, int s, int nnz, const int *scheme, const double *Ke, double *AA)
 {
     const size_t ss = (size_t)s * s;

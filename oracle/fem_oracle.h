@@ -116,6 +116,29 @@ void orc_build_ns_supg_pspg(int nnodes, int nel, int NEQ, const double *x, const
                             const int *ien, const int *lm, const double *U3, const double *fx,
                             const double *fy, double Re, double *Ke, double *F);
 
+/* ------------------------------------------------------------------ Euler (NDOF = 4) */
+
+typedef struct {
+    double gamma;          /* FemFunctions->gamma (constant over the domain) */
+    double Mach;           /* reference Mach number (dimensionless Jacobians only) */
+    double alpha, dt;      /* Alpha_Build, DeltaT_Build */
+    double tolerance;      /* AuxBuild->tolerance of the shock-capture switch */
+    int dimensionless;     /* 1: dimensionless_Ax_Ay_calculations, 0: dimensional (Ax_Ay_calculations.c) */
+    int shock;             /* 1: Delta_CAU (Delta_CAU.c:4-83), 0: delta = 0 */
+} orc_euler_params;
+
+/* eval_U_dU (03_Euler_Equations/2D/01_CommonFiles/eval_U_dU.c:3-55): nodal (rho, rho u, rho v,
+ * rho E) and their increments; prescribed dofs take presc4 and a zero increment. */
+void orc_euler_nodal_state(int nnodes, const int *type4, const int *invP4, const double *u,
+                           const double *du, const double *presc4, double *U4, double *dU4);
+
+/* Build_M_K_F_SUPG (03_Euler_Equations/2D/01_CommonFiles/Build_M_K_F_SUPG.c:6-1057): per element
+ * Ae = Me + alpha*dt*Ke (12x12, dofs rho, rho u, rho v, rho E per vertex) into Ae[nel*144] and
+ * R[lm] += -(Me dUe + Ke Ue) in element order, R[NEQ] = 0. */
+void orc_build_euler_supg(int nnodes, int nel, int NEQ, const double *x, const double *y,
+                          const int *ien, const int *lm, const double *U4, const double *dU4,
+                          const orc_euler_params *p, double *Ae, double *R);
+
 /* csr_assembly (csr_assembly.c:3-54, one rank): AA[nnz+1] zeroed then AA[scheme] += Ke per
  * element in element order; AA[nnz] = 0. */
 void orc_csr_assemble(int nel, int s, int nnz, const int *scheme, const double *Ke, double *AA);

@@ -81,6 +81,9 @@ def lib():
                                                     C.POINTER(TransportProblem), C.c_double,
                                                     C.c_double, f64p, f64p]
         L.orc_csr_assemble.argtypes = [C.c_int, C.c_int, C.c_int, i32p, f64p, f64p]
+        L.orc_euler_nodal_state.argtypes = [C.c_int, i32p, i32p, f64p, f64p, f64p, f64p, f64p]
+        L.orc_build_euler_supg.argtypes = [C.c_int, C.c_int, C.c_int, f64p, f64p, i32p, i32p, f64p, f64p,
+                                           C.POINTER(EulerParams), f64p, f64p]
         L.orc_ns_nodal_state.argtypes = [C.c_int, i32p, i32p, f64p, f64p, f64p]
         L.orc_build_ns_supg_pspg.argtypes = [C.c_int, C.c_int, C.c_int, f64p, f64p, i32p, i32p, f64p,
                                              f64p, f64p, C.c_double, f64p, f64p]
@@ -190,6 +193,36 @@ def build_transport_transient(x, y, type_, invP, ien, lm, ubc, u, du, pb, alpha,
     lib().orc_build_transport_transient(len(x), nel, NEQ, _f(x), _f(y), _i(type_), _i(invP),
                                         _i(ien), _i(lm), _f(ubc), _f(u), _f(du), C.byref(pb),
                                         alpha, dt, Ae, R)
+    return Ae, R
+
+
+class EulerParams(C.Structure):
+    """orc_euler_params."""
+    _fields_ = [("gamma", C.c_double), ("Mach", C.c_double), ("alpha", C.c_double), ("dt", C.c_double),
+                ("tolerance", C.c_double), ("dimensionless", C.c_int), ("shock", C.c_int)]
+
+
+def euler_params(gamma=1.4, Mach=2.0, alpha=0.5, dt=1e-2, tolerance=1e-8, dimensionless=1, shock=1):
+    p = EulerParams()
+    p.gamma, p.Mach, p.alpha, p.dt, p.tolerance = gamma, Mach, alpha, dt, tolerance
+    p.dimensionless, p.shock = dimensionless, shock
+    return p
+
+
+def euler_nodal_state(type4, invP4, u, du, presc4):
+    """eval_U_dU: nodal conservative state and increment."""
+    U4, dU4 = np.empty(len(type4)), np.empty(len(type4))
+    lib().orc_euler_nodal_state(len(type4) // 4, _i(type4), _i(invP4), _f(np.append(u, 0.0)),
+                                _f(np.append(du, 0.0)), _f(presc4), U4, dU4)
+    return U4, dU4
+
+
+def build_euler_supg(x, y, ien, lm, U4, dU4, params, NEQ):
+    """Build_M_K_F_SUPG: Ae[nel*144], R[NEQ+1]."""
+    nel = len(ien) // 3
+    Ae, R = np.empty(144 * nel), np.empty(NEQ + 1)
+    lib().orc_build_euler_supg(len(x), nel, NEQ, _f(x), _f(y), _i(ien), _i(lm), _f(U4), _f(dU4), C.byref(params),
+                               Ae, R)
     return Ae, R
 
 

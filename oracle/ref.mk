@@ -32,7 +32,11 @@ NS_SRC  := $(NS)/Build_K_F_SUPG_PSPG.c $(NS)/eval_U.c $(NS)/setzeros.c $(NS)/ebe
 NS_CF   := $(CF)/Allocation_Operations/mycalloc.c $(CF)/BLAS_Operations/dzero.c \
            $(CF)/BLAS_Operations/ddot.c $(CF)/Reordering/reordering.c $(CF)/Reordering/symrcm.c
 
-all: $(OUT)/st_harness $(OUT)/ns_harness $(OUT)/meshGenerator
+EU      := $(REF)/03_Euler_Equations/2D/01_CommonFiles
+EU_SRC  := $(EU)/Build_M_K_F_SUPG.c $(EU)/eval_U_dU.c $(EU)/Ax_Ay_calculations.c $(EU)/Delta_CAU.c \
+           $(EU)/setzeros.c $(EU)/ebe_assembly.c $(EU)/csr_assembly.c $(EU)/Fill_LM.c $(EU)/csr_Initialization.c
+
+all: $(OUT)/st_harness $(OUT)/ns_harness $(OUT)/eu_harness $(OUT)/meshGenerator
 
 # The structured mesh generator is a self-contained C program.
 $(OUT)/meshGenerator: $(REF)/Mesh_Generator/Structured_Mesh/meshGenerator.c
@@ -51,6 +55,12 @@ $(OUT)/ns_harness: $(HERE)ref_harness_ns.c $(NS_SRC) $(NS_CF)
 	mkdir -p $(OUT)
 	gcc -O2 -fno-fast-math -ffp-contract=off -w -DSSNavierStokesEquations2D -I$(NS) -I$(MPI)/include \
 	    -o $@ $(HERE)ref_harness_ns.c $(NS_SRC) $(NS_CF) \
+	    -L$(MPI)/lib -Wl,-rpath,$(MPI)/lib -lmpi -lm -Wl,--unresolved-symbols=ignore-in-object-files
+
+$(OUT)/eu_harness: $(HERE)ref_harness_eu.c $(EU_SRC) $(NS_CF)
+	mkdir -p $(OUT)
+	gcc -O2 -fno-fast-math -ffp-contract=off -w -DEulerEquations2D -I$(EU) -I$(MPI)/include \
+	    -o $@ $(HERE)ref_harness_eu.c $(EU_SRC) $(NS_CF) \
 	    -L$(MPI)/lib -Wl,-rpath,$(MPI)/lib -lmpi -lm -Wl,--unresolved-symbols=ignore-in-object-files
 
 clean:

@@ -82,6 +82,13 @@ NS_CASES = [
 ]
 
 
+EU_CASES = [
+    dict(name="eu_ebe_dimless_cau_r8", M=8, N=8, cut="right", ebe=1, reorder=0, dimless=1, shock=1),
+    dict(name="eu_csr_rcm_dim_cau_l8", M=8, N=8, cut="left", ebe=0, reorder=1, dimless=0, shock=1),
+    dict(name="eu_csr_not_dimless_none_r6x5", M=6, N=5, cut="right", ebe=0, reorder=0, dimless=1, shock=0),
+]
+
+
 def read_records(path):
     out = {}
     with open(path, "rb") as f:
@@ -153,6 +160,41 @@ def run_ns_case(c, workdir):
     return data
 
 
+def run_eu_case(c, workdir):
+    """Euler assembly on a supersonic state: conservative variables of rho ~ 1, velocity ~ (2, 0.2)
+    and p = 1/1.4 perturbed per node, all four dofs prescribed on the inflow boundary x = 0."""
+    x, y, t1, ien = run_mesh_generator(c["M"], c["N"], c["cut"], workdir)
+    nn = len(x)
+    gamma = 1.4
+    rng = np.random.default_rng(zlib.crc32(c["name"].encode()))
+    rho = 1.0 + 0.2 * rng.uniform(-1, 1, nn)
+    vx = 2.0 + 0.3 * rng.uniform(-1, 1, nn)
+    vy = 0.2 + 0.3 * rng.uniform(-1, 1, nn)
+    pres = (1.0 / gamma) * (1.0 + 0.3 * rng.uniform(-1, 1, nn))
+    Et = pres / ((gamma - 1.0) * rho) + 0.5 * (vx * vx + vy * vy)
+    state = np.stack([rho, rho * vx, rho * vy, rho * Et], 1)
+    incr = 1e-2 * rng.standard_normal((nn, 4))
+    inflow = np.abs(x) <= 1e-15
+    type4 = np.ones((nn, 4), np.int32)
+    type4[inflow] = 0
+    presc = np.where(inflow[:, None], state, 0.0)
+    hd = np.zeros(16, np.int32)
+    hd[:6] = [nn, len(ien) // 3, c["ebe"], c["reorder"], c["dimless"], c["shock"]]
+    pr = np.zeros(16)
+    pr[:5] = [gamma, 2.0, 0.5, 1e-2, 1e-8]
+    inp, outp = os.path.join(workdir, "in.bin"), os.path.join(workdir, "out.bin")
+    with open(inp, "wb") as f:
+        for a in (hd, pr, x, y, type4, ien, presc, state, incr):
+            f.write(np.ascontiguousarray(a).tobytes())
+    subprocess.run([MPIEXEC, "-n", "1", os.path.join(REFBIN, "eu_harness"), inp, outp], check=True,
+                   stdout=subprocess.DEVNULL, timeout=600)
+    rec = read_records(outp)
+    data = dict(x=x, y=y, type4=type4.ravel(), ien=ien, presc=presc.ravel(), state=state.ravel(),
+                incr=incr.ravel(), header=hd, params=pr)
+    data.update({"ref_" + k: v for k, v in rec.items()})
+    return data
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -171,6 +213,14 @@ def main():
             meta["iters"] = d["ref_iters"].tolist()
             manifest[c["name"]] = meta
             print(c["name"], "iters", meta["iters"], "bytes", os.path.getsize(fn))
+        for c in EU_CASES:
+            d = run_eu_case(c, work)
+            fn = os.path.join(HERE, c["name"] + ".npz")
+            np.savez_compressed(fn, **d)
+            meta = {k: v for k, v in c.items() if k != "name"}
+            meta["sha256"] = hashlib.sha256(open(fn, "rb").read()).hexdigest()
+            manifest[c["name"]] = meta
+            print(c["name"], "bytes", os.path.getsize(fn))
         for c in NS_CASES:
             d = run_ns_case(c, work)
             fn = os.path.join(HERE, c["name"] + ".npz")

@@ -192,3 +192,49 @@ def test_ns_structure_and_assembly_bit_exact(oracle, name):
         assert np.array_equal(Ke, d["ref_A"])
     else:
         assert np.array_equal(oracle.csr_assemble(c["scheme"], Ke, c["nnz"], s=9), d["ref_AA"])
+
+
+# ----------------------------------------------------------------------------- Euler (NDOF = 4)
+
+EU_CASES = G.names("eu_")
+
+
+def eu_setup(oracle, d):
+    """Numbering, lm, (CSR) structure, state and increment of an Euler fixture, as the reference
+    builds them (Fill_LM with NDOF = 4; csr_Initialization with s = 12 renumbering lm and invP_id)."""
+    h = d["header"]
+    ebe, reorder = int(h[2]), int(h[3])
+    type4 = d["type4"]
+    invP, neq = oracle.number_unknowns(type4, 4)
+    lm = oracle.fill_lm(d["ien"], type4, invP, neq, 4)
+    assert np.array_equal(lm, d["ref_lm"])
+    c = None
+    if not ebe:
+        c = oracle.csr_init(lm, 12, neq, reorder="SYMRCM" if reorder else "NOT", permute_lm=True,
+                            type_=type4, invP=invP)
+        lm, invP = c["lm"], c["invP"]
+    u, du = np.zeros(neq), np.zeros(neq)
+    unk = type4 != 0
+    u[invP[unk]] = d["state"][unk]
+    du[invP[unk]] = d["incr"][unk]
+    pr = d["params"]
+    params = oracle.euler_params(gamma=pr[0], Mach=pr[1], alpha=pr[2], dt=pr[3], tolerance=pr[4],
+                                 dimensionless=int(h[4]), shock=int(h[5]))
+    return ebe, neq, lm, invP, u, du, c, params
+
+
+@pytest.mark.parametrize("name", EU_CASES)
+def test_euler_structure_and_assembly_bit_exact(oracle, name):
+    d = G.load(name)
+    ebe, neq, lm, invP, u, du, c, params = eu_setup(oracle, d)
+    assert np.array_equal(invP, d["ref_invP_id"])
+    if c is not None:
+        for k in ("IA", "JA", "perm", "scheme"):
+            assert np.array_equal(c[k], d["ref_" + k]), k
+    U4, dU4 = oracle.euler_nodal_state(d["type4"], invP, u, du, d["presc"])
+    Ae, R = oracle.build_euler_supg(d["x"], d["y"], d["ien"], lm, U4, dU4, params, neq)
+    assert np.array_equal(R, d["ref_R"])
+    if ebe:
+        assert np.array_equal(Ae, d["ref_A"])
+    else:
+        assert np.array_equal(oracle.csr_assemble(c["scheme"], Ae, c["nnz"], s=12), d["ref_AA"])
