@@ -284,6 +284,32 @@ int femgpu_ns_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, const d
 int femgpu_assemble_ns(femgpu_mesh *M, double Re, const double *d_u, const double *d_fx,
                        const double *d_fy, double *d_Ke, double *d_F, double *d_AA);
 
+/* =========================================================== Euler assembly (NDOF = 4) */
+
+typedef struct {
+    double gamma;       /* FemFunctions->gamma (constant over the domain) */
+    double Mach;        /* reference Mach number (dimensionless Jacobians) */
+    double alpha, dt;   /* Alpha_Build, DeltaT_Build */
+    double tolerance;   /* AuxBuild->tolerance: CAU acts where |grad U|_A0 >= tolerance */
+    int dimensionless;  /* 1: dimensionless_Ax_Ay_calculations, 0: dimensional_Ax_Ay_calculations */
+    int shock;          /* 1: Delta_CAU, 0: no shock capture */
+} femgpu_euler_problem;
+
+/* Device-resident Euler mesh: as femgpu_ns_mesh_create with 4 dofs per node (rho, rho u, rho v,
+ * rho E; presc4 = rhopresc, v1presc, v2presc, epresc at the node), lm[nel*12] and
+ * scheme[nel*144] (femgpu_csr_init with s = 12). */
+int femgpu_euler_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, const double *x,
+                             const double *y, const int *type4, const int *invP4,
+                             const double *presc4, const int *ien, const int *lm,
+                             const int *scheme, int nnz, femgpu_mesh **out);
+
+/* Build_M_K_F_SUPG (03_Euler_Equations/2D/01_CommonFiles/Build_M_K_F_SUPG.c:6-1057 with
+ * eval_U_dU.c, Ax_Ay_calculations.c and Delta_CAU.c): Ae = Me + alpha*dt*Ke (12x12) into
+ * d_Ae[nel*144] (nullable), R[lm] += -(Me dU + Ke U) into d_R[NEQ+1] (R[NEQ] = 0), CSR values into
+ * d_AA[nnz+1] (nullable). d_u, d_du: current state and increment (neq). */
+int femgpu_assemble_euler(femgpu_mesh *M, const femgpu_euler_problem *pb, const double *d_u,
+                          const double *d_du, double *d_Ae, double *d_R, double *d_AA);
+
 #ifdef __cplusplus
 }
 #endif

@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_elem_ns(int nel, NsIn in, double *__res
 }
 
 // F[I] = sum of the residual entries of unknown I in element order (F[lm] += Res, :655-658);
-// F[NEQ] = 0. src entries are e*9+i.
+// F[NEQ] = 0. src entries index the per-element residual array (e*s+i).
 __global__ void k_rhs_gather(int NEQ, const int *__restrict__ ptr, const int *__restrict__ src,
                              const double *__restrict__ Res, double *__restrict__ F)
 {
@@ -135,7 +135,7 @@ __global__ void k_rhs_gather(int NEQ, const int *__restrict__ ptr, const int *__
     F[I] = f;
 }
 
-__global__ void k_aa_gather81(int nnz, const int *__restrict__ ptr, const int *__restrict__ src,
+__global__ void k_aa_gather_blocks(int nnz, const int *__restrict__ ptr, const int *__restrict__ src,
                               const double *__restrict__ Ke, double *__restrict__ AA)
 {
 #pragma clang fp contract(off)
@@ -151,23 +151,28 @@ __global__ void k_aa_gather81(int nnz, const int *__restrict__ ptr, const int *_
 
 using namespace fg;
 
-extern "C" int femgpu_ns_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, const double *x, const double *y,
-                                     const int *type3, const int *invP3, const double *presc3, const int *ien,
-                                     const int *lm, const int *scheme, int nnz, femgpu_mesh **out)
+namespace fg {
+
+// Device mesh for element blocks of s = 3*ndof dofs (ndof = 3: Navier-Stokes, 4: Euler): per-dof
+// Type/invP_id/prescribed values, lm[nel*s], the residual gather map (unknown -> e*s+i in element
+// order) and, with a slot table scheme[nel*s*s], the CSR value gather map (slot -> e*s*s+q).
+int block_mesh_create(femgpu_ctx *ctx, int ndof, int nnodes, int nel, int neq, const double *x, const double *y,
+                      const int *type, const int *invP, const double *presc, const int *ien, const int *lm,
+                      const int *scheme, int nnz, femgpu_mesh **out)
 {
-    FG_CHECK(ctx && nnodes > 0 && nel >= 0 && neq >= 0 && x && y && type3 && invP3 && presc3 && ien && lm && out,
-             FEMGPU_ERR_ARG, "femgpu_ns_mesh_create: bad arguments");
+    const int sb = 3 * ndof, ss = sb * sb;
+    FG_CHECK(ctx && nnodes > 0 && nel >= 0 && neq >= 0 && x && y && type && invP && presc && ien && lm && out,
+             FEMGPU_ERR_ARG, "mesh create: bad arguments");
     for (size_t q = 0; q < static_cast<size_t>(nel) * 3; ++q)
-        FG_CHECK(ien[q] >= 0 && ien[q] < nnodes, FEMGPU_ERR_ARG, "femgpu_ns_mesh_create: vertex %d out of range",
-                 ien[q]);
-    for (size_t k = 0; k < static_cast<size_t>(nnodes) * 3; ++k)
-        FG_CHECK(type3[k] == 0 || (invP3[k] >= 0 && invP3[k] < neq), FEMGPU_ERR_ARG,
-                 "femgpu_ns_mesh_create: dof %zu is an unknown with invP_id %d out of range", k, invP3[k]);
-    for (size_t q = 0; q < static_cast<size_t>(nel) * 9; ++q)
-        FG_CHECK(lm[q] >= 0 && lm[q] <= neq, FEMGPU_ERR_ARG, "femgpu_ns_mesh_create: lm entry %d out of range", lm[q]);
+        FG_CHECK(ien[q] >= 0 && ien[q] < nnodes, FEMGPU_ERR_ARG, "mesh create: vertex %d out of range", ien[q]);
+    for (size_t k = 0; k < static_cast<size_t>(nnodes) * ndof; ++k)
+        FG_CHECK(type[k] == 0 || (invP[k] >= 0 && invP[k] < neq), FEMGPU_ERR_ARG,
+                 "mesh create: dof %zu is an unknown with invP_id %d out of range", k, invP[k]);
+    for (size_t q = 0; q < static_cast<size_t>(nel) * sb; ++q)
+        FG_CHECK(lm[q] >= 0 && lm[q] <= neq, FEMGPU_ERR_ARG, "mesh create: lm entry %d out of range", lm[q]);
     auto *M = new femgpu_mesh();
     M->ctx = ctx;
-    M->ndof = 3;
+    M->ndof = ndof;
     M->nnodes = nnodes;
     M->nel = nel;
     M->neq = neq;
@@ -177,40 +182,43 @@ extern "C" int femgpu_ns_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int n
     auto up = [&](auto &buf, auto *h, size_t n) {
         if (rc == FEMGPU_OK) rc = buf.upload(h, n, s);
     };
-    const size_t n3 = static_cast<size_t>(nnodes) * 3;
+    const size_t nd = static_cast<size_t>(nnodes) * ndof;
     up(M->x, x, nnodes);
     up(M->y, y, nnodes);
-    up(M->ubc, presc3, n3);
-    up(M->type, type3, n3);
-    up(M->invP, invP3, n3);
+    up(M->ubc, presc, nd);
+    up(M->type, type, nd);
+    up(M->invP, invP, nd);
     up(M->ien, ien, static_cast<size_t>(nel) * 3);
-    up(M->lm, lm, static_cast<size_t>(nel) * 9);
+    up(M->lm, lm, static_cast<size_t>(nel) * sb);
     std::vector<std::pair<int, int>> pairs;
-    pairs.reserve(static_cast<size_t>(nel) * 9);
+    pairs.reserve(static_cast<size_t>(nel) * sb);
     for (int e = 0; e < nel; ++e)
-        for (int i = 0; i < 9; ++i)
-            if (lm[9 * static_cast<size_t>(e) + i] < neq) pairs.emplace_back(lm[9 * static_cast<size_t>(e) + i], 9 * e + i);
+        for (int i = 0; i < sb; ++i) {
+            const int I = lm[static_cast<size_t>(sb) * e + i];
+            if (I < neq) pairs.emplace_back(I, sb * e + i);
+        }
     std::vector<int> ptr, src;
     csr_map(neq, pairs, ptr, src);
     up(M->rhs_ptr, ptr.data(), ptr.size());
     up(M->rhs_src, src.data(), src.size());
     if (rc == FEMGPU_OK && scheme) {
+        FG_CHECK(static_cast<long long>(nel) * ss < (1LL << 31), FEMGPU_ERR_ARG, "mesh create: too many element entries");
         pairs.clear();
         for (int e = 0; e < nel; ++e)
-            for (int q = 0; q < 81; ++q) {
-                const int slot = scheme[81 * static_cast<size_t>(e) + q];
+            for (int q = 0; q < ss; ++q) {
+                const int slot = scheme[static_cast<size_t>(ss) * e + q];
                 if (slot < 0 || slot > nnz) rc = FEMGPU_ERR_ARG;
-                else if (slot < nnz) pairs.emplace_back(slot, 81 * e + q);
+                else if (slot < nnz) pairs.emplace_back(slot, ss * e + q);
             }
-        if (rc == FEMGPU_ERR_ARG) set_error("femgpu_ns_mesh_create: scheme slot out of range");
+        if (rc == FEMGPU_ERR_ARG) set_error("mesh create: scheme slot out of range");
         if (rc == FEMGPU_OK) {
             csr_map(nnz, pairs, ptr, src);
             up(M->aa_ptr, ptr.data(), ptr.size());
             up(M->aa_src, src.data(), src.size());
         }
     }
-    if (rc == FEMGPU_OK) rc = M->fe.alloc(static_cast<size_t>(nel) * 9);
-    if (rc == FEMGPU_OK) rc = M->ke.alloc(static_cast<size_t>(nel) * 81);
+    if (rc == FEMGPU_OK) rc = M->fe.alloc(static_cast<size_t>(nel) * sb);
+    if (rc == FEMGPU_OK) rc = M->ke.alloc(static_cast<size_t>(nel) * ss);
     if (rc == FEMGPU_OK && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (rc != FEMGPU_OK) {
         delete M;
@@ -218,6 +226,27 @@ extern "C" int femgpu_ns_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int n
     }
     *out = M;
     return FEMGPU_OK;
+}
+
+int launch_block_gathers(femgpu_mesh *M, const double *Ke, double *F, double *AA)
+{
+    femgpu_ctx *ctx = M->ctx;
+    k_rhs_gather<<<(M->neq + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->neq, M->rhs_ptr.p, M->rhs_src.p, M->fe.p, F);
+    FG_HIP(hipGetLastError());
+    if (AA) {
+        k_aa_gather_blocks<<<(M->nnz + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->nnz, M->aa_ptr.p, M->aa_src.p, Ke, AA);
+        FG_HIP(hipGetLastError());
+    }
+    return FEMGPU_OK;
+}
+
+}  // namespace fg
+
+extern "C" int femgpu_ns_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, const double *x, const double *y,
+                                     const int *type3, const int *invP3, const double *presc3, const int *ien,
+                                     const int *lm, const int *scheme, int nnz, femgpu_mesh **out)
+{
+    return block_mesh_create(ctx, 3, nnodes, nel, neq, x, y, type3, invP3, presc3, ien, lm, scheme, nnz, out);
 }
 
 extern "C" int femgpu_assemble_ns(femgpu_mesh *M, double Re, const double *d_u, const double *d_fx,
@@ -243,11 +272,5 @@ extern "C" int femgpu_assemble_ns(femgpu_mesh *M, double Re, const double *d_u, 
         k_elem_ns<<<(M->nel + 255) / 256, 256, 0, ctx->stream>>>(M->nel, in, Ke, M->fe.p);
         FG_HIP(hipGetLastError());
     }
-    k_rhs_gather<<<(M->neq + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->neq, M->rhs_ptr.p, M->rhs_src.p, M->fe.p, d_F);
-    FG_HIP(hipGetLastError());
-    if (d_AA) {
-        k_aa_gather81<<<(M->nnz + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->nnz, M->aa_ptr.p, M->aa_src.p, Ke, d_AA);
-        FG_HIP(hipGetLastError());
-    }
-    return FEMGPU_OK;
+    return launch_block_gathers(M, Ke, d_F, d_AA);
 }

@@ -216,6 +216,11 @@ int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *
 int launch_ebe_gather_values(femgpu_ctx *ctx, femgpu_ebe *E, const double *A_elem_order);
 
 int dot64(femgpu_ctx *ctx, int n, const double *x, const double *y, double *d_out);
+// Element-block meshes (assembly_ns.hip): creation and the residual / CSR value gathers.
+int block_mesh_create(femgpu_ctx *ctx, int ndof, int nnodes, int nel, int neq, const double *x, const double *y,
+                      const int *type, const int *invP, const double *presc, const int *ien, const int *lm,
+                      const int *scheme, int nnz, femgpu_mesh **out);
+int launch_block_gathers(femgpu_mesh *M, const double *Ke, double *F, double *AA);
 // Counting sort of (key, source) pairs by key, keeping source order inside a key (assembly maps).
 void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vector<int> &ptr, std::vector<int> &src);
 int check_device_error(femgpu_ctx *ctx);

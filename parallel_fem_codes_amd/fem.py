@@ -359,6 +359,31 @@ class NsMesh:
                                            C.c_void_p(AA_ptr) if AA_ptr else None))
 
 
+def make_euler_problem(gamma=1.4, Mach=2.0, alpha=0.5, dt=1e-2, tolerance=1e-8, dimensionless=1, shock=1):
+    p = N.EulerProblem()
+    p.gamma, p.Mach, p.alpha, p.dt, p.tolerance = gamma, Mach, alpha, dt, tolerance
+    p.dimensionless, p.shock = dimensionless, shock
+    return p
+
+
+class EulerMesh(NsMesh):
+    """Device-resident Euler mesh (rho, rho u, rho v, rho E per node) for Build_M_K_F_SUPG."""
+
+    def __init__(self, dev: Device, x, y, type4, invP4, presc4, ien, lm, neq, scheme=None, nnz=0):
+        self.dev, self.neq, self.nel = dev, neq, len(ien) // 3
+        h = C.c_void_p()
+        sc = _i32(scheme) if scheme is not None else None
+        N.check(N.lib().femgpu_euler_mesh_create(
+            dev.h, len(x), self.nel, neq, _f64(x), _f64(y), _i32(type4), _i32(invP4), _f64(presc4), _i32(ien),
+            _i32(lm), sc.ctypes.data_as(C.c_void_p) if sc is not None else None, int(nnz), C.byref(h)))
+        self.h = h
+
+    def Build_M_K_F_SUPG(self, pb, u, du, R, Ae=None, AA_ptr=None):
+        """Ae = Me + alpha*dt*Ke and R = -(Me dU + Ke U); R has NEQ+1 entries (R[NEQ] = 0)."""
+        N.check(N.lib().femgpu_assemble_euler(self.h, C.byref(pb), _ptr(u), _ptr(du), _ptr(Ae), _ptr(R),
+                                              C.c_void_p(AA_ptr) if AA_ptr else None))
+
+
 def _system(op, ilu):
     S = N.System()
     if isinstance(op, CsrMatrix):

@@ -43,6 +43,12 @@ class SolveInfo(C.Structure):
                 ("true_rel_residual", C.c_double), ("prec_rel_residual", C.c_double)]
 
 
+class EulerProblem(C.Structure):
+    """femgpu_euler_problem."""
+    _fields_ = [("gamma", C.c_double), ("Mach", C.c_double), ("alpha", C.c_double), ("dt", C.c_double),
+                ("tolerance", C.c_double), ("dimensionless", C.c_int), ("shock", C.c_int)]
+
+
 class SweepInfo(C.Structure):
     """femgpu_sweep_info."""
     _fields_ = [(f, C.c_int) for f in ("kernel", "depth_lower", "depth_upper", "steps_lower", "steps_upper",
@@ -115,6 +121,9 @@ _SIGS = {
     "femgpu_ns_mesh_create": (I, [vp, I, I, I, f64p, f64p, i32p, i32p, f64p, i32p, i32p, vp, I,
                                   C.POINTER(vp)]),
     "femgpu_assemble_ns": (I, [vp, D, vp, vp, vp, vp, vp, vp]),
+    "femgpu_euler_mesh_create": (I, [vp, I, I, I, f64p, f64p, i32p, i32p, f64p, i32p, i32p, vp, I,
+                                     C.POINTER(vp)]),
+    "femgpu_assemble_euler": (I, [vp, C.POINTER(EulerProblem), vp, vp, vp, vp, vp]),
     "femgpu_assemble_transport": (I, [vp, C.POINTER(TransportProblem), vp, vp, vp, vp]),
     "femgpu_assemble_transport_transient": (I, [vp, C.POINTER(TransportProblem), D, D, vp, vp,
                                                 vp, vp, vp]),

@@ -579,3 +579,74 @@ def test_ns_assembly_bit_exact(dev, torch_cuda, oracle, name):
     assert np.array_equal(H(F), s["F"])
     if AA is not None:
         assert np.array_equal(H(AA), s["AA"])
+
+
+# ----------------------------------------------------------------------------- Euler assembly
+
+def eu_case(oracle, d=None, m=None, dimensionless=1, shock=1):
+    """Euler system on a golden fixture (as the reference numbered it) or on an m x m supersonic
+    state: numbering, lm, (CSR) structure and the oracle's Ae / AA / R."""
+    if d is not None:
+        x, y, ien, type4 = d["x"], d["y"], d["ien"], d["type4"]
+        presc, state, incr = d["presc"], d["state"], d["incr"]
+        h, pr = d["header"], d["params"]
+        ebe, reorder = int(h[2]), ("SYMRCM" if int(h[3]) else "NOT")
+        prm = dict(gamma=pr[0], Mach=pr[1], alpha=pr[2], dt=pr[3], tolerance=pr[4], dimensionless=int(h[4]),
+                   shock=int(h[5]))
+    else:
+        x, y, ien = oracle.mesh_structured(m, m)
+        nn = len(x)
+        rng = np.random.default_rng(m)
+        rho = 1.0 + 0.2 * rng.uniform(-1, 1, nn)
+        vx, vy = 2.0 + 0.3 * rng.uniform(-1, 1, nn), 0.2 + 0.3 * rng.uniform(-1, 1, nn)
+        pres = (1.0 / 1.4) * (1.0 + 0.3 * rng.uniform(-1, 1, nn))
+        Et = pres / (0.4 * rho) + 0.5 * (vx * vx + vy * vy)
+        st = np.stack([rho, rho * vx, rho * vy, rho * Et], 1)
+        inflow = np.abs(x) <= 1e-15
+        t4 = np.ones((nn, 4), np.int32)
+        t4[inflow] = 0
+        type4, presc = t4.ravel(), np.where(inflow[:, None], st, 0.0).ravel()
+        state, incr = st.ravel(), (1e-2 * rng.standard_normal((nn, 4))).ravel()
+        ebe, reorder = 0, "SYMRCM"
+        prm = dict(gamma=1.4, Mach=2.0, alpha=0.5, dt=1e-2, tolerance=1e-8, dimensionless=dimensionless, shock=shock)
+    invP, neq = oracle.number_unknowns(type4, 4)
+    lm = oracle.fill_lm(ien, type4, invP, neq, 4)
+    c = None
+    if not ebe:
+        c = oracle.csr_init(lm, 12, neq, reorder=reorder, permute_lm=True, type_=type4, invP=invP)
+        lm, invP = c["lm"], c["invP"]
+    u, du = np.zeros(neq), np.zeros(neq)
+    unk = type4 != 0
+    u[invP[unk]] = state[unk]
+    du[invP[unk]] = incr[unk]
+    U4, dU4 = oracle.euler_nodal_state(type4, invP, u, du, presc)
+    Ae, R = oracle.build_euler_supg(x, y, ien, lm, U4, dU4, oracle.euler_params(**prm), neq)
+    AA = oracle.csr_assemble(c["scheme"], Ae, c["nnz"], s=12) if c is not None else None
+    return dict(x=x, y=y, ien=ien, type4=type4, invP=invP, presc=presc, neq=neq, lm=lm, c=c, u=u, du=du,
+                prm=prm, Ae=Ae, R=R, AA=AA)
+
+
+@pytest.mark.parametrize("name", G.names("eu_") + ["m48", "m48dim_noshock"])
+def test_euler_assembly_bit_exact(dev, torch_cuda, oracle, name):
+    torch = torch_cuda
+    if name == "m48":
+        s = eu_case(oracle, m=48)
+    elif name == "m48dim_noshock":
+        s = eu_case(oracle, m=48, dimensionless=0, shock=0)
+    else:
+        d = G.load(name)
+        s = eu_case(oracle, d=d)
+        assert np.array_equal(s["R"], d["ref_R"])      # the oracle side, as pinned by the fixture
+    c = s["c"]
+    M = fem.EulerMesh(dev, s["x"], s["y"], s["type4"], s["invP"], s["presc"], s["ien"], s["lm"], s["neq"],
+                      scheme=c["scheme"] if c is not None else None, nnz=c["nnz"] if c is not None else 0)
+    nel = len(s["ien"]) // 3
+    Ae = torch.empty(144 * nel, dtype=torch.float64, device="cuda")
+    R = torch.empty(s["neq"] + 1, dtype=torch.float64, device="cuda")
+    AA = torch.empty(c["nnz"] + 1, dtype=torch.float64, device="cuda") if c is not None else None
+    M.Build_M_K_F_SUPG(fem.make_euler_problem(**s["prm"]), T(torch, s["u"]), T(torch, s["du"]), R, Ae=Ae,
+                       AA_ptr=AA.data_ptr() if AA is not None else None)
+    assert np.array_equal(H(Ae), s["Ae"])
+    assert np.array_equal(H(R), s["R"])
+    if AA is not None:
+        assert np.array_equal(H(AA), s["AA"])
