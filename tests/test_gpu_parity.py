@@ -463,11 +463,31 @@ def test_jacobi_sweeps(dev, torch_cuda, oracle, name):
     assert rel(H(zg), exact) < 1e-5
 
 
+def transient_system(oracle, m, dt, alpha=0.5):
+    """The transient transport system of one multicorrection (Build_M_K_R_SUPG): Gaussian pulse
+    state (centre (0.3, 0.5), sigma 0.05), rotating field, k = 1e-4, inflow data, SYMRCM-numbered
+    with the transient driver's lm permutation. Returns the CSR operator and the residual."""
+    x, y, ien = oracle.mesh_structured(m, m)
+    t = oracle.unit_square_types(x, y)
+    invP0, neq = oracle.number_unknowns(t)
+    lm0 = oracle.fill_lm(ien, t, invP0, neq)
+    c = oracle.csr_init(lm0, 3, neq, reorder="SYMRCM", permute_lm=True, type_=t, invP=invP0)
+    u_node = np.exp(-((x - 0.3) ** 2 + (y - 0.5) ** 2) / (2 * 0.05 ** 2))
+    u = np.zeros(neq)
+    u[c["invP"][t == 1]] = u_node[t == 1]
+    pb = oracle.make_problem(k=1e-4)
+    Ae, R = oracle.build_transport_transient(x, y, t, c["invP"], ien, c["lm"], oracle.inflow_ubc(x, y), u,
+                                             np.zeros(neq), pb, alpha, dt, neq)
+    AA = oracle.csr_assemble(c["scheme"], Ae, c["nnz"])
+    return dict(IA=c["IA"], JA=c["JA"], AA=AA, F=R[:neq], neq=neq)
+
+
 @pytest.mark.parametrize("factor_sweeps,sweep_mode", [(0, 2), (3, 2), (2, 3)])
 def test_mixed_parallel_preconditioner(dev, torch_cuda, oracle, factor_sweeps, sweep_mode):
-    """The mixed solver with the parallel FP32 preconditioner still reaches 1e-8 (FP64-verified)."""
+    """The mixed solver with the parallel FP32 preconditioner reaches 1e-8 (FP64-verified) on the
+    transient (mass-dominated, Courant number 0.5) system it is meant for."""
     torch = torch_cuda
-    s = steady_system(oracle, m=200, k=1e-2)
+    s = transient_system(oracle, m=200, dt=5e-3)
     A = fem.CsrMatrix(dev, s["IA"], s["JA"])
     A.set_values(s["AA"][:-1])
     if factor_sweeps == 0:
@@ -480,6 +500,23 @@ def test_mixed_parallel_preconditioner(dev, torch_cuda, oracle, factor_sweeps, s
     print(f"factor_sweeps={factor_sweeps} sweep_mode={sweep_mode}: refinements {info.cycles} "
           f"inner steps {info.inner_iterations} true rel residual {res:.2e}")
     assert info.status == 0 and res <= 1e-8
+
+
+def test_mixed_reports_stagnation(dev, torch_cuda, oracle):
+    """A preconditioner too weak for the operator (two Jacobi iterations on a steady, convection-
+    dominated system) must end with status 1 and the residual it reached, not a false success."""
+    torch = torch_cuda
+    s = steady_system(oracle, m=200, k=1e-2)
+    A = fem.CsrMatrix(dev, s["IA"], s["JA"])
+    A.set_values(s["AA"][:-1])
+    A.ilu0_factor()
+    X = torch.empty(s["neq"], dtype=torch.float64, device="cuda")
+    info = fem.gmres_mixed(dev, A, T(torch, s["F"]), X, tol=1e-12, inner_kmax=30, inner_tol=1e-4,
+                           max_refinements=3, sweep_mode=2)
+    x = H(X)
+    res = np.linalg.norm(s["F"] - oracle.csrmv(s["IA"], s["JA"], s["AA"], x)) / np.linalg.norm(s["F"])
+    assert info.status == 1 and info.cycles == 3
+    assert abs(res - info.true_rel_residual) <= 1e-3 * res
 
 
 def test_pcg_on_spd_diffusion(dev, torch_cuda, oracle):
