@@ -117,10 +117,14 @@ typedef struct femgpu_sweep_info {
     int chunks_lower;      /* packed: row chunks of the L sweep */
     int rows_per_chunk;    /* packed: row bound per chunk */
     int lds_limit;         /* LDS bytes available to one workgroup */
+    int factor_kernel;     /* 0: streamed single-workgroup factorisation, 1: level-synchronous lists */
 } femgpu_sweep_info;
 int femgpu_csr_sweep_info(femgpu_csr *A, femgpu_sweep_info *info);
 /* Select the sweep kernel: 0 = streamed when the plan allows it (default), 1 = packed. */
 int femgpu_csr_set_sweep_kernel(femgpu_csr *A, int kernel);
+/* Select the numeric factorisation kernel: 0 = streamed when the plan allows it (default),
+ * 1 = level-synchronous update lists. Both give the same factors. */
+int femgpu_csr_set_factor_kernel(femgpu_csr *A, int kernel);
 
 /* csrmv (CSR/csrmv.c:4-65), local block: Q = A P. FP64 and FP32 (FP32 copy of the values). */
 int femgpu_csrmv(femgpu_csr *A, const double *d_P, double *d_Q);

@@ -190,7 +190,10 @@ extern "C" int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA,
     if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), true, A->pup);
     if (!rc) rc = build_stream_plan(ctx, n, IA, JA, diag.data(), false, A->slo);
     if (!rc) rc = build_stream_plan(ctx, n, IA, JA, diag.data(), true, A->sup);
-    if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data());
+    if (!rc) rc = build_stream_maps(ctx, A);
+    if (!rc) rc = build_stream_factor(ctx, A, IA, JA, diag.data());
+    if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data(), false, A->fplan);
+    if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data(), true, A->fplan_nat);
     if (!rc && hipMemsetAsync(A->AA.p, 0, sizeof(double) * (nnz + 1), s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (rc) {
@@ -247,6 +250,14 @@ extern "C" int femgpu_csr_sweep_info(femgpu_csr *A, femgpu_sweep_info *info)
     info->chunks_lower = A->plo.nchunks;
     info->rows_per_chunk = A->plo.R;
     info->lds_limit = static_cast<int>(A->ctx->lds_limit);
+    info->factor_kernel = (A->factor_kernel == 0 && A->sfac.ok) ? 0 : 1;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_csr_set_factor_kernel(femgpu_csr *A, int kernel)
+{
+    FG_CHECK(A && (kernel == 0 || kernel == 1), FEMGPU_ERR_ARG, "femgpu_csr_set_factor_kernel: bad arguments");
+    A->factor_kernel = kernel;
     return FEMGPU_OK;
 }
 

@@ -115,6 +115,24 @@ struct StreamSweep {
     DevBuf<double> sink;            // kStreamThreads idle-thread results
 };
 
+// Streamed numeric factorisation (factor_stream.hip): the lower sweep's schedule and ring slots,
+// with each row's values padded to kFactorRow entries and, for every (entry e, L entry t) pair, the
+// index of column e in the U part of row JA[t] (4 bits; 15: absent). Results of recent rows (their
+// U part and inverse pivot) live in an LDS ring of kFactorRing values per slot.
+constexpr int kFactorRow = 8;    // entries per row
+constexpr int kFactorU = 4;      // U entries per row kept in the ring
+constexpr int kFactorRing = kFactorU + 1;
+
+struct StreamFactor {
+    bool ok = false;
+    DevBuf<uint4> codes;     // n: 32 codes of 4 bits, code (e, t) at bit 4*(4e+t)
+    DevBuf<int4> meta;       // n: {p0, len, nL, row}
+    DevBuf<double> aval;     // n*kFactorRow: values in schedule order (gathered from AA)
+    DevBuf<double> out;      // n*kFactorRow: factored row
+    DevBuf<double> dinv;     // n: inverse pivots in schedule order
+    DevBuf<double> sink;     // idle-thread results
+};
+
 // Numeric ILU(0) plan: for every row in the lower sweep's level schedule, its CSR position, length,
 // number of L entries, and its update list -- (U entry position k of a dependency row, target slot
 // in this row, index t of the L entry it belongs to) -- in the serial factorisation's order.
@@ -156,11 +174,18 @@ struct femgpu_csr {
     fg::PackedSweep plo, pup;       // packed lower / upper sweeps
     fg::StreamSweep slo, sup;       // streamed single-workgroup sweeps (used when both exist)
     int sweep_kernel = 0;           // 0: streamed when available, 1: packed
+    int factor_kernel = 0;          // 0: streamed when available, 1: level-synchronous update lists
+    fg::DevBuf<int> stream_l2u;     // upper-schedule position -> lower-schedule position of its row
+    fg::DevBuf<int> stream_pos_up;  // row -> upper-schedule position
+    fg::DevBuf<double> stream_t64;  // 2n: schedule-ordered vectors of the streamed pair
+    fg::DevBuf<float> stream_t32;
     bool f32_factored = false;          // LU32/Dinv32 hold factors of the current values
     fg::DevBuf<float> LU32, Dinv32;     // FP32 factors in A's layout (mixed solver's Jacobi sweeps)
     fg::DevBuf<float> LU32b, Dinv32b;   // second iterate of the parallel factorisation
     fg::DevBuf<float> jac_a, jac_b;     // Jacobi sweep iterates
-    fg::FactorPlan fplan;           // numeric factorisation
+    fg::FactorPlan fplan;           // numeric factorisation (level schedule)
+    fg::FactorPlan fplan_nat;       // the same lists in row order (parallel FP32 factorisation)
+    fg::StreamFactor sfac;          // streamed single-workgroup factorisation (when available)
     int spmv_lanes = 8;             // lanes per row of the SpMV kernel
 };
 
@@ -195,7 +220,13 @@ int packed_refresh(femgpu_ctx *ctx, femgpu_csr *A);
 int build_stream_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, const int *diag, bool upper,
                       StreamSweep &plan);
 size_t stream_lds_bytes(const StreamSweep &P, size_t elem);
-int build_factor_plan(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
+int build_stream_maps(femgpu_ctx *ctx, femgpu_csr *A);
+int build_stream_factor(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
+size_t factor_lds_bytes(const StreamSweep &P);
+int launch_stream_factor(femgpu_ctx *ctx, femgpu_csr *A);
+// natural: rows in index order (the parallel factorisation); else the lower sweep's level schedule.
+int build_factor_plan(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag, bool natural,
+                      FactorPlan &F);
 template <typename T>
 size_t packed_lds_bytes(const PackedSweep &P);
 

@@ -47,4 +47,13 @@ __device__ __forceinline__ bool last_block(unsigned *counter)
     return s_last != 0;
 }
 
+// Barrier that orders LDS only: global stores and prefetch loads stay in flight across it, so a
+// level step of the single-workgroup sweeps never waits for a memory round trip.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 }  // namespace fg

@@ -249,7 +249,9 @@ int launch_ilu_factor(femgpu_ctx *ctx, femgpu_csr *A)
 {
     if (A->n == 0) return FEMGPU_OK;
     const FactorPlan &F = A->fplan;
-    if (F.maxrow <= 8) {
+    if (A->sfac.ok && A->factor_kernel == 0) {
+        FG_TRY(launch_stream_factor(ctx, A));
+    } else if (F.maxrow <= 8) {
         k_ilu0_factor_lists<8, 8, 24, 512><<<1, 512, 0, ctx->stream>>>(A->plo.depth, A->plo.lvl_ptr.p, F.rowinfo.p,
                                                                         F.upd.p, A->JA.p, A->AA.p, A->LU.p, A->Dinv.p,
                                                                         ctx->err.p);

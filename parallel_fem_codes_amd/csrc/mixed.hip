@@ -34,8 +34,8 @@ __global__ void k_parilu_init(int n, const int *__restrict__ IA, const int *__re
     Dinv[i] = 1.0f / AA[d];
 }
 
-// One Jacobi-style sweep of the parallel ILU(0), one row per thread (rows in the plan's order; any
-// order gives the same result). Row entries are kept in registers when the row has at most MAXROW
+// One Jacobi-style sweep of the parallel ILU(0), one row per thread (rows in the plan's order, row
+// order by default so neighbouring threads touch neighbouring rows; any order gives the same result). Row entries are kept in registers when the row has at most MAXROW
 // entries and its list at most MAXU updates; other rows take the in-memory path.
 template <int MAXROW, int MAXU>
 __global__ __launch_bounds__(256) void k_parilu_sweep(int n, const int4 *__restrict__ rowinfo, const int2 *__restrict__ upd,
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_parilu_sweep(int n, const int4 *__restr
     if (q >= n) return;
     const int4 ri = rowinfo[q];
     const int p0 = ri.x, len = ri.y, nL = ri.z, ub = ri.w, nu = rowinfo[q + 1].w - ub;
-    const int i = row_of[q];
+    const int i = row_of ? row_of[q] : q;
     float piv;
     if (len <= MAXROW && nu <= MAXU) {
         float v[MAXROW];
@@ -141,7 +141,7 @@ int launch_parilu32(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
 {
     if (A->n == 0) return FEMGPU_OK;
     FG_TRY(ensure_par(A));
-    const FactorPlan &F = A->fplan;
+    const FactorPlan &F = A->fplan_nat;
     const int grid = (A->n + 255) / 256;
     // the final sweep must land in LU32/Dinv32: start in whichever buffer makes the parity right
     float *L0 = (sweeps % 2 == 0) ? A->LU32.p : A->LU32b.p, *L1 = (sweeps % 2 == 0) ? A->LU32b.p : A->LU32.p;
@@ -151,10 +151,10 @@ int launch_parilu32(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
     for (int s = 0; s < sweeps; ++s) {
         if (F.maxrow <= 8 && F.maxupd <= 16)
             k_parilu_sweep<8, 16><<<grid, 256, 0, ctx->stream>>>(A->n, F.rowinfo.p, F.upd.p, A->JA.p, A->AA32.p, L0, D0,
-                                                                 L1, D1, A->plo.row_of.p);
+                                                                 L1, D1, nullptr);
         else
             k_parilu_sweep<1, 1><<<grid, 256, 0, ctx->stream>>>(A->n, F.rowinfo.p, F.upd.p, A->JA.p, A->AA32.p, L0, D0,
-                                                                L1, D1, A->plo.row_of.p);
+                                                                L1, D1, nullptr);
         FG_HIP(hipGetLastError());
         std::swap(L0, L1);
         std::swap(D0, D1);

@@ -674,11 +674,87 @@ int build_stream_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, cons
     return FEMGPU_OK;
 }
 
-int build_factor_plan(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag)
+// Maps between the two streamed schedules (used when both plans exist): for every upper-schedule
+// position the lower-schedule position of its row, and for every row its upper-schedule position.
+int build_stream_maps(femgpu_ctx *ctx, femgpu_csr *A)
+{
+    if (!A->slo.ok || !A->sup.ok) return FEMGPU_OK;
+    const int n = A->n;
+    std::vector<int> rlo(n), rup(n), pos_lo(n), l2u(n), pos_up(n);
+    FG_HIP(hipMemcpy(rlo.data(), A->slo.row_of.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    FG_HIP(hipMemcpy(rup.data(), A->sup.row_of.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (int q = 0; q < n; ++q) {
+        pos_lo[rlo[q]] = q;
+        pos_up[rup[q]] = q;
+    }
+    for (int q = 0; q < n; ++q) l2u[q] = pos_lo[rup[q]];
+    FG_TRY(A->stream_l2u.upload(l2u.data(), n, ctx->stream));
+    FG_TRY(A->stream_pos_up.upload(pos_up.data(), n, ctx->stream));
+    FG_TRY(A->stream_t64.alloc(2 * static_cast<size_t>(n)));
+    FG_TRY(A->stream_t32.alloc(2 * static_cast<size_t>(n)));
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    return FEMGPU_OK;
+}
+
+size_t factor_lds_bytes(const StreamSweep &P)
+{
+    return sizeof(double) * kFactorRing * (static_cast<size_t>(P.W) + 2) +
+           sizeof(int) * (static_cast<size_t>(P.nsteps) + 2 * kStreamPad + 1);
+}
+
+// Plan of the streamed factorisation, on the lower sweep's streamed schedule (whose ring slots of
+// the L entries are the dependency slots here too). Exists when every row fits the padded format.
+int build_stream_factor(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag)
+{
+    StreamFactor &F = A->sfac;
+    F.ok = false;
+    const StreamSweep &P = A->slo;
+    if (!P.ok || P.K != 4 || A->n == 0) return FEMGPU_OK;
+    const size_t limit = ctx->lds_limit > 1024 ? ctx->lds_limit - 1024 : ctx->lds_limit;
+    if (factor_lds_bytes(P) > limit) return FEMGPU_OK;
+    const int n = A->n;
+    for (int i = 0; i < n; ++i) {
+        const int len = IA[i + 1] - IA[i], nL = diag[i] - IA[i], nU = len - nL - 1;
+        if (len > kFactorRow || nL > 4 || nU > kFactorU) return FEMGPU_OK;
+    }
+    std::vector<int> row_of(n);
+    FG_HIP(hipMemcpy(row_of.data(), P.row_of.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::vector<uint4> codes(n);
+    std::vector<int4> meta(n);
+    for (int q = 0; q < n; ++q) {
+        const int i = row_of[q], p0 = IA[i], len = IA[i + 1] - p0, nL = diag[i] - p0;
+        unsigned w[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+        for (int e = 0; e < len; ++e)
+            for (int t = 0; t < nL && t < e; ++t) {
+                const int j = JA[p0 + t], col = JA[p0 + e];
+                unsigned code = 15u;
+                for (int u = 0; u < IA[j + 1] - diag[j] - 1; ++u)
+                    if (JA[diag[j] + 1 + u] == col) code = static_cast<unsigned>(u);
+                const int bit = 4 * (4 * e + t);
+                w[bit / 32] = (w[bit / 32] & ~(15u << (bit % 32))) | (code << (bit % 32));
+            }
+        codes[q] = make_uint4(w[0], w[1], w[2], w[3]);
+        meta[q] = make_int4(p0, len, nL, i);
+    }
+    FG_TRY(F.codes.upload(codes.data(), n, ctx->stream));
+    FG_TRY(F.meta.upload(meta.data(), n, ctx->stream));
+    FG_TRY(F.aval.alloc(static_cast<size_t>(n) * kFactorRow));
+    FG_TRY(F.out.alloc(static_cast<size_t>(n) * kFactorRow));
+    FG_TRY(F.dinv.alloc(n));
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    F.ok = true;
+    return FEMGPU_OK;
+}
+
+int build_factor_plan(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag, bool natural,
+                      FactorPlan &F)
 {
     const int n = A->n;
     std::vector<int> row_of(n);
-    FG_HIP(hipMemcpy(row_of.data(), A->plo.row_of.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (natural)
+        for (int q = 0; q < n; ++q) row_of[q] = q;
+    else
+        FG_HIP(hipMemcpy(row_of.data(), A->plo.row_of.p, sizeof(int) * n, hipMemcpyDeviceToHost));
     std::vector<int4> info(static_cast<size_t>(n) + 1);
     std::vector<int2> upd;
     upd.reserve(static_cast<size_t>(IA[n]) * 2);
@@ -703,11 +779,11 @@ int build_factor_plan(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *
         FG_CHECK(len < 65536 && nL < 32768, FEMGPU_ERR_ARG, "row %d too long for the factorisation plan", i);
     }
     info[n] = make_int4(0, 0, 0, static_cast<int>(upd.size()));   // sentinel: end of the last list
-    A->fplan.maxrow = maxrow;
-    A->fplan.nupd = static_cast<int>(upd.size());
-    A->fplan.maxupd = maxupd;
-    FG_TRY(A->fplan.rowinfo.upload(info.data(), info.size(), ctx->stream));
-    FG_TRY(A->fplan.upd.upload(upd.data(), upd.size(), ctx->stream));
+    F.maxrow = maxrow;
+    F.nupd = static_cast<int>(upd.size());
+    F.maxupd = maxupd;
+    FG_TRY(F.rowinfo.upload(info.data(), info.size(), ctx->stream));
+    FG_TRY(F.upd.upload(upd.data(), upd.size(), ctx->stream));
     FG_HIP(hipStreamSynchronize(ctx->stream));
     return FEMGPU_OK;
 }

@@ -20,6 +20,7 @@
 #include <climits>
 
 #include "common.h"
+#include "device_util.h"
 
 namespace fg {
 
@@ -131,15 +132,6 @@ struct RemoteGather<float, N> {
 };
 
 constexpr int kPackedThreads = 512;
-
-// Barrier that orders LDS only: global stores (results, mailboxes) stay in flight across it, so a
-// level step never waits for a memory round trip.
-__device__ __forceinline__ void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 // UPPER: x = (b - U x) * Dinv; else x = b - L x. b may alias x (each row reads its own b during
 // staging, before any row of the launch is written, and other chunks never read x: they read the
@@ -262,18 +254,21 @@ __global__ __launch_bounds__(kPackedThreads) void k_sweep_packed(PackedArgs S, c
 // Streamed sweep: one workgroup walks the whole level schedule, one step (at most kStreamThreads
 // rows of one level) per LDS barrier, thread t taking row t of the step. Results of recent rows live
 // in an LDS ring, so a row reads its dependencies from LDS only and the per-level cost is one LDS
-// round trip plus one barrier -- no cross-workgroup hand-off. The row data of a step (padded factor
-// entries, ring slots, inverse pivot, right-hand side) does not depend on any result, so it is
-// fetched into registers D steps ahead (the row index, which the right-hand-side gather needs, 2D
-// steps ahead); those loads and the result stores stay in flight across the LDS-only barriers.
-// Every thread issues the same loads and stores on every step (idle threads use row 0 and store to
-// a sink), so the compiler's wait counts see one path and wait only for the step being computed.
+// round trip plus one barrier -- no cross-workgroup hand-off. The vectors are in schedule order
+// (permuted by full-device gathers before and after the pair of sweeps), so every load and store of
+// a step is contiguous; a single CU's memory pipeline is what bounds a step, and scattered row
+// accesses would multiply the cache lines it has to move. The row data of a step (padded factor
+// entries, ring slots, inverse pivot, right-hand side) does not depend on any result and is fetched
+// into registers D steps ahead; those loads and the result stores stay in flight across the
+// LDS-only barriers. Every thread issues the same loads and stores on every step (idle threads read
+// position 0 and store to a sink), so the compiler's wait counts see one path and wait only for the
+// step being computed.
 
 struct StreamArgs {
     int nsteps;          // kStreamPad empty steps, the schedule, empty steps up to a multiple of
                          // kStreamPad; stepq has nsteps + 2*kStreamPad + 1 entries
     int W;
-    const int *stepq, *row_of;
+    const int *stepq;
     const unsigned short *slot;
     void *sink;          // kStreamThreads results of idle threads
 };
@@ -314,62 +309,30 @@ __device__ __forceinline__ int stream_pos(const int *sq, int s)
     return q < sq[s + 1] ? q : -1;
 }
 
-// Stage step s into g: entries, slots and pivot by position, right-hand side through the row index
-// fetched D steps earlier.
+// Stage step s into g: entries, slots, pivot and right-hand side by schedule position.
 template <typename T, bool UPPER, int K>
 __device__ __forceinline__ void stream_fetch(StreamStage<T, K> &g, const StreamArgs &a, const int *sq, int s,
-                                             unsigned row, const T *__restrict__ val, const T *__restrict__ dinv,
-                                             const T *b)
+                                             const T *__restrict__ val, const T *__restrict__ dinv, const T *b)
 {
     const int q = stream_pos(sq, s);
     const int qq = q < 0 ? 0 : q;
     load_bytes<K * sizeof(T)>(val + static_cast<size_t>(qq) * K, g.v);
     load_bytes<K * 2>(a.slot + static_cast<size_t>(qq) * K, g.s);
     if (UPPER) g.d = dinv[qq];
-    g.b = b[static_cast<size_t>(row)];
+    g.b = b[qq];
     g.q = q;
 }
 
-// Row indices are handled as unsigned so that no instruction touches a freshly loaded index (a sign
-// extension) before the step that uses it.
-__device__ __forceinline__ unsigned stream_row(const StreamArgs &a, const int *sq, int s)
-{
-    const int q = stream_pos(sq, s);
-    return reinterpret_cast<const unsigned *>(a.row_of)[q < 0 ? 0 : q];
-}
-
-// One step: compute the staged rows (cur = their row indices), then restage for step s + D, whose
-// row indices are in nxt, and fetch the row indices of step s + 2D into cur.
-template <typename T, bool UPPER, int K, int D>
-__device__ __forceinline__ void stream_step(StreamStage<T, K> &g, unsigned &cur, unsigned nxt, const StreamArgs &a,
-                                            const int *sq, int s, T *ring, T *sink, const T *__restrict__ val,
-                                            const T *__restrict__ dinv, const T *b, T *x)
-{
-#pragma clang fp contract(off)
-    // keep each step's work in its own region: hoisting a later step's slot decoding would make
-    // this step wait for that step's loads
-    __builtin_amdgcn_sched_barrier(0);
-    T acc = g.b;
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc -= g.v[k] * ring[g.s[k]];
-    if (UPPER) acc *= g.d;
-    const bool on = g.q >= 0;
-    ring[on ? (g.q & (a.W - 1)) : a.W + 1] = acc;
-    *(on ? x + static_cast<size_t>(cur) : sink) = acc;
-    stream_fetch<T, UPPER, K>(g, a, sq, s + D, nxt, val, dinv, b);
-    cur = stream_row(a, sq, s + 2 * D);
-    lds_barrier();
-}
-
 // UPPER: x = (b - U x) * Dinv; else x = b - L x (ILUp_precond.c:26-44, same operation order,
-// contraction off; padding entries subtract 0 * 0). b may alias x: a row's b is read D steps before
-// the row's own result is stored, and no other row reads it.
+// contraction off; padding entries subtract 0 * 0). b and x are in schedule order; b may alias x:
+// a row's b is read D steps before its own result is stored, and no other row reads it.
 template <typename T, bool UPPER, int K, int D>
 __global__ __launch_bounds__(kStreamThreads) void k_sweep_stream(StreamArgs a, const T *__restrict__ val,
                                                                  const T *__restrict__ dinv, const T *b, T *x,
                                                                  const int *__restrict__ skip)
 {
-    static_assert(2 * D <= kStreamPad, "step padding too small for the lookahead");
+#pragma clang fp contract(off)
+    static_assert(kStreamPad % D == 0 && D <= kStreamPad, "step padding must cover the lookahead");
     if (skip && *skip) return;
     extern __shared__ unsigned char smem_raw[];
     T *ring = reinterpret_cast<T *>(smem_raw);                  // W results, a zero, an idle slot
@@ -378,12 +341,12 @@ __global__ __launch_bounds__(kStreamThreads) void k_sweep_stream(StreamArgs a, c
     for (int i = threadIdx.x; i < nsq; i += blockDim.x) sq[i] = a.stepq[i];
     if (threadIdx.x == 0) ring[a.W] = T(0);
     __syncthreads();
+    const int wmask = a.W - 1;
     T *sink = static_cast<T *>(a.sink) + threadIdx.x;
     // The schedule starts with kStreamPad empty steps: the loop itself stages the first real
     // steps, so there is no separate prologue (whose different order of loads would make the wait
     // counts at the loop head conservative).
     StreamStage<T, K> st[D];
-    unsigned ra[D], rb[D];   // row indices; the roles (current / next) swap every D steps
 #pragma unroll
     for (int u = 0; u < D; ++u) {
 #pragma unroll
@@ -393,16 +356,35 @@ __global__ __launch_bounds__(kStreamThreads) void k_sweep_stream(StreamArgs a, c
         }
         st[u].b = st[u].d = T(0);
         st[u].q = -1;
-        ra[u] = rb[u] = 0u;
     }
-    for (int s0 = 0; s0 < a.nsteps; s0 += 2 * D) {
+    for (int s0 = 0; s0 < a.nsteps; s0 += D) {
 #pragma unroll
-        for (int u = 0; u < D; ++u)
-            stream_step<T, UPPER, K, D>(st[u], ra[u], rb[u], a, sq, s0 + u, ring, sink, val, dinv, b, x);
+        for (int u = 0; u < D; ++u) {
+            // keep each step's work in its own region: hoisting a later step's slot decoding
+            // would make this step wait for that step's loads
+            __builtin_amdgcn_sched_barrier(0);
+            StreamStage<T, K> &g = st[u];
+            T acc = g.b;
 #pragma unroll
-        for (int u = 0; u < D; ++u)
-            stream_step<T, UPPER, K, D>(st[u], rb[u], ra[u], a, sq, s0 + D + u, ring, sink, val, dinv, b, x);
+            for (int k = 0; k < K; ++k) acc -= g.v[k] * ring[g.s[k]];
+            if (UPPER) acc *= g.d;
+            const bool on = g.q >= 0;
+            ring[on ? (g.q & wmask) : a.W + 1] = acc;
+            *(on ? x + g.q : sink) = acc;
+            stream_fetch<T, UPPER, K>(g, a, sq, s0 + u + D, val, dinv, b);
+            lds_barrier();
+        }
     }
+}
+
+// dst[i] = src[idx[i]]: the permutations into and out of schedule order.
+template <typename T>
+__global__ void k_permute(int n, const int *__restrict__ idx, const T *__restrict__ src, T *__restrict__ dst,
+                          const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
 }
 
 // Streamed-plan values (padding 0) and inverse pivots in schedule order from the factors.
@@ -427,7 +409,6 @@ static StreamArgs stream_args(const StreamSweep &P)
     a.nsteps = P.nsteps;
     a.W = P.W;
     a.stepq = P.stepq.p;
-    a.row_of = P.row_of.p;
     a.slot = P.slot.p;
     a.sink = P.sink.p;
     return a;
@@ -443,6 +424,22 @@ static int stream_launch(femgpu_ctx *ctx, const StreamSweep &P, const T *val, co
         k_sweep_stream<T, UPPER, 4, 8><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
     else
         k_sweep_stream<T, UPPER, 8, 4><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+// z = U^-1 L^-1 p with both sweeps streamed: p into lower-schedule order, lower sweep in place, into
+// upper-schedule order, upper sweep in place, back to row order. p may alias z.
+template <typename T>
+static int stream_pair(femgpu_ctx *ctx, femgpu_csr *A, const T *b, T *x, const int *skip, const T *vlo,
+                       const T *vup, const T *dup, T *t1, T *t2)
+{
+    const int n = A->n, grid = (n + 255) / 256;
+    k_permute<T><<<grid, 256, 0, ctx->stream>>>(n, A->slo.row_of.p, b, t1, skip);
+    FG_TRY((stream_launch<T, false>(ctx, A->slo, vlo, nullptr, t1, t1, skip)));
+    k_permute<T><<<grid, 256, 0, ctx->stream>>>(n, A->stream_l2u.p, t1, t2, skip);
+    FG_TRY((stream_launch<T, true>(ctx, A->sup, vup, dup, t2, t2, skip)));
+    k_permute<T><<<grid, 256, 0, ctx->stream>>>(n, A->stream_pos_up.p, t2, x, skip);
     FG_HIP(hipGetLastError());
     return FEMGPU_OK;
 }
@@ -538,20 +535,18 @@ static int packed_pair(femgpu_ctx *ctx, femgpu_csr *A, const T *b, T *x, const i
 int launch_sweep64(femgpu_ctx *ctx, femgpu_csr *A, const double *b, double *x, const int *skip)
 {
     if (A->n == 0) return FEMGPU_OK;
-    if (use_stream(A)) {
-        FG_TRY((stream_launch<double, false>(ctx, A->slo, A->slo.val64.p, nullptr, b, x, skip)));
-        return stream_launch<double, true>(ctx, A->sup, A->sup.val64.p, A->sup.dinv64.p, x, x, skip);
-    }
+    if (use_stream(A))
+        return stream_pair<double>(ctx, A, b, x, skip, A->slo.val64.p, A->sup.val64.p, A->sup.dinv64.p,
+                                   A->stream_t64.p, A->stream_t64.p + A->n);
     return packed_pair<double>(ctx, A, b, x, skip, A->plo.val64.p, A->pup.val64.p, A->pup.dinv64.p);
 }
 
 int launch_sweep32(femgpu_ctx *ctx, femgpu_csr *A, const float *b, float *x, const int *skip)
 {
     if (A->n == 0) return FEMGPU_OK;
-    if (use_stream(A)) {
-        FG_TRY((stream_launch<float, false>(ctx, A->slo, A->slo.val32.p, nullptr, b, x, skip)));
-        return stream_launch<float, true>(ctx, A->sup, A->sup.val32.p, A->sup.dinv32.p, x, x, skip);
-    }
+    if (use_stream(A))
+        return stream_pair<float>(ctx, A, b, x, skip, A->slo.val32.p, A->sup.val32.p, A->sup.dinv32.p,
+                                  A->stream_t32.p, A->stream_t32.p + A->n);
     return packed_pair<float>(ctx, A, b, x, skip, A->plo.val32.p, A->pup.val32.p, A->pup.dinv32.p);
 }
 

@@ -225,6 +225,11 @@ class CsrMatrix:
         N.check(N.lib().femgpu_csr_sweep_info(self.h, C.byref(si)))
         return {f: getattr(si, f) for f, _ in N.SweepInfo._fields_}
 
+    def set_factor_kernel(self, kind):
+        """"stream" (single-workgroup streamed factorisation, the default where its plan exists) or
+        "lists" (level-synchronous update lists)."""
+        N.check(N.lib().femgpu_csr_set_factor_kernel(self.h, {"stream": 0, "lists": 1}[kind]))
+
     def set_sweep_kernel(self, kind):
         """"stream" (single-workgroup streamed sweep, the default where its plan exists) or "packed"."""
         N.check(N.lib().femgpu_csr_set_sweep_kernel(self.h, {"stream": 0, "packed": 1}[kind]))

@@ -52,7 +52,8 @@ class EulerProblem(C.Structure):
 class SweepInfo(C.Structure):
     """femgpu_sweep_info."""
     _fields_ = [(f, C.c_int) for f in ("kernel", "depth_lower", "depth_upper", "steps_lower", "steps_upper",
-                                        "ring", "entries_per_row", "chunks_lower", "rows_per_chunk", "lds_limit")]
+                                        "ring", "entries_per_row", "chunks_lower", "rows_per_chunk", "lds_limit",
+                                        "factor_kernel")]
 
 
 class MixedOpts(C.Structure):
@@ -90,6 +91,7 @@ _SIGS = {
     "femgpu_csr_values_changed": (I, [vp]),
     "femgpu_csr_sweep_info": (I, [vp, C.POINTER(SweepInfo)]),
     "femgpu_csr_set_sweep_kernel": (I, [vp, I]),
+    "femgpu_csr_set_factor_kernel": (I, [vp, I]),
     "femgpu_csrmv": (I, [vp, vp, vp]),
     "femgpu_csrmv_f32": (I, [vp, vp, vp]),
     "femgpu_ilu0_factor": (I, [vp]),

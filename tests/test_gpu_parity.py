@@ -185,11 +185,12 @@ def test_ilu0_factor_and_sweeps_bit_exact(dev, torch_cuda, oracle, name, kernel)
     s = ilu_case(oracle, name)
     A = fem.CsrMatrix(dev, s["IA"], s["JA"])
     A.set_sweep_kernel(kernel)
+    A.set_factor_kernel("stream" if kernel == "stream" else "lists")
     info = A.sweep_info()
     if kernel == "packed" or name == "synthetic12":
-        assert info["kernel"] == 1
+        assert info["kernel"] == 1 and info["factor_kernel"] == 1
     else:
-        assert info["kernel"] == 0 and info["entries_per_row"] == 4
+        assert info["kernel"] == 0 and info["entries_per_row"] == 4 and info["factor_kernel"] == 0
         if name == "m600not":
             assert info["steps_lower"] > info["depth_lower"]
     A.set_values(s["AA"][:-1])
