@@ -112,10 +112,8 @@ extern "C" int femgpu_create(int device, void *stream, femgpu_ctx **out)
         ctx->lds_limit = prop.sharedMemPerBlock;
     }
     int rc = ctx->partials.alloc(kReduceBlocks);
-    if (!rc) rc = ctx->counter.alloc(4);
     if (!rc) rc = ctx->scalars.alloc(16);
     if (!rc) rc = ctx->err.alloc(4);
-    if (!rc && hipMemset(ctx->counter.p, 0, 4 * sizeof(unsigned)) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (!rc && hipMemset(ctx->err.p, 0, 4 * sizeof(int)) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (!rc && hipHostMalloc((void **)&ctx->h_pinned, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         rc = FEMGPU_ERR_HIP;

@@ -150,7 +150,6 @@ struct femgpu_ctx {
     int num_cus = 256;
     size_t lds_limit = 65536;      // dynamic LDS available to one workgroup
     fg::DevBuf<double> partials;   // per-block partial sums
-    fg::DevBuf<unsigned> counter;  // last-block detection of the reductions
     fg::DevBuf<double> scalars;    // small device scalars (dot results)
     fg::DevBuf<int> err;           // sticky device error flag (bounded waits)
     double *h_pinned = nullptr;    // pinned host mirror for scalar results

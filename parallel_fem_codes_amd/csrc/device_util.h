@@ -31,22 +31,6 @@ __device__ __forceinline__ T block_sum(T v, T *red)
     return v;
 }
 
-// Called by all threads after each block stored its partial: true in the block that finished
-// last (it then sees every partial). Release before the count, acquire after it.
-__device__ __forceinline__ bool last_block(unsigned *counter)
-{
-    __shared__ int s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return s_last != 0;
-}
-
 // Barrier that orders LDS only: global stores and prefetch loads stay in flight across it, so a
 // level step of the single-workgroup sweeps never waits for a memory round trip.
 __device__ __forceinline__ void lds_barrier()

@@ -369,29 +369,32 @@ int launch_ebe_gather_values(femgpu_ctx *ctx, femgpu_ebe *E, const double *A_ele
 }
 
 // ------------------------------------------------------------------------------- dot product
+// Two kernels: per-block partials, then one block sums them in a fixed order (deterministic, no
+// atomics). The inputs may be any device vectors, so the loads are scalar.
 __global__ __launch_bounds__(256) void k_dot(int n, const double *__restrict__ x, const double *__restrict__ y,
-                                             double *partials, unsigned *counter, double *out)
+                                             double *partials)
 {
     __shared__ double red[256 / kWave];
     double acc = 0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) acc += x[i] * y[i];
     acc = block_sum<256>(acc, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = acc;
-    if (last_block(counter)) {
-        double v = 0;
-        for (int b = threadIdx.x; b < gridDim.x; b += 256) v += partials[b];
-        v = block_sum<256>(v, red);
-        if (threadIdx.x == 0) {
-            *out = v;
-            *counter = 0;
-        }
-    }
+}
+
+__global__ __launch_bounds__(256) void k_dot_finish(int nb, const double *__restrict__ partials, double *out)
+{
+    __shared__ double red[256 / kWave];
+    double v = 0;
+    for (int b = threadIdx.x; b < nb; b += 256) v += partials[b];
+    v = block_sum<256>(v, red);
+    if (threadIdx.x == 0) *out = v;
 }
 
 int dot64(femgpu_ctx *ctx, int n, const double *x, const double *y, double *d_out)
 {
     const int grid = std::max(1, std::min(kReduceBlocks, (n + 1023) / 1024));
-    k_dot<<<grid, 256, 0, ctx->stream>>>(n, x, y, ctx->partials.p, ctx->counter.p, d_out);
+    k_dot<<<grid, 256, 0, ctx->stream>>>(n, x, y, ctx->partials.p);
+    k_dot_finish<<<1, 256, 0, ctx->stream>>>(grid, ctx->partials.p, d_out);
     FG_HIP(hipGetLastError());
     return FEMGPU_OK;
 }

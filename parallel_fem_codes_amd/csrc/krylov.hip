@@ -5,7 +5,7 @@
 //   spmv   : w' = A (w * 1/h), storing the normalised basis vector V[i] = w * 1/h on the way
 //   sweeps : w' = U^-1 L^-1 w' (level-scheduled ILU(0), in place)
 //   mgs j  : w' -= h_{j-1,i} V[j-1] fused with h_{j,i} = <V[j], w'>, j = 0..i (one pass each)
-//   norm   : w' -= h_{i,i} V[i] fused with ||w'||; the last block finishes the column: Givens
+//   norm   : w' -= h_{i,i} V[i] fused with ||w'||; a one-block kernel finishes the column: Givens
 //            rotations, residual estimate, convergence flag (pgmres.c:118-156)
 // Every launch of a step returns immediately once the cycle-done flag is set, so the host can
 // enqueue one step ahead of the convergence check it reads back.
@@ -24,7 +24,6 @@ struct GDev {
     int *flags;       // [0] cycle done  [1] steps in cycle
     int ld, kmax;
     double *partials;
-    unsigned *counter;
     int *h_status;    // host-mapped: per step {done} ; h_rho host-mapped rho per step
     double *h_rho;
     int prec_mode;    // FP32 preconditioner (host side only): 0 exact sweeps, k > 0 Jacobi iterations
@@ -60,44 +59,97 @@ __device__ void givens_column(const GDev &G, int i)
     __hip_atomic_store(&G.h_status[i], done ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// MGS pass j of step i on w (vector type T, accumulation in FP64):
+// 16-byte vectors of the basis type (the basis and work vectors are 256-byte aligned).
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    using type = float4;
+    static constexpr int N = 4;
+    static __device__ __forceinline__ float at(const float4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+    static __device__ __forceinline__ void set(float4 &v, int k, float x)
+    {
+        if (k == 0) v.x = x; else if (k == 1) v.y = x; else if (k == 2) v.z = x; else v.w = x;
+    }
+};
+template <>
+struct Vec16<double> {
+    using type = double2;
+    static constexpr int N = 2;
+    static __device__ __forceinline__ double at(const double2 &v, int k) { return k == 0 ? v.x : v.y; }
+    static __device__ __forceinline__ void set(double2 &v, int k, double x)
+    {
+        if (k == 0) v.x = x; else v.y = x;
+    }
+};
+
+// Grid reductions run as two kernels: per-block partials (plain stores; the kernel boundary makes
+// them visible), then one block that sums them in a fixed order and acts on the sum. No atomics or
+// device-scope fences, and the result does not depend on block timing.
+
+// MGS pass j of step i on w (vector type T, accumulation in FP64), first kernel:
 //   uprev != null : w -= h_{j-1,i} * uprev
-//   unext != null : h_{j,i} = <unext, w>         (unext == null: norm pass, then Givens)
+//   unext != null : partial of h_{j,i} = <unext, w>   (unext == null: partial of ||w||^2)
 template <typename T>
 __global__ __launch_bounds__(256) void k_mgs(GDev G, int n, int i, int j, T *__restrict__ w,
                                              const T *__restrict__ uprev, const T *__restrict__ unext)
 {
     if (G.flags[0]) return;
+    using V = Vec16<T>;
+    using VT = typename V::type;
     __shared__ double red[4];
-    const double hp = uprev ? Hat(G, j - 1, i) : 0.0;
-    const T h = static_cast<T>(hp);
+    const T h = uprev ? static_cast<T>(Hat(G, j - 1, i)) : T(0);
     double acc = 0;
-    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
-        T wv = w[k];
+    const int nv = n / V::N;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < nv; k += gridDim.x * 256) {
+        VT wv = reinterpret_cast<const VT *>(w)[k];
         if (uprev) {
-            wv = wv - h * uprev[k];
-            w[k] = wv;
+            const VT pv = reinterpret_cast<const VT *>(uprev)[k];
+#pragma unroll
+            for (int c = 0; c < V::N; ++c) V::set(wv, c, V::at(wv, c) - h * V::at(pv, c));
+            reinterpret_cast<VT *>(w)[k] = wv;
         }
-        const T u = unext ? unext[k] : wv;
-        acc += static_cast<double>(u) * static_cast<double>(wv);
+        const VT uv = unext ? reinterpret_cast<const VT *>(unext)[k] : wv;
+#pragma unroll
+        for (int c = 0; c < V::N; ++c) acc += static_cast<double>(V::at(uv, c)) * static_cast<double>(V::at(wv, c));
     }
+    if (blockIdx.x == 0)
+        for (int k = nv * V::N + threadIdx.x; k < n; k += 256) {
+            T wv = w[k];
+            if (uprev) {
+                wv = wv - h * uprev[k];
+                w[k] = wv;
+            }
+            const T u = unext ? unext[k] : wv;
+            acc += static_cast<double>(u) * static_cast<double>(wv);
+        }
     acc = block_sum<256>(acc, red);
     if (threadIdx.x == 0) G.partials[blockIdx.x] = acc;
-    if (last_block(G.counter)) {
-        double v = 0;
-        for (int b = threadIdx.x; b < gridDim.x; b += 256) v += G.partials[b];
-        v = block_sum<256>(v, red);
-        if (threadIdx.x == 0) {
-            *G.counter = 0;
-            if (unext) {
-                Hat(G, j, i) = v;
-            } else {
-                const double hn = sqrt(v);
-                Hat(G, i + 1, i) = hn;
-                G.hnorm[i + 1] = hn;
-                G.invh[i + 1] = 1.0 / hn;
-                givens_column(G, i);
-            }
+}
+
+// Sum of the nb partials in a fixed order (one block of 256 threads; the sum is valid in thread 0).
+__device__ __forceinline__ double sum_partials(const double *partials, int nb, double *red)
+{
+    double v = 0;
+    for (int b = threadIdx.x; b < nb; b += 256) v += partials[b];
+    return block_sum<256>(v, red);
+}
+
+// MGS pass, second kernel: h_{j,i}, or the norm h_{i+1,i} followed by the Givens update.
+__global__ __launch_bounds__(256) void k_mgs_finish(GDev G, int nb, int i, int j, int has_next)
+{
+    if (G.flags[0]) return;
+    __shared__ double red[4];
+    const double v = sum_partials(G.partials, nb, red);
+    if (threadIdx.x == 0) {
+        if (has_next) {
+            Hat(G, j, i) = v;
+        } else {
+            const double hn = sqrt(v);
+            Hat(G, i + 1, i) = hn;
+            G.hnorm[i + 1] = hn;
+            G.invh[i + 1] = 1.0 / hn;
+            givens_column(G, i);
         }
     }
 }
@@ -106,7 +158,7 @@ __global__ __launch_bounds__(256) void k_mgs(GDev G, int n, int i, int j, T *__r
 //   negate = 1: w = B - w (pgmres.c:73-74), then e0 = ||w||, invh[0] = 1/e0, rho = e0, flags reset
 template <typename T>
 __global__ __launch_bounds__(256) void k_cycle_start(GDev G, int n, T *__restrict__ w, const T *__restrict__ B,
-                                                     int negate, int first, double tol)
+                                                     int negate)
 {
     __shared__ double red[4];
     double acc = 0;
@@ -120,24 +172,24 @@ __global__ __launch_bounds__(256) void k_cycle_start(GDev G, int n, T *__restric
     }
     acc = block_sum<256>(acc, red);
     if (threadIdx.x == 0) G.partials[blockIdx.x] = acc;
-    if (last_block(G.counter)) {
-        double v = 0;
-        for (int b = threadIdx.x; b < gridDim.x; b += 256) v += G.partials[b];
-        v = block_sum<256>(v, red);
-        if (threadIdx.x == 0) {
-            *G.counter = 0;
-            const double e0 = sqrt(v);
-            if (first) {   // pgmres.c:42-46: ||B|| and eps
-                G.scal[2] = e0;
-                G.scal[0] = tol * e0;
-            }
-            G.e[0] = e0;
-            G.hnorm[0] = e0;
-            G.invh[0] = 1.0 / e0;
-            G.scal[1] = e0;
-            G.flags[0] = 0;
-            G.flags[1] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_cycle_start_finish(GDev G, int nb, int first, double tol)
+{
+    __shared__ double red[4];
+    const double v = sum_partials(G.partials, nb, red);
+    if (threadIdx.x == 0) {
+        const double e0 = sqrt(v);
+        if (first) {   // pgmres.c:42-46: ||B|| and eps
+            G.scal[2] = e0;
+            G.scal[0] = tol * e0;
         }
+        G.e[0] = e0;
+        G.hnorm[0] = e0;
+        G.invh[0] = 1.0 / e0;
+        G.scal[1] = e0;
+        G.flags[0] = 0;
+        G.flags[1] = 0;
     }
 }
 
@@ -191,6 +243,27 @@ __global__ void k_sqrt_inplace(double *v)
 
 static int grid_for(int n) { return std::max(1, std::min(kReduceBlocks, (n + 1023) / 1024)); }
 
+// The MGS passes of Arnoldi step i (j = 0..i, then the norm pass), two launches each.
+template <typename T>
+static void mgs_step(femgpu_ctx *ctx, const GDev &G, int n, int i, T *wn, const T *V, size_t ldv)
+{
+    const int grid = grid_for(n);
+    for (int j = 0; j <= i + 1; ++j) {
+        const T *uprev = j ? V + (j - 1) * ldv : nullptr;
+        const T *unext = j <= i ? V + j * ldv : nullptr;
+        k_mgs<T><<<grid, 256, 0, ctx->stream>>>(G, n, i, j, wn, uprev, unext);
+        k_mgs_finish<<<1, 256, 0, ctx->stream>>>(G, grid, i, j, unext != nullptr);
+    }
+}
+
+template <typename T>
+static void cycle_start(femgpu_ctx *ctx, const GDev &G, int n, T *w, const T *B, int negate, int first, double tol)
+{
+    const int grid = grid_for(n);
+    k_cycle_start<T><<<grid, 256, 0, ctx->stream>>>(G, n, w, B, negate);
+    k_cycle_start_finish<<<1, 256, 0, ctx->stream>>>(G, grid, first, tol);
+}
+
 // ------------------------------------------------------------------------------- workspace
 template <typename T>
 static int ensure_ws(femgpu_ctx *ctx, int n, int kmax, DevBuf<T> &basis, DevBuf<T> &wbuf, size_t &ldv)
@@ -220,7 +293,6 @@ static int make_gdev(femgpu_ctx *ctx, int kmax, GDev &G)
     G.ld = ld;
     G.kmax = kmax;
     G.partials = ctx->partials.p;
-    G.counter = ctx->counter.p;
     G.h_status = ctx->d_status;
     G.h_rho = reinterpret_cast<double *>(ctx->d_status + 256);
     FG_HIP(hipMemsetAsync(G.H, 0, sizeof(double) * ld * ld, ctx->stream));
@@ -251,16 +323,13 @@ int run_cycle<double>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, d
     hipEvent_t ev[2];
     FG_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     FG_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    const int grid = grid_for(n);
     double *wc = wA, *wn = wB;
     int rc = FEMGPU_OK, steps = 0;
     for (int i = 0; i < kmax - 1; ++i) {
         ctx->h_status[i] = 0;
         rc = apply_op64(ctx, S, wc, G.invh + i, V + i * ldv, wn, G.flags);
         if (rc) break;
-        for (int j = 0; j <= i; ++j)
-            k_mgs<double><<<grid, 256, 0, ctx->stream>>>(G, n, i, j, wn, j ? V + (j - 1) * ldv : nullptr, V + j * ldv);
-        k_mgs<double><<<grid, 256, 0, ctx->stream>>>(G, n, i, i + 1, wn, V + i * ldv, nullptr);
+        mgs_step<double>(ctx, G, n, i, wn, V, ldv);
         if (hipGetLastError() != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
         if (hipEventRecord(ev[i & 1], ctx->stream) != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
         std::swap(wc, wn);
@@ -284,7 +353,6 @@ int run_cycle<float>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, fl
     hipEvent_t ev[2];
     FG_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     FG_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    const int grid = grid_for(n);
     float *wc = wA, *wn = wB;
     int rc = FEMGPU_OK, steps = 0;
     for (int i = 0; i < kmax - 1; ++i) {
@@ -292,9 +360,7 @@ int run_cycle<float>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, fl
         rc = launch_spmv32(ctx, S->csr, wc, G.invh + i, V + i * ldv, wn, G.flags);
         if (!rc) rc = prec32(ctx, S->ilu, G.prec_mode, wn, wn, G.flags);
         if (rc) break;
-        for (int j = 0; j <= i; ++j)
-            k_mgs<float><<<grid, 256, 0, ctx->stream>>>(G, n, i, j, wn, j ? V + (j - 1) * ldv : nullptr, V + j * ldv);
-        k_mgs<float><<<grid, 256, 0, ctx->stream>>>(G, n, i, i + 1, wn, V + i * ldv, nullptr);
+        mgs_step<float>(ctx, G, n, i, wn, V, ldv);
         if (hipGetLastError() != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
         if (hipEventRecord(ev[i & 1], ctx->stream) != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
         std::swap(wc, wn);
@@ -366,7 +432,7 @@ extern "C" int femgpu_pgmres(femgpu_ctx *ctx, const femgpu_system *S, int kmax, 
     FG_HIP(hipMemsetAsync(d_X, 0, sizeof(double) * n, ctx->stream));   // pgmres.c:39
     // first cycle: the residual is B itself (X = 0)
     FG_HIP(hipMemcpyAsync(wA, d_B, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
-    k_cycle_start<double><<<grid, 256, 0, ctx->stream>>>(G, n, wA, d_B, 0, 1, tol);
+    cycle_start<double>(ctx, G, n, wA, d_B, 0, 1, tol);
     FG_HIP(hipGetLastError());
     double normb = 0;
     FG_HIP(hipMemcpyAsync(&ctx->h_pinned[0], G.scal + 2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -389,7 +455,7 @@ extern "C" int femgpu_pgmres(femgpu_ctx *ctx, const femgpu_system *S, int kmax, 
                 FG_TRY(launch_ebemv(ctx, S->ebe, d_X, wT));
             }
             if (S->ilu) FG_TRY(launch_sweep64(ctx, S->ilu, wT, wT, nullptr));
-            k_cycle_start<double><<<grid, 256, 0, ctx->stream>>>(G, n, wT, d_B, 1, 0, tol);
+            cycle_start<double>(ctx, G, n, wT, d_B, 1, 0, tol);
             FG_HIP(hipGetLastError());
             std::swap(wA, wT);
         }
@@ -405,10 +471,7 @@ extern "C" int femgpu_pgmres(femgpu_ctx *ctx, const femgpu_system *S, int kmax, 
             for (int i = 0; i < kmax - 1; ++i) {
                 ctx->h_status[i] = 0;
                 FG_TRY(apply_step_input(ctx, S, n, wc, G.invh + i, V + i * ldv, wn, G.flags));
-                for (int j = 0; j <= i; ++j)
-                    k_mgs<double><<<grid, 256, 0, ctx->stream>>>(G, n, i, j, wn, j ? V + (j - 1) * ldv : nullptr,
-                                                                V + j * ldv);
-                k_mgs<double><<<grid, 256, 0, ctx->stream>>>(G, n, i, i + 1, wn, V + i * ldv, nullptr);
+                mgs_step<double>(ctx, G, n, i, wn, V, ldv);
                 FG_HIP(hipGetLastError());
                 FG_HIP(hipEventRecord(ev[i & 1], ctx->stream));
                 std::swap(wc, wn);
@@ -503,7 +566,7 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
         FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wA, wA, nullptr));
         for (int cyc = 0; cyc < std::max(1, opt->inner_cycles); ++cyc) {
             if (cyc > 0) break;   // one inner cycle per refinement step
-            k_cycle_start<float><<<grid, 256, 0, ctx->stream>>>(G, n, wA, nullptr, 0, 1, opt->inner_tol);
+            cycle_start<float>(ctx, G, n, wA, nullptr, 0, 1, opt->inner_tol);
             FG_HIP(hipGetLastError());
             int steps = 0;
             FG_TRY(run_cycle<float>(ctx, S, G, n, V, ldv, wA, wB, kin, &steps));
