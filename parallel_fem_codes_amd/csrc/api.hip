@@ -111,7 +111,7 @@ extern "C" int femgpu_create(int device, void *stream, femgpu_ctx **out)
         ctx->num_cus = prop.multiProcessorCount;
         ctx->lds_limit = prop.sharedMemPerBlock;
     }
-    int rc = ctx->partials.alloc(kReduceBlocks);
+    int rc = ctx->partials.alloc(2 * kReduceBlocks);
     if (!rc) rc = ctx->scalars.alloc(16);
     if (!rc) rc = ctx->err.alloc(4);
     if (!rc && hipMemset(ctx->err.p, 0, 4 * sizeof(int)) != hipSuccess) rc = FEMGPU_ERR_HIP;
@@ -192,6 +192,7 @@ extern "C" int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA,
     if (!rc) rc = build_stream_factor(ctx, A, IA, JA, diag.data());
     if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data(), false, A->fplan);
     if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data(), true, A->fplan_nat);
+    if (!rc) rc = build_jacobi_ell(ctx, A, IA, JA, diag.data());
     if (!rc && hipMemsetAsync(A->AA.p, 0, sizeof(double) * (nnz + 1), s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (rc) {

@@ -163,14 +163,15 @@ __global__ __launch_bounds__(256) void k_elem_steady(int nel, ElemIn in, double 
     mask[e] = m;
 }
 
-// Transient element kernel: Ae = Me + alpha*dt*Ke (9), loads fe (3), coupling terms
-// lift[3a+b] = me[a][b]*due[b] + ke[a][b]*ue[b] (Build_M_K_R_SUPG.c:148-184).
-__global__ __launch_bounds__(256) void k_elem_transient(int nel, ElemIn in, double *__restrict__ Ae,
-                                                        double *__restrict__ fe, double *__restrict__ lift)
+// Transient element evaluation (Build_M_K_R_SUPG.c:47-184): Ae = Me + alpha*dt*Ke, the loads and
+// the coupling terms lift[a][b] = me[a][b]*due[b] + ke[a][b]*ue[b].
+struct TransientElem {
+    double Ae[3][3], lift[3][3], Fe[3];
+};
+
+__device__ __forceinline__ void transient_elem(const ElemIn &in, int e, TransientElem &T)
 {
 #pragma clang fp contract(off)
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= nel) return;
     double X[3], Y[3], ue[3], due[3];
     for (int a = 0; a < 3; ++a) {
         const int v = in.ien[3 * e + a];
@@ -194,28 +195,45 @@ __global__ __launch_bounds__(256) void k_elem_transient(int nel, ElemIn in, doub
     const double C[3] = {sixth * (E.Be_x * E.y23 + E.Be_y * E.x32), sixth * (E.Be_x * E.y31 + E.Be_y * E.x13),
                          sixth * (E.Be_x * E.y12 + E.Be_y * E.x21)};
     const double sixA = sixth * E.Area, twA = twelfth * E.Area;
-    double Fe[3];
-    elem_loads(E, Fe);
-    for (int a = 0; a < 3; ++a) {
-        fe[3 * e + a] = Fe[a];
+    elem_loads(E, T.Fe);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
         for (int b = 0; b < 3; ++b) {
             const double me = ((a == b) ? sixA : twA) + E.tau * C[a];
-            Ae[9 * static_cast<size_t>(e) + 3 * a + b] = me + in.alpha_dt * E.ke[a][b];
-            lift[9 * static_cast<size_t>(e) + 3 * a + b] = me * due[b] + E.ke[a][b] * ue[b];
+            T.Ae[a][b] = me + in.alpha_dt * E.ke[a][b];
+            T.lift[a][b] = me * due[b] + E.ke[a][b] * ue[b];
         }
+}
+
+// Transient element kernel: Ae (9) and, per element row a, the record {fe_a, lift_a0, lift_a1,
+// lift_a2} that the residual gather reads in one piece (rec[4*(3e+a) ..]).
+__global__ __launch_bounds__(256) void k_elem_transient(int nel, ElemIn in, double *__restrict__ Ae,
+                                                        double *__restrict__ rec)
+{
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= nel) return;
+    TransientElem T;
+    transient_elem(in, e, T);
+    double2 *r = reinterpret_cast<double2 *>(rec + 12 * static_cast<size_t>(e));
+    for (int a = 0; a < 3; ++a) {
+        r[2 * a] = make_double2(T.Fe[a], T.lift[a][0]);
+        r[2 * a + 1] = make_double2(T.lift[a][1], T.lift[a][2]);
+        for (int b = 0; b < 3; ++b) Ae[9 * static_cast<size_t>(e) + 3 * a + b] = T.Ae[a][b];
     }
 }
 
 // RHS gathers. rhs_src entries are e*3+a in element order.
-__global__ void k_rhs_steady(int neq, const int *__restrict__ ptr, const int *__restrict__ src,
-                             const double *__restrict__ fe, const double *__restrict__ lift,
-                             const int *__restrict__ mask, double *__restrict__ F)
+__global__ void k_rhs_steady(int neq, const int *__restrict__ dst, const int *__restrict__ ptr,
+                             const int *__restrict__ src, const double *__restrict__ fe,
+                             const double *__restrict__ lift, const int *__restrict__ mask, double *__restrict__ F)
 {
 #pragma clang fp contract(off)
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= neq) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= neq) return;
     double f = 0;
-    for (int q = ptr[I]; q < ptr[I + 1]; ++q) {
+    for (int q = ptr[k]; q < ptr[k + 1]; ++q) {
         const int ea = src[q], e = ea / 3, a = ea % 3;
         f += fe[ea];
         const int m = mask[e];
@@ -226,43 +244,46 @@ __global__ void k_rhs_steady(int neq, const int *__restrict__ ptr, const int *__
             ++t;
         }
     }
-    F[I] = f;
+    F[dst[k]] = f;
 }
 
-__global__ void k_rhs_transient(int NEQ, const int *__restrict__ ptr, const int *__restrict__ src,
-                                const double *__restrict__ fe, const double *__restrict__ lift,
-                                double *__restrict__ R)
+__global__ void k_rhs_transient(int NEQ, const int *__restrict__ dst, const int *__restrict__ ptr,
+                                const int *__restrict__ src, const double *__restrict__ rec, double *__restrict__ R)
 {
 #pragma clang fp contract(off)
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I > NEQ) return;
-    if (I == NEQ) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > NEQ) return;
+    if (k == NEQ) {
         R[NEQ] = 0.0;
         return;
     }
     double r = 0;
-    for (int q = ptr[I]; q < ptr[I + 1]; ++q) {
-        const int ea = src[q];
-        r += fe[ea];
-        const double *t = lift + 9 * static_cast<size_t>(ea / 3) + 3 * (ea % 3);
-        r -= t[0];
-        r -= t[1];
-        r -= t[2];
+    for (int q = ptr[k]; q < ptr[k + 1]; ++q) {
+        const double2 *t = reinterpret_cast<const double2 *>(rec + 4 * static_cast<size_t>(src[q]));
+        const double2 f0 = t[0], f1 = t[1];   // load of row a, then its three coupling terms
+        r += f0.x;
+        r -= f0.y;
+        r -= f1.x;
+        r -= f1.y;
     }
-    R[I] = r;
+    R[dst[k]] = r;
 }
 
-// CSR values: AA[slot] = sum of its element entries in element order; AA[nnz] = 0.
-__global__ void k_aa_gather(int nnz, const int *__restrict__ ptr, const int *__restrict__ src,
-                            const double *__restrict__ Ke, double *__restrict__ AA)
+// CSR values: AA[slot] = sum of its element entries in element order (list k for slot dst[k]);
+// AA[nnz] = 0.
+__global__ void k_aa_gather(int nnz, const int *__restrict__ dst, const int *__restrict__ ptr,
+                            const int *__restrict__ src, const double *__restrict__ Ke, double *__restrict__ AA)
 {
 #pragma clang fp contract(off)
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > nnz) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nnz) return;
+    if (k == nnz) {
+        AA[nnz] = 0.0;
+        return;
+    }
     double v = 0;
-    if (s < nnz)
-        for (int q = ptr[s]; q < ptr[s + 1]; ++q) v += Ke[src[q]];
-    AA[s] = v;
+    for (int q = ptr[k]; q < ptr[k + 1]; ++q) v += Ke[src[q]];
+    AA[dst[k]] = v;
 }
 
 static ElemIn elem_in(femgpu_mesh *M, const femgpu_transport_problem *pb, const double *u, const double *du,
@@ -282,15 +303,31 @@ static ElemIn elem_in(femgpu_mesh *M, const femgpu_transport_problem *pb, const 
     return in;
 }
 
-void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vector<int> &ptr,
-                    std::vector<int> &src)
+// Lists in order of first source keep a gather's threads on neighbouring elements (the element
+// data they read is shared through the caches); each list's sum order is unchanged.
+void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vector<int> &ptr, std::vector<int> &src,
+             std::vector<int> &dst)
 {
+    std::vector<int> rank(static_cast<size_t>(nkeys), -1), count(static_cast<size_t>(nkeys), 0);
+    dst.clear();
+    dst.reserve(static_cast<size_t>(nkeys));
+    for (auto &p : pairs) {
+        if (rank[p.first] < 0) {
+            rank[p.first] = static_cast<int>(dst.size());
+            dst.push_back(p.first);
+        }
+        count[p.first]++;
+    }
+    for (int k = 0; k < nkeys; ++k)
+        if (rank[k] < 0) {
+            rank[k] = static_cast<int>(dst.size());
+            dst.push_back(k);
+        }
     ptr.assign(static_cast<size_t>(nkeys) + 1, 0);
-    for (auto &p : pairs) ptr[p.first + 1]++;
-    for (int k = 0; k < nkeys; ++k) ptr[k + 1] += ptr[k];
+    for (int k = 0; k < nkeys; ++k) ptr[static_cast<size_t>(k) + 1] = ptr[k] + count[dst[k]];
     src.resize(pairs.size());
     std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-    for (auto &p : pairs) src[fill[p.first]++] = p.second;
+    for (auto &p : pairs) src[fill[rank[p.first]]++] = p.second;
 }
 
 }  // namespace fg
@@ -338,11 +375,12 @@ extern "C" int femgpu_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq,
             }
         }
     if (rc == FEMGPU_ERR_ARG) set_error("femgpu_mesh_create: invP_id out of range");
-    std::vector<int> ptr, src;
+    std::vector<int> ptr, src, dst, rptr, rsrc, rdst;
     if (rc == FEMGPU_OK) {
-        csr_map(neq, pairs, ptr, src);
-        up(M->rhs_ptr, ptr.data(), ptr.size());
-        up(M->rhs_src, src.data(), src.size());
+        csr_map(neq, pairs, rptr, rsrc, rdst);
+        up(M->rhs_ptr, rptr.data(), rptr.size());
+        up(M->rhs_src, rsrc.data(), rsrc.size());
+        up(M->rhs_dst, rdst.data(), rdst.size());
     }
     if (rc == FEMGPU_OK && scheme) {
         pairs.clear();
@@ -354,13 +392,14 @@ extern "C" int femgpu_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq,
             }
         if (rc == FEMGPU_ERR_ARG) set_error("femgpu_mesh_create: scheme slot out of range");
         if (rc == FEMGPU_OK) {
-            csr_map(nnz, pairs, ptr, src);
+            csr_map(nnz, pairs, ptr, src, dst);
             up(M->aa_ptr, ptr.data(), ptr.size());
             up(M->aa_src, src.data(), src.size());
+            up(M->aa_dst, dst.data(), dst.size());
         }
     }
     if (rc == FEMGPU_OK) rc = M->fe.alloc(static_cast<size_t>(nel) * 3);
-    if (rc == FEMGPU_OK) rc = M->lift.alloc(static_cast<size_t>(nel) * 9);
+    if (rc == FEMGPU_OK) rc = M->lift.alloc(static_cast<size_t>(nel) * 12);   // steady: 6, transient: 12
     if (rc == FEMGPU_OK) rc = M->mask.alloc(static_cast<size_t>(nel));
     if (rc == FEMGPU_OK) rc = M->ke.alloc(static_cast<size_t>(nel) * 9);
     if (rc == FEMGPU_OK && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
@@ -393,12 +432,13 @@ extern "C" int femgpu_assemble_transport(femgpu_mesh *M, const femgpu_transport_
         FG_HIP(hipGetLastError());
     }
     if (M->neq > 0) {
-        k_rhs_steady<<<(M->neq + 255) / 256, 256, 0, ctx->stream>>>(M->neq, M->rhs_ptr.p, M->rhs_src.p, M->fe.p,
-                                                                    M->lift.p, M->mask.p, d_F);
+        k_rhs_steady<<<(M->neq + 255) / 256, 256, 0, ctx->stream>>>(M->neq, M->rhs_dst.p, M->rhs_ptr.p, M->rhs_src.p,
+                                                                    M->fe.p, M->lift.p, M->mask.p, d_F);
         FG_HIP(hipGetLastError());
     }
     if (d_AA) {
-        k_aa_gather<<<(M->nnz + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->nnz, M->aa_ptr.p, M->aa_src.p, Ke, d_AA);
+        k_aa_gather<<<(M->nnz + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->nnz, M->aa_dst.p, M->aa_ptr.p, M->aa_src.p, Ke,
+                                                                        d_AA);
         FG_HIP(hipGetLastError());
     }
     return FEMGPU_OK;
@@ -414,15 +454,16 @@ extern "C" int femgpu_assemble_transport_transient(femgpu_mesh *M, const femgpu_
     double *Ae = d_Ae ? d_Ae : M->ke.p;
     const ElemIn in = elem_in(M, pb, d_u, d_du, alpha * dt);
     if (M->nel > 0) {
-        k_elem_transient<<<(M->nel + 255) / 256, 256, 0, ctx->stream>>>(M->nel, in, Ae, M->fe.p, M->lift.p);
+        k_elem_transient<<<(M->nel + 255) / 256, 256, 0, ctx->stream>>>(M->nel, in, Ae, M->lift.p);
         FG_HIP(hipGetLastError());
     }
-    k_rhs_transient<<<(M->neq + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->neq, M->rhs_ptr.p, M->rhs_src.p, M->fe.p,
-                                                                       M->lift.p, d_R);
+    k_rhs_transient<<<(M->neq + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->neq, M->rhs_dst.p, M->rhs_ptr.p,
+                                                                       M->rhs_src.p, M->lift.p, d_R);
     FG_HIP(hipGetLastError());
     if (d_AA) {
         FG_CHECK(M->aa_ptr.p, FEMGPU_ERR_STATE, "femgpu_assemble_transport_transient: no CSR slot table");
-        k_aa_gather<<<(M->nnz + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->nnz, M->aa_ptr.p, M->aa_src.p, Ae, d_AA);
+        k_aa_gather<<<(M->nnz + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->nnz, M->aa_dst.p, M->aa_ptr.p, M->aa_src.p, Ae,
+                                                                        d_AA);
         FG_HIP(hipGetLastError());
     }
     return FEMGPU_OK;

@@ -123,28 +123,34 @@ __global__ __launch_bounds__(256) void k_elem_ns(int nel, NsIn in, double *__res
 
 // F[I] = sum of the residual entries of unknown I in element order (F[lm] += Res, :655-658);
 // F[NEQ] = 0. src entries index the per-element residual array (e*s+i).
-__global__ void k_rhs_gather(int NEQ, const int *__restrict__ ptr, const int *__restrict__ src,
-                             const double *__restrict__ Res, double *__restrict__ F)
+__global__ void k_rhs_gather(int NEQ, const int *__restrict__ dst, const int *__restrict__ ptr,
+                             const int *__restrict__ src, const double *__restrict__ Res, double *__restrict__ F)
 {
 #pragma clang fp contract(off)
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I > NEQ) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > NEQ) return;
+    if (k == NEQ) {
+        F[NEQ] = 0.0;
+        return;
+    }
     double f = 0.0;
-    if (I < NEQ)
-        for (int q = ptr[I]; q < ptr[I + 1]; ++q) f += Res[src[q]];
-    F[I] = f;
+    for (int q = ptr[k]; q < ptr[k + 1]; ++q) f += Res[src[q]];
+    F[dst[k]] = f;
 }
 
-__global__ void k_aa_gather_blocks(int nnz, const int *__restrict__ ptr, const int *__restrict__ src,
-                              const double *__restrict__ Ke, double *__restrict__ AA)
+__global__ void k_aa_gather_blocks(int nnz, const int *__restrict__ dst, const int *__restrict__ ptr,
+                                   const int *__restrict__ src, const double *__restrict__ Ke, double *__restrict__ AA)
 {
 #pragma clang fp contract(off)
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > nnz) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nnz) return;
+    if (k == nnz) {
+        AA[nnz] = 0.0;
+        return;
+    }
     double v = 0;
-    if (s < nnz)
-        for (int q = ptr[s]; q < ptr[s + 1]; ++q) v += Ke[src[q]];
-    AA[s] = v;
+    for (int q = ptr[k]; q < ptr[k + 1]; ++q) v += Ke[src[q]];
+    AA[dst[k]] = v;
 }
 
 }  // namespace fg
@@ -197,10 +203,13 @@ int block_mesh_create(femgpu_ctx *ctx, int ndof, int nnodes, int nel, int neq, c
             const int I = lm[static_cast<size_t>(sb) * e + i];
             if (I < neq) pairs.emplace_back(I, sb * e + i);
         }
-    std::vector<int> ptr, src;
-    csr_map(neq, pairs, ptr, src);
+    std::vector<int> ptr, src, dst;
+    csr_map(neq, pairs, ptr, src, dst);
     up(M->rhs_ptr, ptr.data(), ptr.size());
     up(M->rhs_src, src.data(), src.size());
+    up(M->rhs_dst, dst.data(), dst.size());
+    // the uploads are asynchronous: the host lists must outlive them (the aa map uses new vectors)
+    std::vector<int> aptr, asrc, adst;
     if (rc == FEMGPU_OK && scheme) {
         FG_CHECK(static_cast<long long>(nel) * ss < (1LL << 31), FEMGPU_ERR_ARG, "mesh create: too many element entries");
         pairs.clear();
@@ -212,9 +221,10 @@ int block_mesh_create(femgpu_ctx *ctx, int ndof, int nnodes, int nel, int neq, c
             }
         if (rc == FEMGPU_ERR_ARG) set_error("mesh create: scheme slot out of range");
         if (rc == FEMGPU_OK) {
-            csr_map(nnz, pairs, ptr, src);
-            up(M->aa_ptr, ptr.data(), ptr.size());
-            up(M->aa_src, src.data(), src.size());
+            csr_map(nnz, pairs, aptr, asrc, adst);
+            up(M->aa_ptr, aptr.data(), aptr.size());
+            up(M->aa_src, asrc.data(), asrc.size());
+            up(M->aa_dst, adst.data(), adst.size());
         }
     }
     if (rc == FEMGPU_OK) rc = M->fe.alloc(static_cast<size_t>(nel) * sb);
@@ -231,10 +241,12 @@ int block_mesh_create(femgpu_ctx *ctx, int ndof, int nnodes, int nel, int neq, c
 int launch_block_gathers(femgpu_mesh *M, const double *Ke, double *F, double *AA)
 {
     femgpu_ctx *ctx = M->ctx;
-    k_rhs_gather<<<(M->neq + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->neq, M->rhs_ptr.p, M->rhs_src.p, M->fe.p, F);
+    k_rhs_gather<<<(M->neq + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->neq, M->rhs_dst.p, M->rhs_ptr.p, M->rhs_src.p,
+                                                                    M->fe.p, F);
     FG_HIP(hipGetLastError());
     if (AA) {
-        k_aa_gather_blocks<<<(M->nnz + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->nnz, M->aa_ptr.p, M->aa_src.p, Ke, AA);
+        k_aa_gather_blocks<<<(M->nnz + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->nnz, M->aa_dst.p, M->aa_ptr.p,
+                                                                              M->aa_src.p, Ke, AA);
         FG_HIP(hipGetLastError());
     }
     return FEMGPU_OK;

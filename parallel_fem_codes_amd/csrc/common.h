@@ -127,8 +127,8 @@ struct StreamFactor {
     bool ok = false;
     DevBuf<uint4> codes;     // n: 32 codes of 4 bits, code (e, t) at bit 4*(4e+t)
     DevBuf<int4> meta;       // n: {p0, len, nL, row}
-    DevBuf<double> aval;     // n*kFactorRow: values in schedule order (gathered from AA)
-    DevBuf<double> out;      // n*kFactorRow: factored row
+    DevBuf<double> aval;     // kFactorRow*n: value e of schedule position q at e*n + q (from AA)
+    DevBuf<double> out;      // kFactorRow*n: factored rows, same layout
     DevBuf<double> dinv;     // n: inverse pivots in schedule order
     DevBuf<double> sink;     // idle-thread results
 };
@@ -149,7 +149,7 @@ struct femgpu_ctx {
     hipStream_t stream = nullptr;
     int num_cus = 256;
     size_t lds_limit = 65536;      // dynamic LDS available to one workgroup
-    fg::DevBuf<double> partials;   // per-block partial sums
+    fg::DevBuf<double> partials;   // per-block partial sums (two sets of kReduceBlocks)
     fg::DevBuf<double> scalars;    // small device scalars (dot results)
     fg::DevBuf<int> err;           // sticky device error flag (bounded waits)
     double *h_pinned = nullptr;    // pinned host mirror for scalar results
@@ -182,6 +182,9 @@ struct femgpu_csr {
     fg::DevBuf<float> LU32, Dinv32;     // FP32 factors in A's layout (mixed solver's Jacobi sweeps)
     fg::DevBuf<float> LU32b, Dinv32b;   // second iterate of the parallel factorisation
     fg::DevBuf<float> jac_a, jac_b;     // Jacobi sweep iterates
+    int ell_kl = 0, ell_ku = 0;         // Jacobi ELL widths of the strict triangles (0: CSR kernels)
+    fg::DevBuf<int> ell_cl, ell_cu;     // ELL columns, entry k of row i at k*n + i (padding: column i)
+    fg::DevBuf<float> ell_vl, ell_vu;   // their FP32 factor values (padding 0), refreshed with LU32
     fg::FactorPlan fplan;           // numeric factorisation (level schedule)
     fg::FactorPlan fplan_nat;       // the same lists in row order (parallel FP32 factorisation)
     fg::StreamFactor sfac;          // streamed single-workgroup factorisation (when available)
@@ -205,6 +208,7 @@ struct femgpu_mesh {
     fg::DevBuf<int> type, invP, ien, lm;   // type/invP per dof, lm[nel*3*ndof]
     fg::DevBuf<int> rhs_ptr, rhs_src;   // unknown -> (element*s + i) contributions, element order
     fg::DevBuf<int> aa_ptr, aa_src;     // CSR slot -> (element*s*s + ij) contributions, element order
+    fg::DevBuf<int> rhs_dst, aa_dst;    // the lists are ordered by first element: list k -> unknown / slot
     fg::DevBuf<double> fe, lift;        // per element: s loads / residual entries, 9 coupling terms
     fg::DevBuf<int> mask;               // per element: Dirichlet-vertex bit mask
     fg::DevBuf<double> ke;              // element matrices when the caller passes none
@@ -240,6 +244,7 @@ int launch_sweep32(femgpu_ctx *ctx, femgpu_csr *A, const float *b, float *x, con
 int launch_ilu_factor(femgpu_ctx *ctx, femgpu_csr *A);
 int launch_parilu32(femgpu_ctx *ctx, femgpu_csr *A, int sweeps);
 int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A);
+int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
 int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip);
 int launch_to_f32(femgpu_ctx *ctx, const double *src, float *dst, size_t n);
 int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *Q);
@@ -252,7 +257,10 @@ int block_mesh_create(femgpu_ctx *ctx, int ndof, int nnodes, int nel, int neq, c
                       const int *scheme, int nnz, femgpu_mesh **out);
 int launch_block_gathers(femgpu_mesh *M, const double *Ke, double *F, double *AA);
 // Counting sort of (key, source) pairs by key, keeping source order inside a key (assembly maps).
-void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vector<int> &ptr, std::vector<int> &src);
+// Gather map of (key, source) pairs given in source order: the keys' lists in order of their first
+// source (empty keys last), dst[k] = key of list k, each list in the pairs' order.
+void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vector<int> &ptr, std::vector<int> &src,
+             std::vector<int> &dst);
 int check_device_error(femgpu_ctx *ctx);
 
 }  // namespace fg

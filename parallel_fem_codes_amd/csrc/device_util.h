@@ -1,5 +1,5 @@
 // Device helpers shared by the kernels: 64-wide wave reductions, block reductions and the
-// last-block detection of single-pass grid reductions.
+// LDS-only barrier of the single-workgroup sweeps.
 #pragma once
 
 #include <hip/hip_runtime.h>

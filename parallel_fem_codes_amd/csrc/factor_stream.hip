@@ -5,7 +5,10 @@
 // reads its dependencies from LDS only. The row's own data -- values padded to kFactorRow entries,
 // the column-match codes of its (entry, L entry) pairs, its length and L count, the ring slots of
 // its L entries -- is in schedule order and fetched into registers D steps ahead; the results are
-// written in schedule order and scattered to A's layout afterwards by a full-device kernel.
+// written in schedule order and scattered to A's layout afterwards by a full-device kernel. Values
+// and results are stored entry-major (entry e of position q at e*n + q), so each load or store of a
+// step's entry is one contiguous run across the threads: one workgroup issues all of the kernel's
+// memory traffic, and a strided layout costs it a cache line per lane.
 //
 // Arithmetic (ILUp_precond_setup.c:84-166, p = 0; contraction off): entries in column order; entry e
 // starts from a_e and subtracts l_t * u_{j_t, col_e} for the L entries t < e in order; an L entry
@@ -16,10 +19,10 @@
 
 namespace fg {
 
-constexpr int kFactorSink = 2 * kFactorRow;   // doubles per idle thread (row and pivot; 16-byte aligned rows)
+constexpr int kFactorSink = kFactorRow + 1;   // sink planes: row entries and pivot, kStreamThreads doubles each
 
 struct FactorArgs {
-    int nsteps, W;
+    int n, nsteps, W;
     const int *stepq;
     const unsigned short *slot;   // lower sweep plan: ring slots of the L entries (4 per position)
     const uint4 *codes;
@@ -46,12 +49,8 @@ __device__ __forceinline__ void factor_fetch(FactorStage &g, const FactorArgs &A
 {
     const int q = factor_pos(sq, s);
     const int qq = q < 0 ? 0 : q;
-    const float4 *pa = reinterpret_cast<const float4 *>(A.aval + static_cast<size_t>(qq) * kFactorRow);
 #pragma unroll
-    for (int i = 0; i < kFactorRow / 2; ++i) {
-        const float4 w = pa[i];
-        __builtin_memcpy(&g.a[2 * i], &w, 16);
-    }
+    for (int e = 0; e < kFactorRow; ++e) g.a[e] = A.aval[static_cast<size_t>(e) * A.n + qq];
     g.codes = A.codes[qq];
     const int4 m = A.meta[qq];
     g.len = m.y;
@@ -81,7 +80,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_factor_stream(FactorArgs A)
     if (threadIdx.x < kFactorRing) ring[kFactorRing * A.W + threadIdx.x] = 0.0;   // slot W: zeros
     __syncthreads();
     const int wmask = A.W - 1;
-    double *sink = A.sink + static_cast<size_t>(threadIdx.x) * kFactorSink;
+    double *sink = A.sink + threadIdx.x;
     FactorStage st[D];
 #pragma unroll
     for (int u = 0; u < D; ++u) {
@@ -147,27 +146,24 @@ __global__ __launch_bounds__(kStreamThreads) void k_factor_stream(FactorArgs A)
                 if (uo >= 0 && e < g.len) ring[kFactorRing * self + uo] = out[e];
             }
             ring[kFactorRing * self + kFactorU] = dinv;
-            double *o = on ? A.out + static_cast<size_t>(g.q) * kFactorRow : sink;
+            double *o = on ? A.out + g.q : sink;
+            const size_t plane = on ? static_cast<size_t>(A.n) : static_cast<size_t>(kStreamThreads);
 #pragma unroll
-            for (int i = 0; i < kFactorRow / 2; ++i) {
-                float4 w;
-                __builtin_memcpy(&w, &out[2 * i], 16);
-                reinterpret_cast<float4 *>(o)[i] = w;
-            }
-            *(on ? A.dinv + g.q : sink + kFactorRow) = dinv;
+            for (int e = 0; e < kFactorRow; ++e) o[e * plane] = out[e];
+            *(on ? A.dinv + g.q : sink + kFactorRow * kStreamThreads) = dinv;
             factor_fetch(g, A, sq, s0 + u + D);
             lds_barrier();
         }
     }
 }
 
-// aval[q*8 + e] = A's value e of the row at schedule position q (0 past the row's end).
+// aval[e*n + q] = A's value e of the row at schedule position q (0 past the row's end).
 __global__ void k_factor_gather(int n, const int4 *__restrict__ meta, const double *__restrict__ AA,
                                 double *__restrict__ aval)
 {
     const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
     if (k >= static_cast<size_t>(n) * kFactorRow) return;
-    const int q = static_cast<int>(k / kFactorRow), e = static_cast<int>(k % kFactorRow);
+    const int e = static_cast<int>(k / n), q = static_cast<int>(k % n);
     const int4 m = meta[q];
     aval[k] = e < m.y ? AA[m.x + e] : 0.0;
 }
@@ -178,7 +174,7 @@ __global__ void k_factor_scatter(int n, const int4 *__restrict__ meta, const dou
 {
     const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
     if (k >= static_cast<size_t>(n) * kFactorRow) return;
-    const int q = static_cast<int>(k / kFactorRow), e = static_cast<int>(k % kFactorRow);
+    const int e = static_cast<int>(k / n), q = static_cast<int>(k % n);
     const int4 m = meta[q];
     if (e < m.y) LU[m.x + e] = out[k];
     if (e == 0) Dinv[m.w] = dq[q];
@@ -194,6 +190,7 @@ int launch_stream_factor(femgpu_ctx *ctx, femgpu_csr *A)
     FG_TRY(F.sink.alloc(static_cast<size_t>(kStreamThreads) * kFactorSink));
     k_factor_gather<<<grid, 256, 0, ctx->stream>>>(n, F.meta.p, A->AA.p, F.aval.p);
     FactorArgs a;
+    a.n = n;
     a.nsteps = P.nsteps;
     a.W = P.W;
     a.stepq = P.stepq.p;

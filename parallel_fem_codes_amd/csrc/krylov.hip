@@ -83,13 +83,27 @@ struct Vec16<double> {
     }
 };
 
-// Grid reductions run as two kernels: per-block partials (plain stores; the kernel boundary makes
-// them visible), then one block that sums them in a fixed order and acts on the sum. No atomics or
-// device-scope fences, and the result does not depend on block timing.
+// Grid reductions leave per-block partials (plain stores; the kernel boundary makes them visible)
+// that a later kernel sums in a fixed order, so results do not depend on block timing and need no
+// atomics or device-scope fences.
 
-// MGS pass j of step i on w (vector type T, accumulation in FP64), first kernel:
-//   uprev != null : w -= h_{j-1,i} * uprev
-//   unext != null : partial of h_{j,i} = <unext, w>   (unext == null: partial of ||w||^2)
+// Sum of the nb partials in a fixed order by one block of 256 threads (valid in every thread).
+__device__ __forceinline__ double sum_partials(const double *partials, int nb, double *red)
+{
+    double v = 0;
+    for (int b = threadIdx.x; b < nb; b += 256) v += partials[b];
+    v = block_sum<256>(v, red);
+    __shared__ double total;
+    if (threadIdx.x == 0) total = v;
+    __syncthreads();
+    return total;
+}
+
+// MGS pass j of step i on w (vector type T, accumulation in FP64); passes alternate between the two
+// partial sets, pass j writing set j & 1:
+//   uprev != null : w -= h_{j-1,i} * uprev, where every block forms h_{j-1,i} from pass j-1's
+//                   partials (the same fixed-order sum in every block) and block 0 stores it
+//   unext != null : partials of h_{j,i} = <unext, w>   (unext == null: partials of ||w||^2)
 template <typename T>
 __global__ __launch_bounds__(256) void k_mgs(GDev G, int n, int i, int j, T *__restrict__ w,
                                              const T *__restrict__ uprev, const T *__restrict__ unext)
@@ -98,7 +112,12 @@ __global__ __launch_bounds__(256) void k_mgs(GDev G, int n, int i, int j, T *__r
     using V = Vec16<T>;
     using VT = typename V::type;
     __shared__ double red[4];
-    const T h = uprev ? static_cast<T>(Hat(G, j - 1, i)) : T(0);
+    T h = T(0);
+    if (uprev) {
+        const double hp = sum_partials(G.partials + ((j - 1) & 1) * kReduceBlocks, gridDim.x, red);
+        if (blockIdx.x == 0 && threadIdx.x == 0) Hat(G, j - 1, i) = hp;
+        h = static_cast<T>(hp);
+    }
     double acc = 0;
     const int nv = n / V::N;
     for (int k = blockIdx.x * 256 + threadIdx.x; k < nv; k += gridDim.x * 256) {
@@ -124,33 +143,21 @@ __global__ __launch_bounds__(256) void k_mgs(GDev G, int n, int i, int j, T *__r
             acc += static_cast<double>(u) * static_cast<double>(wv);
         }
     acc = block_sum<256>(acc, red);
-    if (threadIdx.x == 0) G.partials[blockIdx.x] = acc;
+    if (threadIdx.x == 0) G.partials[(j & 1) * kReduceBlocks + blockIdx.x] = acc;
 }
 
-// Sum of the nb partials in a fixed order (one block of 256 threads; the sum is valid in thread 0).
-__device__ __forceinline__ double sum_partials(const double *partials, int nb, double *red)
-{
-    double v = 0;
-    for (int b = threadIdx.x; b < nb; b += 256) v += partials[b];
-    return block_sum<256>(v, red);
-}
-
-// MGS pass, second kernel: h_{j,i}, or the norm h_{i+1,i} followed by the Givens update.
-__global__ __launch_bounds__(256) void k_mgs_finish(GDev G, int nb, int i, int j, int has_next)
+// After the norm pass (j = i + 1): h_{i+1,i} = ||w||, then the Givens update of column i.
+__global__ __launch_bounds__(256) void k_mgs_finish(GDev G, int nb, int i)
 {
     if (G.flags[0]) return;
     __shared__ double red[4];
-    const double v = sum_partials(G.partials, nb, red);
+    const double v = sum_partials(G.partials + ((i + 1) & 1) * kReduceBlocks, nb, red);
     if (threadIdx.x == 0) {
-        if (has_next) {
-            Hat(G, j, i) = v;
-        } else {
-            const double hn = sqrt(v);
-            Hat(G, i + 1, i) = hn;
-            G.hnorm[i + 1] = hn;
-            G.invh[i + 1] = 1.0 / hn;
-            givens_column(G, i);
-        }
+        const double hn = sqrt(v);
+        Hat(G, i + 1, i) = hn;
+        G.hnorm[i + 1] = hn;
+        G.invh[i + 1] = 1.0 / hn;
+        givens_column(G, i);
     }
 }
 
@@ -243,7 +250,7 @@ __global__ void k_sqrt_inplace(double *v)
 
 static int grid_for(int n) { return std::max(1, std::min(kReduceBlocks, (n + 1023) / 1024)); }
 
-// The MGS passes of Arnoldi step i (j = 0..i, then the norm pass), two launches each.
+// The MGS passes of Arnoldi step i (j = 0..i, then the norm pass) and the column's finish.
 template <typename T>
 static void mgs_step(femgpu_ctx *ctx, const GDev &G, int n, int i, T *wn, const T *V, size_t ldv)
 {
@@ -252,8 +259,8 @@ static void mgs_step(femgpu_ctx *ctx, const GDev &G, int n, int i, T *wn, const 
         const T *uprev = j ? V + (j - 1) * ldv : nullptr;
         const T *unext = j <= i ? V + j * ldv : nullptr;
         k_mgs<T><<<grid, 256, 0, ctx->stream>>>(G, n, i, j, wn, uprev, unext);
-        k_mgs_finish<<<1, 256, 0, ctx->stream>>>(G, grid, i, j, unext != nullptr);
     }
+    k_mgs_finish<<<1, 256, 0, ctx->stream>>>(G, grid, i);
 }
 
 template <typename T>

@@ -17,6 +17,9 @@
 // preconditioning", Euro-Par 2015): y = L^-1 b by y_{k+1} = b - L_s y_k from y_0 = b, and
 // z = U^-1 y by z_{k+1} = Dinv (y - U_s z_k) from z_0 = Dinv y, a fixed number of iterations, so
 // the preconditioner is one fixed linear operator for the Krylov method.
+#include <algorithm>
+#include <vector>
+
 #include "common.h"
 
 namespace fg {
@@ -137,6 +140,100 @@ static int ensure_par(femgpu_csr *A)
     return FEMGPU_OK;
 }
 
+// The strict triangles of the FP32 factors as ELL, column-major (entry k of row i at k*n + i):
+// one thread per row, coalesced loads. Padding entries have value 0 and column i.
+constexpr int kEllMax = 8;
+
+__global__ void k_jacobi_ell_values(int n, const int *__restrict__ IA, const int *__restrict__ diag,
+                                    const float *__restrict__ LU, int kl, int ku, float *__restrict__ vl,
+                                    float *__restrict__ vu)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p0 = IA[i], d = diag[i], p1 = IA[i + 1];
+    for (int k = 0; k < kl; ++k) vl[static_cast<size_t>(k) * n + i] = p0 + k < d ? LU[p0 + k] : 0.0f;
+    for (int k = 0; k < ku; ++k) vu[static_cast<size_t>(k) * n + i] = d + 1 + k < p1 ? LU[d + 1 + k] : 0.0f;
+}
+
+// y_out = b - L_s y_in (y_in == nullptr: y_in = b), ELL form.
+template <int K>
+__global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, int kl, const int *__restrict__ cl,
+                                                          const float *__restrict__ vl, const float *__restrict__ b,
+                                                          const float *__restrict__ yin, float *__restrict__ yout,
+                                                          const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float *src = yin ? yin : b;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (k < kl) {
+            const size_t e = static_cast<size_t>(k) * n + i;
+            acc += vl[e] * src[cl[e]];
+        }
+    yout[i] = b[i] - acc;
+}
+
+// z_out = Dinv (y - U_s z_in) (z_in == nullptr: z_in = Dinv y), ELL form.
+template <int K>
+__global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, int ku, const int *__restrict__ cu,
+                                                          const float *__restrict__ vu, const float *__restrict__ Dinv,
+                                                          const float *__restrict__ y, const float *__restrict__ zin,
+                                                          float *__restrict__ zout, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (k < ku) {
+            const size_t e = static_cast<size_t>(k) * n + i;
+            const int j = cu[e];
+            acc += vu[e] * (zin ? zin[j] : Dinv[j] * y[j]);
+        }
+    zout[i] = Dinv[i] * (y[i] - acc);
+}
+
+int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag)
+{
+    const int n = A->n;
+    int kl = 0, ku = 0;
+    for (int i = 0; i < n; ++i) {
+        kl = std::max(kl, diag[i] - IA[i]);
+        ku = std::max(ku, IA[i + 1] - diag[i] - 1);
+    }
+    A->ell_kl = A->ell_ku = 0;
+    if (n == 0 || kl > kEllMax || ku > kEllMax) return FEMGPU_OK;
+    kl = std::max(kl, 1);
+    ku = std::max(ku, 1);
+    std::vector<int> cl(static_cast<size_t>(kl) * n), cu(static_cast<size_t>(ku) * n);
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < kl; ++k) cl[static_cast<size_t>(k) * n + i] = IA[i] + k < diag[i] ? JA[IA[i] + k] : i;
+        for (int k = 0; k < ku; ++k)
+            cu[static_cast<size_t>(k) * n + i] = diag[i] + 1 + k < IA[i + 1] ? JA[diag[i] + 1 + k] : i;
+    }
+    FG_TRY(A->ell_cl.upload(cl.data(), cl.size(), ctx->stream));
+    FG_TRY(A->ell_cu.upload(cu.data(), cu.size(), ctx->stream));
+    FG_TRY(A->ell_vl.alloc(cl.size()));
+    FG_TRY(A->ell_vu.alloc(cu.size()));
+    FG_HIP(hipStreamSynchronize(ctx->stream));   // the host arrays end with this call
+    A->ell_kl = kl;
+    A->ell_ku = ku;
+    return FEMGPU_OK;
+}
+
+static int refresh_ell(femgpu_ctx *ctx, femgpu_csr *A)
+{
+    if (A->ell_kl == 0) return FEMGPU_OK;
+    k_jacobi_ell_values<<<(A->n + 255) / 256, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->diag.p, A->LU32.p, A->ell_kl,
+                                                                      A->ell_ku, A->ell_vl.p, A->ell_vu.p);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
 int launch_parilu32(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
 {
     if (A->n == 0) return FEMGPU_OK;
@@ -159,23 +256,52 @@ int launch_parilu32(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
         std::swap(L0, L1);
         std::swap(D0, D1);
     }
-    return FEMGPU_OK;
+    return refresh_ell(ctx, A);
 }
 
 int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A)
 {
     FG_TRY(ensure_par(A));
     FG_TRY(launch_to_f32(ctx, A->LU.p, A->LU32.p, A->nnz));
-    return launch_to_f32(ctx, A->Dinv.p, A->Dinv32.p, A->n);
+    FG_TRY(launch_to_f32(ctx, A->Dinv.p, A->Dinv32.p, A->n));
+    return refresh_ell(ctx, A);
 }
 
 // x = (U~ L~)^-1 b approximated with it Jacobi iterations per triangle on the FP32 factors in
 // LU32/Dinv32. b may alias x.
+template <int K>
+static void jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip)
+{
+    const int n = A->n, grid = (n + 255) / 256;
+    float *t[2] = {A->jac_a.p, A->jac_b.p};
+    const float *prev = nullptr;
+    for (int k = 0; k < it; ++k) {
+        float *out = t[k & 1];
+        k_jacobi_lower_ell<K><<<grid, 256, 0, ctx->stream>>>(n, A->ell_kl, A->ell_cl.p, A->ell_vl.p, b, prev, out, skip);
+        prev = out;
+    }
+    const float *y = prev;
+    float *other = (y == t[0]) ? t[1] : t[0];
+    prev = nullptr;
+    for (int k = 1; k <= it; ++k) {
+        float *out = ((it - k) % 2 == 0) ? x : other;
+        k_jacobi_upper_ell<K><<<grid, 256, 0, ctx->stream>>>(n, A->ell_ku, A->ell_cu.p, A->ell_vu.p, A->Dinv32.p, y,
+                                                             prev, out, skip);
+        prev = out;
+    }
+}
+
 int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip)
 {
     const int n = A->n;
     if (n == 0) return FEMGPU_OK;
     FG_CHECK(it >= 1 && it <= 1024, FEMGPU_ERR_ARG, "Jacobi sweeps: %d iterations out of range", it);
+    if (A->ell_kl > 0) {
+        if (std::max(A->ell_kl, A->ell_ku) <= 4) jacobi_ell<4>(ctx, A, it, b, x, skip);
+        else jacobi_ell<kEllMax>(ctx, A, it, b, x, skip);
+        FG_HIP(hipGetLastError());
+        return FEMGPU_OK;
+    }
     constexpr int LPR = 4;
     const int grid = static_cast<int>((static_cast<long>(n) * LPR + 255) / 256);
     float *t[2] = {A->jac_a.p, A->jac_b.p};

@@ -257,26 +257,35 @@ def test_assembly_steady_bit_exact(dev, torch_cuda, oracle, shock, m, right):
     assert np.array_equal(H(AA), s["AA"])
 
 
-def test_assembly_transient(dev, torch_cuda, oracle):
+@pytest.mark.parametrize("m,n,reorder", [(40, 40, "SYMRCM"), (13, 7, "NOT")])
+def test_assembly_transient(dev, torch_cuda, oracle, m, n, reorder):
+    """Build_M_K_R_SUPG: element matrices, residual and CSR values (without element matrices),
+    bit-identical to the oracle (csr_assembly order)."""
     torch = torch_cuda
-    m = 40
-    x, y, ien = oracle.mesh_structured(m, m)
+    x, y, ien = oracle.mesh_structured(m, n)
     t = oracle.unit_square_types(x, y)
     invP0, neq = oracle.number_unknowns(t)
     lm0 = oracle.fill_lm(ien, t, invP0, neq)
-    c = oracle.csr_init(lm0, 3, neq, reorder="SYMRCM", permute_lm=True, type_=t, invP=invP0)
+    c = oracle.csr_init(lm0, 3, neq, reorder=reorder, permute_lm=True, type_=t, invP=invP0)
     ubc = oracle.inflow_ubc(x, y)
     rng = np.random.default_rng(8)
     u, du = rng.standard_normal(neq) * 0.1, rng.standard_normal(neq) * 0.1
     for shock in (0, 1):
         pb = oracle.make_problem(k=1e-4, shock=shock)
         Ae, R = oracle.build_transport_transient(x, y, t, c["invP"], ien, c["lm"], ubc, u, du, pb, 0.5, 1e-3, neq)
+        AAo = oracle.csr_assemble(c["scheme"], Ae, c["nnz"])
         M = fem.Mesh(dev, x, y, t, c["invP"], ubc, ien, neq, lm=c["lm"], scheme=c["scheme"], nnz=c["nnz"])
         Rg = torch.empty(neq + 1, dtype=torch.float64, device="cuda")
         Aeg = torch.empty(9 * (len(ien) // 3), dtype=torch.float64, device="cuda")
         M.Build_M_K_R_SUPG(gpu_problem(pb), 0.5, 1e-3, T(torch, u), T(torch, du), Rg, Ae=Aeg)
         dev.sync()
         assert np.array_equal(H(Aeg), Ae)
+        assert np.array_equal(H(Rg), R)
+        AAg = torch.full((c["nnz"] + 1,), float("nan"), dtype=torch.float64, device="cuda")
+        Rg.fill_(float("nan"))
+        M.Build_M_K_R_SUPG(gpu_problem(pb), 0.5, 1e-3, T(torch, u), T(torch, du), Rg, AA_ptr=AAg.data_ptr())
+        dev.sync()
+        assert np.array_equal(H(AAg), AAo)
         assert np.array_equal(H(Rg), R)
 
 
@@ -430,12 +439,14 @@ def test_parallel_ilu_fixed_point_is_ilu0(dev, torch_cuda, oracle, name):
         assert np.abs(Di - Dir).max() <= 1e-5 * np.abs(Dir).max(), sweeps
 
 
-@pytest.mark.parametrize("name", ["st_csr_rcm_teste_r8", "st_csr_rcm_cau_r24"])
+@pytest.mark.parametrize("name", ["st_csr_rcm_teste_r8", "st_csr_rcm_cau_r24", "synthetic12s"])
 def test_jacobi_sweeps(dev, torch_cuda, oracle, name):
     """Jacobi iterations on the triangular factors: iteration by iteration against the sparse
-    restatement, and exact (= the level-scheduled sweeps) after depth iterations."""
+    restatement, and exact (= the level-scheduled sweeps) after depth iterations. Mesh rows run
+    the ELL kernels; the synthetic rows (up to 12 entries per triangle) the CSR kernels."""
+    import scipy.sparse as sp
     torch = torch_cuda
-    s = ilu_case(oracle, name)
+    s = synthetic_csr(2000, 12, 5) if name == "synthetic12s" else ilu_case(oracle, name)
     depth = fem.ilu_levels(s["IA"], s["JA"], True)[1]
     A = fem.CsrMatrix(dev, s["IA"], s["JA"])
     A.set_values(s["AA"][:-1])
@@ -444,11 +455,9 @@ def test_jacobi_sweeps(dev, torch_cuda, oracle, name):
     LUo, Dio, rc = oracle.ilu0_factor(s["IA"], s["JA"], s["AA"])
     n, IA, JA = s["neq"], s["IA"], s["JA"]
     rows = np.repeat(np.arange(n), np.diff(IA))
-    Ls = np.zeros((n, n))
-    Us = np.zeros((n, n))
     low, up = JA < rows, JA > rows
-    Ls[rows[low], JA[low]] = LUo[:-1][low]
-    Us[rows[up], JA[up]] = LUo[:-1][up]
+    Ls = sp.csr_matrix((LUo[:-1][low], (rows[low], JA[low])), shape=(n, n))
+    Us = sp.csr_matrix((LUo[:-1][up], (rows[up], JA[up])), shape=(n, n))
     b = np.random.default_rng(9).standard_normal(n)
     exact = oracle.ilu0_apply(IA, JA, LUo, Dio, b)
     for it in (1, 2, depth):
