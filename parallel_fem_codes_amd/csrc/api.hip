@@ -193,6 +193,7 @@ extern "C" int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA,
     if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data(), false, A->fplan);
     if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data(), true, A->fplan_nat);
     if (!rc) rc = build_jacobi_ell(ctx, A, IA, JA, diag.data());
+    if (!rc) rc = build_sell(ctx, A, IA, JA);
     if (!rc && hipMemsetAsync(A->AA.p, 0, sizeof(double) * (nnz + 1), s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (rc) {
@@ -231,6 +232,7 @@ extern "C" int femgpu_csr_values_changed(femgpu_csr *A)
     FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_csr_values_changed: null");
     A->factored = false;
     A->f32_factored = false;
+    FG_TRY(launch_sell_values(A->ctx, A));
     return launch_to_f32(A->ctx, A->AA.p, A->AA32.p, A->nnz);
 }
 

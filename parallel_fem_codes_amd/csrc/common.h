@@ -168,6 +168,10 @@ struct femgpu_csr {
     fg::DevBuf<int> IA, JA, diag;   // diag[i] = position of the diagonal entry of row i
     fg::DevBuf<double> AA;          // nnz+1 (dummy slot nnz)
     fg::DevBuf<float> AA32;
+    int sell_ok = 0;                    // FP32 sliced-ELL copy present (slices of 64 rows)
+    fg::DevBuf<int> sell_ptr;           // slice -> first entry (nslices + 1); slice width = difference / 64
+    fg::DevBuf<int> sell_col;           // entry k of row 64s + l at sell_ptr[s] + 64k + l (padding: the row)
+    fg::DevBuf<float> sell_val;         // FP32 values in that layout (padding 0), refreshed with AA32
     fg::DevBuf<double> LU, Dinv;    // ILU(0) factors in A's layout (FP32 copies live in the packed plans)
     bool factored = false;
     fg::PackedSweep plo, pup;       // packed lower / upper sweeps
@@ -246,6 +250,8 @@ int launch_parilu32(femgpu_ctx *ctx, femgpu_csr *A, int sweeps);
 int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A);
 int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
 int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip);
+int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA);
+int launch_sell_values(femgpu_ctx *ctx, femgpu_csr *A);
 int launch_to_f32(femgpu_ctx *ctx, const double *src, float *dst, size_t n);
 int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *Q);
 int launch_ebe_gather_values(femgpu_ctx *ctx, femgpu_ebe *E, const double *A_elem_order);

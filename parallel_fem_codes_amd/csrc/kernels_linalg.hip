@@ -2,7 +2,9 @@
 // level-scheduled ILU(0) sweeps and numeric factorisation, the coloured EBE matvec, and the
 // reduction / vector kernels. All memory-bound: no MFMA (3x3..12x12 blocks and CSR rows are not
 // dense contractions).
+#include <algorithm>
 #include <climits>
+#include <vector>
 
 #include "common.h"
 #include "device_util.h"
@@ -36,6 +38,84 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ IA,
         y[row] = acc;
         if (xout) xout[row] = x[row] * xs;
     }
+}
+
+// FP32 SpMV on the sliced-ELL copy: one row per thread, a wave's 64 rows form one slice padded to
+// its longest row, entries column-major inside the slice, so every load of a step of the row loop
+// is one contiguous 256-byte run and the loop count is uniform across the wave. Same xs / xout /
+// skip contract as k_spmv.
+__global__ __launch_bounds__(256) void k_spmv_sell(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+                                                   const float *__restrict__ val, const float *__restrict__ x,
+                                                   const double *__restrict__ xs_ptr, float *__restrict__ xout,
+                                                   float *__restrict__ y, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int row = blockIdx.x * 256 + threadIdx.x;
+    if (row >= n) return;
+    const float xs = xs_ptr ? static_cast<float>(*xs_ptr) : 1.0f;
+    const int s = row >> 6, l = row & 63;
+    const int p0 = sptr[s], w = (sptr[s + 1] - p0) >> 6;
+    float acc = 0.0f;
+    for (int k = 0; k < w; ++k) {
+        const int e = p0 + (k << 6) + l;
+        acc += val[e] * (x[col[e]] * xs);
+    }
+    y[row] = acc;
+    if (xout) xout[row] = x[row] * xs;
+}
+
+__global__ void k_sell_values(int n, const int *__restrict__ IA, const int *__restrict__ sptr,
+                              const double *__restrict__ AA, float *__restrict__ val)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    const int s = row >> 6, l = row & 63;
+    const int p0 = sptr[s], w = (sptr[s + 1] - p0) >> 6;
+    const int a0 = IA[row], len = IA[row + 1] - a0;
+    for (int k = 0; k < w; ++k) val[p0 + (k << 6) + l] = k < len ? static_cast<float>(AA[a0 + k]) : 0.0f;
+}
+
+int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA)
+{
+    const int n = A->n;
+    A->sell_ok = 0;
+    if (n == 0) return FEMGPU_OK;
+    const int ns = (n + 63) / 64;
+    std::vector<int> sptr(static_cast<size_t>(ns) + 1, 0);
+    long long total = 0;
+    for (int s = 0; s < ns; ++s) {
+        int w = 0;
+        for (int r = 64 * s; r < std::min(n, 64 * s + 64); ++r) w = std::max(w, IA[r + 1] - IA[r]);
+        total += 64LL * w;
+        if (total >= INT_MAX) return FEMGPU_OK;   // int entry offsets: stay with CSR
+        sptr[static_cast<size_t>(s) + 1] = static_cast<int>(total);
+    }
+    if (total > 2LL * IA[n] + 64LL * ns) return FEMGPU_OK;   // rows too uneven: padding would dominate
+    std::vector<int> col(static_cast<size_t>(total));
+    for (int s = 0; s < ns; ++s) {
+        const int w = (sptr[s + 1] - sptr[s]) / 64;
+        for (int l = 0; l < 64; ++l) {
+            const int r = 64 * s + l;
+            for (int k = 0; k < w; ++k) {
+                const bool has = r < n && k < IA[r + 1] - IA[r];
+                col[static_cast<size_t>(sptr[s]) + 64 * k + l] = has ? JA[IA[r] + k] : std::min(r, n - 1);
+            }
+        }
+    }
+    FG_TRY(A->sell_ptr.upload(sptr.data(), sptr.size(), ctx->stream));
+    FG_TRY(A->sell_col.upload(col.data(), col.size(), ctx->stream));
+    FG_TRY(A->sell_val.alloc(col.size()));
+    FG_HIP(hipStreamSynchronize(ctx->stream));   // the host arrays end with this call
+    A->sell_ok = 1;
+    return FEMGPU_OK;
+}
+
+int launch_sell_values(femgpu_ctx *ctx, femgpu_csr *A)
+{
+    if (!A->sell_ok) return FEMGPU_OK;
+    k_sell_values<<<(A->n + 255) / 256, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->sell_ptr.p, A->AA.p, A->sell_val.p);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
 }
 
 // r = f - A x (FP64 outer residual of the mixed solver).
@@ -82,6 +162,12 @@ int launch_spmv64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const d
 int launch_spmv32(femgpu_ctx *ctx, const femgpu_csr *A, const float *x, const double *xs, float *xout,
                   float *y, const int *skip)
 {
+    if (A->sell_ok && A->n > 0) {
+        k_spmv_sell<<<(A->n + 255) / 256, 256, 0, ctx->stream>>>(A->n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p, x,
+                                                                 xs, xout, y, skip);
+        FG_HIP(hipGetLastError());
+        return FEMGPU_OK;
+    }
     return spmv_dispatch<float>(ctx, A, A->AA32.p, x, xs, xout, y, skip);
 }
 
