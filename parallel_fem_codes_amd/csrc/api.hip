@@ -194,6 +194,7 @@ extern "C" int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA,
     if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data(), true, A->fplan_nat);
     if (!rc) rc = build_jacobi_ell(ctx, A, IA, JA, diag.data());
     if (!rc) rc = build_sell(ctx, A, IA, JA);
+    if (!rc) rc = build_parilu_sell(ctx, A, IA, JA, diag.data());
     if (!rc && hipMemsetAsync(A->AA.p, 0, sizeof(double) * (nnz + 1), s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (rc) {
@@ -338,6 +339,7 @@ extern "C" int femgpu_ilu0_get_f32(femgpu_csr *A, float *LU, float *Dinv)
 {
     FG_CHECK(A && LU && Dinv, FEMGPU_ERR_ARG, "femgpu_ilu0_get_f32: bad arguments");
     FG_CHECK(A->f32_factored, FEMGPU_ERR_STATE, "femgpu_ilu0_get_f32: no FP32 factors");
+    if (A->lu32_in_sell) FG_TRY(launch_parilu_to_csr(A->ctx, A));
     FG_HIP(hipMemcpyAsync(LU, A->LU32.p, sizeof(float) * A->nnz, hipMemcpyDeviceToHost, A->ctx->stream));
     FG_HIP(hipMemcpyAsync(Dinv, A->Dinv32.p, sizeof(float) * A->n, hipMemcpyDeviceToHost, A->ctx->stream));
     FG_HIP(hipStreamSynchronize(A->ctx->stream));

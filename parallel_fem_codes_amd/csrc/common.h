@@ -172,6 +172,16 @@ struct femgpu_csr {
     fg::DevBuf<int> sell_ptr;           // slice -> first entry (nslices + 1); slice width = difference / 64
     fg::DevBuf<int> sell_col;           // entry k of row 64s + l at sell_ptr[s] + 64k + l (padding: the row)
     fg::DevBuf<float> sell_val;         // FP32 values in that layout (padding 0), refreshed with AA32
+    std::vector<int> hsell_ptr;         // host copy of sell_ptr
+    // parallel FP32 ILU(0) on the sliced-ELL layout (rows of at most 16 entries)
+    int par_ok = 0, par_maxw = 0;
+    fg::DevBuf<int> par_uptr;           // slice -> first update (nslices + 1); updates per row = difference / 64
+    fg::DevBuf<int> par_usrc;           // update u of row 64s + l at par_uptr[s] + 64u + l: SELL position of u_kj
+    fg::DevBuf<unsigned char> par_ucode;   // dest entry | (L entry t) << 4; 0xff: padding
+    fg::DevBuf<int> par_rinfo;          // row -> nL | len << 8
+    fg::DevBuf<int> par_dpos;           // row -> SELL position of its diagonal
+    fg::DevBuf<float> par_a, par_b;     // iterates in SELL layout
+    bool lu32_in_sell = false;          // the current FP32 factors are in par_a (LU32 not written)
     fg::DevBuf<double> LU, Dinv;    // ILU(0) factors in A's layout (FP32 copies live in the packed plans)
     bool factored = false;
     fg::PackedSweep plo, pup;       // packed lower / upper sweeps
@@ -251,6 +261,8 @@ int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A);
 int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
 int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip);
 int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA);
+int build_parilu_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
+int launch_parilu_to_csr(femgpu_ctx *ctx, femgpu_csr *A);
 int launch_sell_values(femgpu_ctx *ctx, femgpu_csr *A);
 int launch_to_f32(femgpu_ctx *ctx, const double *src, float *dst, size_t n);
 int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *Q);

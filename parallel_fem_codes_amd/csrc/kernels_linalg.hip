@@ -102,6 +102,7 @@ int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA)
             }
         }
     }
+    A->hsell_ptr = sptr;
     FG_TRY(A->sell_ptr.upload(sptr.data(), sptr.size(), ctx->stream));
     FG_TRY(A->sell_col.upload(col.data(), col.size(), ctx->stream));
     FG_TRY(A->sell_val.alloc(col.size()));

@@ -18,6 +18,7 @@
 // z = U^-1 y by z_{k+1} = Dinv (y - U_s z_k) from z_0 = Dinv y, a fixed number of iterations, so
 // the preconditioner is one fixed linear operator for the Krylov method.
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "common.h"
@@ -128,6 +129,170 @@ __global__ __launch_bounds__(256) void k_jacobi_upper(int n, const int *__restri
     if (lane == 0) zout[row] = Dinv[row] * (y[row] - acc);
 }
 
+// ---- the same iteration on the sliced-ELL layout of the FP32 matrix (slices of 64 rows, entry k of
+// row 64s + l at sell_ptr[s] + 64k + l): every per-row load and store of a step is one contiguous
+// run across the wave, and the update lists are stored the same way (padded per slice).
+__global__ __launch_bounds__(256) void k_parilu_init_sell(int n, const int *__restrict__ sptr,
+                                                          const int *__restrict__ col, const float *__restrict__ AA,
+                                                          const int *__restrict__ rinfo, const int *__restrict__ dpos,
+                                                          float *__restrict__ LU, float *__restrict__ Dinv)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int s = i >> 6, l = i & 63, p0 = sptr[s] + l, w = (sptr[s + 1] - sptr[s]) >> 6;
+    const int nL = rinfo[i] & 0xff;
+    for (int k = 0; k < w; ++k) {
+        const int e = p0 + (k << 6);
+        LU[e] = k < nL ? AA[e] / AA[dpos[col[e]]] : AA[e];
+    }
+    Dinv[i] = 1.0f / AA[dpos[i]];
+}
+
+template <int MAXW>
+__global__ __launch_bounds__(256) void k_parilu_sweep_sell(int n, const int *__restrict__ sptr,
+                                                           const int *__restrict__ col, const float *__restrict__ AA,
+                                                           const int *__restrict__ rinfo,
+                                                           const int *__restrict__ uptr, const int *__restrict__ usrc,
+                                                           const unsigned char *__restrict__ ucode,
+                                                           const float *__restrict__ LUo, const float *__restrict__ Dio,
+                                                           float *__restrict__ LUn, float *__restrict__ Din)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int s = i >> 6, l = i & 63, p0 = sptr[s] + l, w = (sptr[s + 1] - sptr[s]) >> 6;
+    const int u0 = uptr[s] + l, nu = (uptr[s + 1] - uptr[s]) >> 6;
+    const int nL = rinfo[i] & 0xff;
+    float v[MAXW], lo[MAXW];
+#pragma unroll
+    for (int k = 0; k < MAXW; ++k) {
+        v[k] = k < w ? AA[p0 + (k << 6)] : 0.0f;
+        lo[k] = k < w ? LUo[p0 + (k << 6)] : 0.0f;
+    }
+    for (int u = 0; u < nu; ++u) {
+        const unsigned code = ucode[u0 + (u << 6)];
+        const float other = LUo[usrc[u0 + (u << 6)]];
+        if (code != 0xffu) {
+            const unsigned t = code >> 4, dest = code & 15u;
+            float lt = 0.0f;
+#pragma unroll
+            for (int k = 0; k < MAXW; ++k) lt = (k == static_cast<int>(t)) ? lo[k] : lt;
+            const float c = lt * other;
+#pragma unroll
+            for (int k = 0; k < MAXW; ++k)
+                if (k == static_cast<int>(dest)) v[k] -= c;
+        }
+    }
+    float piv = 1.0f;
+#pragma unroll
+    for (int k = 0; k < MAXW; ++k)
+        if (k < w) {
+            const int e = p0 + (k << 6);
+            const float out = k < nL ? v[k] * Dio[col[e]] : v[k];
+            LUn[e] = out;
+            if (k == nL) piv = out;
+        }
+    Din[i] = 1.0f / piv;
+}
+
+// The strict triangles of the SELL-layout factors into the Jacobi ELL arrays.
+__global__ void k_parilu_sell_to_ell(int n, const int *__restrict__ sptr, const int *__restrict__ rinfo,
+                                     const float *__restrict__ LU, int kl, int ku, float *__restrict__ vl,
+                                     float *__restrict__ vu)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p0 = sptr[i >> 6] + (i & 63), nL = rinfo[i] & 0xff, len = rinfo[i] >> 8;
+    for (int k = 0; k < kl; ++k) vl[static_cast<size_t>(k) * n + i] = k < nL ? LU[p0 + (k << 6)] : 0.0f;
+    for (int k = 0; k < ku; ++k) {
+        const int e = nL + 1 + k;
+        vu[static_cast<size_t>(k) * n + i] = e < len ? LU[p0 + (e << 6)] : 0.0f;
+    }
+}
+
+// The SELL-layout factors into A's layout (LU32), for callers that read the factors back.
+__global__ void k_parilu_sell_to_csr(int n, const int *__restrict__ sptr, const int *__restrict__ IA,
+                                     const float *__restrict__ LU, float *__restrict__ LU32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p0 = sptr[i >> 6] + (i & 63), a0 = IA[i], len = IA[i + 1] - a0;
+    for (int k = 0; k < len; ++k) LU32[a0 + k] = LU[p0 + (k << 6)];
+}
+
+int build_parilu_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag)
+{
+    A->par_ok = 0;
+    const int n = A->n;
+    if (!A->sell_ok || n == 0) return FEMGPU_OK;
+    const std::vector<int> &sp = A->hsell_ptr;
+    const int ns = static_cast<int>(sp.size()) - 1;
+    int maxw = 0;
+    for (int s = 0; s < ns; ++s) maxw = std::max(maxw, (sp[s + 1] - sp[s]) / 64);
+    if (maxw > 16) return FEMGPU_OK;
+    auto pos = [&](int row, int k) { return sp[row >> 6] + 64 * k + (row & 63); };
+    std::vector<int> rinfo(n), dpos(n), slot(n, -1);
+    std::vector<std::vector<std::pair<int, unsigned char>>> lists(64);
+    std::vector<int> uptr(static_cast<size_t>(ns) + 1, 0), usrc;
+    std::vector<unsigned char> ucode;
+    for (int s = 0; s < ns; ++s) {
+        size_t nu = 0;
+        for (int l = 0; l < 64; ++l) {
+            lists[l].clear();
+            const int i = 64 * s + l;
+            if (i >= n) continue;
+            const int p0 = IA[i], len = IA[i + 1] - p0, nL = diag[i] - p0;
+            if (nL > 15) return FEMGPU_OK;
+            rinfo[i] = nL | (len << 8);
+            dpos[i] = pos(i, nL);
+            for (int p = p0; p < p0 + len; ++p) slot[JA[p]] = p - p0;
+            for (int t = 0; t < nL; ++t) {   // the IKJ update list of row i
+                const int j = JA[p0 + t];
+                for (int k = diag[j] + 1; k < IA[j + 1]; ++k) {
+                    const int dest = slot[JA[k]];
+                    if (dest >= 0) lists[l].emplace_back(pos(j, k - IA[j]), static_cast<unsigned char>(dest | (t << 4)));
+                }
+            }
+            for (int p = p0; p < p0 + len; ++p) slot[JA[p]] = -1;
+            nu = std::max(nu, lists[l].size());
+        }
+        const size_t base = usrc.size();
+        usrc.resize(base + 64 * nu, 0);
+        ucode.resize(base + 64 * nu, 0xff);
+        for (int l = 0; l < 64; ++l)
+            for (size_t u = 0; u < lists[l].size(); ++u) {
+                usrc[base + 64 * u + l] = lists[l][u].first;
+                ucode[base + 64 * u + l] = lists[l][u].second;
+            }
+        if (usrc.size() >= static_cast<size_t>(INT_MAX)) return FEMGPU_OK;
+        uptr[static_cast<size_t>(s) + 1] = static_cast<int>(usrc.size());
+    }
+    if (usrc.empty()) {   // no L entries at all: keep one padding update so the buffers exist
+        usrc.push_back(0);
+        ucode.push_back(0xff);
+    }
+    const size_t tot = A->sell_col.n;
+    FG_TRY(A->par_uptr.upload(uptr.data(), uptr.size(), ctx->stream));
+    FG_TRY(A->par_usrc.upload(usrc.data(), usrc.size(), ctx->stream));
+    FG_TRY(A->par_ucode.upload(ucode.data(), ucode.size(), ctx->stream));
+    FG_TRY(A->par_rinfo.upload(rinfo.data(), rinfo.size(), ctx->stream));
+    FG_TRY(A->par_dpos.upload(dpos.data(), dpos.size(), ctx->stream));
+    FG_TRY(A->par_a.alloc(tot));
+    FG_TRY(A->par_b.alloc(tot));
+    FG_HIP(hipStreamSynchronize(ctx->stream));   // the host arrays end with this call
+    A->par_maxw = maxw;
+    A->par_ok = 1;
+    return FEMGPU_OK;
+}
+
+int launch_parilu_to_csr(femgpu_ctx *ctx, femgpu_csr *A)
+{
+    k_parilu_sell_to_csr<<<(A->n + 255) / 256, 256, 0, ctx->stream>>>(A->n, A->sell_ptr.p, A->IA.p, A->par_a.p,
+                                                                      A->LU32.p);
+    FG_HIP(hipGetLastError());
+    A->lu32_in_sell = false;
+    return FEMGPU_OK;
+}
+
 static int ensure_par(femgpu_csr *A)
 {
     const size_t nnz = static_cast<size_t>(A->nnz) + 1, n = static_cast<size_t>(A->n) + 1;
@@ -234,10 +399,43 @@ static int refresh_ell(femgpu_ctx *ctx, femgpu_csr *A)
     return FEMGPU_OK;
 }
 
+static int parilu_sell(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
+{
+    const int n = A->n, grid = (n + 255) / 256;
+    // the final iterate must land in par_a / Dinv32
+    float *L0 = (sweeps % 2 == 0) ? A->par_a.p : A->par_b.p, *L1 = (sweeps % 2 == 0) ? A->par_b.p : A->par_a.p;
+    float *D0 = (sweeps % 2 == 0) ? A->Dinv32.p : A->Dinv32b.p, *D1 = (sweeps % 2 == 0) ? A->Dinv32b.p : A->Dinv32.p;
+    k_parilu_init_sell<<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p, A->par_rinfo.p,
+                                                      A->par_dpos.p, L0, D0);
+    for (int s = 0; s < sweeps; ++s) {
+        if (A->par_maxw <= 8)
+            k_parilu_sweep_sell<8><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
+                                                                   A->par_rinfo.p, A->par_uptr.p, A->par_usrc.p,
+                                                                   A->par_ucode.p, L0, D0, L1, D1);
+        else
+            k_parilu_sweep_sell<16><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
+                                                                    A->par_rinfo.p, A->par_uptr.p, A->par_usrc.p,
+                                                                    A->par_ucode.p, L0, D0, L1, D1);
+        std::swap(L0, L1);
+        std::swap(D0, D1);
+    }
+    FG_HIP(hipGetLastError());
+    A->lu32_in_sell = true;
+    if (A->ell_kl > 0) {
+        k_parilu_sell_to_ell<<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->par_rinfo.p, A->par_a.p, A->ell_kl,
+                                                            A->ell_ku, A->ell_vl.p, A->ell_vu.p);
+        FG_HIP(hipGetLastError());
+        return FEMGPU_OK;
+    }
+    return launch_parilu_to_csr(ctx, A);   // the CSR Jacobi kernels read LU32
+}
+
 int launch_parilu32(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
 {
     if (A->n == 0) return FEMGPU_OK;
     FG_TRY(ensure_par(A));
+    if (A->par_ok) return parilu_sell(ctx, A, sweeps);
+    A->lu32_in_sell = false;
     const FactorPlan &F = A->fplan_nat;
     const int grid = (A->n + 255) / 256;
     // the final sweep must land in LU32/Dinv32: start in whichever buffer makes the parity right
@@ -264,6 +462,7 @@ int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A)
     FG_TRY(ensure_par(A));
     FG_TRY(launch_to_f32(ctx, A->LU.p, A->LU32.p, A->nnz));
     FG_TRY(launch_to_f32(ctx, A->Dinv.p, A->Dinv32.p, A->n));
+    A->lu32_in_sell = false;
     return refresh_ell(ctx, A);
 }
 

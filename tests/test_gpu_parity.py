@@ -416,12 +416,13 @@ def parilu_reference(IA, JA, AA, sweeps):
     return LU, 1.0 / LU[diag]
 
 
-@pytest.mark.parametrize("name", ["st_csr_rcm_teste_r8", "st_csr_rcm_cau_r24"])
+@pytest.mark.parametrize("name", ["st_csr_rcm_teste_r8", "st_csr_rcm_cau_r24", "synthetic12s"])
 def test_parallel_ilu_fixed_point_is_ilu0(dev, torch_cuda, oracle, name):
     """After twice as many sweeps as the factorisation has dependency levels (the entries of one
     row depend on each other in sequence), the parallel iteration has reached its fixed point: the
-    ILU(0) factors (FP32 rounding)."""
-    s = ilu_case(oracle, name)
+    ILU(0) factors (FP32 rounding). Mesh rows run the sliced-ELL kernels, the synthetic rows (up to
+    25 entries) the CSR-layout kernels."""
+    s = synthetic_csr(2000, 12, 5) if name == "synthetic12s" else ilu_case(oracle, name)
     depth = fem.ilu_levels(s["IA"], s["JA"], True)[1]
     A = fem.CsrMatrix(dev, s["IA"], s["JA"])
     A.set_values(s["AA"][:-1])
