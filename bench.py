@@ -9,9 +9,13 @@ Two solvers are timed on identical inputs (resident in HBM before the timed regi
     at ||R - A x|| <= 1e-8 ||R|| verified in FP64;
   * fp64: the reference's pgmres semantics (left ILU(0), stop at ||M^-1 r|| <= 1e-8 ||M^-1 R||).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--m 1000] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--m 1000] [--no-cpu] [--cpu-seconds 15]
+    python bench.py --profile-kernels      # short run for hardware-counter passes
 
-Prints one JSON line. The synthetic input is the Gaussian pulse initial state of the workload
+Prints one JSON line. The roofline object describes k_jacobi_upper_ell, the largest kernel of the
+mixed step, timed with HIP events over back-to-back launches; its `traffic` (HBM bytes per launch)
+comes from the FETCH_SIZE / WRITE_SIZE counter passes recorded in profiles/traffic.json. The CPU
+baseline is the single-threaded C oracle on the same workload. The synthetic input is the Gaussian pulse initial state of the workload
 (centre (0.3, 0.5), sigma 0.05) with u = 1 on the inflow segment x = 0, 0.25 < y < 0.5.
 """
 import argparse
@@ -43,16 +47,19 @@ def build_problem(m):
     return dict(x=x, y=y, ien=ien, type=t, neq=neq, c=c, ubc=ubc, u0=u0)
 
 
-def cpu_baseline(P, dt, alpha, kmax, tol, reps=5):
+def cpu_baseline(P, dt, alpha, kmax, tol, seconds=15.0, max_reps=200):
     """The oracle (single-threaded C, FP64) on full steps of the same workload: assembly,
-    ILU(0) factorisation, M^-1 R and pgmres, repeated reps times (about 10 s)."""
+    ILU(0) factorisation, M^-1 R and pgmres, repeated until `seconds` have passed (at least two
+    steps, at most max_reps)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
     c = P["c"]
     pb = O.make_problem(k=1e-4)
     t0 = time.perf_counter()
-    for _ in range(reps):
+    reps = 0
+    while reps < 2 or (time.perf_counter() - t0 < seconds and reps < max_reps):
+        reps += 1
         Ae, R = O.build_transport_transient(P["x"], P["y"], P["type"], c.invP_id, P["ien"], c.lm, P["ubc"],
                                             P["u0"], np.zeros(P["neq"]), pb, alpha, dt, P["neq"])
         AA = O.csr_assemble(c.scheme, Ae, c.nnz)
@@ -76,6 +83,9 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--factor-sweeps", type=int, default=2, help="parallel FP32 ILU(0) sweeps (mixed)")
     ap.add_argument("--jacobi", type=int, default=2, help="Jacobi iterations per triangle (mixed)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="length of the CPU baseline sample")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="counter runs: one mixed step, 20 launches of the roofline kernel and a 256 MiB copy")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -115,6 +125,21 @@ def main():
         else:
             info_box[mode] = fem.gmres_mixed(dev, A, r, X, tol=tol, inner_kmax=kmax, inner_tol=a.inner_tol,
                                              sweep_mode=a.jacobi, factor_sweeps=a.factor_sweeps)
+
+    nnz_u = int(np.count_nonzero(c.JA > np.repeat(np.arange(neq), np.diff(c.IA))))
+    if a.profile_kernels:
+        step("mixed")
+        y32 = torch.rand(neq, dtype=torch.float32, device="cuda")
+        z1, z2 = torch.empty_like(y32), torch.empty_like(y32)
+        A.ilu0_jacobi_iteration_f32(True, y32, z1)
+        for _ in range(20):
+            A.ilu0_jacobi_iteration_f32(True, y32, z2, z1)
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")   # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_kernels": "done", "copy_bytes": src.numel() * 4, "n": neq, "nnz_upper": nnz_u}))
+        return
 
     results = {}
     for mode in ("fp64", "mixed_exact_ilu", "mixed"):
@@ -190,16 +215,32 @@ def main():
         "parallel_ilu0_f32": dict(us=t_par * 1e6, sweeps=a.factor_sweeps),
         "jacobi_apply_f32": dict(us=t_jac * 1e6, GBs=bytes_jac / t_jac / 1e9, iterations=a.jacobi),
     }
-    si = A.sweep_info()
-    dom = kernels["ilu0_apply_f64"]
-    roofline = dict(bound="hbm", kernel="ILU(0) apply: lower+upper level-scheduled sweeps (FP64)",
-                    achieved=round(dom["GBs"], 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(dom["GBs"] / HBM_PEAK_GBS, 5), traffic=None,
-                    bytes_per_launch=bytes_ilu, sweep=si)
+    # Roofline kernel: the upper-triangle Jacobi iteration on the FP32 factors (k_jacobi_upper_ell),
+    # the largest kernel by time in the mixed step (2 launches per preconditioner application).
+    # Algorithmic bytes per launch: strict-upper entries (FP32 value + int32 column) once, and the
+    # vectors y, Dinv, z_in (gathered, counted once) and z_out: 8*nnz_U + 16*n.
+    A.ilu0_factor_parallel_f32(a.factor_sweeps)
+    y32 = torch.rand(neq, dtype=torch.float32, device="cuda")
+    z1, z2 = torch.empty_like(y32), torch.empty_like(y32)
+    A.ilu0_jacobi_iteration_f32(True, y32, z1)
+    t_up = timed(lambda: A.ilu0_jacobi_iteration_f32(True, y32, z2, z1), reps=50)
+    bytes_up = 8 * nnz_u + 16 * n
+    kernels["jacobi_upper_f32"] = dict(us=t_up * 1e6, GBs=bytes_up / t_up / 1e9)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh).get("k_jacobi_upper_ell", {})
+        if tr.get("n") == neq:
+            traffic = tr.get("hbm_bytes_per_launch")
+    ach = bytes_up / t_up / 1e9
+    roofline = dict(bound="hbm", kernel="k_jacobi_upper_ell (upper-triangle Jacobi iteration, FP32 factors)",
+                    achieved=round(ach, 2), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 5),
+                    traffic=traffic, bytes_per_launch=bytes_up, us_per_launch=round(t_up * 1e6, 3))
 
     cpu = None
     if not a.no_cpu:
-        cpu, Xo, ito = cpu_baseline(P, dt, alpha, kmax, tol)
+        cpu, Xo, ito = cpu_baseline(P, dt, alpha, kmax, tol, seconds=a.cpu_seconds)
         cpu["gpu_fp64_iterations"] = results["fp64"]["iterations"]
         cpu["oracle_iterations"] = ito
 
@@ -232,7 +273,8 @@ def main():
                             "preconditioner": "FP32 copy of the FP64 ILU(0), exact level-scheduled sweeps"},
         "fp64": {"solves_per_s": round(fp["solves_per_s"], 4), "ms_per_step": round(fp["ms"], 3),
                  "p50_ms": round(fp["p50_ms"], 3), "p99_ms": round(fp["p99_ms"], 3),
-                 "iterations": fp["iterations"], "cycles": fp["cycles"], "true_rel_residual": true_rel_fp64},
+                 "iterations": fp["iterations"], "cycles": fp["cycles"], "true_rel_residual": true_rel_fp64,
+                 "sweep": A.sweep_info()},
         "speedup_mixed_vs_fp64": round(mx["solves_per_s"] / fp["solves_per_s"], 3),
         "kernels": {k: {kk: round(vv, 2) for kk, vv in v.items()} for k, v in kernels.items()},
         "roofline": roofline,

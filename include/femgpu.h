@@ -144,14 +144,21 @@ int femgpu_ilu0_get(femgpu_csr *A, double *LU, double *Dinv);
 
 /* FP32 preconditioner pieces of the mixed solver (no reference counterpart).
  * Parallel ILU(0): FP32 factors in A's layout by `sweeps` fixed-point sweeps of the Chow-Patel
- * iteration from the current values (converges to the ILU(0) factors; exact after as many sweeps
- * as the factorisation has dependency levels). */
+ * iteration from the current values (converges to the ILU(0) factors; at the fixed point after
+ * twice as many sweeps as the factorisation has dependency levels). */
 int femgpu_ilu0_factor_parallel_f32(femgpu_csr *A, int sweeps);
 /* FP32 copy of the factors of femgpu_ilu0_factor into the same FP32 factor storage. */
 int femgpu_ilu0_factor_to_f32(femgpu_csr *A);
 /* z ~ U^-1 L^-1 p with `iterations` (1..1024) Jacobi iterations per triangle on the FP32 factors
  * (exact after as many iterations as the sweep has dependency levels); d_p == d_z allowed. */
 int femgpu_ilu0_apply_jacobi_f32(femgpu_csr *A, int iterations, const float *d_p, float *d_z);
+/* One Jacobi iteration on one triangle of the FP32 factors (the kernels femgpu_ilu0_apply_jacobi_f32
+ * chains): upper = 0: d_out = d_rhs - L_s d_in; upper = 1: d_out = Dinv (d_rhs - U_s d_in). d_in
+ * NULL stands for the starting iterate (d_rhs for L, Dinv d_rhs for U); d_out must not alias d_in
+ * or d_rhs. No reference counterpart (the preconditioner's building block, for callers that
+ * schedule the iterations themselves). */
+int femgpu_ilu0_jacobi_iteration_f32(femgpu_csr *A, int upper, const float *d_rhs, const float *d_in,
+                                     float *d_out);
 /* Copy the FP32 factors to the host (layout as femgpu_ilu0_get). */
 int femgpu_ilu0_get_f32(femgpu_csr *A, float *LU, float *Dinv);
 

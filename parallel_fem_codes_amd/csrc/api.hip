@@ -335,6 +335,16 @@ extern "C" int femgpu_ilu0_apply_jacobi_f32(femgpu_csr *A, int iterations, const
     return launch_jacobi_apply32(A->ctx, A, iterations, d_p, d_z, nullptr);
 }
 
+extern "C" int femgpu_ilu0_jacobi_iteration_f32(femgpu_csr *A, int upper, const float *d_rhs, const float *d_in,
+                                                float *d_out)
+{
+    FG_CHECK(A && d_rhs && d_out && (upper == 0 || upper == 1) && d_out != d_in && d_out != d_rhs, FEMGPU_ERR_ARG,
+             "femgpu_ilu0_jacobi_iteration_f32: bad arguments");
+    FG_CHECK(A->f32_factored, FEMGPU_ERR_STATE, "femgpu_ilu0_jacobi_iteration_f32: no FP32 factors");
+    if (!A->ell_kl && A->lu32_in_sell) FG_TRY(launch_parilu_to_csr(A->ctx, A));
+    return launch_jacobi_iteration32(A->ctx, A, upper, d_rhs, d_in, d_out);
+}
+
 extern "C" int femgpu_ilu0_get_f32(femgpu_csr *A, float *LU, float *Dinv)
 {
     FG_CHECK(A && LU && Dinv, FEMGPU_ERR_ARG, "femgpu_ilu0_get_f32: bad arguments");

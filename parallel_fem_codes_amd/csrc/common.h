@@ -258,6 +258,8 @@ int launch_sweep32(femgpu_ctx *ctx, femgpu_csr *A, const float *b, float *x, con
 int launch_ilu_factor(femgpu_ctx *ctx, femgpu_csr *A);
 int launch_parilu32(femgpu_ctx *ctx, femgpu_csr *A, int sweeps);
 int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A);
+int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const float *rhs, const float *in,
+                              float *out);
 int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
 int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip);
 int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA);

@@ -163,9 +163,10 @@ __global__ void k_factor_gather(int n, const int4 *__restrict__ meta, const doub
 {
     const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
     if (k >= static_cast<size_t>(n) * kFactorRow) return;
-    const int e = static_cast<int>(k / n), q = static_cast<int>(k % n);
+    // consecutive threads take consecutive entries of a row: the reads of A's row are contiguous
+    const int q = static_cast<int>(k / kFactorRow), e = static_cast<int>(k % kFactorRow);
     const int4 m = meta[q];
-    aval[k] = e < m.y ? AA[m.x + e] : 0.0;
+    aval[static_cast<size_t>(e) * n + q] = e < m.y ? AA[m.x + e] : 0.0;
 }
 
 // The factored rows into A's layout, and the inverse pivots by row.
@@ -174,9 +175,9 @@ __global__ void k_factor_scatter(int n, const int4 *__restrict__ meta, const dou
 {
     const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
     if (k >= static_cast<size_t>(n) * kFactorRow) return;
-    const int e = static_cast<int>(k / n), q = static_cast<int>(k % n);
+    const int q = static_cast<int>(k / kFactorRow), e = static_cast<int>(k % kFactorRow);
     const int4 m = meta[q];
-    if (e < m.y) LU[m.x + e] = out[k];
+    if (e < m.y) LU[m.x + e] = out[static_cast<size_t>(e) * n + q];
     if (e == 0) Dinv[m.w] = dq[q];
 }
 

@@ -490,6 +490,40 @@ static void jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, f
     }
 }
 
+int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const float *rhs, const float *in,
+                              float *out)
+{
+    const int n = A->n;
+    if (n == 0) return FEMGPU_OK;
+    const int grid = (n + 255) / 256;
+    if (A->ell_kl > 0) {
+        const bool k4 = std::max(A->ell_kl, A->ell_ku) <= 4;
+        if (upper && k4)
+            k_jacobi_upper_ell<4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_ku, A->ell_cu.p, A->ell_vu.p, A->Dinv32.p,
+                                                                 rhs, in, out, nullptr);
+        else if (upper)
+            k_jacobi_upper_ell<kEllMax><<<grid, 256, 0, ctx->stream>>>(n, A->ell_ku, A->ell_cu.p, A->ell_vu.p,
+                                                                       A->Dinv32.p, rhs, in, out, nullptr);
+        else if (k4)
+            k_jacobi_lower_ell<4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_kl, A->ell_cl.p, A->ell_vl.p, rhs, in, out,
+                                                                 nullptr);
+        else
+            k_jacobi_lower_ell<kEllMax><<<grid, 256, 0, ctx->stream>>>(n, A->ell_kl, A->ell_cl.p, A->ell_vl.p, rhs, in,
+                                                                       out, nullptr);
+    } else {
+        constexpr int LPR = 4;
+        const int g4 = static_cast<int>((static_cast<long>(n) * LPR + 255) / 256);
+        if (upper)
+            k_jacobi_upper<LPR><<<g4, 256, 0, ctx->stream>>>(n, A->IA.p, A->JA.p, A->diag.p, A->LU32.p, A->Dinv32.p, rhs,
+                                                             in, out, nullptr);
+        else
+            k_jacobi_lower<LPR><<<g4, 256, 0, ctx->stream>>>(n, A->IA.p, A->JA.p, A->diag.p, A->LU32.p, rhs, in, out,
+                                                             nullptr);
+    }
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
 int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip)
 {
     const int n = A->n;

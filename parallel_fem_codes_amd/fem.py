@@ -257,6 +257,10 @@ class CsrMatrix:
     def ilu0_factor_to_f32(self):
         N.check(N.lib().femgpu_ilu0_factor_to_f32(self.h))
 
+    def ilu0_jacobi_iteration_f32(self, upper, rhs, out, inp=None):
+        """One Jacobi iteration on one triangle of the FP32 factors (out must not alias rhs or inp)."""
+        N.check(N.lib().femgpu_ilu0_jacobi_iteration_f32(self.h, 1 if upper else 0, _ptr(rhs), _ptr(inp), _ptr(out)))
+
     def ilu0_apply_jacobi_f32(self, iterations, p, z):
         N.check(N.lib().femgpu_ilu0_apply_jacobi_f32(self.h, int(iterations), _ptr(p), _ptr(z)))
 

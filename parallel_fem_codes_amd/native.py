@@ -102,6 +102,7 @@ _SIGS = {
     "femgpu_ilu0_factor_parallel_f32": (I, [vp, I]),
     "femgpu_ilu0_factor_to_f32": (I, [vp]),
     "femgpu_ilu0_apply_jacobi_f32": (I, [vp, I, vp, vp]),
+    "femgpu_ilu0_jacobi_iteration_f32": (I, [vp, I, vp, vp, vp]),
     "femgpu_ilu0_get_f32": (I, [vp, f32p, f32p]),
     "femgpu_ebe_create": (I, [vp, I, I, I, i32p, C.POINTER(vp)]),
     "femgpu_ebe_destroy": (I, [vp]),

@@ -472,6 +472,17 @@ def test_jacobi_sweeps(dev, torch_cuda, oracle, name):
         A.ilu0_apply_jacobi_f32(it, zg, zg)
         assert rel(H(zg), z) < 1e-5, it
     assert rel(H(zg), exact) < 1e-5
+    # the single-iteration entry point chained by hand: two iterations per triangle
+    bg = T(torch, b, torch.float32)
+    y1, y2, z1, z2 = (torch.empty_like(bg) for _ in range(4))
+    A.ilu0_jacobi_iteration_f32(False, bg, y1)
+    A.ilu0_jacobi_iteration_f32(False, bg, y2, y1)
+    A.ilu0_jacobi_iteration_f32(True, y2, z1)
+    A.ilu0_jacobi_iteration_f32(True, y2, z2, z1)
+    y = b - Ls @ (b - Ls @ b)
+    z = Dio * (y - Us @ (Dio * (y - Us @ (Dio * y))))
+    assert rel(H(y2), y) < 1e-5
+    assert rel(H(z2), z) < 1e-5
 
 
 def transient_system(oracle, m, dt, alpha=0.5):
