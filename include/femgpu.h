@@ -321,6 +321,51 @@ int femgpu_euler_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, cons
 int femgpu_assemble_euler(femgpu_mesh *M, const femgpu_euler_problem *pb, const double *d_u,
                           const double *d_du, double *d_Ae, double *d_R, double *d_AA);
 
+/* =========================================================== nonlinear and time drivers */
+
+typedef struct {
+    int solver;            /* 0: ILUp_precond_setup + pgmres (FP64, the reference's pair);
+                              1: femgpu_gmres_mixed on the raw right-hand side */
+    int kmax, lmax;        /* KrylovBasisVectorsQuantity, LinearMaxIter */
+    double tol;            /* SolverTolerance */
+    femgpu_mixed_opts mixed;   /* solver = 1 */
+    double nonlinear_tol;  /* NonLinearTolerance */
+    int nonlinear_maxiter; /* NonLinearMaxIter */
+    int stop_by_norm;      /* time step: 1 StopByNorm, 0 StopByIterations (Criteria.c:25-41) */
+} femgpu_driver_opts;
+
+typedef struct {
+    int iterations;        /* fixed-point / Picard iterations, or multicorrections of the step */
+    int linear_iterations; /* Arnoldi steps summed over the linear solves */
+    int status;            /* 0 stopped by the criterion, 1 by the iteration cap, 2 a linear solve
+                              did not converge */
+    int pad;
+    double normdiff, normu;    /* steady: last ||u - uold|| (NS: ||s||) and ||u|| */
+    double norm_a, norm_Da;    /* time step: last ||a|| and ||Da|| */
+    int iters[64];         /* Arnoldi steps of each linear solve (the first 64) */
+} femgpu_driver_info;
+
+/* Steady transport fixed point (01_SS_Transport_Equation/2D/01_CommonFiles/Process.c:29-86):
+ * repeat Build_K_F_SUPG at u (CSR values assembled into A), solve, until
+ * ||u - uold|| <= NonLinearTolerance ||u|| or nonlinear_maxiter iterations. Scaling "NOT" only.
+ * d_u (neq): initial iterate in, final solution out. M must carry A's slot table. */
+int femgpu_st_process(femgpu_mesh *M, femgpu_csr *A, const femgpu_transport_problem *pb,
+                      const femgpu_driver_opts *opt, double *d_u, femgpu_driver_info *info);
+
+/* Navier-Stokes Picard loop (04_SS_NavierStokes_Equations/2D/01_CommonFiles/Process.c:27-52):
+ * repeat Build_K_F_SUPG_PSPG at u, solve for s, u += s, until ||s|| / ||u|| <= NonLinearTolerance
+ * or nonlinear_maxiter iterations. d_fx, d_fy as femgpu_assemble_ns. */
+int femgpu_ns_process(femgpu_mesh *M, femgpu_csr *A, double Re, const double *d_fx, const double *d_fy,
+                      const femgpu_driver_opts *opt, double *d_u, femgpu_driver_info *info);
+
+/* One step of the predictor-multicorrector (00_CommonFiles/Time_Integration/Predictor_Old.c:53-123)
+ * for the transient transport: u_old = u, u += (1-alpha) dt a, a = 0, then multicorrections
+ * (Build_M_K_R_SUPG at (u, a), solve for Da, a += Da, u += alpha dt Da) until the stop criterion.
+ * d_u, d_a (neq): the state, updated in place; the caller advances t. */
+int femgpu_tt_predictor_old_step(femgpu_mesh *M, femgpu_csr *A, const femgpu_transport_problem *pb,
+                                 double dt, double alpha, const femgpu_driver_opts *opt, double *d_u,
+                                 double *d_a, femgpu_driver_info *info);
+
 #ifdef __cplusplus
 }
 #endif

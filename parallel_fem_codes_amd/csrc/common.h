@@ -157,6 +157,7 @@ struct femgpu_ctx {
     fg::DevBuf<double> gm_basis, gm_small, gm_w, gm_r;
     fg::DevBuf<float> gm_basis32, gm_w32;
     fg::DevBuf<int> gm_flags;
+    fg::DevBuf<double> drv_ws;         // nonlinear/time drivers: F, X, uold
     volatile int *h_status = nullptr;  // pinned, device-visible solver status words
     int *d_status = nullptr;
 };

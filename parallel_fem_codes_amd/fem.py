@@ -436,3 +436,42 @@ def pcg(dev: Device, A: CsrMatrix, B, X, itmax=1000, tol=1e-8, precond: CsrMatri
     S = _system(A, precond)
     N.check(N.lib().femgpu_pcg(dev.h, C.byref(S), int(itmax), float(tol), _ptr(B), _ptr(X), C.byref(info)))
     return info
+
+
+# ----------------------------------------------------------------------------- drivers
+
+def driver_opts(solver="fp64", kmax=30, lmax=1000, tol=1e-8, nonlinear_tol=1e-8, nonlinear_maxiter=50,
+                stop_by_norm=True, inner_kmax=30, inner_tol=1e-4, max_refinements=50, sweep_mode=2,
+                factor_sweeps=2):
+    """Options of the nonlinear/time drivers: solver "fp64" = ILUp_precond_setup + pgmres (the
+    reference's pair), "mixed" = gmres_mixed with the given FP32 preconditioner."""
+    o = N.DriverOpts()
+    o.solver = {"fp64": 0, "mixed": 1}[solver]
+    o.kmax, o.lmax, o.tol = kmax, lmax, tol
+    o.mixed.inner_kmax, o.mixed.inner_cycles, o.mixed.inner_tol = inner_kmax, 1, inner_tol
+    o.mixed.max_refinements, o.mixed.sweep_mode, o.mixed.factor_sweeps = max_refinements, sweep_mode, factor_sweeps
+    o.nonlinear_tol, o.nonlinear_maxiter, o.stop_by_norm = nonlinear_tol, nonlinear_maxiter, 1 if stop_by_norm else 0
+    return o
+
+
+def Process(M: "Mesh", A: CsrMatrix, pb, u, opts):
+    """Steady transport fixed point (ST Process.c:29-86) on the device; u is updated in place."""
+    info = N.DriverInfo()
+    N.check(N.lib().femgpu_st_process(M.h, A.h, C.byref(pb), C.byref(opts), _ptr(u), C.byref(info)))
+    return info
+
+
+def Process_NS(M: "NsMesh", A: CsrMatrix, Re, u, opts, fx=None, fy=None):
+    """Navier-Stokes Picard loop (NS Process.c:27-52) on the device; u is updated in place."""
+    info = N.DriverInfo()
+    N.check(N.lib().femgpu_ns_process(M.h, A.h, float(Re), _ptr(fx), _ptr(fy), C.byref(opts), _ptr(u),
+                                      C.byref(info)))
+    return info
+
+
+def Predictor_Old_step(M: "Mesh", A: CsrMatrix, pb, dt, alpha, u, a, opts):
+    """One predictor-multicorrector step (Predictor_Old.c:53-123); u and a are updated in place."""
+    info = N.DriverInfo()
+    N.check(N.lib().femgpu_tt_predictor_old_step(M.h, A.h, C.byref(pb), float(dt), float(alpha), C.byref(opts),
+                                                 _ptr(u), _ptr(a), C.byref(info)))
+    return info

@@ -62,6 +62,20 @@ class MixedOpts(C.Structure):
                 ("max_refinements", C.c_int), ("sweep_mode", C.c_int), ("factor_sweeps", C.c_int)]
 
 
+class DriverOpts(C.Structure):
+    """femgpu_driver_opts."""
+    _fields_ = [("solver", C.c_int), ("kmax", C.c_int), ("lmax", C.c_int), ("tol", C.c_double),
+                ("mixed", MixedOpts), ("nonlinear_tol", C.c_double), ("nonlinear_maxiter", C.c_int),
+                ("stop_by_norm", C.c_int)]
+
+
+class DriverInfo(C.Structure):
+    """femgpu_driver_info."""
+    _fields_ = [("iterations", C.c_int), ("linear_iterations", C.c_int), ("status", C.c_int), ("pad", C.c_int),
+                ("normdiff", C.c_double), ("normu", C.c_double), ("norm_a", C.c_double), ("norm_Da", C.c_double),
+                ("iters", C.c_int * 64)]
+
+
 _LIB = None
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -118,6 +132,11 @@ _SIGS = {
     "femgpu_gmres_mixed": (I, [vp, C.POINTER(System), C.POINTER(MixedOpts), D, vp, vp,
                               C.POINTER(SolveInfo)]),
     "femgpu_pcg": (I, [vp, C.POINTER(System), I, D, vp, vp, C.POINTER(SolveInfo)]),
+    "femgpu_st_process": (I, [vp, vp, C.POINTER(TransportProblem), C.POINTER(DriverOpts), vp,
+                              C.POINTER(DriverInfo)]),
+    "femgpu_ns_process": (I, [vp, vp, D, vp, vp, C.POINTER(DriverOpts), vp, C.POINTER(DriverInfo)]),
+    "femgpu_tt_predictor_old_step": (I, [vp, vp, C.POINTER(TransportProblem), D, D, C.POINTER(DriverOpts), vp, vp,
+                                         C.POINTER(DriverInfo)]),
     "femgpu_mesh_create": (I, [vp, I, I, I, f64p, f64p, i32p, i32p, f64p, i32p, vp, vp, I,
                               C.POINTER(vp)]),
     "femgpu_mesh_destroy": (I, [vp]),

@@ -709,3 +709,135 @@ def test_euler_assembly_bit_exact(dev, torch_cuda, oracle, name):
     assert np.array_equal(H(R), s["R"])
     if AA is not None:
         assert np.array_equal(H(AA), s["AA"])
+
+
+# ----------------------------------------------------------------------------- drivers
+
+ST_CSR_CASES = [n for n in G.names() if n.startswith("st_csr_")]
+
+
+@pytest.mark.parametrize("name", ST_CSR_CASES)
+def test_st_process_matches_reference_sequence(dev, torch_cuda, oracle, name):
+    """The device fixed point (Process.c:29-86) against the reference harness's own run: the
+    same number of fixed-point iterations, GMRES iterations within +-2 each, and the final state
+    within 1e-9 relative L2."""
+    torch = torch_cuda
+    d = G.load(name)
+    s = steady_system(oracle, d=d)
+    h = d["header"]
+    kmax, lmax, nl = int(h[5]), int(h[6]), int(h[7])
+    nltol = float(d["params"][10])
+    M = fem.Mesh(dev, s["x"], s["y"], s["type"], s["invP"], s["ubc"], s["ien"], s["neq"], scheme=s["scheme"],
+                 nnz=s["nnz"])
+    A = fem.CsrMatrix(dev, s["IA"], s["JA"])
+    u = torch.zeros(s["neq"], dtype=torch.float64, device="cuda")
+    opts = fem.driver_opts("fp64", kmax=kmax, lmax=lmax, tol=1e-8, nonlinear_tol=nltol, nonlinear_maxiter=nl)
+    info = fem.Process(M, A, gpu_problem(s["pb"]), u, opts)
+    ref_iters = d["ref_iters"].tolist()
+    assert info.iterations == len(ref_iters)
+    for k, r in enumerate(ref_iters):
+        assert abs(info.iters[k] - r) <= 2, (k, info.iters[k], r)
+    assert rel(H(u), d["ref_u"]) < 1e-9
+
+
+def _predictor_old_oracle(oracle, sysd, pb, dt, alpha, steps, nltol, maxit, kmax=30, tol=1e-8):
+    """Predictor_Old.c:53-123 driven through the oracle primitives (StopByNorm)."""
+    x, y, t, ien, c, ubc, u = (sysd[k] for k in ("x", "y", "type", "ien", "c", "ubc", "u0"))
+    neq = sysd["neq"]
+    u, a = u.copy(), np.zeros(neq)
+    out = []
+    for _ in range(steps):
+        u += (1.0 - alpha) * dt * a
+        a[:] = 0.0
+        i = 0
+        while True:
+            i += 1
+            Ae, R = oracle.build_transport_transient(x, y, t, c["invP"], ien, c["lm"], ubc, u, a, pb, alpha, dt, neq)
+            AA = oracle.csr_assemble(c["scheme"], Ae, c["nnz"])
+            LU, Dinv, rc = oracle.ilu0_factor(c["IA"], c["JA"], AA)
+            B = oracle.ilu0_apply(c["IA"], c["JA"], LU, Dinv, R[:neq])
+            S = oracle.make_system(0, neq, IA=c["IA"], JA=c["JA"], AA=AA, ilu=(c["IA"], c["JA"], LU, Dinv))
+            Da, it, rho, cyc = oracle.pgmres(S, B, kmax=kmax, lmax=1000, tol=tol)
+            Da = Da[:neq]
+            a += Da
+            u += alpha * dt * Da
+            na, nda = np.sqrt(oracle.ddot(a, a)), np.sqrt(oracle.ddot(Da, Da))
+            if nda < nltol * na or i >= maxit:
+                break
+        out.append((i, u.copy(), a.copy()))
+    return out
+
+
+def _transient_setup(oracle, m):
+    x, y, ien = oracle.mesh_structured(m, m)
+    t = oracle.unit_square_types(x, y)
+    invP0, neq = oracle.number_unknowns(t)
+    lm0 = oracle.fill_lm(ien, t, invP0, neq)
+    c = oracle.csr_init(lm0, 3, neq, reorder="SYMRCM", permute_lm=True, type_=t, invP=invP0)
+    u_node = np.exp(-((x - 0.3) ** 2 + (y - 0.5) ** 2) / (2 * 0.05 ** 2))
+    u0 = np.zeros(neq)
+    u0[c["invP"][t == 1]] = u_node[t == 1]
+    return dict(x=x, y=y, type=t, ien=ien, c=c, ubc=oracle.inflow_ubc(x, y), u0=u0, neq=neq)
+
+
+@pytest.mark.parametrize("shock", [0, 1])
+def test_predictor_old_matches_oracle(dev, torch_cuda, oracle, shock):
+    """Three predictor-multicorrector steps on the device against the same loop over the oracle:
+    equal multicorrection counts and states within 1e-9 relative L2 after every step; the mixed
+    solver stays within 1e-7 of the FP64 states."""
+    torch = torch_cuda
+    sysd = _transient_setup(oracle, 30)
+    dt, alpha, nltol, maxit, steps = 2e-2, 0.5, 1e-4, 8, 3
+    pbo = oracle.make_problem(k=1e-4, shock=shock)
+    ref = _predictor_old_oracle(oracle, sysd, pbo, dt, alpha, steps, nltol, maxit)
+    c = sysd["c"]
+    M = fem.Mesh(dev, sysd["x"], sysd["y"], sysd["type"], c["invP"], sysd["ubc"], sysd["ien"], sysd["neq"],
+                 lm=c["lm"], scheme=c["scheme"], nnz=c["nnz"])
+    A = fem.CsrMatrix(dev, c["IA"], c["JA"])
+    pb = gpu_problem(pbo)
+    for solver, bar in (("fp64", 1e-9), ("mixed", 1e-7)):
+        opts = fem.driver_opts(solver, nonlinear_tol=nltol, nonlinear_maxiter=maxit)
+        u = T(torch, sysd["u0"])
+        a = torch.zeros_like(u)
+        for k in range(steps):
+            info = fem.Predictor_Old_step(M, A, pb, dt, alpha, u, a, opts)
+            i_ref, u_ref, a_ref = ref[k]
+            if solver == "fp64":
+                assert info.iterations == i_ref, (k, info.iterations, i_ref)
+            assert info.status in (0, 1)
+            assert rel(H(u), u_ref) < bar, (solver, k)
+            assert rel(H(a), a_ref) < 1e3 * bar, (solver, k)
+
+
+def test_ns_process_picard_matches_oracle(dev, torch_cuda, oracle):
+    """The device Picard loop (NS Process.c:27-52) on a small lid-driven cavity against the same
+    loop over the oracle: equal Picard iteration counts and final states within 1e-8."""
+    torch = torch_cuda
+    s = ns_case(oracle, m=10, Re=100.0)
+    s["u"][:] = 0.0
+    c = s["c"]
+    nltol, maxit, kmax = 1e-8, 40, 50
+    # oracle loop
+    u = np.zeros(s["neq"])
+    it_ref = 0
+    delta2 = nltol + 1
+    while delta2 > nltol and it_ref < maxit:
+        it_ref += 1
+        U3 = oracle.ns_nodal_state(s["type3"], s["invP"], u, s["presc"])
+        Ke, F = oracle.build_ns_supg_pspg(s["x"], s["y"], s["ien"], s["lm"], U3, s["fx"], s["fy"], s["Re"], s["neq"])
+        AA = oracle.csr_assemble(c["scheme"], Ke, c["nnz"], s=9)
+        LU, Dinv, rc = oracle.ilu0_factor(c["IA"], c["JA"], AA)
+        B = oracle.ilu0_apply(c["IA"], c["JA"], LU, Dinv, F[:s["neq"]])
+        S = oracle.make_system(0, s["neq"], IA=c["IA"], JA=c["JA"], AA=AA, ilu=(c["IA"], c["JA"], LU, Dinv))
+        sv, its, rho, cyc = oracle.pgmres(S, B, kmax=kmax, lmax=1000, tol=1e-10)
+        u += sv[:s["neq"]]
+        delta2 = np.sqrt(oracle.ddot(sv[:s["neq"]], sv[:s["neq"]])) / np.sqrt(oracle.ddot(u, u))
+    M = fem.NsMesh(dev, s["x"], s["y"], s["type3"], s["invP"], s["presc"], s["ien"], s["lm"], s["neq"],
+                   scheme=c["scheme"], nnz=c["nnz"])
+    A = fem.CsrMatrix(dev, c["IA"], c["JA"])
+    ug = torch.zeros(s["neq"], dtype=torch.float64, device="cuda")
+    opts = fem.driver_opts("fp64", kmax=kmax, tol=1e-10, nonlinear_tol=nltol, nonlinear_maxiter=maxit)
+    info = fem.Process_NS(M, A, s["Re"], ug, opts, fx=T(torch, s["fx"]), fy=T(torch, s["fy"]))
+    assert info.status == 0 and it_ref < maxit
+    assert abs(info.iterations - it_ref) <= 1
+    assert rel(H(ug), u) < 1e-8
