@@ -321,6 +321,37 @@ int femgpu_euler_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, cons
 int femgpu_assemble_euler(femgpu_mesh *M, const femgpu_euler_problem *pb, const double *d_u,
                           const double *d_du, double *d_Ae, double *d_R, double *d_AA);
 
+/* =========================================================== meshes and mesh files (host) */
+
+/* The structured generator of Mesh_Generator/Structured_Mesh/meshGenerator.c:95-190: (M+1)(N+1)
+ * nodes row by row on [xinf, xsup] x [yinf, ysup], 2MN triangles, two per square, squares row by
+ * row, cut along the lower-left/upper-right diagonal when right = 1. x, y: (M+1)(N+1); ien: 6MN. */
+int femgpu_structured_mesh(int M, int N, double xinf, double yinf, double xsup, double ysup, int right,
+                           double *x, double *y, int *ien);
+
+/* setmark_according_boundary (meshGenerator.c:221-363) for the named problem (OBLIQUO, REFLETIDO,
+ * EXPLOSION, SOD, BAROCLINIC, PUDIM, CONE, CARTOLA, TESTE, CAVITY): the boundary mark of every node
+ * and (type non-null: nnodes*NDOF) the Type of every dof from the mark table (:16-19). Returns NDOF
+ * (1, 3 or 4) or a negative status for an unknown problem. */
+int femgpu_structured_marks(const char *problem, int nnodes, const double *x, const double *y, int *mark,
+                            int *type);
+
+typedef struct {
+    int nnodes, nel, ndof, format;
+} femgpu_mesh_file_info;
+
+/* Reads a mesh file. format 0: the generator's `<PROBLEM>_<nn>_<ne>.dat` (meshGenerator.c:193-211);
+ * format 1: one rank's `RANK_<r>_<PROBLEM>_<nn>_<ne>.dat` as the drivers read it
+ * (01_SS_Transport_Equation/2D/01_CommonFiles/Preprocess.c:127-168: per dof "0" or
+ * "Type Send invP_id"). Call with null arrays for the sizes, then with x, y (nnodes), type and invP
+ * (nnodes*ndof; invP -1 where the file has none), ien (3*nel), etype (nel; -1 where absent). */
+int femgpu_mesh_file_read(const char *path, int format, int ndof, femgpu_mesh_file_info *info, double *x,
+                          double *y, int *type, int *invP, int *ien, int *etype);
+
+/* Writes the generator's format (meshGenerator.c:193-211), byte for byte as the generator prints. */
+int femgpu_mesh_file_write(const char *path, int nnodes, int nel, int ndof, const double *x, const double *y,
+                           const int *type, const int *ien);
+
 /* =========================================================== nonlinear and time drivers */
 
 typedef struct {

@@ -35,27 +35,48 @@ def _ptr(t):
 
 # ------------------------------------------------------------------------------ mesh generation
 
-def structured_mesh(M, N, box=(0.0, 0.0, 1.0, 1.0), right=True):
-    """M x N squares cut on one diagonal, numbered like MG/Structured_Mesh/meshGenerator.c
+def structured_mesh(mx, my, box=(0.0, 0.0, 1.0, 1.0), right=True):
+    """mx x my squares cut on one diagonal, numbered like MG/Structured_Mesh/meshGenerator.c
     (nodes row by row, squares row by row, two triangles each; "right" cut from the lower-left to
     the upper-right corner, "left" cut the other diagonal). Returns x, y, ien (int32, nel*3)."""
     x0, y0, x1, y1 = box
-    nosx = M + 1
-    hx, hy = (x1 - x0) / M, (y1 - y0) / N
-    i = np.arange(nosx, dtype=np.float64)
-    j = np.arange(N + 1, dtype=np.float64)
-    x = np.tile(x0 + i * hx, N + 1)
-    y = np.repeat(y0 + j * hy, nosx)
-    jj, ii = np.meshgrid(np.arange(N), np.arange(M), indexing="ij")
-    a = (jj * nosx + ii).ravel().astype(np.int64)
-    ien = np.empty((M * N, 2, 3), np.int64)
-    if right:
-        ien[:, 0] = np.stack([a, a + 1, a + 1 + nosx], 1)
-        ien[:, 1] = np.stack([a, a + 1 + nosx, a + nosx], 1)
-    else:
-        ien[:, 0] = np.stack([a, a + 1, a + nosx], 1)
-        ien[:, 1] = np.stack([a + 1, a + 1 + nosx, a + nosx], 1)
-    return x, y, ien.reshape(-1).astype(np.int32)
+    nn = (mx + 1) * (my + 1)
+    x, y, ien = np.empty(nn), np.empty(nn), np.empty(6 * mx * my, np.int32)
+    N.check(N.lib().femgpu_structured_mesh(int(mx), int(my), float(x0), float(y0), float(x1), float(y1),
+                                           1 if right else 0, x, y, ien))
+    return x, y, ien
+
+
+def structured_marks(problem, x, y):
+    """setmark_according_boundary of the generator: (NDOF, mark per node, Type per dof)."""
+    x, y = np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64)
+    mark = np.empty(len(x), np.int32)
+    ndof = N.check(N.lib().femgpu_structured_marks(problem.encode(), len(x), x, y, mark, None))
+    t = np.empty(len(x) * ndof, np.int32)
+    N.check(N.lib().femgpu_structured_marks(problem.encode(), len(x), x, y, mark, t.ctypes.data_as(C.c_void_p)))
+    return ndof, mark, t
+
+
+def read_mesh(path, ndof=1, format=0):
+    """A mesh file: format 0 = the generator's <PROBLEM>_<nn>_<ne>.dat, 1 = one rank's RANK_* file.
+    Returns dict(x, y, type, invP, ien, etype)."""
+    info = N.MeshFileInfo()
+    p = str(path).encode()
+    N.check(N.lib().femgpu_mesh_file_read(p, format, ndof, C.byref(info), None, None, None, None, None, None))
+    x, y = np.empty(info.nnodes), np.empty(info.nnodes)
+    t, inv = np.empty(info.nnodes * ndof, np.int32), np.empty(info.nnodes * ndof, np.int32)
+    ien, et = np.empty(3 * info.nel, np.int32), np.empty(info.nel, np.int32)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    N.check(N.lib().femgpu_mesh_file_read(p, format, ndof, C.byref(info), vp(x), vp(y), vp(t), vp(inv), vp(ien),
+                                          vp(et)))
+    return dict(x=x, y=y, type=t, invP=inv, ien=ien, etype=et)
+
+
+def write_mesh(path, x, y, type_, ien, ndof=1):
+    """Writes the generator's format (meshGenerator.c:193-211)."""
+    x, y = np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64)
+    N.check(N.lib().femgpu_mesh_file_write(str(path).encode(), len(x), len(ien) // 3, ndof, x, y, _i32(type_),
+                                           _i32(ien)))
 
 
 def boundary_types(x, y, box=(0.0, 0.0, 1.0, 1.0), tol=1e-15):

@@ -76,6 +76,11 @@ class DriverInfo(C.Structure):
                 ("iters", C.c_int * 64)]
 
 
+class MeshFileInfo(C.Structure):
+    """femgpu_mesh_file_info."""
+    _fields_ = [("nnodes", C.c_int), ("nel", C.c_int), ("ndof", C.c_int), ("format", C.c_int)]
+
+
 _LIB = None
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -132,6 +137,10 @@ _SIGS = {
     "femgpu_gmres_mixed": (I, [vp, C.POINTER(System), C.POINTER(MixedOpts), D, vp, vp,
                               C.POINTER(SolveInfo)]),
     "femgpu_pcg": (I, [vp, C.POINTER(System), I, D, vp, vp, C.POINTER(SolveInfo)]),
+    "femgpu_structured_mesh": (I, [I, I, D, D, D, D, I, f64p, f64p, i32p]),
+    "femgpu_structured_marks": (I, [C.c_char_p, I, f64p, f64p, i32p, vp]),
+    "femgpu_mesh_file_read": (I, [C.c_char_p, I, I, C.POINTER(MeshFileInfo), vp, vp, vp, vp, vp, vp]),
+    "femgpu_mesh_file_write": (I, [C.c_char_p, I, I, I, f64p, f64p, i32p, i32p]),
     "femgpu_st_process": (I, [vp, vp, C.POINTER(TransportProblem), C.POINTER(DriverOpts), vp,
                               C.POINTER(DriverInfo)]),
     "femgpu_ns_process": (I, [vp, vp, D, vp, vp, C.POINTER(DriverOpts), vp, C.POINTER(DriverInfo)]),
