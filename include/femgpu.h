@@ -186,10 +186,13 @@ int femgpu_dzero(femgpu_ctx *ctx, int n, double *d_x);
 
 /* =========================================================== solvers */
 
+typedef struct femgpu_blockdiag femgpu_blockdiag;
+
 typedef struct {
     femgpu_csr *csr;   /* operator, when non-null */
     femgpu_ebe *ebe;   /* operator, when csr is null */
     femgpu_csr *ilu;   /* factored ILU(0) preconditioner (left), or NULL for none */
+    femgpu_blockdiag *bdiag;   /* nodal block-diagonal preconditioner (left), used when ilu is NULL */
 } femgpu_system;
 
 typedef struct {
@@ -320,6 +323,28 @@ int femgpu_euler_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, cons
  * d_AA[nnz+1] (nullable). d_u, d_du: current state and increment (neq). */
 int femgpu_assemble_euler(femgpu_mesh *M, const femgpu_euler_problem *pb, const double *d_u,
                           const double *d_du, double *d_Ae, double *d_R, double *d_AA);
+
+/* =========================================================== nodal block-diagonal preconditioner */
+
+/* BlockDiag_precond_EBE_setup / BlockDiagDOF3_precond_EBE_setup (00_CommonFiles/
+ * Solvers_and_Preconditioners/EBE/BlockDiag_precond_EBE_setup.c:6-88, EBE/BlockDiagDOF3_precond_EBE_setup.c)
+ * for ndof = 4 / 3 (1: point Jacobi): the node -> dof equations (type/invP per dof, nnodes*ndof;
+ * prescribed dofs map to the sink NEQ) and the node -> element incidences of ien[nel*3]. */
+int femgpu_blockdiag_create(femgpu_ctx *ctx, int nnodes, int nel, int ndof, int NEQ, const int *ien,
+                            const int *type, const int *invP, femgpu_blockdiag **out);
+int femgpu_blockdiag_destroy(femgpu_blockdiag *B);
+/* The ndof x ndof diagonal block of every node summed over its elements in element order
+ * (BlockDiag_precond_EBE_setup.c:66-77) from d_Ae (nel*s*s, s = 3*ndof, element order, the layout
+ * of the assembly kernels), inverted (:79-80). The reference then applies it to F (:84): call
+ * femgpu_blockdiag_apply(B, F, F). */
+int femgpu_blockdiag_setup(femgpu_blockdiag *B, const double *d_Ae);
+/* BlockDiag_precond / BlockDiagDOF3_precond (BlockDiag_precond.c:3-29): z = D^-1 p per node,
+ * prescribed dofs read as 0 and left unwritten. d_p == d_z allowed. */
+int femgpu_blockdiag_apply(femgpu_blockdiag *B, const double *d_p, double *d_z);
+/* Block_scaling_EBE (Block_scaling.c:10-118): blocks from the unscaled d_Ae, F := D^-1 F (d_F
+ * nullable), then every element row block a := D_{v_a}^-1 (row block with the rows and columns of
+ * prescribed dofs zeroed and a unit diagonal), in place. */
+int femgpu_block_scaling(femgpu_blockdiag *B, double *d_Ae, double *d_F);
 
 /* =========================================================== meshes and mesh files (host) */
 

@@ -184,7 +184,7 @@ int ILUp_precond(ParametersType *Parameters, MatrixDataType *MatrixData, FemStru
 int pgmres(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs,
            FemFunctionsType *FemFunctions, double *B, double *X)
 {
-    femgpu_system S = {NULL, NULL, NULL};
+    femgpu_system S = {NULL, NULL, NULL, NULL};
     femgpu_solve_info info;
     compat_init(Parameters);
     if (FemFunctions->precond == ILUp_precond) {

@@ -494,6 +494,7 @@ extern "C" int femgpu_pcg(femgpu_ctx *ctx, const femgpu_system *S, int itmax, do
     FG_CHECK(ctx && S && S->csr && d_B && d_X, FEMGPU_ERR_ARG, "femgpu_pcg: needs a CSR operator");
     FG_CHECK(!S->ilu || S->ilu->factored, FEMGPU_ERR_STATE, "femgpu_pcg: ILU(0) not factored");
     const int n = S->csr->n;
+    if (!S->ilu && S->bdiag) FG_TRY(blockdiag_check(S->bdiag, n));
     DevBuf<double> P, Q, R, Z;
     FG_TRY(P.alloc(n));
     FG_TRY(Q.alloc(n));
@@ -507,6 +508,7 @@ extern "C" int femgpu_pcg(femgpu_ctx *ctx, const femgpu_system *S, int itmax, do
     };
     auto precond = [&](double *z) -> int {
         if (S->ilu) FG_TRY(launch_sweep64(ctx, S->ilu, z, z, nullptr));
+        else if (S->bdiag) FG_TRY(launch_blockdiag_apply(ctx, S->bdiag, z, z, nullptr));
         return FEMGPU_OK;
     };
     // pcg.c:31-47

@@ -261,6 +261,8 @@ int launch_parilu32(femgpu_ctx *ctx, femgpu_csr *A, int sweeps);
 int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A);
 int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const float *rhs, const float *in,
                               float *out);
+int blockdiag_check(const femgpu_blockdiag *B, int n);
+int launch_blockdiag_apply(femgpu_ctx *ctx, const femgpu_blockdiag *B, const double *p, double *z, const int *skip);
 int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
 int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip);
 int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA);

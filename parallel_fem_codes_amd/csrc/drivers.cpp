@@ -44,7 +44,7 @@ int check_opts(const femgpu_driver_opts *o)
 int linear_solve(femgpu_ctx *ctx, femgpu_csr *A, const femgpu_driver_opts *o, double *F, double *X,
                  femgpu_solve_info *si)
 {
-    femgpu_system S = {A, nullptr, A};
+    femgpu_system S = {A, nullptr, A, nullptr};
     if (o->solver == 0) {
         FG_TRY(femgpu_ilu0_factor(A));                  // ILUp_precond_setup.c:8-61, p = 0
         FG_TRY(femgpu_ilu0_apply(A, F, F));             // F := M^-1 F (:57)

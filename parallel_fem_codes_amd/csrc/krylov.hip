@@ -394,6 +394,7 @@ static int apply_op64(femgpu_ctx *ctx, const femgpu_system *S, const double *x, 
         FG_TRY(launch_ebemv(ctx, S->ebe, x, w_out));
     }
     if (S->ilu) FG_TRY(launch_sweep64(ctx, S->ilu, w_out, w_out, skip));
+    else if (S->bdiag) FG_TRY(launch_blockdiag_apply(ctx, S->bdiag, w_out, w_out, skip));
     return FEMGPU_OK;
 }
 
@@ -414,6 +415,7 @@ static int apply_step_input(femgpu_ctx *ctx, const femgpu_system *S, int n, cons
     FG_HIP(hipGetLastError());
     FG_TRY(launch_ebemv(ctx, S->ebe, Vi, wn));
     if (S->ilu) FG_TRY(launch_sweep64(ctx, S->ilu, wn, wn, skip));
+    else if (S->bdiag) FG_TRY(launch_blockdiag_apply(ctx, S->bdiag, wn, wn, skip));
     return FEMGPU_OK;
 }
 
@@ -430,6 +432,7 @@ extern "C" int femgpu_pgmres(femgpu_ctx *ctx, const femgpu_system *S, int kmax, 
     FG_CHECK(!S->ilu || S->ilu->factored, FEMGPU_ERR_STATE, "femgpu_pgmres: ILU(0) preconditioner not factored");
     const int n = sys_n(S);
     FG_CHECK(!S->ilu || S->ilu->n == n, FEMGPU_ERR_ARG, "femgpu_pgmres: preconditioner size mismatch");
+    if (!S->ilu && S->bdiag) FG_TRY(blockdiag_check(S->bdiag, n));
     GDev G;
     FG_TRY(make_gdev(ctx, kmax, G));
     size_t ldv;
@@ -462,6 +465,7 @@ extern "C" int femgpu_pgmres(femgpu_ctx *ctx, const femgpu_system *S, int kmax, 
                 FG_TRY(launch_ebemv(ctx, S->ebe, d_X, wT));
             }
             if (S->ilu) FG_TRY(launch_sweep64(ctx, S->ilu, wT, wT, nullptr));
+            else if (S->bdiag) FG_TRY(launch_blockdiag_apply(ctx, S->bdiag, wT, wT, nullptr));
             cycle_start<double>(ctx, G, n, wT, d_B, 1, 0, tol);
             FG_HIP(hipGetLastError());
             std::swap(wA, wT);

@@ -414,19 +414,53 @@ class EulerMesh(NsMesh):
                                               C.c_void_p(AA_ptr) if AA_ptr else None))
 
 
-def _system(op, ilu):
+class BlockDiag:
+    """Nodal block-diagonal preconditioner (BlockDiag_precond_EBE_setup / BlockDiag_precond) and
+    block scaling (Block_scaling_EBE) for element matrices with ndof unknowns per node."""
+
+    def __init__(self, dev: Device, nnodes, ien, ndof, type_, invP, NEQ):
+        self.dev, self.ndof, self.NEQ = dev, ndof, NEQ
+        h = C.c_void_p()
+        N.check(N.lib().femgpu_blockdiag_create(dev.h, int(nnodes), len(ien) // 3, int(ndof), int(NEQ), _i32(ien),
+                                                _i32(type_), _i32(invP), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            N.lib().femgpu_blockdiag_destroy(self.h)
+            self.h = None
+
+    def setup(self, Ae, F=None):
+        """Blocks from the element matrices Ae (device, element order); F := D^-1 F when given."""
+        N.check(N.lib().femgpu_blockdiag_setup(self.h, _ptr(Ae)))
+        if F is not None:
+            self.apply(F, F)
+
+    def apply(self, p, z):
+        N.check(N.lib().femgpu_blockdiag_apply(self.h, _ptr(p), _ptr(z)))
+
+    def scale(self, Ae, F=None):
+        """Block_scaling_EBE: Ae and F scaled in place."""
+        N.check(N.lib().femgpu_block_scaling(self.h, _ptr(Ae), _ptr(F)))
+
+
+def _system(op, prec):
     S = N.System()
     if isinstance(op, CsrMatrix):
         S.csr = op.h
     else:
         S.ebe = op.h
-    S.ilu = ilu.h if ilu is not None else None
+    if isinstance(prec, BlockDiag):
+        S.bdiag = prec.h
+    else:
+        S.ilu = prec.h if prec is not None else None
     return S
 
 
 def pgmres(dev: Device, op, B, X, KrylovBasisVectorsQuantity=30, LinearMaxIter=1000,
-           SolverTolerance=1e-8, precond: CsrMatrix | None = None):
+           SolverTolerance=1e-8, precond=None):
     """pgmres (pgmres.c:5-200) on device vectors: B already preconditioned, X overwritten.
+    precond: a factored CsrMatrix (ILU(0)) or a BlockDiag, applied on the left.
     Returns the SolveInfo (iterations = Arnoldi steps, as added to Parameters->iterations)."""
     info = N.SolveInfo()
     S = _system(op, precond)

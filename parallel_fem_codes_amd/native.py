@@ -33,7 +33,7 @@ class TransportProblem(C.Structure):
 
 class System(C.Structure):
     """femgpu_system."""
-    _fields_ = [("csr", C.c_void_p), ("ebe", C.c_void_p), ("ilu", C.c_void_p)]
+    _fields_ = [("csr", C.c_void_p), ("ebe", C.c_void_p), ("ilu", C.c_void_p), ("bdiag", C.c_void_p)]
 
 
 class SolveInfo(C.Structure):
@@ -137,6 +137,11 @@ _SIGS = {
     "femgpu_gmres_mixed": (I, [vp, C.POINTER(System), C.POINTER(MixedOpts), D, vp, vp,
                               C.POINTER(SolveInfo)]),
     "femgpu_pcg": (I, [vp, C.POINTER(System), I, D, vp, vp, C.POINTER(SolveInfo)]),
+    "femgpu_blockdiag_create": (I, [vp, I, I, I, I, i32p, i32p, i32p, vp]),
+    "femgpu_blockdiag_destroy": (I, [vp]),
+    "femgpu_blockdiag_setup": (I, [vp, vp]),
+    "femgpu_blockdiag_apply": (I, [vp, vp, vp]),
+    "femgpu_block_scaling": (I, [vp, vp, vp]),
     "femgpu_structured_mesh": (I, [I, I, D, D, D, D, I, f64p, f64p, i32p]),
     "femgpu_structured_marks": (I, [C.c_char_p, I, f64p, f64p, i32p, vp]),
     "femgpu_mesh_file_read": (I, [C.c_char_p, I, I, C.POINTER(MeshFileInfo), vp, vp, vp, vp, vp, vp]),

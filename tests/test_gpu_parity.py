@@ -642,7 +642,7 @@ def test_ns_assembly_bit_exact(dev, torch_cuda, oracle, name):
 
 # ----------------------------------------------------------------------------- Euler assembly
 
-def eu_case(oracle, d=None, m=None, dimensionless=1, shock=1):
+def eu_case(oracle, d=None, m=None, dimensionless=1, shock=1, ebe_storage=None):
     """Euler system on a golden fixture (as the reference numbered it) or on an m x m supersonic
     state: numbering, lm, (CSR) structure and the oracle's Ae / AA / R."""
     if d is not None:
@@ -666,7 +666,7 @@ def eu_case(oracle, d=None, m=None, dimensionless=1, shock=1):
         t4[inflow] = 0
         type4, presc = t4.ravel(), np.where(inflow[:, None], st, 0.0).ravel()
         state, incr = st.ravel(), (1e-2 * rng.standard_normal((nn, 4))).ravel()
-        ebe, reorder = 0, "SYMRCM"
+        ebe, reorder = (1 if ebe_storage else 0), "SYMRCM"
         prm = dict(gamma=1.4, Mach=2.0, alpha=0.5, dt=1e-2, tolerance=1e-8, dimensionless=dimensionless, shock=shock)
     invP, neq = oracle.number_unknowns(type4, 4)
     lm = oracle.fill_lm(ien, type4, invP, neq, 4)
@@ -841,3 +841,104 @@ def test_ns_process_picard_matches_oracle(dev, torch_cuda, oracle):
     assert info.status == 0 and it_ref < maxit
     assert abs(info.iterations - it_ref) <= 1
     assert rel(H(ug), u) < 1e-8
+
+
+# ----------------------------------------------------------------------------- block diagonal
+
+def blockdiag_reference(ien, type_, invP, NEQ, Ae, ndof):
+    """BlockDiag_precond_EBE_setup.c:66-80 restated: nodal diagonal blocks summed in element
+    order, inverted (numpy)."""
+    nn, s = len(type_) // ndof, 3 * ndof
+    A3 = Ae.reshape(-1, s, s)
+    B = np.zeros((nn, ndof, ndof))
+    for e in range(len(ien) // 3):
+        for a in range(3):
+            B[ien[3 * e + a]] += A3[e, ndof * a:ndof * a + ndof, ndof * a:ndof * a + ndof]
+    ids = np.where(np.asarray(type_) == 0, NEQ, invP).reshape(nn, ndof)
+    return np.linalg.inv(B), ids
+
+
+def blockdiag_apply_reference(inv, ids, p, NEQ):
+    """BlockDiag_precond.c:12-26: z = D^-1 p per node, p[NEQ] = 0, the sink not kept."""
+    pp = np.append(p, 0.0)
+    z = np.append(p.copy(), 0.0)
+    for v in range(len(ids)):
+        z[ids[v]] = inv[v] @ pp[ids[v]]
+    return z[:NEQ]
+
+
+def block_scaling_reference(ien, invP, Ae, inv, ndof):
+    """Block_scaling.c:30-56 restated: per element, blocks adjusted for the prescribed dofs
+    (matrix_Bondary_Conditions, :73-91, in the reference's vertex/dof order), then each row block
+    multiplied by the inverse block of its vertex (matrix_Scaling, :94-118)."""
+    s = 3 * ndof
+    out = Ae.reshape(-1, s, s).copy()
+    for e in range(len(ien) // 3):
+        blk = [[out[e, ndof * a:ndof * a + ndof, ndof * b:ndof * b + ndof].copy() for b in range(3)] for a in range(3)]
+        v = ien[3 * e:3 * e + 3]
+        for t in range(ndof):
+            for a in range(3):
+                if invP[v[a] * ndof + t] < 0:
+                    o1, o2 = [b for b in range(3) if b != a]
+                    blk[a][a][t, :] = 0
+                    blk[a][a][:, t] = 0
+                    blk[a][o1][t, :] = 0
+                    blk[a][o2][t, :] = 0
+                    blk[o1][a][:, t] = 0
+                    blk[o2][a][:, t] = 0
+                    blk[a][a][t, t] = 1
+        for a in range(3):
+            for b in range(3):
+                out[e, ndof * a:ndof * a + ndof, ndof * b:ndof * b + ndof] = inv[v[a]] @ blk[a][b]
+    return out.ravel()
+
+
+@pytest.mark.parametrize("ndof", [1, 3, 4])
+def test_blockdiag_and_scaling_match_restatement(dev, torch_cuda, oracle, ndof):
+    torch = torch_cuda
+    x, y, ien = oracle.mesh_structured(6, 5)
+    nn, nel, s = len(x), len(ien) // 3, 3 * ndof
+    rng = np.random.default_rng(ndof)
+    t = (rng.uniform(size=nn * ndof) > 0.2).astype(np.int32)
+    invP, neq = oracle.number_unknowns(t, ndof)
+    Ae = rng.uniform(-1, 1, (nel, s, s)) + 6.0 * np.eye(s)
+    Ae = Ae.ravel()
+    inv, ids = blockdiag_reference(ien, t, invP, neq, Ae, ndof)
+    B = fem.BlockDiag(dev, nn, ien, ndof, t, invP, neq)
+    Ag = T(torch, Ae)
+    B.setup(Ag)
+    p = rng.standard_normal(neq)
+    z = torch.empty(neq, dtype=torch.float64, device="cuda")
+    B.apply(T(torch, p), z)
+    assert rel(H(z), blockdiag_apply_reference(inv, ids, p, neq)) < 1e-12
+    F = T(torch, p)
+    B.scale(Ag, F)
+    ref = block_scaling_reference(ien, invP, Ae, inv, ndof)
+    assert np.abs(H(Ag) - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert rel(H(F), blockdiag_apply_reference(inv, ids, p, neq)) < 1e-12
+
+
+def test_pgmres_ebe_blockdiag_euler(dev, torch_cuda, oracle):
+    """GMRES on the 12x12 Euler EBE operator with the nodal block-diagonal preconditioner
+    (BlockDiag_precond_EBE_setup + BlockDiag_precond, left): the solution of the assembled system
+    to the solver tolerance, against a dense solve."""
+    torch = torch_cuda
+    s = eu_case(oracle, m=10, ebe_storage=True)
+    neq, nn = s["neq"], len(s["x"])
+    E = fem.EbeMatrix(dev, s["lm"], 12, neq)
+    Ag = T(torch, s["Ae"])
+    E.set_values(Ag)
+    B = fem.BlockDiag(dev, nn, s["ien"], 4, s["type4"], s["invP"], neq)
+    F = T(torch, s["R"][:neq])
+    B.setup(Ag, F)                       # F := D^-1 F, as the setup leaves it
+    X = torch.empty(neq, dtype=torch.float64, device="cuda")
+    info = fem.pgmres(dev, E, F, X, 50, 100, 1e-12, precond=B)
+    # dense reference: the assembled operator from the element matrices
+    Ad = np.zeros((neq + 1, neq + 1))
+    A3 = s["Ae"].reshape(-1, 12, 12)
+    lm = s["lm"].reshape(-1, 12)
+    for e in range(len(lm)):
+        Ad[np.ix_(lm[e], lm[e])] += A3[e]
+    xd = np.linalg.solve(Ad[:neq, :neq], s["R"][:neq])
+    assert info.status == 0
+    assert rel(H(X), xd) < 1e-8
