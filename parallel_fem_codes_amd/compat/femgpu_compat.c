@@ -19,6 +19,12 @@
  *   SPARILU_clean / SPARMAT_clean  release the device objects (Postprocess.c:137-141 calls them
  *                       for the ILU preconditioners; the reference structures are never allocated)
  *
+ * Navier-Stokes (9x9 blocks) and Euler (12x12) builds also provide the nodal block-diagonal
+ * preconditioner (BlockDiagDOF3_precond(_EBE_setup) / BlockDiag_precond(_EBE_setup)) and, for Euler,
+ * Block_scaling_EBE with the NO_scaling / NO_unscaling pair of the same file (Block_scaling.c); the
+ * element matrices and F are uploaded, processed on the device and written back, so the driver's
+ * host arrays stay the reference's state.
+ *
  * One rank only (the GPU holds the whole mesh): NPROC != 1 stops with a message. Failures print the
  * library's message and exit(1), as the reference does on fatal errors (mycalloc.c:10-13).
  * ILU(p) with p > 0 and preconditioners other than NO/ILU0 are not provided by the library.
@@ -32,6 +38,7 @@
 #include "MatrixVector_Operations/matvec.h"
 #include "Solvers_and_Preconditioners/solvers.h"
 #include "Solvers_and_Preconditioners/preconditioners.h"
+#include "Solvers_and_Preconditioners/scaling.h"
 
 #include "femgpu.h"
 
@@ -47,6 +54,8 @@ static struct {
     femgpu_ctx *ctx;
     femgpu_csr *csr;
     femgpu_ebe *ebe;
+    femgpu_blockdiag *bdiag;           /* nodal block diagonal (9x9 / 12x12 builds) */
+    double *dA;                        /* element matrices on the device (block diagonal, scaling) */
     const MatrixDataType *csr_owner;   /* the MatrixData whose pattern csr holds */
     int factored;                      /* csr holds ILU(0) factors of the current AA */
     double *dP, *dQ;                   /* NEQ + 1 doubles each */
@@ -121,6 +130,137 @@ static femgpu_ebe *ebe_of(ParametersType *Parameters, MatrixDataType *MatrixData
     return G.ebe;
 }
 
+#if COMPAT_S > 3
+#define COMPAT_ND (COMPAT_S / 3)
+
+/* The node -> dof equations and node -> element lists, built on first use from the driver's Node
+ * and Element arrays (prescribed dofs map to the sink NEQ). */
+static femgpu_blockdiag *bdiag_of(ParametersType *Parameters, FemStructsType *FemStructs)
+{
+    if (G.bdiag)
+        return G.bdiag;
+    const int nn = Parameters->nnodes, nel = Parameters->nel;
+    int *ien = malloc(sizeof(int) * 3 * (size_t)nel), *type = malloc(sizeof(int) * COMPAT_ND * (size_t)nn);
+    int *invP = malloc(sizeof(int) * COMPAT_ND * (size_t)nn);
+    if (!ien || !type || !invP) {
+        printf("libfemgpu: host allocation failed\n");
+        exit(1);
+    }
+    for (int e = 0; e < nel; e++)
+        for (int a = 0; a < 3; a++)
+            ien[3 * e + a] = FemStructs->Element[e].Vertex[a];
+    for (int v = 0; v < nn; v++)
+        for (int j = 0; j < COMPAT_ND; j++) {
+            type[COMPAT_ND * v + j] = FemStructs->Node[v].Type[j] == 0 ? 0 : 1;
+            invP[COMPAT_ND * v + j] = FemStructs->Node[v].Type[j] == 0 ? -1 : FemStructs->Node[v].invP_id[j];
+        }
+    CALL(femgpu_blockdiag_create(G.ctx, nn, nel, COMPAT_ND, Parameters->NEQ, ien, type, invP, &G.bdiag));
+    free(ien);
+    free(type);
+    free(invP);
+    return G.bdiag;
+}
+
+static double *elements_up(ParametersType *Parameters, MatrixDataType *MatrixData)
+{
+    const size_t n = (size_t)Parameters->nel * COMPAT_S * COMPAT_S;
+    if (!G.dA && hipMalloc((void **)&G.dA, sizeof(double) * n) != hipSuccess)
+        fail("element matrices: device allocation");
+    if (hipMemcpy(G.dA, MatrixData->Aaux, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+        fail("element matrices: upload");
+    return G.dA;
+}
+
+/* BlockDiag(DOF3)_precond_EBE_setup: blocks of the current element matrices, F := D^-1 F. */
+static int bdiag_setup(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs, double *F)
+{
+    compat_init(Parameters);
+    femgpu_blockdiag *B = bdiag_of(Parameters, FemStructs);
+    CALL(femgpu_blockdiag_setup(B, elements_up(Parameters, MatrixData)));
+    F[Parameters->NEQ] = 0.0;
+    up(G.dP, F, Parameters->neq);
+    CALL(femgpu_blockdiag_apply(B, G.dP, G.dP));
+    down(F, G.dP, Parameters->neq);
+    return 0;
+}
+
+/* BlockDiag(DOF3)_precond: z = D^-1 p (p[NEQ] and z[NEQ] set to 0, as the reference does). */
+static int bdiag_apply(ParametersType *Parameters, FemStructsType *FemStructs, double *p, double *z)
+{
+    compat_init(Parameters);
+    femgpu_blockdiag *B = bdiag_of(Parameters, FemStructs);
+    p[Parameters->NEQ] = 0.0;
+    up(G.dP, p, Parameters->neq);
+    CALL(femgpu_blockdiag_apply(B, G.dP, G.dQ));
+    down(z, G.dQ, Parameters->neq);
+    z[Parameters->NEQ] = 0.0;
+    return 0;
+}
+
+#if COMPAT_S == 12
+int BlockDiag_precond_EBE_setup(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs,
+                                int tag, double *F)
+{
+    (void)tag;
+    return bdiag_setup(Parameters, MatrixData, FemStructs, F);
+}
+
+int BlockDiag_precond(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs, double *p,
+                      double *z)
+{
+    (void)MatrixData;
+    return bdiag_apply(Parameters, FemStructs, p, z);
+}
+
+/* Block_scaling_EBE (Block_scaling.c:10-59): element matrices and F scaled on the device. */
+int Block_scaling_EBE(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs)
+{
+    compat_init(Parameters);
+    femgpu_blockdiag *B = bdiag_of(Parameters, FemStructs);
+    double *dA = elements_up(Parameters, MatrixData);
+    up(G.dP, FemStructs->F, Parameters->neq);
+    CALL(femgpu_block_scaling(B, dA, G.dP));
+    down(FemStructs->F, G.dP, Parameters->neq);
+    FemStructs->F[Parameters->NEQ] = 0.0;
+    if (hipMemcpy(MatrixData->Aaux, dA, sizeof(double) * (size_t)Parameters->nel * COMPAT_S * COMPAT_S,
+                  hipMemcpyDeviceToHost) != hipSuccess)
+        fail("element matrices: download");
+    return 0;
+}
+
+int NO_scaling(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs)
+{
+    (void)Parameters;
+    (void)MatrixData;
+    (void)FemStructs;
+    return 0;
+}
+
+int NO_unscaling(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs, double *x)
+{
+    (void)Parameters;
+    (void)MatrixData;
+    (void)FemStructs;
+    (void)x;
+    return 0;
+}
+#else
+int BlockDiagDOF3_precond_EBE_setup(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs,
+                                    int tag, double *F)
+{
+    (void)tag;
+    return bdiag_setup(Parameters, MatrixData, FemStructs, F);
+}
+
+int BlockDiagDOF3_precond(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs,
+                          double *p, double *z)
+{
+    (void)MatrixData;
+    return bdiag_apply(Parameters, FemStructs, p, z);
+}
+#endif
+#endif
+
 int csrmv(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs, double *P, double *Q)
 {
     (void)FemStructs;
@@ -191,8 +331,16 @@ int pgmres(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsTyp
         S.ilu = csr_of(Parameters, MatrixData, 0);
         if (!G.factored)
             fail("pgmres: ILU(0) preconditioner used before ILUp_precond_setup");
-    } else if (FemFunctions->precond != NO_precond) {
-        printf("libfemgpu: preconditioner %s is not provided; use NOT or ILU0\n", Parameters->Preconditioner);
+    }
+#if COMPAT_S == 12
+    else if (FemFunctions->precond == BlockDiag_precond)
+        S.bdiag = bdiag_of(Parameters, FemStructs);
+#elif COMPAT_S == 9
+    else if (FemFunctions->precond == BlockDiagDOF3_precond)
+        S.bdiag = bdiag_of(Parameters, FemStructs);
+#endif
+    else if (FemFunctions->precond != NO_precond) {
+        printf("libfemgpu: preconditioner %s is not provided\n", Parameters->Preconditioner);
         exit(1);
     }
     if (FemFunctions->mv == csrmv)
@@ -227,8 +375,11 @@ void SPARILU_clean(SparILU *lu)
         femgpu_csr_destroy(G.csr);
     if (G.ebe)
         femgpu_ebe_destroy(G.ebe);
+    if (G.bdiag)
+        femgpu_blockdiag_destroy(G.bdiag);
     G.csr = NULL;
     G.ebe = NULL;
+    G.bdiag = NULL;
     G.csr_owner = NULL;
     G.factored = 0;
 }
