@@ -55,7 +55,7 @@ int femgpu_csr_nnz(int nel, int s, int neq, const int *lm);
 
 /* csr_Initialization for one rank (01_SS_Transport_Equation/2D/01_CommonFiles/csr_Initialization.c:5-571):
  * IA[neq+1], JA[nnz], scheme[nel*s*s] (slot of each element entry, nnz = dummy), perm[neq]
- * (new -> old). reorder: 0 = NOT, 1 = SYMRCM (Spectral needs HSL_MC73 and is not provided).
+ * (new -> old). reorder: 0 = NOT, 1 = SYMRCM, 2 = Spectral (femgpu_spectral).
  * invP_id of Type-1 dofs is renumbered; lm too when permute_lm != 0 (the transient driver's
  * variant, 02_Transport_Equation/2D/01_CommonFiles/csr_Initialization.c:452-454). */
 int femgpu_csr_init(int nel, int s, int neq, int *lm, int reorder, int permute_lm,
@@ -64,6 +64,15 @@ int femgpu_csr_init(int nel, int s, int neq, int *lm, int reorder, int permute_l
 
 /* REORDERING_SYMRCM (00_CommonFiles/Reordering/symrcm.c:4-237). perm[new] = old. */
 int femgpu_symrcm(int n, const int *IA, const int *JA, int *perm);
+
+/* REORDERING_SPECTRAL (00_CommonFiles/Reordering/reordering.c:27-77): perm[new] = old, the
+ * unknowns by ascending entry of the Fiedler vector of the pattern's graph Laplacian (ties by
+ * index). The reference takes the vector from HSL_MC73 (third party); femgpu_fiedler computes it
+ * independently (multilevel: heavy-edge coarsening, dense Jacobi eigensolve of the coarsest
+ * graph, power-iteration refinement with refine_iters steps per level; femgpu_spectral uses 60),
+ * so the permutation is not the reference's bit for bit. fvec: n entries, zero mean, unit norm. */
+int femgpu_fiedler(int n, const int *IA, const int *JA, int refine_iters, double *fvec);
+int femgpu_spectral(int n, const int *IA, const int *JA, int *perm);
 
 /* Permutation_of_LM (01_SS_Transport_Equation/2D/01_CommonFiles/Permutation_of_LM.c:37-105):
  * Perm/invPerm of NEQ+1 entries, Perm[NEQ] = invPerm[NEQ] = NEQ. */

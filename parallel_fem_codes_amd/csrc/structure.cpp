@@ -270,8 +270,8 @@ extern "C" int femgpu_csr_init(int nel, int s, int neq, int *lm, int reorder, in
 {
     FG_CHECK(nel >= 0 && s > 0 && neq >= 0 && lm && IA && JA && scheme && perm, FEMGPU_ERR_ARG,
              "femgpu_csr_init: bad arguments");
-    FG_CHECK(reorder == 0 || reorder == 1, FEMGPU_ERR_ARG,
-             "femgpu_csr_init: reorder must be 0 (NOT) or 1 (SYMRCM)");
+    FG_CHECK(reorder >= 0 && reorder <= 2, FEMGPU_ERR_ARG,
+             "femgpu_csr_init: reorder must be 0 (NOT), 1 (SYMRCM) or 2 (Spectral)");
     std::vector<int> ia, ja;
     element_pattern(nel, s, neq, lm, ia, ja);
     const int nnz = static_cast<int>(ja.size());
@@ -293,6 +293,7 @@ extern "C" int femgpu_csr_init(int nel, int s, int neq, int *lm, int reorder, in
     // carrying the original slot of every entry (reordering.c:101-167).
     std::iota(perm, perm + neq, 0);
     if (reorder == 1) FG_TRY(femgpu_symrcm(neq, ia.data(), ja.data(), perm));
+    if (reorder == 2) FG_TRY(femgpu_spectral(neq, ia.data(), ja.data(), perm));
     std::vector<int> inv(static_cast<size_t>(neq) + 1);
     for (int i = 0; i < neq; ++i) inv[perm[i]] = i;
     inv[neq] = neq;
@@ -328,15 +329,16 @@ extern "C" int femgpu_ebe_permutation(int nel, int NEQ, const int *ien, const in
 {
     FG_CHECK(nel >= 0 && NEQ >= 0 && ien && type && invP_id && Perm && invPerm, FEMGPU_ERR_ARG,
              "femgpu_ebe_permutation: bad arguments");
-    FG_CHECK(reorder == 0 || reorder == 1, FEMGPU_ERR_ARG, "femgpu_ebe_permutation: bad reorder");
+    FG_CHECK(reorder >= 0 && reorder <= 2, FEMGPU_ERR_ARG, "femgpu_ebe_permutation: bad reorder");
     std::vector<int> lmu(static_cast<size_t>(nel) * 3);
     for (size_t q = 0; q < lmu.size(); ++q) lmu[q] = type[ien[q]] ? invP_id[ien[q]] : -1;
     std::iota(Perm, Perm + NEQ, 0);
     Perm[NEQ] = NEQ;
-    if (reorder == 1) {
+    if (reorder != 0) {
         std::vector<int> IA, JA;
         element_pattern(nel, 3, NEQ, lmu.data(), IA, JA);
-        FG_TRY(femgpu_symrcm(NEQ, IA.data(), JA.data(), Perm));
+        if (reorder == 1) FG_TRY(femgpu_symrcm(NEQ, IA.data(), JA.data(), Perm));
+        else FG_TRY(femgpu_spectral(NEQ, IA.data(), JA.data(), Perm));
     }
     for (int i = 0; i < NEQ; ++i) invPerm[Perm[i]] = i;
     invPerm[NEQ] = NEQ;

@@ -115,7 +115,7 @@ class CsrStructure:
 
 def csr_Initialization(lm, s, neq, reordering="SYMRCM", permute_lm=False, type_=None, invP_id=None):
     """csr_Initialization for one rank: sorted CSR from lm, slot table, reordering and remaps."""
-    code = {"NOT": 0, "SYMRCM": 1}[reordering.upper()]
+    code = {"NOT": 0, "SYMRCM": 1, "SPECTRAL": 2}[reordering.upper()]
     L = N.lib()
     lm = _i32(lm).copy()
     nel = len(lm) // s
@@ -140,8 +140,24 @@ def symrcm(IA, JA):
     return p[:n]
 
 
+def spectral(IA, JA):
+    """REORDERING_SPECTRAL: perm[new] = old by ascending Fiedler-vector entry."""
+    n = len(IA) - 1
+    p = np.empty(max(n, 1), np.int32)
+    N.check(N.lib().femgpu_spectral(n, _i32(IA), _i32(JA), p))
+    return p[:n]
+
+
+def fiedler(IA, JA, refine_iters=60):
+    """The Fiedler vector of the pattern's graph Laplacian (zero mean, unit norm)."""
+    n = len(IA) - 1
+    f = np.empty(max(n, 1))
+    N.check(N.lib().femgpu_fiedler(n, _i32(IA), _i32(JA), int(refine_iters), f))
+    return f[:n]
+
+
 def Permutation_of_LM(ien, type_, invP_id, NEQ, reordering="SYMRCM"):
-    code = {"NOT": 0, "SYMRCM": 1}[reordering.upper()]
+    code = {"NOT": 0, "SYMRCM": 1, "SPECTRAL": 2}[reordering.upper()]
     Perm, invPerm = np.empty(NEQ + 1, np.int32), np.empty(NEQ + 1, np.int32)
     N.check(N.lib().femgpu_ebe_permutation(len(ien) // 3, NEQ, _i32(ien), _i32(type_), _i32(invP_id),
                                            code, Perm, invPerm))

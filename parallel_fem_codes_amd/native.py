@@ -97,6 +97,8 @@ _SIGS = {
     "femgpu_csr_nnz": (I, [I, I, I, i32p]),
     "femgpu_csr_init": (I, [I, I, I, i32p, I, I, I, i32p, i32p, i32p, i32p, i32p, i32p]),
     "femgpu_symrcm": (I, [I, i32p, i32p, i32p]),
+    "femgpu_fiedler": (I, [I, i32p, i32p, I, f64p]),
+    "femgpu_spectral": (I, [I, i32p, i32p, i32p]),
     "femgpu_ebe_permutation": (I, [I, I, i32p, i32p, i32p, I, i32p, i32p]),
     "femgpu_ebe_colour": (I, [I, I, I, i32p, i32p]),
     "femgpu_ilu_levels": (I, [I, i32p, i32p, I, i32p]),
