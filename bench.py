@@ -73,6 +73,41 @@ def cpu_baseline(P, dt, alpha, kmax, tol, seconds=15.0, max_reps=200):
                        f"same {P['neq']}-unknown system, single thread, {el:.2f} s"), X, it
 
 
+def time_integration(fem, torch, M, A, pb, P, dt, alpha, kmax, tol, a):
+    """The workload's time loop: Predictor_Old steps on the device (StopByNorm multicorrection),
+    with the FP64 solver (the oracle path) and the mixed solver from the same initial state; the
+    mixed state's relative L2 error against the FP64 state at steps 1, 100 and the last."""
+    snaps = sorted({1, min(100, a.time_steps), a.time_steps})
+    out = {"steps": a.time_steps, "dt": dt, "alpha": alpha, "nonlinear_tol": a.nonlinear_tol}
+    states = {}
+    for solver in ("fp64", "mixed"):
+        opts = fem.driver_opts(solver, kmax=kmax, tol=tol, nonlinear_tol=a.nonlinear_tol, nonlinear_maxiter=10,
+                               sweep_mode=a.jacobi, factor_sweeps=a.factor_sweeps, inner_tol=a.inner_tol)
+        u = torch.from_numpy(P["u0"]).cuda()
+        acc = torch.zeros_like(u)
+        mc, lin = 0, 0
+        states[solver] = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(1, a.time_steps + 1):
+            info = fem.Predictor_Old_step(M, A, pb, dt, alpha, u, acc, opts)
+            mc += info.iterations
+            lin += info.linear_iterations
+            if k in snaps:
+                states[solver][k] = u.clone()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[solver] = {"s": round(el, 3), "ms_per_step": round(1e3 * el / a.time_steps, 3),
+                       "multicorrections_per_step": round(mc / a.time_steps, 3),
+                       "linear_iterations_per_step": round(lin / a.time_steps, 3)}
+        print(f"# time loop {solver}: {el:.2f} s for {a.time_steps} steps, {mc / a.time_steps:.2f} "
+              f"multicorrections/step", file=sys.stderr, flush=True)
+    out["mixed_rel_err_vs_fp64"] = {
+        str(k): float(torch.linalg.norm(states["mixed"][k] - states["fp64"][k]) / torch.linalg.norm(states["fp64"][k]))
+        for k in snaps}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +119,9 @@ def main():
     ap.add_argument("--factor-sweeps", type=int, default=2, help="parallel FP32 ILU(0) sweeps (mixed)")
     ap.add_argument("--jacobi", type=int, default=2, help="Jacobi iterations per triangle (mixed)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="length of the CPU baseline sample")
+    ap.add_argument("--time-steps", type=int, default=200,
+                    help="predictor-multicorrector steps run after the timed region (0: skip)")
+    ap.add_argument("--nonlinear-tol", type=float, default=1e-4, help="multicorrection stop (StopByNorm)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="counter runs: one mixed step, 20 launches of the roofline kernel and a 256 MiB copy")
     a = ap.parse_args()
@@ -238,6 +276,10 @@ def main():
                     achieved=round(ach, 2), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 5),
                     traffic=traffic, bytes_per_launch=bytes_up, us_per_launch=round(t_up * 1e6, 3))
 
+    time_int = None
+    if a.time_steps > 0:
+        time_int = time_integration(fem, torch, M, A, pb, P, dt, alpha, kmax, tol, a)
+
     cpu = None
     if not a.no_cpu:
         cpu, Xo, ito = cpu_baseline(P, dt, alpha, kmax, tol, seconds=a.cpu_seconds)
@@ -278,6 +320,7 @@ def main():
         "speedup_mixed_vs_fp64": round(mx["solves_per_s"] / fp["solves_per_s"], 3),
         "kernels": {k: {kk: round(vv, 2) for kk, vv in v.items()} for k, v in kernels.items()},
         "roofline": roofline,
+        "time_integration": time_int,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
