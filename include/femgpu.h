@@ -106,6 +106,15 @@ typedef struct femgpu_csr femgpu_csr;
  * sweeps. Values are set separately. */
 int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA, const int *JA,
                       femgpu_csr **out);
+/* As femgpu_csr_create with a choice of the plans built: FEMGPU_PLANS_FP64 (exact ILU(0)
+ * factorisation and level-scheduled sweeps, needed by femgpu_ilu0_factor / _apply, pgmres with
+ * ILU(0) and the mixed solver with sweep_mode 0) and FEMGPU_PLANS_MIXED (parallel FP32 ILU(0) and
+ * Jacobi iterations). femgpu_csr_create builds both; the serving loop builds only the mixed ones.
+ * No reference counterpart. */
+#define FEMGPU_PLANS_FP64 1
+#define FEMGPU_PLANS_MIXED 2
+int femgpu_csr_create_ex(femgpu_ctx *ctx, int n, int nnz, const int *IA, const int *JA, int plans,
+                         femgpu_csr **out);
 int femgpu_csr_destroy(femgpu_csr *A);
 /* AA: nnz values (the reference's dummy slot AA[nnz] is not needed); on_device selects whether
  * AA is a host or a device pointer. Also refreshes the FP32 copy. */
@@ -242,6 +251,27 @@ typedef struct {
  * counterpart. */
 int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *sys, const femgpu_mixed_opts *opt,
                        double tol, const double *d_F, double *d_X, femgpu_solve_info *info);
+
+typedef struct {
+    int refinements;       /* outer FP64 refinement steps while the system was active */
+    int inner_iterations;  /* FP32 Arnoldi steps of the cycles the system took part in */
+    int status;            /* 0 converged, 1 refinement cap reached */
+    int pad;
+    double normF;          /* ||F_s|| */
+    double true_rel_residual;  /* ||F_s - A_s x_s|| / ||F_s|| in FP64 */
+} femgpu_batch_info;
+
+/* Mixed precision for nsys independent systems stored as one block-diagonal CSR matrix (system s on
+ * rows [row_off[s], row_off[s+1]), host array, row_off[0] = 0, row_off[nsys] = n; no entries
+ * across blocks) with its ILU(0) preconditioner (block diagonal as well). FP64 refinement per
+ * system: each system's residual is verified in FP64 and the system is masked out (its block of
+ * the inner right-hand side zeroed, so every Krylov vector is exactly zero there) as soon as
+ * ||F_s - A_s x_s|| <= tol ||F_s||; active systems enter each inner FP32 GMRES cycle with their
+ * residual scaled to unit norm, and the cycle's correction is scaled back per system. One launch
+ * sequence serves the whole batch. info: nsys entries. No reference counterpart. */
+int femgpu_gmres_mixed_batched(femgpu_ctx *ctx, const femgpu_system *sys, int nsys, const int *row_off,
+                               const femgpu_mixed_opts *opt, double tol, const double *d_F, double *d_X,
+                               femgpu_batch_info *info);
 
 /* PCG following Solvers_and_Preconditioners/pcg.c:4-93 (dead code in the reference): stops when
  * gamma < tol^2 gamma0. Returns the iteration count in info->iterations. */
@@ -385,6 +415,57 @@ int femgpu_mesh_file_read(const char *path, int format, int ndof, femgpu_mesh_fi
 /* Writes the generator's format (meshGenerator.c:193-211), byte for byte as the generator prints. */
 int femgpu_mesh_file_write(const char *path, int nnodes, int nel, int ndof, const double *x, const double *y,
                            const int *type, const int *ien);
+
+/* =========================================================== serving loop */
+
+typedef struct femgpu_server femgpu_server;
+
+typedef struct {
+    size_t hbm_budget;     /* device bytes the work in flight may hold (estimated per request) */
+    int executors;         /* concurrent HIP streams, each with its own library context (1..8) */
+    int prep_threads;      /* host threads preparing requests (numbering, lm, SYMRCM CSR) */
+    int small_rows;        /* requests with at most this many unknowns are packed into batches */
+    int batch_rows;        /* unknowns of one batch at most */
+    int max_batch;         /* requests per batch; 0: tuned online among 4, 8, 16, 32, 64 */
+    int kmax;              /* inner GMRES basis size */
+    int pad;
+    double tol;            /* FP64-verified relative residual of every request */
+    femgpu_mixed_opts mixed;   /* FP32 preconditioner (inner_kmax is taken from kmax) */
+} femgpu_server_opts;
+
+typedef struct {
+    int n, nnz;            /* unknowns and CSR entries of the request's system */
+    int batch_id, batch_size;
+    int iterations;        /* FP32 Arnoldi steps the request took part in */
+    int refinements;       /* FP64 refinement steps */
+    int status;            /* 0 converged, 1 refinement cap, < 0 an error status */
+    int pad;
+    double true_rel_residual;      /* ||F - A x|| / ||F|| in FP64 */
+    double t_submit, t_start, t_end;   /* seconds on the server clock */
+} femgpu_request_info;
+
+/* A request loop for steady transport solves (Build_K_F_SUPG + mixed-precision GMRES with the
+ * parallel ILU(0) / Jacobi preconditioner) on meshes of any size, with continuous batching: host
+ * threads prepare requests while executors solve; a large request runs alone, consecutive small
+ * ones are packed into one block-diagonal batched solve (femgpu_gmres_mixed_batched); admission
+ * keeps the estimated device memory of the work in flight under hbm_budget. No reference
+ * counterpart. */
+int femgpu_server_create(int device, const femgpu_server_opts *opt, femgpu_server **out);
+/* Queue one request (host arrays copied: nodes x, y, Type 0/1, Dirichlet values ubc; ien[nel*3];
+ * the problem). Thread-safe. */
+int femgpu_server_submit(femgpu_server *S, int nnodes, int nel, const double *x, const double *y,
+                         const int *type, const double *ubc, const int *ien,
+                         const femgpu_transport_problem *pb, long long *ticket);
+/* Wait up to timeout_s for a request: returns FEMGPU_OK with the nodal solution u[nnodes] and
+ * its info (the ticket is then released), or 1 on timeout. */
+int femgpu_server_wait(femgpu_server *S, long long ticket, double timeout_s, double *u,
+                       femgpu_request_info *info);
+/* Seconds since the server started (the clock of the info timestamps). */
+double femgpu_server_clock(femgpu_server *S);
+/* The batch size in use (0 while the online tuning is still sampling). */
+int femgpu_server_batch_tuning(femgpu_server *S, int *chosen);
+/* Stops accepting requests, finishes the queued ones and joins the threads. */
+int femgpu_server_destroy(femgpu_server *S);
 
 /* =========================================================== nonlinear and time drivers */
 

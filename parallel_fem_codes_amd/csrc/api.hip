@@ -157,7 +157,14 @@ extern "C" int femgpu_sync(femgpu_ctx *ctx)
 
 extern "C" int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA, const int *JA, femgpu_csr **out)
 {
-    FG_CHECK(ctx && out && n >= 0 && nnz >= 0 && IA && JA, FEMGPU_ERR_ARG, "femgpu_csr_create: bad arguments");
+    return femgpu_csr_create_ex(ctx, n, nnz, IA, JA, FEMGPU_PLANS_FP64 | FEMGPU_PLANS_MIXED, out);
+}
+
+extern "C" int femgpu_csr_create_ex(femgpu_ctx *ctx, int n, int nnz, const int *IA, const int *JA, int plans,
+                                    femgpu_csr **out)
+{
+    FG_CHECK(ctx && out && n >= 0 && nnz >= 0 && IA && JA && (plans & ~3) == 0, FEMGPU_ERR_ARG,
+             "femgpu_csr_create: bad arguments");
     FG_CHECK(IA[0] == 0 && IA[n] == nnz, FEMGPU_ERR_ARG, "femgpu_csr_create: IA[0]=%d IA[n]=%d nnz=%d", IA[0], IA[n], nnz);
     std::vector<int> diag(n);
     for (int i = 0; i < n; ++i) {
@@ -184,17 +191,22 @@ extern "C" int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA,
     if (!rc) rc = A->AA32.alloc(static_cast<size_t>(nnz) + 1);
     if (!rc) rc = A->LU.alloc(static_cast<size_t>(nnz) + 1);
     if (!rc) rc = A->Dinv.alloc(static_cast<size_t>(n) + 1);
-    if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), false, A->plo);
-    if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), true, A->pup);
-    if (!rc) rc = build_stream_plan(ctx, n, IA, JA, diag.data(), false, A->slo);
-    if (!rc) rc = build_stream_plan(ctx, n, IA, JA, diag.data(), true, A->sup);
-    if (!rc) rc = build_stream_maps(ctx, A);
-    if (!rc) rc = build_stream_factor(ctx, A, IA, JA, diag.data());
-    if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data(), false, A->fplan);
-    if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data(), true, A->fplan_nat);
-    if (!rc) rc = build_jacobi_ell(ctx, A, IA, JA, diag.data());
+    A->plans = plans;
+    if (plans & FEMGPU_PLANS_FP64) {   // exact factorisation and level-scheduled sweeps
+        if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), false, A->plo);
+        if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), true, A->pup);
+        if (!rc) rc = build_stream_plan(ctx, n, IA, JA, diag.data(), false, A->slo);
+        if (!rc) rc = build_stream_plan(ctx, n, IA, JA, diag.data(), true, A->sup);
+        if (!rc) rc = build_stream_maps(ctx, A);
+        if (!rc) rc = build_stream_factor(ctx, A, IA, JA, diag.data());
+        if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data(), false, A->fplan);
+    }
     if (!rc) rc = build_sell(ctx, A, IA, JA);
-    if (!rc) rc = build_parilu_sell(ctx, A, IA, JA, diag.data());
+    if (plans & FEMGPU_PLANS_MIXED) {   // parallel FP32 ILU(0) and Jacobi iterations
+        if (!rc) rc = build_factor_plan(ctx, A, IA, JA, diag.data(), true, A->fplan_nat);
+        if (!rc) rc = build_jacobi_ell(ctx, A, IA, JA, diag.data());
+        if (!rc) rc = build_parilu_sell(ctx, A, IA, JA, diag.data());
+    }
     if (!rc && hipMemsetAsync(A->AA.p, 0, sizeof(double) * (nnz + 1), s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (rc) {
@@ -285,6 +297,7 @@ extern "C" int femgpu_csrmv_f32(femgpu_csr *A, const float *d_P, float *d_Q)
 extern "C" int femgpu_ilu0_factor(femgpu_csr *A)
 {
     FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_ilu0_factor: null");
+    FG_CHECK(A->plans & FEMGPU_PLANS_FP64, FEMGPU_ERR_STATE, "femgpu_ilu0_factor: matrix created without FP64 plans");
     FG_TRY(launch_ilu_factor(A->ctx, A));
     FG_HIP(hipStreamSynchronize(A->ctx->stream));
     FG_TRY(check_device_error(A->ctx));
@@ -295,6 +308,7 @@ extern "C" int femgpu_ilu0_factor(femgpu_csr *A)
 extern "C" int femgpu_ilu0_factor_host(femgpu_csr *A)
 {
     FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_ilu0_factor_host: null");
+    FG_CHECK(A->plans & FEMGPU_PLANS_FP64, FEMGPU_ERR_STATE, "femgpu_ilu0_factor_host: matrix created without FP64 plans");
     femgpu_ctx *ctx = A->ctx;
     std::vector<double> LU(A->nnz), Dinv(A->n);
     std::vector<int> diag(A->n);

@@ -155,6 +155,8 @@ struct femgpu_ctx {
     double *h_pinned = nullptr;    // pinned host mirror for scalar results
     // Solver workspaces (grown on demand, kept across calls).
     fg::DevBuf<double> gm_basis, gm_small, gm_w, gm_r;
+    fg::DevBuf<int> bt_rowoff, bt_sys;        // batched solver: system row offsets, system of each row
+    fg::DevBuf<double> bt_scal;               // batched solver: per-system norms and scales
     fg::DevBuf<float> gm_basis32, gm_w32;
     fg::DevBuf<int> gm_flags;
     fg::DevBuf<double> drv_ws;         // nonlinear/time drivers: F, X, uold
@@ -204,6 +206,7 @@ struct femgpu_csr {
     fg::FactorPlan fplan_nat;       // the same lists in row order (parallel FP32 factorisation)
     fg::StreamFactor sfac;          // streamed single-workgroup factorisation (when available)
     int spmv_lanes = 8;             // lanes per row of the SpMV kernel
+    int plans = 3;                  // FEMGPU_PLANS_* built at creation
 };
 
 struct femgpu_ebe {

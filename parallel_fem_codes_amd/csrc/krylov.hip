@@ -11,6 +11,7 @@
 // enqueue one step ahead of the convergence check it reads back.
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "common.h"
 #include "device_util.h"
@@ -246,6 +247,69 @@ __global__ void k_scale_to_f32(int n, const double *__restrict__ r, const double
 __global__ void k_sqrt_inplace(double *v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *v = sqrt(*v);
+}
+
+// ---- batched systems: K independent systems stored as one block-diagonal matrix, system s on rows
+// [off[s], off[s+1]).
+
+// System of every row (binary search over the K+1 offsets).
+__global__ void k_sys_of_row(int n, int K, const int *__restrict__ off, int *__restrict__ sys)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int lo = 0, hi = K;   // off[lo] <= i < off[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (off[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        sys[i] = lo;
+    }
+}
+
+// ||v_s||^2 of every system, one block per system (fixed order: deterministic).
+__global__ __launch_bounds__(256) void k_seg_norm2(const int *__restrict__ off, const double *__restrict__ v,
+                                                   double *__restrict__ out)
+{
+    __shared__ double red[4];
+    const int s = blockIdx.x;
+    double acc = 0.0;
+    for (int i = off[s] + threadIdx.x; i < off[s + 1]; i += 256) acc += v[i] * v[i];
+    acc = block_sum<256>(acc, red);
+    if (threadIdx.x == 0) out[s] = acc;
+}
+
+// r32 = r * scale[system]: every active system's residual normalised to unit norm, finished
+// systems zeroed (their blocks of every Krylov vector then stay exactly zero).
+__global__ void k_scale_to_f32_seg(int n, const int *__restrict__ sys, const double *__restrict__ r,
+                                   const double *__restrict__ scale, float *__restrict__ r32)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        r32[k] = static_cast<float>(r[k] * scale[sys[k]]);
+}
+
+// X += rn[system] * (V y): the cycle's correction of each system scaled back by its residual norm.
+__global__ __launch_bounds__(256) void k_cycle_finish_seg(GDev G, int n, const float *__restrict__ V, size_t ldv,
+                                                          double *__restrict__ X, const int *__restrict__ sys,
+                                                          const double *__restrict__ rn)
+{
+#pragma clang fp contract(off)
+    __shared__ double ys[160];
+    const int i = G.flags[1] - 1;
+    if (threadIdx.x == 0 && i >= 0) {
+        ys[i] = G.e[i] / Hat(G, i, i);
+        for (int j = i - 1; j >= 0; --j) {
+            double soma = 0.0;
+            for (int k = j + 1; k <= i; ++k) soma = soma + Hat(G, j, k) * ys[k];
+            ys[j] = (G.e[j] - soma) / Hat(G, j, j);
+        }
+    }
+    __syncthreads();
+    if (i < 0) return;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        double d = 0;
+        for (int j = 0; j <= i; ++j) d = d + static_cast<double>(V[j * ldv + k]) * ys[j];
+        X[k] = X[k] + rn[sys[k]] * d;
+    }
 }
 
 static int grid_for(int n) { return std::max(1, std::min(kReduceBlocks, (n + 1023) / 1024)); }
@@ -601,5 +665,103 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
         info->true_rel_residual = rn / normF;
         info->status = (rn <= tol * normF) ? 0 : 1;
     }
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_gmres_mixed_batched(femgpu_ctx *ctx, const femgpu_system *S, int nsys, const int *row_off,
+                                          const femgpu_mixed_opts *opt, double tol, const double *d_F, double *d_X,
+                                          femgpu_batch_info *info)
+{
+    FG_CHECK(ctx && S && S->csr && S->ilu && opt && d_F && d_X && nsys >= 1 && row_off && info, FEMGPU_ERR_ARG,
+             "femgpu_gmres_mixed_batched: bad arguments");
+    const int n = S->csr->n;
+    FG_CHECK(row_off[0] == 0 && row_off[nsys] == n, FEMGPU_ERR_ARG,
+             "femgpu_gmres_mixed_batched: row offsets must run from 0 to n = %d", n);
+    for (int s = 0; s < nsys; ++s)
+        FG_CHECK(row_off[s + 1] >= row_off[s], FEMGPU_ERR_ARG, "femgpu_gmres_mixed_batched: offsets not monotone");
+    FG_CHECK(opt->sweep_mode >= 0 && opt->sweep_mode <= 16 && opt->factor_sweeps >= 0 && opt->factor_sweeps <= 16 &&
+                 (opt->factor_sweeps == 0 || opt->sweep_mode > 0),
+             FEMGPU_ERR_ARG, "femgpu_gmres_mixed_batched: bad preconditioner options");
+    FG_CHECK(opt->factor_sweeps > 0 || S->ilu->factored, FEMGPU_ERR_STATE,
+             "femgpu_gmres_mixed_batched: ILU(0) not factored");
+    const int kin = opt->inner_kmax;
+    memset(info, 0, sizeof(*info) * static_cast<size_t>(nsys));
+    GDev G;
+    FG_TRY(make_gdev(ctx, kin, G));
+    G.prec_mode = opt->sweep_mode;
+    if (opt->factor_sweeps > 0) {
+        FG_TRY(launch_parilu32(ctx, S->ilu, opt->factor_sweeps));
+        S->ilu->f32_factored = true;
+    } else if (opt->sweep_mode > 0) {
+        FG_TRY(launch_factor_to_f32(ctx, S->ilu));
+        S->ilu->f32_factored = true;
+    }
+    size_t ldv;
+    FG_TRY(ensure_ws<float>(ctx, n, kin, ctx->gm_basis32, ctx->gm_w32, ldv));
+    FG_TRY(ctx->gm_r.alloc(static_cast<size_t>(n) + 64));
+    FG_TRY(ctx->bt_rowoff.upload(row_off, static_cast<size_t>(nsys) + 1, ctx->stream));
+    FG_TRY(ctx->bt_sys.alloc(static_cast<size_t>(n) + 1));
+    FG_TRY(ctx->bt_scal.alloc(3 * static_cast<size_t>(nsys)));
+    double *d_norm = ctx->bt_scal.p, *d_scale = d_norm + nsys, *d_rn = d_scale + nsys;
+    float *V = ctx->gm_basis32.p, *wA = ctx->gm_w32.p, *wB = wA + ldv;
+    double *r = ctx->gm_r.p;
+    const int grid = grid_for(n);
+    if (n > 0) k_sys_of_row<<<grid * 4, 256, 0, ctx->stream>>>(n, nsys, ctx->bt_rowoff.p, ctx->bt_sys.p);
+    FG_HIP(hipMemsetAsync(d_X, 0, sizeof(double) * n, ctx->stream));
+    std::vector<double> normF(nsys), rn(nsys), scale(nsys);
+    std::vector<int> refin(nsys, 0), done(nsys, 0);
+    int inner_total = 0, refinements = 0;
+    auto seg_norms = [&](const double *v, std::vector<double> &out) -> int {
+        k_seg_norm2<<<nsys, 256, 0, ctx->stream>>>(ctx->bt_rowoff.p, v, d_norm);
+        FG_HIP(hipGetLastError());
+        FG_HIP(hipMemcpyAsync(out.data(), d_norm, sizeof(double) * nsys, hipMemcpyDeviceToHost, ctx->stream));
+        FG_HIP(hipStreamSynchronize(ctx->stream));
+        for (double &x : out) x = sqrt(x);
+        return FEMGPU_OK;
+    };
+    FG_TRY(seg_norms(d_F, normF));
+    for (;;) {
+        if (refinements == 0) {
+            FG_HIP(hipMemcpyAsync(r, d_F, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        } else {
+            FG_TRY(launch_residual64(ctx, S->csr, d_X, d_F, r));
+        }
+        FG_TRY(seg_norms(r, rn));
+        int active = 0;
+        for (int s = 0; s < nsys; ++s) {
+            if (!done[s] && (rn[s] <= tol * normF[s] || refin[s] >= opt->max_refinements)) done[s] = 1;
+            scale[s] = done[s] ? 0.0 : 1.0 / rn[s];
+            active += !done[s];
+        }
+        if (active == 0) break;
+        for (int s = 0; s < nsys; ++s) refin[s] += !done[s];
+        FG_HIP(hipMemcpyAsync(d_scale, scale.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, ctx->stream));
+        FG_HIP(hipMemcpyAsync(d_rn, rn.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, ctx->stream));
+        k_scale_to_f32_seg<<<grid * 4, 256, 0, ctx->stream>>>(n, ctx->bt_sys.p, r, d_scale, wA);
+        FG_HIP(hipGetLastError());
+        FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wA, wA, nullptr));
+        cycle_start<float>(ctx, G, n, wA, nullptr, 0, 1, opt->inner_tol);
+        FG_HIP(hipGetLastError());
+        int steps = 0;
+        FG_TRY(run_cycle<float>(ctx, S, G, n, V, ldv, wA, wB, kin, &steps));
+        k_cycle_finish_seg<<<grid, 256, 0, ctx->stream>>>(G, n, V, ldv, d_X, ctx->bt_sys.p, d_rn);
+        FG_HIP(hipGetLastError());
+        FG_HIP(hipMemcpyAsync(&ctx->h_pinned[1], G.flags + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        FG_HIP(hipStreamSynchronize(ctx->stream));
+        FG_TRY(check_device_error(ctx));
+        int dsteps;
+        memcpy(&dsteps, &ctx->h_pinned[1], sizeof(int));
+        inner_total += dsteps;
+        ++refinements;
+        for (int s = 0; s < nsys; ++s)
+            if (!done[s]) info[s].inner_iterations += dsteps;
+    }
+    for (int s = 0; s < nsys; ++s) {
+        info[s].refinements = refin[s];
+        info[s].normF = normF[s];
+        info[s].true_rel_residual = normF[s] > 0 ? rn[s] / normF[s] : 0.0;
+        info[s].status = (rn[s] <= tol * normF[s]) ? 0 : 1;
+    }
+    (void)inner_total;
     return FEMGPU_OK;
 }

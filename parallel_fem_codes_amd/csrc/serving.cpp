@@ -1,0 +1,437 @@
+// Serving loop: steady transport solve requests on meshes of different sizes, solved in mixed
+// precision on one GPU with continuous batching. No reference counterpart (the reference solves one
+// system per MPI job).
+//
+//   submit   -> the request's host arrays are copied and queued (any thread)
+//   prepare  -> host threads number the unknowns, build lm and the SYMRCM-ordered CSR structure
+//   execute  -> executor threads, each with its own HIP stream and library context, take prepared
+//               requests in arrival order: a large system runs alone, consecutive small ones are
+//               packed into one block-diagonal system (one assembly per request into its slice of
+//               the shared CSR values, then one batched mixed-precision solve with per-system
+//               FP64 verification and convergence masks); admission keeps the device memory of
+//               the work in flight under the budget, so requests that arrive while others are on
+//               the GPU start as soon as an executor and memory free up, and the executors' work
+//               overlaps on the device
+//   wait     -> the caller collects the nodal solution and the timings
+// The batch size is autotuned online when max_batch = 0: the executors cycle through candidate
+// sizes, record the throughput (unknowns solved per second) each achieves, and then keep the best.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "common.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    long long id = 0;
+    int nnodes = 0, nel = 0;
+    std::vector<double> x, y, ubc;
+    std::vector<int> type, ien;
+    femgpu_transport_problem pb{};
+    // prepared
+    int n = 0, nnz = 0;
+    std::vector<int> invP, IA, JA, scheme;
+    size_t bytes = 0;
+    int prep_rc = FEMGPU_OK;
+    std::string prep_err;
+    // result
+    std::vector<double> u;
+    femgpu_request_info info{};
+    bool done = false;
+};
+
+using Req = std::shared_ptr<Request>;
+
+constexpr int kBatchCandidates[] = {4, 8, 16, 32, 64};
+
+}  // namespace
+
+struct femgpu_server {
+    int device = 0;
+    femgpu_server_opts o{};
+    Clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv_prep, cv_exec, cv_done;
+    std::deque<Req> submitted;              // waiting for host preparation
+    std::map<long long, Req> prepared;      // ready for the GPU, by arrival id
+    std::map<long long, Req> all;           // every request not yet collected
+    long long next_id = 0, next_batch = 0;
+    size_t inflight = 0;                    // device bytes of the work on the GPU
+    int preparing = 0;                      // requests taken by a preparation thread
+    bool stop = false;
+    std::vector<std::thread> threads;
+    // online batch-size tuning: unknowns per second per candidate, samples
+    double tune_rate[5] = {0, 0, 0, 0, 0};
+    int tune_n[5] = {0, 0, 0, 0, 0};
+    int tuned = -1;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+size_t footprint(const femgpu_server_opts &o, int n, int nnz, int nel)
+{
+    // CSR values (FP64 + FP32), columns, sliced ELL, parallel-ILU iterates and update lists,
+    // Jacobi ELL, FP32 basis of kmax+1 vectors, solver vectors, mesh and element arrays
+    return static_cast<size_t>(nnz) * 64 + static_cast<size_t>(n) * (4 * (o.kmax + 1) + 96) +
+           static_cast<size_t>(nel) * 200 + (1u << 20);
+}
+
+void prepare(Request &r, const femgpu_server_opts &o)
+{
+    r.invP.assign(r.nnodes, -1);
+    int neq = femgpu_number_unknowns(r.nnodes, 1, r.type.data(), r.invP.data());
+    if (neq < 0) {
+        r.prep_rc = neq;
+        r.prep_err = femgpu_last_error();
+        return;
+    }
+    std::vector<int> lm(static_cast<size_t>(r.nel) * 3);
+    int rc = femgpu_fill_lm(r.nel, 1, neq, r.ien.data(), r.type.data(), r.invP.data(), lm.data());
+    int nnz = rc == FEMGPU_OK ? femgpu_csr_nnz(r.nel, 3, neq, lm.data()) : rc;
+    if (nnz < 0) {
+        r.prep_rc = nnz;
+        r.prep_err = femgpu_last_error();
+        return;
+    }
+    r.IA.resize(static_cast<size_t>(neq) + 1);
+    r.JA.resize(std::max(nnz, 1));
+    r.scheme.resize(static_cast<size_t>(r.nel) * 9);
+    std::vector<int> perm(std::max(neq, 1));
+    rc = femgpu_csr_init(r.nel, 3, neq, lm.data(), 1, 0, r.nnodes, r.type.data(), r.invP.data(), r.IA.data(),
+                         r.JA.data(), r.scheme.data(), perm.data());
+    if (rc < 0) {
+        r.prep_rc = rc;
+        r.prep_err = femgpu_last_error();
+        return;
+    }
+    r.n = neq;
+    r.nnz = nnz;
+    r.bytes = footprint(o, neq, nnz, r.nel);
+}
+
+void prep_loop(femgpu_server *S)
+{
+    for (;;) {
+        Req r;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_prep.wait(lk, [&] { return S->stop || !S->submitted.empty(); });
+            if (S->stop && S->submitted.empty()) return;
+            r = S->submitted.front();
+            S->submitted.pop_front();
+            ++S->preparing;
+        }
+        prepare(*r, S->o);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->prepared[r->id] = r;
+            --S->preparing;
+        }
+        S->cv_exec.notify_all();
+    }
+}
+
+int batch_limit(femgpu_server *S, int &cand)
+{
+    cand = -1;
+    if (S->o.max_batch > 0) return S->o.max_batch;
+    if (S->tuned >= 0) return kBatchCandidates[S->tuned];
+    // the candidate with the fewest samples (two each), then the best rate
+    int best = 0;
+    for (int c = 1; c < 5; ++c)
+        if (S->tune_n[c] < S->tune_n[best]) best = c;
+    if (S->tune_n[best] >= 2) {
+        int b = 0;
+        for (int c = 1; c < 5; ++c)
+            if (S->tune_rate[c] / S->tune_n[c] > S->tune_rate[b] / S->tune_n[b]) b = c;
+        S->tuned = b;
+        return kBatchCandidates[b];
+    }
+    cand = best;
+    return kBatchCandidates[best];
+}
+
+// Solve one unit (one large request or a batch of small ones) on the executor's context.
+void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long long batch_id)
+{
+    const int K = static_cast<int>(unit.size());
+    std::vector<int> row_off(K + 1, 0);
+    std::vector<long long> nnz_off(K + 1, 0);
+    for (int s = 0; s < K; ++s) {
+        row_off[s + 1] = row_off[s] + unit[s]->n;
+        nnz_off[s + 1] = nnz_off[s] + unit[s]->nnz;
+    }
+    const int N = row_off[K];
+    const int NNZ = static_cast<int>(nnz_off[K]);
+    std::vector<int> IA(static_cast<size_t>(N) + 1), JA(std::max(NNZ, 1));
+    for (int s = 0; s < K; ++s) {
+        const Request &r = *unit[s];
+        for (int i = 0; i < r.n; ++i) IA[row_off[s] + i] = r.IA[i] + static_cast<int>(nnz_off[s]);
+        for (int p = 0; p < r.nnz; ++p) JA[nnz_off[s] + p] = r.JA[p] + row_off[s];
+    }
+    IA[N] = NNZ;
+    int rc = FEMGPU_OK;
+    std::string err;
+    femgpu_csr *A = nullptr;
+    double *dF = nullptr, *dX = nullptr;
+    std::vector<femgpu_mesh *> meshes(K, nullptr);
+    std::vector<femgpu_batch_info> binfo(K);
+    const double t_start = S->now();
+    auto fail = [&](int code) {
+        if (rc == FEMGPU_OK) {
+            rc = code;
+            err = femgpu_last_error();
+        }
+    };
+    if (N > 0) {
+        if (femgpu_csr_create_ex(ctx, N, NNZ, IA.data(), JA.data(), FEMGPU_PLANS_MIXED, &A) != FEMGPU_OK)
+            fail(FEMGPU_ERR_ARG);
+        if (rc == FEMGPU_OK && (hipMalloc((void **)&dF, sizeof(double) * (N + 64)) != hipSuccess ||
+                                hipMalloc((void **)&dX, sizeof(double) * (N + 64)) != hipSuccess)) {
+            fg::set_error("serving: device allocation failed");
+            fail(FEMGPU_ERR_NOMEM);
+        }
+        double *AA = nullptr;
+        if (rc == FEMGPU_OK && femgpu_csr_values_ptr(A, &AA) != FEMGPU_OK) fail(FEMGPU_ERR_ARG);
+        if (rc == FEMGPU_OK && hipMemsetAsync(dX, 0, sizeof(double) * N, ctx->stream) != hipSuccess) fail(FEMGPU_ERR_HIP);
+        // one assembly per request into its slice of the shared values (ascending order: a slice's
+        // dummy slot is the next slice's first entry until that slice is assembled)
+        for (int s = 0; s < K && rc == FEMGPU_OK; ++s) {
+            const Request &r = *unit[s];
+            if (r.n == 0) continue;
+            if (femgpu_mesh_create(ctx, r.nnodes, r.nel, r.n, r.x.data(), r.y.data(), r.type.data(), r.invP.data(),
+                                   r.ubc.data(), r.ien.data(), nullptr, r.scheme.data(), r.nnz, &meshes[s]) != FEMGPU_OK ||
+                femgpu_assemble_transport(meshes[s], &r.pb, dX + row_off[s], nullptr, dF + row_off[s],
+                                          AA + nnz_off[s]) != FEMGPU_OK)
+                fail(FEMGPU_ERR_ARG);
+        }
+        if (rc == FEMGPU_OK && femgpu_csr_values_changed(A) != FEMGPU_OK) fail(FEMGPU_ERR_ARG);
+        femgpu_system sys = {A, nullptr, A, nullptr};
+        femgpu_mixed_opts mo = S->o.mixed;
+        mo.inner_kmax = S->o.kmax;
+        if (rc == FEMGPU_OK &&
+            femgpu_gmres_mixed_batched(ctx, &sys, K, row_off.data(), &mo, S->o.tol, dF, dX, binfo.data()) != FEMGPU_OK)
+            fail(FEMGPU_ERR_ARG);
+    }
+    std::vector<double> X(static_cast<size_t>(N) + 1, 0.0);
+    if (rc == FEMGPU_OK && N > 0 &&
+        (hipMemcpyAsync(X.data(), dX, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+         hipStreamSynchronize(ctx->stream) != hipSuccess))
+        fail(FEMGPU_ERR_HIP);
+    for (auto *M : meshes)
+        if (M) femgpu_mesh_destroy(M);
+    if (A) femgpu_csr_destroy(A);
+    if (dF) (void)hipFree(dF);
+    if (dX) (void)hipFree(dX);
+    const double t_end = S->now();
+    // results
+    for (int s = 0; s < K; ++s) {
+        Request &r = *unit[s];
+        r.u.assign(r.nnodes, 0.0);
+        for (int v = 0; v < r.nnodes; ++v)
+            r.u[v] = r.type[v] == 0 ? r.ubc[v] : X[row_off[s] + r.invP[v]];
+        r.info.n = r.n;
+        r.info.nnz = r.nnz;
+        r.info.batch_id = static_cast<int>(batch_id);
+        r.info.batch_size = K;
+        r.info.iterations = binfo[s].inner_iterations;
+        r.info.refinements = binfo[s].refinements;
+        r.info.status = rc == FEMGPU_OK ? binfo[s].status : rc;
+        r.info.true_rel_residual = binfo[s].true_rel_residual;
+        r.info.t_start = t_start;
+        r.info.t_end = t_end;
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (auto &r : unit) {
+        r->done = true;
+        S->inflight -= r->bytes;
+    }
+    (void)err;
+}
+
+void exec_loop(femgpu_server *S)
+{
+    if (hipSetDevice(S->device) != hipSuccess) return;
+    hipStream_t stream = nullptr;
+    femgpu_ctx *ctx = nullptr;
+    if (hipStreamCreate(&stream) != hipSuccess || femgpu_create(S->device, stream, &ctx) != FEMGPU_OK) {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+        S->cv_exec.notify_all();
+        S->cv_prep.notify_all();
+        return;
+    }
+    for (;;) {
+        std::vector<Req> unit;
+        long long batch_id = 0;
+        int cand = -1;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            for (;;) {
+                if (S->stop && S->prepared.empty() && S->submitted.empty() && S->preparing == 0) break;
+                if (!S->prepared.empty()) {
+                    // arrival order: the oldest prepared request leads the unit
+                    Req head = S->prepared.begin()->second;
+                    const size_t budget = S->o.hbm_budget;
+                    const bool fits_alone = S->inflight == 0 || S->inflight + head->bytes <= budget;
+                    if (head->prep_rc != FEMGPU_OK || fits_alone) {
+                        unit.push_back(head);
+                        S->prepared.erase(S->prepared.begin());
+                        if (head->prep_rc == FEMGPU_OK && head->n <= S->o.small_rows) {
+                            const int limit = batch_limit(S, cand);
+                            size_t bytes = head->bytes;
+                            int rows = head->n;
+                            while (!S->prepared.empty() && static_cast<int>(unit.size()) < limit) {
+                                Req nx = S->prepared.begin()->second;
+                                if (nx->prep_rc != FEMGPU_OK || nx->n > S->o.small_rows) break;
+                                if (rows + nx->n > S->o.batch_rows) break;
+                                if (S->inflight + bytes + nx->bytes > budget) break;
+                                unit.push_back(nx);
+                                S->prepared.erase(S->prepared.begin());
+                                bytes += nx->bytes;
+                                rows += nx->n;
+                            }
+                        }
+                        for (auto &r : unit) S->inflight += r->prep_rc == FEMGPU_OK ? r->bytes : 0;
+                        batch_id = S->next_batch++;
+                        break;
+                    }
+                }
+                S->cv_exec.wait(lk);
+            }
+        }
+        if (unit.empty()) break;
+        // requests whose preparation failed are answered without the GPU
+        std::vector<Req> good;
+        for (auto &r : unit) {
+            if (r->prep_rc != FEMGPU_OK) {
+                std::lock_guard<std::mutex> lk(S->mu);
+                r->info.status = r->prep_rc;
+                r->info.t_start = r->info.t_end = S->now();
+                r->done = true;
+            } else {
+                good.push_back(r);
+            }
+        }
+        if (!good.empty()) {
+            run_unit(S, ctx, good, batch_id);
+            if (cand >= 0 && good.size() > 1) {
+                double rows = 0;
+                for (auto &r : good) rows += r->n;
+                const double dt = good[0]->info.t_end - good[0]->info.t_start;
+                std::lock_guard<std::mutex> lk(S->mu);
+                if (dt > 0) {
+                    S->tune_rate[cand] += rows / dt;
+                    S->tune_n[cand] += 1;
+                }
+            }
+        }
+        S->cv_done.notify_all();
+        S->cv_exec.notify_all();
+    }
+    femgpu_destroy(ctx);
+    (void)hipStreamDestroy(stream);
+}
+
+}  // namespace
+
+extern "C" int femgpu_server_create(int device, const femgpu_server_opts *o, femgpu_server **out)
+{
+    FG_CHECK(o && out && o->executors >= 1 && o->executors <= 8 && o->prep_threads >= 1 && o->prep_threads <= 64 &&
+                 o->small_rows >= 0 && o->batch_rows >= 1 && o->max_batch >= 0 && o->kmax >= 2 && o->kmax <= 150 &&
+                 o->tol > 0 && o->hbm_budget > 0,
+             FEMGPU_ERR_ARG, "femgpu_server_create: bad options");
+    auto *S = new femgpu_server();
+    S->device = device;
+    S->o = *o;
+    S->t0 = Clock::now();
+    for (int i = 0; i < o->prep_threads; ++i) S->threads.emplace_back(prep_loop, S);
+    for (int i = 0; i < o->executors; ++i) S->threads.emplace_back(exec_loop, S);
+    *out = S;
+    return FEMGPU_OK;
+}
+
+extern "C" double femgpu_server_clock(femgpu_server *S) { return S ? S->now() : 0.0; }
+
+extern "C" int femgpu_server_submit(femgpu_server *S, int nnodes, int nel, const double *x, const double *y,
+                                    const int *type, const double *ubc, const int *ien,
+                                    const femgpu_transport_problem *pb, long long *ticket)
+{
+    FG_CHECK(S && nnodes > 0 && nel >= 0 && x && y && type && ubc && ien && pb && ticket, FEMGPU_ERR_ARG,
+             "femgpu_server_submit: bad arguments");
+    for (int v = 0; v < nnodes; ++v)
+        FG_CHECK(type[v] == 0 || type[v] == 1, FEMGPU_ERR_ARG, "femgpu_server_submit: node %d has Type %d", v, type[v]);
+    for (size_t q = 0; q < static_cast<size_t>(nel) * 3; ++q)
+        FG_CHECK(ien[q] >= 0 && ien[q] < nnodes, FEMGPU_ERR_ARG, "femgpu_server_submit: vertex out of range");
+    auto r = std::make_shared<Request>();
+    r->nnodes = nnodes;
+    r->nel = nel;
+    r->x.assign(x, x + nnodes);
+    r->y.assign(y, y + nnodes);
+    r->ubc.assign(ubc, ubc + nnodes);
+    r->type.assign(type, type + nnodes);
+    r->ien.assign(ien, ien + static_cast<size_t>(nel) * 3);
+    r->pb = *pb;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        FG_CHECK(!S->stop, FEMGPU_ERR_STATE, "femgpu_server_submit: server stopped");
+        r->id = S->next_id++;
+        r->info.t_submit = S->now();
+        S->submitted.push_back(r);
+        S->all[r->id] = r;
+        *ticket = r->id;
+    }
+    S->cv_prep.notify_one();
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_server_wait(femgpu_server *S, long long ticket, double timeout_s, double *u,
+                                  femgpu_request_info *info)
+{
+    FG_CHECK(S, FEMGPU_ERR_ARG, "femgpu_server_wait: null");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->all.find(ticket);
+    FG_CHECK(it != S->all.end(), FEMGPU_ERR_ARG, "femgpu_server_wait: unknown ticket %lld", ticket);
+    Req r = it->second;
+    const bool ok = S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] { return r->done; });
+    if (!ok) return 1;
+    if (u) std::copy(r->u.begin(), r->u.end(), u);
+    if (info) *info = r->info;
+    S->all.erase(it);
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_server_destroy(femgpu_server *S)
+{
+    if (!S) return FEMGPU_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_prep.notify_all();
+    S->cv_exec.notify_all();
+    for (auto &t : S->threads) t.join();
+    delete S;
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_server_batch_tuning(femgpu_server *S, int *chosen)
+{
+    FG_CHECK(S && chosen, FEMGPU_ERR_ARG, "femgpu_server_batch_tuning: null");
+    std::lock_guard<std::mutex> lk(S->mu);
+    *chosen = S->o.max_batch > 0 ? S->o.max_batch : (S->tuned >= 0 ? kBatchCandidates[S->tuned] : 0);
+    return FEMGPU_OK;
+}

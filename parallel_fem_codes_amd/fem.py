@@ -502,6 +502,37 @@ def gmres_mixed(dev: Device, A: CsrMatrix, F, X, tol=1e-8, inner_kmax=30, inner_
     return info
 
 
+def concat_csr(structures):
+    """Block-diagonal concatenation of CSR patterns [(IA, JA), ...]: (IA, JA, row_off, nnz_off)."""
+    row_off, nnz_off = [0], [0]
+    IAs, JAs = [], []
+    for IA, JA in structures:
+        IA, JA = np.asarray(IA), np.asarray(JA)
+        IAs.append(IA[:-1] + nnz_off[-1])
+        JAs.append(JA + row_off[-1])
+        row_off.append(row_off[-1] + len(IA) - 1)
+        nnz_off.append(nnz_off[-1] + int(IA[-1]))
+    IA = np.concatenate(IAs + [np.array([nnz_off[-1]])]).astype(np.int32)
+    JA = np.concatenate(JAs).astype(np.int32)
+    return IA, JA, np.array(row_off, np.int32), np.array(nnz_off, np.int64)
+
+
+def gmres_mixed_batched(dev: Device, A: CsrMatrix, row_off, F, X, tol=1e-8, inner_kmax=30, inner_tol=1e-4,
+                        max_refinements=50, sweep_mode=2, factor_sweeps=2):
+    """Mixed precision over the independent systems of a block-diagonal A (rows row_off[s] ..
+    row_off[s+1]), each verified and masked on its own; returns one BatchInfo per system."""
+    ro = _i32(row_off)
+    K = len(ro) - 1
+    infos = (N.BatchInfo * K)()
+    o = N.MixedOpts()
+    o.inner_kmax, o.inner_cycles, o.inner_tol = inner_kmax, 1, inner_tol
+    o.max_refinements, o.sweep_mode, o.factor_sweeps = max_refinements, sweep_mode, factor_sweeps
+    S = _system(A, A)
+    N.check(N.lib().femgpu_gmres_mixed_batched(dev.h, C.byref(S), K, ro, C.byref(o), float(tol), _ptr(F), _ptr(X),
+                                               infos))
+    return list(infos)
+
+
 def pcg(dev: Device, A: CsrMatrix, B, X, itmax=1000, tol=1e-8, precond: CsrMatrix | None = None):
     info = N.SolveInfo()
     S = _system(A, precond)

@@ -62,6 +62,27 @@ class MixedOpts(C.Structure):
                 ("max_refinements", C.c_int), ("sweep_mode", C.c_int), ("factor_sweeps", C.c_int)]
 
 
+class BatchInfo(C.Structure):
+    """femgpu_batch_info."""
+    _fields_ = [("refinements", C.c_int), ("inner_iterations", C.c_int), ("status", C.c_int), ("pad", C.c_int),
+                ("normF", C.c_double), ("true_rel_residual", C.c_double)]
+
+
+class ServerOpts(C.Structure):
+    """femgpu_server_opts."""
+    _fields_ = [("hbm_budget", C.c_size_t), ("executors", C.c_int), ("prep_threads", C.c_int),
+                ("small_rows", C.c_int), ("batch_rows", C.c_int), ("max_batch", C.c_int), ("kmax", C.c_int),
+                ("pad", C.c_int), ("tol", C.c_double), ("mixed", MixedOpts)]
+
+
+class RequestInfo(C.Structure):
+    """femgpu_request_info."""
+    _fields_ = [("n", C.c_int), ("nnz", C.c_int), ("batch_id", C.c_int), ("batch_size", C.c_int),
+                ("iterations", C.c_int), ("refinements", C.c_int), ("status", C.c_int), ("pad", C.c_int),
+                ("true_rel_residual", C.c_double), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_end", C.c_double)]
+
+
 class DriverOpts(C.Structure):
     """femgpu_driver_opts."""
     _fields_ = [("solver", C.c_int), ("kmax", C.c_int), ("lmax", C.c_int), ("tol", C.c_double),
@@ -106,6 +127,7 @@ _SIGS = {
     "femgpu_destroy": (I, [vp]),
     "femgpu_sync": (I, [vp]),
     "femgpu_csr_create": (I, [vp, I, I, i32p, i32p, C.POINTER(vp)]),
+    "femgpu_csr_create_ex": (I, [vp, I, I, i32p, i32p, I, vp]),
     "femgpu_csr_destroy": (I, [vp]),
     "femgpu_csr_set_values": (I, [vp, vp, I]),
     "femgpu_csr_values_ptr": (I, [vp, C.POINTER(vp)]),
@@ -138,6 +160,7 @@ _SIGS = {
     "femgpu_pgmres": (I, [vp, C.POINTER(System), I, I, D, vp, vp, C.POINTER(SolveInfo)]),
     "femgpu_gmres_mixed": (I, [vp, C.POINTER(System), C.POINTER(MixedOpts), D, vp, vp,
                               C.POINTER(SolveInfo)]),
+    "femgpu_gmres_mixed_batched": (I, [vp, C.POINTER(System), I, i32p, C.POINTER(MixedOpts), D, vp, vp, vp]),
     "femgpu_pcg": (I, [vp, C.POINTER(System), I, D, vp, vp, C.POINTER(SolveInfo)]),
     "femgpu_blockdiag_create": (I, [vp, I, I, I, I, i32p, i32p, i32p, vp]),
     "femgpu_blockdiag_destroy": (I, [vp]),
@@ -148,6 +171,13 @@ _SIGS = {
     "femgpu_structured_marks": (I, [C.c_char_p, I, f64p, f64p, i32p, vp]),
     "femgpu_mesh_file_read": (I, [C.c_char_p, I, I, C.POINTER(MeshFileInfo), vp, vp, vp, vp, vp, vp]),
     "femgpu_mesh_file_write": (I, [C.c_char_p, I, I, I, f64p, f64p, i32p, i32p]),
+    "femgpu_server_create": (I, [I, C.POINTER(ServerOpts), vp]),
+    "femgpu_server_submit": (I, [vp, I, I, f64p, f64p, i32p, f64p, i32p, C.POINTER(TransportProblem),
+                                 C.POINTER(C.c_longlong)]),
+    "femgpu_server_wait": (I, [vp, C.c_longlong, D, vp, C.POINTER(RequestInfo)]),
+    "femgpu_server_clock": (D, [vp]),
+    "femgpu_server_batch_tuning": (I, [vp, C.POINTER(I)]),
+    "femgpu_server_destroy": (I, [vp]),
     "femgpu_st_process": (I, [vp, vp, C.POINTER(TransportProblem), C.POINTER(DriverOpts), vp,
                               C.POINTER(DriverInfo)]),
     "femgpu_ns_process": (I, [vp, vp, D, vp, vp, C.POINTER(DriverOpts), vp, C.POINTER(DriverInfo)]),

@@ -1,0 +1,61 @@
+"""The serving loop (continuous batching, mixed precision) on the GPU: every request converges to
+the FP64-verified 1e-8 residual, its result is the same alone and in the mix and matches the
+plain FP64 path (ILU(0) + pgmres) within the tolerance the 1e-8 residual allows, and small
+requests are actually packed into batches."""
+import numpy as np
+import pytest
+
+from parallel_fem_codes_amd import fem, native, serving
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def reqs():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    native.lib()
+    small = serving.make_requests(10, 7, 200, 2000, rate=1e4)
+    large = serving.make_requests(2, 8, 15000, 20000, rate=1e4)
+    r = small[:5] + large[:1] + small[5:] + large[1:]
+    for k, q in enumerate(r):
+        q["arrival"] = 1e-4 * k
+    return r
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_server_results_alone_in_the_mix_and_fp64(reqs):
+    opts = serving.server_opts(executors=2, prep_threads=4, small_rows=5000, batch_rows=100000, max_batch=4)
+    mix, infos, wall, batch = serving.run_server(reqs, opts)
+    assert batch == 4
+    assert max(i.batch_size for i in infos) > 1                    # small requests were packed
+    assert all(i.status == 0 for i in infos)
+    assert max(i.true_rel_residual for i in infos) <= 1e-8
+    assert all(i.t_end >= i.t_start >= i.t_submit for i in infos)
+    alone_opts = serving.server_opts(executors=1, prep_threads=2, small_rows=5000, batch_rows=100000, max_batch=1)
+    alone, ainfos, _, _ = serving.run_server(reqs, alone_opts)
+    assert all(i.batch_size == 1 and i.status == 0 for i in ainfos)
+    dev = fem.Device(0)
+    worst_alone = worst_fp64 = 0.0
+    for k, r in enumerate(reqs):
+        u64, info = serving.solve_fp64(dev, r)
+        assert info.status == 0
+        worst_alone = max(worst_alone, rel(mix[k], alone[k]))
+        worst_fp64 = max(worst_fp64, rel(mix[k], u64))
+        assert rel(mix[k], alone[k]) < 1e-6, k
+        assert rel(mix[k], u64) < 1e-5, k
+        # Dirichlet values are carried through exactly
+        assert np.array_equal(mix[k][r["type"] == 0], r["ubc"][r["type"] == 0])
+    print(f"worst relative difference: alone vs mix {worst_alone:.2e}, mix vs FP64 {worst_fp64:.2e}")
+    dev.close()
+
+
+def test_server_autotunes_batch_size(reqs):
+    many = serving.make_requests(48, 11, 200, 1500, rate=1e4)
+    opts = serving.server_opts(executors=1, prep_threads=4, small_rows=5000, batch_rows=200000, max_batch=0)
+    sols, infos, wall, batch = serving.run_server(many, opts)
+    assert all(i.status == 0 for i in infos)
+    assert batch in (0, 4, 8, 16, 32, 64)
