@@ -108,6 +108,33 @@ def time_integration(fem, torch, M, A, pb, P, dt, alpha, kmax, tol, a):
     return out
 
 
+def serving_run(count, rate=100.0, nmin=2000, nmax=50000, seed=20261202):
+    """The serving workload (steady solves on Delaunay meshes of 2k-50k points, Poisson arrivals)
+    through the library's request loop, and the same requests through a sequential plain-FP64 GPU
+    loop (the reference's ILU(0) + pgmres)."""
+    from parallel_fem_codes_amd import serving
+    reqs = serving.make_requests(count, seed, nmin, nmax, rate)
+    sols, infos, wall, batch = serving.run_server(reqs, serving.server_opts())
+    lat = np.array([i.t_end - i.t_submit for i in infos])
+    s64, service, lat64, end = serving.run_sequential_fp64(reqs)
+    diff = max(float(np.linalg.norm(u - v) / max(np.linalg.norm(v), 1e-300)) for u, v in zip(sols, s64))
+    out = {"requests": count, "arrival_rate_per_s": rate, "points": [nmin, nmax],
+           "server": {"solves_per_s": round(count / wall, 3), "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+                      "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+                      "max_true_rel_residual": float(max(i.true_rel_residual for i in infos)),
+                      "converged": int(sum(i.status == 0 for i in infos)), "batch_size": batch,
+                      "mean_prep_ms": round(1e3 * float(np.mean([i.prep_s for i in infos])), 3),
+                      "mean_setup_ms": round(1e3 * float(np.mean([i.setup_s for i in infos])), 3),
+                      "mean_solve_ms": round(1e3 * float(np.mean([i.solve_s for i in infos])), 3)},
+           "fp64_sequential": {"solves_per_s": round(count / end, 3),
+                               "p50_ms": round(1e3 * float(np.percentile(lat64, 50)), 3),
+                               "p99_ms": round(1e3 * float(np.percentile(lat64, 99)), 3),
+                               "mean_service_ms": round(1e3 * float(service.mean()), 3)},
+           "max_rel_diff_vs_fp64": diff}
+    print(f"# serving: {out['server']}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +149,8 @@ def main():
     ap.add_argument("--time-steps", type=int, default=200,
                     help="predictor-multicorrector steps run after the timed region (0: skip)")
     ap.add_argument("--nonlinear-tol", type=float, default=1e-4, help="multicorrection stop (StopByNorm)")
+    ap.add_argument("--serving-requests", type=int, default=100,
+                    help="requests of the serving workload run after the timed region (0: skip)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="counter runs: one mixed step, 20 launches of the roofline kernel and a 256 MiB copy")
     a = ap.parse_args()
@@ -280,6 +309,10 @@ def main():
     if a.time_steps > 0:
         time_int = time_integration(fem, torch, M, A, pb, P, dt, alpha, kmax, tol, a)
 
+    serving_out = None
+    if a.serving_requests > 0:
+        serving_out = serving_run(a.serving_requests)
+
     cpu = None
     if not a.no_cpu:
         cpu, Xo, ito = cpu_baseline(P, dt, alpha, kmax, tol, seconds=a.cpu_seconds)
@@ -321,6 +354,7 @@ def main():
         "kernels": {k: {kk: round(vv, 2) for kk, vv in v.items()} for k, v in kernels.items()},
         "roofline": roofline,
         "time_integration": time_int,
+        "serving": serving_out,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -442,6 +442,9 @@ typedef struct {
     int pad;
     double true_rel_residual;      /* ||F - A x|| / ||F|| in FP64 */
     double t_submit, t_start, t_end;   /* seconds on the server clock */
+    double prep_s;         /* host preparation of the request (numbering, lm, CSR + SYMRCM) */
+    double setup_s;        /* its unit's device set-up: CSR plans, meshes, assembly */
+    double solve_s;        /* its unit's batched solve and result download */
 } femgpu_request_info;
 
 /* A request loop for steady transport solves (Build_K_F_SUPG + mixed-precision GMRES with the

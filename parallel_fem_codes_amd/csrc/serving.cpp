@@ -134,7 +134,9 @@ void prep_loop(femgpu_server *S)
             S->submitted.pop_front();
             ++S->preparing;
         }
+        const double t_prep = S->now();
         prepare(*r, S->o);
+        r->info.prep_s = S->now() - t_prep;
         {
             std::lock_guard<std::mutex> lk(S->mu);
             S->prepared[r->id] = r;
@@ -190,6 +192,7 @@ void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long lo
     std::vector<femgpu_mesh *> meshes(K, nullptr);
     std::vector<femgpu_batch_info> binfo(K);
     const double t_start = S->now();
+    double t_setup = t_start;
     auto fail = [&](int code) {
         if (rc == FEMGPU_OK) {
             rc = code;
@@ -219,6 +222,8 @@ void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long lo
                 fail(FEMGPU_ERR_ARG);
         }
         if (rc == FEMGPU_OK && femgpu_csr_values_changed(A) != FEMGPU_OK) fail(FEMGPU_ERR_ARG);
+        if (rc == FEMGPU_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) fail(FEMGPU_ERR_HIP);
+        t_setup = S->now();
         femgpu_system sys = {A, nullptr, A, nullptr};
         femgpu_mixed_opts mo = S->o.mixed;
         mo.inner_kmax = S->o.kmax;
@@ -253,6 +258,8 @@ void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long lo
         r.info.true_rel_residual = binfo[s].true_rel_residual;
         r.info.t_start = t_start;
         r.info.t_end = t_end;
+        r.info.setup_s = t_setup - t_start;
+        r.info.solve_s = t_end - t_setup;
     }
     std::lock_guard<std::mutex> lk(S->mu);
     for (auto &r : unit) {

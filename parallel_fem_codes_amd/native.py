@@ -80,7 +80,7 @@ class RequestInfo(C.Structure):
     _fields_ = [("n", C.c_int), ("nnz", C.c_int), ("batch_id", C.c_int), ("batch_size", C.c_int),
                 ("iterations", C.c_int), ("refinements", C.c_int), ("status", C.c_int), ("pad", C.c_int),
                 ("true_rel_residual", C.c_double), ("t_submit", C.c_double), ("t_start", C.c_double),
-                ("t_end", C.c_double)]
+                ("t_end", C.c_double), ("prep_s", C.c_double), ("setup_s", C.c_double), ("solve_s", C.c_double)]
 
 
 class DriverOpts(C.Structure):

@@ -68,7 +68,7 @@ def make_requests(count, seed, nmin, nmax, rate, kmin=1e-5, kmax=1e-2):
 
 # ----------------------------------------------------------------------------- server
 
-def server_opts(hbm_budget=64 << 30, executors=2, prep_threads=8, small_rows=20000, batch_rows=400000,
+def server_opts(hbm_budget=64 << 30, executors=4, prep_threads=8, small_rows=20000, batch_rows=400000,
                 max_batch=0, kmax=30, tol=1e-8, sweep_mode=2, factor_sweeps=2, inner_tol=1e-4, max_refinements=50):
     o = N.ServerOpts()
     o.hbm_budget, o.executors, o.prep_threads = int(hbm_budget), executors, prep_threads
@@ -202,7 +202,7 @@ def main():
     ap.add_argument("--nmin", type=int, default=2000)
     ap.add_argument("--nmax", type=int, default=50000)
     ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--executors", type=int, default=2)
+    ap.add_argument("--executors", type=int, default=4)
     ap.add_argument("--max-batch", type=int, default=0)
     ap.add_argument("--no-fp64", action="store_true")
     a = ap.parse_args()
@@ -219,7 +219,11 @@ def main():
            "server": {"solves_per_s": len(reqs) / wall, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
                       "p99_ms": 1e3 * float(np.percentile(lat, 99)), "max_true_rel_residual": float(res.max()),
                       "converged": int(sum(i.status == 0 for i in infos)), "batch_size": batch,
-                      "batches": len({i.batch_id for i in infos})}}
+                      "batches": len({i.batch_id for i in infos}),
+                      "mean_prep_ms": 1e3 * float(np.mean([i.prep_s for i in infos])),
+                      "mean_setup_ms": 1e3 * float(np.mean([i.setup_s for i in infos])),
+                      "mean_solve_ms": 1e3 * float(np.mean([i.solve_s for i in infos])),
+                      "mean_queue_ms": 1e3 * float(np.mean([i.t_start - i.t_submit - i.prep_s for i in infos]))}}
     print(f"# server: {out['server']}", file=sys.stderr, flush=True)
     if not a.no_fp64:
         s64, service, lat64, end = run_sequential_fp64(reqs)
