@@ -82,7 +82,8 @@ def time_integration(fem, torch, M, A, pb, P, dt, alpha, kmax, tol, a):
     states = {}
     for solver in ("fp64", "mixed"):
         opts = fem.driver_opts(solver, kmax=kmax, tol=tol, nonlinear_tol=a.nonlinear_tol, nonlinear_maxiter=10,
-                               sweep_mode=a.jacobi, factor_sweeps=a.factor_sweeps, inner_tol=a.inner_tol)
+                               sweep_mode=a.jacobi, factor_sweeps=a.factor_sweeps, inner_tol=a.inner_tol,
+                               orthogonalization=a.orth)
         u = torch.from_numpy(P["u0"]).cuda()
         acc = torch.zeros_like(u)
         mc, lin = 0, 0
@@ -145,6 +146,7 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--factor-sweeps", type=int, default=2, help="parallel FP32 ILU(0) sweeps (mixed)")
     ap.add_argument("--jacobi", type=int, default=2, help="Jacobi iterations per triangle (mixed)")
+    ap.add_argument("--orth", choices=("MGS", "CGS"), default="MGS", help="inner Arnoldi Gram-Schmidt (mixed)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="length of the CPU baseline sample")
     ap.add_argument("--time-steps", type=int, default=200,
                     help="predictor-multicorrector steps run after the timed region (0: skip)")
@@ -191,7 +193,8 @@ def main():
             info_box[mode] = fem.gmres_mixed(dev, A, r, X, tol=tol, inner_kmax=kmax, inner_tol=a.inner_tol)
         else:
             info_box[mode] = fem.gmres_mixed(dev, A, r, X, tol=tol, inner_kmax=kmax, inner_tol=a.inner_tol,
-                                             sweep_mode=a.jacobi, factor_sweeps=a.factor_sweeps)
+                                             sweep_mode=a.jacobi, factor_sweeps=a.factor_sweeps,
+                                             orthogonalization=a.orth)
 
     nnz_u = int(np.count_nonzero(c.JA > np.repeat(np.arange(neq), np.diff(c.IA))))
     if a.profile_kernels:
@@ -341,7 +344,8 @@ def main():
         "mixed": {"solves_per_s": round(mx["solves_per_s"], 4), "inner_iterations": mx["iterations"],
                   "refinements": mx["cycles"], "true_rel_residual": true_rel, "rel_err_vs_fp64": err_vs_fp64,
                   "preconditioner": f"FP32 parallel ILU(0), {a.factor_sweeps} sweeps; {a.jacobi} Jacobi "
-                                    f"iterations per triangle"},
+                                    f"iterations per triangle",
+                  "orthogonalization": a.orth},
         "mixed_exact_ilu": {"solves_per_s": round(mxe["solves_per_s"], 4), "ms_per_step": round(mxe["ms"], 3),
                             "inner_iterations": mxe["iterations"], "refinements": mxe["cycles"],
                             "true_rel_residual": true_rel_mx,

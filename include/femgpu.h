@@ -242,6 +242,9 @@ typedef struct {
                               femgpu_ilu0_factor; k > 0 = k sweeps of the parallel (fixed-point)
                               ILU(0) from the current values, no FP64 factorisation needed
                               (requires sweep_mode > 0) */
+    int orthogonalization; /* inner Arnoldi: 0 modified Gram-Schmidt (one pass per basis vector),
+                              1 classical Gram-Schmidt (all coefficients from one pass, one
+                              update pass: fewer passes over the basis) */
 } femgpu_mixed_opts;
 
 /* Mixed precision: FP64 iterative refinement (residual F - A x, update x += d, Givens and least

@@ -28,6 +28,7 @@ struct GDev {
     int *h_status;    // host-mapped: per step {done} ; h_rho host-mapped rho per step
     double *h_rho;
     int prec_mode;    // FP32 preconditioner (host side only): 0 exact sweeps, k > 0 Jacobi iterations
+    int cgs;          // FP32 inner Arnoldi with classical Gram-Schmidt (host side only)
 };
 
 __device__ __forceinline__ double &Hat(const GDev &G, int r, int c) { return G.H[r * G.ld + c]; }
@@ -153,6 +154,124 @@ __global__ __launch_bounds__(256) void k_mgs_finish(GDev G, int nb, int i)
     if (G.flags[0]) return;
     __shared__ double red[4];
     const double v = sum_partials(G.partials + ((i + 1) & 1) * kReduceBlocks, nb, red);
+    if (threadIdx.x == 0) {
+        const double hn = sqrt(v);
+        Hat(G, i + 1, i) = hn;
+        G.hnorm[i + 1] = hn;
+        G.invh[i + 1] = 1.0 / hn;
+        givens_column(G, i);
+    }
+}
+
+// Classical Gram-Schmidt for the FP32 inner solver: all coefficients h_{j,i} = <V[j], w> from one
+// pass over w and the basis (per-block partial sums, FP64), one kernel that reduces them (a block
+// per coefficient), then one pass w -= sum_j h_{j,i} V[j] (j ascending) fused with the partials of
+// ||w||^2. The coefficients come from the same w (not the successively updated one of MGS), which
+// the inner solve of the refinement tolerates; the outer FP64 loop verifies the answer.
+constexpr int kCgsBlocks = kReduceBlocks, kCgsChunk = 8;
+
+__global__ __launch_bounds__(256) void k_cgs_dots(GDev G, int n, int i, const float *__restrict__ w,
+                                                  const float *__restrict__ V, size_t ldv, double *__restrict__ part)
+{
+    if (G.flags[0]) return;
+    __shared__ double red[4 * kCgsChunk];
+    const int nv = n / 4;
+    for (int j0 = 0; j0 <= i; j0 += kCgsChunk) {
+        double acc[kCgsChunk];
+#pragma unroll
+        for (int c = 0; c < kCgsChunk; ++c) acc[c] = 0.0;
+        for (int k = blockIdx.x * 256 + threadIdx.x; k < nv; k += gridDim.x * 256) {
+            const float4 wv = reinterpret_cast<const float4 *>(w)[k];
+            float4 v[kCgsChunk];
+#pragma unroll
+            for (int c = 0; c < kCgsChunk; ++c)
+                if (j0 + c <= i) v[c] = reinterpret_cast<const float4 *>(V + (j0 + c) * ldv)[k];
+#pragma unroll
+            for (int c = 0; c < kCgsChunk; ++c)
+                if (j0 + c <= i)
+                    acc[c] += static_cast<double>(v[c].x) * wv.x + static_cast<double>(v[c].y) * wv.y +
+                              static_cast<double>(v[c].z) * wv.z + static_cast<double>(v[c].w) * wv.w;
+        }
+        if (blockIdx.x == 0)
+            for (int k = nv * 4 + threadIdx.x; k < n; k += 256)
+#pragma unroll
+                for (int c = 0; c < kCgsChunk; ++c)
+                    if (j0 + c <= i) acc[c] += static_cast<double>(V[(j0 + c) * ldv + k]) * w[k];
+        // all coefficients of the chunk reduced together: wave sums, one barrier, then thread c
+        // adds the four wave sums of coefficient c in wave order
+#pragma unroll
+        for (int c = 0; c < kCgsChunk; ++c) acc[c] = wave_sum(acc[c]);
+        __syncthreads();
+        if (threadIdx.x % 64 == 0)
+#pragma unroll
+            for (int c = 0; c < kCgsChunk; ++c) red[c * 4 + threadIdx.x / 64] = acc[c];
+        __syncthreads();
+        if (threadIdx.x < kCgsChunk && j0 + threadIdx.x <= i)
+            part[(j0 + threadIdx.x) * kCgsBlocks + blockIdx.x] =
+                ((red[threadIdx.x * 4] + red[threadIdx.x * 4 + 1]) + red[threadIdx.x * 4 + 2]) + red[threadIdx.x * 4 + 3];
+    }
+}
+
+// Block j: h_{j,i} = the fixed-order sum of the nb partials of coefficient j.
+__global__ __launch_bounds__(256) void k_cgs_reduce(GDev G, int nb, int i, const double *__restrict__ part)
+{
+    if (G.flags[0]) return;
+    __shared__ double red[4];
+    const int j = blockIdx.x;
+    const double v = sum_partials(part + j * kCgsBlocks, nb, red);
+    if (threadIdx.x == 0) Hat(G, j, i) = v;
+}
+
+// w -= sum_j h_{j,i} V[j] (FP32, j ascending; loads of a chunk of basis vectors issued together) and
+// the partials of ||w||^2.
+__global__ __launch_bounds__(256) void k_cgs_update(GDev G, int n, int i, float *__restrict__ w,
+                                                    const float *__restrict__ V, size_t ldv, double *__restrict__ part)
+{
+    if (G.flags[0]) return;
+    __shared__ double red[4];
+    __shared__ float h[160];
+    for (int j = threadIdx.x; j <= i; j += 256) h[j] = static_cast<float>(Hat(G, j, i));
+    __syncthreads();
+    double acc = 0.0;
+    const int nv = n / 4;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < nv; k += gridDim.x * 256) {
+        float4 wv = reinterpret_cast<const float4 *>(w)[k];
+        for (int j0 = 0; j0 <= i; j0 += kCgsChunk) {
+            float4 v[kCgsChunk];
+#pragma unroll
+            for (int c = 0; c < kCgsChunk; ++c)
+                if (j0 + c <= i) v[c] = reinterpret_cast<const float4 *>(V + (j0 + c) * ldv)[k];
+#pragma unroll
+            for (int c = 0; c < kCgsChunk; ++c)
+                if (j0 + c <= i) {
+                    const float hj = h[j0 + c];
+                    wv.x -= hj * v[c].x;
+                    wv.y -= hj * v[c].y;
+                    wv.z -= hj * v[c].z;
+                    wv.w -= hj * v[c].w;
+                }
+        }
+        reinterpret_cast<float4 *>(w)[k] = wv;
+        acc += static_cast<double>(wv.x) * wv.x + static_cast<double>(wv.y) * wv.y +
+               static_cast<double>(wv.z) * wv.z + static_cast<double>(wv.w) * wv.w;
+    }
+    if (blockIdx.x == 0)
+        for (int k = nv * 4 + threadIdx.x; k < n; k += 256) {
+            float wv = w[k];
+            for (int j = 0; j <= i; ++j) wv -= h[j] * V[j * ldv + k];
+            w[k] = wv;
+            acc += static_cast<double>(wv) * wv;
+        }
+    acc = block_sum<256>(acc, red);
+    if (threadIdx.x == 0) part[(i + 1) * kCgsBlocks + blockIdx.x] = acc;
+}
+
+// h_{i+1,i} = ||w||, then the Givens update of column i.
+__global__ __launch_bounds__(256) void k_cgs_finish(GDev G, int nb, int i, const double *__restrict__ part)
+{
+    if (G.flags[0]) return;
+    __shared__ double red[4];
+    const double v = sum_partials(part + (i + 1) * kCgsBlocks, nb, red);
     if (threadIdx.x == 0) {
         const double hn = sqrt(v);
         Hat(G, i + 1, i) = hn;
@@ -431,7 +550,15 @@ int run_cycle<float>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, fl
         rc = launch_spmv32(ctx, S->csr, wc, G.invh + i, V + i * ldv, wn, G.flags);
         if (!rc) rc = prec32(ctx, S->ilu, G.prec_mode, wn, wn, G.flags);
         if (rc) break;
-        mgs_step<float>(ctx, G, n, i, wn, V, ldv);
+        if (G.cgs) {
+            const int nb = std::max(1, std::min(kCgsBlocks, (n / 4 + 255) / 256));
+            k_cgs_dots<<<nb, 256, 0, ctx->stream>>>(G, n, i, wn, V, ldv, ctx->cgs_partials.p);
+            k_cgs_reduce<<<i + 1, 256, 0, ctx->stream>>>(G, nb, i, ctx->cgs_partials.p);
+            k_cgs_update<<<nb, 256, 0, ctx->stream>>>(G, n, i, wn, V, ldv, ctx->cgs_partials.p);
+            k_cgs_finish<<<1, 256, 0, ctx->stream>>>(G, nb, i, ctx->cgs_partials.p);
+        } else {
+            mgs_step<float>(ctx, G, n, i, wn, V, ldv);
+        }
         if (hipGetLastError() != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
         if (hipEventRecord(ev[i & 1], ctx->stream) != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
         std::swap(wc, wn);
@@ -599,6 +726,8 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
     GDev G;
     FG_TRY(make_gdev(ctx, kin, G));
     G.prec_mode = opt->sweep_mode;
+    G.cgs = opt->orthogonalization == 1;
+    if (G.cgs) FG_TRY(ctx->cgs_partials.alloc(static_cast<size_t>(kin + 2) * kCgsBlocks));
     if (opt->factor_sweeps > 0) {
         FG_TRY(launch_parilu32(ctx, S->ilu, opt->factor_sweeps));
         S->ilu->f32_factored = true;
@@ -689,6 +818,8 @@ extern "C" int femgpu_gmres_mixed_batched(femgpu_ctx *ctx, const femgpu_system *
     GDev G;
     FG_TRY(make_gdev(ctx, kin, G));
     G.prec_mode = opt->sweep_mode;
+    G.cgs = opt->orthogonalization == 1;
+    if (G.cgs) FG_TRY(ctx->cgs_partials.alloc(static_cast<size_t>(kin + 2) * kCgsBlocks));
     if (opt->factor_sweeps > 0) {
         FG_TRY(launch_parilu32(ctx, S->ilu, opt->factor_sweeps));
         S->ilu->f32_factored = true;

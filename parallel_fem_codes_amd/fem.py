@@ -485,17 +485,26 @@ def pgmres(dev: Device, op, B, X, KrylovBasisVectorsQuantity=30, LinearMaxIter=1
     return info
 
 
+def mixed_opts(inner_kmax=30, inner_tol=1e-4, max_refinements=50, sweep_mode=0, factor_sweeps=0,
+               orthogonalization="MGS"):
+    """femgpu_mixed_opts from keyword values (one inner cycle per refinement)."""
+    o = N.MixedOpts()
+    o.inner_kmax, o.inner_cycles, o.inner_tol = inner_kmax, 1, inner_tol
+    o.max_refinements, o.sweep_mode, o.factor_sweeps = max_refinements, sweep_mode, factor_sweeps
+    o.orthogonalization = {"MGS": 0, "CGS": 1}[orthogonalization]
+    return o
+
+
 def gmres_mixed(dev: Device, A: CsrMatrix, F, X, tol=1e-8, inner_kmax=30, inner_tol=1e-4,
-                max_refinements=50, sweep_mode=0, factor_sweeps=0):
+                max_refinements=50, sweep_mode=0, factor_sweeps=0, orthogonalization="MGS"):
     """FP32 ILU(0)-GMRES inside FP64 iterative refinement; F is the raw right-hand side.
 
     sweep_mode 0 applies the exact FP32 sweeps of A's FP64 factorisation (A.ilu0_factor() first);
     sweep_mode k > 0 applies k Jacobi iterations per triangle. factor_sweeps k > 0 builds the FP32
-    factors by k parallel ILU(0) sweeps instead, with no FP64 factorisation."""
+    factors by k parallel ILU(0) sweeps instead, with no FP64 factorisation. orthogonalization
+    "MGS" or "CGS" selects the inner Arnoldi's Gram-Schmidt variant."""
     info = N.SolveInfo()
-    o = N.MixedOpts()
-    o.inner_kmax, o.inner_cycles, o.inner_tol = inner_kmax, 1, inner_tol
-    o.max_refinements, o.sweep_mode, o.factor_sweeps = max_refinements, sweep_mode, factor_sweeps
+    o = mixed_opts(inner_kmax, inner_tol, max_refinements, sweep_mode, factor_sweeps, orthogonalization)
     S = _system(A, A)
     N.check(N.lib().femgpu_gmres_mixed(dev.h, C.byref(S), C.byref(o), float(tol), _ptr(F), _ptr(X),
                                        C.byref(info)))
@@ -518,15 +527,13 @@ def concat_csr(structures):
 
 
 def gmres_mixed_batched(dev: Device, A: CsrMatrix, row_off, F, X, tol=1e-8, inner_kmax=30, inner_tol=1e-4,
-                        max_refinements=50, sweep_mode=2, factor_sweeps=2):
+                        max_refinements=50, sweep_mode=2, factor_sweeps=2, orthogonalization="MGS"):
     """Mixed precision over the independent systems of a block-diagonal A (rows row_off[s] ..
     row_off[s+1]), each verified and masked on its own; returns one BatchInfo per system."""
     ro = _i32(row_off)
     K = len(ro) - 1
     infos = (N.BatchInfo * K)()
-    o = N.MixedOpts()
-    o.inner_kmax, o.inner_cycles, o.inner_tol = inner_kmax, 1, inner_tol
-    o.max_refinements, o.sweep_mode, o.factor_sweeps = max_refinements, sweep_mode, factor_sweeps
+    o = mixed_opts(inner_kmax, inner_tol, max_refinements, sweep_mode, factor_sweeps, orthogonalization)
     S = _system(A, A)
     N.check(N.lib().femgpu_gmres_mixed_batched(dev.h, C.byref(S), K, ro, C.byref(o), float(tol), _ptr(F), _ptr(X),
                                                infos))
@@ -544,14 +551,13 @@ def pcg(dev: Device, A: CsrMatrix, B, X, itmax=1000, tol=1e-8, precond: CsrMatri
 
 def driver_opts(solver="fp64", kmax=30, lmax=1000, tol=1e-8, nonlinear_tol=1e-8, nonlinear_maxiter=50,
                 stop_by_norm=True, inner_kmax=30, inner_tol=1e-4, max_refinements=50, sweep_mode=2,
-                factor_sweeps=2):
+                factor_sweeps=2, orthogonalization="MGS"):
     """Options of the nonlinear/time drivers: solver "fp64" = ILUp_precond_setup + pgmres (the
     reference's pair), "mixed" = gmres_mixed with the given FP32 preconditioner."""
     o = N.DriverOpts()
     o.solver = {"fp64": 0, "mixed": 1}[solver]
     o.kmax, o.lmax, o.tol = kmax, lmax, tol
-    o.mixed.inner_kmax, o.mixed.inner_cycles, o.mixed.inner_tol = inner_kmax, 1, inner_tol
-    o.mixed.max_refinements, o.mixed.sweep_mode, o.mixed.factor_sweeps = max_refinements, sweep_mode, factor_sweeps
+    o.mixed = mixed_opts(inner_kmax, inner_tol, max_refinements, sweep_mode, factor_sweeps, orthogonalization)
     o.nonlinear_tol, o.nonlinear_maxiter, o.stop_by_norm = nonlinear_tol, nonlinear_maxiter, 1 if stop_by_norm else 0
     return o
 

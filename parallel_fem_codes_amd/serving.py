@@ -69,12 +69,12 @@ def make_requests(count, seed, nmin, nmax, rate, kmin=1e-5, kmax=1e-2):
 # ----------------------------------------------------------------------------- server
 
 def server_opts(hbm_budget=64 << 30, executors=4, prep_threads=8, small_rows=20000, batch_rows=400000,
-                max_batch=0, kmax=30, tol=1e-8, sweep_mode=2, factor_sweeps=2, inner_tol=1e-4, max_refinements=50):
+                max_batch=0, kmax=30, tol=1e-8, sweep_mode=2, factor_sweeps=2, inner_tol=1e-4, max_refinements=50,
+                orthogonalization="MGS"):
     o = N.ServerOpts()
     o.hbm_budget, o.executors, o.prep_threads = int(hbm_budget), executors, prep_threads
     o.small_rows, o.batch_rows, o.max_batch, o.kmax, o.tol = small_rows, batch_rows, max_batch, kmax, tol
-    o.mixed.inner_kmax, o.mixed.inner_cycles, o.mixed.inner_tol = kmax, 1, inner_tol
-    o.mixed.max_refinements, o.mixed.sweep_mode, o.mixed.factor_sweeps = max_refinements, sweep_mode, factor_sweeps
+    o.mixed = fem.mixed_opts(kmax, inner_tol, max_refinements, sweep_mode, factor_sweeps, orthogonalization)
     return o
 
 

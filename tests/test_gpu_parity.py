@@ -504,8 +504,9 @@ def transient_system(oracle, m, dt, alpha=0.5):
     return dict(IA=c["IA"], JA=c["JA"], AA=AA, F=R[:neq], neq=neq)
 
 
-@pytest.mark.parametrize("factor_sweeps,sweep_mode", [(0, 2), (3, 2), (2, 3)])
-def test_mixed_parallel_preconditioner(dev, torch_cuda, oracle, factor_sweeps, sweep_mode):
+@pytest.mark.parametrize("factor_sweeps,sweep_mode,orth", [(0, 2, "MGS"), (3, 2, "MGS"), (2, 3, "MGS"),
+                                                           (2, 2, "CGS"), (0, 0, "CGS")])
+def test_mixed_parallel_preconditioner(dev, torch_cuda, oracle, factor_sweeps, sweep_mode, orth):
     """The mixed solver with the parallel FP32 preconditioner reaches 1e-8 (FP64-verified) on the
     transient (mass-dominated, Courant number 0.5) system it is meant for."""
     torch = torch_cuda
@@ -516,10 +517,10 @@ def test_mixed_parallel_preconditioner(dev, torch_cuda, oracle, factor_sweeps, s
         A.ilu0_factor()
     X = torch.empty(s["neq"], dtype=torch.float64, device="cuda")
     info = fem.gmres_mixed(dev, A, T(torch, s["F"]), X, tol=1e-8, inner_kmax=30, inner_tol=1e-4,
-                           sweep_mode=sweep_mode, factor_sweeps=factor_sweeps)
+                           sweep_mode=sweep_mode, factor_sweeps=factor_sweeps, orthogonalization=orth)
     x = H(X)
     res = np.linalg.norm(s["F"] - oracle.csrmv(s["IA"], s["JA"], s["AA"], x)) / np.linalg.norm(s["F"])
-    print(f"factor_sweeps={factor_sweeps} sweep_mode={sweep_mode}: refinements {info.cycles} "
+    print(f"factor_sweeps={factor_sweeps} sweep_mode={sweep_mode} {orth}: refinements {info.cycles} "
           f"inner steps {info.inner_iterations} true rel residual {res:.2e}")
     assert info.status == 0 and res <= 1e-8
 
