@@ -276,6 +276,19 @@ int femgpu_gmres_mixed_batched(femgpu_ctx *ctx, const femgpu_system *sys, int ns
                                const femgpu_mixed_opts *opt, double tol, const double *d_F, double *d_X,
                                femgpu_batch_info *info);
 
+/* The same batched mixed-precision solve as one persistent launch with a workgroup per system:
+ * each system runs its own FP64 refinement and FP32 GMRES(inner_kmax) (its own Krylov space and
+ * Hessenberg matrix, fused MGS, Givens and least squares in LDS) on the sliced-ELL copy of A with
+ * the parallel ILU(0) factors applied by Jacobi iterations, so a system's answer is bit-identical
+ * alone and in any batch, and the whole solve costs one launch. Needs sys->csr == sys->ilu created
+ * with FEMGPU_PLANS_MIXED (rows of at most 16 entries), factor_sweeps 1..16, sweep_mode 1..16,
+ * inner_kmax <= 64. threads: 256, 512 or 1024 per workgroup, 0 = chosen from the mean system size.
+ * For many small systems (launch latency, not bandwidth, bounds their multi-launch solve). No
+ * reference counterpart. */
+int femgpu_gmres_mixed_wg(femgpu_ctx *ctx, const femgpu_system *sys, int nsys, const int *row_off,
+                          const femgpu_mixed_opts *opt, double tol, int threads, const double *d_F, double *d_X,
+                          femgpu_batch_info *info);
+
 /* PCG following Solvers_and_Preconditioners/pcg.c:4-93 (dead code in the reference): stops when
  * gamma < tol^2 gamma0. Returns the iteration count in info->iterations. */
 int femgpu_pcg(femgpu_ctx *ctx, const femgpu_system *sys, int itmax, double tol,

@@ -157,6 +157,7 @@ struct femgpu_ctx {
     fg::DevBuf<double> gm_basis, gm_small, gm_w, gm_r;
     fg::DevBuf<int> bt_rowoff, bt_sys;        // batched solver: system row offsets, system of each row
     fg::DevBuf<double> cgs_partials;          // classical Gram-Schmidt: per-block partial dot products
+    fg::DevBuf<femgpu_batch_info> wg_info;    // persistent batched solver: per-system results
     fg::DevBuf<double> bt_scal;               // batched solver: per-system norms and scales
     fg::DevBuf<float> gm_basis32, gm_w32;
     fg::DevBuf<int> gm_flags;

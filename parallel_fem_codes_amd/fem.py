@@ -540,6 +540,20 @@ def gmres_mixed_batched(dev: Device, A: CsrMatrix, row_off, F, X, tol=1e-8, inne
     return list(infos)
 
 
+def gmres_mixed_wg(dev: Device, A: CsrMatrix, row_off, F, X, tol=1e-8, inner_kmax=30, inner_tol=1e-4,
+                   max_refinements=50, sweep_mode=2, factor_sweeps=2, threads=0):
+    """The batched mixed-precision solve as one persistent launch, a workgroup per system (its own
+    Krylov space: the answer does not depend on the batch); returns one BatchInfo per system."""
+    ro = _i32(row_off)
+    K = len(ro) - 1
+    infos = (N.BatchInfo * K)()
+    o = mixed_opts(inner_kmax, inner_tol, max_refinements, sweep_mode, factor_sweeps)
+    S = _system(A, A)
+    N.check(N.lib().femgpu_gmres_mixed_wg(dev.h, C.byref(S), K, ro, C.byref(o), float(tol), int(threads), _ptr(F),
+                                          _ptr(X), infos))
+    return list(infos)
+
+
 def pcg(dev: Device, A: CsrMatrix, B, X, itmax=1000, tol=1e-8, precond: CsrMatrix | None = None):
     info = N.SolveInfo()
     S = _system(A, precond)

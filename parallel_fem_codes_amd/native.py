@@ -162,6 +162,7 @@ _SIGS = {
     "femgpu_gmres_mixed": (I, [vp, C.POINTER(System), C.POINTER(MixedOpts), D, vp, vp,
                               C.POINTER(SolveInfo)]),
     "femgpu_gmres_mixed_batched": (I, [vp, C.POINTER(System), I, i32p, C.POINTER(MixedOpts), D, vp, vp, vp]),
+    "femgpu_gmres_mixed_wg": (I, [vp, C.POINTER(System), I, i32p, C.POINTER(MixedOpts), D, I, vp, vp, vp]),
     "femgpu_pcg": (I, [vp, C.POINTER(System), I, D, vp, vp, C.POINTER(SolveInfo)]),
     "femgpu_blockdiag_create": (I, [vp, I, I, I, I, i32p, i32p, i32p, vp]),
     "femgpu_blockdiag_destroy": (I, [vp]),
