@@ -115,8 +115,9 @@ def serving_run(count, rate=100.0, nmin=2000, nmax=50000, seed=20261202):
     loop (the reference's ILU(0) + pgmres)."""
     from parallel_fem_codes_amd import serving
     reqs = serving.make_requests(count, seed, nmin, nmax, rate)
-    sols, infos, wall, batch = serving.run_server(reqs, serving.server_opts())
+    sols, infos, wall, batch, wg = serving.run_server(reqs, serving.server_opts(), warmup=8)
     lat = np.array([i.t_end - i.t_submit for i in infos])
+    iso = serving.run_isolated(reqs)
     s64, service, lat64, end = serving.run_sequential_fp64(reqs)
     diff = max(float(np.linalg.norm(u - v) / max(np.linalg.norm(v), 1e-300)) for u, v in zip(sols, s64))
     out = {"requests": count, "arrival_rate_per_s": rate, "points": [nmin, nmax],
@@ -124,9 +125,13 @@ def serving_run(count, rate=100.0, nmin=2000, nmax=50000, seed=20261202):
                       "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
                       "max_true_rel_residual": float(max(i.true_rel_residual for i in infos)),
                       "converged": int(sum(i.status == 0 for i in infos)), "batch_size": batch,
+                      "wg_threads": wg, "persistent_launch_requests": int(sum(i.wg_threads > 0 for i in infos)),
                       "mean_prep_ms": round(1e3 * float(np.mean([i.prep_s for i in infos])), 3),
                       "mean_setup_ms": round(1e3 * float(np.mean([i.setup_s for i in infos])), 3),
                       "mean_solve_ms": round(1e3 * float(np.mean([i.solve_s for i in infos])), 3)},
+           "isolated": {"p50_ms": round(1e3 * float(np.percentile(iso, 50)), 3),
+                        "p99_ms": round(1e3 * float(np.percentile(iso, 99)), 3)},
+           "p99_loaded_over_isolated": round(float(np.percentile(lat, 99) / np.percentile(iso, 99)), 3),
            "fp64_sequential": {"solves_per_s": round(count / end, 3),
                                "p50_ms": round(1e3 * float(np.percentile(lat64, 50)), 3),
                                "p99_ms": round(1e3 * float(np.percentile(lat64, 99)), 3),

@@ -444,7 +444,13 @@ typedef struct {
     int batch_rows;        /* unknowns of one batch at most */
     int max_batch;         /* requests per batch; 0: tuned online among 4, 8, 16, 32, 64 */
     int kmax;              /* inner GMRES basis size */
-    int pad;
+    int small_solver;      /* batches of small requests: 0 the block-diagonal shared-Krylov solver
+                              (femgpu_gmres_mixed_batched), 1 the persistent launch with a workgroup
+                              per system (femgpu_gmres_mixed_wg) when every system of the batch has
+                              at most 3072 rows, else 0, 2 always the persistent launch; the
+                              persistent launch needs kmax <= 64 */
+    int wg_threads;        /* threads per workgroup of the persistent launch (256, 512, 1024); 0:
+                              tuned online among the three once the batch size is settled */
     double tol;            /* FP64-verified relative residual of every request */
     femgpu_mixed_opts mixed;   /* FP32 preconditioner (inner_kmax is taken from kmax) */
 } femgpu_server_opts;
@@ -455,7 +461,7 @@ typedef struct {
     int iterations;        /* FP32 Arnoldi steps the request took part in */
     int refinements;       /* FP64 refinement steps */
     int status;            /* 0 converged, 1 refinement cap, < 0 an error status */
-    int pad;
+    int wg_threads;        /* threads per workgroup when solved by the persistent launch, else 0 */
     double true_rel_residual;      /* ||F - A x|| / ||F|| in FP64 */
     double t_submit, t_start, t_end;   /* seconds on the server clock */
     double prep_s;         /* host preparation of the request (numbering, lm, CSR + SYMRCM) */
@@ -483,6 +489,8 @@ int femgpu_server_wait(femgpu_server *S, long long ticket, double timeout_s, dou
 double femgpu_server_clock(femgpu_server *S);
 /* The batch size in use (0 while the online tuning is still sampling). */
 int femgpu_server_batch_tuning(femgpu_server *S, int *chosen);
+/* The persistent launch's threads per workgroup in use (0 while still sampling). */
+int femgpu_server_wg_tuning(femgpu_server *S, int *chosen);
 /* Stops accepting requests, finishes the queued ones and joins the threads. */
 int femgpu_server_destroy(femgpu_server *S);
 

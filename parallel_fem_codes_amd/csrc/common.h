@@ -290,5 +290,6 @@ int launch_block_gathers(femgpu_mesh *M, const double *Ke, double *F, double *AA
 void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vector<int> &ptr, std::vector<int> &src,
              std::vector<int> &dst);
 int check_device_error(femgpu_ctx *ctx);
+int wg_threads_for(long long rows, int nsys);   // persistent batched solver: default workgroup size
 
 }  // namespace fg

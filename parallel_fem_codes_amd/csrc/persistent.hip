@@ -51,61 +51,66 @@ __device__ __forceinline__ double wg_sum(double v, double *red)
     return t;
 }
 
-__device__ __forceinline__ int sell_pos(const int *sptr, int r, int &w)
+// Row loops are unrolled so that a thread has several rows' loads in flight, and the entry loop of
+// a row is unrolled to the matrix's row-width bound QW with its loads unconditional (positions past
+// the row's end are clamped to its first entry and their terms not added): the loads of a row go
+// out together instead of one dependent round trip per entry, and the sums are the same for any QW.
+template <int QW>
+__device__ __forceinline__ float sell_row_dot(const WgArgs &a, const float *vals, int p, int q0, int q1,
+                                              const float *x)
 {
-    const int p0 = sptr[r >> 6];
-    w = (sptr[(r >> 6) + 1] - p0) >> 6;
-    return p0 + (r & 63);
+    float t = 0.0f;
+#pragma unroll
+    for (int u = 0; u < QW; ++u) {
+        const int q = q0 + u;
+        const bool in = q < q1;
+        const int e = p + ((in ? q : 0) << 6);
+        const float v = vals[e] * x[a.scol[e]];
+        t = in ? t + v : t;
+    }
+    return t;
 }
 
-// out = (U~ L~)^-1 b approximated by jac Jacobi iterations per triangle (the iteration of the
-// multi-kernel path): y_k = b - L y_{k-1} (y_0 = b), then z_k = Dinv (y - U z_{k-1}) (z_0 = Dinv y).
-// b is T0; T1, T2 take the lower iterates; the upper iterates alternate over the two free buffers
-// and the last one lands in out.
-template <int BS>
+// z = (U~ L~)^-1 b approximated by jac Jacobi iterations per triangle (the iteration of the
+// multi-kernel path): y_k = b - L y_{k-1} (y_0 = b), then z_k = Dinv (y - U z_{k-1})
+// (z_0 = Dinv y, formed first into a free buffer). b is T0; the lower iterates go to T1/T2, the
+// upper ones alternate over the two free buffers, and the last one lands in out.
+template <int BS, int QW>
 __device__ void wg_jacobi(const WgArgs &a, int r0, int r1, float *T0, float *T1, float *T2, float *out)
 {
     const float *prev = nullptr;
     for (int k = 0; k < a.jac; ++k) {
         float *dst = (k & 1) ? T2 : T1;
         const float *src = prev ? prev : T0;
+#pragma unroll 2
         for (int r = r0 + threadIdx.x; r < r1; r += BS) {
-            int w;
-            const int p = sell_pos(a.sptr, r, w);
-            const int nL = a.rinfo[r] & 0xff;
-            float acc = 0.0f;
-            for (int q = 0; q < nL; ++q) {
-                const int e = p + (q << 6);
-                acc += a.LU[e] * src[a.scol[e]];
-            }
-            dst[r] = T0[r] - acc;
+            const int p = a.sptr[r >> 6] + (r & 63), nl = a.rinfo[r] & 0xff;
+            dst[r] = T0[r] - sell_row_dot<QW>(a, a.LU, p, 0, nl, src);
         }
         __syncthreads();
         prev = dst;
     }
     const float *y = prev;
     float *fa = T0, *fb = (y == T1) ? T2 : T1;
-    const float *zin = nullptr;
+    // z_0 = Dinv y into fb; the iterates then alternate fa, fb, ..., the last one into out
+#pragma unroll 2
+    for (int r = r0 + threadIdx.x; r < r1; r += BS) fb[r] = a.Dinv[r] * y[r];
+    __syncthreads();
+    const float *zin = fb;
     for (int k = 1; k <= a.jac; ++k) {
         float *dst = (k == a.jac) ? out : ((k & 1) ? fa : fb);
+#pragma unroll 2
         for (int r = r0 + threadIdx.x; r < r1; r += BS) {
-            int w;
-            const int p = sell_pos(a.sptr, r, w);
-            const int nL = a.rinfo[r] & 0xff, len = a.rinfo[r] >> 8;
-            float acc = 0.0f;
-            for (int q = nL + 1; q < len; ++q) {
-                const int e = p + (q << 6);
-                const int j = a.scol[e];
-                acc += a.LU[e] * (zin ? zin[j] : a.Dinv[j] * y[j]);
-            }
-            dst[r] = a.Dinv[r] * (y[r] - acc);
+            const int p = a.sptr[r >> 6] + (r & 63), inf = a.rinfo[r];
+            const int nl = inf & 0xff, len = inf >> 8;
+            dst[r] = a.Dinv[r] * (y[r] - sell_row_dot<QW>(a, a.LU, p, nl + 1, len, zin));
         }
         __syncthreads();
         zin = dst;
     }
 }
 
-template <int BS>
+template <int BS, int QW>
 __global__ __launch_bounds__(BS) void k_gmres_wg(WgArgs a)
 {
     extern __shared__ double sm[];
@@ -126,13 +131,21 @@ __global__ __launch_bounds__(BS) void k_gmres_wg(WgArgs a)
     int refinements = 0, inner = 0;
     double rn = normF;
     while (normF > 0.0) {
-        // outer FP64 residual r = F - A X (X = 0 on the first pass)
+        // outer FP64 residual r = F - A X (X = 0 on the first pass), CSR entries in row order
         acc = 0.0;
+#pragma unroll 2
         for (int r = r0 + tid; r < r1; r += BS) {
             double v = a.F[r];
             if (refinements > 0) {
+                const int p0 = a.IA[r], len = a.IA[r + 1] - p0;
                 double s = 0.0;
-                for (int p = a.IA[r]; p < a.IA[r + 1]; ++p) s += a.AA[p] * a.X[a.JA[p]];
+#pragma unroll
+                for (int q = 0; q < QW; ++q) {
+                    const bool in = q < len;
+                    const int p = p0 + (in ? q : 0);
+                    const double t = a.AA[p] * a.X[a.JA[p]];
+                    s = in ? s + t : s;
+                }
                 v = v - s;
             }
             a.R[r] = v;
@@ -143,16 +156,19 @@ __global__ __launch_bounds__(BS) void k_gmres_wg(WgArgs a)
         ++refinements;
         // inner FP32 solve of M^-1 A d = M^-1 r / ||r||, started from d = 0
         const double inv_rn = 1.0 / rn;
+#pragma unroll 4
         for (int r = r0 + tid; r < r1; r += BS) T0[r] = static_cast<float>(a.R[r] * inv_rn);
         __syncthreads();
         float *V0 = a.V;
-        wg_jacobi<BS>(a, r0, r1, T0, T1, T2, V0);
+        wg_jacobi<BS, QW>(a, r0, r1, T0, T1, T2, V0);
         acc = 0.0;
+#pragma unroll 4
         for (int r = r0 + tid; r < r1; r += BS) acc += static_cast<double>(V0[r]) * V0[r];
         const double e0 = sqrt(wg_sum<BS>(acc, red));
         if (!(e0 > 0.0)) break;   // preconditioned residual vanished in FP32: nothing left to correct
         const double eps = a.inner_tol * e0;
         const float inv0 = static_cast<float>(1.0 / e0);
+#pragma unroll 4
         for (int r = r0 + tid; r < r1; r += BS) V0[r] = V0[r] * inv0;
         if (tid == 0) e[0] = e0;
         __syncthreads();
@@ -160,24 +176,20 @@ __global__ __launch_bounds__(BS) void k_gmres_wg(WgArgs a)
         for (int i = 0; i < a.kin - 1; ++i) {
             const float *Vi = a.V + i * a.ldv;
             float *W = a.V + (i + 1) * a.ldv;
+#pragma unroll 2
             for (int r = r0 + tid; r < r1; r += BS) {   // T0 = A v_i (sliced ELL, FP32)
-                int w;
-                const int p = sell_pos(a.sptr, r, w);
-                float s = 0.0f;
-                for (int q = 0; q < w; ++q) {
-                    const int ei = p + (q << 6);
-                    s += a.sval[ei] * Vi[a.scol[ei]];
-                }
-                T0[r] = s;
+                const int p = a.sptr[r >> 6] + (r & 63), len = a.rinfo[r] >> 8;
+                T0[r] = sell_row_dot<QW>(a, a.sval, p, 0, len, Vi);
             }
             __syncthreads();
-            wg_jacobi<BS>(a, r0, r1, T0, T1, T2, W);   // W = M^-1 A v_i
+            wg_jacobi<BS, QW>(a, r0, r1, T0, T1, T2, W);   // W = M^-1 A v_i
             // MGS: pass j subtracts h_{j-1,i} v_{j-1} and accumulates <v_j, w> (j = i + 1: ||w||^2)
             double hprev = 0.0;
             for (int j = 0; j <= i + 1; ++j) {
                 const float *Vj = a.V + j * a.ldv, *Vp = a.V + (j > 0 ? j - 1 : 0) * a.ldv;
                 const float hp = static_cast<float>(hprev);
                 acc = 0.0;
+#pragma unroll 4
                 for (int r = r0 + tid; r < r1; r += BS) {
                     float wv = W[r];
                     if (j > 0) {
@@ -215,6 +227,7 @@ __global__ __launch_bounds__(BS) void k_gmres_wg(WgArgs a)
             steps = i + 1;
             if (s_done) break;
             const float invh = static_cast<float>(1.0 / hn);
+#pragma unroll 4
             for (int r = r0 + tid; r < r1; r += BS) W[r] = W[r] * invh;
             __syncthreads();
         }
@@ -230,6 +243,7 @@ __global__ __launch_bounds__(BS) void k_gmres_wg(WgArgs a)
             }
         }
         __syncthreads();
+#pragma unroll 2
         for (int r = r0 + tid; r < r1; r += BS) {   // X += ||r|| * V y
             double d = 0.0;
             for (int j = 0; j < steps; ++j) d = d + static_cast<double>(a.V[j * a.ldv + r]) * y[j];
@@ -249,15 +263,16 @@ __global__ __launch_bounds__(BS) void k_gmres_wg(WgArgs a)
     }
 }
 
+}  // namespace
+
 // Workgroup size from the systems' mean size when the caller leaves it to the library: 4 waves up
-// to 4096 rows, 8 up to 16384, 16 above.
+// to 2048 rows, 8 up to 8192, 16 above.
 int wg_threads_for(long long rows, int nsys)
 {
     const long long mean = rows / std::max(1, nsys);
-    return mean <= 4096 ? 256 : mean <= 16384 ? 512 : 1024;
+    return mean <= 2048 ? 256 : mean <= 8192 ? 512 : 1024;
 }
 
-}  // namespace
 }  // namespace fg
 
 using namespace fg;
@@ -319,10 +334,21 @@ extern "C" int femgpu_gmres_mixed_wg(femgpu_ctx *ctx, const femgpu_system *S, in
     a.info = ctx->wg_info.p;
     const int ld = kin + 1;
     const size_t lds = sizeof(double) * (static_cast<size_t>(ld) * ld + 4 * ld + 16);
+    // row-width bound of the unrolled entry loops: the sliced-ELL layout holds rows of <= 16 entries
+    const bool narrow = A->par_maxw <= 8;
     switch (threads) {
-    case 256: k_gmres_wg<256><<<nsys, 256, lds, ctx->stream>>>(a); break;
-    case 512: k_gmres_wg<512><<<nsys, 512, lds, ctx->stream>>>(a); break;
-    default: k_gmres_wg<1024><<<nsys, 1024, lds, ctx->stream>>>(a); break;
+    case 256:
+        if (narrow) k_gmres_wg<256, 8><<<nsys, 256, lds, ctx->stream>>>(a);
+        else k_gmres_wg<256, 16><<<nsys, 256, lds, ctx->stream>>>(a);
+        break;
+    case 512:
+        if (narrow) k_gmres_wg<512, 8><<<nsys, 512, lds, ctx->stream>>>(a);
+        else k_gmres_wg<512, 16><<<nsys, 512, lds, ctx->stream>>>(a);
+        break;
+    default:
+        if (narrow) k_gmres_wg<1024, 8><<<nsys, 1024, lds, ctx->stream>>>(a);
+        else k_gmres_wg<1024, 16><<<nsys, 1024, lds, ctx->stream>>>(a);
+        break;
     }
     FG_HIP(hipGetLastError());
     FG_HIP(hipMemcpyAsync(info, ctx->wg_info.p, sizeof(femgpu_batch_info) * nsys, hipMemcpyDeviceToHost, ctx->stream));

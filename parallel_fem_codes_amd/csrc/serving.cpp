@@ -13,8 +13,17 @@
 //               the GPU start as soon as an executor and memory free up, and the executors' work
 //               overlaps on the device
 //   wait     -> the caller collects the nodal solution and the timings
-// The batch size is autotuned online when max_batch = 0: the executors cycle through candidate
-// sizes, record the throughput (unknowns solved per second) each achieves, and then keep the best.
+// Batches of small requests run through the block-diagonal solver, whose kernels spread the batch
+// over the whole GPU, or (small_solver 1 or 2) as one persistent launch with a workgroup per system
+// (femgpu_gmres_mixed_wg: each request's answer is the same alone and in any batch). A solve alone
+// is faster on one compute unit below about 3000 rows (measured: 2x at 300-1000 rows, equal near
+// 3000, 3-8x slower at 10000-20000), so mode 1 takes the persistent launch only when every system
+// of the batch has at most kWgAutoRows rows. Units whose matrix rows exceed the sliced-ELL factor
+// width, and single large requests, always use the block-diagonal solver.
+// Two launch parameters are autotuned online: the batch size when max_batch = 0, then the persistent
+// launch's threads per workgroup (the waves sharing a system's rows) when wg_threads = 0. The
+// executors cycle through the candidates, record the throughput (unknowns solved per second) each
+// achieves over two units, and keep the best.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -53,7 +62,42 @@ struct Request {
 
 using Req = std::shared_ptr<Request>;
 
-constexpr int kBatchCandidates[] = {4, 8, 16, 32, 64};
+constexpr int kWgAutoRows = 3072;
+
+// Online choice among a few candidates: sample each twice (fewest samples first), then keep the one
+// with the best mean throughput.
+struct Tuner {
+    std::vector<int> cand;
+    std::vector<double> rate;
+    std::vector<int> n;
+    int chosen = -1;
+    explicit Tuner(std::vector<int> c) : cand(std::move(c)), rate(cand.size(), 0.0), n(cand.size(), 0) {}
+    // the value to use; idx = the candidate being sampled, or -1 once settled
+    int pick(int &idx)
+    {
+        idx = -1;
+        if (chosen >= 0) return cand[chosen];
+        size_t best = 0;
+        for (size_t c = 1; c < cand.size(); ++c)
+            if (n[c] < n[best]) best = c;
+        if (n[best] >= 2) {
+            size_t b = 0;
+            for (size_t c = 1; c < cand.size(); ++c)
+                if (rate[c] / n[c] > rate[b] / n[b]) b = c;
+            chosen = static_cast<int>(b);
+            return cand[b];
+        }
+        idx = static_cast<int>(best);
+        return cand[best];
+    }
+    void record(int idx, double r)
+    {
+        if (idx < 0 || chosen >= 0) return;
+        rate[idx] += r;
+        n[idx] += 1;
+    }
+    int settled() const { return chosen >= 0 ? cand[chosen] : 0; }
+};
 
 }  // namespace
 
@@ -71,10 +115,9 @@ struct femgpu_server {
     int preparing = 0;                      // requests taken by a preparation thread
     bool stop = false;
     std::vector<std::thread> threads;
-    // online batch-size tuning: unknowns per second per candidate, samples
-    double tune_rate[5] = {0, 0, 0, 0, 0};
-    int tune_n[5] = {0, 0, 0, 0, 0};
-    int tuned = -1;
+    // online tuning: batch size, then threads per workgroup of the persistent launch
+    Tuner batch_tuner{{4, 8, 16, 32, 64}};
+    Tuner wg_tuner{{256, 512, 1024}};
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -150,24 +193,21 @@ int batch_limit(femgpu_server *S, int &cand)
 {
     cand = -1;
     if (S->o.max_batch > 0) return S->o.max_batch;
-    if (S->tuned >= 0) return kBatchCandidates[S->tuned];
-    // the candidate with the fewest samples (two each), then the best rate
-    int best = 0;
-    for (int c = 1; c < 5; ++c)
-        if (S->tune_n[c] < S->tune_n[best]) best = c;
-    if (S->tune_n[best] >= 2) {
-        int b = 0;
-        for (int c = 1; c < 5; ++c)
-            if (S->tune_rate[c] / S->tune_n[c] > S->tune_rate[b] / S->tune_n[b]) b = c;
-        S->tuned = b;
-        return kBatchCandidates[b];
-    }
-    cand = best;
-    return kBatchCandidates[best];
+    return S->batch_tuner.pick(cand);
+}
+
+// Threads per workgroup for a persistent launch: fixed, the library's size rule (0) while the
+// batch size is still being tuned, then the workgroup tuner.
+int wg_threads(femgpu_server *S, int &cand)
+{
+    cand = -1;
+    if (S->o.wg_threads > 0) return S->o.wg_threads;
+    if (S->o.max_batch == 0 && S->batch_tuner.chosen < 0) return 0;
+    return S->wg_tuner.pick(cand);
 }
 
 // Solve one unit (one large request or a batch of small ones) on the executor's context.
-void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long long batch_id)
+void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long long batch_id, int threads)
 {
     const int K = static_cast<int>(unit.size());
     std::vector<int> row_off(K + 1, 0);
@@ -185,7 +225,7 @@ void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long lo
         for (int p = 0; p < r.nnz; ++p) JA[nnz_off[s] + p] = r.JA[p] + row_off[s];
     }
     IA[N] = NNZ;
-    int rc = FEMGPU_OK;
+    int rc = FEMGPU_OK, used_threads = 0;
     std::string err;
     femgpu_csr *A = nullptr;
     double *dF = nullptr, *dX = nullptr;
@@ -227,9 +267,15 @@ void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long lo
         femgpu_system sys = {A, nullptr, A, nullptr};
         femgpu_mixed_opts mo = S->o.mixed;
         mo.inner_kmax = S->o.kmax;
-        if (rc == FEMGPU_OK &&
-            femgpu_gmres_mixed_batched(ctx, &sys, K, row_off.data(), &mo, S->o.tol, dF, dX, binfo.data()) != FEMGPU_OK)
-            fail(FEMGPU_ERR_ARG);
+        if (rc == FEMGPU_OK) {
+            const bool wg = threads >= 0 && A->sell_ok && A->par_ok;
+            used_threads = wg ? (threads > 0 ? threads : fg::wg_threads_for(N, K)) : 0;
+            const int st = wg ? femgpu_gmres_mixed_wg(ctx, &sys, K, row_off.data(), &mo, S->o.tol, used_threads, dF, dX,
+                                                      binfo.data())
+                              : femgpu_gmres_mixed_batched(ctx, &sys, K, row_off.data(), &mo, S->o.tol, dF, dX,
+                                                           binfo.data());
+            if (st != FEMGPU_OK) fail(FEMGPU_ERR_ARG);
+        }
     }
     std::vector<double> X(static_cast<size_t>(N) + 1, 0.0);
     if (rc == FEMGPU_OK && N > 0 &&
@@ -255,6 +301,7 @@ void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long lo
         r.info.iterations = binfo[s].inner_iterations;
         r.info.refinements = binfo[s].refinements;
         r.info.status = rc == FEMGPU_OK ? binfo[s].status : rc;
+        r.info.wg_threads = used_threads;
         r.info.true_rel_residual = binfo[s].true_rel_residual;
         r.info.t_start = t_start;
         r.info.t_end = t_end;
@@ -284,7 +331,7 @@ void exec_loop(femgpu_server *S)
     for (;;) {
         std::vector<Req> unit;
         long long batch_id = 0;
-        int cand = -1;
+        int cand = -1, wcand = -1, threads = -1;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             for (;;) {
@@ -314,6 +361,12 @@ void exec_loop(femgpu_server *S)
                         }
                         for (auto &r : unit) S->inflight += r->prep_rc == FEMGPU_OK ? r->bytes : 0;
                         batch_id = S->next_batch++;
+                        // small requests (alone or packed) may go to the persistent launch
+                        int maxn = 0;
+                        for (auto &r : unit) maxn = std::max(maxn, r->prep_rc == FEMGPU_OK ? r->n : 0);
+                        const bool persistent = S->o.small_solver == 2 || (S->o.small_solver == 1 && maxn <= kWgAutoRows);
+                        if (persistent && head->prep_rc == FEMGPU_OK && head->n <= S->o.small_rows && S->o.kmax <= 64)
+                            threads = wg_threads(S, wcand);
                         break;
                     }
                 }
@@ -334,17 +387,13 @@ void exec_loop(femgpu_server *S)
             }
         }
         if (!good.empty()) {
-            run_unit(S, ctx, good, batch_id);
-            if (cand >= 0 && good.size() > 1) {
-                double rows = 0;
-                for (auto &r : good) rows += r->n;
-                const double dt = good[0]->info.t_end - good[0]->info.t_start;
-                std::lock_guard<std::mutex> lk(S->mu);
-                if (dt > 0) {
-                    S->tune_rate[cand] += rows / dt;
-                    S->tune_n[cand] += 1;
-                }
-            }
+            run_unit(S, ctx, good, batch_id, threads);
+            double rows = 0;
+            for (auto &r : good) rows += r->n;
+            const double dt = good[0]->info.t_end - good[0]->info.t_start;
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (dt > 0 && good.size() > 1) S->batch_tuner.record(cand, rows / dt);
+            if (dt > 0 && good.size() > 1 && good[0]->info.wg_threads > 0) S->wg_tuner.record(wcand, rows / dt);
         }
         S->cv_done.notify_all();
         S->cv_exec.notify_all();
@@ -359,7 +408,8 @@ extern "C" int femgpu_server_create(int device, const femgpu_server_opts *o, fem
 {
     FG_CHECK(o && out && o->executors >= 1 && o->executors <= 8 && o->prep_threads >= 1 && o->prep_threads <= 64 &&
                  o->small_rows >= 0 && o->batch_rows >= 1 && o->max_batch >= 0 && o->kmax >= 2 && o->kmax <= 150 &&
-                 o->tol > 0 && o->hbm_budget > 0,
+                 o->tol > 0 && o->hbm_budget > 0 && o->small_solver >= 0 && o->small_solver <= 2 &&
+                 (o->wg_threads == 0 || o->wg_threads == 256 || o->wg_threads == 512 || o->wg_threads == 1024),
              FEMGPU_ERR_ARG, "femgpu_server_create: bad options");
     auto *S = new femgpu_server();
     S->device = device;
@@ -439,6 +489,14 @@ extern "C" int femgpu_server_batch_tuning(femgpu_server *S, int *chosen)
 {
     FG_CHECK(S && chosen, FEMGPU_ERR_ARG, "femgpu_server_batch_tuning: null");
     std::lock_guard<std::mutex> lk(S->mu);
-    *chosen = S->o.max_batch > 0 ? S->o.max_batch : (S->tuned >= 0 ? kBatchCandidates[S->tuned] : 0);
+    *chosen = S->o.max_batch > 0 ? S->o.max_batch : S->batch_tuner.settled();
+    return FEMGPU_OK;
+}
+
+extern "C" int femgpu_server_wg_tuning(femgpu_server *S, int *chosen)
+{
+    FG_CHECK(S && chosen, FEMGPU_ERR_ARG, "femgpu_server_wg_tuning: null");
+    std::lock_guard<std::mutex> lk(S->mu);
+    *chosen = S->o.wg_threads > 0 ? S->o.wg_threads : S->wg_tuner.settled();
     return FEMGPU_OK;
 }

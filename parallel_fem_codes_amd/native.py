@@ -73,13 +73,13 @@ class ServerOpts(C.Structure):
     """femgpu_server_opts."""
     _fields_ = [("hbm_budget", C.c_size_t), ("executors", C.c_int), ("prep_threads", C.c_int),
                 ("small_rows", C.c_int), ("batch_rows", C.c_int), ("max_batch", C.c_int), ("kmax", C.c_int),
-                ("pad", C.c_int), ("tol", C.c_double), ("mixed", MixedOpts)]
+                ("small_solver", C.c_int), ("wg_threads", C.c_int), ("tol", C.c_double), ("mixed", MixedOpts)]
 
 
 class RequestInfo(C.Structure):
     """femgpu_request_info."""
     _fields_ = [("n", C.c_int), ("nnz", C.c_int), ("batch_id", C.c_int), ("batch_size", C.c_int),
-                ("iterations", C.c_int), ("refinements", C.c_int), ("status", C.c_int), ("pad", C.c_int),
+                ("iterations", C.c_int), ("refinements", C.c_int), ("status", C.c_int), ("wg_threads", C.c_int),
                 ("true_rel_residual", C.c_double), ("t_submit", C.c_double), ("t_start", C.c_double),
                 ("t_end", C.c_double), ("prep_s", C.c_double), ("setup_s", C.c_double), ("solve_s", C.c_double)]
 
@@ -179,6 +179,7 @@ _SIGS = {
     "femgpu_server_wait": (I, [vp, C.c_longlong, D, vp, C.POINTER(RequestInfo)]),
     "femgpu_server_clock": (D, [vp]),
     "femgpu_server_batch_tuning": (I, [vp, C.POINTER(I)]),
+    "femgpu_server_wg_tuning": (I, [vp, C.POINTER(I)]),
     "femgpu_server_destroy": (I, [vp]),
     "femgpu_st_process": (I, [vp, vp, C.POINTER(TransportProblem), C.POINTER(DriverOpts), vp,
                               C.POINTER(DriverInfo)]),

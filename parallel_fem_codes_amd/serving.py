@@ -70,10 +70,15 @@ def make_requests(count, seed, nmin, nmax, rate, kmin=1e-5, kmax=1e-2):
 
 def server_opts(hbm_budget=64 << 30, executors=4, prep_threads=8, small_rows=20000, batch_rows=400000,
                 max_batch=0, kmax=30, tol=1e-8, sweep_mode=2, factor_sweeps=2, inner_tol=1e-4, max_refinements=50,
-                orthogonalization="MGS"):
+                orthogonalization="MGS", small_solver="shared", wg_threads=0):
+    """small_solver for batches of small requests: "shared" (block-diagonal shared-Krylov solver),
+    "persistent" (one launch, a workgroup per system) or "auto" (persistent when every system has at
+    most 3072 rows); wg_threads 0 = tuned online."""
     o = N.ServerOpts()
     o.hbm_budget, o.executors, o.prep_threads = int(hbm_budget), executors, prep_threads
     o.small_rows, o.batch_rows, o.max_batch, o.kmax, o.tol = small_rows, batch_rows, max_batch, kmax, tol
+    o.small_solver = {"shared": 0, "auto": 1, "persistent": 2}[small_solver]
+    o.wg_threads = wg_threads
     o.mixed = fem.mixed_opts(kmax, inner_tol, max_refinements, sweep_mode, factor_sweeps, orthogonalization)
     return o
 
@@ -113,6 +118,11 @@ class Server:
         N.check(N.lib().femgpu_server_batch_tuning(self.h, C.byref(c)))
         return c.value
 
+    def wg_threads(self):
+        c = C.c_int()
+        N.check(N.lib().femgpu_server_wg_tuning(self.h, C.byref(c)))
+        return c.value
+
     def close(self):
         if getattr(self, "h", None):
             N.lib().femgpu_server_destroy(self.h)
@@ -122,10 +132,14 @@ class Server:
         self.close()
 
 
-def run_server(reqs, opts=None, device=0):
+def run_server(reqs, opts=None, device=0, warmup=0):
     """Feeds the requests at their arrival times (relative to the first), collects every result.
-    Returns (solutions, infos, wall seconds)."""
+    warmup > 0 first solves that many of the requests one by one (results dropped), so that the
+    timed stream does not include the first launches of each kernel. Returns (solutions, infos,
+    wall seconds, batch size chosen, workgroup threads chosen)."""
     srv = Server(device, opts)
+    for r in reqs[:warmup]:
+        srv.wait(srv.submit(r), len(r["x"]))
     tickets = [None] * len(reqs)
     t0 = time.perf_counter()
 
@@ -145,9 +159,23 @@ def run_server(reqs, opts=None, device=0):
         sols.append(u)
         infos.append(info)
     wall = time.perf_counter() - t0
-    batch = srv.batch_size()
+    batch, wg = srv.batch_size(), srv.wg_threads()
     srv.close()
-    return sols, infos, wall, batch
+    return sols, infos, wall, batch, wg
+
+
+def run_isolated(reqs, opts=None, device=0):
+    """Each request alone on an idle server (submitted after the previous one finished): the
+    isolated-solve latencies the loaded server's are compared with."""
+    o = opts if opts is not None else server_opts()
+    o.executors, o.max_batch = 1, 1
+    srv = Server(device, o)
+    lat = []
+    for r in reqs:
+        _, info = srv.wait(srv.submit(r), len(r["x"]))
+        lat.append(info.t_end - info.t_submit)
+    srv.close()
+    return np.array(lat)
 
 
 def solve_fp64(dev, r, kmax=30, tol=1e-8):
@@ -204,6 +232,10 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--executors", type=int, default=4)
     ap.add_argument("--max-batch", type=int, default=0)
+    ap.add_argument("--small-solver", choices=("shared", "auto", "persistent"), default="shared")
+    ap.add_argument("--wg-threads", type=int, default=0)
+    ap.add_argument("--isolated", action="store_true", help="also time every request alone on an idle server")
+    ap.add_argument("--warmup", type=int, default=8, help="requests solved before the timed stream")
     ap.add_argument("--no-fp64", action="store_true")
     a = ap.parse_args()
     import torch
@@ -211,20 +243,26 @@ def main():
     t0 = time.perf_counter()
     reqs = make_requests(a.requests, a.seed, a.nmin, a.nmax, a.rate)
     print(f"# {len(reqs)} requests generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    opts = server_opts(executors=a.executors, max_batch=a.max_batch)
-    sols, infos, wall, batch = run_server(reqs, opts)
+    opts = server_opts(executors=a.executors, max_batch=a.max_batch, small_solver=a.small_solver,
+                       wg_threads=a.wg_threads)
+    sols, infos, wall, batch, wg = run_server(reqs, opts, warmup=a.warmup)
     lat = np.array([i.t_end - i.t_submit for i in infos])
     res = np.array([i.true_rel_residual for i in infos])
     out = {"requests": len(reqs), "rate": a.rate, "nmin": a.nmin, "nmax": a.nmax,
            "server": {"solves_per_s": len(reqs) / wall, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
                       "p99_ms": 1e3 * float(np.percentile(lat, 99)), "max_true_rel_residual": float(res.max()),
                       "converged": int(sum(i.status == 0 for i in infos)), "batch_size": batch,
+                      "wg_threads": wg,
                       "batches": len({i.batch_id for i in infos}),
                       "mean_prep_ms": 1e3 * float(np.mean([i.prep_s for i in infos])),
                       "mean_setup_ms": 1e3 * float(np.mean([i.setup_s for i in infos])),
                       "mean_solve_ms": 1e3 * float(np.mean([i.solve_s for i in infos])),
                       "mean_queue_ms": 1e3 * float(np.mean([i.t_start - i.t_submit - i.prep_s for i in infos]))}}
     print(f"# server: {out['server']}", file=sys.stderr, flush=True)
+    if a.isolated:
+        iso = run_isolated(reqs, server_opts(small_solver=a.small_solver, wg_threads=a.wg_threads))
+        out["isolated"] = {"p50_ms": 1e3 * float(np.percentile(iso, 50)), "p99_ms": 1e3 * float(np.percentile(iso, 99))}
+        out["p99_loaded_over_isolated"] = float(np.percentile(lat, 99) / np.percentile(iso, 99))
     if not a.no_fp64:
         s64, service, lat64, end = run_sequential_fp64(reqs)
         diff = max(float(np.linalg.norm(u - v) / max(np.linalg.norm(v), 1e-300)) for u, v in zip(sols, s64))

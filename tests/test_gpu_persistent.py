@@ -54,7 +54,7 @@ def systems(dev, torch_cuda):
     return [assemble(dev, torch_cuda, r) for r in reqs]
 
 
-def batch_solve(dev, torch, syss, threads=0, solver="wg"):
+def batch_solve(dev, torch, syss, threads=256, solver="wg"):
     IA, JA, row_off, nnz_off = fem.concat_csr([(s["IA"], s["JA"]) for s in syss])
     A = fem.CsrMatrix(dev, IA, JA)
     A.set_values(np.concatenate([s["AA"] for s in syss]))
@@ -91,6 +91,7 @@ def test_wg_converges_and_is_verified_in_fp64(dev, torch_cuda, systems):
 
 
 def test_wg_alone_equals_in_batch_bitwise(dev, torch_cuda, systems):
+    """At one workgroup size (the size sets the order of the workgroup's sums)."""
     mix, _ = batch_solve(dev, torch_cuda, systems)
     for k, s in enumerate(systems):
         alone, info = batch_solve(dev, torch_cuda, [s])
@@ -103,7 +104,7 @@ def test_wg_alone_equals_in_batch_bitwise(dev, torch_cuda, systems):
         assert np.array_equal(shuffled[pos], mix[k]), k
 
 
-@pytest.mark.parametrize("threads", [256, 512, 1024])
+@pytest.mark.parametrize("threads", [0, 256, 512, 1024])
 def test_wg_workgroup_sizes(dev, torch_cuda, systems, threads):
     xs, infos = batch_solve(dev, torch_cuda, systems, threads=threads)
     assert all(i.status == 0 for i in infos)

@@ -27,16 +27,27 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def test_server_results_alone_in_the_mix_and_fp64(reqs):
-    opts = serving.server_opts(executors=2, prep_threads=4, small_rows=5000, batch_rows=100000, max_batch=4)
-    mix, infos, wall, batch = serving.run_server(reqs, opts)
+@pytest.mark.parametrize("small_solver", ["persistent", "shared"])
+def test_server_results_alone_in_the_mix_and_fp64(reqs, small_solver):
+    """persistent: every system has its own Krylov space, so alone and in the mix agree bit for bit
+    (at one workgroup size: the size sets the order of the workgroup's sums); shared: the batch shares
+    one Krylov space, so they agree within what the 1e-8 residual allows."""
+    opts = serving.server_opts(executors=2, prep_threads=4, small_rows=5000, batch_rows=100000, max_batch=4,
+                               small_solver=small_solver, wg_threads=512)
+    mix, infos, wall, batch, wg = serving.run_server(reqs, opts)
     assert batch == 4
     assert max(i.batch_size for i in infos) > 1                    # small requests were packed
+    small = [i.n <= 5000 for i in infos]
+    if small_solver == "persistent":
+        assert all(i.wg_threads == (512 if s else 0) for i, s in zip(infos, small))
+    else:
+        assert all(i.wg_threads == 0 for i in infos)
     assert all(i.status == 0 for i in infos)
     assert max(i.true_rel_residual for i in infos) <= 1e-8
     assert all(i.t_end >= i.t_start >= i.t_submit for i in infos)
-    alone_opts = serving.server_opts(executors=1, prep_threads=2, small_rows=5000, batch_rows=100000, max_batch=1)
-    alone, ainfos, _, _ = serving.run_server(reqs, alone_opts)
+    alone_opts = serving.server_opts(executors=1, prep_threads=2, small_rows=5000, batch_rows=100000, max_batch=1,
+                                     small_solver=small_solver, wg_threads=512)
+    alone, ainfos, _, _, _ = serving.run_server(reqs, alone_opts)
     assert all(i.batch_size == 1 and i.status == 0 for i in ainfos)
     dev = fem.Device(0)
     worst_alone = worst_fp64 = 0.0
@@ -45,6 +56,8 @@ def test_server_results_alone_in_the_mix_and_fp64(reqs):
         assert info.status == 0
         worst_alone = max(worst_alone, rel(mix[k], alone[k]))
         worst_fp64 = max(worst_fp64, rel(mix[k], u64))
+        if small_solver == "persistent":
+            assert np.array_equal(mix[k], alone[k]), k
         assert rel(mix[k], alone[k]) < 1e-6, k
         assert rel(mix[k], u64) < 1e-5, k
         # Dirichlet values are carried through exactly
@@ -55,7 +68,10 @@ def test_server_results_alone_in_the_mix_and_fp64(reqs):
 
 def test_server_autotunes_batch_size(reqs):
     many = serving.make_requests(48, 11, 200, 1500, rate=1e4)
-    opts = serving.server_opts(executors=1, prep_threads=4, small_rows=5000, batch_rows=200000, max_batch=0)
-    sols, infos, wall, batch = serving.run_server(many, opts)
+    opts = serving.server_opts(executors=1, prep_threads=4, small_rows=5000, batch_rows=200000, max_batch=0,
+                               small_solver="auto")
+    sols, infos, wall, batch, wg = serving.run_server(many, opts)
     assert all(i.status == 0 for i in infos)
     assert batch in (0, 4, 8, 16, 32, 64)
+    assert wg in (0, 256, 512, 1024)
+    assert all(i.wg_threads in (256, 512, 1024) for i in infos)
