@@ -216,6 +216,14 @@ def main():
         print(json.dumps({"profile_kernels": "done", "copy_bytes": src.numel() * 4, "n": neq, "nnz_upper": nnz_u}))
         return
 
+    # rows per wavefront of the FP64 CSR kernels (csrmv, pgmres operator, FP64 residual), tuned on the
+    # assembled matrix before any timed region
+    M.Build_M_K_R_SUPG(pb, alpha, dt, u, du, R, AA_ptr=aa)
+    A.values_changed()
+    lanes, lane_us = A.autotune()
+    print(f"# CSR lanes per row: {lanes} (" + ", ".join(f"{k}: {v:.1f} us" for k, v in lane_us.items()) + ")",
+          file=sys.stderr, flush=True)
+
     results = {}
     for mode in ("fp64", "mixed_exact_ilu", "mixed"):
         for _ in range(a.warmup):
@@ -282,7 +290,7 @@ def main():
     bytes_ilu = 12 * (nnz - n) + 8 * (n + 1) + 8 * n + 16 * n
     bytes_jac = a.jacobi * (8 * (nnz - n) + 44 * n)
     kernels = {
-        "csrmv_f64": dict(us=t_spmv * 1e6, GBs=bytes_spmv / t_spmv / 1e9),
+        "csrmv_f64": dict(us=t_spmv * 1e6, GBs=bytes_spmv / t_spmv / 1e9, lanes_per_row=lanes),
         "csrmv_f32": dict(us=t_spmv32 * 1e6, GBs=bytes_spmv32 / t_spmv32 / 1e9),
         "ilu0_apply_f64": dict(us=t_ilu * 1e6, GBs=bytes_ilu / t_ilu / 1e9),
         "ilu0_apply_f32": dict(us=t_ilu32 * 1e6, GBs=(bytes_ilu / 2) / t_ilu32 / 1e9),
@@ -361,6 +369,7 @@ def main():
                  "sweep": A.sweep_info()},
         "speedup_mixed_vs_fp64": round(mx["solves_per_s"] / fp["solves_per_s"], 3),
         "kernels": {k: {kk: round(vv, 2) for kk, vv in v.items()} for k, v in kernels.items()},
+        "csr_lanes_autotune_us": {str(k): round(v, 2) for k, v in lane_us.items()},
         "roofline": roofline,
         "time_integration": time_int,
         "serving": serving_out,

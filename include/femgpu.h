@@ -148,6 +148,19 @@ int femgpu_csr_set_factor_kernel(femgpu_csr *A, int kernel);
 int femgpu_csrmv(femgpu_csr *A, const double *d_P, double *d_Q);
 int femgpu_csrmv_f32(femgpu_csr *A, const float *d_P, float *d_Q);
 
+/* Rows per wavefront of the FP64 CSR kernels (csrmv, pgmres's operator, the mixed solver's FP64
+ * residual): L lanes per row, 64 / L rows per wavefront. femgpu_csr_autotune times L = 1, 2, 4, 8,
+ * 16 on A's structure and current values (20 launches each, HIP events) and keeps the fastest;
+ * us[] = microseconds per launch for each L. Synchronises the context's stream. Default L = 8.
+ * No reference counterpart (the reference's csrmv is one loop per row). */
+typedef struct {
+    int lanes;             /* the L chosen */
+    int pad;
+    double us[5];          /* per-launch time of L = 1, 2, 4, 8, 16 */
+} femgpu_tune_info;
+int femgpu_csr_autotune(femgpu_csr *A, femgpu_tune_info *info);
+int femgpu_csr_set_lanes(femgpu_csr *A, int lanes);
+
 /* ILU(0) numeric factorisation of the current values (ILUp_precond_setup.c:67-168 with p = 0),
  * level-scheduled on the device; also builds the FP32 factors. */
 int femgpu_ilu0_factor(femgpu_csr *A);

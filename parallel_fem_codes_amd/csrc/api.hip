@@ -288,6 +288,20 @@ extern "C" int femgpu_csrmv(femgpu_csr *A, const double *d_P, double *d_Q)
     return launch_spmv64(A->ctx, A, d_P, nullptr, nullptr, d_Q, nullptr);
 }
 
+extern "C" int femgpu_csr_autotune(femgpu_csr *A, femgpu_tune_info *info)
+{
+    FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_csr_autotune: null");
+    return csr_autotune(A, info);
+}
+
+extern "C" int femgpu_csr_set_lanes(femgpu_csr *A, int lanes)
+{
+    FG_CHECK(A && (lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16), FEMGPU_ERR_ARG,
+             "femgpu_csr_set_lanes: lanes per row must be 1, 2, 4, 8 or 16");
+    A->spmv_lanes = lanes;
+    return FEMGPU_OK;
+}
+
 extern "C" int femgpu_csrmv_f32(femgpu_csr *A, const float *d_P, float *d_Q)
 {
     FG_CHECK(A && d_P && d_Q && d_P != d_Q, FEMGPU_ERR_ARG, "femgpu_csrmv_f32: bad arguments");

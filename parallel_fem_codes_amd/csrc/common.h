@@ -291,5 +291,6 @@ void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vect
              std::vector<int> &dst);
 int check_device_error(femgpu_ctx *ctx);
 int wg_threads_for(long long rows, int nsys);   // persistent batched solver: default workgroup size
+int csr_autotune(femgpu_csr *A, femgpu_tune_info *info);
 
 }  // namespace fg

@@ -145,6 +145,7 @@ static int spmv_dispatch(femgpu_ctx *ctx, const femgpu_csr *A, const T *AA, cons
     const int grid = static_cast<int>((threads + 255) / 256);
     if (A->n == 0) return FEMGPU_OK;
     switch (L) {
+    case 1: k_spmv<T, 1><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, AA, x, xs, xout, y, skip); break;
     case 2: k_spmv<T, 2><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, AA, x, xs, xout, y, skip); break;
     case 4: k_spmv<T, 4><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, AA, x, xs, xout, y, skip); break;
     case 16: k_spmv<T, 16><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, AA, x, xs, xout, y, skip); break;
@@ -175,10 +176,71 @@ int launch_spmv32(femgpu_ctx *ctx, const femgpu_csr *A, const float *x, const do
 int launch_residual64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *f, double *r)
 {
     if (A->n == 0) return FEMGPU_OK;
-    const int grid = static_cast<int>((static_cast<long>(A->n) * 8 + 255) / 256);
-    k_residual<8><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, A->AA.p, x, f, r);
+    const int L = A->spmv_lanes;
+    const int grid = static_cast<int>((static_cast<long>(A->n) * L + 255) / 256);
+    switch (L) {
+    case 1: k_residual<1><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, A->AA.p, x, f, r); break;
+    case 2: k_residual<2><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, A->AA.p, x, f, r); break;
+    case 4: k_residual<4><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, A->AA.p, x, f, r); break;
+    case 16: k_residual<16><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, A->AA.p, x, f, r); break;
+    default: k_residual<8><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, A->AA.p, x, f, r); break;
+    }
     FG_HIP(hipGetLastError());
     return FEMGPU_OK;
+}
+
+__global__ void k_fill(int n, double v, double *__restrict__ x)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) x[k] = v;
+}
+
+// Rows-per-wavefront autotuning of the FP64 CSR kernels (SpMV and residual share the choice): each
+// candidate lanes-per-row L (64 / L rows per wavefront) is timed with HIP events over 20 launches on
+// the matrix's own structure and values, after two warm-up launches; the fastest is kept.
+int csr_autotune(femgpu_csr *A, femgpu_tune_info *info)
+{
+    femgpu_ctx *ctx = A->ctx;
+    static constexpr int kLanes[5] = {1, 2, 4, 8, 16};
+    femgpu_tune_info out{};
+    out.lanes = A->spmv_lanes;
+    if (A->n == 0) {
+        if (info) *info = out;
+        return FEMGPU_OK;
+    }
+    DevBuf<double> x, y;
+    FG_TRY(x.alloc(A->n));
+    FG_TRY(y.alloc(A->n));
+    k_fill<<<std::min(4096, (A->n + 255) / 256), 256, 0, ctx->stream>>>(A->n, 1.0, x.p);
+    FG_HIP(hipGetLastError());
+    hipEvent_t e0, e1;
+    FG_HIP(hipEventCreate(&e0));
+    FG_HIP(hipEventCreate(&e1));
+    const int saved = A->spmv_lanes;
+    int best = saved, rc = FEMGPU_OK;
+    float best_ms = 0.0f;
+    for (int c = 0; c < 5 && rc == FEMGPU_OK; ++c) {
+        A->spmv_lanes = kLanes[c];
+        for (int w = 0; w < 2 && rc == FEMGPU_OK; ++w) rc = launch_spmv64(ctx, A, x.p, nullptr, nullptr, y.p, nullptr);
+        if (rc == FEMGPU_OK && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = FEMGPU_ERR_HIP;
+        for (int t = 0; t < 20 && rc == FEMGPU_OK; ++t) rc = launch_spmv64(ctx, A, x.p, nullptr, nullptr, y.p, nullptr);
+        float ms = 0.0f;
+        if (rc == FEMGPU_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = FEMGPU_ERR_HIP;
+        if (rc != FEMGPU_OK) break;
+        out.us[c] = 1e3 * ms / 20.0;
+        if (c == 0 || ms < best_ms) {
+            best_ms = ms;
+            best = kLanes[c];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    A->spmv_lanes = rc == FEMGPU_OK ? best : saved;
+    out.lanes = A->spmv_lanes;
+    if (info) *info = out;
+    if (rc != FEMGPU_OK) set_error("femgpu_csr_autotune: timing launches failed");
+    return rc;
 }
 
 // ------------------------------------------------------------------------------- conversions

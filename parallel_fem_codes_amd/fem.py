@@ -257,6 +257,16 @@ class CsrMatrix:
     def values_changed(self):
         N.check(N.lib().femgpu_csr_values_changed(self.h))
 
+    def autotune(self):
+        """Times the FP64 CSR kernels at 1, 2, 4, 8 and 16 lanes per row and keeps the fastest;
+        returns (lanes chosen, {lanes: microseconds per launch})."""
+        t = N.TuneInfo()
+        N.check(N.lib().femgpu_csr_autotune(self.h, C.byref(t)))
+        return t.lanes, {L: t.us[k] for k, L in enumerate((1, 2, 4, 8, 16))}
+
+    def set_lanes(self, lanes):
+        N.check(N.lib().femgpu_csr_set_lanes(self.h, int(lanes)))
+
     def sweep_info(self):
         si = N.SweepInfo()
         N.check(N.lib().femgpu_csr_sweep_info(self.h, C.byref(si)))

@@ -63,6 +63,11 @@ class MixedOpts(C.Structure):
                 ("orthogonalization", C.c_int)]
 
 
+class TuneInfo(C.Structure):
+    """femgpu_tune_info."""
+    _fields_ = [("lanes", C.c_int), ("pad", C.c_int), ("us", C.c_double * 5)]
+
+
 class BatchInfo(C.Structure):
     """femgpu_batch_info."""
     _fields_ = [("refinements", C.c_int), ("inner_iterations", C.c_int), ("status", C.c_int), ("pad", C.c_int),
@@ -137,6 +142,8 @@ _SIGS = {
     "femgpu_csr_set_sweep_kernel": (I, [vp, I]),
     "femgpu_csr_set_factor_kernel": (I, [vp, I]),
     "femgpu_csrmv": (I, [vp, vp, vp]),
+    "femgpu_csr_autotune": (I, [vp, C.POINTER(TuneInfo)]),
+    "femgpu_csr_set_lanes": (I, [vp, I]),
     "femgpu_csrmv_f32": (I, [vp, vp, vp]),
     "femgpu_ilu0_factor": (I, [vp]),
     "femgpu_ilu0_factor_host": (I, [vp]),

@@ -96,6 +96,36 @@ def test_csrmv_fp64_fp32(dev, torch_cuda, oracle, name):
     assert rel(H(q32), ref) < 1e-5
 
 
+def test_csr_rows_per_wavefront_autotune(dev, torch_cuda, oracle):
+    """Every lanes-per-row setting computes csrmv and the mixed solver's FP64 residual to the oracle's
+    1e-13; the autotuner times all five and keeps the fastest."""
+    torch = torch_cuda
+    s = steady_system(oracle, m=200)
+    A = fem.CsrMatrix(dev, s["IA"], s["JA"])
+    A.set_values(s["AA"][:-1])
+    p = np.random.default_rng(3).standard_normal(s["neq"])
+    ref = oracle.csrmv(s["IA"], s["JA"], s["AA"], p)
+    q = torch.empty(s["neq"], dtype=torch.float64, device="cuda")
+    for L in (1, 2, 4, 8, 16):
+        A.set_lanes(L)
+        A.csrmv(T(torch, p), q)
+        assert rel(H(q), ref) < 1e-13, L
+    lanes, us = A.autotune()
+    print(f"lanes chosen {lanes}: " + ", ".join(f"L={k} {v:.2f} us" for k, v in us.items()))
+    assert lanes in (1, 2, 4, 8, 16) and all(v > 0 for v in us.values())
+    assert us[lanes] == min(us.values())
+    A.csrmv(T(torch, p), q)
+    assert rel(H(q), ref) < 1e-13
+    with pytest.raises(Exception):
+        A.set_lanes(3)
+    # the mixed solver's FP64 residual uses the tuned kernel too
+    X = torch.empty(s["neq"], dtype=torch.float64, device="cuda")
+    info = fem.gmres_mixed(dev, A, T(torch, s["F"]), X, tol=1e-8, sweep_mode=2, factor_sweeps=3)
+    x = H(X)
+    res = np.linalg.norm(s["F"] - oracle.csrmv(s["IA"], s["JA"], s["AA"], x)) / np.linalg.norm(s["F"])
+    assert abs(res - info.true_rel_residual) <= 1e-6 * res + 1e-15
+
+
 @pytest.mark.parametrize("m,right", [(12, True), (41, False), (200, True)])
 def test_ebemv_matches_oracle_and_csr(dev, torch_cuda, oracle, m, right):
     torch = torch_cuda
