@@ -445,6 +445,13 @@ int femgpu_mesh_file_read(const char *path, int format, int ndof, femgpu_mesh_fi
 int femgpu_mesh_file_write(const char *path, int nnodes, int nel, int ndof, const double *x, const double *y,
                            const int *type, const int *ien);
 
+/* Delaunay triangulation of n points (host arrays), for unstructured meshes (the reference makes
+ * them with gmsh, MG/Unstructured_Mesh/gmsh2Mesh.c, which is not in its tree): Bowyer-Watson with
+ * ghost triangles over the hull, Hilbert-ordered insertion. tri: 3 * max_tri ints, counter-
+ * clockwise triangles (at most 2n - 5); points on hull edges are vertices; repeated points are
+ * skipped and counted in *duplicates (may be null). Fails if all points are collinear. */
+int femgpu_delaunay(int n, const double *x, const double *y, int max_tri, int *ntri, int *tri, int *duplicates);
+
 /* =========================================================== serving loop */
 
 typedef struct femgpu_server femgpu_server;

@@ -57,6 +57,19 @@ def structured_marks(problem, x, y):
     return ndof, mark, t
 
 
+def delaunay(x, y):
+    """Delaunay triangulation of the points (x, y) by the library (Bowyer-Watson): (ntri, 3) int32
+    counter-clockwise triangles and the number of repeated points skipped."""
+    x, y = _f64(x), _f64(y)
+    n = len(x)
+    cap = max(2 * n, 8)
+    tri = np.empty(3 * cap, np.int32)
+    nt, dup = C.c_int(), C.c_int()
+    N.check(N.lib().femgpu_delaunay(n, x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p), cap,
+                                    C.byref(nt), tri.ctypes.data_as(C.c_void_p), C.byref(dup)))
+    return tri[:3 * nt.value].reshape(-1, 3).copy(), dup.value
+
+
 def read_mesh(path, ndof=1, format=0):
     """A mesh file: format 0 = the generator's <PROBLEM>_<nn>_<ne>.dat, 1 = one rank's RANK_* file.
     Returns dict(x, y, type, invP, ien, etype)."""

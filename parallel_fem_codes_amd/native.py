@@ -179,6 +179,7 @@ _SIGS = {
     "femgpu_structured_mesh": (I, [I, I, D, D, D, D, I, f64p, f64p, i32p]),
     "femgpu_structured_marks": (I, [C.c_char_p, I, f64p, f64p, i32p, vp]),
     "femgpu_mesh_file_read": (I, [C.c_char_p, I, I, C.POINTER(MeshFileInfo), vp, vp, vp, vp, vp, vp]),
+    "femgpu_delaunay": (I, [I, vp, vp, I, C.POINTER(I), vp, C.POINTER(I)]),
     "femgpu_mesh_file_write": (I, [C.c_char_p, I, I, I, f64p, f64p, i32p, i32p]),
     "femgpu_server_create": (I, [I, C.POINTER(ServerOpts), vp]),
     "femgpu_server_submit": (I, [vp, I, I, f64p, f64p, i32p, f64p, i32p, C.POINTER(TransportProblem),

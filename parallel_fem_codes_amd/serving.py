@@ -28,9 +28,9 @@ from . import native as N
 # ----------------------------------------------------------------------------- workload
 
 def square_mesh(npts, rng):
-    """Delaunay triangulation of npts uniform random points in the unit square plus boundary
-    nodes (about sqrt(npts) per side, uniformly spaced); triangles counter-clockwise."""
-    from scipy.spatial import Delaunay
+    """Delaunay triangulation (the library's femgpu_delaunay) of npts uniform random points in the
+    unit square plus boundary nodes (about sqrt(npts) per side, uniformly spaced); triangles
+    counter-clockwise."""
     m = max(4, int(np.sqrt(npts)))
     t = np.linspace(0.0, 1.0, m + 1)[:-1]
     bx = np.concatenate([t, np.ones(m), 1.0 - t, np.zeros(m)])
@@ -39,10 +39,7 @@ def square_mesh(npts, rng):
     py = rng.uniform(0.0, 1.0, npts) * (1 - 2e-6) + 1e-6
     x = np.concatenate([bx, px])
     y = np.concatenate([by, py])
-    tri = Delaunay(np.stack([x, y], 1)).simplices.astype(np.int64)
-    a = (x[tri[:, 1]] - x[tri[:, 0]]) * (y[tri[:, 2]] - y[tri[:, 0]]) - \
-        (x[tri[:, 2]] - x[tri[:, 0]]) * (y[tri[:, 1]] - y[tri[:, 0]])
-    tri[a < 0] = tri[a < 0][:, [0, 2, 1]]
+    tri, _ = fem.delaunay(x, y)
     nb = len(bx)
     typ = np.ones(len(x), np.int32)
     typ[:nb] = 0

@@ -1,0 +1,101 @@
+"""The library's Delaunay triangulation (host code, no GPU) for the serving workload's unstructured
+meshes. Pinned by mathematics and by an independent implementation: the same triangles as
+scipy.spatial.Delaunay (Qhull) for points in general position, the empty-circumcircle property by
+brute force, the triangle count 2n - h - 2 and the hull area covered exactly once, also for a
+degenerate grid where the triangulation is not unique."""
+import numpy as np
+import pytest
+from scipy.spatial import ConvexHull, Delaunay
+
+from parallel_fem_codes_amd import fem
+
+
+def canon(t):
+    return {tuple(sorted(r)) for r in np.asarray(t).tolist()}
+
+
+def areas(x, y, t):
+    a, b, c = t[:, 0], t[:, 1], t[:, 2]
+    return 0.5 * ((x[b] - x[a]) * (y[c] - y[a]) - (x[c] - x[a]) * (y[b] - y[a]))
+
+
+def square_points(npts, rng):
+    m = max(4, int(np.sqrt(npts)))
+    t = np.linspace(0.0, 1.0, m + 1)[:-1]
+    bx = np.concatenate([t, np.ones(m), 1.0 - t, np.zeros(m)])
+    by = np.concatenate([np.zeros(m), t, np.ones(m), 1.0 - t])
+    px = rng.uniform(0.0, 1.0, npts) * (1 - 2e-6) + 1e-6
+    py = rng.uniform(0.0, 1.0, npts) * (1 - 2e-6) + 1e-6
+    return np.concatenate([bx, px]), np.concatenate([by, py]), 4 * m
+
+
+@pytest.mark.parametrize("n,seed", [(3, 0), (10, 1), (200, 2), (3000, 3), (30000, 4)])
+def test_random_points_match_qhull(n, seed):
+    rng = np.random.default_rng(seed)
+    x, y = rng.random(n), rng.random(n)
+    tri, dup = fem.delaunay(x, y)
+    assert dup == 0
+    assert canon(tri) == canon(Delaunay(np.stack([x, y], 1)).simplices)
+    assert np.all(areas(x, y, tri) > 0)
+
+
+@pytest.mark.parametrize("npts,seed", [(300, 5), (5000, 6)])
+def test_square_with_boundary_nodes(npts, seed):
+    """The serving workload's layout: collinear boundary nodes stay vertices, T = 2n - h - 2, the
+    unit square is covered exactly once, and the triangles are Qhull's."""
+    x, y, h = square_points(npts, np.random.default_rng(seed))
+    tri, dup = fem.delaunay(x, y)
+    assert dup == 0 and len(tri) == 2 * len(x) - h - 2
+    ar = areas(x, y, tri)
+    assert np.all(ar > 0) and abs(ar.sum() - 1.0) < 1e-12
+    assert len(np.unique(tri)) == len(x)
+    assert canon(tri) == canon(Delaunay(np.stack([x, y], 1)).simplices)
+
+
+def empty_circles(x, y, tri, tol=1e-10):
+    """No other point strictly inside any triangle's circumcircle (brute force): the incircle
+    determinant of (a, b, c, p), with a tolerance relative to the size of its terms."""
+    for a, b, c in tri:
+        adx, ady = x[a] - x, y[a] - y
+        bdx, bdy = x[b] - x, y[b] - y
+        cdx, cdy = x[c] - x, y[c] - y
+        al, bl, cl = adx * adx + ady * ady, bdx * bdx + bdy * bdy, cdx * cdx + cdy * cdy
+        det = adx * (bdy * cl - cdy * bl) - ady * (bdx * cl - cdx * bl) + al * (bdx * cdy - bdy * cdx)
+        mag = (np.abs(adx) * (np.abs(bdy * cl) + np.abs(cdy * bl)) + np.abs(ady) * (np.abs(bdx * cl) + np.abs(cdx * bl))
+               + al * (np.abs(bdx * cdy) + np.abs(bdy * cdx)))
+        inside = det > tol * mag
+        inside[[a, b, c]] = False
+        if inside.any():
+            return False
+    return True
+
+
+def test_empty_circumcircles_random_and_degenerate_grid():
+    rng = np.random.default_rng(7)
+    x, y = rng.random(300), rng.random(300)
+    tri, _ = fem.delaunay(x, y)
+    assert empty_circles(x, y, tri)
+    # a 7 x 7 grid: every unit square has four cocircular corners (no unique triangulation)
+    g = np.arange(7.0)
+    X, Y = np.meshgrid(g, g)
+    x, y = X.ravel(), Y.ravel()
+    tri, dup = fem.delaunay(x, y)
+    h = len(ConvexHull(np.stack([x, y], 1)).vertices) + 4 * 5   # corners + collinear boundary points
+    assert dup == 0 and len(tri) == 2 * 49 - h - 2 == 72
+    ar = areas(x, y, tri)
+    assert np.all(ar > 0) and abs(ar.sum() - 36.0) < 1e-12
+    assert empty_circles(x, y, tri)
+
+
+def test_duplicates_skipped_and_bad_input_rejected():
+    rng = np.random.default_rng(8)
+    x, y = rng.random(50), rng.random(50)
+    tri0, _ = fem.delaunay(x, y)
+    tri, dup = fem.delaunay(np.r_[x, x[:7]], np.r_[y, y[:7]])
+    assert dup == 7 and canon(tri) == canon(tri0)
+    with pytest.raises(Exception):
+        fem.delaunay(np.arange(5.0), 2 * np.arange(5.0))   # collinear
+    with pytest.raises(Exception):
+        fem.delaunay(np.zeros(2), np.zeros(2))
+    with pytest.raises(Exception):
+        fem.delaunay(np.array([0.0, 1.0, np.nan]), np.array([0.0, 0.0, 1.0]))
