@@ -258,6 +258,11 @@ typedef struct {
     int orthogonalization; /* inner Arnoldi: 0 modified Gram-Schmidt (one pass per basis vector),
                               1 classical Gram-Schmidt (all coefficients from one pass, one
                               update pass: fewer passes over the basis) */
+    int graphs;            /* 1: every Arnoldi step of the inner solver (SpMV, preconditioner,
+                              orthogonalization) is captured once into a hipGraph and replayed;
+                              the graphs are kept by the context and re-captured when the matrix
+                              or the workspace changes. Same kernels, same results. */
+    int pad;
 } femgpu_mixed_opts;
 
 /* Mixed precision: FP64 iterative refinement (residual F - A x, update x += d, Givens and least

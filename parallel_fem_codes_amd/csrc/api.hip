@@ -1,5 +1,6 @@
 // C ABI of libfemgpu.so: contexts, device-resident CSR/EBE operators, ILU(0), BLAS and PCG.
 // Solvers are in krylov.hip, assembly in assembly.hip, integer structure work in structure.cpp.
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -140,6 +141,8 @@ extern "C" int femgpu_create(int device, void *stream, femgpu_ctx **out)
 extern "C" int femgpu_destroy(femgpu_ctx *ctx)
 {
     if (!ctx) return FEMGPU_OK;
+    release_step_graphs(ctx);
+    if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->h_status) (void)hipHostFree(const_cast<int *>(ctx->h_status));
     delete ctx;
@@ -177,7 +180,9 @@ extern "C" int femgpu_csr_create_ex(femgpu_ctx *ctx, int n, int nnz, const int *
         }
         FG_CHECK(diag[i] >= 0, FEMGPU_ERR_ARG, "femgpu_csr_create: row %d has no diagonal entry", i);
     }
+    static std::atomic<unsigned long long> next_serial{1};
     auto *A = new femgpu_csr();
+    A->serial = next_serial++;
     A->ctx = ctx;
     A->n = n;
     A->nnz = nnz;

@@ -164,10 +164,16 @@ struct femgpu_ctx {
     fg::DevBuf<double> drv_ws;         // nonlinear/time drivers: F, X, uold
     volatile int *h_status = nullptr;  // pinned, device-visible solver status words
     int *d_status = nullptr;
+    // captured Arnoldi steps of the FP32 inner solver (one executable graph per step), the
+    // configuration they were captured for, and the private stream they are captured on
+    std::vector<hipGraphExec_t> step_graphs;
+    std::vector<unsigned long long> step_key;
+    hipStream_t cap_stream = nullptr;
 };
 
 struct femgpu_csr {
     femgpu_ctx *ctx = nullptr;
+    unsigned long long serial = 0;  // unique per matrix created in the process
     int n = 0, nnz = 0;
     std::vector<int> hIA, hJA;
     fg::DevBuf<int> IA, JA, diag;   // diag[i] = position of the diagonal entry of row i
@@ -292,5 +298,6 @@ void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vect
 int check_device_error(femgpu_ctx *ctx);
 int wg_threads_for(long long rows, int nsys);   // persistent batched solver: default workgroup size
 int csr_autotune(femgpu_csr *A, femgpu_tune_info *info);
+void release_step_graphs(femgpu_ctx *ctx);
 
 }  // namespace fg

@@ -10,6 +10,7 @@
 // Every launch of a step returns immediately once the cycle-done flag is set, so the host can
 // enqueue one step ahead of the convergence check it reads back.
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -29,6 +30,7 @@ struct GDev {
     double *h_rho;
     int prec_mode;    // FP32 preconditioner (host side only): 0 exact sweeps, k > 0 Jacobi iterations
     int cgs;          // FP32 inner Arnoldi with classical Gram-Schmidt (host side only)
+    int graphs;       // FP32 inner Arnoldi steps replayed from captured hipGraphs (host side only)
 };
 
 __device__ __forceinline__ double &Hat(const GDev &G, int r, int c) { return G.H[r * G.ld + c]; }
@@ -536,6 +538,82 @@ int run_cycle<double>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, d
     return rc;
 }
 
+// The launches of Arnoldi step i of the FP32 inner solver: w_next = M^-1 A (v_i), v_i = w_cur / h
+// stored on the way, then the orthogonalization and the Givens update of column i.
+static int step_kernels32(femgpu_ctx *ctx, const femgpu_system *S, const GDev &G, int n, int i, float *wc,
+                          float *wn, float *V, size_t ldv)
+{
+    FG_TRY(launch_spmv32(ctx, S->csr, wc, G.invh + i, V + i * ldv, wn, G.flags));
+    FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wn, wn, G.flags));
+    if (G.cgs) {
+        const int nb = std::max(1, std::min(kCgsBlocks, (n / 4 + 255) / 256));
+        k_cgs_dots<<<nb, 256, 0, ctx->stream>>>(G, n, i, wn, V, ldv, ctx->cgs_partials.p);
+        k_cgs_reduce<<<i + 1, 256, 0, ctx->stream>>>(G, nb, i, ctx->cgs_partials.p);
+        k_cgs_update<<<nb, 256, 0, ctx->stream>>>(G, n, i, wn, V, ldv, ctx->cgs_partials.p);
+        k_cgs_finish<<<1, 256, 0, ctx->stream>>>(G, nb, i, ctx->cgs_partials.p);
+    } else {
+        mgs_step<float>(ctx, G, n, i, wn, V, ldv);
+    }
+    return hipGetLastError() == hipSuccess ? FEMGPU_OK : FEMGPU_ERR_HIP;
+}
+
+}  // namespace fg
+
+void fg::release_step_graphs(femgpu_ctx *ctx)
+{
+    for (hipGraphExec_t &x : ctx->step_graphs)
+        if (x) (void)hipGraphExecDestroy(x);
+    ctx->step_graphs.clear();
+    ctx->step_key.clear();
+}
+
+namespace fg {
+
+// Arnoldi step i from a captured hipGraph. A step's launches depend only on (n, basis and work
+// vectors, the matrix and its preconditioner, the small GMRES arrays, the orthogonalization and
+// preconditioner modes, the basis size): each step is captured once on the context's private
+// capture stream, instantiated, and replayed on the context's stream for as long as that
+// configuration holds (the device-side done flag makes the kernels of steps past convergence exit at
+// once, as in the eagerly launched path). The steps contain kernel launches only.
+static int graph_step32(femgpu_ctx *ctx, const femgpu_system *S, const GDev &G, int n, int i, float *wA, float *wB,
+                        float *V, size_t ldv)
+{
+    const auto u = [](const void *p) { return static_cast<unsigned long long>(reinterpret_cast<uintptr_t>(p)); };
+    const std::vector<unsigned long long> key = {
+        static_cast<unsigned long long>(n), u(V), ldv, u(wA), u(wB), S->csr->serial, S->ilu->serial, u(G.H),
+        u(G.flags), u(G.partials), u(G.h_status), static_cast<unsigned long long>(G.cgs), u(ctx->cgs_partials.p),
+        static_cast<unsigned long long>(G.prec_mode), static_cast<unsigned long long>(G.kmax),
+        static_cast<unsigned long long>(S->ilu->sweep_kernel)};
+    if (ctx->step_key != key) {
+        release_step_graphs(ctx);
+        ctx->step_key = key;
+        ctx->step_graphs.assign(static_cast<size_t>(G.kmax), nullptr);
+    }
+    float *wc = (i & 1) ? wB : wA, *wn = (i & 1) ? wA : wB;
+    if (!ctx->step_graphs[i]) {
+        if (!ctx->cap_stream) FG_HIP(hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking));
+        const hipStream_t saved = ctx->stream;
+        FG_HIP(hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
+        ctx->stream = ctx->cap_stream;
+        const int rc = step_kernels32(ctx, S, G, n, i, wc, wn, V, ldv);
+        ctx->stream = saved;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->cap_stream, &g);
+        if (rc != FEMGPU_OK || e != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            set_error("mixed GMRES: capture of Arnoldi step %d failed", i);
+            return rc != FEMGPU_OK ? rc : FEMGPU_ERR_HIP;
+        }
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        FG_CHECK(ei == hipSuccess && x, FEMGPU_ERR_HIP, "mixed GMRES: instantiation of Arnoldi step %d failed", i);
+        ctx->step_graphs[i] = x;
+    }
+    FG_HIP(hipGraphLaunch(ctx->step_graphs[i], ctx->stream));
+    return FEMGPU_OK;
+}
+
 template <>
 int run_cycle<float>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, float *V, size_t ldv, float *wA,
                      float *wB, int kmax, int *steps_out)
@@ -547,19 +625,8 @@ int run_cycle<float>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, fl
     int rc = FEMGPU_OK, steps = 0;
     for (int i = 0; i < kmax - 1; ++i) {
         ctx->h_status[i] = 0;
-        rc = launch_spmv32(ctx, S->csr, wc, G.invh + i, V + i * ldv, wn, G.flags);
-        if (!rc) rc = prec32(ctx, S->ilu, G.prec_mode, wn, wn, G.flags);
+        rc = G.graphs ? graph_step32(ctx, S, G, n, i, wA, wB, V, ldv) : step_kernels32(ctx, S, G, n, i, wc, wn, V, ldv);
         if (rc) break;
-        if (G.cgs) {
-            const int nb = std::max(1, std::min(kCgsBlocks, (n / 4 + 255) / 256));
-            k_cgs_dots<<<nb, 256, 0, ctx->stream>>>(G, n, i, wn, V, ldv, ctx->cgs_partials.p);
-            k_cgs_reduce<<<i + 1, 256, 0, ctx->stream>>>(G, nb, i, ctx->cgs_partials.p);
-            k_cgs_update<<<nb, 256, 0, ctx->stream>>>(G, n, i, wn, V, ldv, ctx->cgs_partials.p);
-            k_cgs_finish<<<1, 256, 0, ctx->stream>>>(G, nb, i, ctx->cgs_partials.p);
-        } else {
-            mgs_step<float>(ctx, G, n, i, wn, V, ldv);
-        }
-        if (hipGetLastError() != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
         if (hipEventRecord(ev[i & 1], ctx->stream) != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
         std::swap(wc, wn);
         steps = i + 1;
@@ -624,7 +691,7 @@ extern "C" int femgpu_pgmres(femgpu_ctx *ctx, const femgpu_system *S, int kmax, 
     const int n = sys_n(S);
     FG_CHECK(!S->ilu || S->ilu->n == n, FEMGPU_ERR_ARG, "femgpu_pgmres: preconditioner size mismatch");
     if (!S->ilu && S->bdiag) FG_TRY(blockdiag_check(S->bdiag, n));
-    GDev G;
+    GDev G{};
     FG_TRY(make_gdev(ctx, kmax, G));
     size_t ldv;
     FG_TRY(ensure_ws<double>(ctx, n, kmax, ctx->gm_basis, ctx->gm_w, ldv));
@@ -723,10 +790,11 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
     FG_CHECK(opt->factor_sweeps > 0 || S->ilu->factored, FEMGPU_ERR_STATE, "femgpu_gmres_mixed: ILU(0) not factored");
     const int n = S->csr->n;
     const int kin = opt->inner_kmax;
-    GDev G;
+    GDev G{};
     FG_TRY(make_gdev(ctx, kin, G));
     G.prec_mode = opt->sweep_mode;
     G.cgs = opt->orthogonalization == 1;
+    G.graphs = opt->graphs == 1;
     if (G.cgs) FG_TRY(ctx->cgs_partials.alloc(static_cast<size_t>(kin + 2) * kCgsBlocks));
     if (opt->factor_sweeps > 0) {
         FG_TRY(launch_parilu32(ctx, S->ilu, opt->factor_sweeps));
@@ -815,10 +883,11 @@ extern "C" int femgpu_gmres_mixed_batched(femgpu_ctx *ctx, const femgpu_system *
              "femgpu_gmres_mixed_batched: ILU(0) not factored");
     const int kin = opt->inner_kmax;
     memset(info, 0, sizeof(*info) * static_cast<size_t>(nsys));
-    GDev G;
+    GDev G{};
     FG_TRY(make_gdev(ctx, kin, G));
     G.prec_mode = opt->sweep_mode;
     G.cgs = opt->orthogonalization == 1;
+    G.graphs = opt->graphs == 1;
     if (G.cgs) FG_TRY(ctx->cgs_partials.alloc(static_cast<size_t>(kin + 2) * kCgsBlocks));
     if (opt->factor_sweeps > 0) {
         FG_TRY(launch_parilu32(ctx, S->ilu, opt->factor_sweeps));

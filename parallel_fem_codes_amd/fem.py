@@ -509,25 +509,27 @@ def pgmres(dev: Device, op, B, X, KrylovBasisVectorsQuantity=30, LinearMaxIter=1
 
 
 def mixed_opts(inner_kmax=30, inner_tol=1e-4, max_refinements=50, sweep_mode=0, factor_sweeps=0,
-               orthogonalization="MGS"):
+               orthogonalization="MGS", graphs=False):
     """femgpu_mixed_opts from keyword values (one inner cycle per refinement)."""
     o = N.MixedOpts()
     o.inner_kmax, o.inner_cycles, o.inner_tol = inner_kmax, 1, inner_tol
     o.max_refinements, o.sweep_mode, o.factor_sweeps = max_refinements, sweep_mode, factor_sweeps
     o.orthogonalization = {"MGS": 0, "CGS": 1}[orthogonalization]
+    o.graphs = 1 if graphs else 0
     return o
 
 
 def gmres_mixed(dev: Device, A: CsrMatrix, F, X, tol=1e-8, inner_kmax=30, inner_tol=1e-4,
-                max_refinements=50, sweep_mode=0, factor_sweeps=0, orthogonalization="MGS"):
+                max_refinements=50, sweep_mode=0, factor_sweeps=0, orthogonalization="MGS", graphs=False):
     """FP32 ILU(0)-GMRES inside FP64 iterative refinement; F is the raw right-hand side.
 
     sweep_mode 0 applies the exact FP32 sweeps of A's FP64 factorisation (A.ilu0_factor() first);
     sweep_mode k > 0 applies k Jacobi iterations per triangle. factor_sweeps k > 0 builds the FP32
     factors by k parallel ILU(0) sweeps instead, with no FP64 factorisation. orthogonalization
-    "MGS" or "CGS" selects the inner Arnoldi's Gram-Schmidt variant."""
+    "MGS" or "CGS" selects the inner Arnoldi's Gram-Schmidt variant; graphs=True replays each inner
+    Arnoldi step from a captured hipGraph."""
     info = N.SolveInfo()
-    o = mixed_opts(inner_kmax, inner_tol, max_refinements, sweep_mode, factor_sweeps, orthogonalization)
+    o = mixed_opts(inner_kmax, inner_tol, max_refinements, sweep_mode, factor_sweeps, orthogonalization, graphs)
     S = _system(A, A)
     N.check(N.lib().femgpu_gmres_mixed(dev.h, C.byref(S), C.byref(o), float(tol), _ptr(F), _ptr(X),
                                        C.byref(info)))

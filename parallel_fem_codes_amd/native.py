@@ -60,7 +60,7 @@ class MixedOpts(C.Structure):
     """femgpu_mixed_opts."""
     _fields_ = [("inner_kmax", C.c_int), ("inner_cycles", C.c_int), ("inner_tol", C.c_double),
                 ("max_refinements", C.c_int), ("sweep_mode", C.c_int), ("factor_sweeps", C.c_int),
-                ("orthogonalization", C.c_int)]
+                ("orthogonalization", C.c_int), ("graphs", C.c_int), ("pad", C.c_int)]
 
 
 class TuneInfo(C.Structure):

@@ -555,6 +555,41 @@ def test_mixed_parallel_preconditioner(dev, torch_cuda, oracle, factor_sweeps, s
     assert info.status == 0 and res <= 1e-8
 
 
+@pytest.mark.parametrize("sweep_mode,factor_sweeps,orth", [(2, 2, "MGS"), (0, 0, "MGS"), (2, 2, "CGS")])
+def test_mixed_graph_replay_equals_eager(dev, torch_cuda, oracle, sweep_mode, factor_sweeps, orth):
+    """Arnoldi steps replayed from captured hipGraphs give the eager launches' results bit for bit,
+    on every replay (eager solves of another matrix in between force re-captures), with the same
+    iteration counts."""
+    torch = torch_cuda
+    s1 = transient_system(oracle, m=120, dt=5e-3)
+    s2 = transient_system(oracle, m=90, dt=2e-3)
+    mats = []
+    for s in (s1, s2):
+        A = fem.CsrMatrix(dev, s["IA"], s["JA"])
+        A.set_values(s["AA"][:-1])
+        if factor_sweeps == 0:
+            A.ilu0_factor()
+        mats.append((A, s))
+
+    def solve(A, s, graphs):
+        X = torch.empty(s["neq"], dtype=torch.float64, device="cuda")
+        info = fem.gmres_mixed(dev, A, T(torch, s["F"]), X, tol=1e-8, inner_kmax=30, inner_tol=1e-4,
+                               sweep_mode=sweep_mode, factor_sweeps=factor_sweeps, orthogonalization=orth,
+                               graphs=graphs)
+        return H(X), info
+
+    ref = [solve(A, s, False) for A, s in mats]
+    for rep in range(3):
+        for k, (A, s) in enumerate(mats):
+            x, info = solve(A, s, True)
+            assert np.array_equal(x, ref[k][0]), (rep, k)
+            assert (info.inner_iterations, info.cycles) == (ref[k][1].inner_iterations, ref[k][1].cycles)
+            x2, _ = solve(A, s, False)          # eager work between the replays
+            assert np.array_equal(x2, ref[k][0])
+    res = np.linalg.norm(s1["F"] - oracle.csrmv(s1["IA"], s1["JA"], s1["AA"], ref[0][0])) / np.linalg.norm(s1["F"])
+    assert res <= 1e-8
+
+
 def test_mixed_reports_stagnation(dev, torch_cuda, oracle):
     """A preconditioner too weak for the operator (two Jacobi iterations on a steady, convection-
     dominated system) must end with status 1 and the residual it reached, not a false success."""
