@@ -207,8 +207,9 @@ struct femgpu_csr {
     fg::DevBuf<float> LU32, Dinv32;     // FP32 factors in A's layout (mixed solver's Jacobi sweeps)
     fg::DevBuf<float> LU32b, Dinv32b;   // second iterate of the parallel factorisation
     fg::DevBuf<float> jac_a, jac_b;     // Jacobi sweep iterates
-    int ell_kl = 0, ell_ku = 0;         // Jacobi ELL widths of the strict triangles (0: CSR kernels)
-    fg::DevBuf<int> ell_cl, ell_cu;     // ELL columns, entry k of row i at k*n + i (padding: column i)
+    int ell_kl = 0, ell_ku = 0;         // Jacobi sliced-ELL widest strict triangles (0: CSR kernels)
+    fg::DevBuf<int> ell_sl, ell_su;     // slice pointers (64-row slices) of the lower / upper parts
+    fg::DevBuf<int> ell_cl, ell_cu;     // columns, entry k of row i at sl[i/64] + 64k + i%64 (padding: column i)
     fg::DevBuf<float> ell_vl, ell_vu;   // their FP32 factor values (padding 0), refreshed with LU32
     fg::FactorPlan fplan;           // numeric factorisation (level schedule)
     fg::FactorPlan fplan_nat;       // the same lists in row order (parallel FP32 factorisation)

@@ -42,26 +42,38 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ IA,
 
 // FP32 SpMV on the sliced-ELL copy: one row per thread, a wave's 64 rows form one slice padded to
 // its longest row, entries column-major inside the slice, so every load of a step of the row loop
-// is one contiguous 256-byte run and the loop count is uniform across the wave. Same xs / xout /
-// skip contract as k_spmv.
+// is one contiguous 256-byte run and the loop count is uniform across the wave. A block of 256
+// threads takes kSellRows * 256 rows, each thread kSellRows rows 256 apart, their loads in flight
+// together. Same xs / xout / skip contract as k_spmv.
+constexpr int kSellRows = 2;
+
 __global__ __launch_bounds__(256) void k_spmv_sell(int n, const int *__restrict__ sptr, const int *__restrict__ col,
                                                    const float *__restrict__ val, const float *__restrict__ x,
                                                    const double *__restrict__ xs_ptr, float *__restrict__ xout,
                                                    float *__restrict__ y, const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    const int row = blockIdx.x * 256 + threadIdx.x;
-    if (row >= n) return;
     const float xs = xs_ptr ? static_cast<float>(*xs_ptr) : 1.0f;
-    const int s = row >> 6, l = row & 63;
-    const int p0 = sptr[s], w = (sptr[s + 1] - p0) >> 6;
-    float acc = 0.0f;
-    for (int k = 0; k < w; ++k) {
-        const int e = p0 + (k << 6) + l;
-        acc += val[e] * (x[col[e]] * xs);
+    float acc[kSellRows];
+#pragma unroll
+    for (int r = 0; r < kSellRows; ++r) {
+        const int row = (blockIdx.x * kSellRows + r) * 256 + threadIdx.x;
+        acc[r] = 0.0f;
+        if (row >= n) continue;
+        const int s = row >> 6, l = row & 63;
+        const int p0 = sptr[s], w = (sptr[s + 1] - p0) >> 6;
+        for (int k = 0; k < w; ++k) {
+            const int e = p0 + (k << 6) + l;
+            acc[r] += val[e] * (x[col[e]] * xs);
+        }
     }
-    y[row] = acc;
-    if (xout) xout[row] = x[row] * xs;
+#pragma unroll
+    for (int r = 0; r < kSellRows; ++r) {
+        const int row = (blockIdx.x * kSellRows + r) * 256 + threadIdx.x;
+        if (row >= n) continue;
+        y[row] = acc[r];
+        if (xout) xout[row] = x[row] * xs;
+    }
 }
 
 __global__ void k_sell_values(int n, const int *__restrict__ IA, const int *__restrict__ sptr,
@@ -165,8 +177,9 @@ int launch_spmv32(femgpu_ctx *ctx, const femgpu_csr *A, const float *x, const do
                   float *y, const int *skip)
 {
     if (A->sell_ok && A->n > 0) {
-        k_spmv_sell<<<(A->n + 255) / 256, 256, 0, ctx->stream>>>(A->n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p, x,
-                                                                 xs, xout, y, skip);
+        const int grid = (A->n + 256 * kSellRows - 1) / (256 * kSellRows);
+        k_spmv_sell<<<grid, 256, 0, ctx->stream>>>(A->n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p, x, xs, xout, y,
+                                                   skip);
         FG_HIP(hipGetLastError());
         return FEMGPU_OK;
     }

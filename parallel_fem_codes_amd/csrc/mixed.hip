@@ -196,16 +196,18 @@ __global__ __launch_bounds__(256) void k_parilu_sweep_sell(int n, const int *__r
 
 // The strict triangles of the SELL-layout factors into the Jacobi ELL arrays.
 __global__ void k_parilu_sell_to_ell(int n, const int *__restrict__ sptr, const int *__restrict__ rinfo,
-                                     const float *__restrict__ LU, int kl, int ku, float *__restrict__ vl,
-                                     float *__restrict__ vu)
+                                     const float *__restrict__ LU, const int *__restrict__ sl,
+                                     const int *__restrict__ su, float *__restrict__ vl, float *__restrict__ vu)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int p0 = sptr[i >> 6] + (i & 63), nL = rinfo[i] & 0xff, len = rinfo[i] >> 8;
-    for (int k = 0; k < kl; ++k) vl[static_cast<size_t>(k) * n + i] = k < nL ? LU[p0 + (k << 6)] : 0.0f;
-    for (int k = 0; k < ku; ++k) {
+    const int s = i >> 6, l = i & 63;
+    const int p0 = sptr[s] + l, nL = rinfo[i] & 0xff, len = rinfo[i] >> 8;
+    const int ql = sl[s] + l, wl = (sl[s + 1] - sl[s]) >> 6, qu = su[s] + l, wu = (su[s + 1] - su[s]) >> 6;
+    for (int k = 0; k < wl; ++k) vl[ql + (k << 6)] = k < nL ? LU[p0 + (k << 6)] : 0.0f;
+    for (int k = 0; k < wu; ++k) {
         const int e = nL + 1 + k;
-        vu[static_cast<size_t>(k) * n + i] = e < len ? LU[p0 + (e << 6)] : 0.0f;
+        vu[qu + (k << 6)] = e < len ? LU[p0 + (e << 6)] : 0.0f;
     }
 }
 
@@ -305,61 +307,89 @@ static int ensure_par(femgpu_csr *A)
     return FEMGPU_OK;
 }
 
-// The strict triangles of the FP32 factors as ELL, column-major (entry k of row i at k*n + i):
-// one thread per row, coalesced loads. Padding entries have value 0 and column i.
+// The strict triangles of the FP32 factors as sliced ELL: rows in slices of 64 (one wavefront),
+// each slice padded to its own longest row, entries column-major inside the slice (entry k of row
+// i at sl[i / 64] + 64 k + i % 64): one thread per row, every load of a step of the row loop one
+// contiguous 256-byte run, a loop count uniform across the wavefront, and padding only where a
+// slice's rows differ in length. Padding entries have value 0 and column i.
 constexpr int kEllMax = 8;
 
 __global__ void k_jacobi_ell_values(int n, const int *__restrict__ IA, const int *__restrict__ diag,
-                                    const float *__restrict__ LU, int kl, int ku, float *__restrict__ vl,
-                                    float *__restrict__ vu)
+                                    const float *__restrict__ LU, const int *__restrict__ sl,
+                                    const int *__restrict__ su, float *__restrict__ vl, float *__restrict__ vu)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int s = i >> 6, l = i & 63;
     const int p0 = IA[i], d = diag[i], p1 = IA[i + 1];
-    for (int k = 0; k < kl; ++k) vl[static_cast<size_t>(k) * n + i] = p0 + k < d ? LU[p0 + k] : 0.0f;
-    for (int k = 0; k < ku; ++k) vu[static_cast<size_t>(k) * n + i] = d + 1 + k < p1 ? LU[d + 1 + k] : 0.0f;
+    const int ql = sl[s] + l, wl = (sl[s + 1] - sl[s]) >> 6, qu = su[s] + l, wu = (su[s + 1] - su[s]) >> 6;
+    for (int k = 0; k < wl; ++k) vl[ql + (k << 6)] = p0 + k < d ? LU[p0 + k] : 0.0f;
+    for (int k = 0; k < wu; ++k) vu[qu + (k << 6)] = d + 1 + k < p1 ? LU[d + 1 + k] : 0.0f;
 }
 
-// y_out = b - L_s y_in (y_in == nullptr: y_in = b), ELL form.
+// A block of 256 threads takes kJacRows * 256 consecutive rows, each thread kJacRows rows 256 apart
+// (same lane, so a wavefront still covers whole 64-row slices): the loads of all its rows are in
+// flight together, and one round of resident wavefronts covers the matrix.
+constexpr int kJacRows = 2;
+
+// y_out = b - L_s y_in (y_in == nullptr: y_in = b), sliced ELL.
 template <int K>
-__global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, int kl, const int *__restrict__ cl,
+__global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__restrict__ sl, const int *__restrict__ cl,
                                                           const float *__restrict__ vl, const float *__restrict__ b,
                                                           const float *__restrict__ yin, float *__restrict__ yout,
                                                           const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
     const float *src = yin ? yin : b;
-    float acc = 0.0f;
+    float acc[kJacRows];
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-        if (k < kl) {
-            const size_t e = static_cast<size_t>(k) * n + i;
-            acc += vl[e] * src[cl[e]];
-        }
-    yout[i] = b[i] - acc;
+    for (int r = 0; r < kJacRows; ++r) {
+        const int i = (blockIdx.x * kJacRows + r) * 256 + threadIdx.x;
+        acc[r] = 0.0f;
+        if (i >= n) continue;
+        const int s = i >> 6, q = sl[s] + (i & 63), w = (sl[s + 1] - sl[s]) >> 6;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (k < w) {
+                const int e = q + (k << 6);
+                acc[r] += vl[e] * src[cl[e]];
+            }
+    }
+#pragma unroll
+    for (int r = 0; r < kJacRows; ++r) {
+        const int i = (blockIdx.x * kJacRows + r) * 256 + threadIdx.x;
+        if (i < n) yout[i] = b[i] - acc[r];
+    }
 }
 
-// z_out = Dinv (y - U_s z_in) (z_in == nullptr: z_in = Dinv y), ELL form.
+// z_out = Dinv (y - U_s z_in) (z_in == nullptr: z_in = Dinv y), sliced ELL.
 template <int K>
-__global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, int ku, const int *__restrict__ cu,
+__global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, const int *__restrict__ su, const int *__restrict__ cu,
                                                           const float *__restrict__ vu, const float *__restrict__ Dinv,
                                                           const float *__restrict__ y, const float *__restrict__ zin,
                                                           float *__restrict__ zout, const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    float acc = 0.0f;
+    float acc[kJacRows];
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-        if (k < ku) {
-            const size_t e = static_cast<size_t>(k) * n + i;
-            const int j = cu[e];
-            acc += vu[e] * (zin ? zin[j] : Dinv[j] * y[j]);
-        }
-    zout[i] = Dinv[i] * (y[i] - acc);
+    for (int r = 0; r < kJacRows; ++r) {
+        const int i = (blockIdx.x * kJacRows + r) * 256 + threadIdx.x;
+        acc[r] = 0.0f;
+        if (i >= n) continue;
+        const int s = i >> 6, q = su[s] + (i & 63), w = (su[s + 1] - su[s]) >> 6;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (k < w) {
+                const int e = q + (k << 6);
+                const int j = cu[e];
+                acc[r] += vu[e] * (zin ? zin[j] : Dinv[j] * y[j]);
+            }
+    }
+#pragma unroll
+    for (int r = 0; r < kJacRows; ++r) {
+        const int i = (blockIdx.x * kJacRows + r) * 256 + threadIdx.x;
+        if (i < n) zout[i] = Dinv[i] * (y[i] - acc[r]);
+    }
 }
 
 int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag)
@@ -372,14 +402,34 @@ int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *J
     }
     A->ell_kl = A->ell_ku = 0;
     if (n == 0 || kl > kEllMax || ku > kEllMax) return FEMGPU_OK;
+    // slice pointers: each 64-row slice padded to its own longest strict lower / upper part
+    const int ns = (n + 63) / 64;
+    std::vector<int> sl(static_cast<size_t>(ns) + 1, 0), su(static_cast<size_t>(ns) + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+        int wl = 0, wu = 0;
+        for (int i = 64 * s; i < std::min(n, 64 * s + 64); ++i) {
+            wl = std::max(wl, diag[i] - IA[i]);
+            wu = std::max(wu, IA[i + 1] - diag[i] - 1);
+        }
+        sl[static_cast<size_t>(s) + 1] = sl[s] + 64 * wl;
+        su[static_cast<size_t>(s) + 1] = su[s] + 64 * wu;
+    }
+    std::vector<int> cl(std::max(sl[ns], 1)), cu(std::max(su[ns], 1));
+    for (int s = 0; s < ns; ++s) {
+        const int wl = (sl[s + 1] - sl[s]) / 64, wu = (su[s + 1] - su[s]) / 64;
+        for (int l = 0; l < 64; ++l) {
+            const int i = 64 * s + l, r = std::min(i, n - 1);
+            for (int k = 0; k < wl; ++k)
+                cl[static_cast<size_t>(sl[s]) + 64 * k + l] = i < n && IA[i] + k < diag[i] ? JA[IA[i] + k] : r;
+            for (int k = 0; k < wu; ++k)
+                cu[static_cast<size_t>(su[s]) + 64 * k + l] =
+                    i < n && diag[i] + 1 + k < IA[i + 1] ? JA[diag[i] + 1 + k] : r;
+        }
+    }
     kl = std::max(kl, 1);
     ku = std::max(ku, 1);
-    std::vector<int> cl(static_cast<size_t>(kl) * n), cu(static_cast<size_t>(ku) * n);
-    for (int i = 0; i < n; ++i) {
-        for (int k = 0; k < kl; ++k) cl[static_cast<size_t>(k) * n + i] = IA[i] + k < diag[i] ? JA[IA[i] + k] : i;
-        for (int k = 0; k < ku; ++k)
-            cu[static_cast<size_t>(k) * n + i] = diag[i] + 1 + k < IA[i + 1] ? JA[diag[i] + 1 + k] : i;
-    }
+    FG_TRY(A->ell_sl.upload(sl.data(), sl.size(), ctx->stream));
+    FG_TRY(A->ell_su.upload(su.data(), su.size(), ctx->stream));
     FG_TRY(A->ell_cl.upload(cl.data(), cl.size(), ctx->stream));
     FG_TRY(A->ell_cu.upload(cu.data(), cu.size(), ctx->stream));
     FG_TRY(A->ell_vl.alloc(cl.size()));
@@ -393,8 +443,9 @@ int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *J
 static int refresh_ell(femgpu_ctx *ctx, femgpu_csr *A)
 {
     if (A->ell_kl == 0) return FEMGPU_OK;
-    k_jacobi_ell_values<<<(A->n + 255) / 256, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->diag.p, A->LU32.p, A->ell_kl,
-                                                                      A->ell_ku, A->ell_vl.p, A->ell_vu.p);
+    k_jacobi_ell_values<<<(A->n + 255) / 256, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->diag.p, A->LU32.p,
+                                                                      A->ell_sl.p, A->ell_su.p, A->ell_vl.p,
+                                                                      A->ell_vu.p);
     FG_HIP(hipGetLastError());
     return FEMGPU_OK;
 }
@@ -422,8 +473,8 @@ static int parilu_sell(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
     FG_HIP(hipGetLastError());
     A->lu32_in_sell = true;
     if (A->ell_kl > 0) {
-        k_parilu_sell_to_ell<<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->par_rinfo.p, A->par_a.p, A->ell_kl,
-                                                            A->ell_ku, A->ell_vl.p, A->ell_vu.p);
+        k_parilu_sell_to_ell<<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->par_rinfo.p, A->par_a.p,
+                                                            A->ell_sl.p, A->ell_su.p, A->ell_vl.p, A->ell_vu.p);
         FG_HIP(hipGetLastError());
         return FEMGPU_OK;
     }
@@ -471,12 +522,12 @@ int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A)
 template <int K>
 static void jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip)
 {
-    const int n = A->n, grid = (n + 255) / 256;
+    const int n = A->n, grid = (n + 256 * kJacRows - 1) / (256 * kJacRows);
     float *t[2] = {A->jac_a.p, A->jac_b.p};
     const float *prev = nullptr;
     for (int k = 0; k < it; ++k) {
         float *out = t[k & 1];
-        k_jacobi_lower_ell<K><<<grid, 256, 0, ctx->stream>>>(n, A->ell_kl, A->ell_cl.p, A->ell_vl.p, b, prev, out, skip);
+        k_jacobi_lower_ell<K><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, b, prev, out, skip);
         prev = out;
     }
     const float *y = prev;
@@ -484,7 +535,7 @@ static void jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, f
     prev = nullptr;
     for (int k = 1; k <= it; ++k) {
         float *out = ((it - k) % 2 == 0) ? x : other;
-        k_jacobi_upper_ell<K><<<grid, 256, 0, ctx->stream>>>(n, A->ell_ku, A->ell_cu.p, A->ell_vu.p, A->Dinv32.p, y,
+        k_jacobi_upper_ell<K><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p, A->Dinv32.p, y,
                                                              prev, out, skip);
         prev = out;
     }
@@ -495,20 +546,20 @@ int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const f
 {
     const int n = A->n;
     if (n == 0) return FEMGPU_OK;
-    const int grid = (n + 255) / 256;
+    const int grid = (n + 256 * kJacRows - 1) / (256 * kJacRows);
     if (A->ell_kl > 0) {
         const bool k4 = std::max(A->ell_kl, A->ell_ku) <= 4;
         if (upper && k4)
-            k_jacobi_upper_ell<4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_ku, A->ell_cu.p, A->ell_vu.p, A->Dinv32.p,
+            k_jacobi_upper_ell<4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p, A->Dinv32.p,
                                                                  rhs, in, out, nullptr);
         else if (upper)
-            k_jacobi_upper_ell<kEllMax><<<grid, 256, 0, ctx->stream>>>(n, A->ell_ku, A->ell_cu.p, A->ell_vu.p,
+            k_jacobi_upper_ell<kEllMax><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
                                                                        A->Dinv32.p, rhs, in, out, nullptr);
         else if (k4)
-            k_jacobi_lower_ell<4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_kl, A->ell_cl.p, A->ell_vl.p, rhs, in, out,
+            k_jacobi_lower_ell<4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs, in, out,
                                                                  nullptr);
         else
-            k_jacobi_lower_ell<kEllMax><<<grid, 256, 0, ctx->stream>>>(n, A->ell_kl, A->ell_cl.p, A->ell_vl.p, rhs, in,
+            k_jacobi_lower_ell<kEllMax><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs, in,
                                                                        out, nullptr);
     } else {
         constexpr int LPR = 4;
