@@ -247,6 +247,12 @@ __global__ void k_rhs_steady(int neq, const int *__restrict__ dst, const int *__
     F[dst[k]] = f;
 }
 
+// Gathers: the first kGatherUnroll entries of a list are loaded together (positions past the list's
+// end clamped to its first entry, their terms not added), then summed in list order -- the same
+// additions in the same order as a loop over the list, with the loads overlapped; longer lists
+// finish in a loop.
+constexpr int kGatherUnroll = 8;
+
 __global__ void k_rhs_transient(int NEQ, const int *__restrict__ dst, const int *__restrict__ ptr,
                                 const int *__restrict__ src, const double *__restrict__ rec, double *__restrict__ R)
 {
@@ -257,14 +263,33 @@ __global__ void k_rhs_transient(int NEQ, const int *__restrict__ dst, const int 
         R[NEQ] = 0.0;
         return;
     }
+    const int p0 = ptr[k], p1 = ptr[k + 1];
     double r = 0;
-    for (int q = ptr[k]; q < ptr[k + 1]; ++q) {
-        const double2 *t = reinterpret_cast<const double2 *>(rec + 4 * static_cast<size_t>(src[q]));
-        const double2 f0 = t[0], f1 = t[1];   // load of row a, then its three coupling terms
-        r += f0.x;
-        r -= f0.y;
-        r -= f1.x;
-        r -= f1.y;
+    if (p1 > p0) {
+        double2 f0[kGatherUnroll], f1[kGatherUnroll];
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; ++u) {
+            const int q = p0 + u < p1 ? p0 + u : p0;
+            const double2 *t = reinterpret_cast<const double2 *>(rec + 4 * static_cast<size_t>(src[q]));
+            f0[u] = t[0];   // load of row a, then its three coupling terms
+            f1[u] = t[1];
+        }
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; ++u)
+            if (p0 + u < p1) {
+                r += f0[u].x;
+                r -= f0[u].y;
+                r -= f1[u].x;
+                r -= f1[u].y;
+            }
+        for (int q = p0 + kGatherUnroll; q < p1; ++q) {
+            const double2 *t = reinterpret_cast<const double2 *>(rec + 4 * static_cast<size_t>(src[q]));
+            const double2 g0 = t[0], g1 = t[1];
+            r += g0.x;
+            r -= g0.y;
+            r -= g1.x;
+            r -= g1.y;
+        }
     }
     R[dst[k]] = r;
 }
@@ -281,8 +306,17 @@ __global__ void k_aa_gather(int nnz, const int *__restrict__ dst, const int *__r
         AA[nnz] = 0.0;
         return;
     }
+    const int p0 = ptr[k], p1 = ptr[k + 1];
     double v = 0;
-    for (int q = ptr[k]; q < ptr[k + 1]; ++q) v += Ke[src[q]];
+    if (p1 > p0) {
+        double c[kGatherUnroll];
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; ++u) c[u] = Ke[src[p0 + u < p1 ? p0 + u : p0]];
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; ++u)
+            if (p0 + u < p1) v += c[u];
+        for (int q = p0 + kGatherUnroll; q < p1; ++q) v += Ke[src[q]];
+    }
     AA[dst[k]] = v;
 }
 
