@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "common.h"
+#include "device_util.h"
 
 namespace fg {
 
@@ -232,8 +233,34 @@ __global__ void k_rhs_steady(int neq, const int *__restrict__ dst, const int *__
 #pragma clang fp contract(off)
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= neq) return;
+    const int p0 = ptr[k], p1 = ptr[k + 1];
     double f = 0;
-    for (int q = ptr[k]; q < ptr[k + 1]; ++q) {
+    // entry ea = 3e + a adds fe[ea], then subtracts the lifts of the element's other Dirichlet
+    // vertices b != a in increasing b (lift slots 2a, 2a + 1); the first four entries' loads
+    // overlap, the arithmetic is the loop's
+    constexpr int U = 4;
+    if (p1 > p0) {
+        double fv[U], l0[U], l1[U];
+        int ms[U], as[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int ea = src[p0 + u < p1 ? p0 + u : p0], e = ea / 3, a = ea % 3;
+            fv[u] = fe[ea];
+            ms[u] = mask[e];
+            as[u] = a;
+            l0[u] = lift[6 * e + 2 * a];
+            l1[u] = lift[6 * e + 2 * a + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (p0 + u < p1) {
+                const int a = as[u], b0 = a == 0 ? 1 : 0, b1 = a == 2 ? 1 : 2;
+                f += fv[u];
+                if (ms[u] & (1 << b0)) f -= l0[u];
+                if (ms[u] & (1 << b1)) f -= l1[u];
+            }
+    }
+    for (int q = p0 + U; q < p1; ++q) {
         const int ea = src[q], e = ea / 3, a = ea % 3;
         f += fe[ea];
         const int m = mask[e];
@@ -306,18 +333,7 @@ __global__ void k_aa_gather(int nnz, const int *__restrict__ dst, const int *__r
         AA[nnz] = 0.0;
         return;
     }
-    const int p0 = ptr[k], p1 = ptr[k + 1];
-    double v = 0;
-    if (p1 > p0) {
-        double c[kGatherUnroll];
-#pragma unroll
-        for (int u = 0; u < kGatherUnroll; ++u) c[u] = Ke[src[p0 + u < p1 ? p0 + u : p0]];
-#pragma unroll
-        for (int u = 0; u < kGatherUnroll; ++u)
-            if (p0 + u < p1) v += c[u];
-        for (int q = p0 + kGatherUnroll; q < p1; ++q) v += Ke[src[q]];
-    }
-    AA[dst[k]] = v;
+    AA[dst[k]] = gather_sum<kGatherUnroll>(src, Ke, ptr[k], ptr[k + 1]);
 }
 
 static ElemIn elem_in(femgpu_mesh *M, const femgpu_transport_problem *pb, const double *u, const double *du,

@@ -10,6 +10,7 @@
 // kernel), so element matrices, residuals, CSR values and right-hand sides are bit-identical.
 // Global accumulation is a gather in element order, as for the transport assembly.
 #include "common.h"
+#include "device_util.h"
 
 namespace fg {
 
@@ -133,9 +134,7 @@ __global__ void k_rhs_gather(int NEQ, const int *__restrict__ dst, const int *__
         F[NEQ] = 0.0;
         return;
     }
-    double f = 0.0;
-    for (int q = ptr[k]; q < ptr[k + 1]; ++q) f += Res[src[q]];
-    F[dst[k]] = f;
+    F[dst[k]] = gather_sum<8>(src, Res, ptr[k], ptr[k + 1]);
 }
 
 __global__ void k_aa_gather_blocks(int nnz, const int *__restrict__ dst, const int *__restrict__ ptr,
@@ -148,9 +147,7 @@ __global__ void k_aa_gather_blocks(int nnz, const int *__restrict__ dst, const i
         AA[nnz] = 0.0;
         return;
     }
-    double v = 0;
-    for (int q = ptr[k]; q < ptr[k + 1]; ++q) v += Ke[src[q]];
-    AA[dst[k]] = v;
+    AA[dst[k]] = gather_sum<8>(src, Ke, ptr[k], ptr[k + 1]);
 }
 
 }  // namespace fg

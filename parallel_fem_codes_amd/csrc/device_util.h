@@ -40,4 +40,22 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// sum of vals[src[q]] for q = p0 .. p1-1, added in list order. The first U entries are loaded
+// together (positions past the end clamped to p0, their terms not added), so their loads overlap;
+// the additions are those of the plain loop, in the same order.
+template <int U>
+__device__ __forceinline__ double gather_sum(const int *__restrict__ src, const double *__restrict__ vals, int p0, int p1)
+{
+    double v = 0.0;
+    if (p1 <= p0) return v;
+    double c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = vals[src[p0 + u < p1 ? p0 + u : p0]];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (p0 + u < p1) v += c[u];
+    for (int q = p0 + U; q < p1; ++q) v += vals[src[q]];
+    return v;
+}
+
 }  // namespace fg
