@@ -208,21 +208,41 @@ __device__ __forceinline__ void transient_elem(const ElemIn &in, int e, Transien
 }
 
 // Transient element kernel: Ae (9) and, per element row a, the record {fe_a, lift_a0, lift_a1,
-// lift_a2} that the residual gather reads in one piece (rec[4*(3e+a) ..]).
+// lift_a2} that the residual gather reads in one piece (rec[4*(3e+a) ..]). A block's 256 elements
+// own one contiguous run of Ae and one of rec: each is staged in LDS and written out with 16-byte
+// stores of consecutive lanes to consecutive addresses (a thread writing its own element's 72 or 96
+// bytes would make every store instruction touch dozens of cache lines).
 __global__ __launch_bounds__(256) void k_elem_transient(int nel, ElemIn in, double *__restrict__ Ae,
                                                         double *__restrict__ rec)
 {
 #pragma clang fp contract(off)
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= nel) return;
+    __shared__ double2 stage[256 * 6];   // 256 records of 12 doubles, or 256 Ae of 9
+    double *sd = reinterpret_cast<double *>(stage);
+    const int e0 = blockIdx.x * 256, e = e0 + static_cast<int>(threadIdx.x);
+    const int nb = min(256, nel - e0);
     TransientElem T;
-    transient_elem(in, e, T);
-    double2 *r = reinterpret_cast<double2 *>(rec + 12 * static_cast<size_t>(e));
-    for (int a = 0; a < 3; ++a) {
-        r[2 * a] = make_double2(T.Fe[a], T.lift[a][0]);
-        r[2 * a + 1] = make_double2(T.lift[a][1], T.lift[a][2]);
-        for (int b = 0; b < 3; ++b) Ae[9 * static_cast<size_t>(e) + 3 * a + b] = T.Ae[a][b];
+    if (e < nel) {
+        transient_elem(in, e, T);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) sd[9 * threadIdx.x + 3 * a + b] = T.Ae[a][b];
     }
+    __syncthreads();
+    const int nA = 9 * nb;   // Ae + 9 e0 is 16-byte aligned (e0 is a multiple of 256)
+    double *dA = Ae + 9 * static_cast<size_t>(e0);
+    for (int q = threadIdx.x; q < nA / 2; q += 256) reinterpret_cast<double2 *>(dA)[q] = stage[q];
+    if ((nA & 1) && threadIdx.x == 0) dA[nA - 1] = sd[nA - 1];
+    __syncthreads();
+    if (e < nel)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            stage[6 * threadIdx.x + 2 * a] = make_double2(T.Fe[a], T.lift[a][0]);
+            stage[6 * threadIdx.x + 2 * a + 1] = make_double2(T.lift[a][1], T.lift[a][2]);
+        }
+    __syncthreads();
+    double2 *dR = reinterpret_cast<double2 *>(rec + 12 * static_cast<size_t>(e0));
+    for (int q = threadIdx.x; q < 6 * nb; q += 256) dR[q] = stage[q];
 }
 
 // RHS gathers. rhs_src entries are e*3+a in element order.
