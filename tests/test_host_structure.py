@@ -74,7 +74,9 @@ def test_structure_matches_oracle_at_scale(L, oracle, m, right, reorder):
                       ("invP_id", "invP")):
             assert np.array_equal(getattr(c, k), o[ok]), (k, permute)
     if m == 200:
-        assert c.nnz == 275617   # the survey's count for the 201x201-node mesh
+        # (m-1)^2 interior unknowns on a grid cut along one diagonal: each couples to itself and
+        # along 2 (m-1)(m-2) axis edges and (m-2)^2 diagonal edges, both ways
+        assert c.nnz == (m - 1) ** 2 + 2 * (2 * (m - 1) * (m - 2) + (m - 2) ** 2) == 275617
     Perm, invPerm = fem.Permutation_of_LM(ien, t, invP, neq, reorder)
     oP, oiP = oracle.ebe_permutation(ien, t, invP, neq, reorder=reorder)
     assert np.array_equal(Perm, oP) and np.array_equal(invPerm, oiP)
