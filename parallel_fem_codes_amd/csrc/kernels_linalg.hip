@@ -31,7 +31,14 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ IA,
     const T xs = xs_ptr ? static_cast<T>(*xs_ptr) : T(1);
     const int p0 = IA[row], p1 = IA[row + 1];
     T acc = 0;
-    for (int p = p0 + lane; p < p1; p += LPR) acc += AA[p] * (x[JA[p]] * xs);
+    // two of the lane's entries per step, loaded together (the second clamped to the first when
+    // past the row's end and then not added): the plain loop's additions, in its order
+    for (int p = p0 + lane; p < p1; p += 2 * LPR) {
+        const int p2 = p + LPR, q2 = p2 < p1 ? p2 : p;
+        const T t1 = AA[p] * (x[JA[p]] * xs), t2 = AA[q2] * (x[JA[q2]] * xs);
+        acc += t1;
+        acc = p2 < p1 ? acc + t2 : acc;
+    }
 #pragma unroll
     for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
     if (lane == 0) {
@@ -142,7 +149,12 @@ __global__ __launch_bounds__(256) void k_residual(int n, const int *__restrict__
     if (row >= n) return;
     const int p0 = IA[row], p1 = IA[row + 1];
     double acc = 0;
-    for (int p = p0 + lane; p < p1; p += LPR) acc += AA[p] * x[JA[p]];
+    for (int p = p0 + lane; p < p1; p += 2 * LPR) {   // as in k_spmv: two entries per step
+        const int p2 = p + LPR, q2 = p2 < p1 ? p2 : p;
+        const double t1 = AA[p] * x[JA[p]], t2 = AA[q2] * x[JA[q2]];
+        acc += t1;
+        acc = p2 < p1 ? acc + t2 : acc;
+    }
 #pragma unroll
     for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
     if (lane == 0) r[row] = f[row] - acc;

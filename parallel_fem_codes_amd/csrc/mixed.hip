@@ -168,19 +168,31 @@ __global__ __launch_bounds__(256) void k_parilu_sweep_sell(int n, const int *__r
         v[k] = k < w ? AA[p0 + (k << 6)] : 0.0f;
         lo[k] = k < w ? LUo[p0 + (k << 6)] : 0.0f;
     }
-    for (int u = 0; u < nu; ++u) {
-        const unsigned code = ucode[u0 + (u << 6)];
-        const float other = LUo[usrc[u0 + (u << 6)]];
-        if (code != 0xffu) {
-            const unsigned t = code >> 4, dest = code & 15u;
-            float lt = 0.0f;
+    // update terms in groups of four: the group's codes, sources and factor values are loaded
+    // together (positions past the row's list clamped to its first, their terms skipped), then
+    // applied in list order
+    constexpr int G = 4;
+    for (int u = 0; u < nu; u += G) {
+        unsigned code[G];
+        float other[G];
 #pragma unroll
-            for (int k = 0; k < MAXW; ++k) lt = (k == static_cast<int>(t)) ? lo[k] : lt;
-            const float c = lt * other;
-#pragma unroll
-            for (int k = 0; k < MAXW; ++k)
-                if (k == static_cast<int>(dest)) v[k] -= c;
+        for (int g = 0; g < G; ++g) {
+            const int q = u0 + ((u + g < nu ? u + g : 0) << 6);
+            code[g] = u + g < nu ? ucode[q] : 0xffu;
+            other[g] = LUo[usrc[q]];
         }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (code[g] != 0xffu) {
+                const unsigned t = code[g] >> 4, dest = code[g] & 15u;
+                float lt = 0.0f;
+#pragma unroll
+                for (int k = 0; k < MAXW; ++k) lt = (k == static_cast<int>(t)) ? lo[k] : lt;
+                const float c = lt * other[g];
+#pragma unroll
+                for (int k = 0; k < MAXW; ++k)
+                    if (k == static_cast<int>(dest)) v[k] -= c;
+            }
     }
     float piv = 1.0f;
 #pragma unroll
