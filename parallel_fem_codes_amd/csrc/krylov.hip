@@ -326,7 +326,7 @@ __global__ __launch_bounds__(256) void k_cycle_start_finish(GDev G, int nb, int 
 // in the reference's per-entry order (pgmres.c:173-175).
 template <typename T, typename TX>
 __global__ __launch_bounds__(256) void k_cycle_finish(GDev G, int n, const T *__restrict__ V, size_t ldv,
-                                                      TX *__restrict__ X, double scale)
+                                                      TX *__restrict__ X, double scale, int assign)
 {
 #pragma clang fp contract(off)
     __shared__ double ys[160];
@@ -344,14 +344,15 @@ __global__ __launch_bounds__(256) void k_cycle_finish(GDev G, int n, const T *__
     __syncthreads();
     if (i < 0) return;
     for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        // assign: X was not initialised (x0 = 0): the sums start from 0 instead of X
         if (scale == 1.0) {
-            TX xv = X[k];
+            TX xv = assign ? TX(0) : X[k];
             for (int j = 0; j <= i; ++j) xv = xv + static_cast<TX>(V[j * ldv + k]) * static_cast<TX>(ys[j]);
             X[k] = xv;
         } else {
             double d = 0;
             for (int j = 0; j <= i; ++j) d = d + static_cast<double>(V[j * ldv + k]) * ys[j];
-            X[k] = static_cast<TX>(static_cast<double>(X[k]) + scale * d);
+            X[k] = static_cast<TX>((assign ? 0.0 : static_cast<double>(X[k])) + scale * d);
         }
     }
 }
@@ -368,6 +369,30 @@ __global__ void k_scale_to_f32(int n, const double *__restrict__ r, const double
 __global__ void k_sqrt_inplace(double *v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *v = sqrt(*v);
+}
+
+// ||x||: per-block partial sums of squares, then one block sums them in a fixed order and stores
+// the square root both in device memory (for later kernels) and in a host-mapped word (read by the
+// host after the stream synchronises, with no copy).
+__global__ __launch_bounds__(256) void k_sumsq(int n, const double *__restrict__ x, double *__restrict__ partials)
+{
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) acc += x[k] * x[k];
+    acc = block_sum<256>(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_norm_finish(int nb, const double *__restrict__ partials, double *d_out,
+                                                     double *h_out)
+{
+    __shared__ double red[4];
+    const double v = sum_partials(partials, nb, red);
+    if (threadIdx.x == 0) {
+        const double r = sqrt(v);
+        *d_out = r;
+        __hip_atomic_store(h_out, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // ---- batched systems: K independent systems stored as one block-diagonal matrix, system s on rows
@@ -434,6 +459,21 @@ __global__ __launch_bounds__(256) void k_cycle_finish_seg(GDev G, int n, const f
 }
 
 static int grid_for(int n) { return std::max(1, std::min(kReduceBlocks, (n + 1023) / 1024)); }
+
+// ||x|| into d_out and the host-mapped slot of the context (slots 248..255 of the mapped doubles,
+// past the per-step residual words), then waits for the stream and returns it.
+static int norm64_sync(femgpu_ctx *ctx, int n, const double *x, double *d_out, int slot, double &host)
+{
+    const int grid = grid_for(n);
+    double *dmap = reinterpret_cast<double *>(ctx->d_status + 256) + 248 + slot;
+    volatile double *hmap = reinterpret_cast<volatile double *>(ctx->h_status + 256) + 248 + slot;
+    k_sumsq<<<grid, 256, 0, ctx->stream>>>(n, x, ctx->partials.p);
+    k_norm_finish<<<1, 256, 0, ctx->stream>>>(grid, ctx->partials.p, d_out, dmap);
+    FG_HIP(hipGetLastError());
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    host = *hmap;
+    return FEMGPU_OK;
+}
 
 // The MGS passes of Arnoldi step i (j = 0..i, then the norm pass) and the column's finish.
 template <typename T>
@@ -753,7 +793,7 @@ extern "C" int femgpu_pgmres(femgpu_ctx *ctx, const femgpu_system *S, int kmax, 
             (void)hipEventDestroy(ev[0]);
             (void)hipEventDestroy(ev[1]);
         }
-        k_cycle_finish<double, double><<<grid, 256, 0, ctx->stream>>>(G, n, V, ldv, d_X, 1.0);
+        k_cycle_finish<double, double><<<grid, 256, 0, ctx->stream>>>(G, n, V, ldv, d_X, 1.0, 0);
         FG_HIP(hipGetLastError());
         // device truth: steps and rho of this cycle
         FG_HIP(hipMemcpyAsync(&ctx->h_pinned[0], G.scal + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -810,30 +850,33 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
     double *r = ctx->gm_r.p;
     double *d_rn = ctx->scalars.p, *d_fn = ctx->scalars.p + 1;
     const int grid = grid_for(n);
-    FG_HIP(hipMemsetAsync(d_X, 0, sizeof(double) * n, ctx->stream));
-    FG_TRY(dot64(ctx, n, d_F, d_F, d_fn));
-    FG_HIP(hipMemcpyAsync(&ctx->h_pinned[0], d_fn, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    FG_HIP(hipStreamSynchronize(ctx->stream));
-    const double normF = sqrt(ctx->h_pinned[0]);
     if (info) memset(info, 0, sizeof(*info));
-    if (normF == 0.0) return FEMGPU_OK;
+    double normF = 0.0;
+    FG_TRY(norm64_sync(ctx, n, d_F, d_fn, 0, normF));
+    if (normF == 0.0 || opt->max_refinements == 0) {   // X = 0 is the answer (or no refinement allowed)
+        FG_HIP(hipMemsetAsync(d_X, 0, sizeof(double) * n, ctx->stream));
+        if (info) {
+            info->normb = normF;
+            info->rho = normF;
+            info->true_rel_residual = normF > 0.0 ? 1.0 : 0.0;
+            info->status = normF > 0.0 ? 1 : 0;
+        }
+        return FEMGPU_OK;
+    }
     int refinements = 0, inner_total = 0;
     double rn = normF;
     for (;;) {
-        // outer FP64 residual r = F - A X (X = 0 on the first pass: r = F)
-        if (refinements == 0) {
-            FG_HIP(hipMemcpyAsync(r, d_F, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
-        } else {
+        // outer FP64 residual r = F - A X; on the first pass X = 0 (not yet written): r = F, ||r|| = ||F||
+        const double *rr = d_F, *d_rcur = d_fn;
+        if (refinements > 0) {
             FG_TRY(launch_residual64(ctx, S->csr, d_X, d_F, r));
+            FG_TRY(norm64_sync(ctx, n, r, d_rn, 1, rn));
+            rr = r;
+            d_rcur = d_rn;
         }
-        FG_TRY(dot64(ctx, n, r, r, d_rn));
-        k_sqrt_inplace<<<1, 64, 0, ctx->stream>>>(d_rn);
-        FG_HIP(hipMemcpyAsync(&ctx->h_pinned[2], d_rn, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        FG_HIP(hipStreamSynchronize(ctx->stream));
-        rn = ctx->h_pinned[2];
         if (rn <= tol * normF || refinements >= opt->max_refinements) break;
         // inner FP32 solve of M^-1 A d = M^-1 r / ||r||
-        k_scale_to_f32<<<grid * 4, 256, 0, ctx->stream>>>(n, r, d_rn, wA);
+        k_scale_to_f32<<<grid * 4, 256, 0, ctx->stream>>>(n, rr, d_rcur, wA);
         FG_HIP(hipGetLastError());
         FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wA, wA, nullptr));
         for (int cyc = 0; cyc < std::max(1, opt->inner_cycles); ++cyc) {
@@ -842,13 +885,17 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
             FG_HIP(hipGetLastError());
             int steps = 0;
             FG_TRY(run_cycle<float>(ctx, S, G, n, V, ldv, wA, wB, kin, &steps));
-            k_cycle_finish<float, double><<<grid, 256, 0, ctx->stream>>>(G, n, V, ldv, d_X, rn);
+            k_cycle_finish<float, double><<<grid, 256, 0, ctx->stream>>>(G, n, V, ldv, d_X, rn, refinements == 0);
             FG_HIP(hipGetLastError());
-            FG_HIP(hipMemcpyAsync(&ctx->h_pinned[1], G.flags + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
             FG_HIP(hipStreamSynchronize(ctx->stream));
             FG_TRY(check_device_error(ctx));
-            int dsteps;
-            memcpy(&dsteps, &ctx->h_pinned[1], sizeof(int));
+            // the device's step count: the first step that reported the cycle done
+            int dsteps = steps;
+            for (int i = 0; i < steps; ++i)
+                if (ctx->h_status[i]) {
+                    dsteps = i + 1;
+                    break;
+                }
             inner_total += dsteps;
         }
         ++refinements;
