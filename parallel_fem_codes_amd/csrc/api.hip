@@ -251,6 +251,8 @@ extern "C" int femgpu_csr_values_changed(femgpu_csr *A)
     A->factored = false;
     A->f32_factored = false;
     FG_TRY(launch_sell_values(A->ctx, A));
+    // the FP32 CSR copy is read only where the sliced-ELL copy or its factor layout is missing
+    if (A->sell_ok && A->par_ok) return FEMGPU_OK;
     return launch_to_f32(A->ctx, A->AA.p, A->AA32.p, A->nnz);
 }
 

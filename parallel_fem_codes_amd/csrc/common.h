@@ -178,7 +178,7 @@ struct femgpu_csr {
     std::vector<int> hIA, hJA;
     fg::DevBuf<int> IA, JA, diag;   // diag[i] = position of the diagonal entry of row i
     fg::DevBuf<double> AA;          // nnz+1 (dummy slot nnz)
-    fg::DevBuf<float> AA32;
+    fg::DevBuf<float> AA32;         // FP32 CSR values, kept only when sell_ok && par_ok is false
     int sell_ok = 0;                    // FP32 sliced-ELL copy present (slices of 64 rows)
     fg::DevBuf<int> sell_ptr;           // slice -> first entry (nslices + 1); slice width = difference / 64
     fg::DevBuf<int> sell_col;           // entry k of row 64s + l at sell_ptr[s] + 64k + l (padding: the row)

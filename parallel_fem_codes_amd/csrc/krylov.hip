@@ -103,6 +103,8 @@ __device__ __forceinline__ double sum_partials(const double *partials, int nb, d
     return total;
 }
 
+constexpr int kMgsVec = 4;
+
 // MGS pass j of step i on w (vector type T, accumulation in FP64); passes alternate between the two
 // partial sets, pass j writing set j & 1:
 //   uprev != null : w -= h_{j-1,i} * uprev, where every block forms h_{j-1,i} from pass j-1's
@@ -123,18 +125,33 @@ __global__ __launch_bounds__(256) void k_mgs(GDev G, int n, int i, int j, T *__r
         h = static_cast<T>(hp);
     }
     double acc = 0;
-    const int nv = n / V::N;
-    for (int k = blockIdx.x * 256 + threadIdx.x; k < nv; k += gridDim.x * 256) {
-        VT wv = reinterpret_cast<const VT *>(w)[k];
-        if (uprev) {
-            const VT pv = reinterpret_cast<const VT *>(uprev)[k];
+    const int nv = n / V::N, stride = gridDim.x * 256;
+    // kMgsVec vector chunks per thread per step, their loads issued together
+    for (int k0 = blockIdx.x * 256 + threadIdx.x; k0 < nv; k0 += kMgsVec * stride) {
+        VT wv[kMgsVec], pv[kMgsVec], uv[kMgsVec];
 #pragma unroll
-            for (int c = 0; c < V::N; ++c) V::set(wv, c, V::at(wv, c) - h * V::at(pv, c));
-            reinterpret_cast<VT *>(w)[k] = wv;
+        for (int u = 0; u < kMgsVec; ++u) {
+            const int k = k0 + u * stride;
+            if (k < nv) {
+                wv[u] = reinterpret_cast<const VT *>(w)[k];
+                if (uprev) pv[u] = reinterpret_cast<const VT *>(uprev)[k];
+                if (unext) uv[u] = reinterpret_cast<const VT *>(unext)[k];
+            }
         }
-        const VT uv = unext ? reinterpret_cast<const VT *>(unext)[k] : wv;
 #pragma unroll
-        for (int c = 0; c < V::N; ++c) acc += static_cast<double>(V::at(uv, c)) * static_cast<double>(V::at(wv, c));
+        for (int u = 0; u < kMgsVec; ++u) {
+            const int k = k0 + u * stride;
+            if (k >= nv) continue;
+            if (uprev) {
+#pragma unroll
+                for (int c = 0; c < V::N; ++c) V::set(wv[u], c, V::at(wv[u], c) - h * V::at(pv[u], c));
+                reinterpret_cast<VT *>(w)[k] = wv[u];
+            }
+            const VT &uu = unext ? uv[u] : wv[u];
+#pragma unroll
+            for (int c = 0; c < V::N; ++c)
+                acc += static_cast<double>(V::at(uu, c)) * static_cast<double>(V::at(wv[u], c));
+        }
     }
     if (blockIdx.x == 0)
         for (int k = nv * V::N + threadIdx.x; k < n; k += 256) {
@@ -479,7 +496,8 @@ static int norm64_sync(femgpu_ctx *ctx, int n, const double *x, double *d_out, i
 template <typename T>
 static void mgs_step(femgpu_ctx *ctx, const GDev &G, int n, int i, T *wn, const T *V, size_t ldv)
 {
-    const int grid = grid_for(n);
+    const int nv = n / Vec16<T>::N;
+    const int grid = std::max(1, std::min(kReduceBlocks, (nv + 256 * kMgsVec - 1) / (256 * kMgsVec)));
     for (int j = 0; j <= i + 1; ++j) {
         const T *uprev = j ? V + (j - 1) * ldv : nullptr;
         const T *unext = j <= i ? V + j * ldv : nullptr;
