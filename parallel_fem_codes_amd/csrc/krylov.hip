@@ -308,13 +308,22 @@ __global__ __launch_bounds__(256) void k_cycle_start(GDev G, int n, T *__restric
 {
     __shared__ double red[4];
     double acc = 0;
-    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
-        T v = w[k];
-        if (negate) {
-            v = B[k] - v;
-            w[k] = v;
+    const int stride = gridDim.x * 256;
+    for (int k0 = blockIdx.x * 256 + threadIdx.x; k0 < n; k0 += 4 * stride) {   // four loads in flight
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u * stride;
+            v[u] = k < n ? w[k] : T(0);
+            if (negate && k < n) v[u] = B[k] - v[u];
         }
-        acc += static_cast<double>(v) * static_cast<double>(v);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u * stride;
+            if (k >= n) continue;
+            if (negate) w[k] = v[u];
+            acc += static_cast<double>(v[u]) * static_cast<double>(v[u]);
+        }
     }
     acc = block_sum<256>(acc, red);
     if (threadIdx.x == 0) G.partials[blockIdx.x] = acc;
@@ -395,7 +404,14 @@ __global__ __launch_bounds__(256) void k_sumsq(int n, const double *__restrict__
 {
     __shared__ double red[4];
     double acc = 0.0;
-    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) acc += x[k] * x[k];
+    const int stride = gridDim.x * 256;
+    for (int k0 = blockIdx.x * 256 + threadIdx.x; k0 < n; k0 += 4 * stride) {   // four loads in flight
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = k0 + u * stride < n ? x[k0 + u * stride] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += v[u] * v[u];
+    }
     acc = block_sum<256>(acc, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
@@ -476,12 +492,14 @@ __global__ __launch_bounds__(256) void k_cycle_finish_seg(GDev G, int n, const f
 }
 
 static int grid_for(int n) { return std::max(1, std::min(kReduceBlocks, (n + 1023) / 1024)); }
+// reductions whose threads take four elements per step
+static int grid4_for(int n) { return std::max(1, std::min(kReduceBlocks, (n + 4095) / 4096)); }
 
 // ||x|| into d_out and the host-mapped slot of the context (slots 248..255 of the mapped doubles,
 // past the per-step residual words), then waits for the stream and returns it.
 static int norm64_sync(femgpu_ctx *ctx, int n, const double *x, double *d_out, int slot, double &host)
 {
-    const int grid = grid_for(n);
+    const int grid = grid4_for(n);
     double *dmap = reinterpret_cast<double *>(ctx->d_status + 256) + 248 + slot;
     volatile double *hmap = reinterpret_cast<volatile double *>(ctx->h_status + 256) + 248 + slot;
     k_sumsq<<<grid, 256, 0, ctx->stream>>>(n, x, ctx->partials.p);
@@ -509,7 +527,7 @@ static void mgs_step(femgpu_ctx *ctx, const GDev &G, int n, int i, T *wn, const 
 template <typename T>
 static void cycle_start(femgpu_ctx *ctx, const GDev &G, int n, T *w, const T *B, int negate, int first, double tol)
 {
-    const int grid = grid_for(n);
+    const int grid = grid4_for(n);
     k_cycle_start<T><<<grid, 256, 0, ctx->stream>>>(G, n, w, B, negate);
     k_cycle_start_finish<<<1, 256, 0, ctx->stream>>>(G, grid, first, tol);
 }
