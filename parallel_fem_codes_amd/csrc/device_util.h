@@ -40,6 +40,19 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// XCD-aware block order. Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
+// b mod 8), each XCD with its own L2. A launch of 8 * per blocks renumbers workgroup b as
+// (b mod 8) * per + b / 8, so XCD x processes the contiguous range [x * per, (x + 1) * per) of the
+// logical blocks: with a banded matrix its gathers stay mostly inside what its own L2 holds.
+// Returns the logical block; the launch rounds its grid up to a multiple of 8 and blocks whose
+// logical number reaches the needed count exit.
+__device__ __forceinline__ int xcd_block()
+{
+    const int per = gridDim.x / 8;
+    return static_cast<int>(blockIdx.x % 8) * per + static_cast<int>(blockIdx.x / 8);
+}
+inline int xcd_grid(int blocks) { return (blocks + 7) / 8 * 8; }
+
 // sum of vals[src[q]] for q = p0 .. p1-1, added in list order. The first U entries are loaded
 // together (positions past the end clamped to p0, their terms not added), so their loads overlap;
 // the additions are those of the plain loop, in the same order.

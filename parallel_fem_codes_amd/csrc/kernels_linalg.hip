@@ -61,10 +61,11 @@ __global__ __launch_bounds__(256) void k_spmv_sell(int n, const int *__restrict_
 {
     if (skip && *skip) return;
     const float xs = xs_ptr ? static_cast<float>(*xs_ptr) : 1.0f;
+    const int blk = xcd_block();
     float acc[kSellRows];
 #pragma unroll
     for (int r = 0; r < kSellRows; ++r) {
-        const int row = (blockIdx.x * kSellRows + r) * 256 + threadIdx.x;
+        const int row = (blk * kSellRows + r) * 256 + threadIdx.x;
         acc[r] = 0.0f;
         if (row >= n) continue;
         const int s = row >> 6, l = row & 63;
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(256) void k_spmv_sell(int n, const int *__restrict_
     }
 #pragma unroll
     for (int r = 0; r < kSellRows; ++r) {
-        const int row = (blockIdx.x * kSellRows + r) * 256 + threadIdx.x;
+        const int row = (blk * kSellRows + r) * 256 + threadIdx.x;
         if (row >= n) continue;
         y[row] = acc[r];
         if (xout) xout[row] = x[row] * xs;
@@ -189,7 +190,7 @@ int launch_spmv32(femgpu_ctx *ctx, const femgpu_csr *A, const float *x, const do
                   float *y, const int *skip)
 {
     if (A->sell_ok && A->n > 0) {
-        const int grid = (A->n + 256 * kSellRows - 1) / (256 * kSellRows);
+        const int grid = xcd_grid((A->n + 256 * kSellRows - 1) / (256 * kSellRows));
         k_spmv_sell<<<grid, 256, 0, ctx->stream>>>(A->n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p, x, xs, xout, y,
                                                    skip);
         FG_HIP(hipGetLastError());

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "common.h"
+#include "device_util.h"
 
 namespace fg {
 
@@ -356,7 +357,7 @@ __global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__re
     float acc[kJacRows];
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
-        const int i = (blockIdx.x * kJacRows + r) * 256 + threadIdx.x;
+        const int i = (xcd_block() * kJacRows + r) * 256 + threadIdx.x;
         acc[r] = 0.0f;
         if (i >= n) continue;
         const int s = i >> 6, q = sl[s] + (i & 63), w = (sl[s + 1] - sl[s]) >> 6;
@@ -369,7 +370,7 @@ __global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__re
     }
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
-        const int i = (blockIdx.x * kJacRows + r) * 256 + threadIdx.x;
+        const int i = (xcd_block() * kJacRows + r) * 256 + threadIdx.x;
         if (i < n) yout[i] = b[i] - acc[r];
     }
 }
@@ -385,7 +386,7 @@ __global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, const int *__re
     float acc[kJacRows];
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
-        const int i = (blockIdx.x * kJacRows + r) * 256 + threadIdx.x;
+        const int i = (xcd_block() * kJacRows + r) * 256 + threadIdx.x;
         acc[r] = 0.0f;
         if (i >= n) continue;
         const int s = i >> 6, q = su[s] + (i & 63), w = (su[s + 1] - su[s]) >> 6;
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, const int *__re
     }
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
-        const int i = (blockIdx.x * kJacRows + r) * 256 + threadIdx.x;
+        const int i = (xcd_block() * kJacRows + r) * 256 + threadIdx.x;
         if (i < n) zout[i] = Dinv[i] * (y[i] - acc[r]);
     }
 }
@@ -534,7 +535,7 @@ int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A)
 template <int K>
 static void jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip)
 {
-    const int n = A->n, grid = (n + 256 * kJacRows - 1) / (256 * kJacRows);
+    const int n = A->n, grid = xcd_grid((n + 256 * kJacRows - 1) / (256 * kJacRows));
     float *t[2] = {A->jac_a.p, A->jac_b.p};
     const float *prev = nullptr;
     for (int k = 0; k < it; ++k) {
@@ -558,7 +559,7 @@ int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const f
 {
     const int n = A->n;
     if (n == 0) return FEMGPU_OK;
-    const int grid = (n + 256 * kJacRows - 1) / (256 * kJacRows);
+    const int grid = xcd_grid((n + 256 * kJacRows - 1) / (256 * kJacRows));
     if (A->ell_kl > 0) {
         const bool k4 = std::max(A->ell_kl, A->ell_ku) <= 4;
         if (upper && k4)
