@@ -218,7 +218,8 @@ __global__ __launch_bounds__(256) void k_elem_transient(int nel, ElemIn in, doub
 #pragma clang fp contract(off)
     __shared__ double2 stage[256 * 6];   // 256 records of 12 doubles, or 256 Ae of 9
     double *sd = reinterpret_cast<double *>(stage);
-    const int e0 = blockIdx.x * 256, e = e0 + static_cast<int>(threadIdx.x);
+    const int e0 = xcd_block() * 256, e = e0 + static_cast<int>(threadIdx.x);
+    if (e0 >= nel) return;   // the whole block: padding of the XCD-ordered grid
     const int nb = min(256, nel - e0);
     TransientElem T;
     if (e < nel) {
@@ -304,7 +305,7 @@ __global__ void k_rhs_transient(int NEQ, const int *__restrict__ dst, const int 
                                 const int *__restrict__ src, const double *__restrict__ rec, double *__restrict__ R)
 {
 #pragma clang fp contract(off)
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = xcd_block() * blockDim.x + threadIdx.x;
     if (k > NEQ) return;
     if (k == NEQ) {
         R[NEQ] = 0.0;
@@ -347,7 +348,7 @@ __global__ void k_aa_gather(int nnz, const int *__restrict__ dst, const int *__r
                             const int *__restrict__ src, const double *__restrict__ Ke, double *__restrict__ AA)
 {
 #pragma clang fp contract(off)
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = xcd_block() * blockDim.x + threadIdx.x;
     if (k > nnz) return;
     if (k == nnz) {
         AA[nnz] = 0.0;
@@ -507,7 +508,7 @@ extern "C" int femgpu_assemble_transport(femgpu_mesh *M, const femgpu_transport_
         FG_HIP(hipGetLastError());
     }
     if (d_AA) {
-        k_aa_gather<<<(M->nnz + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->nnz, M->aa_dst.p, M->aa_ptr.p, M->aa_src.p, Ke,
+        k_aa_gather<<<xcd_grid((M->nnz + 1 + 255) / 256), 256, 0, ctx->stream>>>(M->nnz, M->aa_dst.p, M->aa_ptr.p, M->aa_src.p, Ke,
                                                                         d_AA);
         FG_HIP(hipGetLastError());
     }
@@ -524,15 +525,15 @@ extern "C" int femgpu_assemble_transport_transient(femgpu_mesh *M, const femgpu_
     double *Ae = d_Ae ? d_Ae : M->ke.p;
     const ElemIn in = elem_in(M, pb, d_u, d_du, alpha * dt);
     if (M->nel > 0) {
-        k_elem_transient<<<(M->nel + 255) / 256, 256, 0, ctx->stream>>>(M->nel, in, Ae, M->lift.p);
+        k_elem_transient<<<xcd_grid((M->nel + 255) / 256), 256, 0, ctx->stream>>>(M->nel, in, Ae, M->lift.p);
         FG_HIP(hipGetLastError());
     }
-    k_rhs_transient<<<(M->neq + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->neq, M->rhs_dst.p, M->rhs_ptr.p,
+    k_rhs_transient<<<xcd_grid((M->neq + 1 + 255) / 256), 256, 0, ctx->stream>>>(M->neq, M->rhs_dst.p, M->rhs_ptr.p,
                                                                        M->rhs_src.p, M->lift.p, d_R);
     FG_HIP(hipGetLastError());
     if (d_AA) {
         FG_CHECK(M->aa_ptr.p, FEMGPU_ERR_STATE, "femgpu_assemble_transport_transient: no CSR slot table");
-        k_aa_gather<<<(M->nnz + 1 + 255) / 256, 256, 0, ctx->stream>>>(M->nnz, M->aa_dst.p, M->aa_ptr.p, M->aa_src.p, Ae,
+        k_aa_gather<<<xcd_grid((M->nnz + 1 + 255) / 256), 256, 0, ctx->stream>>>(M->nnz, M->aa_dst.p, M->aa_ptr.p, M->aa_src.p, Ae,
                                                                         d_AA);
         FG_HIP(hipGetLastError());
     }

@@ -25,7 +25,7 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ IA,
                                               T *__restrict__ y, const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    const int gt = blockIdx.x * 256 + threadIdx.x;
+    const int gt = xcd_block() * 256 + threadIdx.x;
     const int row = gt / LPR, lane = gt % LPR;
     if (row >= n) return;
     const T xs = xs_ptr ? static_cast<T>(*xs_ptr) : T(1);
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_residual(int n, const int *__restrict__
                                                   const double *__restrict__ AA, const double *__restrict__ x,
                                                   const double *__restrict__ f, double *__restrict__ r)
 {
-    const int gt = blockIdx.x * 256 + threadIdx.x;
+    const int gt = xcd_block() * 256 + threadIdx.x;
     const int row = gt / LPR, lane = gt % LPR;
     if (row >= n) return;
     const int p0 = IA[row], p1 = IA[row + 1];
@@ -167,7 +167,7 @@ static int spmv_dispatch(femgpu_ctx *ctx, const femgpu_csr *A, const T *AA, cons
 {
     const int L = A->spmv_lanes;
     const long threads = static_cast<long>(A->n) * L;
-    const int grid = static_cast<int>((threads + 255) / 256);
+    const int grid = xcd_grid(static_cast<int>((threads + 255) / 256));
     if (A->n == 0) return FEMGPU_OK;
     switch (L) {
     case 1: k_spmv<T, 1><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, AA, x, xs, xout, y, skip); break;
@@ -203,7 +203,7 @@ int launch_residual64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, con
 {
     if (A->n == 0) return FEMGPU_OK;
     const int L = A->spmv_lanes;
-    const int grid = static_cast<int>((static_cast<long>(A->n) * L + 255) / 256);
+    const int grid = xcd_grid(static_cast<int>((static_cast<long>(A->n) * L + 255) / 256));
     switch (L) {
     case 1: k_residual<1><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, A->AA.p, x, f, r); break;
     case 2: k_residual<2><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, A->AA.p, x, f, r); break;

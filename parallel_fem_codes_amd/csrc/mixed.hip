@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void k_parilu_sweep_sell(int n, const int *__r
                                                            const float *__restrict__ LUo, const float *__restrict__ Dio,
                                                            float *__restrict__ LUn, float *__restrict__ Din)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = xcd_block() * 256 + threadIdx.x;
     if (i >= n) return;
     const int s = i >> 6, l = i & 63, p0 = sptr[s] + l, w = (sptr[s + 1] - sptr[s]) >> 6;
     const int u0 = uptr[s] + l, nu = (uptr[s + 1] - uptr[s]) >> 6;
@@ -465,7 +465,7 @@ static int refresh_ell(femgpu_ctx *ctx, femgpu_csr *A)
 
 static int parilu_sell(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
 {
-    const int n = A->n, grid = (n + 255) / 256;
+    const int n = A->n, grid = xcd_grid((n + 255) / 256);
     // the final iterate must land in par_a / Dinv32
     float *L0 = (sweeps % 2 == 0) ? A->par_a.p : A->par_b.p, *L1 = (sweeps % 2 == 0) ? A->par_b.p : A->par_a.p;
     float *D0 = (sweeps % 2 == 0) ? A->Dinv32.p : A->Dinv32b.p, *D1 = (sweeps % 2 == 0) ? A->Dinv32b.p : A->Dinv32.p;
