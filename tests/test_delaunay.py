@@ -99,3 +99,42 @@ def test_duplicates_skipped_and_bad_input_rejected():
         fem.delaunay(np.zeros(2), np.zeros(2))
     with pytest.raises(Exception):
         fem.delaunay(np.array([0.0, 1.0, np.nan]), np.array([0.0, 0.0, 1.0]))
+
+
+def test_degenerate_point_sets_cover_their_hull_once():
+    """Seeded fuzz over degenerate inputs (integer grids with repeats, tight clusters, nearly
+    collinear runs, cocircular points, boundary-only squares, badly scaled clouds): every call
+    returns counter-clockwise triangles (slivers of nearly collinear points may round to zero area)
+    covering the convex hull exactly once, and counts the repeated points."""
+    rng = np.random.default_rng(123)
+    for trial in range(120):
+        kind, n = trial % 6, int(rng.integers(3, 300))
+        if kind == 0:
+            x, y = rng.integers(0, 8, n).astype(float), rng.integers(0, 8, n).astype(float)
+        elif kind == 1:
+            x, y = rng.random(n) * 1e-6 + 1.0, rng.random(n) * 1e-6 - 2.0
+        elif kind == 2:
+            t = rng.random(n // 2)
+            x, y = np.r_[t, rng.random(n - n // 2)], np.r_[2 * t + 1, rng.random(n - n // 2)]
+        elif kind == 3:
+            a = rng.random(n) * 2 * np.pi
+            x, y = np.r_[np.cos(a), 0.0], np.r_[np.sin(a), 0.0]
+        elif kind == 4:
+            m = max(4, n // 4)
+            t = np.linspace(0, 1, m + 1)[:-1]
+            x = np.r_[t, np.ones(m), 1 - t, np.zeros(m), rng.random(3)]
+            y = np.r_[np.zeros(m), t, np.ones(m), 1 - t, rng.random(3)]
+        else:
+            x, y = rng.standard_normal(n) * 1e3, rng.standard_normal(n) * 1e-3
+        P = np.stack([x, y], 1)
+        uniq = np.unique(P, axis=0)
+        if len(uniq) < 3 or np.linalg.matrix_rank(uniq - uniq[0]) < 2:
+            with pytest.raises(Exception):
+                fem.delaunay(x, y)
+            continue
+        tri, dup = fem.delaunay(x, y)
+        ar = areas(x, y, tri)
+        hull = ConvexHull(uniq).volume
+        assert np.all(ar >= 0) and (kind == 2 or np.all(ar > 0)), (trial, kind)
+        assert abs(ar.sum() - hull) <= 1e-9 * max(1.0, hull), (trial, kind)
+        assert dup == len(P) - len(uniq), (trial, kind)
