@@ -325,7 +325,7 @@ static int ensure_par(femgpu_csr *A)
 // i at sl[i / 64] + 64 k + i % 64): one thread per row, every load of a step of the row loop one
 // contiguous 256-byte run, a loop count uniform across the wavefront, and padding only where a
 // slice's rows differ in length. Padding entries have value 0 and column i.
-constexpr int kEllMax = 8;
+constexpr int kEllMax = 16;   // widest strict part kept in sliced ELL (kernels unrolled to 4, 8 or 16)
 
 __global__ void k_jacobi_ell_values(int n, const int *__restrict__ IA, const int *__restrict__ diag,
                                     const float *__restrict__ LU, const int *__restrict__ sl,
@@ -561,19 +561,28 @@ int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const f
     if (n == 0) return FEMGPU_OK;
     const int grid = xcd_grid((n + 256 * kJacRows - 1) / (256 * kJacRows));
     if (A->ell_kl > 0) {
-        const bool k4 = std::max(A->ell_kl, A->ell_ku) <= 4;
-        if (upper && k4)
-            k_jacobi_upper_ell<4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p, A->Dinv32.p,
-                                                                 rhs, in, out, nullptr);
-        else if (upper)
-            k_jacobi_upper_ell<kEllMax><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
-                                                                       A->Dinv32.p, rhs, in, out, nullptr);
-        else if (k4)
-            k_jacobi_lower_ell<4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs, in, out,
-                                                                 nullptr);
-        else
-            k_jacobi_lower_ell<kEllMax><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs, in,
-                                                                       out, nullptr);
+        const int w = std::max(A->ell_kl, A->ell_ku);
+        if (upper) {
+            if (w <= 4)
+                k_jacobi_upper_ell<4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
+                                                                     A->Dinv32.p, rhs, in, out, nullptr);
+            else if (w <= 8)
+                k_jacobi_upper_ell<8><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
+                                                                     A->Dinv32.p, rhs, in, out, nullptr);
+            else
+                k_jacobi_upper_ell<16><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
+                                                                      A->Dinv32.p, rhs, in, out, nullptr);
+        } else {
+            if (w <= 4)
+                k_jacobi_lower_ell<4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs, in,
+                                                                     out, nullptr);
+            else if (w <= 8)
+                k_jacobi_lower_ell<8><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs, in,
+                                                                     out, nullptr);
+            else
+                k_jacobi_lower_ell<16><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs,
+                                                                      in, out, nullptr);
+        }
     } else {
         constexpr int LPR = 4;
         const int g4 = static_cast<int>((static_cast<long>(n) * LPR + 255) / 256);
@@ -594,8 +603,10 @@ int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b
     if (n == 0) return FEMGPU_OK;
     FG_CHECK(it >= 1 && it <= 1024, FEMGPU_ERR_ARG, "Jacobi sweeps: %d iterations out of range", it);
     if (A->ell_kl > 0) {
-        if (std::max(A->ell_kl, A->ell_ku) <= 4) jacobi_ell<4>(ctx, A, it, b, x, skip);
-        else jacobi_ell<kEllMax>(ctx, A, it, b, x, skip);
+        const int w = std::max(A->ell_kl, A->ell_ku);
+        if (w <= 4) jacobi_ell<4>(ctx, A, it, b, x, skip);
+        else if (w <= 8) jacobi_ell<8>(ctx, A, it, b, x, skip);
+        else jacobi_ell<16>(ctx, A, it, b, x, skip);
         FG_HIP(hipGetLastError());
         return FEMGPU_OK;
     }

@@ -67,10 +67,12 @@ def make_requests(count, seed, nmin, nmax, rate, kmin=1e-5, kmax=1e-2):
 
 def server_opts(hbm_budget=64 << 30, executors=4, prep_threads=8, small_rows=20000, batch_rows=400000,
                 max_batch=0, kmax=30, tol=1e-8, sweep_mode=2, factor_sweeps=2, inner_tol=1e-4, max_refinements=50,
-                orthogonalization="MGS", small_solver="shared", wg_threads=0):
+                orthogonalization="CGS", small_solver="shared", wg_threads=0):
     """small_solver for batches of small requests: "shared" (block-diagonal shared-Krylov solver),
     "persistent" (one launch, a workgroup per system) or "auto" (persistent when every system has at
-    most 3072 rows); wg_threads 0 = tuned online."""
+    most 3072 rows); wg_threads 0 = tuned online. Classical Gram-Schmidt by default: the serving
+    systems' inner cycles run long (up to kmax - 1 steps), where MGS costs i + 3 launches per step
+    and CGS four."""
     o = N.ServerOpts()
     o.hbm_budget, o.executors, o.prep_threads = int(hbm_budget), executors, prep_threads
     o.small_rows, o.batch_rows, o.max_batch, o.kmax, o.tol = small_rows, batch_rows, max_batch, kmax, tol
@@ -230,6 +232,7 @@ def main():
     ap.add_argument("--executors", type=int, default=4)
     ap.add_argument("--max-batch", type=int, default=0)
     ap.add_argument("--small-solver", choices=("shared", "auto", "persistent"), default="shared")
+    ap.add_argument("--orth", choices=("MGS", "CGS"), default=None, help="inner Gram-Schmidt (default: server_opts')")
     ap.add_argument("--wg-threads", type=int, default=0)
     ap.add_argument("--isolated", action="store_true", help="also time every request alone on an idle server")
     ap.add_argument("--warmup", type=int, default=8, help="requests solved before the timed stream")
@@ -240,8 +243,9 @@ def main():
     t0 = time.perf_counter()
     reqs = make_requests(a.requests, a.seed, a.nmin, a.nmax, a.rate)
     print(f"# {len(reqs)} requests generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    extra = {"orthogonalization": a.orth} if a.orth else {}
     opts = server_opts(executors=a.executors, max_batch=a.max_batch, small_solver=a.small_solver,
-                       wg_threads=a.wg_threads)
+                       wg_threads=a.wg_threads, **extra)
     sols, infos, wall, batch, wg = run_server(reqs, opts, warmup=a.warmup)
     lat = np.array([i.t_end - i.t_submit for i in infos])
     res = np.array([i.true_rel_residual for i in infos])
@@ -257,7 +261,7 @@ def main():
                       "mean_queue_ms": 1e3 * float(np.mean([i.t_start - i.t_submit - i.prep_s for i in infos]))}}
     print(f"# server: {out['server']}", file=sys.stderr, flush=True)
     if a.isolated:
-        iso = run_isolated(reqs, server_opts(small_solver=a.small_solver, wg_threads=a.wg_threads))
+        iso = run_isolated(reqs, server_opts(small_solver=a.small_solver, wg_threads=a.wg_threads, **extra))
         out["isolated"] = {"p50_ms": 1e3 * float(np.percentile(iso, 50)), "p99_ms": 1e3 * float(np.percentile(iso, 99))}
         out["p99_loaded_over_isolated"] = float(np.percentile(lat, 99) / np.percentile(iso, 99))
     if not a.no_fp64:
