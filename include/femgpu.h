@@ -115,6 +115,11 @@ int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA, const int 
 #define FEMGPU_PLANS_MIXED 2
 int femgpu_csr_create_ex(femgpu_ctx *ctx, int n, int nnz, const int *IA, const int *JA, int plans,
                          femgpu_csr **out);
+/* Re-plans A for a new structure (same plan flags), reusing its device buffers where they are large
+ * enough: for callers that solve a stream of different systems (the request loop) without a
+ * hipMalloc/hipFree per system. Values, factors and captured graphs of the old structure are void.
+ * No reference counterpart. */
+int femgpu_csr_rebuild(femgpu_csr *A, int n, int nnz, const int *IA, const int *JA);
 int femgpu_csr_destroy(femgpu_csr *A);
 /* AA: nnz values (the reference's dummy slot AA[nnz] is not needed); on_device selects whether
  * AA is a host or a device pointer. Also refreshes the FP32 copy. */
@@ -334,6 +339,11 @@ int femgpu_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, const doub
                        const double *y, const int *type, const int *invP_id, const double *ubc,
                        const int *ien, const int *lm, const int *scheme, int nnz,
                        femgpu_mesh **out);
+/* As femgpu_mesh_create, into an existing transport mesh (its device buffers reused where large
+ * enough). No reference counterpart. */
+int femgpu_mesh_rebuild(femgpu_mesh *M, int nnodes, int nel, int neq, const double *x, const double *y,
+                        const int *type, const int *invP_id, const double *ubc, const int *ien, const int *lm,
+                        const int *scheme, int nnz);
 int femgpu_mesh_destroy(femgpu_mesh *M);
 
 /* Build_K_F_SUPG (01_SS_Transport_Equation/2D/01_CommonFiles/Build_K_F_SUPG.c:4-187 with

@@ -163,16 +163,14 @@ extern "C" int femgpu_csr_create(femgpu_ctx *ctx, int n, int nnz, const int *IA,
     return femgpu_csr_create_ex(ctx, n, nnz, IA, JA, FEMGPU_PLANS_FP64 | FEMGPU_PLANS_MIXED, out);
 }
 
-extern "C" int femgpu_csr_create_ex(femgpu_ctx *ctx, int n, int nnz, const int *IA, const int *JA, int plans,
-                                    femgpu_csr **out)
+// The structure's checks and the position of every diagonal entry.
+static int csr_check(int n, int nnz, const int *IA, const int *JA, std::vector<int> &diag)
 {
-    FG_CHECK(ctx && out && n >= 0 && nnz >= 0 && IA && JA && (plans & ~3) == 0, FEMGPU_ERR_ARG,
-             "femgpu_csr_create: bad arguments");
+    FG_CHECK(n >= 0 && nnz >= 0 && IA && JA, FEMGPU_ERR_ARG, "femgpu_csr_create: bad arguments");
     FG_CHECK(IA[0] == 0 && IA[n] == nnz, FEMGPU_ERR_ARG, "femgpu_csr_create: IA[0]=%d IA[n]=%d nnz=%d", IA[0], IA[n], nnz);
-    std::vector<int> diag(n);
+    diag.assign(static_cast<size_t>(n), -1);
     for (int i = 0; i < n; ++i) {
         FG_CHECK(IA[i] <= IA[i + 1], FEMGPU_ERR_ARG, "femgpu_csr_create: IA not monotone at row %d", i);
-        diag[i] = -1;
         for (int p = IA[i]; p < IA[i + 1]; ++p) {
             FG_CHECK(JA[p] >= 0 && JA[p] < n, FEMGPU_ERR_ARG, "femgpu_csr_create: column %d out of range", JA[p]);
             FG_CHECK(p == IA[i] || JA[p] > JA[p - 1], FEMGPU_ERR_ARG, "femgpu_csr_create: row %d not sorted", i);
@@ -180,10 +178,20 @@ extern "C" int femgpu_csr_create_ex(femgpu_ctx *ctx, int n, int nnz, const int *
         }
         FG_CHECK(diag[i] >= 0, FEMGPU_ERR_ARG, "femgpu_csr_create: row %d has no diagonal entry", i);
     }
+    return FEMGPU_OK;
+}
+
+// Structure, plans and value buffers of A for (n, nnz, IA, JA); device buffers that are large
+// enough are reused. A new serial number: anything captured for the previous structure is stale.
+static int csr_setup(femgpu_csr *A, int n, int nnz, const int *IA, const int *JA, const std::vector<int> &diag)
+{
     static std::atomic<unsigned long long> next_serial{1};
-    auto *A = new femgpu_csr();
+    femgpu_ctx *ctx = A->ctx;
+    const int plans = A->plans;
     A->serial = next_serial++;
-    A->ctx = ctx;
+    A->factored = false;
+    A->f32_factored = false;
+    A->lu32_in_sell = false;
     A->n = n;
     A->nnz = nnz;
     A->hIA.assign(IA, IA + n + 1);
@@ -196,7 +204,6 @@ extern "C" int femgpu_csr_create_ex(femgpu_ctx *ctx, int n, int nnz, const int *
     if (!rc) rc = A->AA32.alloc(static_cast<size_t>(nnz) + 1);
     if (!rc) rc = A->LU.alloc(static_cast<size_t>(nnz) + 1);
     if (!rc) rc = A->Dinv.alloc(static_cast<size_t>(n) + 1);
-    A->plans = plans;
     if (plans & FEMGPU_PLANS_FP64) {   // exact factorisation and level-scheduled sweeps
         if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), false, A->plo);
         if (!rc) rc = build_packed_plan(ctx, n, IA, JA, diag.data(), true, A->pup);
@@ -214,12 +221,33 @@ extern "C" int femgpu_csr_create_ex(femgpu_ctx *ctx, int n, int nnz, const int *
     }
     if (!rc && hipMemsetAsync(A->AA.p, 0, sizeof(double) * (nnz + 1), s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
+    return rc;
+}
+
+extern "C" int femgpu_csr_create_ex(femgpu_ctx *ctx, int n, int nnz, const int *IA, const int *JA, int plans,
+                                    femgpu_csr **out)
+{
+    FG_CHECK(ctx && out && (plans & ~3) == 0, FEMGPU_ERR_ARG, "femgpu_csr_create: bad arguments");
+    std::vector<int> diag;
+    FG_TRY(csr_check(n, nnz, IA, JA, diag));
+    auto *A = new femgpu_csr();
+    A->ctx = ctx;
+    A->plans = plans;
+    const int rc = csr_setup(A, n, nnz, IA, JA, diag);
     if (rc) {
         delete A;
         return rc;
     }
     *out = A;
     return FEMGPU_OK;
+}
+
+extern "C" int femgpu_csr_rebuild(femgpu_csr *A, int n, int nnz, const int *IA, const int *JA)
+{
+    FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_csr_rebuild: null");
+    std::vector<int> diag;
+    FG_TRY(csr_check(n, nnz, IA, JA, diag));
+    return csr_setup(A, n, nnz, IA, JA, diag);
 }
 
 extern "C" int femgpu_csr_destroy(femgpu_csr *A)

@@ -405,19 +405,19 @@ void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vect
 
 using namespace fg;
 
-extern "C" int femgpu_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, const double *x, const double *y,
-                                  const int *type, const int *invP_id, const double *ubc, const int *ien,
-                                  const int *lm, const int *scheme, int nnz, femgpu_mesh **out)
+// A transport mesh's device data and gather lists in M (device buffers that are large enough are
+// reused).
+static int mesh_setup(femgpu_mesh *M, int nnodes, int nel, int neq, const double *x, const double *y, const int *type,
+                      const int *invP_id, const double *ubc, const int *ien, const int *lm, const int *scheme, int nnz)
 {
-    FG_CHECK(ctx && nnodes > 0 && nel >= 0 && neq >= 0 && x && y && type && invP_id && ubc && ien && out,
-             FEMGPU_ERR_ARG, "femgpu_mesh_create: bad arguments");
+    FG_CHECK(nnodes > 0 && nel >= 0 && neq >= 0 && x && y && type && invP_id && ubc && ien, FEMGPU_ERR_ARG,
+             "femgpu_mesh_create: bad arguments");
     for (int v = 0; v < nnodes; ++v)
         FG_CHECK(type[v] == 0 || type[v] == 1, FEMGPU_ERR_ARG, "femgpu_mesh_create: node %d has Type %d (one rank: 0 or 1)",
                  v, type[v]);
     for (size_t q = 0; q < static_cast<size_t>(nel) * 3; ++q)
         FG_CHECK(ien[q] >= 0 && ien[q] < nnodes, FEMGPU_ERR_ARG, "femgpu_mesh_create: vertex %d out of range", ien[q]);
-    auto *M = new femgpu_mesh();
-    M->ctx = ctx;
+    femgpu_ctx *ctx = M->ctx;
     M->nnodes = nnodes;
     M->nel = nel;
     M->neq = neq;
@@ -474,12 +474,31 @@ extern "C" int femgpu_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq,
     if (rc == FEMGPU_OK) rc = M->mask.alloc(static_cast<size_t>(nel));
     if (rc == FEMGPU_OK) rc = M->ke.alloc(static_cast<size_t>(nel) * 9);
     if (rc == FEMGPU_OK && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
+    return rc;
+}
+
+extern "C" int femgpu_mesh_create(femgpu_ctx *ctx, int nnodes, int nel, int neq, const double *x, const double *y,
+                                  const int *type, const int *invP_id, const double *ubc, const int *ien,
+                                  const int *lm, const int *scheme, int nnz, femgpu_mesh **out)
+{
+    FG_CHECK(ctx && out, FEMGPU_ERR_ARG, "femgpu_mesh_create: bad arguments");
+    auto *M = new femgpu_mesh();
+    M->ctx = ctx;
+    const int rc = mesh_setup(M, nnodes, nel, neq, x, y, type, invP_id, ubc, ien, lm, scheme, nnz);
     if (rc != FEMGPU_OK) {
         delete M;
         return rc;
     }
     *out = M;
     return FEMGPU_OK;
+}
+
+extern "C" int femgpu_mesh_rebuild(femgpu_mesh *M, int nnodes, int nel, int neq, const double *x, const double *y,
+                                   const int *type, const int *invP_id, const double *ubc, const int *ien,
+                                   const int *lm, const int *scheme, int nnz)
+{
+    FG_CHECK(M && M->ndof == 1, FEMGPU_ERR_ARG, "femgpu_mesh_rebuild: needs a transport mesh");
+    return mesh_setup(M, nnodes, nel, neq, x, y, type, invP_id, ubc, ien, lm, scheme, nnz);
 }
 
 extern "C" int femgpu_mesh_destroy(femgpu_mesh *M)

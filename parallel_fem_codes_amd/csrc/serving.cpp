@@ -207,8 +207,23 @@ int wg_threads(femgpu_server *S, int &cand)
 }
 
 // Solve one unit (one large request or a batch of small ones) on the executor's context.
-void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long long batch_id, int threads)
+// What an executor keeps across units: one matrix, a pool of meshes and the right-hand side and
+// solution vectors, re-planned or grown for each unit instead of allocated and freed.
+struct Executor {
+    femgpu_ctx *ctx = nullptr;
+    femgpu_csr *A = nullptr;
+    std::vector<femgpu_mesh *> meshes;
+    fg::DevBuf<double> F, X;
+    ~Executor()
+    {
+        for (auto *M : meshes) femgpu_mesh_destroy(M);
+        if (A) femgpu_csr_destroy(A);
+    }
+};
+
+void run_unit(femgpu_server *S, Executor &E, std::vector<Req> &unit, long long batch_id, int threads)
 {
+    femgpu_ctx *ctx = E.ctx;
     const int K = static_cast<int>(unit.size());
     std::vector<int> row_off(K + 1, 0);
     std::vector<long long> nnz_off(K + 1, 0);
@@ -229,7 +244,6 @@ void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long lo
     std::string err;
     femgpu_csr *A = nullptr;
     double *dF = nullptr, *dX = nullptr;
-    std::vector<femgpu_mesh *> meshes(K, nullptr);
     std::vector<femgpu_batch_info> binfo(K);
     const double t_start = S->now();
     double t_setup = t_start;
@@ -240,25 +254,34 @@ void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long lo
         }
     };
     if (N > 0) {
-        if (femgpu_csr_create_ex(ctx, N, NNZ, IA.data(), JA.data(), FEMGPU_PLANS_MIXED, &A) != FEMGPU_OK)
-            fail(FEMGPU_ERR_ARG);
-        if (rc == FEMGPU_OK && (hipMalloc((void **)&dF, sizeof(double) * (N + 64)) != hipSuccess ||
-                                hipMalloc((void **)&dX, sizeof(double) * (N + 64)) != hipSuccess)) {
-            fg::set_error("serving: device allocation failed");
+        const int crc = E.A ? femgpu_csr_rebuild(E.A, N, NNZ, IA.data(), JA.data())
+                            : femgpu_csr_create_ex(ctx, N, NNZ, IA.data(), JA.data(), FEMGPU_PLANS_MIXED, &E.A);
+        if (crc != FEMGPU_OK) fail(FEMGPU_ERR_ARG);
+        A = E.A;
+        if (rc == FEMGPU_OK && (E.F.alloc(static_cast<size_t>(N) + 64) != FEMGPU_OK ||
+                                E.X.alloc(static_cast<size_t>(N) + 64) != FEMGPU_OK))
             fail(FEMGPU_ERR_NOMEM);
-        }
+        dF = E.F.p;
+        dX = E.X.p;
         double *AA = nullptr;
         if (rc == FEMGPU_OK && femgpu_csr_values_ptr(A, &AA) != FEMGPU_OK) fail(FEMGPU_ERR_ARG);
         if (rc == FEMGPU_OK && hipMemsetAsync(dX, 0, sizeof(double) * N, ctx->stream) != hipSuccess) fail(FEMGPU_ERR_HIP);
         // one assembly per request into its slice of the shared values (ascending order: a slice's
         // dummy slot is the next slice's first entry until that slice is assembled)
+        if (static_cast<int>(E.meshes.size()) < K) E.meshes.resize(K, nullptr);
         for (int s = 0; s < K && rc == FEMGPU_OK; ++s) {
             const Request &r = *unit[s];
             if (r.n == 0) continue;
-            if (femgpu_mesh_create(ctx, r.nnodes, r.nel, r.n, r.x.data(), r.y.data(), r.type.data(), r.invP.data(),
-                                   r.ubc.data(), r.ien.data(), nullptr, r.scheme.data(), r.nnz, &meshes[s]) != FEMGPU_OK ||
-                femgpu_assemble_transport(meshes[s], &r.pb, dX + row_off[s], nullptr, dF + row_off[s],
-                                          AA + nnz_off[s]) != FEMGPU_OK)
+            femgpu_mesh *&M = E.meshes[s];
+            const int mrc = M ? femgpu_mesh_rebuild(M, r.nnodes, r.nel, r.n, r.x.data(), r.y.data(), r.type.data(),
+                                                    r.invP.data(), r.ubc.data(), r.ien.data(), nullptr, r.scheme.data(),
+                                                    r.nnz)
+                              : femgpu_mesh_create(ctx, r.nnodes, r.nel, r.n, r.x.data(), r.y.data(), r.type.data(),
+                                                   r.invP.data(), r.ubc.data(), r.ien.data(), nullptr, r.scheme.data(),
+                                                   r.nnz, &M);
+            if (mrc != FEMGPU_OK ||
+                femgpu_assemble_transport(M, &r.pb, dX + row_off[s], nullptr, dF + row_off[s], AA + nnz_off[s]) !=
+                    FEMGPU_OK)
                 fail(FEMGPU_ERR_ARG);
         }
         if (rc == FEMGPU_OK && femgpu_csr_values_changed(A) != FEMGPU_OK) fail(FEMGPU_ERR_ARG);
@@ -282,11 +305,6 @@ void run_unit(femgpu_server *S, femgpu_ctx *ctx, std::vector<Req> &unit, long lo
         (hipMemcpyAsync(X.data(), dX, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
          hipStreamSynchronize(ctx->stream) != hipSuccess))
         fail(FEMGPU_ERR_HIP);
-    for (auto *M : meshes)
-        if (M) femgpu_mesh_destroy(M);
-    if (A) femgpu_csr_destroy(A);
-    if (dF) (void)hipFree(dF);
-    if (dX) (void)hipFree(dX);
     const double t_end = S->now();
     // results
     for (int s = 0; s < K; ++s) {
@@ -321,6 +339,7 @@ void exec_loop(femgpu_server *S)
     if (hipSetDevice(S->device) != hipSuccess) return;
     hipStream_t stream = nullptr;
     femgpu_ctx *ctx = nullptr;
+    auto E = std::make_unique<Executor>();
     if (hipStreamCreate(&stream) != hipSuccess || femgpu_create(S->device, stream, &ctx) != FEMGPU_OK) {
         std::lock_guard<std::mutex> lk(S->mu);
         S->stop = true;
@@ -387,7 +406,8 @@ void exec_loop(femgpu_server *S)
             }
         }
         if (!good.empty()) {
-            run_unit(S, ctx, good, batch_id, threads);
+            E->ctx = ctx;
+            run_unit(S, *E, good, batch_id, threads);
             double rows = 0;
             for (auto &r : good) rows += r->n;
             const double dt = good[0]->info.t_end - good[0]->info.t_start;
@@ -398,6 +418,7 @@ void exec_loop(femgpu_server *S)
         S->cv_done.notify_all();
         S->cv_exec.notify_all();
     }
+    E.reset();   // the executor's matrix and meshes before their context
     femgpu_destroy(ctx);
     (void)hipStreamDestroy(stream);
 }

@@ -134,6 +134,7 @@ _SIGS = {
     "femgpu_sync": (I, [vp]),
     "femgpu_csr_create": (I, [vp, I, I, i32p, i32p, C.POINTER(vp)]),
     "femgpu_csr_create_ex": (I, [vp, I, I, i32p, i32p, I, vp]),
+    "femgpu_csr_rebuild": (I, [vp, I, I, i32p, i32p]),
     "femgpu_csr_destroy": (I, [vp]),
     "femgpu_csr_set_values": (I, [vp, vp, I]),
     "femgpu_csr_values_ptr": (I, [vp, C.POINTER(vp)]),
@@ -197,6 +198,7 @@ _SIGS = {
     "femgpu_mesh_create": (I, [vp, I, I, I, f64p, f64p, i32p, i32p, f64p, i32p, vp, vp, I,
                               C.POINTER(vp)]),
     "femgpu_mesh_destroy": (I, [vp]),
+    "femgpu_mesh_rebuild": (I, [vp, I, I, I, f64p, f64p, i32p, i32p, f64p, i32p, vp, vp, I]),
     "femgpu_ns_mesh_create": (I, [vp, I, I, I, f64p, f64p, i32p, i32p, f64p, i32p, i32p, vp, I,
                                   C.POINTER(vp)]),
     "femgpu_assemble_ns": (I, [vp, D, vp, vp, vp, vp, vp, vp]),
