@@ -270,6 +270,11 @@ class CsrMatrix:
     def values_changed(self):
         N.check(N.lib().femgpu_csr_values_changed(self.h))
 
+    def rebuild(self, IA, JA):
+        """Re-plans the matrix for a new structure, reusing its device buffers where large enough."""
+        self.n, self.nnz = len(IA) - 1, len(JA)
+        N.check(N.lib().femgpu_csr_rebuild(self.h, self.n, self.nnz, _i32(IA), _i32(JA)))
+
     def autotune(self):
         """Times the FP64 CSR kernels at 1, 2, 4, 8 and 16 lanes per row and keeps the fastest;
         returns (lanes chosen, {lanes: microseconds per launch})."""
