@@ -372,6 +372,15 @@ def main():
                  "iterations": fp["iterations"], "cycles": fp["cycles"], "true_rel_residual": true_rel_fp64,
                  "sweep": A.sweep_info()},
         "speedup_mixed_vs_fp64": round(mx["solves_per_s"] / fp["solves_per_s"], 3),
+        "throughput": {
+            "matvecs_per_s_f64": round(1e6 / kernels["csrmv_f64"]["us"], 1),
+            "matvecs_per_s_f32": round(1e6 / kernels["csrmv_f32"]["us"], 1),
+            "hbm_frac_matvec_f64": round(kernels["csrmv_f64"]["GBs"] / 8000.0, 4),
+            "hbm_frac_matvec_f32": round(kernels["csrmv_f32"]["GBs"] / 8000.0, 4),
+            "gmres_iterations_per_s_mixed": round(mx["iterations"] * mx["solves_per_s"], 1),
+            "gmres_iterations_per_s_fp64": round(fp["iterations"] * fp["solves_per_s"], 1),
+            "solves_per_s_mixed": round(mx["solves_per_s"], 3),
+            "solves_per_s_fp64": round(fp["solves_per_s"], 3)},
         "kernels": {k: {kk: round(vv, 2) for kk, vv in v.items()} for k, v in kernels.items()},
         "csr_lanes_autotune_us": {str(k): round(v, 2) for k, v in lane_us.items()},
         "roofline": roofline,
