@@ -448,15 +448,29 @@ int launch_ilu_factor(femgpu_ctx *ctx, femgpu_csr *A)
 // Elements are stored colour by colour; inside one colour no two elements share an unknown, so
 // each colour is one launch whose read-modify-write of Q needs no atomics. s = 3: one thread per
 // element; s = 9/12: one thread per element row, the element's lm staged in LDS.
+// One element per thread. A block's 256 element matrices and lm triples are one contiguous run
+// each (colour order): staged in LDS with consecutive lanes on consecutive words, instead of every
+// thread reading its own 72 bytes (each load instruction would touch dozens of cache lines). Blocks
+// in XCD order.
 __global__ __launch_bounds__(256) void k_ebe3(int e0, int e1, int NEQ, const int *__restrict__ lm,
                                               const double *__restrict__ A, const double *__restrict__ P,
                                               double *__restrict__ Q)
 {
-    const int k = e0 + blockIdx.x * 256 + threadIdx.x;
-    if (k >= e1) return;
-    const int i1 = lm[3 * k], i2 = lm[3 * k + 1], i3 = lm[3 * k + 2];
+    __shared__ double sa[256 * 9];
+    __shared__ int sl[256 * 3];
+    const int b0 = e0 + xcd_block() * 256;
+    if (b0 >= e1) return;   // the whole block
+    const int nb = min(256, e1 - b0);
+    const double *ga = A + 9 * static_cast<size_t>(b0);
+    const int *gl = lm + 3 * static_cast<size_t>(b0);
+    for (int q = threadIdx.x; q < 9 * nb; q += 256) sa[q] = ga[q];
+    for (int q = threadIdx.x; q < 3 * nb; q += 256) sl[q] = gl[q];
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t >= nb) return;
+    const int i1 = sl[3 * t], i2 = sl[3 * t + 1], i3 = sl[3 * t + 2];
     const double p1 = (i1 < NEQ) ? P[i1] : 0.0, p2 = (i2 < NEQ) ? P[i2] : 0.0, p3 = (i3 < NEQ) ? P[i3] : 0.0;
-    const double *a = A + 9 * static_cast<size_t>(k);
+    const double *a = sa + 9 * t;
     if (i1 < NEQ) Q[i1] += a[0] * p1 + a[1] * p2 + a[2] * p3;
     if (i2 < NEQ) Q[i2] += a[3] * p1 + a[4] * p2 + a[5] * p3;
     if (i3 < NEQ) Q[i3] += a[6] * p1 + a[7] * p2 + a[8] * p3;
@@ -506,7 +520,7 @@ int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *
         const int e0 = E->colour_ptr[c], e1 = E->colour_ptr[c + 1];
         if (e1 <= e0) continue;
         if (E->s == 3) {
-            k_ebe3<<<(e1 - e0 + 255) / 256, 256, 0, ctx->stream>>>(e0, e1, E->NEQ, E->lm.p, E->A.p, P, Q);
+            k_ebe3<<<xcd_grid((e1 - e0 + 255) / 256), 256, 0, ctx->stream>>>(e0, e1, E->NEQ, E->lm.p, E->A.p, P, Q);
         } else if (E->s == 9) {
             constexpr int EPB = 256 / 9;
             k_ebe_block<9><<<(e1 - e0 + EPB - 1) / EPB, 256, 0, ctx->stream>>>(e0, e1, E->NEQ, E->lm.p, E->A.p, P, Q);
