@@ -44,10 +44,12 @@ __device__ __forceinline__ void lds_barrier()
 // b mod 8), each XCD with its own L2. A launch of 8 * per blocks renumbers workgroup b as
 // (b mod 8) * per + b / 8, so XCD x processes the contiguous range [x * per, (x + 1) * per) of the
 // logical blocks: with a banded matrix its gathers stay mostly inside what its own L2 holds.
-// Returns the logical block; the launch rounds its grid up to a multiple of 8 and blocks whose
-// logical number reaches the needed count exit.
+// Returns the logical block; the launch rounds its grid up to a multiple of 8 (xcd_grid) and
+// blocks whose logical number reaches the needed count exit. A grid that is not a multiple of 8
+// keeps the identity order (the renumbering is a bijection only for multiples of 8).
 __device__ __forceinline__ int xcd_block()
 {
+    if (gridDim.x & 7u) return static_cast<int>(blockIdx.x);
     const int per = gridDim.x / 8;
     return static_cast<int>(blockIdx.x % 8) * per + static_cast<int>(blockIdx.x / 8);
 }
