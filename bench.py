@@ -216,13 +216,13 @@ def main():
         print(json.dumps({"profile_kernels": "done", "copy_bytes": src.numel() * 4, "n": neq, "nnz_upper": nnz_u}))
         return
 
-    # rows per wavefront of the FP64 CSR kernels (csrmv, pgmres operator, FP64 residual), tuned on the
-    # assembled matrix before any timed region
+    # kernel of the FP64 products (csrmv, pgmres operator, FP64 residual): CSR lanes per row or
+    # sliced ELL (0), tuned on the assembled matrix before any timed region
     M.Build_M_K_R_SUPG(pb, alpha, dt, u, du, R, AA_ptr=aa)
     A.values_changed()
     lanes, lane_us = A.autotune()
-    print(f"# CSR lanes per row: {lanes} (" + ", ".join(f"{k}: {v:.1f} us" for k, v in lane_us.items()) + ")",
-          file=sys.stderr, flush=True)
+    print(f"# FP64 product kernel (lanes per row, 0 = sliced ELL): {lanes} (" +
+          ", ".join(f"{k}: {v:.1f} us" for k, v in lane_us.items()) + ")", file=sys.stderr, flush=True)
 
     results = {}
     for mode in ("fp64", "mixed_exact_ilu", "mixed_graphs", "mixed"):
@@ -290,7 +290,8 @@ def main():
     bytes_ilu = 12 * (nnz - n) + 8 * (n + 1) + 8 * n + 16 * n
     bytes_jac = a.jacobi * (8 * (nnz - n) + 44 * n)
     kernels = {
-        "csrmv_f64": dict(us=t_spmv * 1e6, GBs=bytes_spmv / t_spmv / 1e9, lanes_per_row=lanes),
+        "csrmv_f64": dict(us=t_spmv * 1e6, GBs=bytes_spmv / t_spmv / 1e9, lanes_per_row=lanes,
+                          kernel="k_spmv_sell<double> (sliced ELL)" if lanes == 0 else f"k_spmv<double,{lanes}> (CSR)"),
         "csrmv_f32": dict(us=t_spmv32 * 1e6, GBs=bytes_spmv32 / t_spmv32 / 1e9),
         "ilu0_apply_f64": dict(us=t_ilu * 1e6, GBs=bytes_ilu / t_ilu / 1e9),
         "ilu0_apply_f32": dict(us=t_ilu32 * 1e6, GBs=(bytes_ilu / 2) / t_ilu32 / 1e9),
@@ -397,7 +398,7 @@ def main():
             "gmres_iterations_per_s_fp64": round(fp["iterations"] * fp["solves_per_s"], 1),
             "solves_per_s_mixed": round(mx["solves_per_s"], 3),
             "solves_per_s_fp64": round(fp["solves_per_s"], 3)},
-        "kernels": {k: {kk: (float(f"{vv:.3g}") if kk == "rel_diff_vs_csrmv" else round(vv, 2)) for kk, vv in v.items()}
+        "kernels": {k: {kk: (vv if isinstance(vv, str) else float(f"{vv:.3g}") if kk == "rel_diff_vs_csrmv" else round(vv, 2)) for kk, vv in v.items()}
                     for k, v in kernels.items()},
         "csr_lanes_autotune_us": {str(k): round(v, 2) for k, v in lane_us.items()},
         "roofline": roofline,

@@ -153,15 +153,20 @@ int femgpu_csr_set_factor_kernel(femgpu_csr *A, int kernel);
 int femgpu_csrmv(femgpu_csr *A, const double *d_P, double *d_Q);
 int femgpu_csrmv_f32(femgpu_csr *A, const float *d_P, float *d_Q);
 
-/* Rows per wavefront of the FP64 CSR kernels (csrmv, pgmres's operator, the mixed solver's FP64
- * residual): L lanes per row, 64 / L rows per wavefront. femgpu_csr_autotune times L = 1, 2, 4, 8,
- * 16 on A's structure and current values (20 launches each, HIP events) and keeps the fastest;
- * us[] = microseconds per launch for each L. Synchronises the context's stream. Default L = 8.
- * No reference counterpart (the reference's csrmv is one loop per row). */
+/* Kernel of the FP64 products (csrmv, pgmres's operator, the mixed solver's FP64 residual):
+ * L = 1, 2, 4, 8, 16 lanes per row of the CSR kernel (64 / L rows per wavefront), or L = 0: one row
+ * per thread on the 64-row sliced-ELL copy, with an FP64 copy of the values in that layout (kept,
+ * and refreshed by femgpu_csr_values_changed, only while L = 0; needs the sliced-ELL copy with
+ * slices at most 16 wide, else FEMGPU_ERR_ARG). femgpu_csr_autotune times every candidate on A's
+ * structure and current values (20 launches each, HIP events) and keeps the fastest; us[] =
+ * microseconds per launch for L = 1, 2, 4, 8, 16, us_sell for L = 0 (0 where not possible).
+ * Synchronises the context's stream. Default L = 8. No reference counterpart (the reference's
+ * csrmv is one loop per row). */
 typedef struct {
     int lanes;             /* the L chosen */
     int pad;
     double us[5];          /* per-launch time of L = 1, 2, 4, 8, 16 */
+    double us_sell;        /* per-launch time of L = 0 */
 } femgpu_tune_info;
 int femgpu_csr_autotune(femgpu_csr *A, femgpu_tune_info *info);
 int femgpu_csr_set_lanes(femgpu_csr *A, int lanes);

@@ -219,7 +219,10 @@ static int csr_setup(femgpu_csr *A, int n, int nnz, const int *IA, const int *JA
         if (!rc) rc = build_jacobi_ell(ctx, A, IA, JA, diag.data());
         if (!rc) rc = build_parilu_sell(ctx, A, IA, JA, diag.data());
     }
+    // a rebuilt matrix keeps FP64 sliced ELL only where its new structure allows it
+    if (!rc && A->spmv_lanes == 0 && !sell64_possible(A)) rc = set_spmv_lanes(A, 8);
     if (!rc && hipMemsetAsync(A->AA.p, 0, sizeof(double) * (nnz + 1), s) != hipSuccess) rc = FEMGPU_ERR_HIP;
+    if (!rc && sell64_active(A)) rc = launch_sell_values(ctx, A);
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     return rc;
 }
@@ -331,10 +334,9 @@ extern "C" int femgpu_csr_autotune(femgpu_csr *A, femgpu_tune_info *info)
 
 extern "C" int femgpu_csr_set_lanes(femgpu_csr *A, int lanes)
 {
-    FG_CHECK(A && (lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16), FEMGPU_ERR_ARG,
-             "femgpu_csr_set_lanes: lanes per row must be 1, 2, 4, 8 or 16");
-    A->spmv_lanes = lanes;
-    return FEMGPU_OK;
+    FG_CHECK(A && (lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16),
+             FEMGPU_ERR_ARG, "femgpu_csr_set_lanes: lanes per row must be 1, 2, 4, 8 or 16 (0: sliced ELL)");
+    return set_spmv_lanes(A, lanes);
 }
 
 extern "C" int femgpu_csrmv_f32(femgpu_csr *A, const float *d_P, float *d_Q)

@@ -180,9 +180,11 @@ struct femgpu_csr {
     fg::DevBuf<double> AA;          // nnz+1 (dummy slot nnz)
     fg::DevBuf<float> AA32;         // FP32 CSR values, kept only when sell_ok && par_ok is false
     int sell_ok = 0;                    // FP32 sliced-ELL copy present (slices of 64 rows)
+    int sell_wmax = 0;                  // widest slice (entries per row)
     fg::DevBuf<int> sell_ptr;           // slice -> first entry (nslices + 1); slice width = difference / 64
     fg::DevBuf<int> sell_col;           // entry k of row 64s + l at sell_ptr[s] + 64k + l (padding: the row)
     fg::DevBuf<float> sell_val;         // FP32 values in that layout (padding 0), refreshed with AA32
+    fg::DevBuf<double> sell_val64;      // FP64 values in that layout, kept while spmv_lanes == 0
     std::vector<int> hsell_ptr;         // host copy of sell_ptr
     // parallel FP32 ILU(0) on the sliced-ELL layout (rows of at most 16 entries)
     int par_ok = 0, par_maxw = 0;
@@ -214,7 +216,7 @@ struct femgpu_csr {
     fg::FactorPlan fplan;           // numeric factorisation (level schedule)
     fg::FactorPlan fplan_nat;       // the same lists in row order (parallel FP32 factorisation)
     fg::StreamFactor sfac;          // streamed single-workgroup factorisation (when available)
-    int spmv_lanes = 8;             // lanes per row of the SpMV kernel
+    int spmv_lanes = 8;             // lanes per row of the FP64 CSR kernels; 0: sliced ELL (sell_val64)
     int plans = 3;                  // FEMGPU_PLANS_* built at creation
 };
 
@@ -281,6 +283,9 @@ int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA);
 int build_parilu_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
 int launch_parilu_to_csr(femgpu_ctx *ctx, femgpu_csr *A);
 int launch_sell_values(femgpu_ctx *ctx, femgpu_csr *A);
+bool sell64_possible(const femgpu_csr *A);   // sliced-ELL copy present and narrow enough for FP64
+bool sell64_active(const femgpu_csr *A);     // the FP64 kernels read sell_val64
+int set_spmv_lanes(femgpu_csr *A, int lanes);   // 0: FP64 sliced ELL (allocates and fills sell_val64)
 int launch_to_f32(femgpu_ctx *ctx, const double *src, float *dst, size_t n);
 int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *Q);
 int launch_ebe_gather_values(femgpu_ctx *ctx, femgpu_ebe *E, const double *A_elem_order);

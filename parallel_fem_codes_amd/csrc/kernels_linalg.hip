@@ -47,52 +47,130 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ IA,
     }
 }
 
-// FP32 SpMV on the sliced-ELL copy: one row per thread, a wave's 64 rows form one slice padded to
-// its longest row, entries column-major inside the slice, so every load of a step of the row loop
-// is one contiguous 256-byte run and the loop count is uniform across the wave. A block of 256
-// threads takes kSellRows * 256 rows, each thread kSellRows rows 256 apart, their loads in flight
-// together. Same xs / xout / skip contract as k_spmv.
-constexpr int kSellRows = 2;
+// SpMV on the sliced-ELL copy: one row per thread, a wave's 64 rows form one slice padded to its
+// longest row, entries column-major inside the slice, so every load of a step of the row loop is
+// one contiguous run and the loop count is uniform across the wave. Same xs / xout / skip contract
+// as k_spmv.
+//
+// k_spmv_sell<T, K> (slices at most K wide) issues all K column and value loads of the row before
+// the first gather (steps past the slice width re-read step 0 and are not added, so the sum is the
+// plain loop's over the row, in its order). Plain loads: non-temporal ones were faster in isolation
+// but slower inside the solver, where the matrix is re-read every Arnoldi step and stays cached.
+// k_spmv_sell_loop takes any width.
+constexpr int kSell64MaxWidth = 16;   // widest slice the FP64 sliced-ELL kernels take
 
+template <typename T, int K>
 __global__ __launch_bounds__(256) void k_spmv_sell(int n, const int *__restrict__ sptr, const int *__restrict__ col,
-                                                   const float *__restrict__ val, const float *__restrict__ x,
-                                                   const double *__restrict__ xs_ptr, float *__restrict__ xout,
-                                                   float *__restrict__ y, const int *__restrict__ skip)
+                                                   const T *__restrict__ val, const T *__restrict__ x,
+                                                   const double *__restrict__ xs_ptr, T *__restrict__ xout,
+                                                   T *__restrict__ y, const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    const float xs = xs_ptr ? static_cast<float>(*xs_ptr) : 1.0f;
-    const int blk = xcd_block();
-    float acc[kSellRows];
+    const int row = xcd_block() * 256 + threadIdx.x;
+    const int rr = row < n ? row : n - 1;
+    const int s = rr >> 6, l = rr & 63;
+    const int p0 = sptr[s], w = (sptr[s + 1] - p0) >> 6;
+    int c[K];
+    T v[K];
 #pragma unroll
-    for (int r = 0; r < kSellRows; ++r) {
-        const int row = (blk * kSellRows + r) * 256 + threadIdx.x;
-        acc[r] = 0.0f;
-        if (row >= n) continue;
-        const int s = row >> 6, l = row & 63;
-        const int p0 = sptr[s], w = (sptr[s + 1] - p0) >> 6;
-        for (int k = 0; k < w; ++k) {
-            const int e = p0 + (k << 6) + l;
-            acc[r] += val[e] * (x[col[e]] * xs);
-        }
+    for (int k = 0; k < K; ++k) {
+        const int e = p0 + ((k < w ? k : 0) << 6) + l;
+        c[k] = col[e];
+        v[k] = val[e];
     }
+    const T xs = xs_ptr ? static_cast<T>(*xs_ptr) : T(1);
+    T acc = 0;
 #pragma unroll
-    for (int r = 0; r < kSellRows; ++r) {
-        const int row = (blk * kSellRows + r) * 256 + threadIdx.x;
-        if (row >= n) continue;
-        y[row] = acc[r];
-        if (xout) xout[row] = x[row] * xs;
+    for (int k = 0; k < K; ++k) {
+        const T t = v[k] * (x[c[k]] * xs);
+        acc = k < w ? acc + t : acc;
     }
+    if (row >= n) return;
+    y[row] = acc;
+    if (xout) xout[row] = x[row] * xs;
 }
 
-__global__ void k_sell_values(int n, const int *__restrict__ IA, const int *__restrict__ sptr,
-                              const double *__restrict__ AA, float *__restrict__ val)
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmv_sell_loop(int n, const int *__restrict__ sptr,
+                                                        const int *__restrict__ col, const T *__restrict__ val,
+                                                        const T *__restrict__ x, const double *__restrict__ xs_ptr,
+                                                        T *__restrict__ xout, T *__restrict__ y,
+                                                        const int *__restrict__ skip)
 {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (skip && *skip) return;
+    const int row = xcd_block() * 256 + threadIdx.x;
+    if (row >= n) return;
+    const T xs = xs_ptr ? static_cast<T>(*xs_ptr) : T(1);
+    const int s = row >> 6, l = row & 63;
+    const int p0 = sptr[s], w = (sptr[s + 1] - p0) >> 6;
+    T acc = 0;
+    for (int k = 0; k < w; ++k) {
+        const int e = p0 + (k << 6) + l;
+        acc += val[e] * (x[col[e]] * xs);
+    }
+    y[row] = acc;
+    if (xout) xout[row] = x[row] * xs;
+}
+
+// r = f - A x on the FP64 sliced-ELL copy (loads as in k_spmv_sell<double, K>).
+template <int K>
+__global__ __launch_bounds__(256) void k_residual_sell(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+                                                       const double *__restrict__ val, const double *__restrict__ x,
+                                                       const double *__restrict__ f, double *__restrict__ r)
+{
+    const int row = xcd_block() * 256 + threadIdx.x;
+    const int rr = row < n ? row : n - 1;
+    const int s = rr >> 6, l = rr & 63;
+    const int p0 = sptr[s], w = (sptr[s + 1] - p0) >> 6;
+    int c[K];
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int e = p0 + ((k < w ? k : 0) << 6) + l;
+        c[k] = col[e];
+        v[k] = val[e];
+    }
+    const double fr = f[rr];
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double t = v[k] * x[c[k]];
+        acc = k < w ? acc + t : acc;
+    }
+    if (row < n) r[row] = fr - acc;
+}
+
+// The CSR values into the sliced-ELL layout (FP32, and FP64 when val64 is set). K > 0: slices at
+// most K wide, the row's K values loaded together (positions past the row clamped to its first,
+// or to the dummy slot nnz for an empty row); K = 0: any width.
+template <int K>
+__global__ __launch_bounds__(256) void k_sell_values(int n, const int *__restrict__ IA, const int *__restrict__ sptr,
+                                                     const double *__restrict__ AA, float *__restrict__ val,
+                                                     double *__restrict__ val64)
+{
+    const int row = xcd_block() * 256 + threadIdx.x;
     if (row >= n) return;
     const int s = row >> 6, l = row & 63;
     const int p0 = sptr[s], w = (sptr[s + 1] - p0) >> 6;
     const int a0 = IA[row], len = IA[row + 1] - a0;
-    for (int k = 0; k < w; ++k) val[p0 + (k << 6) + l] = k < len ? static_cast<float>(AA[a0 + k]) : 0.0f;
+    if constexpr (K > 0) {
+        double a[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) a[k] = AA[a0 + (k < len ? k : 0)];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (k < w) {
+                const double v = k < len ? a[k] : 0.0;
+                val[p0 + (k << 6) + l] = static_cast<float>(v);
+                if (val64) val64[p0 + (k << 6) + l] = v;
+            }
+    } else {
+        for (int k = 0; k < w; ++k) {
+            const double v = k < len ? AA[a0 + k] : 0.0;
+            val[p0 + (k << 6) + l] = static_cast<float>(v);
+            if (val64) val64[p0 + (k << 6) + l] = v;
+        }
+    }
 }
 
 int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA)
@@ -103,9 +181,11 @@ int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA)
     const int ns = (n + 63) / 64;
     std::vector<int> sptr(static_cast<size_t>(ns) + 1, 0);
     long long total = 0;
+    int wmax = 0;
     for (int s = 0; s < ns; ++s) {
         int w = 0;
         for (int r = 64 * s; r < std::min(n, 64 * s + 64); ++r) w = std::max(w, IA[r + 1] - IA[r]);
+        wmax = std::max(wmax, w);
         total += 64LL * w;
         if (total >= INT_MAX) return FEMGPU_OK;   // int entry offsets: stay with CSR
         sptr[static_cast<size_t>(s) + 1] = static_cast<int>(total);
@@ -126,7 +206,9 @@ int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA)
     FG_TRY(A->sell_ptr.upload(sptr.data(), sptr.size(), ctx->stream));
     FG_TRY(A->sell_col.upload(col.data(), col.size(), ctx->stream));
     FG_TRY(A->sell_val.alloc(col.size()));
+    if (A->spmv_lanes == 0 && wmax <= kSell64MaxWidth) FG_TRY(A->sell_val64.alloc(col.size()));
     FG_HIP(hipStreamSynchronize(ctx->stream));   // the host arrays end with this call
+    A->sell_wmax = wmax;
     A->sell_ok = 1;
     return FEMGPU_OK;
 }
@@ -134,8 +216,37 @@ int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA)
 int launch_sell_values(femgpu_ctx *ctx, femgpu_csr *A)
 {
     if (!A->sell_ok) return FEMGPU_OK;
-    k_sell_values<<<(A->n + 255) / 256, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->sell_ptr.p, A->AA.p, A->sell_val.p);
+    const int grid = xcd_grid((A->n + 255) / 256);
+    double *v64 = A->spmv_lanes == 0 ? A->sell_val64.p : nullptr;
+    if (A->sell_wmax <= 8)
+        k_sell_values<8><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->sell_ptr.p, A->AA.p, A->sell_val.p, v64);
+    else if (A->sell_wmax <= 16)
+        k_sell_values<16><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->sell_ptr.p, A->AA.p, A->sell_val.p, v64);
+    else
+        k_sell_values<0><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->sell_ptr.p, A->AA.p, A->sell_val.p, v64);
     FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+bool sell64_possible(const femgpu_csr *A) { return A->sell_ok && A->sell_wmax <= kSell64MaxWidth; }
+
+bool sell64_active(const femgpu_csr *A) { return A->spmv_lanes == 0 && sell64_possible(A) && A->sell_val64.p; }
+
+int set_spmv_lanes(femgpu_csr *A, int lanes)
+{
+    if (lanes == 0) {
+        if (!sell64_possible(A)) {
+            set_error("FP64 sliced ELL needs the sliced-ELL copy with slices at most %d wide", kSell64MaxWidth);
+            return FEMGPU_ERR_ARG;
+        }
+        const bool fresh = A->spmv_lanes != 0;
+        FG_TRY(A->sell_val64.alloc(A->sell_col.n));
+        A->spmv_lanes = 0;
+        if (fresh) FG_TRY(launch_sell_values(A->ctx, A));   // fill from the current values
+        return FEMGPU_OK;
+    }
+    A->spmv_lanes = lanes;
+    A->sell_val64.release();
     return FEMGPU_OK;
 }
 
@@ -165,7 +276,7 @@ template <typename T>
 static int spmv_dispatch(femgpu_ctx *ctx, const femgpu_csr *A, const T *AA, const T *x, const double *xs,
                          T *xout, T *y, const int *skip)
 {
-    const int L = A->spmv_lanes;
+    const int L = A->spmv_lanes > 0 ? A->spmv_lanes : 8;   // 0 (sliced ELL) is dispatched before this
     const long threads = static_cast<long>(A->n) * L;
     const int grid = xcd_grid(static_cast<int>((threads + 255) / 256));
     if (A->n == 0) return FEMGPU_OK;
@@ -180,29 +291,51 @@ static int spmv_dispatch(femgpu_ctx *ctx, const femgpu_csr *A, const T *AA, cons
     return FEMGPU_OK;
 }
 
+template <typename T>
+static int sell_dispatch(femgpu_ctx *ctx, const femgpu_csr *A, const T *val, const T *x, const double *xs, T *xout,
+                         T *y, const int *skip)
+{
+    const int n = A->n, grid = xcd_grid((n + 255) / 256);
+    const int *sp = A->sell_ptr.p, *sc = A->sell_col.p;
+    if (A->sell_wmax <= 8)
+        k_spmv_sell<T, 8><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, val, x, xs, xout, y, skip);
+    else if (A->sell_wmax <= 16)
+        k_spmv_sell<T, 16><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, val, x, xs, xout, y, skip);
+    else
+        k_spmv_sell_loop<T><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, val, x, xs, xout, y, skip);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
 int launch_spmv64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *xs, double *xout,
                   double *y, const int *skip)
 {
+    if (A->n > 0 && sell64_active(A)) return sell_dispatch<double>(ctx, A, A->sell_val64.p, x, xs, xout, y, skip);
     return spmv_dispatch<double>(ctx, A, A->AA.p, x, xs, xout, y, skip);
 }
 
 int launch_spmv32(femgpu_ctx *ctx, const femgpu_csr *A, const float *x, const double *xs, float *xout,
                   float *y, const int *skip)
 {
-    if (A->sell_ok && A->n > 0) {
-        const int grid = xcd_grid((A->n + 256 * kSellRows - 1) / (256 * kSellRows));
-        k_spmv_sell<<<grid, 256, 0, ctx->stream>>>(A->n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p, x, xs, xout, y,
-                                                   skip);
-        FG_HIP(hipGetLastError());
-        return FEMGPU_OK;
-    }
+    if (A->sell_ok && A->n > 0) return sell_dispatch<float>(ctx, A, A->sell_val.p, x, xs, xout, y, skip);
     return spmv_dispatch<float>(ctx, A, A->AA32.p, x, xs, xout, y, skip);
 }
 
 int launch_residual64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *f, double *r)
 {
     if (A->n == 0) return FEMGPU_OK;
-    const int L = A->spmv_lanes;
+    if (sell64_active(A)) {
+        const int n = A->n, grid = xcd_grid((n + 255) / 256);
+        const int *sp = A->sell_ptr.p, *sc = A->sell_col.p;
+        const double *sv = A->sell_val64.p;
+        if (A->sell_wmax <= 8)   // sell64_active implies sell_wmax <= kSell64MaxWidth
+            k_residual_sell<8><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, sv, x, f, r);
+        else
+            k_residual_sell<kSell64MaxWidth><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, sv, x, f, r);
+        FG_HIP(hipGetLastError());
+        return FEMGPU_OK;
+    }
+    const int L = A->spmv_lanes > 0 ? A->spmv_lanes : 8;
     const int grid = xcd_grid(static_cast<int>((static_cast<long>(A->n) * L + 255) / 256));
     switch (L) {
     case 1: k_residual<1><<<grid, 256, 0, ctx->stream>>>(A->n, A->IA.p, A->JA.p, A->AA.p, x, f, r); break;
@@ -226,7 +359,8 @@ __global__ void k_fill(int n, double v, double *__restrict__ x)
 int csr_autotune(femgpu_csr *A, femgpu_tune_info *info)
 {
     femgpu_ctx *ctx = A->ctx;
-    static constexpr int kLanes[5] = {1, 2, 4, 8, 16};
+    // candidates: CSR with 1, 2, 4, 8, 16 lanes per row, then FP64 sliced ELL (lanes 0) where possible
+    static constexpr int kLanes[6] = {1, 2, 4, 8, 16, 0};
     femgpu_tune_info out{};
     out.lanes = A->spmv_lanes;
     if (A->n == 0) {
@@ -242,10 +376,12 @@ int csr_autotune(femgpu_csr *A, femgpu_tune_info *info)
     FG_HIP(hipEventCreate(&e0));
     FG_HIP(hipEventCreate(&e1));
     const int saved = A->spmv_lanes;
+    const int ncand = sell64_possible(A) ? 6 : 5;
     int best = saved, rc = FEMGPU_OK;
     float best_ms = 0.0f;
-    for (int c = 0; c < 5 && rc == FEMGPU_OK; ++c) {
-        A->spmv_lanes = kLanes[c];
+    for (int c = 0; c < ncand && rc == FEMGPU_OK; ++c) {
+        if (kLanes[c] == 0) rc = set_spmv_lanes(A, 0);   // allocates and fills the FP64 sliced-ELL values
+        else A->spmv_lanes = kLanes[c];
         for (int w = 0; w < 2 && rc == FEMGPU_OK; ++w) rc = launch_spmv64(ctx, A, x.p, nullptr, nullptr, y.p, nullptr);
         if (rc == FEMGPU_OK && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = FEMGPU_ERR_HIP;
         for (int t = 0; t < 20 && rc == FEMGPU_OK; ++t) rc = launch_spmv64(ctx, A, x.p, nullptr, nullptr, y.p, nullptr);
@@ -254,7 +390,7 @@ int csr_autotune(femgpu_csr *A, femgpu_tune_info *info)
                                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
             rc = FEMGPU_ERR_HIP;
         if (rc != FEMGPU_OK) break;
-        out.us[c] = 1e3 * ms / 20.0;
+        (kLanes[c] == 0 ? out.us_sell : out.us[c]) = 1e3 * ms / 20.0;
         if (c == 0 || ms < best_ms) {
             best_ms = ms;
             best = kLanes[c];
@@ -262,11 +398,13 @@ int csr_autotune(femgpu_csr *A, femgpu_tune_info *info)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    A->spmv_lanes = rc == FEMGPU_OK ? best : saved;
+    const int keep = rc == FEMGPU_OK ? best : saved;
+    A->spmv_lanes = 8;   // set_spmv_lanes(0) then (re)fills the FP64 sliced-ELL values
+    const int rc2 = set_spmv_lanes(A, keep);
     out.lanes = A->spmv_lanes;
     if (info) *info = out;
     if (rc != FEMGPU_OK) set_error("femgpu_csr_autotune: timing launches failed");
-    return rc;
+    return rc != FEMGPU_OK ? rc : rc2;
 }
 
 // ------------------------------------------------------------------------------- conversions

@@ -133,20 +133,33 @@ __global__ __launch_bounds__(256) void k_jacobi_upper(int n, const int *__restri
 // ---- the same iteration on the sliced-ELL layout of the FP32 matrix (slices of 64 rows, entry k of
 // row 64s + l at sell_ptr[s] + 64k + l): every per-row load and store of a step is one contiguous
 // run across the wave, and the update lists are stored the same way (padded per slice).
+template <int MAXW>
 __global__ __launch_bounds__(256) void k_parilu_init_sell(int n, const int *__restrict__ sptr,
                                                           const int *__restrict__ col, const float *__restrict__ AA,
                                                           const int *__restrict__ rinfo, const int *__restrict__ dpos,
                                                           float *__restrict__ LU, float *__restrict__ Dinv)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = xcd_block() * 256 + threadIdx.x;
     if (i >= n) return;
     const int s = i >> 6, l = i & 63, p0 = sptr[s] + l, w = (sptr[s + 1] - sptr[s]) >> 6;
     const int nL = rinfo[i] & 0xff;
-    for (int k = 0; k < w; ++k) {
-        const int e = p0 + (k << 6);
-        LU[e] = k < nL ? AA[e] / AA[dpos[col[e]]] : AA[e];
+    // all loads of the row first (steps past the slice width re-read step 0), then the two-level
+    // gathers of the column pivots
+    int c[MAXW];
+    float a[MAXW], piv[MAXW];
+#pragma unroll
+    for (int k = 0; k < MAXW; ++k) {
+        const int e = p0 + ((k < w ? k : 0) << 6);
+        c[k] = col[e];
+        a[k] = AA[e];
     }
-    Dinv[i] = 1.0f / AA[dpos[i]];
+    const float own = AA[dpos[i]];
+#pragma unroll
+    for (int k = 0; k < MAXW; ++k) piv[k] = AA[dpos[c[k]]];
+#pragma unroll
+    for (int k = 0; k < MAXW; ++k)
+        if (k < w) LU[p0 + (k << 6)] = k < nL ? a[k] / piv[k] : a[k];
+    Dinv[i] = 1.0f / own;
 }
 
 template <int MAXW>
@@ -342,8 +355,47 @@ __global__ void k_jacobi_ell_values(int n, const int *__restrict__ IA, const int
 
 // A block of 256 threads takes kJacRows * 256 consecutive rows, each thread kJacRows rows 256 apart
 // (same lane, so a wavefront still covers whole 64-row slices): the loads of all its rows are in
-// flight together, and one round of resident wavefronts covers the matrix.
+// flight together, and one round of resident wavefronts covers the matrix. Every row issues all K
+// column and value loads of its slice unconditionally (steps past the slice width re-read step 0,
+// rows past n re-read row n-1) and only then gathers: a load behind a branch on the slice width
+// would wait for the loads before it. The terms past the width are not added, so the sum is the
+// one of the plain loop over the slice, in the same order.
 constexpr int kJacRows = 2;
+
+template <int K>
+struct EllRows {
+    int col[kJacRows][K];
+    float val[kJacRows][K];
+    int w[kJacRows];
+};
+
+// Loads the slice entries of the thread's rows; w[r] = 0 for rows past n and for empty slices.
+template <int K>
+__device__ __forceinline__ void ell_load(int n, int blk, const int *__restrict__ sp, const int *__restrict__ col,
+                                         const float *__restrict__ val, EllRows<K> &e)
+{
+#pragma unroll
+    for (int r = 0; r < kJacRows; ++r) {
+        const int i = (blk * kJacRows + r) * 256 + threadIdx.x;
+        const int ii = i < n ? i : n - 1;
+        const int s = ii >> 6, q = sp[s] + (ii & 63), w = (sp[s + 1] - sp[s]) >> 6;
+        e.w[r] = i < n ? w : 0;
+        if (w > 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int p = q + ((k < w ? k : 0) << 6);
+                e.col[r][k] = col[p];
+                e.val[r][k] = val[p];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                e.col[r][k] = ii;
+                e.val[r][k] = 0.0f;
+            }
+        }
+    }
+}
 
 // y_out = b - L_s y_in (y_in == nullptr: y_in = b), sliced ELL.
 template <int K>
@@ -354,24 +406,25 @@ __global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__re
 {
     if (skip && *skip) return;
     const float *src = yin ? yin : b;
-    float acc[kJacRows];
+    const int blk = xcd_block();
+    EllRows<K> e;
+    ell_load<K>(n, blk, sl, cl, vl, e);
+    float bi[kJacRows];
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
-        const int i = (xcd_block() * kJacRows + r) * 256 + threadIdx.x;
-        acc[r] = 0.0f;
-        if (i >= n) continue;
-        const int s = i >> 6, q = sl[s] + (i & 63), w = (sl[s + 1] - sl[s]) >> 6;
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (k < w) {
-                const int e = q + (k << 6);
-                acc[r] += vl[e] * src[cl[e]];
-            }
+        const int i = (blk * kJacRows + r) * 256 + threadIdx.x;
+        bi[r] = b[i < n ? i : n - 1];
     }
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
-        const int i = (xcd_block() * kJacRows + r) * 256 + threadIdx.x;
-        if (i < n) yout[i] = b[i] - acc[r];
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const float t = e.val[r][k] * src[e.col[r][k]];
+            acc = k < e.w[r] ? acc + t : acc;
+        }
+        const int i = (blk * kJacRows + r) * 256 + threadIdx.x;
+        if (i < n) yout[i] = bi[r] - acc;
     }
 }
 
@@ -383,25 +436,27 @@ __global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, const int *__re
                                                           float *__restrict__ zout, const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    float acc[kJacRows];
+    const int blk = xcd_block();
+    EllRows<K> e;
+    ell_load<K>(n, blk, su, cu, vu, e);
+    float di[kJacRows], yi[kJacRows];
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
-        const int i = (xcd_block() * kJacRows + r) * 256 + threadIdx.x;
-        acc[r] = 0.0f;
-        if (i >= n) continue;
-        const int s = i >> 6, q = su[s] + (i & 63), w = (su[s + 1] - su[s]) >> 6;
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (k < w) {
-                const int e = q + (k << 6);
-                const int j = cu[e];
-                acc[r] += vu[e] * (zin ? zin[j] : Dinv[j] * y[j]);
-            }
+        const int i = (blk * kJacRows + r) * 256 + threadIdx.x, ii = i < n ? i : n - 1;
+        di[r] = Dinv[ii];
+        yi[r] = y[ii];
     }
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
-        const int i = (xcd_block() * kJacRows + r) * 256 + threadIdx.x;
-        if (i < n) zout[i] = Dinv[i] * (y[i] - acc[r]);
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int j = e.col[r][k];
+            const float t = e.val[r][k] * (zin ? zin[j] : Dinv[j] * y[j]);
+            acc = k < e.w[r] ? acc + t : acc;
+        }
+        const int i = (blk * kJacRows + r) * 256 + threadIdx.x;
+        if (i < n) zout[i] = di[r] * (yi[r] - acc);
     }
 }
 
@@ -469,8 +524,12 @@ static int parilu_sell(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
     // the final iterate must land in par_a / Dinv32
     float *L0 = (sweeps % 2 == 0) ? A->par_a.p : A->par_b.p, *L1 = (sweeps % 2 == 0) ? A->par_b.p : A->par_a.p;
     float *D0 = (sweeps % 2 == 0) ? A->Dinv32.p : A->Dinv32b.p, *D1 = (sweeps % 2 == 0) ? A->Dinv32b.p : A->Dinv32.p;
-    k_parilu_init_sell<<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p, A->par_rinfo.p,
-                                                      A->par_dpos.p, L0, D0);
+    if (A->par_maxw <= 8)
+        k_parilu_init_sell<8><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
+                                                             A->par_rinfo.p, A->par_dpos.p, L0, D0);
+    else
+        k_parilu_init_sell<16><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
+                                                              A->par_rinfo.p, A->par_dpos.p, L0, D0);
     for (int s = 0; s < sweeps; ++s) {
         if (A->par_maxw <= 8)
             k_parilu_sweep_sell<8><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,

@@ -276,13 +276,18 @@ class CsrMatrix:
         N.check(N.lib().femgpu_csr_rebuild(self.h, self.n, self.nnz, _i32(IA), _i32(JA)))
 
     def autotune(self):
-        """Times the FP64 CSR kernels at 1, 2, 4, 8 and 16 lanes per row and keeps the fastest;
-        returns (lanes chosen, {lanes: microseconds per launch})."""
+        """Times the FP64 product kernels (CSR at 1, 2, 4, 8 and 16 lanes per row; sliced ELL, lanes
+        0, where the structure allows it) and keeps the fastest; returns (lanes chosen, {lanes:
+        microseconds per launch})."""
         t = N.TuneInfo()
         N.check(N.lib().femgpu_csr_autotune(self.h, C.byref(t)))
-        return t.lanes, {L: t.us[k] for k, L in enumerate((1, 2, 4, 8, 16))}
+        us = {L: t.us[k] for k, L in enumerate((1, 2, 4, 8, 16))}
+        if t.us_sell > 0:
+            us[0] = t.us_sell
+        return t.lanes, us
 
     def set_lanes(self, lanes):
+        """Lanes per row of the FP64 CSR kernels (1, 2, 4, 8, 16) or 0 for FP64 sliced ELL."""
         N.check(N.lib().femgpu_csr_set_lanes(self.h, int(lanes)))
 
     def sweep_info(self):

@@ -65,7 +65,7 @@ class MixedOpts(C.Structure):
 
 class TuneInfo(C.Structure):
     """femgpu_tune_info."""
-    _fields_ = [("lanes", C.c_int), ("pad", C.c_int), ("us", C.c_double * 5)]
+    _fields_ = [("lanes", C.c_int), ("pad", C.c_int), ("us", C.c_double * 5), ("us_sell", C.c_double)]
 
 
 class BatchInfo(C.Structure):

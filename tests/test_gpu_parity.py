@@ -117,8 +117,8 @@ def test_csr_rebuild_reuses_the_matrix(dev, torch_cuda, oracle):
 
 
 def test_csr_rows_per_wavefront_autotune(dev, torch_cuda, oracle):
-    """Every lanes-per-row setting computes csrmv and the mixed solver's FP64 residual to the oracle's
-    1e-13; the autotuner times all five and keeps the fastest."""
+    """Every lanes-per-row setting and FP64 sliced ELL (lanes 0) compute csrmv and the mixed solver's
+    FP64 residual to the oracle's 1e-13; the autotuner times all six and keeps the fastest."""
     torch = torch_cuda
     s = steady_system(oracle, m=200)
     A = fem.CsrMatrix(dev, s["IA"], s["JA"])
@@ -126,13 +126,26 @@ def test_csr_rows_per_wavefront_autotune(dev, torch_cuda, oracle):
     p = np.random.default_rng(3).standard_normal(s["neq"])
     ref = oracle.csrmv(s["IA"], s["JA"], s["AA"], p)
     q = torch.empty(s["neq"], dtype=torch.float64, device="cuda")
-    for L in (1, 2, 4, 8, 16):
+    for L in (1, 2, 4, 8, 16, 0, 2, 0):
         A.set_lanes(L)
         A.csrmv(T(torch, p), q)
         assert rel(H(q), ref) < 1e-13, L
+    # FP64 sliced ELL: one row per thread in the row's order, as the reference's loop
+    r = torch.empty_like(q)
+    A.set_lanes(0)
+    A.csrmv(T(torch, p), r)
+    A.set_lanes(1)
+    A.csrmv(T(torch, p), q)
+    assert torch.equal(q, r)
+    # values changed while sliced ELL is active: the FP64 copy follows
+    A.set_lanes(0)
+    A.set_values(2.0 * s["AA"][:-1])
+    A.csrmv(T(torch, p), q)
+    assert rel(H(q), 2.0 * ref) < 1e-13
+    A.set_values(s["AA"][:-1])
     lanes, us = A.autotune()
     print(f"lanes chosen {lanes}: " + ", ".join(f"L={k} {v:.2f} us" for k, v in us.items()))
-    assert lanes in (1, 2, 4, 8, 16) and all(v > 0 for v in us.values())
+    assert lanes in (0, 1, 2, 4, 8, 16) and set(us) == {0, 1, 2, 4, 8, 16} and all(v > 0 for v in us.values())
     assert us[lanes] == min(us.values())
     A.csrmv(T(torch, p), q)
     assert rel(H(q), ref) < 1e-13
