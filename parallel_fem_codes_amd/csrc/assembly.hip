@@ -20,7 +20,48 @@ struct ElemIn {
     const int *type, *invP, *ien;
     femgpu_transport_problem pb;
     double alpha_dt;
+    int neq;
 };
+
+// An element's three vertices with their coordinates, marks and state, loaded in three rounds (the
+// connectivity; coordinates, marks, numbering and boundary values; the state of the free vertices),
+// every load of a round issued together: loading vertex by vertex behind a branch on its mark would
+// chain about ten dependent memory round trips per element. A Dirichlet vertex (type 0) takes
+// ubc and du = 0; a free vertex takes u and du at its equation number. du is read when requested.
+struct ElemNodes {
+    int v[3], t[3];
+    double X[3], Y[3], ub[3], u[3], du[3];
+};
+
+__device__ __forceinline__ void load_nodes(const ElemIn &in, int e, bool with_du, ElemNodes &N)
+{
+    int ip[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) N.v[a] = in.ien[3 * e + a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int v = N.v[a];
+        N.X[a] = in.x[v];
+        N.Y[a] = in.y[v];
+        N.t[a] = in.type[v];
+        N.ub[a] = in.ubc[v];
+        ip[a] = in.invP[v];
+    }
+    double uf[3] = {0.0, 0.0, 0.0}, duf[3] = {0.0, 0.0, 0.0};
+    if (in.neq > 0) {   // else every vertex is a Dirichlet vertex
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int q = N.t[a] != 0 ? ip[a] : 0;
+            uf[a] = in.u[q];
+            if (with_du) duf[a] = in.du[q];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        N.u[a] = N.t[a] == 0 ? N.ub[a] : uf[a];
+        N.du[a] = N.t[a] == 0 ? 0.0 : duf[a];
+    }
+}
 
 struct SupgElem {
     double y23, y31, y12, x32, x13, x21, Area, invArea, tau, Be_x, Be_y, fe, feb;
@@ -132,17 +173,12 @@ __global__ __launch_bounds__(256) void k_elem_steady(int nel, ElemIn in, double 
 #pragma clang fp contract(off)
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= nel) return;
-    int v[3];
-    double X[3], Y[3], U[3];
+    ElemNodes N;
+    load_nodes(in, e, false, N);
+    const double *X = N.X, *Y = N.Y, *U = N.u;
     int m = 0;
-    for (int a = 0; a < 3; ++a) {
-        v[a] = in.ien[3 * e + a];
-        X[a] = in.x[v[a]];
-        Y[a] = in.y[v[a]];
-        const int t = in.type[v[a]];
-        U[a] = (t == 0) ? in.ubc[v[a]] : in.u[in.invP[v[a]]];
-        if (t == 0) m |= 1 << a;
-    }
+    for (int a = 0; a < 3; ++a)
+        if (N.t[a] == 0) m |= 1 << a;
     SupgElem E;
     supg_geometry(E, X, Y, in.pb);
     const double third = 1.0 / 3.0;
@@ -156,7 +192,7 @@ __global__ __launch_bounds__(256) void k_elem_steady(int nel, ElemIn in, double 
         int t = 0;
         for (int b = 0; b < 3; ++b) {
             if (b == a) continue;
-            lift[6 * e + 2 * a + t] = E.ke[a][b] * in.ubc[v[b]];
+            lift[6 * e + 2 * a + t] = E.ke[a][b] * N.ub[b];
             ++t;
         }
         for (int b = 0; b < 3; ++b) Ke[9 * static_cast<size_t>(e) + 3 * a + b] = E.ke[a][b];
@@ -173,21 +209,11 @@ struct TransientElem {
 __device__ __forceinline__ void transient_elem(const ElemIn &in, int e, TransientElem &T)
 {
 #pragma clang fp contract(off)
-    double X[3], Y[3], ue[3], due[3];
-    for (int a = 0; a < 3; ++a) {
-        const int v = in.ien[3 * e + a];
-        X[a] = in.x[v];
-        Y[a] = in.y[v];
-        if (in.type[v] == 0) {
-            ue[a] = in.ubc[v];
-            due[a] = 0.0;
-        } else {
-            ue[a] = in.u[in.invP[v]];
-            due[a] = in.du[in.invP[v]];
-        }
-    }
+    ElemNodes N;
+    load_nodes(in, e, true, N);
+    const double *ue = N.u, *due = N.du;
     SupgElem E;
-    supg_geometry(E, X, Y, in.pb);
+    supg_geometry(E, N.X, N.Y, in.pb);
     const double third = 1.0 / 3.0, sixth = 1.0 / 6.0, twelfth = 1.0 / 12.0;
     const double ueb = third * (ue[0] + ue[1] + ue[2]);
     const double dueb = third * (due[0] + due[1] + due[2]);
@@ -371,6 +397,7 @@ static ElemIn elem_in(femgpu_mesh *M, const femgpu_transport_problem *pb, const 
     in.ien = M->ien.p;
     in.pb = *pb;
     in.alpha_dt = alpha_dt;
+    in.neq = M->neq;
     return in;
 }
 
