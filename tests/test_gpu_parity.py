@@ -96,12 +96,15 @@ def test_csrmv_fp64_fp32(dev, torch_cuda, oracle, name):
     assert rel(H(q32), ref) < 1e-5
 
 
-def test_csr_rebuild_reuses_the_matrix(dev, torch_cuda, oracle):
+@pytest.mark.parametrize("lanes", [8, 0])
+def test_csr_rebuild_reuses_the_matrix(dev, torch_cuda, oracle, lanes):
     """One matrix object re-planned for a larger, then a smaller structure: csrmv matches the oracle
-    each time and the mixed solver solves the re-planned system to its FP64-verified tolerance."""
+    each time and the mixed solver solves the re-planned system to its FP64-verified tolerance (with
+    the CSR kernel, and with FP64 sliced ELL, which the rebuilds keep)."""
     torch = torch_cuda
     small, large = transient_system(oracle, m=60, dt=5e-3), transient_system(oracle, m=140, dt=5e-3)
     A = fem.CsrMatrix(dev, small["IA"], small["JA"])
+    A.set_lanes(lanes)
     for s in (small, large, small):
         A.rebuild(s["IA"], s["JA"])
         A.set_values(s["AA"][:-1])
@@ -114,6 +117,33 @@ def test_csr_rebuild_reuses_the_matrix(dev, torch_cuda, oracle):
         info = fem.gmres_mixed(dev, A, T(torch, s["F"]), X, tol=1e-8, sweep_mode=2, factor_sweeps=2)
         res = np.linalg.norm(s["F"] - oracle.csrmv(s["IA"], s["JA"], s["AA"], H(X))) / np.linalg.norm(s["F"])
         assert info.status == 0 and res <= 1e-8
+
+
+def test_fp64_sliced_ell_needs_narrow_slices(dev, torch_cuda, oracle):
+    """A row of 40 entries makes a slice wider than the FP64 sliced-ELL kernels take: lanes 0 is
+    refused, and a matrix on lanes 0 rebuilt to that structure falls back to the CSR kernel; csrmv
+    stays exact to the oracle throughout."""
+    torch = torch_cuda
+    n = 3000   # large enough that the sliced-ELL copy exists (its padding stays bounded)
+    rows = [sorted(set([i] + ([i - 1] if i else []) + ([i + 1] if i + 1 < n else []))) for i in range(n)]
+    rows[0] = list(range(40))
+    for j in range(1, 40):
+        rows[j] = sorted(set(rows[j]) | {0})
+    IA = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    JA = np.concatenate(rows).astype(np.int32)
+    AA = np.random.default_rng(5).standard_normal(len(JA)) + 10.0 * np.isin(np.arange(len(JA)), IA[:-1])
+    p = np.random.default_rng(6).standard_normal(n)
+    ref = oracle.csrmv(IA, JA, np.r_[AA, 0.0], p)
+    q = torch.empty(n, dtype=torch.float64, device="cuda")
+    s = steady_system(oracle, m=30)
+    A = fem.CsrMatrix(dev, s["IA"], s["JA"])
+    A.set_lanes(0)
+    A.rebuild(IA, JA)
+    A.set_values(AA)
+    A.csrmv(T(torch, p), q)
+    assert rel(H(q), ref) < 1e-13
+    with pytest.raises(Exception):
+        A.set_lanes(0)
 
 
 def test_csr_rows_per_wavefront_autotune(dev, torch_cuda, oracle):
