@@ -299,8 +299,8 @@ def main():
         "parallel_ilu0_f32": dict(us=t_par * 1e6, sweeps=a.factor_sweeps),
         "jacobi_apply_f32": dict(us=t_jac * 1e6, GBs=bytes_jac / t_jac / 1e9, iterations=a.jacobi),
     }
-    # EBE matvec (ebemv, 3x3 element blocks, greedy colouring) on the same operator: element matrices
-    # from the device assembly, checked against the CSR product in the same numbering
+    # EBE matvec (ebemv, 3x3 element blocks gathered per unknown) on the same operator: element
+    # matrices from the device assembly, checked against the CSR product in the same numbering
     nel = len(P["ien"]) // 3
     Ae_t = torch.empty(nel * 9, dtype=torch.float64, device="cuda")
     M.Build_M_K_R_SUPG(pb, alpha, dt, u, du, R, Ae=Ae_t, AA_ptr=aa)

@@ -466,13 +466,34 @@ extern "C" int femgpu_ebe_create(femgpu_ctx *ctx, int nel, int s, int NEQ, const
     for (int e = 0; e < nel; ++e) E->colour_ptr[colour[e] + 1]++;
     for (int c = 0; c < nc; ++c) E->colour_ptr[c + 1] += E->colour_ptr[c];
     std::vector<int> fill(E->colour_ptr.begin(), E->colour_ptr.end() - 1), elem(nel);
-    for (int e = 0; e < nel; ++e) elem[fill[colour[e]]++] = e;
+    if (s == 3)   // gathered per unknown: element order
+        for (int e = 0; e < nel; ++e) elem[e] = e;
+    else
+        for (int e = 0; e < nel; ++e) elem[fill[colour[e]]++] = e;
     std::vector<int> lmc(static_cast<size_t>(nel) * s);
     for (int k = 0; k < nel; ++k)
         for (int a = 0; a < s; ++a) lmc[static_cast<size_t>(k) * s + a] = lm[static_cast<size_t>(elem[k]) * s + a];
     int rc = E->elem.upload(elem.data(), nel, ctx->stream);
     if (!rc) rc = E->lm.upload(lmc.data(), lmc.size(), ctx->stream);
     if (!rc) rc = E->A.alloc(static_cast<size_t>(nel) * s * s);
+    std::vector<int> rptr, rsrc, rdst, rec;   // live until the synchronisation below
+    if (!rc && s == 3) {
+        std::vector<std::pair<int, int>> pairs;
+        pairs.reserve(static_cast<size_t>(nel) * 3);
+        for (int e = 0; e < nel; ++e)
+            for (int a = 0; a < 3; ++a)
+                if (lm[3 * static_cast<size_t>(e) + a] < NEQ) pairs.emplace_back(lm[3 * static_cast<size_t>(e) + a], 3 * e + a);
+        csr_map(NEQ, pairs, rptr, rsrc, rdst);
+        rec.resize(4 * rsrc.size());
+        for (size_t q = 0; q < rsrc.size(); ++q) {
+            const int ea = rsrc[q], e = ea / 3;
+            rec[4 * q] = ea;
+            for (int b = 0; b < 3; ++b) rec[4 * q + 1 + b] = lm[3 * static_cast<size_t>(e) + b];
+        }
+        rc = E->rptr.upload(rptr.data(), rptr.size(), ctx->stream);
+        if (!rc) rc = E->rdst.upload(rdst.data(), rdst.size(), ctx->stream);
+        if (!rc) rc = E->rrec.upload(rec.data(), rec.size(), ctx->stream);
+    }
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (rc) {
         delete E;

@@ -224,9 +224,12 @@ struct femgpu_ebe {
     femgpu_ctx *ctx = nullptr;
     int nel = 0, s = 0, NEQ = 0, ncolours = 0;
     std::vector<int> colour_ptr;    // elements of colour c: [colour_ptr[c], colour_ptr[c+1])
-    fg::DevBuf<int> elem;           // colour-sorted position -> element id
-    fg::DevBuf<int> lm;             // lm in colour-sorted order
-    fg::DevBuf<double> A;           // element matrices in colour-sorted order
+    fg::DevBuf<int> elem;           // storage position -> element id (colour-sorted; identity for s = 3)
+    fg::DevBuf<int> lm;             // lm in storage order
+    fg::DevBuf<double> A;           // element matrices in storage order
+    // s = 3: the product is gathered per unknown. List k (for unknown rdst[k]) holds the element rows
+    // (e, a) with lm[e][a] = rdst[k], in element order, as records {3e + a, lm[e][0..2]}
+    fg::DevBuf<int> rptr, rdst, rrec;
 };
 
 struct femgpu_mesh {

@@ -583,35 +583,51 @@ int launch_ilu_factor(femgpu_ctx *ctx, femgpu_csr *A)
 }
 
 // ------------------------------------------------------------------------------- EBE matvec
-// Elements are stored colour by colour; inside one colour no two elements share an unknown, so
-// each colour is one launch whose read-modify-write of Q needs no atomics. s = 3: one thread per
-// element; s = 9/12: one thread per element row, the element's lm staged in LDS.
-// One element per thread. A block's 256 element matrices and lm triples are one contiguous run
-// each (colour order): staged in LDS with consecutive lanes on consecutive words, instead of every
-// thread reading its own 72 bytes (each load instruction would touch dozens of cache lines). Blocks
-// in XCD order.
-__global__ __launch_bounds__(256) void k_ebe3(int e0, int e1, int NEQ, const int *__restrict__ lm,
-                                              const double *__restrict__ A, const double *__restrict__ P,
-                                              double *__restrict__ Q)
+// s = 9/12: elements are stored colour by colour; inside one colour no two elements share an
+// unknown, so each colour is one launch whose read-modify-write of Q needs no atomics, one thread
+// per element row, the element's lm staged in LDS. s = 3: gathered per unknown (k_ebe3_rows).
+// EBE product for 3x3 blocks gathered per unknown: thread k sums, in element order, the rows
+// (e, a) of list k (Q[i] += A_e[a][0] p1 + A_e[a][1] p2 + A_e[a][2] p3 of ebemv.c:40-55 for every
+// element with lm[e][a] = i), so Q is written once, with no colouring and no read-modify-write. The
+// first kEbeUnroll records of a list are loaded together, then their matrix rows and P entries
+// (P of the Dirichlet sink NEQ read as 0), then summed in list order. No contraction into fused
+// multiply-adds: the products and sums round as the reference's, so Q is bit-identical to its
+// element loop.
+constexpr int kEbeUnroll = 6;
+
+__device__ __forceinline__ double ebe_row(const int4 r, int NEQ, const double *__restrict__ A,
+                                          const double *__restrict__ P)
 {
-    __shared__ double sa[256 * 9];
-    __shared__ int sl[256 * 3];
-    const int b0 = e0 + xcd_block() * 256;
-    if (b0 >= e1) return;   // the whole block
-    const int nb = min(256, e1 - b0);
-    const double *ga = A + 9 * static_cast<size_t>(b0);
-    const int *gl = lm + 3 * static_cast<size_t>(b0);
-    for (int q = threadIdx.x; q < 9 * nb; q += 256) sa[q] = ga[q];
-    for (int q = threadIdx.x; q < 3 * nb; q += 256) sl[q] = gl[q];
-    __syncthreads();
-    const int t = threadIdx.x;
-    if (t >= nb) return;
-    const int i1 = sl[3 * t], i2 = sl[3 * t + 1], i3 = sl[3 * t + 2];
-    const double p1 = (i1 < NEQ) ? P[i1] : 0.0, p2 = (i2 < NEQ) ? P[i2] : 0.0, p3 = (i3 < NEQ) ? P[i3] : 0.0;
-    const double *a = sa + 9 * t;
-    if (i1 < NEQ) Q[i1] += a[0] * p1 + a[1] * p2 + a[2] * p3;
-    if (i2 < NEQ) Q[i2] += a[3] * p1 + a[4] * p2 + a[5] * p3;
-    if (i3 < NEQ) Q[i3] += a[6] * p1 + a[7] * p2 + a[8] * p3;
+#pragma clang fp contract(off)
+    const double *a = A + 3 * static_cast<size_t>(r.x);
+    const double a0 = a[0], a1 = a[1], a2 = a[2];
+    const double v1 = P[r.y < NEQ ? r.y : 0], v2 = P[r.z < NEQ ? r.z : 0], v3 = P[r.w < NEQ ? r.w : 0];
+    const double p1 = r.y < NEQ ? v1 : 0.0, p2 = r.z < NEQ ? v2 : 0.0, p3 = r.w < NEQ ? v3 : 0.0;
+    return a0 * p1 + a1 * p2 + a2 * p3;
+}
+
+__global__ __launch_bounds__(256) void k_ebe3_rows(int NEQ, const int *__restrict__ ptr, const int *__restrict__ dst,
+                                                   const int4 *__restrict__ rec, const double *__restrict__ A,
+                                                   const double *__restrict__ P, double *__restrict__ Q)
+{
+#pragma clang fp contract(off)
+    const int k = xcd_block() * 256 + static_cast<int>(threadIdx.x);
+    if (k >= NEQ) return;
+    const int p0 = ptr[k], p1 = ptr[k + 1], d = dst[k];
+    double acc = 0.0;
+    if (p1 > p0) {
+        int4 r[kEbeUnroll];
+#pragma unroll
+        for (int u = 0; u < kEbeUnroll; ++u) r[u] = rec[p0 + u < p1 ? p0 + u : p0];
+        double t[kEbeUnroll];
+#pragma unroll
+        for (int u = 0; u < kEbeUnroll; ++u) t[u] = ebe_row(r[u], NEQ, A, P);
+#pragma unroll
+        for (int u = 0; u < kEbeUnroll; ++u)
+            if (p0 + u < p1) acc += t[u];
+        for (int q = p0 + kEbeUnroll; q < p1; ++q) acc += ebe_row(rec[q], NEQ, A, P);
+    }
+    Q[d] = acc;
 }
 
 template <int S>
@@ -650,6 +666,13 @@ __global__ void k_zero(double *__restrict__ q, int n)
 
 int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *Q)
 {
+    if (E->s == 3) {
+        if (E->NEQ > 0)
+            k_ebe3_rows<<<xcd_grid((E->NEQ + 255) / 256), 256, 0, ctx->stream>>>(
+                E->NEQ, E->rptr.p, E->rdst.p, reinterpret_cast<const int4 *>(E->rrec.p), E->A.p, P, Q);
+        FG_HIP(hipGetLastError());
+        return FEMGPU_OK;
+    }
     if (E->NEQ > 0) {
         k_zero<<<std::min(4096, (E->NEQ + 255) / 256), 256, 0, ctx->stream>>>(Q, E->NEQ);
         FG_HIP(hipGetLastError());
@@ -657,9 +680,7 @@ int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *
     for (int c = 0; c < E->ncolours; ++c) {
         const int e0 = E->colour_ptr[c], e1 = E->colour_ptr[c + 1];
         if (e1 <= e0) continue;
-        if (E->s == 3) {
-            k_ebe3<<<xcd_grid((e1 - e0 + 255) / 256), 256, 0, ctx->stream>>>(e0, e1, E->NEQ, E->lm.p, E->A.p, P, Q);
-        } else if (E->s == 9) {
+        if (E->s == 9) {
             constexpr int EPB = 256 / 9;
             k_ebe_block<9><<<(e1 - e0 + EPB - 1) / EPB, 256, 0, ctx->stream>>>(e0, e1, E->NEQ, E->lm.p, E->A.p, P, Q);
         } else if (E->s == 12) {
