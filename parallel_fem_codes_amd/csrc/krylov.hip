@@ -118,17 +118,11 @@ __global__ __launch_bounds__(256) void k_mgs(GDev G, int n, int i, int j, T *__r
     using V = Vec16<T>;
     using VT = typename V::type;
     __shared__ double red[4];
-    T h = T(0);
-    if (uprev) {
-        const double hp = sum_partials(G.partials + ((j - 1) & 1) * kReduceBlocks, gridDim.x, red);
-        if (blockIdx.x == 0 && threadIdx.x == 0) Hat(G, j - 1, i) = hp;
-        h = static_cast<T>(hp);
-    }
-    double acc = 0;
     const int nv = n / V::N, stride = gridDim.x * 256;
-    // kMgsVec vector chunks per thread per step, their loads issued together
-    for (int k0 = blockIdx.x * 256 + threadIdx.x; k0 < nv; k0 += kMgsVec * stride) {
-        VT wv[kMgsVec], pv[kMgsVec], uv[kMgsVec];
+    // kMgsVec vector chunks per thread per step, their loads issued together; the first step's loads
+    // are issued before the previous pass's partial sums are combined into h, so the two overlap
+    VT wv[kMgsVec], pv[kMgsVec], uv[kMgsVec];
+    auto load = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < kMgsVec; ++u) {
             const int k = k0 + u * stride;
@@ -138,6 +132,18 @@ __global__ __launch_bounds__(256) void k_mgs(GDev G, int n, int i, int j, T *__r
                 if (unext) uv[u] = reinterpret_cast<const VT *>(unext)[k];
             }
         }
+    };
+    const int kfirst = blockIdx.x * 256 + threadIdx.x;
+    load(kfirst);
+    T h = T(0);
+    if (uprev) {
+        const double hp = sum_partials(G.partials + ((j - 1) & 1) * kReduceBlocks, gridDim.x, red);
+        if (blockIdx.x == 0 && threadIdx.x == 0) Hat(G, j - 1, i) = hp;
+        h = static_cast<T>(hp);
+    }
+    double acc = 0;
+    for (int k0 = kfirst; k0 < nv; k0 += kMgsVec * stride) {
+        if (k0 != kfirst) load(k0);
 #pragma unroll
         for (int u = 0; u < kMgsVec; ++u) {
             const int k = k0 + u * stride;
