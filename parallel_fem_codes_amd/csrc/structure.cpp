@@ -40,11 +40,16 @@ void element_pattern(int nel, int s, int nrows, const int *lm, std::vector<int> 
     IA.assign(static_cast<size_t>(nrows) + 1, 0);
     JA.clear();
     JA.reserve(buf.size() / 2 + 1);
+    // per row: drop repeated columns with a last-row marker, then sort the (few) distinct ones
+    std::vector<int> mark(static_cast<size_t>(nrows), -1);
     for (int i = 0; i < nrows; ++i) {
-        auto first = buf.begin() + cnt[i], last = buf.begin() + cnt[i + 1];
-        std::sort(first, last);
-        auto end = std::unique(first, last);
-        JA.insert(JA.end(), first, end);
+        const size_t start = JA.size();
+        for (int q = cnt[i]; q < cnt[i + 1]; ++q)
+            if (mark[buf[q]] != i) {
+                mark[buf[q]] = i;
+                JA.push_back(buf[q]);
+            }
+        std::sort(JA.begin() + static_cast<std::ptrdiff_t>(start), JA.end());
         IA[i + 1] = static_cast<int>(JA.size());
     }
 }
