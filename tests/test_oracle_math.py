@@ -155,6 +155,96 @@ def test_transient_assembly_against_steady_build(oracle):
         assert R[neq] == 0.0
 
 
+def _p1_supg_mass(x, y, ien, pb):
+    """Per-element SUPG mass from the textbook P1 integrals: M_ab = A/12 (1 + delta_ab) plus the
+    streamline term tau (beta . grad N_a) A/3, with grad N_a = (b_a, c_a)/(2A), h = sqrt|2A|,
+    tau = h / (2|beta|) and beta taken at the centroid."""
+    X, Y = x[ien.reshape(-1, 3)], y[ien.reshape(-1, 3)]
+    b = np.stack([Y[:, 1] - Y[:, 2], Y[:, 2] - Y[:, 0], Y[:, 0] - Y[:, 1]], 1)
+    c = np.stack([X[:, 2] - X[:, 1], X[:, 0] - X[:, 2], X[:, 1] - X[:, 0]], 1)
+    twoA = c[:, 2] * b[:, 1] - c[:, 1] * b[:, 2]
+    A = 0.5 * twoA
+    XB, YB = X.mean(1), Y.mean(1)
+    bx = pb.bx[0] + pb.bx[1] * XB + pb.bx[2] * YB
+    by = pb.by[0] + pb.by[1] * XB + pb.by[2] * YB
+    tau = np.sqrt(np.abs(twoA)) / (2 * np.hypot(bx, by))
+    beta_grad = (bx[:, None] * b + by[:, None] * c) / twoA[:, None]          # beta . grad N_a
+    M = A[:, None, None] / 12.0 * (np.ones((3, 3)) + np.eye(3))
+    return M + (tau[:, None] * beta_grad * A[:, None] / 3.0)[:, :, None]
+
+
+def test_transient_residual_with_nonzero_increment(oracle):
+    """Build_M_K_R_SUPG.c:148-181 with du != 0, f = 0, u_D = 0 and no shock capture: the residual on
+    the unknowns is R = -K u - M du, with K the reference-pinned steady SUPG stiffness and M the SUPG
+    mass assembled here from the textbook P1 integrals (so tau's magnitude is checked too). A dropped,
+    sign-flipped or transposed M du, or a wrong tau, fails; so does Ae != M + alpha*dt*K."""
+    x, y, ien, t, invP, neq, lm = _mesh(oracle, 15, right=False)
+    c = oracle.csr_init(lm, 3, neq, reorder="SYMRCM", permute_lm=True, type_=t, invP=invP)
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal(neq)
+    du = rng.standard_normal(neq)
+    ubc = np.zeros(len(x))
+    alpha, dt = 0.5, 2e-3
+    pb = oracle.make_problem(k=2e-3, shock=0)
+    Ae, R = oracle.build_transport_transient(x, y, t, c["invP"], ien, c["lm"], ubc, u, du, pb, alpha, dt, neq)
+    Ke, F = oracle.build_transport_supg(x, y, t, c["invP"], ien, ubc, u, pb, neq)
+    assert not F.any()
+    Me = _p1_supg_mass(x, y, ien, pb)
+    assert np.allclose(Ae.reshape(-1, 3, 3), Me + alpha * dt * Ke.reshape(-1, 3, 3), rtol=0,
+                       atol=1e-14 * np.abs(Me).max())
+    Mg = np.zeros((neq + 1, neq + 1))
+    L = c["lm"].reshape(-1, 3)
+    for e in range(len(L)):
+        Mg[np.ix_(L[e], L[e])] += Me[e]
+    Mg = Mg[:neq, :neq]
+    AA = oracle.csr_assemble(c["scheme"], Ke, c["nnz"])
+    expect = -oracle.csrmv(c["IA"], c["JA"], AA, u) - Mg @ du
+    scale = np.abs(expect).max()
+    assert np.allclose(R[:neq], expect, rtol=0, atol=1e-13 * scale)
+    assert R[neq] == 0.0
+    # the M du part is far above the tolerance, so dropping it or flipping its sign cannot pass
+    assert np.abs(Mg @ du).max() > 1e6 * 1e-13 * scale
+    assert not np.allclose(R[:neq], -oracle.csrmv(c["IA"], c["JA"], AA, u) - Mg.T @ du, rtol=0,
+                           atol=1e-13 * scale)
+
+
+def test_transient_shock_capture_residual_uses_increment(oracle):
+    """CAU_ShockCapture.c:10 (transient form): R_u = dueb + beta.grad u + gamma ueb - feb. With a
+    linear u (grad u constant), constant beta, gamma = f = 0 and du = -beta.grad u at every unknown,
+    R_u vanishes on elements whose three vertices are unknowns, so their Ae equals the Ae without
+    shock capture; on elements touching the boundary (du = 0 there) and with du = 0 or du = +beta.grad u
+    everywhere, the CAU term is active. A dropped or sign-flipped dueb fails."""
+    x, y, ien, t, invP, neq, lm = _mesh(oracle, 14)
+    c = oracle.csr_init(lm, 3, neq, reorder="SYMRCM", permute_lm=True, type_=t, invP=invP)
+    gx, gy, b0, b1 = 0.7, -1.3, 0.8, 0.45
+    g = gx * b0 + gy * b1                                   # beta . grad u
+    ulin = gx * x + gy * y + 0.2
+    u = np.empty(neq)
+    free = t == 1
+    u[c["invP"][free]] = ulin[free]
+    ubc = np.where(free, 0.0, ulin)
+    alpha, dt = 0.5, 1e-3
+    kw = dict(k=1e-3, bx=(b0, 0.0, 0.0), by=(b1, 0.0, 0.0))
+    args = (x, y, t, c["invP"], ien, c["lm"], ubc, u)
+    A0, _ = oracle.build_transport_transient(*args, np.full(neq, -g), oracle.make_problem(shock=0, **kw),
+                                             alpha, dt, neq)
+    A0 = A0.reshape(-1, 3, 3)
+    inner = free[ien.reshape(-1, 3)].all(1)
+    assert inner.any() and (~inner).any()
+    scale = np.abs(A0).max()
+    diff = {}
+    for name, dval in (("cancel", -g), ("zero", 0.0), ("flip", g)):
+        A1, _ = oracle.build_transport_transient(*args, np.full(neq, dval), oracle.make_problem(shock=1, **kw),
+                                                 alpha, dt, neq)
+        diff[name] = np.abs(A1.reshape(-1, 3, 3) - A0).max(axis=(1, 2)) / scale
+    assert diff["cancel"][inner].max() < 1e-13             # R_u = 0: no shock-capture diffusion
+    assert diff["cancel"][~inner].max() > 1e-6             # boundary elements: du = 0 at Dirichlet nodes
+    assert diff["zero"][inner].min() > 1e-6                # dueb dropped would look like this
+    assert diff["flip"][inner].min() > 1e-6                # dueb with the wrong sign
+    # the CAU diffusion grows with |R_u|: |R_u| = 2|g| (flip) beats |R_u| = |g| (zero)
+    assert np.all(diff["flip"][inner] > diff["zero"][inner])
+
+
 @pytest.mark.parametrize("s", [3, 9, 12])
 def test_block_ebemv_against_dense_assembly(oracle, s):
     ndof = s // 3
