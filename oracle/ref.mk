@@ -36,7 +36,7 @@ EU      := $(REF)/03_Euler_Equations/2D/01_CommonFiles
 EU_SRC  := $(EU)/Build_M_K_F_SUPG.c $(EU)/eval_U_dU.c $(EU)/Ax_Ay_calculations.c $(EU)/Delta_CAU.c \
            $(EU)/setzeros.c $(EU)/ebe_assembly.c $(EU)/csr_assembly.c $(EU)/Fill_LM.c $(EU)/csr_Initialization.c
 
-all: $(OUT)/st_harness $(OUT)/ns_harness $(OUT)/eu_harness $(OUT)/meshGenerator
+all: $(OUT)/st_harness $(OUT)/st_harness_ofast $(OUT)/ns_harness $(OUT)/eu_harness $(OUT)/meshGenerator
 
 # The structured mesh generator is a self-contained C program.
 $(OUT)/meshGenerator: $(REF)/Mesh_Generator/Structured_Mesh/meshGenerator.c
@@ -49,6 +49,15 @@ $(OUT)/st_harness: $(HERE)ref_harness_st.c $(ST_SRC) $(CF_SRC)
 	mkdir -p $(OUT)
 	gcc $(CFLAGS) -o $@ $(HERE)ref_harness_st.c $(ST_SRC) $(CF_SRC) \
 	    -L$(MPI)/lib -Wl,-rpath,$(MPI)/lib -lmpi -lm \
+	    -Wl,--unresolved-symbols=ignore-in-object-files
+
+# The same harness built with the reference Makefile's optimisation flags (-Ofast -march=native,
+# 01_SS_Transport_Equation/2D/01_CommonFiles/Makefile:4; its -Ddebug only adds progress prints):
+# the timing baseline of the reference CPU path (oracle/cpu_ratio.py), not a fixture source.
+$(OUT)/st_harness_ofast: $(HERE)ref_harness_st.c $(ST_SRC) $(CF_SRC)
+	mkdir -p $(OUT)
+	gcc -Ofast -march=native -w -DSSTranspEquation2D -I$(ST) -I$(MPI)/include -o $@ $(HERE)ref_harness_st.c \
+	    $(ST_SRC) $(CF_SRC) -L$(MPI)/lib -Wl,-rpath,$(MPI)/lib -lmpi -lm \
 	    -Wl,--unresolved-symbols=ignore-in-object-files
 
 $(OUT)/ns_harness: $(HERE)ref_harness_ns.c $(NS_SRC) $(NS_CF)

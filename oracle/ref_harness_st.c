@@ -18,8 +18,10 @@
  * Input (binary, little endian): int32 header[16], float64 params[16], then x[nn], y[nn],
  * type[nn] (int32), ien[3*nel] (int32), ubc[nn], probe[nn] (float64).
  * Output: a stream of records {char name[32]; int32 kind (0 int32, 1 float64); int64 count;
- * payload}.
+ * payload}. The record "timers" holds the wall seconds spent in Build_K_F_SUPG, precond_setup and
+ * pgmres, summed over the nonlinear iterations (clock_gettime, CLOCK_MONOTONIC).
  */
+#include <time.h>
 #include "SSTranspEquation.h"
 #include "../../../00_CommonFiles/Solvers_and_Preconditioners/preconditioners.h"
 #include "../../../00_CommonFiles/Solvers_and_Preconditioners/solvers.h"
@@ -53,6 +55,13 @@ static double h_noshock(double *a, double *b, int e, double kx, double ky, doubl
 
 void MPI_VectorUpdate(int RANK, int NPROC, ParametersType *Parameters, FemStructsType *FemStructs,
                       double *P, int tag) { (void)RANK; (void)NPROC; (void)Parameters; (void)FemStructs; (void)P; (void)tag; }
+
+static double now(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
 
 static FILE *g_out;
 static void rec(const char *name, int kind, long n, const void *p)
@@ -161,10 +170,13 @@ int main(int argc, char **argv)
     double normdiff = nltol + 1, normu = 1.0 / nltol;
     int it = 0;
     int *iters = calloc(nl_max + 1, sizeof(int));
+    double timers[3] = {0, 0, 0}, t0;
     while (normdiff > nltol * normu && it < nl_max) {
         it++;
         dcopy(neq, U, uold);
+        t0 = now();
         Build_K_F_SUPG(P, MD, FS, FF);
+        timers[0] += now() - t0;
         if (it == 1) {
             if (ebe) rec("A", 1, 9L * nel, MD->Aaux); else rec("AA", 1, P->nnzero + 1, MD->AA);
             rec("F", 1, neq, F);
@@ -175,7 +187,9 @@ int main(int argc, char **argv)
             rec("mv", 1, neq, qv);
             free(pv); free(qv);
         }
+        t0 = now();
         FF->precond_setup(P, MD, FS, it, F);
+        timers[1] += now() - t0;
         if (it == 1) {
             rec("MinvF", 1, neq, F);
             if (!ebe) {
@@ -193,7 +207,9 @@ int main(int argc, char **argv)
             }
         }
         const int before = P->iterations;
+        t0 = now();
         pgmres(P, MD, FS, FF, F, U);
+        timers[2] += now() - t0;
         iters[it - 1] = P->iterations - before;
         if (it == 1) rec("X1", 1, neq, U);
         daxpy(neq, -1, U, uold);
@@ -201,6 +217,7 @@ int main(int argc, char **argv)
         normu = sqrt(ddot(neq, U, U));
     }
     rec("iters", 0, it, iters);
+    rec("timers", 1, 3, timers);
     rec("u", 1, neq, U);
     fclose(g_out);
     MPI_Finalize();
