@@ -166,6 +166,72 @@ def time_integration(fem, torch, M, A, pb, P, dt, alpha, kmax, tol, a):
     return out
 
 
+def euler_run(fem, torch, dev, timed, timed_cold, mx=800, my=400, steps=3, multicorrections=2, kmax=50, tol=1e-8):
+    """The compressible Euler workload: the reference's OBLIQUO problem (Mach 2 onto a wall at 10
+    degrees) on (mx+1) x (my+1) nodes over [0, 2] x [0, 1], 2 mx my triangles, 4 unknowns per node,
+    12x12 SUPG element blocks with CAU shock capture, dt from CFL 5, GMRES(kmax) on the gathered
+    12x12 EBE product with the nodal BlockDiag preconditioner. `steps` pseudo-time steps of
+    Predictor_Old with a fixed number of multicorrections (StopByIterations), once with the FP64 path
+    (the reference's BlockDiag setup + pgmres) and once with the mixed path (FP32 EBE + FP32 blocks
+    inside FP64 refinement), both from the initial state and both solving every linear system to the
+    same relative tolerance; then the 12x12 product alone, FP64 and FP32."""
+    P = fem.obliquo_problem(mx, my)
+    neq, nn, nel = P["neq"], len(P["x"]), len(P["ien"]) // 3
+    prm = dict(gamma=1.4, Mach=2.0, alpha=0.5, dt=P["dt"], tolerance=1e-8, dimensionless=0, shock=1)
+    t0 = time.perf_counter()
+    M = fem.EulerMesh(dev, P["x"], P["y"], P["type4"], P["invP"], P["presc"], P["ien"], P["lm"], neq)
+    E = fem.EbeMatrix(dev, P["lm"], 12, neq)
+    B = fem.BlockDiag(dev, nn, P["ien"], 4, P["type4"], P["invP"], neq)
+    setup_s = time.perf_counter() - t0
+    pb = fem.make_euler_problem(**prm)
+    out = {"workload": f"OBLIQUO Euler, {mx + 1}x{my + 1} nodes ({nel} triangles), neq={neq}, CFL 5 (dt {P['dt']:.4g}), "
+                       f"GMRES({kmax}) tol {tol}, EBE 12x12 + BlockDiag, {multicorrections} multicorrections per step",
+           "host_setup_s": round(setup_s, 2)}
+    states = {}
+    for solver in ("fp64", "mixed"):
+        opts = fem.driver_opts(solver, kmax=kmax, lmax=400, tol=tol, nonlinear_tol=1e-30,
+                               nonlinear_maxiter=multicorrections, stop_by_norm=False, inner_kmax=kmax, inner_tol=1e-4)
+        u = torch.from_numpy(P["u0"]).cuda()
+        a = torch.zeros_like(u)
+        per, lin = [], 0
+        for k in range(steps):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            info = fem.Predictor_Old_step_Euler(M, E, B, pb, u, a, opts)
+            torch.cuda.synchronize()
+            per.append(time.perf_counter() - t1)
+            lin += info.linear_iterations
+            if info.status == 2:
+                print(f"# euler {solver}: a linear solve did not converge at step {k + 1}", file=sys.stderr, flush=True)
+        states[solver] = u
+        # the first step includes one-off allocation; report the mean of the others
+        rest = per[1:] if len(per) > 1 else per
+        out[solver] = {"ms_per_step": round(1e3 * float(np.mean(rest)), 3), "first_step_ms": round(1e3 * per[0], 3),
+                       "linear_iterations_per_step": round(lin / steps, 2), "status_last": info.status}
+        print(f"# euler {solver}: {out[solver]}", file=sys.stderr, flush=True)
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    out["mixed_rel_err_vs_fp64"] = float(torch.linalg.norm(states["mixed"] - states["fp64"]) /
+                                         torch.linalg.norm(states["fp64"]))
+    inf = E.info
+    p64 = torch.rand(neq, dtype=torch.float64, device="cuda")
+    q64 = torch.empty_like(p64)
+    p32, q32 = p64.float(), torch.empty(neq, dtype=torch.float32, device="cuda")
+    t64, t32 = timed(lambda: E.ebemv(p64, q64)), timed(lambda: E.ebemv(p32, q32))
+    t64c = timed_cold(lambda: E.ebemv(p64, q64))
+    alg64 = nel * (8 * 144 + 4 * 12) + 16 * neq
+    alg32 = nel * (4 * 144 + 4 * 12) + 8 * neq
+    out["ebemv12"] = {"kernel": "k_ebe_gather<double/float, short, 4>", "algorithmic_bytes_f64": alg64,
+                      "layout_bytes_f64": int(inf.bytes_f64), "us_f64": round(t64 * 1e6, 2),
+                      "frac_f64": round(alg64 / t64 / 1e9 / HBM_PEAK_GBS, 4), "us_f64_cold": round(t64c * 1e6, 2),
+                      "frac_f64_cold": round(alg64 / t64c / 1e9 / HBM_PEAK_GBS, 4),
+                      "algorithmic_bytes_f32": alg32, "layout_bytes_f32": int(inf.bytes_f32),
+                      "us_f32": round(t32 * 1e6, 2), "frac_f32": round(alg32 / t32 / 1e9 / HBM_PEAK_GBS, 4),
+                      "column_bytes": inf.column_bytes}
+    print(f"# euler ebemv12: {out['ebemv12']}", file=sys.stderr, flush=True)
+    del M, E, B
+    return out
+
+
 def serving_run(count, rate=100.0, nmin=2000, nmax=50000, seed=20261202):
     """The serving workload (steady solves on Delaunay meshes of 2k-50k points, Poisson arrivals)
     through the library's request loop, and the same requests through a sequential plain-FP64 GPU
@@ -213,6 +279,8 @@ def main():
     ap.add_argument("--time-steps", type=int, default=200,
                     help="predictor-multicorrector steps run after the timed region (0: skip)")
     ap.add_argument("--nonlinear-tol", type=float, default=1e-4, help="multicorrection stop (StopByNorm)")
+    ap.add_argument("--euler-steps", type=int, default=3,
+                    help="pseudo-time steps of the 801x401 Euler workload run after the timed region (0: skip)")
     ap.add_argument("--serving-requests", type=int, default=100,
                     help="requests of the serving workload run after the timed region (0: skip)")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -333,15 +401,17 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps * 1e-3
 
-    flush = torch.empty(256 << 20, dtype=torch.float32, device="cuda")   # 1 GiB, 4x the last-level cache
+    flush = torch.ones(256 << 20, dtype=torch.float32, device="cuda")   # 1 GiB, 4x the last-level cache
+    flush_sum = torch.empty((), dtype=torch.float32, device="cuda")
 
     def timed_cold(fn, reps=20):
-        """Mean duration of single launches that each start after a 1 GiB write has evicted the
-        256 MB last-level cache: every operand streams from HBM."""
+        """Mean duration of single launches that each start after a 1 GiB read has evicted the
+        256 MB last-level cache (with clean lines: no write-back competes with the launch), so every
+        operand streams from HBM."""
         fn()
         evs = []
         for _ in range(reps):
-            flush.fill_(1.0)
+            torch.sum(flush, dim=0, out=flush_sum)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
             fn()
@@ -391,7 +461,8 @@ def main():
     A.csrmv(p, q)
     ebe_vs_csr = float(torch.linalg.norm(qe - q) / torch.linalg.norm(q))
     bytes_ebe = nel * (8 * 9 + 4 * 3) + 16 * n
-    kernels["ebemv_f64"] = dict(us=t_ebe * 1e6, GBs=bytes_ebe / t_ebe / 1e9, rel_diff_vs_csrmv=ebe_vs_csr)
+    kernels["ebemv_f64"] = dict(us=t_ebe * 1e6, GBs=bytes_ebe / t_ebe / 1e9, rel_diff_vs_csrmv=ebe_vs_csr,
+                                layout_bytes=float(E.info.bytes_f64), algorithmic_bytes=float(bytes_ebe))
     kernels["ebemv_f64_cold"] = dict(us=t_ebe_cold * 1e6, GBs=bytes_ebe / t_ebe_cold / 1e9)
     del Ae_t, E
     # Roofline kernel: the upper-triangle Jacobi iteration on the FP32 factors (k_jacobi_upper_ell),
@@ -407,7 +478,6 @@ def main():
     bytes_up = 8 * nnz_u + 16 * n
     kernels["jacobi_upper_f32"] = dict(us=t_up * 1e6, GBs=bytes_up / t_up / 1e9)
     kernels["jacobi_upper_f32_cold"] = dict(us=t_up_cold * 1e6, GBs=bytes_up / t_up_cold / 1e9)
-    del flush
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
@@ -424,7 +494,12 @@ def main():
                             "working set fits the 256 MB last-level cache)",
                     achieved_cold=round(ach_cold, 2), frac_cold=round(ach_cold / HBM_PEAK_GBS, 5),
                     us_per_launch_cold=round(t_up_cold * 1e6, 3),
-                    cold_is="each launch after a 1 GiB write evicts the last-level cache: operands from HBM")
+                    cold_is="each launch after a 1 GiB read evicts the last-level cache: operands from HBM")
+
+    euler = None
+    if a.euler_steps > 0:
+        euler = euler_run(fem, torch, dev, timed, timed_cold, steps=a.euler_steps)
+    del flush, flush_sum
 
     time_int = None
     if a.time_steps > 0:
@@ -496,6 +571,7 @@ def main():
         "csr_lanes_autotune_us": {str(k): round(v, 2) for k, v in lane_us.items()},
         "roofline": roofline,
         "time_integration": time_int,
+        "euler": euler,
         "serving": serving_out,
         "cpu_baseline": cpu,
     }

@@ -208,14 +208,31 @@ int femgpu_ilu0_get_f32(femgpu_csr *A, float *LU, float *Dinv);
 typedef struct femgpu_ebe femgpu_ebe;
 
 /* Element-by-element operator with s x s blocks (s = 3 transport, 9 Navier-Stokes, 12 Euler),
- * lm[nel*s] host array with Dirichlet sink NEQ. Colours the elements for an atomics-free scatter. */
+ * lm[nel*s] host array with Dirichlet sink NEQ, lm[e][ndof*a + d] = unknown of dof d at vertex a
+ * (ndof = s/3; Fill_LM's layout). The product is gathered per node: each node's element rows are
+ * stored together in element order, so the sums keep the reference's order with no atomics. */
 int femgpu_ebe_create(femgpu_ctx *ctx, int nel, int s, int NEQ, const int *lm, femgpu_ebe **out);
 int femgpu_ebe_destroy(femgpu_ebe *E);
-/* A: nel*s*s row-major element matrices (ebe_assembly.c layout); host or device pointer. */
+/* A: nel*s*s row-major element matrices (ebe_assembly.c layout); host or device pointer. Fills the
+ * FP64 operator and its FP32 copy (the mixed-precision inner solver's). */
 int femgpu_ebe_set_values(femgpu_ebe *E, const double *A, int on_device);
-int femgpu_ebe_ncolours(femgpu_ebe *E);
-/* ebemv / ebemvNDOF3 / ebemvNDOF4: Q[0..NEQ) = sum_e A_e P|_e with P[NEQ] read as 0. */
+
+typedef struct {
+    int groups;          /* nodes with at least one unknown (one thread of the product each) */
+    int max_incidences;  /* most elements at one node */
+    int column_bytes;    /* 2: int16 offsets from the node's first unknown; 4: int32 unknowns */
+    int pad;
+    long long bytes_f64; /* bytes one FP64 product moves in this layout (values, columns, P, Q) */
+    long long bytes_f32; /* the same for the FP32 product */
+} femgpu_ebe_info;
+int femgpu_ebe_get_info(femgpu_ebe *E, femgpu_ebe_info *info);
+/* ebemv / ebemvNDOF3 / ebemvNDOF4 (MatrixVector_Operations/EBE/ebemv.c:4-64, ebemvNDOF3.c:3-92,
+ * ebemvNDOF4.c:4-121): Q[0..NEQ) = sum_e A_e P|_e with P[NEQ] read as 0, bit-identical to the
+ * reference's element loop. */
 int femgpu_ebemv(femgpu_ebe *E, const double *d_P, double *d_Q);
+/* The same product on the FP32 copy of the element matrices (no reference counterpart: the
+ * mixed-precision inner solver's operator). */
+int femgpu_ebemv_f32(femgpu_ebe *E, const float *d_P, float *d_Q);
 
 /* =========================================================== BLAS on device vectors */
 
@@ -276,10 +293,12 @@ typedef struct {
 } femgpu_mixed_opts;
 
 /* Mixed precision: FP64 iterative refinement (residual F - A x, update x += d, Givens and least
- * squares in FP64) around an FP32 left-ILU(0)-preconditioned inner GMRES. Stops when
+ * squares in FP64) around an FP32 left-preconditioned inner GMRES. Stops when
  * ||F - A x|| <= tol * ||F|| (FP64-verified). F is the raw (unpreconditioned) right-hand side.
- * Requires sys->csr and sys->ilu (factored unless opt->factor_sweeps > 0). No reference
- * counterpart. */
+ * Two system forms: sys->csr with sys->ilu (ILU(0), factored unless opt->factor_sweeps > 0; the
+ * inner solver uses the FP32 sliced-ELL copy and FP32 factors), or sys->ebe with sys->bdiag (set up
+ * on the current element matrices; the inner solver uses the FP32 element matrices and FP32 inverse
+ * blocks, sweep_mode / factor_sweeps ignored, graphs must be 0). No reference counterpart. */
 int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *sys, const femgpu_mixed_opts *opt,
                        double tol, const double *d_F, double *d_X, femgpu_solve_info *info);
 
@@ -577,6 +596,20 @@ int femgpu_ns_process(femgpu_mesh *M, femgpu_csr *A, double Re, const double *d_
  * d_u, d_a (neq): the state, updated in place; the caller advances t. */
 int femgpu_tt_predictor_old_step(femgpu_mesh *M, femgpu_csr *A, const femgpu_transport_problem *pb,
                                  double dt, double alpha, const femgpu_driver_opts *opt, double *d_u,
+                                 double *d_a, femgpu_driver_info *info);
+
+/* One step of the predictor-multicorrector for the Euler equations (03_Euler_Equations/2D/
+ * 01_CommonFiles/Process.c:4-21 -> 00_CommonFiles/Time_Integration/Predictor_Old.c:53-123) with the
+ * EBE scheme and the BlockDiag preconditioner: u_old = u, u += (1-alpha) dt a, a = 0, then
+ * multicorrections -- Build_M_K_F_SUPG at (u, a) (element matrices into E, residual R), the nodal
+ * block-diagonal blocks (BlockDiag_precond_EBE_setup), and either R := D^-1 R and pgmres with
+ * ebemvNDOF4 and BlockDiag_precond (solver 0) or the mixed solver on the raw R with FP32 element
+ * matrices and blocks (solver 1) for Da; a += Da, u += alpha dt Da -- until the stop criterion.
+ * alpha and dt are pb's. E: femgpu_ebe_create(s = 12) on M's lm; B: femgpu_blockdiag_create(ndof 4)
+ * on M's nodes. Process.c sets a scaling function (setScaling.c) that Predictor_Old never calls,
+ * so none is applied. d_u, d_a (neq): the state, updated in place. */
+int femgpu_eu_predictor_old_step(femgpu_mesh *M, femgpu_ebe *E, femgpu_blockdiag *B,
+                                 const femgpu_euler_problem *pb, const femgpu_driver_opts *opt, double *d_u,
                                  double *d_a, femgpu_driver_info *info);
 
 #ifdef __cplusplus

@@ -12,7 +12,11 @@
  *   orc_build_transport_supg, orc_csr_assemble           -- golden fixtures + TESTE/patch
  *                                                            identities [test_oracle_golden,
  *                                                            test_oracle_math]
- *   orc_build_transport_transient                        -- golden fixtures (transient harness)
+ *   orc_build_transport_transient                        -- the steady fixtures' stiffness plus the
+ *                                                            textbook SUPG mass with du != 0, and the
+ *                                                            transient CAU residual [test_oracle_math]
+ *   orc_blockdiag_setup, orc_blockdiag_apply             -- dense inverse of the summed nodal
+ *                                                            blocks [test_oracle_math]
  *   orc_ebemv, orc_ebemv_block, orc_csrmv                 -- golden fixtures + EBE == CSR +
  *                                                            dense brute force [test_oracle_math]
  *   orc_ilu0_factor, orc_ilu0_apply                      -- golden fixtures + exact LU on
@@ -1144,6 +1148,86 @@ void orc_ilu0_apply(int n, const int *IA, const int *JA, const double *LU, const
     }
 }
 
+/* ====================================================================== nodal block diagonal */
+
+/* 4x4 and 3x3 inverses by cofactors: matrix_Inverse (EBE/BlockDiag_precond_EBE_setup.c:92-147) and
+ * DOF3_matrix_Inverse (EBE/BlockDiagDOF3_precond_EBE_setup.c), same expressions, same order. */
+static void inv4(const double *B, double *R)
+{
+    const double a = B[0], b = B[1], c = B[2], d = B[3], e = B[4], f = B[5], g = B[6], h = B[7], i = B[8],
+                 j = B[9], k = B[10], l = B[11], m = B[12], n = B[13], o = B[14], p = B[15];
+    const double det = (a*f*k*p - a*f*l*o - a*g*j*p + a*g*l*n + a*h*j*o - a*h*k*n
+                        - b*e*k*p + b*e*l*o + b*g*i*p - b*g*l*m - b*h*i*o + b*h*k*m +
+                        c*(e*j*p - e*l*n - f*i*p + f*l*m + h*i*n - h*j*m) +
+                        d*(-e*j*o + e*k*n + f*i*o - f*k*m - g*i*n + g*j*m));
+    const double q = 1.0 / det;
+    R[0] = q*(-h*k*n + g*l*n + h*j*o - f*l*o - g*j*p + f*k*p);
+    R[1] = q*(d*k*n - c*l*n - d*j*o + b*l*o + c*j*p - b*k*p);
+    R[2] = q*(-d*g*n + c*h*n + d*f*o - b*h*o - c*f*p + b*g*p);
+    R[3] = q*(d*g*j - c*h*j - d*f*k + b*h*k + c*f*l - b*g*l);
+    R[4] = q*(h*k*m - g*l*m - h*i*o + e*l*o + g*i*p - e*k*p);
+    R[5] = q*(-d*k*m + c*l*m + d*i*o - a*l*o - c*i*p + a*k*p);
+    R[6] = q*(d*g*m - c*h*m - d*e*o + a*h*o + c*e*p - a*g*p);
+    R[7] = q*(-d*g*i + c*h*i + d*e*k - a*h*k - c*e*l + a*g*l);
+    R[8] = q*(-h*j*m + f*l*m + h*i*n - e*l*n - f*i*p + e*j*p);
+    R[9] = q*(d*j*m - b*l*m - d*i*n + a*l*n + b*i*p - a*j*p);
+    R[10] = q*(-d*f*m + b*h*m + d*e*n - a*h*n - b*e*p + a*f*p);
+    R[11] = q*(d*f*i - b*h*i - d*e*j + a*h*j + b*e*l - a*f*l);
+    R[12] = q*(g*j*m - f*k*m - g*i*n + e*k*n + f*i*o - e*j*o);
+    R[13] = q*(-c*j*m + b*k*m + c*i*n - a*k*n - b*i*o + a*j*o);
+    R[14] = q*(c*j*m - b*g*m - c*e*n + a*g*n + b*e*o - a*f*o);
+    R[15] = q*(-c*f*i + b*g*i + c*e*j - a*g*j - b*e*k + a*f*k);
+}
+
+static void inv3(const double *B, double *R)
+{
+    const double a = B[0], b = B[1], c = B[2], d = B[3], e = B[4], f = B[5], g = B[6], h = B[7], i = B[8];
+    const double det = a*e*i - a*f*h - b*d*i + b*f*g + c*d*h - c*e*g;
+    const double q = 1.0 / det;
+    R[0] = q*(e*i - f*h); R[1] = q*(c*h - b*i); R[2] = q*(b*f - c*e);
+    R[3] = q*(f*g - d*i); R[4] = q*(a*i - c*g); R[5] = q*(c*d - a*f);
+    R[6] = q*(d*h - e*g); R[7] = q*(b*g - a*h); R[8] = q*(a*e - b*d);
+}
+
+void orc_blockdiag_setup(int nnodes, int nel, int ndof, const int *ien, const double *A, double *inv)
+{
+    const int s = 3 * ndof, bb = ndof * ndof;
+    double *B = (double *)calloc((size_t)nnodes * bb, sizeof(double));
+    /* BlockDiag_precond_EBE_setup.c:66-77: the diagonal block of vertex a of every element, added
+     * to its node's block in element order */
+    for (int e = 0; e < nel; e++)
+        for (int a = 0; a < 3; a++) {
+            const int v = ien[3 * e + a];
+            for (int I = 0; I < ndof; I++)
+                for (int J = 0; J < ndof; J++)
+                    B[(size_t)v * bb + ndof * I + J] += A[(size_t)e * s * s + (size_t)(ndof * a + I) * s + ndof * a + J];
+        }
+    for (int v = 0; v < nnodes; v++) {   /* :79-80 */
+        if (ndof == 4) inv4(B + (size_t)v * bb, inv + (size_t)v * bb);
+        else if (ndof == 3) inv3(B + (size_t)v * bb, inv + (size_t)v * bb);
+        else inv[v] = 1.0 / B[v];
+    }
+    free(B);
+}
+
+void orc_blockdiag_apply(int nnodes, int ndof, int NEQ, const int *id, const double *inv, const double *p,
+                         double *z)
+{
+    /* BlockDiag_precond.c:12-26: p[NEQ] = 0, z[id] = inv * p[id] node by node (p == z allowed: a
+     * node reads its own dofs before writing them), z[NEQ] dropped */
+    double x[4];
+    for (int v = 0; v < nnodes; v++) {
+        const int *I = id + (size_t)v * ndof;
+        const double *m = inv + (size_t)v * ndof * ndof;
+        for (int j = 0; j < ndof; j++) x[j] = (I[j] < NEQ) ? p[I[j]] : 0.0;
+        for (int i = 0; i < ndof; i++) {
+            double t = m[ndof * i] * x[0];
+            for (int j = 1; j < ndof; j++) t = t + m[ndof * i + j] * x[j];
+            if (I[i] < NEQ) z[I[i]] = t;
+        }
+    }
+}
+
 /* ====================================================================== solvers */
 
 static void sys_mv(const orc_system *S, const double *v, double *w)
@@ -1156,6 +1240,7 @@ static void sys_mv(const orc_system *S, const double *v, double *w)
 static void sys_precond(const orc_system *S, double *v)
 {
     if (S->precond == 1) orc_ilu0_apply(S->n, S->pIA, S->pJA, S->LU, S->Dinv, v, v);
+    else if (S->precond == 2) orc_blockdiag_apply(S->bd_nnodes, S->bd_ndof, S->n, S->bd_id, S->bd_inv, v, v);
 }
 
 int orc_pgmres(const orc_system *S, int kmax, int lmax, double tol, const double *B, double *X,

@@ -178,6 +178,19 @@ int orc_ilu0_factor(int n, const int *IA, const int *JA, const double *AA, doubl
 void orc_ilu0_apply(int n, const int *IA, const int *JA, const double *LU, const double *Dinv,
                     const double *p, double *z);
 
+/* ------------------------------------------------------------------ nodal block diagonal */
+
+/* BlockDiag_precond_EBE_setup / BlockDiagDOF3_precond_EBE_setup numeric part
+ * (00_CommonFiles/Solvers_and_Preconditioners/EBE/BlockDiag_precond_EBE_setup.c:66-80 and the DOF3
+ * file): per node the ndof x ndof diagonal block of its elements' matrices A[nel*s*s] (s = 3 ndof,
+ * element order), inverted by the reference's cofactor formulas into inv[nnodes*ndof*ndof]. */
+void orc_blockdiag_setup(int nnodes, int nel, int ndof, const int *ien, const double *A, double *inv);
+
+/* BlockDiag_precond / BlockDiagDOF3_precond (BlockDiag_precond.c:3-29): z[id] = inv p[id] per
+ * node, id[nnodes*ndof] = unknown or NEQ (prescribed, read as 0, not written); p == z allowed. */
+void orc_blockdiag_apply(int nnodes, int ndof, int NEQ, const int *id, const double *inv, const double *p,
+                         double *z);
+
 /* ------------------------------------------------------------------ solvers */
 
 /* Operator + preconditioner selection for the oracle's pgmres. */
@@ -189,9 +202,12 @@ typedef struct {
     const double *AA;
     const int *lm, *Perm, *invPerm;  /* EBE operator */
     const double *A;
-    int precond;           /* 0: NOT, 1: ILU(0) from (pIA,pJA,LU,Dinv) */
+    int precond;           /* 0: NOT, 1: ILU(0) from (pIA,pJA,LU,Dinv), 2: nodal block diagonal */
     const int *pIA, *pJA;
     const double *LU, *Dinv;
+    int bd_nnodes, bd_ndof;  /* precond 2: orc_blockdiag_apply(bd_nnodes, bd_ndof, n, bd_id, bd_inv) */
+    const int *bd_id;
+    const double *bd_inv;
 } orc_system;
 
 /* pgmres (00_CommonFiles/Solvers_and_Preconditioners/pgmres.c:5-200): restarted, left

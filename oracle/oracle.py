@@ -40,7 +40,8 @@ class System(C.Structure):
                 ("lm", C.c_void_p), ("Perm", C.c_void_p), ("invPerm", C.c_void_p),
                 ("A", C.c_void_p), ("precond", C.c_int),
                 ("pIA", C.c_void_p), ("pJA", C.c_void_p), ("LU", C.c_void_p),
-                ("Dinv", C.c_void_p)]
+                ("Dinv", C.c_void_p), ("bd_nnodes", C.c_int), ("bd_ndof", C.c_int),
+                ("bd_id", C.c_void_p), ("bd_inv", C.c_void_p)]
 
 
 def build(force: bool = False) -> str:
@@ -95,6 +96,8 @@ def lib():
         L.orc_ilu0_factor.argtypes = [C.c_int, i32p, i32p, f64p, f64p, f64p]
         L.orc_ilu0_factor.restype = C.c_int
         L.orc_ilu0_apply.argtypes = [C.c_int, i32p, i32p, f64p, f64p, f64p, f64p]
+        L.orc_blockdiag_setup.argtypes = [C.c_int, C.c_int, C.c_int, i32p, f64p, f64p]
+        L.orc_blockdiag_apply.argtypes = [C.c_int, C.c_int, C.c_int, i32p, f64p, f64p, f64p]
         L.orc_pgmres.argtypes = [C.POINTER(System), C.c_int, C.c_int, C.c_double, f64p, f64p,
                                  C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.orc_pgmres.restype = C.c_int
@@ -297,8 +300,26 @@ class SystemArrays:
     arrays: dict = field(default_factory=dict)
 
 
+def blockdiag_ids(type_, invP, ndof, NEQ):
+    """Per node and dof: the unknown, or NEQ for a prescribed dof (BlockDiag_precond_EBE_setup.c:28-52)."""
+    return np.where(np.asarray(type_) == 0, NEQ, np.asarray(invP)).astype(np.int32)
+
+
+def blockdiag_setup(ien, A, nnodes, ndof):
+    """Inverse nodal diagonal blocks of the element matrices A (nel*s*s, element order)."""
+    inv = np.empty(nnodes * ndof * ndof)
+    lib().orc_blockdiag_setup(nnodes, len(ien) // 3, ndof, _i(ien), _f(A), inv)
+    return inv
+
+
+def blockdiag_apply(ids, inv, p, NEQ, ndof):
+    z = np.array(p[:NEQ], dtype=np.float64, copy=True)
+    lib().orc_blockdiag_apply(len(ids) // ndof, ndof, NEQ, _i(ids), _f(inv), z, z)
+    return z
+
+
 def make_system(kind, n, *, IA=None, JA=None, AA=None, lm=None, Perm=None, invPerm=None, A=None,
-                nel=0, s=3, ilu=None):
+                nel=0, s=3, ilu=None, bdiag=None):
     keep = {}
 
     def ptr(name, a, dt):
@@ -318,6 +339,11 @@ def make_system(kind, n, *, IA=None, JA=None, AA=None, lm=None, Perm=None, invPe
         S.precond = 1
         S.pIA, S.pJA = ptr("pIA", pIA, np.int32), ptr("pJA", pJA, np.int32)
         S.LU, S.Dinv = ptr("LU", LU, np.float64), ptr("Dinv", Dinv, np.float64)
+    elif bdiag is not None:
+        ids, inv, ndof = bdiag
+        S.precond = 2
+        S.bd_nnodes, S.bd_ndof = len(ids) // ndof, ndof
+        S.bd_id, S.bd_inv = ptr("bd_id", ids, np.int32), ptr("bd_inv", inv, np.float64)
     else:
         S.precond = 0
     S._keep = keep

@@ -34,7 +34,11 @@ NS_CF   := $(CF)/Allocation_Operations/mycalloc.c $(CF)/BLAS_Operations/dzero.c 
 
 EU      := $(REF)/03_Euler_Equations/2D/01_CommonFiles
 EU_SRC  := $(EU)/Build_M_K_F_SUPG.c $(EU)/eval_U_dU.c $(EU)/Ax_Ay_calculations.c $(EU)/Delta_CAU.c \
-           $(EU)/setzeros.c $(EU)/ebe_assembly.c $(EU)/csr_assembly.c $(EU)/Fill_LM.c $(EU)/csr_Initialization.c
+           $(EU)/setzeros.c $(EU)/ebe_assembly.c $(EU)/csr_assembly.c $(EU)/Fill_LM.c $(EU)/csr_Initialization.c \
+           $(CF)/Solvers_and_Preconditioners/EBE/BlockDiag_precond_EBE_setup.c \
+           $(CF)/Solvers_and_Preconditioners/BlockDiag_precond.c $(CF)/Solvers_and_Preconditioners/NO_precond.c \
+           $(CF)/Solvers_and_Preconditioners/pgmres.c $(CF)/MatrixVector_Operations/EBE/ebemvNDOF4.c \
+           $(CF)/BLAS_Operations/daxpy.c $(CF)/BLAS_Operations/dcopy.c $(CF)/BLAS_Operations/dscal.c
 
 all: $(OUT)/st_harness $(OUT)/st_harness_ofast $(OUT)/ns_harness $(OUT)/eu_harness $(OUT)/meshGenerator
 

@@ -10,12 +10,20 @@
  * the problem callbacks (gamma, rhopresc, v1presc, v2presc, epresc) and MPI_VectorUpdate for one
  * rank; prescribed values are per-node tables looked up by exact coordinates.
  *
- * Input (binary, little endian): int32 header[16] {nn, nel, ebe, reorder, dimensionless, shock},
- * float64 params[16] {gamma, Mach, alpha, dt, tolerance}, then x[nn], y[nn], type[4nn] (int32),
+ * With the EBE scheme and header[6] = kmax > 0 it then runs the linear solve of the Euler
+ * predictor (Predictor_Old.c:95-96): the reference's BlockDiag_precond_EBE_setup (nodal blocks,
+ * inverses, R := D^-1 R) and pgmres with ebemvNDOF4 and BlockDiag_precond.
+ *
+ * Input (binary, little endian): int32 header[16] {nn, nel, ebe, reorder, dimensionless, shock,
+ * kmax, lmax}, float64 params[16] {gamma, Mach, alpha, dt, tolerance, solver tolerance}, then
+ * x[nn], y[nn], type[4nn] (int32),
  * ien[3nel] (int32), presc[4nn], state[4nn], incr[4nn] (float64; values of the unknown dofs).
  * Output: records {char name[32]; int32 kind (0 int32, 1 float64); int64 count; payload}.
  */
 #include "EulerEquations.h"
+#include "../../../00_CommonFiles/Solvers_and_Preconditioners/preconditioners.h"
+#include "../../../00_CommonFiles/Solvers_and_Preconditioners/solvers.h"
+#include "../../../00_CommonFiles/MatrixVector_Operations/matvec.h"
 
 int Fill_LM(ParametersType *, int **, NodeType *, ElementType *);
 void csr_Initialization(ParametersType *, MatrixDataType *, FemStructsType *);
@@ -148,6 +156,26 @@ int main(int argc, char **argv)
     Build_M_K_F_SUPG(P, MD, FS, FF);
     if (ebe) rec("A", 1, 144L * nel, MD->Aaux); else rec("AA", 1, P->nnzero + 1, MD->AA);
     rec("R", 1, neq + 1, R);
+    if (ebe && hd[6] > 0) {
+        /* the Euler linear solve of Predictor_Old.c:95-96 with the EBE scheme: BlockDiag_precond_EBE_setup
+         * (nodal blocks, their inverses, R := D^-1 R), then pgmres with ebemvNDOF4 and BlockDiag_precond */
+        BlockDiag_precond_EBE_setup(P, MD, FS, 1, R);
+        rec("invBlockDiag", 1, 16L * nn, MD->invBlockDiag[0]);
+        rec("MinvR", 1, neq, R);
+        FF->mv = ebemvNDOF4;
+        FF->precond = BlockDiag_precond;
+        FF->precondR = NO_precond;
+        P->KrylovBasisVectorsQuantity = hd[6];
+        P->LinearMaxIter = hd[7];
+        P->SolverTolerance = pr[5];
+        P->iterations = 0;
+        double *X = calloc(neq + 1, sizeof(double));
+        pgmres(P, MD, FS, FF, R, X);
+        int it = P->iterations;
+        rec("iters", 0, 1, &it);
+        rec("X", 1, neq, X);
+        free(X);
+    }
     fclose(g_out);
     MPI_Finalize();
     return 0;

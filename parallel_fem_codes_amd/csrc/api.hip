@@ -445,91 +445,6 @@ extern "C" int femgpu_ilu0_get(femgpu_csr *A, double *LU, double *Dinv)
     return FEMGPU_OK;
 }
 
-// ------------------------------------------------------------------------------- EBE
-
-extern "C" int femgpu_ebe_create(femgpu_ctx *ctx, int nel, int s, int NEQ, const int *lm, femgpu_ebe **out)
-{
-    FG_CHECK(ctx && out && nel >= 0 && NEQ >= 0 && lm && (s == 3 || s == 9 || s == 12), FEMGPU_ERR_ARG,
-             "femgpu_ebe_create: bad arguments (s must be 3, 9 or 12)");
-    for (size_t q = 0; q < static_cast<size_t>(nel) * s; ++q)
-        FG_CHECK(lm[q] >= 0 && lm[q] <= NEQ, FEMGPU_ERR_ARG, "femgpu_ebe_create: lm entry %d out of [0, NEQ]", lm[q]);
-    std::vector<int> colour(nel);
-    const int nc = femgpu_ebe_colour(nel, s, NEQ, lm, colour.data());
-    if (nc < 0) return nc;
-    auto *E = new femgpu_ebe();
-    E->ctx = ctx;
-    E->nel = nel;
-    E->s = s;
-    E->NEQ = NEQ;
-    E->ncolours = nc;
-    E->colour_ptr.assign(nc + 1, 0);
-    for (int e = 0; e < nel; ++e) E->colour_ptr[colour[e] + 1]++;
-    for (int c = 0; c < nc; ++c) E->colour_ptr[c + 1] += E->colour_ptr[c];
-    std::vector<int> fill(E->colour_ptr.begin(), E->colour_ptr.end() - 1), elem(nel);
-    if (s == 3)   // gathered per unknown: element order
-        for (int e = 0; e < nel; ++e) elem[e] = e;
-    else
-        for (int e = 0; e < nel; ++e) elem[fill[colour[e]]++] = e;
-    std::vector<int> lmc(static_cast<size_t>(nel) * s);
-    for (int k = 0; k < nel; ++k)
-        for (int a = 0; a < s; ++a) lmc[static_cast<size_t>(k) * s + a] = lm[static_cast<size_t>(elem[k]) * s + a];
-    int rc = E->elem.upload(elem.data(), nel, ctx->stream);
-    if (!rc) rc = E->lm.upload(lmc.data(), lmc.size(), ctx->stream);
-    if (!rc) rc = E->A.alloc(static_cast<size_t>(nel) * s * s);
-    std::vector<int> rptr, rsrc, rdst, rec;   // live until the synchronisation below
-    if (!rc && s == 3) {
-        std::vector<std::pair<int, int>> pairs;
-        pairs.reserve(static_cast<size_t>(nel) * 3);
-        for (int e = 0; e < nel; ++e)
-            for (int a = 0; a < 3; ++a)
-                if (lm[3 * static_cast<size_t>(e) + a] < NEQ) pairs.emplace_back(lm[3 * static_cast<size_t>(e) + a], 3 * e + a);
-        csr_map(NEQ, pairs, rptr, rsrc, rdst);
-        rec.resize(4 * rsrc.size());
-        for (size_t q = 0; q < rsrc.size(); ++q) {
-            const int ea = rsrc[q], e = ea / 3;
-            rec[4 * q] = ea;
-            for (int b = 0; b < 3; ++b) rec[4 * q + 1 + b] = lm[3 * static_cast<size_t>(e) + b];
-        }
-        rc = E->rptr.upload(rptr.data(), rptr.size(), ctx->stream);
-        if (!rc) rc = E->rdst.upload(rdst.data(), rdst.size(), ctx->stream);
-        if (!rc) rc = E->rrec.upload(rec.data(), rec.size(), ctx->stream);
-    }
-    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = FEMGPU_ERR_HIP;
-    if (rc) {
-        delete E;
-        return rc;
-    }
-    *out = E;
-    return FEMGPU_OK;
-}
-
-extern "C" int femgpu_ebe_destroy(femgpu_ebe *E)
-{
-    delete E;
-    return FEMGPU_OK;
-}
-
-extern "C" int femgpu_ebe_ncolours(femgpu_ebe *E) { return E ? E->ncolours : FEMGPU_ERR_ARG; }
-
-extern "C" int femgpu_ebe_set_values(femgpu_ebe *E, const double *A, int on_device)
-{
-    FG_CHECK(E && A, FEMGPU_ERR_ARG, "femgpu_ebe_set_values: bad arguments");
-    femgpu_ctx *ctx = E->ctx;
-    const size_t total = static_cast<size_t>(E->nel) * E->s * E->s;
-    if (on_device) return launch_ebe_gather_values(ctx, E, A);
-    DevBuf<double> tmp;
-    FG_TRY(tmp.upload(A, total, ctx->stream));
-    FG_TRY(launch_ebe_gather_values(ctx, E, tmp.p));
-    FG_HIP(hipStreamSynchronize(ctx->stream));
-    return FEMGPU_OK;
-}
-
-extern "C" int femgpu_ebemv(femgpu_ebe *E, const double *d_P, double *d_Q)
-{
-    FG_CHECK(E && d_P && d_Q && d_P != d_Q, FEMGPU_ERR_ARG, "femgpu_ebemv: bad arguments");
-    return launch_ebemv(E->ctx, E, d_P, d_Q);
-}
-
 // ------------------------------------------------------------------------------- BLAS
 
 extern "C" int femgpu_ddot(femgpu_ctx *ctx, int n, const double *x, const double *y, double *result)

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "common.h"
+#include "device_util.h"
 
 struct femgpu_blockdiag {
     femgpu_ctx *ctx = nullptr;
@@ -21,66 +22,78 @@ struct femgpu_blockdiag {
     fg::DevBuf<int> ien;                 // nel*3
     fg::DevBuf<int> inc_ptr, inc_src, inc_dst;   // node incidences e*3+a in element order
     fg::DevBuf<double> inv;              // nnodes*ndof^2 inverse blocks (row-major)
+    fg::DevBuf<float> inv32;             // their FP32 copy (mixed-precision inner solver)
     bool ready = false;
 };
 
 namespace fg {
 
-// Inverse of a small dense block by Gauss-Jordan elimination with partial pivoting (the blocks
-// are the diagonal blocks of stabilised element matrices; pivoting keeps the 4x4 Euler blocks
-// accurate when a leading entry is small).
+// Inverse of a nodal block by its cofactor expansion, with the reference's own expressions
+// (matrix_Inverse, EBE/BlockDiag_precond_EBE_setup.c:92-147 for 4x4; DOF3_matrix_Inverse,
+// EBE/BlockDiagDOF3_precond_EBE_setup.c for 3x3), evaluated left to right with contraction off, so
+// the inverse blocks are the reference's bit for bit.
 template <int ND>
-__device__ __forceinline__ void block_inverse(double (&a)[ND][ND], double (&inv)[ND][ND])
+__device__ __forceinline__ void block_inverse(const double (&m)[ND][ND], double (&r)[ND][ND]);
+
+template <>
+__device__ __forceinline__ void block_inverse<1>(const double (&m)[1][1], double (&r)[1][1])
 {
-#pragma unroll
-    for (int i = 0; i < ND; ++i)
-#pragma unroll
-        for (int j = 0; j < ND; ++j) inv[i][j] = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-    for (int c = 0; c < ND; ++c) {
-        int piv = c;
-        double best = fabs(a[c][c]);
-#pragma unroll
-        for (int r = c + 1; r < ND; ++r)
-            if (fabs(a[r][c]) > best) {
-                best = fabs(a[r][c]);
-                piv = r;
-            }
-#pragma unroll
-        for (int r = c + 1; r < ND; ++r)
-            if (r == piv)
-#pragma unroll
-                for (int j = 0; j < ND; ++j) {
-                    double t = a[c][j];
-                    a[c][j] = a[r][j];
-                    a[r][j] = t;
-                    t = inv[c][j];
-                    inv[c][j] = inv[r][j];
-                    inv[r][j] = t;
-                }
-        const double d = 1.0 / a[c][c];
-#pragma unroll
-        for (int j = 0; j < ND; ++j) {
-            a[c][j] *= d;
-            inv[c][j] *= d;
-        }
-#pragma unroll
-        for (int r = 0; r < ND; ++r)
-            if (r != c) {
-                const double f = a[r][c];
-#pragma unroll
-                for (int j = 0; j < ND; ++j) {
-                    a[r][j] -= f * a[c][j];
-                    inv[r][j] -= f * inv[c][j];
-                }
-            }
-    }
+    r[0][0] = 1.0 / m[0][0];
+}
+
+template <>
+__device__ __forceinline__ void block_inverse<3>(const double (&m)[3][3], double (&r)[3][3])
+{
+#pragma clang fp contract(off)
+    const double a = m[0][0], b = m[0][1], c = m[0][2], d = m[1][0], e = m[1][1], f = m[1][2], g = m[2][0],
+                 h = m[2][1], i = m[2][2];
+    const double det = a * e * i - a * f * h - b * d * i + b * f * g + c * d * h - c * e * g;
+    const double q = 1.0 / det;
+    r[0][0] = q * (e * i - f * h);
+    r[0][1] = q * (c * h - b * i);
+    r[0][2] = q * (b * f - c * e);
+    r[1][0] = q * (f * g - d * i);
+    r[1][1] = q * (a * i - c * g);
+    r[1][2] = q * (c * d - a * f);
+    r[2][0] = q * (d * h - e * g);
+    r[2][1] = q * (b * g - a * h);
+    r[2][2] = q * (a * e - b * d);
+}
+
+template <>
+__device__ __forceinline__ void block_inverse<4>(const double (&M)[4][4], double (&r)[4][4])
+{
+#pragma clang fp contract(off)
+    const double a = M[0][0], b = M[0][1], c = M[0][2], d = M[0][3], e = M[1][0], f = M[1][1], g = M[1][2],
+                 h = M[1][3], i = M[2][0], j = M[2][1], k = M[2][2], l = M[2][3], m = M[3][0], n = M[3][1],
+                 o = M[3][2], p = M[3][3];
+    const double det = (a * f * k * p - a * f * l * o - a * g * j * p + a * g * l * n + a * h * j * o - a * h * k * n
+                        - b * e * k * p + b * e * l * o + b * g * i * p - b * g * l * m - b * h * i * o + b * h * k * m +
+                        c * (e * j * p - e * l * n - f * i * p + f * l * m + h * i * n - h * j * m) +
+                        d * (-e * j * o + e * k * n + f * i * o - f * k * m - g * i * n + g * j * m));
+    const double q = 1.0 / det;
+    r[0][0] = q * (-h * k * n + g * l * n + h * j * o - f * l * o - g * j * p + f * k * p);
+    r[0][1] = q * (d * k * n - c * l * n - d * j * o + b * l * o + c * j * p - b * k * p);
+    r[0][2] = q * (-d * g * n + c * h * n + d * f * o - b * h * o - c * f * p + b * g * p);
+    r[0][3] = q * (d * g * j - c * h * j - d * f * k + b * h * k + c * f * l - b * g * l);
+    r[1][0] = q * (h * k * m - g * l * m - h * i * o + e * l * o + g * i * p - e * k * p);
+    r[1][1] = q * (-d * k * m + c * l * m + d * i * o - a * l * o - c * i * p + a * k * p);
+    r[1][2] = q * (d * g * m - c * h * m - d * e * o + a * h * o + c * e * p - a * g * p);
+    r[1][3] = q * (-d * g * i + c * h * i + d * e * k - a * h * k - c * e * l + a * g * l);
+    r[2][0] = q * (-h * j * m + f * l * m + h * i * n - e * l * n - f * i * p + e * j * p);
+    r[2][1] = q * (d * j * m - b * l * m - d * i * n + a * l * n + b * i * p - a * j * p);
+    r[2][2] = q * (-d * f * m + b * h * m + d * e * n - a * h * n - b * e * p + a * f * p);
+    r[2][3] = q * (d * f * i - b * h * i - d * e * j + a * h * j + b * e * l - a * f * l);
+    r[3][0] = q * (g * j * m - f * k * m - g * i * n + e * k * n + f * i * o - e * j * o);
+    r[3][1] = q * (-c * j * m + b * k * m + c * i * n - a * k * n - b * i * o + a * j * o);
+    r[3][2] = q * (c * j * m - b * g * m - c * e * n + a * g * n + b * e * o - a * f * o);
+    r[3][3] = q * (-c * f * i + b * g * i + c * e * j - a * g * j - b * e * k + a * f * k);
 }
 
 template <int ND>
 __global__ __launch_bounds__(256) void k_bd_setup(int nnodes, const int *__restrict__ dst, const int *__restrict__ ptr,
                                                   const int *__restrict__ src, const double *__restrict__ Ae,
-                                                  double *__restrict__ inv)
+                                                  double *__restrict__ inv, float *__restrict__ inv32)
 {
 #pragma clang fp contract(off)
     constexpr int S = 3 * ND;
@@ -101,33 +114,36 @@ __global__ __launch_bounds__(256) void k_bd_setup(int nnodes, const int *__restr
     }
     double r[ND][ND];
     block_inverse<ND>(b, r);
-    double *o = inv + static_cast<size_t>(dst[k]) * ND * ND;
+    const size_t o = static_cast<size_t>(dst[k]) * ND * ND;
 #pragma unroll
     for (int i = 0; i < ND; ++i)
 #pragma unroll
-        for (int j = 0; j < ND; ++j) o[i * ND + j] = r[i][j];
+        for (int j = 0; j < ND; ++j) {
+            inv[o + i * ND + j] = r[i][j];
+            inv32[o + i * ND + j] = static_cast<float>(r[i][j]);
+        }
 }
 
-template <int ND>
+template <typename T, int ND>
 __global__ __launch_bounds__(256) void k_bd_apply(int nnodes, int NEQ, const int *__restrict__ id,
-                                                  const double *__restrict__ inv, const double *p, double *z,
+                                                  const T *__restrict__ inv, const T *p, T *z,
                                                   const int *__restrict__ skip)
 {
 #pragma clang fp contract(off)
     if (skip && *skip) return;
-    const int v = blockIdx.x * 256 + threadIdx.x;
+    const int v = xcd_block() * 256 + threadIdx.x;
     if (v >= nnodes) return;
     int eq[ND];
-    double x[ND];
+    T x[ND];
 #pragma unroll
     for (int j = 0; j < ND; ++j) {
         eq[j] = id[v * ND + j];
-        x[j] = eq[j] < NEQ ? p[eq[j]] : 0.0;
+        x[j] = eq[j] < NEQ ? p[eq[j]] : T(0);
     }
-    const double *m = inv + static_cast<size_t>(v) * ND * ND;
+    const T *m = inv + static_cast<size_t>(v) * ND * ND;
 #pragma unroll
     for (int i = 0; i < ND; ++i) {
-        double s = m[i * ND] * x[0];
+        T s = m[i * ND] * x[0];
 #pragma unroll
         for (int j = 1; j < ND; ++j) s = s + m[i * ND + j] * x[j];
         if (eq[i] < NEQ) z[eq[i]] = s;
@@ -176,6 +192,14 @@ __global__ __launch_bounds__(256) void k_block_scaling(int nel, int NEQ, const i
         }
 }
 
+int blockdiag_size_check(const femgpu_blockdiag *B, int NEQ, int nnodes, int ndof)
+{
+    FG_CHECK(B->NEQ == NEQ && B->nnodes == nnodes && B->ndof == ndof, FEMGPU_ERR_ARG,
+             "block-diagonal preconditioner: built for %d nodes x %d dofs, NEQ %d; the system has %d x %d, NEQ %d",
+             B->nnodes, B->ndof, B->NEQ, nnodes, ndof, NEQ);
+    return FEMGPU_OK;
+}
+
 int blockdiag_check(const femgpu_blockdiag *B, int n)
 {
     FG_CHECK(B->ready, FEMGPU_ERR_STATE, "block-diagonal preconditioner: not set up");
@@ -183,18 +207,29 @@ int blockdiag_check(const femgpu_blockdiag *B, int n)
     return FEMGPU_OK;
 }
 
-int launch_blockdiag_apply(femgpu_ctx *ctx, const femgpu_blockdiag *B, const double *p, double *z, const int *skip)
+template <typename T>
+static int bd_apply(femgpu_ctx *ctx, const femgpu_blockdiag *B, const T *inv, const T *p, T *z, const int *skip)
 {
     FG_CHECK(B->ready, FEMGPU_ERR_STATE, "block-diagonal preconditioner: not set up");
-    const int grid = (B->nnodes + 255) / 256;
+    const int grid = xcd_grid((B->nnodes + 255) / 256);
     if (B->nnodes == 0) return FEMGPU_OK;
     switch (B->ndof) {
-    case 1: k_bd_apply<1><<<grid, 256, 0, ctx->stream>>>(B->nnodes, B->NEQ, B->id.p, B->inv.p, p, z, skip); break;
-    case 3: k_bd_apply<3><<<grid, 256, 0, ctx->stream>>>(B->nnodes, B->NEQ, B->id.p, B->inv.p, p, z, skip); break;
-    default: k_bd_apply<4><<<grid, 256, 0, ctx->stream>>>(B->nnodes, B->NEQ, B->id.p, B->inv.p, p, z, skip); break;
+    case 1: k_bd_apply<T, 1><<<grid, 256, 0, ctx->stream>>>(B->nnodes, B->NEQ, B->id.p, inv, p, z, skip); break;
+    case 3: k_bd_apply<T, 3><<<grid, 256, 0, ctx->stream>>>(B->nnodes, B->NEQ, B->id.p, inv, p, z, skip); break;
+    default: k_bd_apply<T, 4><<<grid, 256, 0, ctx->stream>>>(B->nnodes, B->NEQ, B->id.p, inv, p, z, skip); break;
     }
     FG_HIP(hipGetLastError());
     return FEMGPU_OK;
+}
+
+int launch_blockdiag_apply(femgpu_ctx *ctx, const femgpu_blockdiag *B, const double *p, double *z, const int *skip)
+{
+    return bd_apply<double>(ctx, B, B->inv.p, p, z, skip);
+}
+
+int launch_blockdiag_apply32(femgpu_ctx *ctx, const femgpu_blockdiag *B, const float *p, float *z, const int *skip)
+{
+    return bd_apply<float>(ctx, B, B->inv32.p, p, z, skip);
 }
 
 }  // namespace fg
@@ -236,6 +271,7 @@ extern "C" int femgpu_blockdiag_create(femgpu_ctx *ctx, int nnodes, int nel, int
     if (!rc) rc = B->inc_src.upload(src.data(), src.size(), s);
     if (!rc) rc = B->inc_dst.upload(dst.data(), dst.size(), s);
     if (!rc) rc = B->inv.alloc(static_cast<size_t>(nnodes) * ndof * ndof);
+    if (!rc) rc = B->inv32.alloc(static_cast<size_t>(nnodes) * ndof * ndof);
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = FEMGPU_ERR_HIP;
     if (rc) {
         delete B;
@@ -258,9 +294,9 @@ extern "C" int femgpu_blockdiag_setup(femgpu_blockdiag *B, const double *d_Ae)
     const int grid = (B->nnodes + 255) / 256;
     if (B->nnodes > 0) {
         switch (B->ndof) {
-        case 1: k_bd_setup<1><<<grid, 256, 0, ctx->stream>>>(B->nnodes, B->inc_dst.p, B->inc_ptr.p, B->inc_src.p, d_Ae, B->inv.p); break;
-        case 3: k_bd_setup<3><<<grid, 256, 0, ctx->stream>>>(B->nnodes, B->inc_dst.p, B->inc_ptr.p, B->inc_src.p, d_Ae, B->inv.p); break;
-        default: k_bd_setup<4><<<grid, 256, 0, ctx->stream>>>(B->nnodes, B->inc_dst.p, B->inc_ptr.p, B->inc_src.p, d_Ae, B->inv.p); break;
+        case 1: k_bd_setup<1><<<grid, 256, 0, ctx->stream>>>(B->nnodes, B->inc_dst.p, B->inc_ptr.p, B->inc_src.p, d_Ae, B->inv.p, B->inv32.p); break;
+        case 3: k_bd_setup<3><<<grid, 256, 0, ctx->stream>>>(B->nnodes, B->inc_dst.p, B->inc_ptr.p, B->inc_src.p, d_Ae, B->inv.p, B->inv32.p); break;
+        default: k_bd_setup<4><<<grid, 256, 0, ctx->stream>>>(B->nnodes, B->inc_dst.p, B->inc_ptr.p, B->inc_src.p, d_Ae, B->inv.p, B->inv32.p); break;
         }
         FG_HIP(hipGetLastError());
     }

@@ -162,6 +162,7 @@ struct femgpu_ctx {
     fg::DevBuf<float> gm_basis32, gm_w32;
     fg::DevBuf<int> gm_flags;
     fg::DevBuf<double> drv_ws;         // nonlinear/time drivers: F, X, uold
+    fg::DevBuf<double> drv_ae;         // Euler driver: element matrices in element order
     volatile int *h_status = nullptr;  // pinned, device-visible solver status words
     int *d_status = nullptr;
     // captured Arnoldi steps of the FP32 inner solver (one executable graph per step), the
@@ -222,14 +223,27 @@ struct femgpu_csr {
 
 struct femgpu_ebe {
     femgpu_ctx *ctx = nullptr;
-    int nel = 0, s = 0, NEQ = 0, ncolours = 0;
-    std::vector<int> colour_ptr;    // elements of colour c: [colour_ptr[c], colour_ptr[c+1])
-    fg::DevBuf<int> elem;           // storage position -> element id (colour-sorted; identity for s = 3)
-    fg::DevBuf<int> lm;             // lm in storage order
-    fg::DevBuf<double> A;           // element matrices in storage order
-    // s = 3: the product is gathered per unknown. List k (for unknown rdst[k]) holds the element rows
-    // (e, a) with lm[e][a] = rdst[k], in element order, as records {3e + a, lm[e][0..2]}
-    fg::DevBuf<int> rptr, rdst, rrec;
+    int nel = 0, s = 0, ndof = 0, NEQ = 0;
+    // The product is gathered per node: group g (a node with at least one unknown; groups in order
+    // of the node's first unknown) sums, in element order, the ndof element rows of each element at
+    // that node, so Q is written once, with no colouring, atomics or read-modify-write. Groups go
+    // in slices of 64 (one lane each); slice sl owns incidence rows [iptr[sl], iptr[sl+1]) (its
+    // longest list). Incidence row r, lane l: value (d, b) at ((r*ndof + d)*s + b)*64 + l, column b
+    // at (r*s + b)*64 + l, so every load of a wavefront is one contiguous run.
+    int ngroups = 0, nrows = 0, maxinc = 0;
+    int col16 = 0;                  // 1: int16 column offsets from the group's first unknown
+    bool zero_q = false;            // some unknown belongs to no element: Q is cleared first
+    std::vector<int> hiptr;
+    fg::DevBuf<int> iptr;           // slice -> first incidence row (nslices + 1)
+    fg::DevBuf<int> len;            // group -> incidence count
+    fg::DevBuf<int> eq;             // (slice*ndof + d)*64 + lane -> unknown of dof d (NEQ: prescribed)
+    fg::DevBuf<int> key;            // group -> its first unknown (origin of the int16 offsets)
+    fg::DevBuf<short> col16v;       // offsets (-32768: the Dirichlet sink)
+    fg::DevBuf<int> col32v;         // unknowns (NEQ: the sink)
+    fg::DevBuf<int> src;            // incidence slot -> e*s*s + ndof*a*s (its rows in element order), -1 padding
+    fg::DevBuf<double> val;         // FP64 values in the gathered layout
+    fg::DevBuf<float> val32;        // FP32 copy (mixed-precision inner solver)
+    bool has_values = false;
 };
 
 struct femgpu_mesh {
@@ -279,6 +293,7 @@ int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A);
 int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const float *rhs, const float *in,
                               float *out);
 int blockdiag_check(const femgpu_blockdiag *B, int n);
+int blockdiag_size_check(const femgpu_blockdiag *B, int NEQ, int nnodes, int ndof);
 int launch_blockdiag_apply(femgpu_ctx *ctx, const femgpu_blockdiag *B, const double *p, double *z, const int *skip);
 int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
 int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip);
@@ -290,8 +305,9 @@ bool sell64_possible(const femgpu_csr *A);   // sliced-ELL copy present and narr
 bool sell64_active(const femgpu_csr *A);     // the FP64 kernels read sell_val64
 int set_spmv_lanes(femgpu_csr *A, int lanes);   // 0: FP64 sliced ELL (allocates and fills sell_val64)
 int launch_to_f32(femgpu_ctx *ctx, const double *src, float *dst, size_t n);
-int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *Q);
-int launch_ebe_gather_values(femgpu_ctx *ctx, femgpu_ebe *E, const double *A_elem_order);
+int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *Q, const int *skip = nullptr);
+int launch_ebemv32(femgpu_ctx *ctx, const femgpu_ebe *E, const float *P, float *Q, const int *skip = nullptr);
+int launch_blockdiag_apply32(femgpu_ctx *ctx, const femgpu_blockdiag *B, const float *p, float *z, const int *skip);
 
 int dot64(femgpu_ctx *ctx, int n, const double *x, const double *y, double *d_out);
 // Element-block meshes (assembly_ns.hip): creation and the residual / CSR value gathers.

@@ -2,8 +2,10 @@
 // (01_SS_Transport_Equation/2D/01_CommonFiles/Process.c:29-86), the Navier-Stokes Picard loop
 // (04_SS_NavierStokes_Equations/2D/01_CommonFiles/Process.c:27-52) and one step of the
 // predictor-multicorrector (00_CommonFiles/Time_Integration/Predictor_Old.c:53-123 with the stop
-// criteria of Criteria.c:25-41). The state vectors stay in device memory; per nonlinear iteration
-// the host reads back only the two norms the stopping tests need.
+// criteria of Criteria.c:25-41) for the transient transport and for the Euler equations (the
+// latter with the EBE scheme and the nodal block-diagonal preconditioner). The state vectors stay
+// in device memory; per nonlinear iteration the host reads back only the two norms the stopping
+// tests need.
 //
 // Each linear solve is either the reference's pair precond_setup + solver (ILU(0) factorisation of
 // the assembled CSR values, F := M^-1 F, pgmres on the preconditioned system: solver = 0) or the
@@ -171,6 +173,71 @@ extern "C" int femgpu_tt_predictor_old_step(femgpu_mesh *M, femgpu_csr *A, const
         norm_Da = norm(ctx, neq, w.X, rc);
         FG_TRY(rc);
         // StopByNorm / StopByIterations (Criteria.c:25-41)
+        if (o->stop_by_norm ? (norm_Da < o->nonlinear_tol * norm_a || i >= o->nonlinear_maxiter)
+                            : (i >= o->nonlinear_maxiter))
+            break;
+    }
+    if (info) {
+        info->iterations = i;
+        info->norm_a = norm_a;
+        info->norm_Da = norm_Da;
+        if (info->status == 0 && o->stop_by_norm && !(norm_Da < o->nonlinear_tol * norm_a)) info->status = 1;
+    }
+    return FEMGPU_OK;
+}
+
+// Euler linear solve of one multicorrection (Predictor_Old.c:95-96 with the EBE scheme): the
+// nodal block-diagonal blocks from the element matrices (BlockDiag_precond_EBE_setup), then
+// F := D^-1 F and pgmres with the gathered 12x12 product and BlockDiag_precond (solver 0), or the
+// mixed solver on the raw F with the FP32 copies of both (solver 1).
+static int euler_solve(femgpu_ctx *ctx, femgpu_ebe *E, femgpu_blockdiag *B, const femgpu_driver_opts *o,
+                       const double *Ae, double *F, double *X, femgpu_solve_info *si)
+{
+    femgpu_system S = {nullptr, E, nullptr, B};
+    FG_TRY(femgpu_ebe_set_values(E, Ae, 1));
+    FG_TRY(femgpu_blockdiag_setup(B, Ae));
+    if (o->solver == 0) {
+        FG_TRY(femgpu_blockdiag_apply(B, F, F));
+        return femgpu_pgmres(ctx, &S, o->kmax, o->lmax, o->tol, F, X, si);
+    }
+    femgpu_mixed_opts mo = o->mixed;
+    mo.graphs = 0;
+    return femgpu_gmres_mixed(ctx, &S, &mo, o->tol, F, X, si);
+}
+
+extern "C" int femgpu_eu_predictor_old_step(femgpu_mesh *M, femgpu_ebe *E, femgpu_blockdiag *B,
+                                            const femgpu_euler_problem *pb, const femgpu_driver_opts *o,
+                                            double *d_u, double *d_a, femgpu_driver_info *info)
+{
+    FG_CHECK(M && E && B && pb && d_u && d_a && M->ndof == 4 && E->s == 12 && E->NEQ == M->neq && E->nel == M->nel &&
+                 pb->dt > 0,
+             FEMGPU_ERR_ARG, "femgpu_eu_predictor_old_step: bad arguments");
+    FG_TRY(check_opts(o));
+    FG_TRY(blockdiag_size_check(B, M->neq, M->nnodes, 4));
+    femgpu_ctx *ctx = M->ctx;
+    const int neq = M->neq;
+    const double dt = pb->dt, alpha = pb->alpha;
+    Work w;
+    FG_TRY(driver_work(ctx, neq, w));
+    FG_TRY(ctx->drv_ae.alloc(static_cast<size_t>(M->nel) * 144));
+    if (info) memset(info, 0, sizeof(*info));
+    // predictor (Predictor_Old.c:63-66)
+    FG_TRY(femgpu_dcopy(ctx, neq, d_u, w.uold));
+    FG_TRY(femgpu_daxpy(ctx, neq, (1.0 - alpha) * dt, d_a, d_u));
+    FG_TRY(femgpu_dzero(ctx, neq, d_a));
+    double norm_a = 0.0, norm_Da = 0.0;
+    int i = 0, rc = FEMGPU_OK;
+    for (;;) {   // multicorrection (:68-116)
+        ++i;
+        FG_TRY(femgpu_assemble_euler(M, pb, d_u, d_a, ctx->drv_ae.p, w.F, nullptr));   // Build_M_K_F_SUPG
+        femgpu_solve_info si;
+        FG_TRY(euler_solve(ctx, E, B, o, ctx->drv_ae.p, w.F, w.X, &si));               // precond_setup + solver
+        record(info, i, si);
+        FG_TRY(femgpu_daxpy(ctx, neq, 1.0, w.X, d_a));           // a += Da
+        FG_TRY(femgpu_daxpy(ctx, neq, alpha * dt, w.X, d_u));    // u += alpha dt Da
+        norm_a = norm(ctx, neq, d_a, rc);
+        norm_Da = norm(ctx, neq, w.X, rc);
+        FG_TRY(rc);
         if (o->stop_by_norm ? (norm_Da < o->nonlinear_tol * norm_a || i >= o->nonlinear_maxiter)
                             : (i >= o->nonlinear_maxiter))
             break;

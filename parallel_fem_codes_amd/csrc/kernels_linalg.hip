@@ -582,140 +582,6 @@ int launch_ilu_factor(femgpu_ctx *ctx, femgpu_csr *A)
     return packed_refresh(ctx, A);
 }
 
-// ------------------------------------------------------------------------------- EBE matvec
-// s = 9/12: elements are stored colour by colour; inside one colour no two elements share an
-// unknown, so each colour is one launch whose read-modify-write of Q needs no atomics, one thread
-// per element row, the element's lm staged in LDS. s = 3: gathered per unknown (k_ebe3_rows).
-// EBE product for 3x3 blocks gathered per unknown: thread k sums, in element order, the rows
-// (e, a) of list k (Q[i] += A_e[a][0] p1 + A_e[a][1] p2 + A_e[a][2] p3 of ebemv.c:40-55 for every
-// element with lm[e][a] = i), so Q is written once, with no colouring and no read-modify-write. The
-// first kEbeUnroll records of a list are loaded together, then their matrix rows and P entries
-// (P of the Dirichlet sink NEQ read as 0), then summed in list order. No contraction into fused
-// multiply-adds: the products and sums round as the reference's, so Q is bit-identical to its
-// element loop.
-constexpr int kEbeUnroll = 6;
-
-__device__ __forceinline__ double ebe_row(const int4 r, int NEQ, const double *__restrict__ A,
-                                          const double *__restrict__ P)
-{
-#pragma clang fp contract(off)
-    const double *a = A + 3 * static_cast<size_t>(r.x);
-    const double a0 = a[0], a1 = a[1], a2 = a[2];
-    const double v1 = P[r.y < NEQ ? r.y : 0], v2 = P[r.z < NEQ ? r.z : 0], v3 = P[r.w < NEQ ? r.w : 0];
-    const double p1 = r.y < NEQ ? v1 : 0.0, p2 = r.z < NEQ ? v2 : 0.0, p3 = r.w < NEQ ? v3 : 0.0;
-    return a0 * p1 + a1 * p2 + a2 * p3;
-}
-
-__global__ __launch_bounds__(256) void k_ebe3_rows(int NEQ, const int *__restrict__ ptr, const int *__restrict__ dst,
-                                                   const int4 *__restrict__ rec, const double *__restrict__ A,
-                                                   const double *__restrict__ P, double *__restrict__ Q)
-{
-#pragma clang fp contract(off)
-    const int k = xcd_block() * 256 + static_cast<int>(threadIdx.x);
-    if (k >= NEQ) return;
-    const int p0 = ptr[k], p1 = ptr[k + 1], d = dst[k];
-    double acc = 0.0;
-    if (p1 > p0) {
-        int4 r[kEbeUnroll];
-#pragma unroll
-        for (int u = 0; u < kEbeUnroll; ++u) r[u] = rec[p0 + u < p1 ? p0 + u : p0];
-        double t[kEbeUnroll];
-#pragma unroll
-        for (int u = 0; u < kEbeUnroll; ++u) t[u] = ebe_row(r[u], NEQ, A, P);
-#pragma unroll
-        for (int u = 0; u < kEbeUnroll; ++u)
-            if (p0 + u < p1) acc += t[u];
-        for (int q = p0 + kEbeUnroll; q < p1; ++q) acc += ebe_row(rec[q], NEQ, A, P);
-    }
-    Q[d] = acc;
-}
-
-template <int S>
-__global__ __launch_bounds__(256) void k_ebe_block(int e0, int e1, int NEQ, const int *__restrict__ lm,
-                                                   const double *__restrict__ A, const double *__restrict__ P,
-                                                   double *__restrict__ Q)
-{
-    constexpr int EPB = 256 / S;   // elements per block
-    __shared__ int sl[EPB * S];
-    __shared__ double sp[EPB * S];
-    const int ebase = e0 + blockIdx.x * EPB;
-    for (int t = threadIdx.x; t < EPB * S; t += 256) {
-        const int e = ebase + t / S;
-        int v = NEQ;
-        if (e < e1) v = lm[static_cast<size_t>(e) * S + t % S];
-        sl[t] = v;
-        sp[t] = (v < NEQ) ? P[v] : 0.0;
-    }
-    __syncthreads();
-    const int le = threadIdx.x / S, a = threadIdx.x % S;
-    const int e = ebase + le;
-    if (le >= EPB || e >= e1) return;
-    const int row = sl[le * S + a];
-    if (row >= NEQ) return;
-    const double *ar = A + (static_cast<size_t>(e) * S + a) * S;
-    double acc = 0;
-#pragma unroll
-    for (int b = 0; b < S; ++b) acc += ar[b] * sp[le * S + b];
-    Q[row] += acc;
-}
-
-__global__ void k_zero(double *__restrict__ q, int n)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) q[i] = 0.0;
-}
-
-int launch_ebemv(femgpu_ctx *ctx, const femgpu_ebe *E, const double *P, double *Q)
-{
-    if (E->s == 3) {
-        if (E->NEQ > 0)
-            k_ebe3_rows<<<xcd_grid((E->NEQ + 255) / 256), 256, 0, ctx->stream>>>(
-                E->NEQ, E->rptr.p, E->rdst.p, reinterpret_cast<const int4 *>(E->rrec.p), E->A.p, P, Q);
-        FG_HIP(hipGetLastError());
-        return FEMGPU_OK;
-    }
-    if (E->NEQ > 0) {
-        k_zero<<<std::min(4096, (E->NEQ + 255) / 256), 256, 0, ctx->stream>>>(Q, E->NEQ);
-        FG_HIP(hipGetLastError());
-    }
-    for (int c = 0; c < E->ncolours; ++c) {
-        const int e0 = E->colour_ptr[c], e1 = E->colour_ptr[c + 1];
-        if (e1 <= e0) continue;
-        if (E->s == 9) {
-            constexpr int EPB = 256 / 9;
-            k_ebe_block<9><<<(e1 - e0 + EPB - 1) / EPB, 256, 0, ctx->stream>>>(e0, e1, E->NEQ, E->lm.p, E->A.p, P, Q);
-        } else if (E->s == 12) {
-            constexpr int EPB = 256 / 12;
-            k_ebe_block<12><<<(e1 - e0 + EPB - 1) / EPB, 256, 0, ctx->stream>>>(e0, e1, E->NEQ, E->lm.p, E->A.p, P, Q);
-        } else {
-            set_error("femgpu_ebemv: unsupported block size %d", E->s);
-            return FEMGPU_ERR_ARG;
-        }
-        FG_HIP(hipGetLastError());
-    }
-    return FEMGPU_OK;
-}
-
-// Element matrices from element order into colour order.
-__global__ void k_gather_blocks(const int *__restrict__ elem, int nel, int ss, const double *__restrict__ src,
-                                double *__restrict__ dst)
-{
-    const size_t total = static_cast<size_t>(nel) * ss;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const size_t k = t / ss, q = t % ss;
-        dst[t] = src[static_cast<size_t>(elem[k]) * ss + q];
-    }
-}
-
-int launch_ebe_gather_values(femgpu_ctx *ctx, femgpu_ebe *E, const double *A_elem_order)
-{
-    const size_t total = static_cast<size_t>(E->nel) * E->s * E->s;
-    if (!total) return FEMGPU_OK;
-    const int grid = static_cast<int>(std::min<size_t>((total + 255) / 256, 8192));
-    k_gather_blocks<<<grid, 256, 0, ctx->stream>>>(E->elem.p, E->nel, E->s * E->s, A_elem_order, E->A.p);
-    FG_HIP(hipGetLastError());
-    return FEMGPU_OK;
-}
-
 // ------------------------------------------------------------------------------- dot product
 // Two kernels: per-block partials, then one block sums them in a fixed order (deterministic, no
 // atomics). The inputs may be any device vectors, so the loads are scalar.

@@ -399,6 +399,12 @@ __global__ void k_scale_to_f32(int n, const double *__restrict__ r, const double
         r32[k] = static_cast<float>(r[k] * inv);
 }
 
+// r = F - r (the FP64 residual of the EBE operator: r holds A x on entry)
+__global__ void k_rsub(int n, const double *__restrict__ F, double *__restrict__ r)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) r[k] = F[k] - r[k];
+}
+
 __global__ void k_sqrt_inplace(double *v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *v = sqrt(*v);
@@ -620,13 +626,37 @@ int run_cycle<double>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, d
     return rc;
 }
 
+template <typename T>
+__global__ void k_scale_copy(int n, const T *__restrict__ x, const double *__restrict__ xs, T *__restrict__ out,
+                             const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const T s = static_cast<T>(*xs);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) out[k] = x[k] * s;
+}
+
+// FP32 operator and preconditioner of the inner solver: w = M^-1 A (x * xs), with x * xs stored
+// into xout (the basis vector). CSR: the sliced-ELL SpMV with the scaling fused, then the ILU(0)
+// sweeps; EBE: the scaled copy, the gathered element product, then the nodal block-diagonal blocks.
+static int op_prec32(femgpu_ctx *ctx, const femgpu_system *S, int prec_mode, const float *x, const double *xs,
+                     float *xout, float *w, const int *skip)
+{
+    if (S->csr) {
+        FG_TRY(launch_spmv32(ctx, S->csr, x, xs, xout, w, skip));
+        return prec32(ctx, S->ilu, prec_mode, w, w, skip);
+    }
+    k_scale_copy<float><<<grid_for(S->ebe->NEQ) * 4, 256, 0, ctx->stream>>>(S->ebe->NEQ, x, xs, xout, skip);
+    FG_HIP(hipGetLastError());
+    FG_TRY(launch_ebemv32(ctx, S->ebe, xout, w, skip));
+    return launch_blockdiag_apply32(ctx, S->bdiag, w, w, skip);
+}
+
 // The launches of Arnoldi step i of the FP32 inner solver: w_next = M^-1 A (v_i), v_i = w_cur / h
 // stored on the way, then the orthogonalization and the Givens update of column i.
 static int step_kernels32(femgpu_ctx *ctx, const femgpu_system *S, const GDev &G, int n, int i, float *wc,
                           float *wn, float *V, size_t ldv)
 {
-    FG_TRY(launch_spmv32(ctx, S->csr, wc, G.invh + i, V + i * ldv, wn, G.flags));
-    FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wn, wn, G.flags));
+    FG_TRY(op_prec32(ctx, S, G.prec_mode, wc, G.invh + i, V + i * ldv, wn, G.flags));
     if (G.cgs) {
         const int nb = std::max(1, std::min(kCgsBlocks, (n / 4 + 255) / 256));
         k_cgs_dots<<<nb, 256, 0, ctx->stream>>>(G, n, i, wn, V, ldv, ctx->cgs_partials.p);
@@ -738,20 +768,12 @@ static int apply_op64(femgpu_ctx *ctx, const femgpu_system *S, const double *x, 
     return FEMGPU_OK;
 }
 
-__global__ void k_scale_copy(int n, const double *__restrict__ x, const double *__restrict__ xs,
-                             double *__restrict__ out, const int *__restrict__ skip)
-{
-    if (skip && *skip) return;
-    const double s = *xs;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) out[k] = x[k] * s;
-}
-
 // EBE operators take unscaled input: scale into the basis first, then apply to the basis vector.
 static int apply_step_input(femgpu_ctx *ctx, const femgpu_system *S, int n, const double *wc, const double *xs,
                             double *Vi, double *wn, const int *skip)
 {
     if (S->csr) return apply_op64(ctx, S, wc, xs, Vi, wn, skip);
-    k_scale_copy<<<grid_for(n) * 4, 256, 0, ctx->stream>>>(n, wc, xs, Vi, skip);
+    k_scale_copy<double><<<grid_for(n) * 4, 256, 0, ctx->stream>>>(n, wc, xs, Vi, skip);
     FG_HIP(hipGetLastError());
     FG_TRY(launch_ebemv(ctx, S->ebe, Vi, wn));
     if (S->ilu) FG_TRY(launch_sweep64(ctx, S->ilu, wn, wn, skip));
@@ -862,15 +884,24 @@ extern "C" int femgpu_pgmres(femgpu_ctx *ctx, const femgpu_system *S, int kmax, 
 extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const femgpu_mixed_opts *opt,
                                   double tol, const double *d_F, double *d_X, femgpu_solve_info *info)
 {
-    FG_CHECK(ctx && S && S->csr && S->ilu && opt && d_F && d_X, FEMGPU_ERR_ARG,
-             "femgpu_gmres_mixed: needs a CSR operator, a factored ILU(0) and options");
-    FG_CHECK(opt->sweep_mode >= 0 && opt->sweep_mode <= 16 && opt->factor_sweeps >= 0 && opt->factor_sweeps <= 16,
-             FEMGPU_ERR_ARG, "femgpu_gmres_mixed: sweep_mode %d / factor_sweeps %d out of range", opt->sweep_mode,
-             opt->factor_sweeps);
-    FG_CHECK(opt->factor_sweeps == 0 || opt->sweep_mode > 0, FEMGPU_ERR_ARG,
-             "femgpu_gmres_mixed: the parallel factorisation is applied with Jacobi sweeps (sweep_mode > 0)");
-    FG_CHECK(opt->factor_sweeps > 0 || S->ilu->factored, FEMGPU_ERR_STATE, "femgpu_gmres_mixed: ILU(0) not factored");
-    const int n = S->csr->n;
+    const bool csr = S && S->csr && S->ilu;
+    const bool ebe = S && !S->csr && S->ebe && S->bdiag && !S->ilu;
+    FG_CHECK(ctx && (csr || ebe) && opt && d_F && d_X, FEMGPU_ERR_ARG,
+             "femgpu_gmres_mixed: needs a CSR operator with its ILU(0), or an EBE operator with its nodal "
+             "block-diagonal preconditioner, and options");
+    const int n = sys_n(S);
+    if (csr) {
+        FG_CHECK(opt->sweep_mode >= 0 && opt->sweep_mode <= 16 && opt->factor_sweeps >= 0 && opt->factor_sweeps <= 16,
+                 FEMGPU_ERR_ARG, "femgpu_gmres_mixed: sweep_mode %d / factor_sweeps %d out of range", opt->sweep_mode,
+                 opt->factor_sweeps);
+        FG_CHECK(opt->factor_sweeps == 0 || opt->sweep_mode > 0, FEMGPU_ERR_ARG,
+                 "femgpu_gmres_mixed: the parallel factorisation is applied with Jacobi sweeps (sweep_mode > 0)");
+        FG_CHECK(opt->factor_sweeps > 0 || S->ilu->factored, FEMGPU_ERR_STATE, "femgpu_gmres_mixed: ILU(0) not factored");
+    } else {
+        FG_TRY(blockdiag_check(S->bdiag, n));
+        FG_CHECK(S->ebe->has_values, FEMGPU_ERR_STATE, "femgpu_gmres_mixed: element matrices not set");
+        FG_CHECK(opt->graphs == 0, FEMGPU_ERR_ARG, "femgpu_gmres_mixed: graph replay needs a CSR operator");
+    }
     const int kin = opt->inner_kmax;
     GDev G{};
     FG_TRY(make_gdev(ctx, kin, G));
@@ -878,10 +909,10 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
     G.cgs = opt->orthogonalization == 1;
     G.graphs = opt->graphs == 1;
     if (G.cgs) FG_TRY(ctx->cgs_partials.alloc(static_cast<size_t>(kin + 2) * kCgsBlocks));
-    if (opt->factor_sweeps > 0) {
+    if (csr && opt->factor_sweeps > 0) {
         FG_TRY(launch_parilu32(ctx, S->ilu, opt->factor_sweeps));
         S->ilu->f32_factored = true;
-    } else if (opt->sweep_mode > 0) {
+    } else if (csr && opt->sweep_mode > 0) {
         FG_TRY(launch_factor_to_f32(ctx, S->ilu));
         S->ilu->f32_factored = true;
     }
@@ -911,7 +942,13 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
         // outer FP64 residual r = F - A X; on the first pass X = 0 (not yet written): r = F, ||r|| = ||F||
         const double *rr = d_F, *d_rcur = d_fn;
         if (refinements > 0) {
-            FG_TRY(launch_residual64(ctx, S->csr, d_X, d_F, r));
+            if (csr) {
+                FG_TRY(launch_residual64(ctx, S->csr, d_X, d_F, r));
+            } else {
+                FG_TRY(launch_ebemv(ctx, S->ebe, d_X, r));
+                k_rsub<<<grid * 4, 256, 0, ctx->stream>>>(n, d_F, r);
+                FG_HIP(hipGetLastError());
+            }
             FG_TRY(norm64_sync(ctx, n, r, d_rn, 1, rn));
             rr = r;
             d_rcur = d_rn;
@@ -920,7 +957,8 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
         // inner FP32 solve of M^-1 A d = M^-1 r / ||r||
         k_scale_to_f32<<<grid * 4, 256, 0, ctx->stream>>>(n, rr, d_rcur, wA);
         FG_HIP(hipGetLastError());
-        FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wA, wA, nullptr));
+        if (csr) FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wA, wA, nullptr));
+        else FG_TRY(launch_blockdiag_apply32(ctx, S->bdiag, wA, wA, nullptr));
         for (int cyc = 0; cyc < std::max(1, opt->inner_cycles); ++cyc) {
             if (cyc > 0) break;   // one inner cycle per refinement step
             cycle_start<float>(ctx, G, n, wA, nullptr, 0, 1, opt->inner_tol);
@@ -932,7 +970,7 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
             FG_HIP(hipStreamSynchronize(ctx->stream));
             // only the level-scheduled sweeps (mode 0) can raise the device error word: the
             // Jacobi-preconditioned cycle skips reading it back
-            if (G.prec_mode == 0) FG_TRY(check_device_error(ctx));
+            if (csr && G.prec_mode == 0) FG_TRY(check_device_error(ctx));
             // the device's step count: the first step that reported the cycle done
             int dsteps = steps;
             for (int i = 0; i < steps; ++i)

@@ -361,8 +361,12 @@ class EbeMatrix:
             self.h = None
 
     @property
-    def ncolours(self):
-        return N.check(N.lib().femgpu_ebe_ncolours(self.h))
+    def info(self):
+        """Layout of the gathered product: groups, max incidences per node, column bytes and the
+        bytes one FP64 / FP32 product moves."""
+        i = N.EbeInfo()
+        N.check(N.lib().femgpu_ebe_get_info(self.h, C.byref(i)))
+        return i
 
     def set_values(self, A):
         if hasattr(A, "is_cuda"):
@@ -372,7 +376,11 @@ class EbeMatrix:
             N.check(N.lib().femgpu_ebe_set_values(self.h, a.ctypes.data_as(C.c_void_p), 0))
 
     def ebemv(self, P, Q):
-        N.check(N.lib().femgpu_ebemv(self.h, _ptr(P), _ptr(Q)))
+        """Q = A P; FP64 tensors use the FP64 operator, FP32 tensors its FP32 copy."""
+        if P.dtype == Q.dtype and str(P.dtype) == "torch.float32":
+            N.check(N.lib().femgpu_ebemv_f32(self.h, _ptr(P), _ptr(Q)))
+        else:
+            N.check(N.lib().femgpu_ebemv(self.h, _ptr(P), _ptr(Q)))
 
 
 def make_problem(k=1e-4, gamma=0.0, f0=0.0, bx=(0.5, 0.0, -1.0), by=(-0.5, 1.0, 0.0), shock=0):
@@ -632,3 +640,47 @@ def Predictor_Old_step(M: "Mesh", A: CsrMatrix, pb, dt, alpha, u, a, opts):
     N.check(N.lib().femgpu_tt_predictor_old_step(M.h, A.h, C.byref(pb), float(dt), float(alpha), C.byref(opts),
                                                  _ptr(u), _ptr(a), C.byref(info)))
     return info
+
+
+def Predictor_Old_step_Euler(M: "EulerMesh", E: EbeMatrix, B: BlockDiag, pb, u, a, opts):
+    """One Euler predictor-multicorrector step (EU Process.c:4-21 -> Predictor_Old.c:53-123) on the
+    device with the EBE scheme and the BlockDiag preconditioner; u and a are updated in place.
+    pb: make_euler_problem(...) carrying alpha and dt."""
+    info = N.DriverInfo()
+    N.check(N.lib().femgpu_eu_predictor_old_step(M.h, E.h, B.h, C.byref(pb), C.byref(opts), _ptr(u), _ptr(a),
+                                                 C.byref(info)))
+    return info
+
+
+OBLIQUO_STATE = (1.0, 0.98481, -0.17365, 0.94643)   # rho, rho u, rho v, rho E (OBLIQUO_InitialSolution.c)
+
+
+def obliquo_problem(mx, my, length=2.0, right=True, cfl=5.0, gamma=1.4):
+    """The reference's OBLIQUO problem (03_Euler_Equations/2D/OBLIQUO): supersonic flow at Mach 2
+    entering at -10 degrees onto the wall y = 0 (a 10-degree compression in the wall's frame), on a
+    structured mesh of mx x my squares over [0, length] x [0, 1]. Marks of the generator's
+    setmark_according_boundary (x = 0 and y = 1: all four dofs prescribed; y = 0: rho v
+    prescribed), prescribed values of OBLIQUO_{rho,v1,v2,e}presc.c (rho v = 0 on the wall past the
+    leading edge) and the initial state of OBLIQUO_InitialSolution.c at the unknowns. dt follows
+    from the CFL number on the mesh spacing h = sqrt(2 A) and the freestream c + |v| (the reference
+    takes dt as an input). Returns a dict of host arrays and the Euler problem parameters."""
+    x, y, ien = structured_mesh(mx, my, box=(0.0, 0.0, float(length), 1.0), right=right)
+    ndof, mark, t = structured_marks("OBLIQUO", x, y)
+    assert ndof == 4
+    invP, neq = number_unknowns(t, 4)
+    lm = Fill_LM(ien, t, invP, neq, 4)
+    nn = len(x)
+    presc = np.tile(np.array(OBLIQUO_STATE), (nn, 1))
+    presc[(np.abs(y) <= 1e-15) & (np.abs(x) > 0), 2] = 0.0
+    presc = presc.ravel()
+    u0 = np.zeros(neq)
+    st = np.tile(np.array(OBLIQUO_STATE), nn)
+    unk = t != 0
+    u0[invP[unk]] = st[unk]
+    rho, m1, m2, rE = OBLIQUO_STATE
+    vmag = np.hypot(m1, m2) / rho
+    c = np.sqrt(gamma * (gamma - 1.0) * (rE / rho - 0.5 * vmag * vmag))
+    h = np.sqrt(float(length) / mx * 1.0 / my)
+    dt = cfl * h / (c + vmag)
+    return dict(x=x, y=y, ien=ien, type4=t, invP=invP, neq=neq, lm=lm, presc=presc, u0=u0, dt=dt,
+                gamma=gamma, mach=vmag / c)

@@ -43,6 +43,12 @@ class SolveInfo(C.Structure):
                 ("true_rel_residual", C.c_double), ("prec_rel_residual", C.c_double)]
 
 
+class EbeInfo(C.Structure):
+    """femgpu_ebe_info."""
+    _fields_ = [("groups", C.c_int), ("max_incidences", C.c_int), ("column_bytes", C.c_int), ("pad", C.c_int),
+                ("bytes_f64", C.c_longlong), ("bytes_f32", C.c_longlong)]
+
+
 class EulerProblem(C.Structure):
     """femgpu_euler_problem."""
     _fields_ = [("gamma", C.c_double), ("Mach", C.c_double), ("alpha", C.c_double), ("dt", C.c_double),
@@ -159,8 +165,9 @@ _SIGS = {
     "femgpu_ebe_create": (I, [vp, I, I, I, i32p, C.POINTER(vp)]),
     "femgpu_ebe_destroy": (I, [vp]),
     "femgpu_ebe_set_values": (I, [vp, vp, I]),
-    "femgpu_ebe_ncolours": (I, [vp]),
+    "femgpu_ebe_get_info": (I, [vp, vp]),
     "femgpu_ebemv": (I, [vp, vp, vp]),
+    "femgpu_ebemv_f32": (I, [vp, vp, vp]),
     "femgpu_ddot": (I, [vp, I, vp, vp, C.POINTER(D)]),
     "femgpu_daxpy": (I, [vp, I, D, vp, vp]),
     "femgpu_dscal": (I, [vp, I, D, vp]),
@@ -205,6 +212,8 @@ _SIGS = {
     "femgpu_euler_mesh_create": (I, [vp, I, I, I, f64p, f64p, i32p, i32p, f64p, i32p, i32p, vp, I,
                                      C.POINTER(vp)]),
     "femgpu_assemble_euler": (I, [vp, C.POINTER(EulerProblem), vp, vp, vp, vp, vp]),
+    "femgpu_eu_predictor_old_step": (I, [vp, vp, vp, C.POINTER(EulerProblem), C.POINTER(DriverOpts), vp, vp,
+                                         C.POINTER(DriverInfo)]),
     "femgpu_assemble_transport": (I, [vp, C.POINTER(TransportProblem), vp, vp, vp, vp]),
     "femgpu_assemble_transport_transient": (I, [vp, C.POINTER(TransportProblem), D, D, vp, vp,
                                                 vp, vp, vp]),

@@ -83,7 +83,8 @@ NS_CASES = [
 
 
 EU_CASES = [
-    dict(name="eu_ebe_dimless_cau_r8", M=8, N=8, cut="right", ebe=1, reorder=0, dimless=1, shock=1),
+    dict(name="eu_ebe_dimless_cau_r8", M=8, N=8, cut="right", ebe=1, reorder=0, dimless=1, shock=1, kmax=50,
+         lmax=40, tol=1e-10),
     dict(name="eu_csr_rcm_dim_cau_l8", M=8, N=8, cut="left", ebe=0, reorder=1, dimless=0, shock=1),
     dict(name="eu_csr_not_dimless_none_r6x5", M=6, N=5, cut="right", ebe=0, reorder=0, dimless=1, shock=0),
 ]
@@ -179,9 +180,9 @@ def run_eu_case(c, workdir):
     type4[inflow] = 0
     presc = np.where(inflow[:, None], state, 0.0)
     hd = np.zeros(16, np.int32)
-    hd[:6] = [nn, len(ien) // 3, c["ebe"], c["reorder"], c["dimless"], c["shock"]]
+    hd[:8] = [nn, len(ien) // 3, c["ebe"], c["reorder"], c["dimless"], c["shock"], c.get("kmax", 0), c.get("lmax", 0)]
     pr = np.zeros(16)
-    pr[:5] = [gamma, 2.0, 0.5, 1e-2, 1e-8]
+    pr[:6] = [gamma, 2.0, 0.5, 1e-2, 1e-8, c.get("tol", 0.0)]
     inp, outp = os.path.join(workdir, "in.bin"), os.path.join(workdir, "out.bin")
     with open(inp, "wb") as f:
         for a in (hd, pr, x, y, type4, ien, presc, state, incr):
@@ -198,13 +199,16 @@ def run_eu_case(c, workdir):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--only", nargs="*", default=None, help="regenerate only these fixtures (keep the others)")
     a = ap.parse_args()
     subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "ref.mk"), f"REF={a.ref}"],
                    check=True)
-    manifest = {}
+    mpath = os.path.join(HERE, "manifest.json")
+    manifest = json.load(open(mpath)) if a.only and os.path.exists(mpath) else {}
+    keep = (lambda c: c["name"] in a.only) if a.only else (lambda c: True)
     work = tempfile.mkdtemp()
     try:
-        for c in CASES:
+        for c in filter(keep, CASES):
             d = run_case(c, work)
             fn = os.path.join(HERE, c["name"] + ".npz")
             np.savez_compressed(fn, **d)
@@ -213,7 +217,7 @@ def main():
             meta["iters"] = d["ref_iters"].tolist()
             manifest[c["name"]] = meta
             print(c["name"], "iters", meta["iters"], "bytes", os.path.getsize(fn))
-        for c in EU_CASES:
+        for c in filter(keep, EU_CASES):
             d = run_eu_case(c, work)
             fn = os.path.join(HERE, c["name"] + ".npz")
             np.savez_compressed(fn, **d)
@@ -221,7 +225,7 @@ def main():
             meta["sha256"] = hashlib.sha256(open(fn, "rb").read()).hexdigest()
             manifest[c["name"]] = meta
             print(c["name"], "bytes", os.path.getsize(fn))
-        for c in NS_CASES:
+        for c in filter(keep, NS_CASES):
             d = run_ns_case(c, work)
             fn = os.path.join(HERE, c["name"] + ".npz")
             np.savez_compressed(fn, **d)

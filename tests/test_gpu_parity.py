@@ -200,7 +200,8 @@ def test_ebemv_matches_oracle_and_csr(dev, torch_cuda, oracle, m, right):
     pb = oracle.make_problem(k=1e-3)
     Ke, F = oracle.build_transport_supg(x, y, t, invP, ien, oracle.inflow_ubc(x, y), np.zeros(neq), pb, neq)
     E = fem.EbeMatrix(dev, lm, 3, neq)
-    assert 3 <= E.ncolours <= 16
+    inf = E.info
+    assert inf.groups == neq and inf.max_incidences == 6 and inf.column_bytes == 2
     E.set_values(Ke)
     p = np.random.default_rng(2).standard_normal(neq)
     q = torch.empty(neq, dtype=torch.float64, device="cuda")
@@ -233,7 +234,36 @@ def test_block_ebemv_matches_oracle(dev, torch_cuda, oracle, s_blk):
     p = np.random.default_rng(4).standard_normal(neq)
     q = torch.empty(neq, dtype=torch.float64, device="cuda")
     E.ebemv(T(torch, p), q)
-    assert rel(H(q), oracle.ebemv_block(lm, A, p, neq, s_blk)) < 1e-13
+    ref = oracle.ebemv_block(lm, A, p, neq, s_blk)
+    # each node's element rows summed in element order, row sums left to right: bit-identical
+    assert np.array_equal(H(q), ref)
+    # the FP32 copy of the operator
+    q32 = torch.empty(neq, dtype=torch.float32, device="cuda")
+    E.ebemv(T(torch, p, torch.float32), q32)
+    assert rel(H(q32).astype(np.float64), ref) < 2e-6
+
+
+def test_ebemv_int32_columns_and_uncovered_unknowns(dev, torch_cuda, oracle):
+    """Unknown numbers too far apart for int16 offsets (the operator falls back to int32 columns)
+    and unknowns that no element touches (their Q entries are zero), against the oracle."""
+    torch = torch_cuda
+    x, y, ien = oracle.mesh_structured(30, 4)
+    t = oracle.unit_square_types(x, y)
+    invP, neq = oracle.number_unknowns(t)
+    lm = oracle.fill_lm(ien, t, invP, neq)
+    rng = np.random.default_rng(9)
+    big = neq + 70000                     # spread the numbering: offsets beyond +-32767
+    spread = np.sort(rng.choice(big, size=neq, replace=False)).astype(np.int32)
+    lm2 = np.where(lm < neq, spread[np.minimum(lm, neq - 1)], big).astype(np.int32)
+    nel = len(ien) // 3
+    A = rng.standard_normal(nel * 9)
+    E = fem.EbeMatrix(dev, lm2, 3, big)
+    assert E.info.column_bytes == 4
+    E.set_values(A)
+    p = rng.standard_normal(big)
+    q = torch.full((big,), np.nan, dtype=torch.float64, device="cuda")
+    E.ebemv(T(torch, p), q)
+    assert np.array_equal(H(q), oracle.ebemv_block(lm2, A, p, big, 3))
 
 
 # ----------------------------------------------------------------------------- ILU(0)
@@ -975,17 +1005,13 @@ def test_ns_process_picard_matches_oracle(dev, torch_cuda, oracle):
 
 # ----------------------------------------------------------------------------- block diagonal
 
-def blockdiag_reference(ien, type_, invP, NEQ, Ae, ndof):
-    """BlockDiag_precond_EBE_setup.c:66-80 restated: nodal diagonal blocks summed in element
-    order, inverted (numpy)."""
-    nn, s = len(type_) // ndof, 3 * ndof
-    A3 = Ae.reshape(-1, s, s)
-    B = np.zeros((nn, ndof, ndof))
-    for e in range(len(ien) // 3):
-        for a in range(3):
-            B[ien[3 * e + a]] += A3[e, ndof * a:ndof * a + ndof, ndof * a:ndof * a + ndof]
+def blockdiag_reference(oracle, ien, type_, invP, NEQ, Ae, ndof):
+    """BlockDiag_precond_EBE_setup.c:66-80: the oracle's nodal blocks and inverses (the reference's
+    cofactor formulas, pinned against its own output)."""
+    nn = len(type_) // ndof
+    inv = oracle.blockdiag_setup(ien, Ae, nn, ndof).reshape(nn, ndof, ndof)
     ids = np.where(np.asarray(type_) == 0, NEQ, invP).reshape(nn, ndof)
-    return np.linalg.inv(B), ids
+    return inv, ids
 
 
 def blockdiag_apply_reference(inv, ids, p, NEQ):
@@ -1033,13 +1059,15 @@ def test_blockdiag_and_scaling_match_restatement(dev, torch_cuda, oracle, ndof):
     invP, neq = oracle.number_unknowns(t, ndof)
     Ae = rng.uniform(-1, 1, (nel, s, s)) + 6.0 * np.eye(s)
     Ae = Ae.ravel()
-    inv, ids = blockdiag_reference(ien, t, invP, neq, Ae, ndof)
+    inv, ids = blockdiag_reference(oracle, ien, t, invP, neq, Ae, ndof)
     B = fem.BlockDiag(dev, nn, ien, ndof, t, invP, neq)
     Ag = T(torch, Ae)
     B.setup(Ag)
     p = rng.standard_normal(neq)
     z = torch.empty(neq, dtype=torch.float64, device="cuda")
     B.apply(T(torch, p), z)
+    # blocks summed in element order, cofactor inverses, apply row by row: bit-identical
+    assert np.array_equal(H(z), oracle.blockdiag_apply(ids.ravel(), inv.ravel(), p, neq, ndof))
     assert rel(H(z), blockdiag_apply_reference(inv, ids, p, neq)) < 1e-12
     F = T(torch, p)
     B.scale(Ag, F)
@@ -1072,3 +1100,71 @@ def test_pgmres_ebe_blockdiag_euler(dev, torch_cuda, oracle):
     xd = np.linalg.solve(Ad[:neq, :neq], s["R"][:neq])
     assert info.status == 0
     assert rel(H(X), xd) < 1e-8
+
+
+# ----------------------------------------------------------------------------- Euler driver
+
+def _euler_predictor_oracle(oracle, P, prm, steps, nltol, maxit, kmax, lmax, tol):
+    """Predictor_Old.c:53-123 over the oracle for the Euler equations with the EBE scheme:
+    Build_M_K_F_SUPG, BlockDiag_precond_EBE_setup (blocks, cofactor inverses, R := D^-1 R) and
+    pgmres with the 12x12 EBE product and BlockDiag_precond per multicorrection."""
+    neq, nn, nel = P["neq"], len(P["x"]), len(P["ien"]) // 3
+    params = oracle.euler_params(**prm)
+    alpha, dt = prm["alpha"], prm["dt"]
+    ids = oracle.blockdiag_ids(P["type4"], P["invP"], 4, neq)
+    u, a = P["u0"].copy(), np.zeros(neq)
+    out = []
+    for _ in range(steps):
+        u += (1.0 - alpha) * dt * a
+        a[:] = 0.0
+        i, its = 0, []
+        while True:
+            i += 1
+            U4, dU4 = oracle.euler_nodal_state(P["type4"], P["invP"], u, a, P["presc"])
+            Ae, R = oracle.build_euler_supg(P["x"], P["y"], P["ien"], P["lm"], U4, dU4, params, neq)
+            inv = oracle.blockdiag_setup(P["ien"], Ae, nn, 4)
+            B = oracle.blockdiag_apply(ids, inv, R, neq, 4)
+            S = oracle.make_system(2, neq, lm=P["lm"], A=Ae, nel=nel, s=12, bdiag=(ids, inv, 4))
+            Da, it, rho, cyc = oracle.pgmres(S, B, kmax=kmax, lmax=lmax, tol=tol)
+            its.append(it)
+            a += Da
+            u += alpha * dt * Da
+            if np.sqrt(oracle.ddot(Da, Da)) < nltol * np.sqrt(oracle.ddot(a, a)) or i >= maxit:
+                break
+        out.append((i, its, u.copy(), a.copy()))
+    return out
+
+
+def test_euler_predictor_old_matches_oracle(dev, torch_cuda, oracle):
+    """The OBLIQUO problem (Mach 2 onto a wall at 10 degrees), CFL 5, three predictor-multicorrector
+    steps on the device (femgpu_eu_predictor_old_step) against the same loop over the oracle.
+    FP64 path: equal multicorrection counts, Arnoldi steps within +-2 per solve, states within 1e-9
+    relative L2 after every step. Mixed path (FP32 EBE + FP32 BlockDiag inner GMRES in FP64
+    refinement, each solve FP64-verified to 1e-8): states within 1e-6 of the oracle's."""
+    torch = torch_cuda
+    P = fem.obliquo_problem(24, 12)
+    prm = dict(gamma=1.4, Mach=2.0, alpha=0.5, dt=P["dt"], tolerance=1e-8, dimensionless=0, shock=1)
+    steps, nltol, maxit, kmax, lmax, tol = 3, 0.1, 4, 50, 200, 1e-8   # 3, 4 and 3 multicorrections
+    ref = _euler_predictor_oracle(oracle, P, prm, steps, nltol, maxit, kmax, lmax, tol)
+    nn = len(P["x"])
+    M = fem.EulerMesh(dev, P["x"], P["y"], P["type4"], P["invP"], P["presc"], P["ien"], P["lm"], P["neq"])
+    E = fem.EbeMatrix(dev, P["lm"], 12, P["neq"])
+    B = fem.BlockDiag(dev, nn, P["ien"], 4, P["type4"], P["invP"], P["neq"])
+    pb = fem.make_euler_problem(**prm)
+    for solver, bar in (("fp64", 1e-9), ("mixed", 1e-6)):
+        opts = fem.driver_opts(solver, kmax=kmax, lmax=lmax, tol=tol, nonlinear_tol=nltol, nonlinear_maxiter=maxit,
+                               inner_kmax=kmax, inner_tol=1e-4)
+        u = T(torch, P["u0"])
+        a = torch.zeros_like(u)
+        for k in range(steps):
+            info = fem.Predictor_Old_step_Euler(M, E, B, pb, u, a, opts)
+            i_ref, its_ref, u_ref, a_ref = ref[k]
+            assert info.status in (0, 1), (solver, k, info.status)
+            if solver == "fp64":
+                assert info.iterations == i_ref, (k, info.iterations, i_ref)
+                for j in range(i_ref):
+                    assert abs(info.iters[j] - its_ref[j]) <= 2, (k, j, info.iters[j], its_ref[j])
+            e = rel(H(u), u_ref)
+            print(f"euler {solver} step {k + 1}: multicorrections {info.iterations} (oracle {i_ref}), "
+                  f"rel err u {e:.2e}")
+            assert e < bar, (solver, k, e)

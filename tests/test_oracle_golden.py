@@ -238,3 +238,24 @@ def test_euler_structure_and_assembly_bit_exact(oracle, name):
         assert np.array_equal(Ae, d["ref_A"])
     else:
         assert np.array_equal(oracle.csr_assemble(c["scheme"], Ae, c["nnz"], s=12), d["ref_AA"])
+
+
+def test_euler_blockdiag_gmres_matches_reference(oracle):
+    """The Euler linear solve of the EBE scheme as the reference runs it (BlockDiag_precond_EBE_setup,
+    then pgmres with ebemvNDOF4 and BlockDiag_precond): the inverse nodal blocks and D^-1 R are the
+    reference's bit for bit (its 4x4 cofactor inverse included), the Arnoldi step count is equal and
+    the solution agrees to 1e-10 relative L2."""
+    d = G.load("eu_ebe_dimless_cau_r8")
+    ebe, neq, lm, invP, u, du, c, params = eu_setup(oracle, d)
+    assert ebe
+    nn = len(d["x"])
+    inv = oracle.blockdiag_setup(d["ien"], d["ref_A"], nn, 4)
+    assert np.array_equal(inv, d["ref_invBlockDiag"])
+    ids = oracle.blockdiag_ids(d["type4"], invP, 4, neq)
+    B = oracle.blockdiag_apply(ids, inv, d["ref_R"], neq, 4)
+    assert np.array_equal(B, d["ref_MinvR"])
+    kmax, lmax, tol = int(d["header"][6]), int(d["header"][7]), float(d["params"][5])
+    S = oracle.make_system(2, neq, lm=lm, A=d["ref_A"], nel=len(d["ien"]) // 3, s=12, bdiag=(ids, inv, 4))
+    X, it, rho, cyc = oracle.pgmres(S, B, kmax=kmax, lmax=lmax, tol=tol)
+    assert it == int(d["ref_iters"][0])
+    assert np.linalg.norm(X - d["ref_X"]) <= 1e-10 * np.linalg.norm(d["ref_X"])

@@ -281,3 +281,43 @@ def test_pcg_matches_direct_solve(oracle):
     b = rng.standard_normal(n)
     x, it = oracle.pcg(S, b, itmax=500, tol=1e-13)
     assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("ndof", [1, 3, 4])
+def test_blockdiag_inverse_and_apply(oracle, ndof):
+    """BlockDiag_precond_EBE_setup.c:66-80 + BlockDiag_precond.c:12-26: the nodal diagonal blocks
+    summed over the elements, inverted by the cofactor formulas, equal numpy's inverse of the same
+    sums; the apply leaves prescribed dofs alone and reads them as 0 (in place, p == z)."""
+    x, y, ien = oracle.mesh_structured(7, 6)
+    nn, nel, s = len(x), len(ien) // 3, 3 * ndof
+    rng = np.random.default_rng(40 + ndof)
+    t = (rng.uniform(size=nn * ndof) > 0.25).astype(np.int32)
+    invP, neq = oracle.number_unknowns(t, ndof)
+    A = (rng.uniform(-1, 1, (nel, s, s)) + 5.0 * np.eye(s)).ravel()
+    inv = oracle.blockdiag_setup(ien, A, nn, ndof).reshape(nn, ndof, ndof)
+    B = np.zeros((nn, ndof, ndof))
+    A3 = A.reshape(nel, s, s)
+    for e in range(nel):
+        for a in range(3):
+            B[ien[3 * e + a]] += A3[e, ndof * a:ndof * (a + 1), ndof * a:ndof * (a + 1)]
+    ref = np.linalg.inv(B)
+    exact = np.ones((ndof, ndof), bool)
+    if ndof == 4:
+        # the reference's 4x4 cofactor formula writes c*j*m where the cofactor of entry [3][2] has
+        # c*f*m (BlockDiag_precond_EBE_setup.c:144); that entry is pinned by the reference's own
+        # output instead (test_oracle_golden.py::test_euler_blockdiag_gmres_matches_reference)
+        exact[3, 2] = False
+        assert np.abs(inv[:, 3, 2] - ref[:, 3, 2]).max() > 1e-8 * np.abs(ref).max()
+    assert np.allclose(inv[:, exact], ref[:, exact], rtol=1e-11, atol=1e-13 * np.abs(ref).max())
+    ids = oracle.blockdiag_ids(t, invP, ndof, neq)
+    p = rng.standard_normal(neq)
+    z = oracle.blockdiag_apply(ids, inv.ravel(), p, neq, ndof)
+    pp = np.append(p, 0.0)
+    expect = p.copy()
+    for v in range(nn):
+        I = ids[v * ndof:(v + 1) * ndof]
+        w = inv[v] @ pp[I]
+        for d in range(ndof):
+            if I[d] < neq:
+                expect[I[d]] = w[d]
+    assert np.allclose(z, expect, rtol=1e-11, atol=1e-12)
