@@ -176,8 +176,19 @@ int femgpu_csr_set_lanes(femgpu_csr *A, int lanes);
 int femgpu_ilu0_factor(femgpu_csr *A);
 /* Factor on the host from the current values and upload (same arithmetic, serial order). */
 int femgpu_ilu0_factor_host(femgpu_csr *A);
-/* ILUp_precond (CSR/ILUp_precond.c:4-47): z = U^-1 L^-1 p; d_p == d_z allowed. */
+/* ILUp_precond (CSR/ILUp_precond.c:4-47): z = U^-1 L^-1 p; d_p == d_z allowed. With an apply mode
+ * set by femgpu_ilu0_set_apply, the FP64 Jacobi-applied parallel factors instead. */
 int femgpu_ilu0_apply(femgpu_csr *A, const double *d_p, double *d_z);
+/* FP64 ILU(0) factors by `sweeps` fixed-point sweeps of the parallel ILU(0) iteration (Chow and
+ * Patel, SIAM J. Sci. Comput. 37(2), 2015) from the current values; no exact factorisation. Needs
+ * rows and strict triangles of at most 16 entries. No reference counterpart: the FP64 form of the
+ * mixed solver's preconditioner (a precision-only comparison with it). */
+int femgpu_ilu0_factor_parallel(femgpu_csr *A, int sweeps);
+/* How the FP64 ILU(0) preconditioner of A is applied (femgpu_ilu0_apply, and pgmres / pcg with A as
+ * preconditioner): 0 = the exact level-scheduled sweeps on the factors of femgpu_ilu0_factor (the
+ * default, ILUp_precond); k > 0 = k Jacobi iterations per triangle on the factors of
+ * femgpu_ilu0_factor_parallel. No reference counterpart. */
+int femgpu_ilu0_set_apply(femgpu_csr *A, int jacobi_iterations);
 int femgpu_ilu0_apply_f32(femgpu_csr *A, const float *d_p, float *d_z);
 /* Copy the factors to the host: LU[nnz] in A's layout (L multipliers below, U above the
  * diagonal), Dinv[n] = 1/pivot. */

@@ -281,6 +281,7 @@ extern "C" int femgpu_csr_values_changed(femgpu_csr *A)
     FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_csr_values_changed: null");
     A->factored = false;
     A->f32_factored = false;
+    A->par64_factored = false;
     FG_TRY(launch_sell_values(A->ctx, A));
     // the FP32 CSR copy is read only where the sliced-ELL copy or its factor layout is missing
     if (A->sell_ok && A->par_ok) return FEMGPU_OK;
@@ -424,8 +425,23 @@ extern "C" int femgpu_ilu0_get_f32(femgpu_csr *A, float *LU, float *Dinv)
 extern "C" int femgpu_ilu0_apply(femgpu_csr *A, const double *d_p, double *d_z)
 {
     FG_CHECK(A && d_p && d_z, FEMGPU_ERR_ARG, "femgpu_ilu0_apply: bad arguments");
-    FG_CHECK(A->factored, FEMGPU_ERR_STATE, "femgpu_ilu0_apply: not factored");
+    FG_CHECK(ilu64_ready(A), FEMGPU_ERR_STATE, "femgpu_ilu0_apply: not factored");
     return launch_sweep64(A->ctx, A, d_p, d_z, nullptr);
+}
+
+extern "C" int femgpu_ilu0_factor_parallel(femgpu_csr *A, int sweeps)
+{
+    FG_CHECK(A, FEMGPU_ERR_ARG, "femgpu_ilu0_factor_parallel: null");
+    return launch_parilu64(A->ctx, A, sweeps);
+}
+
+extern "C" int femgpu_ilu0_set_apply(femgpu_csr *A, int jacobi_iterations)
+{
+    FG_CHECK(A && jacobi_iterations >= 0 && jacobi_iterations <= 1024, FEMGPU_ERR_ARG, "femgpu_ilu0_set_apply: bad arguments");
+    FG_CHECK(jacobi_iterations == 0 || A->ell_kl > 0, FEMGPU_ERR_ARG,
+             "femgpu_ilu0_set_apply: Jacobi iterations need the sliced-ELL triangles (at most 16 entries each)");
+    A->prec64_jacobi = jacobi_iterations;
+    return FEMGPU_OK;
 }
 
 extern "C" int femgpu_ilu0_apply_f32(femgpu_csr *A, const float *d_p, float *d_z)
@@ -493,7 +509,7 @@ extern "C" int femgpu_pcg(femgpu_ctx *ctx, const femgpu_system *S, int itmax, do
                           double *d_X, femgpu_solve_info *info)
 {
     FG_CHECK(ctx && S && S->csr && d_B && d_X, FEMGPU_ERR_ARG, "femgpu_pcg: needs a CSR operator");
-    FG_CHECK(!S->ilu || S->ilu->factored, FEMGPU_ERR_STATE, "femgpu_pcg: ILU(0) not factored");
+    FG_CHECK(!S->ilu || ilu64_ready(S->ilu), FEMGPU_ERR_STATE, "femgpu_pcg: ILU(0) not factored");
     const int n = S->csr->n;
     if (!S->ilu && S->bdiag) FG_TRY(blockdiag_check(S->bdiag, n));
     DevBuf<double> P, Q, R, Z;

@@ -8,6 +8,7 @@
 // with no atomics. Contraction is off in this file: the element arithmetic follows the
 // reference's expression order operation for operation.
 #include <algorithm>
+#include <cstdint>
 #include <vector>
 
 #include "common.h"
@@ -272,6 +273,86 @@ __global__ __launch_bounds__(256) void k_elem_transient(int nel, ElemIn in, doub
     for (int q = threadIdx.x; q < 6 * nb; q += 256) dR[q] = stage[q];
 }
 
+// Fused transient assembly: a block of kFzRows rows (a compact patch of the mesh) first evaluates
+// every element that touches its rows, once, into LDS -- for each vertex row a of the element that
+// belongs to the block, the slot {Ae[a][0..2], Fe_a, lift[a][0..2]} -- then each thread gathers its
+// row from LDS: the CSR values of the row (each entry summed over the row's incidences in element
+// order, as csr_assembly.c adds them) and the residual (Fe_a, then the three coupling terms, in
+// element order, as Build_M_K_R_SUPG.c:173-182). The sums are those of the element-array path, so
+// the results are bit-identical to it, and no element array is written or read back: elements on
+// a patch border are evaluated once per patch that touches them instead.
+constexpr int kFzRows = 128;        // rows (threads) per block
+constexpr int kFzMaxSlots = 1152;   // LDS slots per block: 1152 x 7 doubles = 63 KB
+constexpr int kFzVals = 7;
+
+template <int MAXLEN>
+__global__ __launch_bounds__(kFzRows) void k_transient_fused(int nblocks, ElemIn in, const int *__restrict__ eptr,
+                                                             const int *__restrict__ elem,
+                                                             const ushort4 *__restrict__ eq,
+                                                             const int *__restrict__ iptr,
+                                                             const unsigned *__restrict__ inc,
+                                                             const int4 *__restrict__ rinfo, int nnz,
+                                                             double *__restrict__ AA, double *__restrict__ R)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double slot[];
+    const int w = xcd_block();
+    if (w >= nblocks) return;
+    const int t = static_cast<int>(threadIdx.x);
+    if (w == 0 && t == 0) {
+        AA[nnz] = 0.0;        // csr_assembly's dummy slot
+        R[in.neq] = 0.0;      // Build_M_K_R_SUPG.c:183
+    }
+    for (int le = eptr[w] + t; le < eptr[w + 1]; le += kFzRows) {
+        const int e = elem[le];
+        const ushort4 q = eq[le];
+        TransientElem T;
+        transient_elem(in, e, T);
+        const unsigned short qa[3] = {q.x, q.y, q.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (qa[a] != 0xffffu) {
+                double *s = slot + kFzVals * qa[a];
+                s[0] = T.Ae[a][0];
+                s[1] = T.Ae[a][1];
+                s[2] = T.Ae[a][2];
+                s[3] = T.Fe[a];
+                s[4] = T.lift[a][0];
+                s[5] = T.lift[a][1];
+                s[6] = T.lift[a][2];
+            }
+    }
+    __syncthreads();
+    const int4 ri = rinfo[w * kFzRows + t];   // {row, first CSR slot, row length, incidences}
+    if (ri.x < 0) return;
+    double acc[MAXLEN];
+#pragma unroll
+    for (int k = 0; k < MAXLEN; ++k) acc[k] = 0.0;
+    double r = 0.0;
+    const unsigned *ip = inc + iptr[w] + t;
+    for (int j = 0; j < ri.w; ++j) {
+        const unsigned c = ip[j * kFzRows];
+        const double *s = slot + kFzVals * (c & 0xffffu);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const unsigned k = (c >> (16 + 4 * b)) & 15u;   // 15: a Dirichlet column
+            const double v = s[b];
+#pragma unroll
+            for (int kk = 0; kk < MAXLEN; ++kk)
+                if (static_cast<unsigned>(kk) == k) acc[kk] += v;
+        }
+        r += s[3];
+        r -= s[4];
+        r -= s[5];
+        r -= s[6];
+    }
+    double *out = AA + ri.y;
+#pragma unroll
+    for (int kk = 0; kk < MAXLEN; ++kk)
+        if (kk < ri.z) out[kk] = acc[kk];
+    R[ri.x] = r;
+}
+
 // RHS gathers. rhs_src entries are e*3+a in element order.
 __global__ void k_rhs_steady(int neq, const int *__restrict__ dst, const int *__restrict__ ptr,
                              const int *__restrict__ src, const double *__restrict__ fe,
@@ -428,6 +509,162 @@ void csr_map(int nkeys, const std::vector<std::pair<int, int>> &pairs, std::vect
     for (auto &p : pairs) src[fill[rank[p.first]]++] = p.second;
 }
 
+// The fused assembly's plan, from the mesh's device arrays (connectivity, marks, numbering and the
+// CSR slot lists): rows grouped into compact patches by breadth-first growth over the mesh graph
+// (up to kFzRows rows and kFzMaxSlots incidences), each patch's elements sorted, its incidences
+// numbered row by row as LDS slots.
+static int build_fused_plan(femgpu_mesh *M)
+{
+    M->fz_state = -1;
+    const int nn = M->nnodes, nel = M->nel, neq = M->neq, nnz = M->nnz;
+    if (neq == 0 || nel == 0 || nnz == 0) return FEMGPU_OK;
+    femgpu_ctx *ctx = M->ctx;
+    hipStream_t st = ctx->stream;
+    std::vector<int> ien(static_cast<size_t>(nel) * 3), type(nn), invP(nn);
+    std::vector<int> aptr(static_cast<size_t>(nnz) + 1), adst(nnz);
+    FG_HIP(hipMemcpyAsync(ien.data(), M->ien.p, ien.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    FG_HIP(hipMemcpyAsync(type.data(), M->type.p, type.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    FG_HIP(hipMemcpyAsync(invP.data(), M->invP.p, invP.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    FG_HIP(hipMemcpyAsync(aptr.data(), M->aa_ptr.p, aptr.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    FG_HIP(hipMemcpyAsync(adst.data(), M->aa_dst.p, adst.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    FG_HIP(hipStreamSynchronize(st));
+    std::vector<int> asrc(aptr[nnz]);
+    FG_HIP(hipMemcpyAsync(asrc.data(), M->aa_src.p, asrc.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    FG_HIP(hipStreamSynchronize(st));
+    // slot of every element entry (nnz: none -- a Dirichlet row or column)
+    std::vector<int> scheme(static_cast<size_t>(nel) * 9, nnz);
+    for (int k = 0; k < nnz; ++k)
+        for (int q = aptr[k]; q < aptr[k + 1]; ++q) scheme[asrc[q]] = adst[k];
+    // incidences of every row in element order, the row's first CSR slot and length
+    std::vector<int> rptr(static_cast<size_t>(neq) + 1, 0);
+    for (int e = 0; e < nel; ++e)
+        for (int a = 0; a < 3; ++a) {
+            const int v = ien[3 * static_cast<size_t>(e) + a];
+            if (type[v] == 1) ++rptr[invP[v] + 1];
+        }
+    for (int i = 0; i < neq; ++i) rptr[i + 1] += rptr[i];
+    std::vector<int> rinc(rptr[neq]), fill(rptr.begin(), rptr.end() - 1);
+    for (int e = 0; e < nel; ++e)
+        for (int a = 0; a < 3; ++a) {
+            const int v = ien[3 * static_cast<size_t>(e) + a];
+            if (type[v] == 1) rinc[fill[invP[v]]++] = 3 * e + a;
+        }
+    std::vector<int> rfirst(neq, INT32_MAX), rlast(neq, -1);
+    for (int i = 0; i < neq; ++i)
+        for (int q = rptr[i]; q < rptr[i + 1]; ++q)
+            for (int b = 0; b < 3; ++b) {
+                const int sl = scheme[9 * static_cast<size_t>(rinc[q] / 3) + 3 * (rinc[q] % 3) + b];
+                if (sl < nnz) {
+                    rfirst[i] = std::min(rfirst[i], sl);
+                    rlast[i] = std::max(rlast[i], sl);
+                }
+            }
+    int maxlen = 0;
+    for (int i = 0; i < neq; ++i) {
+        if (rlast[i] < 0 || rptr[i + 1] - rptr[i] > kFzMaxSlots) return FEMGPU_OK;
+        maxlen = std::max(maxlen, rlast[i] - rfirst[i] + 1);
+    }
+    if (maxlen > 15) return FEMGPU_OK;
+    // blocks: breadth-first patches over the row graph
+    std::vector<int> blk(neq, -1), order, bptr{0}, queue;
+    order.reserve(neq);
+    for (int seed = 0; seed < neq; ++seed) {
+        if (blk[seed] >= 0) continue;
+        const int b = static_cast<int>(bptr.size()) - 1;
+        int rows = 0, slots = 0;
+        queue.assign(1, seed);
+        blk[seed] = b;
+        rows = 1;
+        slots = rptr[seed + 1] - rptr[seed];
+        for (size_t h = 0; h < queue.size(); ++h) {
+            const int i = queue[h];
+            order.push_back(i);
+            for (int q = rptr[i]; q < rptr[i + 1] && rows < kFzRows; ++q) {
+                const int e = rinc[q] / 3;
+                for (int c = 0; c < 3 && rows < kFzRows; ++c) {
+                    const int v = ien[3 * static_cast<size_t>(e) + c];
+                    if (type[v] != 1) continue;
+                    const int j = invP[v];
+                    const int sj = rptr[j + 1] - rptr[j];
+                    if (blk[j] >= 0 || slots + sj > kFzMaxSlots) continue;
+                    blk[j] = b;
+                    queue.push_back(j);
+                    ++rows;
+                    slots += sj;
+                }
+            }
+        }
+        bptr.push_back(static_cast<int>(order.size()));
+    }
+    const int nb = static_cast<int>(bptr.size()) - 1;
+    std::vector<int> eptr{0}, elem, iptr{0};
+    std::vector<ushort4> eq;
+    std::vector<unsigned> inc;
+    std::vector<int4> rinfo(static_cast<size_t>(nb) * kFzRows, make_int4(-1, 0, 0, 0));
+    std::vector<int> qof(static_cast<size_t>(nel) * 3, -1), touched;
+    int slots_max = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int r0 = bptr[b], nr = bptr[b + 1] - r0;
+        int nslot = 0, kmax = 0;
+        touched.clear();
+        for (int t = 0; t < nr; ++t) {
+            const int i = order[r0 + t];
+            for (int q = rptr[i]; q < rptr[i + 1]; ++q) {
+                qof[rinc[q]] = nslot++;
+                touched.push_back(rinc[q] / 3);
+            }
+            kmax = std::max(kmax, rptr[i + 1] - rptr[i]);
+            rinfo[static_cast<size_t>(b) * kFzRows + t] = make_int4(i, rfirst[i], rlast[i] - rfirst[i] + 1, rptr[i + 1] - rptr[i]);
+        }
+        slots_max = std::max(slots_max, nslot);
+        std::sort(touched.begin(), touched.end());
+        touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+        for (int e : touched) {
+            elem.push_back(e);
+            unsigned short qa[3];
+            for (int a = 0; a < 3; ++a) {
+                const int q = qof[3 * static_cast<size_t>(e) + a];
+                qa[a] = q >= 0 ? static_cast<unsigned short>(q) : 0xffffu;
+            }
+            eq.push_back(make_ushort4(qa[0], qa[1], qa[2], 0));
+        }
+        eptr.push_back(static_cast<int>(elem.size()));
+        const size_t base = inc.size();
+        inc.resize(base + static_cast<size_t>(kmax) * kFzRows, 0xffffffffu);
+        for (int t = 0; t < nr; ++t) {
+            const int i = order[r0 + t];
+            for (int q = rptr[i]; q < rptr[i + 1]; ++q) {
+                const int ea = rinc[q], e = ea / 3, a = ea % 3;
+                unsigned code = static_cast<unsigned>(qof[ea]);
+                for (int c = 0; c < 3; ++c) {
+                    const int sl = scheme[9 * static_cast<size_t>(e) + 3 * a + c];
+                    const unsigned k = sl < nnz ? static_cast<unsigned>(sl - rfirst[i]) : 15u;
+                    code |= k << (16 + 4 * c);
+                }
+                inc[base + static_cast<size_t>(q - rptr[i]) * kFzRows + t] = code;
+            }
+        }
+        iptr.push_back(static_cast<int>(inc.size()));
+        for (int t = 0; t < nr; ++t) {   // qof back to -1 for the next block
+            const int i = order[r0 + t];
+            for (int q = rptr[i]; q < rptr[i + 1]; ++q) qof[rinc[q]] = -1;
+        }
+    }
+    if (inc.empty()) inc.push_back(0xffffffffu);
+    FG_TRY(M->fz_eptr.upload(eptr.data(), eptr.size(), st));
+    FG_TRY(M->fz_elem.upload(elem.data(), elem.size(), st));
+    FG_TRY(M->fz_eq.upload(eq.data(), eq.size(), st));
+    FG_TRY(M->fz_iptr.upload(iptr.data(), iptr.size(), st));
+    FG_TRY(M->fz_inc.upload(inc.data(), inc.size(), st));
+    FG_TRY(M->fz_row.upload(rinfo.data(), rinfo.size(), st));
+    FG_HIP(hipStreamSynchronize(st));   // the host arrays end here
+    M->fz_nblocks = nb;
+    M->fz_slots = slots_max;
+    M->fz_maxlen = maxlen;
+    M->fz_state = 1;
+    return FEMGPU_OK;
+}
+
 }  // namespace fg
 
 using namespace fg;
@@ -449,6 +686,7 @@ static int mesh_setup(femgpu_mesh *M, int nnodes, int nel, int neq, const double
     M->nel = nel;
     M->neq = neq;
     M->nnz = scheme ? nnz : 0;
+    M->fz_state = 0;   // the fused assembly's plan is rebuilt on first use
     hipStream_t s = ctx->stream;
     int rc = FEMGPU_OK;
     auto up = [&](auto &buf, auto *h, size_t n) {
@@ -570,6 +808,24 @@ extern "C" int femgpu_assemble_transport_transient(femgpu_mesh *M, const femgpu_
     femgpu_ctx *ctx = M->ctx;
     double *Ae = d_Ae ? d_Ae : M->ke.p;
     const ElemIn in = elem_in(M, pb, d_u, d_du, alpha * dt);
+    if (!d_Ae && d_AA && M->aa_ptr.p) {
+        // CSR values and residual only: the fused kernel, no element arrays
+        if (M->fz_state == 0) FG_TRY(build_fused_plan(M));
+        if (M->fz_state == 1) {
+            const size_t lds = sizeof(double) * kFzVals * static_cast<size_t>(std::max(M->fz_slots, 1));
+            const int grid = xcd_grid(M->fz_nblocks);
+            if (M->fz_maxlen <= 8)
+                k_transient_fused<8><<<grid, kFzRows, lds, ctx->stream>>>(
+                    M->fz_nblocks, in, M->fz_eptr.p, M->fz_elem.p, M->fz_eq.p, M->fz_iptr.p, M->fz_inc.p, M->fz_row.p,
+                    M->nnz, d_AA, d_R);
+            else
+                k_transient_fused<16><<<grid, kFzRows, lds, ctx->stream>>>(
+                    M->fz_nblocks, in, M->fz_eptr.p, M->fz_elem.p, M->fz_eq.p, M->fz_iptr.p, M->fz_inc.p, M->fz_row.p,
+                    M->nnz, d_AA, d_R);
+            FG_HIP(hipGetLastError());
+            return FEMGPU_OK;
+        }
+    }
     if (M->nel > 0) {
         k_elem_transient<<<xcd_grid((M->nel + 255) / 256), 256, 0, ctx->stream>>>(M->nel, in, Ae, M->lift.p);
         FG_HIP(hipGetLastError());

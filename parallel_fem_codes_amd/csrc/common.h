@@ -217,6 +217,11 @@ struct femgpu_csr {
     fg::FactorPlan fplan;           // numeric factorisation (level schedule)
     fg::FactorPlan fplan_nat;       // the same lists in row order (parallel FP32 factorisation)
     fg::StreamFactor sfac;          // streamed single-workgroup factorisation (when available)
+    // FP64 parallel ILU(0) applied by Jacobi iterations (the mixed solver's preconditioner at FP64):
+    // factors in the sliced-ELL layout, strict triangles in the Jacobi sliced-ELL arrays
+    int prec64_jacobi = 0;              // FP64 ILU apply: 0 exact sweeps, k > 0 k Jacobi iterations
+    bool par64_factored = false;        // the FP64 parallel factors hold the current values
+    fg::DevBuf<double> par64_v, par64_a, par64_b, par64_d, par64_db, ell64_vl, ell64_vu, jac64_a, jac64_b;
     int spmv_lanes = 8;             // lanes per row of the FP64 CSR kernels; 0: sliced ELL (sell_val64)
     int plans = 3;                  // FEMGPU_PLANS_* built at creation
 };
@@ -258,11 +263,27 @@ struct femgpu_mesh {
     fg::DevBuf<double> fe, lift;        // per element: s loads / residual entries, 9 coupling terms
     fg::DevBuf<int> mask;               // per element: Dirichlet-vertex bit mask
     fg::DevBuf<double> ke;              // element matrices when the caller passes none
+    // Fused transient assembly (CSR values and residual with no element arrays in memory): rows in
+    // blocks of compact mesh patches; per block its elements (sorted, with the LDS slot of each
+    // vertex row in the block), per row thread its incidences (LDS slot | CSR position codes) and
+    // {row, first CSR slot, row length, incidences}. Built on first use; fz_state 0 unknown,
+    // 1 built, -1 not applicable (a row longer than 15 or a block over the LDS budget).
+    int fz_state = 0, fz_nblocks = 0, fz_slots = 0, fz_maxlen = 0;
+    fg::DevBuf<int> fz_eptr, fz_elem, fz_iptr;
+    fg::DevBuf<ushort4> fz_eq;
+    fg::DevBuf<unsigned> fz_inc;
+    fg::DevBuf<int4> fz_row;
 };
 
 namespace fg {
 
+// The FP64 ILU(0) preconditioner of A is ready for its current apply mode.
+inline bool ilu64_ready(const femgpu_csr *A) { return A->prec64_jacobi > 0 ? A->par64_factored : A->factored; }
+
 // Host-side integer work (structure.cpp).
+int csr_init_vec(int nel, int s, int neq, int *lm, int reorder, int permute_lm, int ndofs_total, const int *type,
+                 int *invP_id, std::vector<int> &IA, std::vector<int> &JA, std::vector<int> &scheme,
+                 std::vector<int> &perm);
 int build_packed_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, const int *diag, bool upper,
                       PackedSweep &plan);
 int packed_refresh(femgpu_ctx *ctx, femgpu_csr *A);
@@ -297,6 +318,8 @@ int blockdiag_size_check(const femgpu_blockdiag *B, int NEQ, int nnodes, int ndo
 int launch_blockdiag_apply(femgpu_ctx *ctx, const femgpu_blockdiag *B, const double *p, double *z, const int *skip);
 int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
 int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip);
+int launch_parilu64(femgpu_ctx *ctx, femgpu_csr *A, int sweeps);
+int launch_jacobi_apply64(femgpu_ctx *ctx, femgpu_csr *A, int it, const double *b, double *x, const int *skip);
 int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA);
 int build_parilu_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
 int launch_parilu_to_csr(femgpu_ctx *ctx, femgpu_csr *A);

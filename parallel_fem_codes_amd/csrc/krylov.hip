@@ -791,7 +791,7 @@ extern "C" int femgpu_pgmres(femgpu_ctx *ctx, const femgpu_system *S, int kmax, 
                              const double *d_B, double *d_X, femgpu_solve_info *info)
 {
     FG_CHECK(ctx && S && (S->csr || S->ebe) && d_B && d_X, FEMGPU_ERR_ARG, "femgpu_pgmres: bad arguments");
-    FG_CHECK(!S->ilu || S->ilu->factored, FEMGPU_ERR_STATE, "femgpu_pgmres: ILU(0) preconditioner not factored");
+    FG_CHECK(!S->ilu || ilu64_ready(S->ilu), FEMGPU_ERR_STATE, "femgpu_pgmres: ILU(0) preconditioner not factored");
     const int n = sys_n(S);
     FG_CHECK(!S->ilu || S->ilu->n == n, FEMGPU_ERR_ARG, "femgpu_pgmres: preconditioner size mismatch");
     if (!S->ilu && S->bdiag) FG_TRY(blockdiag_check(S->bdiag, n));

@@ -133,11 +133,11 @@ __global__ __launch_bounds__(256) void k_jacobi_upper(int n, const int *__restri
 // ---- the same iteration on the sliced-ELL layout of the FP32 matrix (slices of 64 rows, entry k of
 // row 64s + l at sell_ptr[s] + 64k + l): every per-row load and store of a step is one contiguous
 // run across the wave, and the update lists are stored the same way (padded per slice).
-template <int MAXW>
+template <typename T, int MAXW>
 __global__ __launch_bounds__(256) void k_parilu_init_sell(int n, const int *__restrict__ sptr,
-                                                          const int *__restrict__ col, const float *__restrict__ AA,
+                                                          const int *__restrict__ col, const T *__restrict__ AA,
                                                           const int *__restrict__ rinfo, const int *__restrict__ dpos,
-                                                          float *__restrict__ LU, float *__restrict__ Dinv)
+                                                          T *__restrict__ LU, T *__restrict__ Dinv)
 {
     const int i = xcd_block() * 256 + threadIdx.x;
     if (i >= n) return;
@@ -146,41 +146,41 @@ __global__ __launch_bounds__(256) void k_parilu_init_sell(int n, const int *__re
     // all loads of the row first (steps past the slice width re-read step 0), then the two-level
     // gathers of the column pivots
     int c[MAXW];
-    float a[MAXW], piv[MAXW];
+    T a[MAXW], piv[MAXW];
 #pragma unroll
     for (int k = 0; k < MAXW; ++k) {
         const int e = p0 + ((k < w ? k : 0) << 6);
         c[k] = col[e];
         a[k] = AA[e];
     }
-    const float own = AA[dpos[i]];
+    const T own = AA[dpos[i]];
 #pragma unroll
     for (int k = 0; k < MAXW; ++k) piv[k] = AA[dpos[c[k]]];
 #pragma unroll
     for (int k = 0; k < MAXW; ++k)
         if (k < w) LU[p0 + (k << 6)] = k < nL ? a[k] / piv[k] : a[k];
-    Dinv[i] = 1.0f / own;
+    Dinv[i] = T(1) / own;
 }
 
-template <int MAXW>
+template <typename T, int MAXW>
 __global__ __launch_bounds__(256) void k_parilu_sweep_sell(int n, const int *__restrict__ sptr,
-                                                           const int *__restrict__ col, const float *__restrict__ AA,
+                                                           const int *__restrict__ col, const T *__restrict__ AA,
                                                            const int *__restrict__ rinfo,
                                                            const int *__restrict__ uptr, const int *__restrict__ usrc,
                                                            const unsigned char *__restrict__ ucode,
-                                                           const float *__restrict__ LUo, const float *__restrict__ Dio,
-                                                           float *__restrict__ LUn, float *__restrict__ Din)
+                                                           const T *__restrict__ LUo, const T *__restrict__ Dio,
+                                                           T *__restrict__ LUn, T *__restrict__ Din)
 {
     const int i = xcd_block() * 256 + threadIdx.x;
     if (i >= n) return;
     const int s = i >> 6, l = i & 63, p0 = sptr[s] + l, w = (sptr[s + 1] - sptr[s]) >> 6;
     const int u0 = uptr[s] + l, nu = (uptr[s + 1] - uptr[s]) >> 6;
     const int nL = rinfo[i] & 0xff;
-    float v[MAXW], lo[MAXW];
+    T v[MAXW], lo[MAXW];
 #pragma unroll
     for (int k = 0; k < MAXW; ++k) {
-        v[k] = k < w ? AA[p0 + (k << 6)] : 0.0f;
-        lo[k] = k < w ? LUo[p0 + (k << 6)] : 0.0f;
+        v[k] = k < w ? AA[p0 + (k << 6)] : T(0);
+        lo[k] = k < w ? LUo[p0 + (k << 6)] : T(0);
     }
     // update terms in groups of four: the group's codes, sources and factor values are loaded
     // together (positions past the row's list clamped to its first, their terms skipped), then
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) void k_parilu_sweep_sell(int n, const int *__r
     constexpr int G = 4;
     for (int u = 0; u < nu; u += G) {
         unsigned code[G];
-        float other[G];
+        T other[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int q = u0 + ((u + g < nu ? u + g : 0) << 6);
@@ -199,41 +199,42 @@ __global__ __launch_bounds__(256) void k_parilu_sweep_sell(int n, const int *__r
         for (int g = 0; g < G; ++g)
             if (code[g] != 0xffu) {
                 const unsigned t = code[g] >> 4, dest = code[g] & 15u;
-                float lt = 0.0f;
+                T lt = T(0);
 #pragma unroll
                 for (int k = 0; k < MAXW; ++k) lt = (k == static_cast<int>(t)) ? lo[k] : lt;
-                const float c = lt * other[g];
+                const T c = lt * other[g];
 #pragma unroll
                 for (int k = 0; k < MAXW; ++k)
                     if (k == static_cast<int>(dest)) v[k] -= c;
             }
     }
-    float piv = 1.0f;
+    T piv = T(1);
 #pragma unroll
     for (int k = 0; k < MAXW; ++k)
         if (k < w) {
             const int e = p0 + (k << 6);
-            const float out = k < nL ? v[k] * Dio[col[e]] : v[k];
+            const T out = k < nL ? v[k] * Dio[col[e]] : v[k];
             LUn[e] = out;
             if (k == nL) piv = out;
         }
-    Din[i] = 1.0f / piv;
+    Din[i] = T(1) / piv;
 }
 
 // The strict triangles of the SELL-layout factors into the Jacobi ELL arrays.
+template <typename T>
 __global__ void k_parilu_sell_to_ell(int n, const int *__restrict__ sptr, const int *__restrict__ rinfo,
-                                     const float *__restrict__ LU, const int *__restrict__ sl,
-                                     const int *__restrict__ su, float *__restrict__ vl, float *__restrict__ vu)
+                                     const T *__restrict__ LU, const int *__restrict__ sl,
+                                     const int *__restrict__ su, T *__restrict__ vl, T *__restrict__ vu)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = i >> 6, l = i & 63;
     const int p0 = sptr[s] + l, nL = rinfo[i] & 0xff, len = rinfo[i] >> 8;
     const int ql = sl[s] + l, wl = (sl[s + 1] - sl[s]) >> 6, qu = su[s] + l, wu = (su[s + 1] - su[s]) >> 6;
-    for (int k = 0; k < wl; ++k) vl[ql + (k << 6)] = k < nL ? LU[p0 + (k << 6)] : 0.0f;
+    for (int k = 0; k < wl; ++k) vl[ql + (k << 6)] = k < nL ? LU[p0 + (k << 6)] : T(0);
     for (int k = 0; k < wu; ++k) {
         const int e = nL + 1 + k;
-        vu[qu + (k << 6)] = e < len ? LU[p0 + (e << 6)] : 0.0f;
+        vu[qu + (k << 6)] = e < len ? LU[p0 + (e << 6)] : T(0);
     }
 }
 
@@ -362,17 +363,17 @@ __global__ void k_jacobi_ell_values(int n, const int *__restrict__ IA, const int
 // one of the plain loop over the slice, in the same order.
 constexpr int kJacRows = 2;
 
-template <int K>
+template <typename T, int K>
 struct EllRows {
     int col[kJacRows][K];
-    float val[kJacRows][K];
+    T val[kJacRows][K];
     int w[kJacRows];
 };
 
 // Loads the slice entries of the thread's rows; w[r] = 0 for rows past n and for empty slices.
-template <int K>
+template <typename T, int K>
 __device__ __forceinline__ void ell_load(int n, int blk, const int *__restrict__ sp, const int *__restrict__ col,
-                                         const float *__restrict__ val, EllRows<K> &e)
+                                         const T *__restrict__ val, EllRows<T, K> &e)
 {
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
@@ -391,25 +392,25 @@ __device__ __forceinline__ void ell_load(int n, int blk, const int *__restrict__
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 e.col[r][k] = ii;
-                e.val[r][k] = 0.0f;
+                e.val[r][k] = T(0);
             }
         }
     }
 }
 
 // y_out = b - L_s y_in (y_in == nullptr: y_in = b), sliced ELL.
-template <int K>
+template <typename T, int K>
 __global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__restrict__ sl, const int *__restrict__ cl,
-                                                          const float *__restrict__ vl, const float *__restrict__ b,
-                                                          const float *__restrict__ yin, float *__restrict__ yout,
+                                                          const T *__restrict__ vl, const T *__restrict__ b,
+                                                          const T *__restrict__ yin, T *__restrict__ yout,
                                                           const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    const float *src = yin ? yin : b;
+    const T *src = yin ? yin : b;
     const int blk = xcd_block();
-    EllRows<K> e;
-    ell_load<K>(n, blk, sl, cl, vl, e);
-    float bi[kJacRows];
+    EllRows<T, K> e;
+    ell_load<T, K>(n, blk, sl, cl, vl, e);
+    T bi[kJacRows];
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
         const int i = (blk * kJacRows + r) * 256 + threadIdx.x;
@@ -417,10 +418,10 @@ __global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__re
     }
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
-        float acc = 0.0f;
+        T acc = T(0);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const float t = e.val[r][k] * src[e.col[r][k]];
+            const T t = e.val[r][k] * src[e.col[r][k]];
             acc = k < e.w[r] ? acc + t : acc;
         }
         const int i = (blk * kJacRows + r) * 256 + threadIdx.x;
@@ -429,17 +430,17 @@ __global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__re
 }
 
 // z_out = Dinv (y - U_s z_in) (z_in == nullptr: z_in = Dinv y), sliced ELL.
-template <int K>
+template <typename T, int K>
 __global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, const int *__restrict__ su, const int *__restrict__ cu,
-                                                          const float *__restrict__ vu, const float *__restrict__ Dinv,
-                                                          const float *__restrict__ y, const float *__restrict__ zin,
-                                                          float *__restrict__ zout, const int *__restrict__ skip)
+                                                          const T *__restrict__ vu, const T *__restrict__ Dinv,
+                                                          const T *__restrict__ y, const T *__restrict__ zin,
+                                                          T *__restrict__ zout, const int *__restrict__ skip)
 {
     if (skip && *skip) return;
     const int blk = xcd_block();
-    EllRows<K> e;
-    ell_load<K>(n, blk, su, cu, vu, e);
-    float di[kJacRows], yi[kJacRows];
+    EllRows<T, K> e;
+    ell_load<T, K>(n, blk, su, cu, vu, e);
+    T di[kJacRows], yi[kJacRows];
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
         const int i = (blk * kJacRows + r) * 256 + threadIdx.x, ii = i < n ? i : n - 1;
@@ -448,11 +449,11 @@ __global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, const int *__re
     }
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
-        float acc = 0.0f;
+        T acc = T(0);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int j = e.col[r][k];
-            const float t = e.val[r][k] * (zin ? zin[j] : Dinv[j] * y[j]);
+            const T t = e.val[r][k] * (zin ? zin[j] : Dinv[j] * y[j]);
             acc = k < e.w[r] ? acc + t : acc;
         }
         const int i = (blk * kJacRows + r) * 256 + threadIdx.x;
@@ -525,18 +526,18 @@ static int parilu_sell(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
     float *L0 = (sweeps % 2 == 0) ? A->par_a.p : A->par_b.p, *L1 = (sweeps % 2 == 0) ? A->par_b.p : A->par_a.p;
     float *D0 = (sweeps % 2 == 0) ? A->Dinv32.p : A->Dinv32b.p, *D1 = (sweeps % 2 == 0) ? A->Dinv32b.p : A->Dinv32.p;
     if (A->par_maxw <= 8)
-        k_parilu_init_sell<8><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
+        k_parilu_init_sell<float, 8><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
                                                              A->par_rinfo.p, A->par_dpos.p, L0, D0);
     else
-        k_parilu_init_sell<16><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
+        k_parilu_init_sell<float, 16><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
                                                               A->par_rinfo.p, A->par_dpos.p, L0, D0);
     for (int s = 0; s < sweeps; ++s) {
         if (A->par_maxw <= 8)
-            k_parilu_sweep_sell<8><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
+            k_parilu_sweep_sell<float, 8><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
                                                                    A->par_rinfo.p, A->par_uptr.p, A->par_usrc.p,
                                                                    A->par_ucode.p, L0, D0, L1, D1);
         else
-            k_parilu_sweep_sell<16><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
+            k_parilu_sweep_sell<float, 16><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
                                                                     A->par_rinfo.p, A->par_uptr.p, A->par_usrc.p,
                                                                     A->par_ucode.p, L0, D0, L1, D1);
         std::swap(L0, L1);
@@ -545,7 +546,7 @@ static int parilu_sell(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
     FG_HIP(hipGetLastError());
     A->lu32_in_sell = true;
     if (A->ell_kl > 0) {
-        k_parilu_sell_to_ell<<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->par_rinfo.p, A->par_a.p,
+        k_parilu_sell_to_ell<float><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->par_rinfo.p, A->par_a.p,
                                                             A->ell_sl.p, A->ell_su.p, A->ell_vl.p, A->ell_vu.p);
         FG_HIP(hipGetLastError());
         return FEMGPU_OK;
@@ -589,28 +590,106 @@ int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A)
     return refresh_ell(ctx, A);
 }
 
-// x = (U~ L~)^-1 b approximated with it Jacobi iterations per triangle on the FP32 factors in
-// LU32/Dinv32. b may alias x.
-template <int K>
-static void jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip)
+// x = (U~ L~)^-1 b approximated with it Jacobi iterations per triangle on factors in the Jacobi
+// sliced-ELL layout (vl, vu strict triangles, dinv inverse pivots); ta, tb: two work vectors.
+// b may alias x.
+template <typename T, int K>
+static void jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, int it, const T *vl, const T *vu, const T *dinv, T *ta, T *tb,
+                       const T *b, T *x, const int *skip)
 {
     const int n = A->n, grid = xcd_grid((n + 256 * kJacRows - 1) / (256 * kJacRows));
-    float *t[2] = {A->jac_a.p, A->jac_b.p};
-    const float *prev = nullptr;
+    T *t[2] = {ta, tb};
+    const T *prev = nullptr;
     for (int k = 0; k < it; ++k) {
-        float *out = t[k & 1];
-        k_jacobi_lower_ell<K><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, b, prev, out, skip);
+        T *out = t[k & 1];
+        k_jacobi_lower_ell<T, K><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, vl, b, prev, out, skip);
         prev = out;
     }
-    const float *y = prev;
-    float *other = (y == t[0]) ? t[1] : t[0];
+    const T *y = prev;
+    T *other = (y == t[0]) ? t[1] : t[0];
     prev = nullptr;
     for (int k = 1; k <= it; ++k) {
-        float *out = ((it - k) % 2 == 0) ? x : other;
-        k_jacobi_upper_ell<K><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p, A->Dinv32.p, y,
-                                                             prev, out, skip);
+        T *out = ((it - k) % 2 == 0) ? x : other;
+        k_jacobi_upper_ell<T, K><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, vu, dinv, y, prev, out, skip);
         prev = out;
     }
+}
+
+// The current values of A into the sliced-ELL layout in FP64 (padding 0).
+__global__ void k_sell_fill64(int n, const int *__restrict__ IA, const int *__restrict__ sptr,
+                              const double *__restrict__ AA, double *__restrict__ out)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    const int s = row >> 6, l = row & 63, p0 = sptr[s], w = (sptr[s + 1] - p0) >> 6;
+    const int a0 = IA[row], len = IA[row + 1] - a0;
+    for (int k = 0; k < w; ++k) out[p0 + (k << 6) + l] = k < len ? AA[a0 + k] : 0.0;
+}
+
+int launch_parilu64(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
+{
+    const int n = A->n;
+    A->par64_factored = false;
+    if (n == 0) {
+        A->par64_factored = true;
+        return FEMGPU_OK;
+    }
+    FG_CHECK(A->par_ok && A->ell_kl > 0, FEMGPU_ERR_ARG,
+             "FP64 parallel ILU(0): needs the sliced-ELL plans (rows and strict triangles of at most 16 entries)");
+    FG_CHECK(sweeps >= 1 && sweeps <= 64, FEMGPU_ERR_ARG, "FP64 parallel ILU(0): %d sweeps out of range", sweeps);
+    const size_t tot = A->sell_col.n, n1 = static_cast<size_t>(n) + 1;
+    FG_TRY(A->par64_v.alloc(tot));
+    FG_TRY(A->par64_a.alloc(tot));
+    FG_TRY(A->par64_b.alloc(tot));
+    FG_TRY(A->par64_d.alloc(n1));
+    FG_TRY(A->par64_db.alloc(n1));
+    FG_TRY(A->ell64_vl.alloc(A->ell_cl.n));
+    FG_TRY(A->ell64_vu.alloc(A->ell_cu.n));
+    FG_TRY(A->jac64_a.alloc(n1));
+    FG_TRY(A->jac64_b.alloc(n1));
+    const int grid = xcd_grid((n + 255) / 256);
+    k_sell_fill64<<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, A->IA.p, A->sell_ptr.p, A->AA.p, A->par64_v.p);
+    // the final iterate must land in par64_a / par64_d
+    double *L0 = (sweeps % 2 == 0) ? A->par64_a.p : A->par64_b.p, *L1 = (sweeps % 2 == 0) ? A->par64_b.p : A->par64_a.p;
+    double *D0 = (sweeps % 2 == 0) ? A->par64_d.p : A->par64_db.p, *D1 = (sweeps % 2 == 0) ? A->par64_db.p : A->par64_d.p;
+    if (A->par_maxw <= 8)
+        k_parilu_init_sell<double, 8><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->par64_v.p,
+                                                                    A->par_rinfo.p, A->par_dpos.p, L0, D0);
+    else
+        k_parilu_init_sell<double, 16><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->par64_v.p,
+                                                                     A->par_rinfo.p, A->par_dpos.p, L0, D0);
+    for (int k = 0; k < sweeps; ++k) {
+        if (A->par_maxw <= 8)
+            k_parilu_sweep_sell<double, 8><<<grid, 256, 0, ctx->stream>>>(
+                n, A->sell_ptr.p, A->sell_col.p, A->par64_v.p, A->par_rinfo.p, A->par_uptr.p, A->par_usrc.p,
+                A->par_ucode.p, L0, D0, L1, D1);
+        else
+            k_parilu_sweep_sell<double, 16><<<grid, 256, 0, ctx->stream>>>(
+                n, A->sell_ptr.p, A->sell_col.p, A->par64_v.p, A->par_rinfo.p, A->par_uptr.p, A->par_usrc.p,
+                A->par_ucode.p, L0, D0, L1, D1);
+        std::swap(L0, L1);
+        std::swap(D0, D1);
+    }
+    k_parilu_sell_to_ell<double><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->par_rinfo.p, A->par64_a.p,
+                                                                A->ell_sl.p, A->ell_su.p, A->ell64_vl.p, A->ell64_vu.p);
+    FG_HIP(hipGetLastError());
+    A->par64_factored = true;
+    return FEMGPU_OK;
+}
+
+int launch_jacobi_apply64(femgpu_ctx *ctx, femgpu_csr *A, int it, const double *b, double *x, const int *skip)
+{
+    const int n = A->n;
+    if (n == 0) return FEMGPU_OK;
+    FG_CHECK(it >= 1 && it <= 1024 && A->ell_kl > 0, FEMGPU_ERR_ARG, "FP64 Jacobi sweeps: %d iterations", it);
+    const int w = std::max(A->ell_kl, A->ell_ku);
+    const double *vl = A->ell64_vl.p, *vu = A->ell64_vu.p, *d = A->par64_d.p;
+    double *ta = A->jac64_a.p, *tb = A->jac64_b.p;
+    if (w <= 4) jacobi_ell<double, 4>(ctx, A, it, vl, vu, d, ta, tb, b, x, skip);
+    else if (w <= 8) jacobi_ell<double, 8>(ctx, A, it, vl, vu, d, ta, tb, b, x, skip);
+    else jacobi_ell<double, 16>(ctx, A, it, vl, vu, d, ta, tb, b, x, skip);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
 }
 
 int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const float *rhs, const float *in,
@@ -623,23 +702,23 @@ int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const f
         const int w = std::max(A->ell_kl, A->ell_ku);
         if (upper) {
             if (w <= 4)
-                k_jacobi_upper_ell<4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
+                k_jacobi_upper_ell<float, 4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
                                                                      A->Dinv32.p, rhs, in, out, nullptr);
             else if (w <= 8)
-                k_jacobi_upper_ell<8><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
+                k_jacobi_upper_ell<float, 8><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
                                                                      A->Dinv32.p, rhs, in, out, nullptr);
             else
-                k_jacobi_upper_ell<16><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
+                k_jacobi_upper_ell<float, 16><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
                                                                       A->Dinv32.p, rhs, in, out, nullptr);
         } else {
             if (w <= 4)
-                k_jacobi_lower_ell<4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs, in,
+                k_jacobi_lower_ell<float, 4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs, in,
                                                                      out, nullptr);
             else if (w <= 8)
-                k_jacobi_lower_ell<8><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs, in,
+                k_jacobi_lower_ell<float, 8><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs, in,
                                                                      out, nullptr);
             else
-                k_jacobi_lower_ell<16><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs,
+                k_jacobi_lower_ell<float, 16><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs,
                                                                       in, out, nullptr);
         }
     } else {
@@ -663,9 +742,11 @@ int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b
     FG_CHECK(it >= 1 && it <= 1024, FEMGPU_ERR_ARG, "Jacobi sweeps: %d iterations out of range", it);
     if (A->ell_kl > 0) {
         const int w = std::max(A->ell_kl, A->ell_ku);
-        if (w <= 4) jacobi_ell<4>(ctx, A, it, b, x, skip);
-        else if (w <= 8) jacobi_ell<8>(ctx, A, it, b, x, skip);
-        else jacobi_ell<16>(ctx, A, it, b, x, skip);
+        const float *vl = A->ell_vl.p, *vu = A->ell_vu.p, *d = A->Dinv32.p;
+        float *ta = A->jac_a.p, *tb = A->jac_b.p;
+        if (w <= 4) jacobi_ell<float, 4>(ctx, A, it, vl, vu, d, ta, tb, b, x, skip);
+        else if (w <= 8) jacobi_ell<float, 8>(ctx, A, it, vl, vu, d, ta, tb, b, x, skip);
+        else jacobi_ell<float, 16>(ctx, A, it, vl, vu, d, ta, tb, b, x, skip);
         FG_HIP(hipGetLastError());
         return FEMGPU_OK;
     }

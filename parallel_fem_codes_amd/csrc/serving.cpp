@@ -26,6 +26,7 @@
 // achieves over two units, and keep the best.
 #include <algorithm>
 #include <chrono>
+#include <cstdint>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -48,9 +49,9 @@ struct Request {
     std::vector<double> x, y, ubc;
     std::vector<int> type, ien;
     femgpu_transport_problem pb{};
-    // prepared
+    // prepared (node arrays renumbered along a Morton curve; order[new] = the request's node)
     int n = 0, nnz = 0;
-    std::vector<int> invP, IA, JA, scheme;
+    std::vector<int> invP, IA, JA, scheme, order;
     size_t bytes = 0;
     int prep_rc = FEMGPU_OK;
     std::string prep_err;
@@ -132,8 +133,75 @@ size_t footprint(const femgpu_server_opts &o, int n, int nnz, int nel)
            static_cast<size_t>(nel) * 200 + (1u << 20);
 }
 
+// The low 21 bits of v spread to the even bit positions (one coordinate of a 2D Morton code).
+uint64_t spread2(uint64_t v)
+{
+    v &= 0x1fffff;
+    v = (v | (v << 16)) & 0x0000ffff0000ffffULL;
+    v = (v | (v << 8)) & 0x00ff00ff00ff00ffULL;
+    v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0fULL;
+    v = (v | (v << 2)) & 0x3333333333333333ULL;
+    v = (v | (v << 1)) & 0x5555555555555555ULL;
+    return v;
+}
+
+// Nodes renumbered along a Morton (Z-order) curve of their coordinates before anything else is
+// built: the unknown numbering, the element pattern and the reordering then run on a numbering
+// with spatial locality (a Delaunay mesh of random points arrives in random node order, which makes
+// every later pass a random walk through memory). Elements keep their order; the solution is
+// mapped back to the request's node order.
+void spatial_renumber(Request &r)
+{
+    const int nn = r.nnodes;
+    double x0 = 0, x1 = 0, y0 = 0, y1 = 0;
+    if (nn > 0) {
+        x0 = x1 = r.x[0];
+        y0 = y1 = r.y[0];
+    }
+    for (int v = 1; v < nn; ++v) {
+        x0 = std::min(x0, r.x[v]);
+        x1 = std::max(x1, r.x[v]);
+        y0 = std::min(y0, r.y[v]);
+        y1 = std::max(y1, r.y[v]);
+    }
+    const double sx = x1 > x0 ? 2097151.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 2097151.0 / (y1 - y0) : 0.0;
+    std::vector<std::pair<uint64_t, int>> key(nn);
+    for (int v = 0; v < nn; ++v) {
+        const uint64_t qx = static_cast<uint64_t>((r.x[v] - x0) * sx), qy = static_cast<uint64_t>((r.y[v] - y0) * sy);
+        key[v] = {spread2(qx) | (spread2(qy) << 1), v};
+    }
+    std::sort(key.begin(), key.end());
+    r.order.resize(nn);
+    std::vector<int> newid(nn);
+    for (int i = 0; i < nn; ++i) {
+        r.order[i] = key[i].second;
+        newid[key[i].second] = i;
+    }
+    std::vector<double> x(nn), y(nn), ubc(nn);
+    std::vector<int> type(nn);
+    for (int i = 0; i < nn; ++i) {
+        const int v = r.order[i];
+        x[i] = r.x[v];
+        y[i] = r.y[v];
+        ubc[i] = r.ubc[v];
+        type[i] = r.type[v];
+    }
+    r.x.swap(x);
+    r.y.swap(y);
+    r.ubc.swap(ubc);
+    r.type.swap(type);
+    for (auto &v : r.ien) v = newid[v];
+}
+
 void prepare(Request &r, const femgpu_server_opts &o)
 {
+    for (int v : r.ien)
+        if (v < 0 || v >= r.nnodes) {
+            r.prep_rc = FEMGPU_ERR_ARG;
+            r.prep_err = "vertex out of range";
+            return;
+        }
+    spatial_renumber(r);
     r.invP.assign(r.nnodes, -1);
     int neq = femgpu_number_unknowns(r.nnodes, 1, r.type.data(), r.invP.data());
     if (neq < 0) {
@@ -143,23 +211,16 @@ void prepare(Request &r, const femgpu_server_opts &o)
     }
     std::vector<int> lm(static_cast<size_t>(r.nel) * 3);
     int rc = femgpu_fill_lm(r.nel, 1, neq, r.ien.data(), r.type.data(), r.invP.data(), lm.data());
-    int nnz = rc == FEMGPU_OK ? femgpu_csr_nnz(r.nel, 3, neq, lm.data()) : rc;
+    std::vector<int> perm;
+    const int nnz = rc == FEMGPU_OK ? fg::csr_init_vec(r.nel, 3, neq, lm.data(), 1, 0, r.nnodes, r.type.data(),
+                                                       r.invP.data(), r.IA, r.JA, r.scheme, perm)
+                                    : rc;
     if (nnz < 0) {
         r.prep_rc = nnz;
         r.prep_err = femgpu_last_error();
         return;
     }
-    r.IA.resize(static_cast<size_t>(neq) + 1);
-    r.JA.resize(std::max(nnz, 1));
-    r.scheme.resize(static_cast<size_t>(r.nel) * 9);
-    std::vector<int> perm(std::max(neq, 1));
-    rc = femgpu_csr_init(r.nel, 3, neq, lm.data(), 1, 0, r.nnodes, r.type.data(), r.invP.data(), r.IA.data(),
-                         r.JA.data(), r.scheme.data(), perm.data());
-    if (rc < 0) {
-        r.prep_rc = rc;
-        r.prep_err = femgpu_last_error();
-        return;
-    }
+    if (r.JA.empty()) r.JA.push_back(0);
     r.n = neq;
     r.nnz = nnz;
     r.bytes = footprint(o, neq, nnz, r.nel);
@@ -310,8 +371,8 @@ void run_unit(femgpu_server *S, Executor &E, std::vector<Req> &unit, long long b
     for (int s = 0; s < K; ++s) {
         Request &r = *unit[s];
         r.u.assign(r.nnodes, 0.0);
-        for (int v = 0; v < r.nnodes; ++v)
-            r.u[v] = r.type[v] == 0 ? r.ubc[v] : X[row_off[s] + r.invP[v]];
+        for (int v = 0; v < r.nnodes; ++v)   // back to the request's node order
+            r.u[r.order[v]] = r.type[v] == 0 ? r.ubc[v] : X[row_off[s] + r.invP[v]];
         r.info.n = r.n;
         r.info.nnz = r.nnz;
         r.info.batch_id = static_cast<int>(batch_id);

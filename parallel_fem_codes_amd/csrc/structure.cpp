@@ -269,12 +269,13 @@ extern "C" int femgpu_csr_nnz(int nel, int s, int neq, const int *lm)
     return static_cast<int>(JA.size());
 }
 
-extern "C" int femgpu_csr_init(int nel, int s, int neq, int *lm, int reorder, int permute_lm,
-                               int ndofs_total, const int *type, int *invP_id, int *IA, int *JA,
-                               int *scheme, int *perm)
+// csr_Initialization (ST/csr_Initialization.c:5-571) into vectors: the sorted element pattern, the
+// slot of every element entry, the reordering and the permuted structure (see femgpu_csr_init).
+int fg::csr_init_vec(int nel, int s, int neq, int *lm, int reorder, int permute_lm, int ndofs_total, const int *type,
+                     int *invP_id, std::vector<int> &IA, std::vector<int> &JA, std::vector<int> &scheme,
+                     std::vector<int> &perm)
 {
-    FG_CHECK(nel >= 0 && s > 0 && neq >= 0 && lm && IA && JA && scheme && perm, FEMGPU_ERR_ARG,
-             "femgpu_csr_init: bad arguments");
+    FG_CHECK(nel >= 0 && s > 0 && neq >= 0 && lm, FEMGPU_ERR_ARG, "femgpu_csr_init: bad arguments");
     FG_CHECK(reorder >= 0 && reorder <= 2, FEMGPU_ERR_ARG,
              "femgpu_csr_init: reorder must be 0 (NOT), 1 (SYMRCM) or 2 (Spectral)");
     std::vector<int> ia, ja;
@@ -282,6 +283,7 @@ extern "C" int femgpu_csr_init(int nel, int s, int neq, int *lm, int reorder, in
     const int nnz = static_cast<int>(ja.size());
     // Slot of every element entry in the unpermuted structure (csr_Initialization.c:101-191).
     const size_t ss = static_cast<size_t>(s) * s;
+    scheme.resize(static_cast<size_t>(nel) * ss);
     for (int e = 0; e < nel; ++e) {
         const int *L = lm + static_cast<size_t>(e) * s;
         for (int a = 0; a < s; ++a)
@@ -296,15 +298,17 @@ extern "C" int femgpu_csr_init(int nel, int s, int neq, int *lm, int reorder, in
     }
     // Reordering (reordering.c:3-22), then rows and columns permuted with per-row re-sort,
     // carrying the original slot of every entry (reordering.c:101-167).
-    std::iota(perm, perm + neq, 0);
-    if (reorder == 1) FG_TRY(femgpu_symrcm(neq, ia.data(), ja.data(), perm));
-    if (reorder == 2) FG_TRY(femgpu_spectral(neq, ia.data(), ja.data(), perm));
+    perm.resize(neq);
+    std::iota(perm.begin(), perm.end(), 0);
+    if (reorder == 1) FG_TRY(femgpu_symrcm(neq, ia.data(), ja.data(), perm.data()));
+    if (reorder == 2) FG_TRY(femgpu_spectral(neq, ia.data(), ja.data(), perm.data()));
     std::vector<int> inv(static_cast<size_t>(neq) + 1);
     for (int i = 0; i < neq; ++i) inv[perm[i]] = i;
     inv[neq] = neq;
     std::vector<int> newslot(static_cast<size_t>(nnz) + 1);
     newslot[nnz] = nnz;
-    IA[0] = 0;
+    IA.assign(static_cast<size_t>(neq) + 1, 0);
+    JA.resize(nnz);
     std::vector<std::pair<int, int>> row;   // (new column, old slot)
     int k = 0;
     for (int i = 0; i < neq; ++i) {
@@ -326,6 +330,22 @@ extern "C" int femgpu_csr_init(int nel, int s, int neq, int *lm, int reorder, in
     if (invP_id && type)
         for (int q = 0; q < ndofs_total; ++q)
             if (type[q] == 1) invP_id[q] = inv[invP_id[q]];
+    return nnz;
+}
+
+extern "C" int femgpu_csr_init(int nel, int s, int neq, int *lm, int reorder, int permute_lm,
+                               int ndofs_total, const int *type, int *invP_id, int *IA, int *JA,
+                               int *scheme, int *perm)
+{
+    FG_CHECK(nel >= 0 && s > 0 && neq >= 0 && lm && IA && JA && scheme && perm, FEMGPU_ERR_ARG,
+             "femgpu_csr_init: bad arguments");
+    std::vector<int> ia, ja, sch, pm;
+    const int nnz = fg::csr_init_vec(nel, s, neq, lm, reorder, permute_lm, ndofs_total, type, invP_id, ia, ja, sch, pm);
+    if (nnz < 0) return nnz;
+    std::copy(ia.begin(), ia.end(), IA);
+    std::copy(ja.begin(), ja.end(), JA);
+    std::copy(sch.begin(), sch.end(), scheme);
+    std::copy(pm.begin(), pm.end(), perm);
     return nnz;
 }
 

@@ -535,6 +535,7 @@ static int packed_pair(femgpu_ctx *ctx, femgpu_csr *A, const T *b, T *x, const i
 int launch_sweep64(femgpu_ctx *ctx, femgpu_csr *A, const double *b, double *x, const int *skip)
 {
     if (A->n == 0) return FEMGPU_OK;
+    if (A->prec64_jacobi > 0) return launch_jacobi_apply64(ctx, A, A->prec64_jacobi, b, x, skip);
     if (use_stream(A))
         return stream_pair<double>(ctx, A, b, x, skip, A->slo.val64.p, A->sup.val64.p, A->sup.dinv64.p,
                                    A->stream_t64.p, A->stream_t64.p + A->n);

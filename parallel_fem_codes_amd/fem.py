@@ -339,6 +339,15 @@ class CsrMatrix:
         N.check(N.lib().femgpu_ilu0_get_f32(self.h, LU, Dinv))
         return LU[:self.nnz], Dinv[:self.n]
 
+    def ilu0_factor_parallel(self, sweeps):
+        """FP64 ILU(0) factors by parallel (fixed-point) sweeps, for ilu0_set_apply(k > 0)."""
+        N.check(N.lib().femgpu_ilu0_factor_parallel(self.h, int(sweeps)))
+
+    def ilu0_set_apply(self, jacobi_iterations):
+        """FP64 preconditioner apply: 0 exact sweeps (ILUp_precond), k > 0 Jacobi iterations on the
+        parallel factors."""
+        N.check(N.lib().femgpu_ilu0_set_apply(self.h, int(jacobi_iterations)))
+
     def ILUp_precond_setup(self, F):
         """ILUp_precond_setup with p = 0: factor the current values, then F := M^-1 F in place."""
         self.ilu0_factor()

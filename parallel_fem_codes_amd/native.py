@@ -155,6 +155,8 @@ _SIGS = {
     "femgpu_ilu0_factor": (I, [vp]),
     "femgpu_ilu0_factor_host": (I, [vp]),
     "femgpu_ilu0_apply": (I, [vp, vp, vp]),
+    "femgpu_ilu0_factor_parallel": (I, [vp, I]),
+    "femgpu_ilu0_set_apply": (I, [vp, I]),
     "femgpu_ilu0_apply_f32": (I, [vp, vp, vp]),
     "femgpu_ilu0_get": (I, [vp, f64p, f64p]),
     "femgpu_ilu0_factor_parallel_f32": (I, [vp, I]),

@@ -47,20 +47,35 @@ def square_mesh(npts, rng):
     return x, y, typ, ubc, tri.ravel().astype(np.int32)
 
 
-def make_requests(count, seed, nmin, nmax, rate, kmin=1e-5, kmax=1e-2):
-    """count requests: arrival times of a Poisson process of the given rate (per second), N and k
-    log-uniform, the velocity angle uniform."""
+def request_params(count, seed, nmin, nmax, rate, kmin=1e-5, kmax=1e-2):
+    """count request descriptions: arrival time (a Poisson process of the given rate, per second),
+    N and k log-uniform, the velocity angle uniform, and the seed of the request's points."""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
-    reqs = []
+    ps = []
     for i in range(count):
         npts = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
         k = float(np.exp(rng.uniform(np.log(kmin), np.log(kmax))))
         th = float(rng.uniform(0.0, 2 * np.pi))
-        x, y, typ, ubc, ien = square_mesh(npts, rng)
-        pb = fem.make_problem(k=k, bx=(np.cos(th), 0.0, 0.0), by=(np.sin(th), 0.0, 0.0))
-        reqs.append(dict(arrival=float(arrivals[i]), x=x, y=y, type=typ, ubc=ubc, ien=ien, pb=pb, k=k, theta=th))
-    return reqs
+        ps.append(dict(index=i, arrival=float(arrivals[i]), npts=npts, k=k, theta=th,
+                       mesh_seed=int(rng.integers(1 << 62))))
+    return ps
+
+
+def build_request(p):
+    """The request's mesh (Delaunay of its seeded points) and problem."""
+    x, y, typ, ubc, ien = square_mesh(p["npts"], np.random.default_rng(p["mesh_seed"]))
+    th = p["theta"]
+    pb = fem.make_problem(k=p["k"], bx=(np.cos(th), 0.0, 0.0), by=(np.sin(th), 0.0, 0.0))
+    return dict(p, x=x, y=y, type=typ, ubc=ubc, ien=ien, pb=pb)
+
+
+def make_requests(count, seed, nmin, nmax, rate, kmin=1e-5, kmax=1e-2, workers=8):
+    """count requests (request_params, meshes built by a thread pool)."""
+    from concurrent.futures import ThreadPoolExecutor
+    ps = request_params(count, seed, nmin, nmax, rate, kmin, kmax)
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(build_request, ps))
 
 
 # ----------------------------------------------------------------------------- server
@@ -163,6 +178,61 @@ def run_server(reqs, opts=None, device=0, warmup=0):
     return sols, infos, wall, batch, wg
 
 
+def run_stream(params, opts=None, device=0, workers=6, lookahead=48, keep=(), warmup=8):
+    """The serving workload as a stream: request meshes are built by a pool of client threads ahead
+    of their arrival (at most `lookahead` in flight) and submitted at their arrival times; results
+    are collected as they finish. Only the requests whose index is in `keep` are retained with
+    their solutions (for comparisons); of the others only the server's timings remain. Returns
+    (per-request dicts in arrival order, kept {index: (request, solution)}, wall seconds, client
+    submit lag per request (seconds behind the arrival time), batch size, workgroup threads)."""
+    import queue
+    from collections import deque
+    from concurrent.futures import ThreadPoolExecutor
+    srv = Server(device, opts)
+    for p in params[:warmup]:
+        r = build_request(p)
+        srv.wait(srv.submit(r), len(r["x"]))
+    keep = set(keep)
+    count = len(params)
+    lag = np.zeros(count)
+    q = queue.Queue()
+    kept = {}
+    t0 = time.perf_counter()
+    a0 = params[0]["arrival"]
+
+    def feed():
+        with ThreadPoolExecutor(workers) as ex:
+            fut, nxt = deque(), 0
+            for i, p in enumerate(params):
+                while nxt < count and nxt < i + lookahead:
+                    fut.append(ex.submit(build_request, params[nxt]))
+                    nxt += 1
+                r = fut.popleft().result()
+                delay = p["arrival"] - a0 - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                lag[i] = max(0.0, -delay)
+                q.put((i, srv.submit(r), len(r["x"]), r if i in keep else None))
+
+    th = threading.Thread(target=feed)
+    th.start()
+    out = [None] * count
+    for _ in range(count):
+        i, ticket, nn, r = q.get()
+        u, info = srv.wait(ticket, nn, timeout=3600.0)
+        out[i] = dict(n=info.n, nnz=info.nnz, latency=info.t_end - info.t_submit, prep=info.prep_s,
+                      setup=info.setup_s, solve=info.solve_s, queue=info.t_start - info.t_submit - info.prep_s,
+                      status=info.status, residual=info.true_rel_residual, batch=info.batch_size,
+                      wg=info.wg_threads, iterations=info.iterations)
+        if r is not None:
+            kept[i] = (r, u)
+    th.join()
+    wall = time.perf_counter() - t0
+    batch, wg = srv.batch_size(), srv.wg_threads()
+    srv.close()
+    return out, kept, wall, lag, batch, wg
+
+
 def run_isolated(reqs, opts=None, device=0):
     """Each request alone on an idle server (submitted after the previous one finished): the
     isolated-solve latencies the loaded server's are compared with."""
@@ -222,6 +292,76 @@ def run_sequential_fp64(reqs):
     return sols, np.array(service), np.array(lat), end
 
 
+def stream_main(a):
+    """The stated serving workload: a long stream of requests, N log-uniform in [nmin, nmax],
+    summarised into one JSON object (latency percentiles, throughput, per-stage host and device
+    times, convergence), with the isolated-solve latencies and the sequential plain-FP64 GPU loop
+    measured on a sample of the same requests."""
+    import torch
+    torch.cuda.init()
+    params = request_params(a.requests, a.seed, a.nmin, a.nmax, a.rate)
+    rng = np.random.default_rng(a.seed + 1)
+    sample = sorted(rng.choice(len(params), size=min(a.sample, len(params)), replace=False).tolist())
+    extra = {"orthogonalization": a.orth} if a.orth else {}
+    opts = server_opts(executors=a.executors, prep_threads=a.prep_threads, max_batch=a.max_batch,
+                       small_solver=a.small_solver, wg_threads=a.wg_threads, **extra)
+    t0 = time.perf_counter()
+    out, kept, wall, lag, batch, wg = run_stream(params, opts, workers=a.client_threads, keep=sample,
+                                                 warmup=a.warmup)
+    print(f"# stream of {len(params)} done in {wall:.1f} s", file=sys.stderr, flush=True)
+    lat = np.array([o["latency"] for o in out])
+    res = np.array([o["residual"] for o in out])
+    ns = np.array([o["n"] for o in out])
+    pct = lambda v, q: round(1e3 * float(np.percentile(v, q)), 3)
+    summary = {
+        "workload": f"{len(params)} steady SUPG transport solves on Delaunay meshes of N log-uniform in "
+                    f"[{a.nmin}, {a.nmax}] random points plus boundary nodes, k log-uniform in [1e-5, 1e-2], "
+                    f"random flow angle; Poisson arrivals at {a.rate}/s (seed {a.seed}); mixed precision, "
+                    f"FP64-verified 1e-8",
+        "requests": len(params), "arrival_rate_per_s": a.rate,
+        "unknowns": {"min": int(ns.min()), "median": int(np.median(ns)), "max": int(ns.max()),
+                     "total": int(ns.sum())},
+        "server": {"solves_per_s": round(len(params) / wall, 3), "unknowns_per_s": round(float(ns.sum()) / wall, 1),
+                   "wall_s": round(wall, 2), "p50_ms": pct(lat, 50), "p90_ms": pct(lat, 90), "p99_ms": pct(lat, 99),
+                   "converged": int(sum(o["status"] == 0 for o in out)), "max_true_rel_residual": float(res.max()),
+                   "batch_size": batch, "wg_threads": wg,
+                   "requests_in_batches": int(sum(o["batch"] > 1 for o in out)),
+                   "persistent_launch_requests": int(sum(o["wg"] > 0 for o in out)),
+                   "mean_prep_ms": round(1e3 * float(np.mean([o["prep"] for o in out])), 3),
+                   "mean_setup_ms": round(1e3 * float(np.mean([o["setup"] for o in out])), 3),
+                   "mean_solve_ms": round(1e3 * float(np.mean([o["solve"] for o in out])), 3),
+                   "mean_queue_ms": round(1e3 * float(np.mean([o["queue"] for o in out])), 3),
+                   "client_submit_lag_ms": {"p50": pct(lag, 50), "p99": pct(lag, 99)}},
+        "executors": a.executors, "prep_threads": a.prep_threads, "client_threads": a.client_threads}
+    print(f"# server: {summary['server']}", file=sys.stderr, flush=True)
+    reqs = [kept[i][0] for i in sample]
+    if a.isolated:
+        iso = run_isolated(reqs, server_opts(small_solver=a.small_solver, wg_threads=a.wg_threads,
+                                             prep_threads=a.prep_threads, **extra))
+        lat_s = np.array([out[i]["latency"] for i in sample])
+        summary["sample"] = {"requests": len(sample), "loaded_p50_ms": pct(lat_s, 50), "loaded_p99_ms": pct(lat_s, 99),
+                             "isolated_p50_ms": pct(iso, 50), "isolated_p99_ms": pct(iso, 99),
+                             "p99_loaded_over_isolated": round(float(np.percentile(lat_s, 99) / np.percentile(iso, 99)), 3),
+                             "median_per_request_ratio": round(float(np.median(lat_s / iso)), 3)}
+        print(f"# sample: {summary['sample']}", file=sys.stderr, flush=True)
+    if not a.no_fp64:
+        s64, service, lat64, end = run_sequential_fp64(reqs)
+        diff = max(float(np.linalg.norm(kept[i][1] - v) / max(np.linalg.norm(v), 1e-300)) for i, v in zip(sample, s64))
+        ns_s = np.array([len(r["x"]) for r in reqs])
+        summary["fp64_sequential_sample"] = {
+            "requests": len(reqs), "mean_service_ms": round(1e3 * float(service.mean()), 3),
+            "solves_per_s_capacity": round(1.0 / float(service.mean()), 3),
+            "note": "the reference's FP64 path (ILU(0) + pgmres) one request at a time on the same GPU, on the "
+                    "sample; its capacity is 1 / mean service time (the stream's mean N matches the sample's)",
+            "sample_mean_nodes": int(ns_s.mean())}
+        summary["max_rel_diff_vs_fp64"] = diff
+    print(json.dumps(summary))
+    if a.out:
+        with open(a.out, "w") as fh:
+            json.dump(summary, fh, indent=1)
+            fh.write("\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--requests", type=int, default=200)
@@ -237,7 +377,15 @@ def main():
     ap.add_argument("--isolated", action="store_true", help="also time every request alone on an idle server")
     ap.add_argument("--warmup", type=int, default=8, help="requests solved before the timed stream")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--stream", action="store_true",
+                    help="streamed run (meshes built by client threads as requests arrive), JSON summary only")
+    ap.add_argument("--prep-threads", type=int, default=8)
+    ap.add_argument("--client-threads", type=int, default=6)
+    ap.add_argument("--sample", type=int, default=100, help="stream: requests kept for the isolated / FP64 legs")
+    ap.add_argument("--out", default=None, help="stream: also write the summary to this file")
     a = ap.parse_args()
+    if a.stream:
+        return stream_main(a)
     import torch
     torch.cuda.init()   # the runtime is initialised by torch before the server's threads use it
     t0 = time.perf_counter()

@@ -381,10 +381,11 @@ def test_assembly_steady_bit_exact(dev, torch_cuda, oracle, shock, m, right):
     assert np.array_equal(H(AA), s["AA"])
 
 
-@pytest.mark.parametrize("m,n,reorder", [(40, 40, "SYMRCM"), (13, 7, "NOT")])
+@pytest.mark.parametrize("m,n,reorder", [(40, 40, "SYMRCM"), (13, 7, "NOT"), (150, 110, "SYMRCM"), (97, 61, "NOT")])
 def test_assembly_transient(dev, torch_cuda, oracle, m, n, reorder):
-    """Build_M_K_R_SUPG: element matrices, residual and CSR values (without element matrices),
-    bit-identical to the oracle (csr_assembly order)."""
+    """Build_M_K_R_SUPG: element matrices, residual and CSR values (without element matrices: the
+    fused kernel, rows in patches with their elements evaluated in LDS), bit-identical to the oracle
+    (csr_assembly order), with and without shock capture and a source term."""
     torch = torch_cuda
     x, y, ien = oracle.mesh_structured(m, n)
     t = oracle.unit_square_types(x, y)
@@ -394,8 +395,8 @@ def test_assembly_transient(dev, torch_cuda, oracle, m, n, reorder):
     ubc = oracle.inflow_ubc(x, y)
     rng = np.random.default_rng(8)
     u, du = rng.standard_normal(neq) * 0.1, rng.standard_normal(neq) * 0.1
-    for shock in (0, 1):
-        pb = oracle.make_problem(k=1e-4, shock=shock)
+    for shock, f0 in ((0, 0.0), (1, 0.0), (1, 0.7)):
+        pb = oracle.make_problem(k=1e-4, shock=shock, f0=f0)
         Ae, R = oracle.build_transport_transient(x, y, t, c["invP"], ien, c["lm"], ubc, u, du, pb, 0.5, 1e-3, neq)
         AAo = oracle.csr_assemble(c["scheme"], Ae, c["nnz"])
         M = fem.Mesh(dev, x, y, t, c["invP"], ubc, ien, neq, lm=c["lm"], scheme=c["scheme"], nnz=c["nnz"])
@@ -474,6 +475,43 @@ def test_pgmres_matches_oracle_config0(dev, torch_cuda, oracle, m, kmax):
         assert tight < 1e-10
     else:
         assert tight <= 10 * rel(Xo, Xstar) * 1e-13 / 1e-8 + 1e-7
+
+
+def test_pgmres_ebe_ilu_config0(dev, torch_cuda, oracle):
+    """Config 0 with EBE storage: the 201x201-node operator as element matrices (numbered like the
+    SYMRCM-ordered CSR matrix) under GMRES(30) with the ILU(0) factors of the same operator, against
+    the oracle's EBE pgmres with the same preconditioner: Arnoldi steps within +-2, both solutions
+    within the preconditioned-residual bound, and the GPU solution as close to a tight oracle
+    solution as the oracle's own (the conditioning bar of the CSR case above)."""
+    torch = torch_cuda
+    x, y, ien = oracle.mesh_structured(200, 200)
+    t = oracle.unit_square_types(x, y)
+    invP0, neq = oracle.number_unknowns(t)
+    lm0 = oracle.fill_lm(ien, t, invP0, neq)
+    c = oracle.csr_init(lm0, 3, neq, reorder="SYMRCM", permute_lm=True, type_=t, invP=invP0)
+    pb = oracle.make_problem(k=1e-4)
+    Ke, F = oracle.build_transport_supg(x, y, t, c["invP"], ien, oracle.inflow_ubc(x, y), np.zeros(neq), pb, neq)
+    AA = oracle.csr_assemble(c["scheme"], Ke, c["nnz"])
+    s = dict(IA=c["IA"], JA=c["JA"], AA=AA, F=F, neq=neq)
+    A, LUo, Dio, B = _prepared(dev, torch, oracle, s)
+    E = fem.EbeMatrix(dev, c["lm"], 3, neq)
+    E.set_values(Ke)
+    ident = np.arange(neq + 1, dtype=np.int32)
+    S = oracle.make_system(1, neq, lm=c["lm"], Perm=ident, invPerm=ident, A=Ke, nel=len(ien) // 3,
+                           ilu=(c["IA"], c["JA"], LUo, Dio))
+    Xo, ito, rho, cyc = oracle.pgmres(S, B, kmax=30, lmax=1000, tol=1e-8)
+    X = torch.empty(neq, dtype=torch.float64, device="cuda")
+    info = fem.pgmres(dev, E, T(torch, B), X, 30, 1000, 1e-8, precond=A)
+    assert abs(info.iterations - ito) <= 2, (info.iterations, ito)
+    assert info.status == 0
+    xg = H(X)
+    rg, ro = _prec_residual(oracle, s, LUo, Dio, B, xg), _prec_residual(oracle, s, LUo, Dio, B, Xo)
+    assert rg < 2e-8 and ro < 2e-8, (rg, ro)
+    Xstar = oracle.pgmres(S, B, kmax=30, lmax=4000, tol=1e-13)[0]
+    eg, eo = rel(xg, Xstar), rel(Xo, Xstar)
+    print(f"config 0 EBE+ILU(0): iterations gpu {info.iterations} oracle {ito}; err vs tight gpu {eg:.2e} "
+          f"oracle {eo:.2e}")
+    assert eg <= 2 * eo + 1e-12, (eg, eo)
 
 
 def test_pgmres_ebe_operator(dev, torch_cuda, oracle):
@@ -607,6 +645,52 @@ def test_jacobi_sweeps(dev, torch_cuda, oracle, name):
     z = Dio * (y - Us @ (Dio * (y - Us @ (Dio * y))))
     assert rel(H(y2), y) < 1e-5
     assert rel(H(z2), z) < 1e-5
+
+
+@pytest.mark.parametrize("name", ["st_csr_rcm_teste_r8", "st_csr_rcm_cau_r24"])
+def test_fp64_parallel_preconditioner(dev, torch_cuda, oracle, name):
+    """The FP64 form of the mixed solver's preconditioner: parallel ILU(0) sweeps reach the exact
+    ILU(0) at their fixed point and follow the entry-by-entry restatement before it; Jacobi
+    iterations reach the exact sweeps after depth iterations; pgmres preconditioned with it solves
+    the system (true residual of its stopping test)."""
+    import scipy.sparse as sp
+    torch = torch_cuda
+    s = ilu_case(oracle, name)
+    depth = fem.ilu_levels(s["IA"], s["JA"], True)[1]
+    n, IA, JA = s["neq"], s["IA"], s["JA"]
+    A = fem.CsrMatrix(dev, IA, JA)
+    A.set_values(s["AA"][:-1])
+    LUo, Dio, rc = oracle.ilu0_factor(IA, JA, s["AA"])
+    b = np.random.default_rng(11).standard_normal(n)
+    exact = oracle.ilu0_apply(IA, JA, LUo, Dio, b)
+    A.ilu0_factor_parallel(2 * depth + 1)
+    A.ilu0_set_apply(depth)
+    z = T(torch, b)
+    A.ilu0_apply(z, z)
+    assert rel(H(z), exact) < 1e-12
+    # two sweeps and two iterations against the restatement
+    A.ilu0_factor_parallel(2)
+    A.ilu0_set_apply(2)
+    LUr, Dir = parilu_reference(IA, JA, s["AA"], 2)
+    rows = np.repeat(np.arange(n), np.diff(IA))
+    low, up = JA < rows, JA > rows
+    Ls = sp.csr_matrix((LUr[low], (rows[low], JA[low])), shape=(n, n))
+    Us = sp.csr_matrix((LUr[up], (rows[up], JA[up])), shape=(n, n))
+    y = b - Ls @ (b - Ls @ b)
+    zr = Dir * (y - Us @ (Dir * (y - Us @ (Dir * y))))
+    z = T(torch, b)
+    A.ilu0_apply(z, z)
+    assert rel(H(z), zr) < 1e-12
+    B = T(torch, s["F"])
+    A.ilu0_apply(B, B)
+    X = torch.empty(n, dtype=torch.float64, device="cuda")
+    info = fem.pgmres(dev, A, B, X, 30, 200, 1e-10, precond=A)
+    assert info.status == 0
+    res = np.linalg.norm(s["F"] - oracle.csrmv(IA, JA, s["AA"], H(X))) / np.linalg.norm(s["F"])
+    assert res < 1e-7, res
+    A.ilu0_set_apply(0)
+    with pytest.raises(native.FemGpuError):
+        A.ilu0_apply(z, z)                # the exact factors were never computed
 
 
 def transient_system(oracle, m, dt, alpha=0.5):
@@ -1001,6 +1085,46 @@ def test_ns_process_picard_matches_oracle(dev, torch_cuda, oracle):
     assert info.status == 0 and it_ref < maxit
     assert abs(info.iterations - it_ref) <= 1
     assert rel(H(ug), u) < 1e-8
+
+
+@pytest.mark.parametrize("sweep_mode,factor_sweeps", [(0, 0), (3, 6)])
+def test_ns_process_mixed_matches_oracle(dev, torch_cuda, oracle, sweep_mode, factor_sweeps):
+    """Config 3's solver form on a lid-driven cavity (Re = 1000, 25x25 nodes): the device Picard loop
+    (NS Process.c:27-52) with every linear system solved by the mixed GMRES(50) (FP32 inner solve,
+    FP64 refinement to a 1e-10 true residual), against the oracle's FP64 Picard loop (ILU(0) +
+    pgmres(50)): both stop on the update bar 1e-8, and the final velocity and pressure fields agree
+    within 1e-6 relative L2 (the nonlinear bar, not the linear one, sets their distance)."""
+    torch = torch_cuda
+    s = ns_case(oracle, m=24, Re=1000.0)
+    c = s["c"]
+    nltol, maxit, kmax = 1e-8, 200, 50
+    u = np.zeros(s["neq"])
+    it_ref, delta2 = 0, nltol + 1
+    while delta2 > nltol and it_ref < maxit:
+        it_ref += 1
+        U3 = oracle.ns_nodal_state(s["type3"], s["invP"], u, s["presc"])
+        Ke, F = oracle.build_ns_supg_pspg(s["x"], s["y"], s["ien"], s["lm"], U3, s["fx"], s["fy"], s["Re"], s["neq"])
+        AA = oracle.csr_assemble(c["scheme"], Ke, c["nnz"], s=9)
+        LU, Dinv, rc = oracle.ilu0_factor(c["IA"], c["JA"], AA)
+        B = oracle.ilu0_apply(c["IA"], c["JA"], LU, Dinv, F[:s["neq"]])
+        S = oracle.make_system(0, s["neq"], IA=c["IA"], JA=c["JA"], AA=AA, ilu=(c["IA"], c["JA"], LU, Dinv))
+        sv = oracle.pgmres(S, B, kmax=kmax, lmax=1000, tol=1e-10)[0][:s["neq"]]
+        u += sv
+        delta2 = np.sqrt(oracle.ddot(sv, sv)) / np.sqrt(oracle.ddot(u, u))
+    assert delta2 <= nltol
+    M = fem.NsMesh(dev, s["x"], s["y"], s["type3"], s["invP"], s["presc"], s["ien"], s["lm"], s["neq"],
+                   scheme=c["scheme"], nnz=c["nnz"])
+    A = fem.CsrMatrix(dev, c["IA"], c["JA"])
+    ug = torch.zeros(s["neq"], dtype=torch.float64, device="cuda")
+    opts = fem.driver_opts("mixed", kmax=kmax, tol=1e-10, nonlinear_tol=nltol, nonlinear_maxiter=maxit,
+                           inner_kmax=kmax, inner_tol=1e-4, sweep_mode=sweep_mode, factor_sweeps=factor_sweeps)
+    info = fem.Process_NS(M, A, s["Re"], ug, opts, fx=T(torch, s["fx"]), fy=T(torch, s["fy"]))
+    err = rel(H(ug), u)
+    print(f"NS mixed (sweep_mode {sweep_mode}, factor_sweeps {factor_sweeps}): Picard {info.iterations} "
+          f"(oracle {it_ref}), linear status {info.status}, rel err of (u, v, p) vs oracle {err:.2e}")
+    assert info.status == 0, info.status
+    assert abs(info.iterations - it_ref) <= 3
+    assert err < 1e-6
 
 
 # ----------------------------------------------------------------------------- block diagonal
