@@ -232,6 +232,48 @@ def euler_run(fem, torch, dev, timed, timed_cold, mx=800, my=400, steps=3, multi
     return out
 
 
+def ns_run(fem, torch, dev, m=512, Re=1000.0, iterations=3, kmax=50, tol=1e-8):
+    """The steady Navier-Stokes workload: the reference's CAVITY problem at Re = 1000 on (m+1)^2 nodes
+    (2 m^2 triangles, u, v, p per node, 9x9 SUPG/PSPG element blocks, CSR), `iterations` Picard
+    iterations of NS Process.c from rest with each linear system solved by GMRES(kmax) + ILU(0) to
+    `tol`: the FP64 path (exact ILU(0), the reference's pgmres) and the mixed path (FP32 inner GMRES
+    with the FP32 copy of the exact ILU(0) inside FP64 refinement), each from the same state; the
+    per-iteration time and the distance between the two states after the same iterations."""
+    P = fem.cavity_problem(m)
+    c, neq = P["c"], P["neq"]
+    t0 = time.perf_counter()
+    M = fem.NsMesh(dev, P["x"], P["y"], P["type3"], P["invP"], P["presc"], P["ien"], P["lm"], neq, scheme=c.scheme,
+                   nnz=c.nnz)
+    A = fem.CsrMatrix(dev, c.IA, c.JA)
+    setup_s = time.perf_counter() - t0
+    fx, fy = torch.zeros(len(P["x"]), dtype=torch.float64, device="cuda"), torch.zeros(len(P["x"]), dtype=torch.float64,
+                                                                                        device="cuda")
+    out = {"workload": f"CAVITY Navier-Stokes, Re {Re}, {m + 1}x{m + 1} nodes ({2 * m * m} triangles), neq={neq}, "
+                       f"nnz={c.nnz}, {iterations} Picard iterations from rest, GMRES({kmax}) + ILU(0) tol {tol}",
+           "host_setup_s": round(setup_s, 2)}
+    states = {}
+    for solver, kw in (("fp64", {}), ("mixed", dict(sweep_mode=0, factor_sweeps=0))):
+        opts = fem.driver_opts(solver, kmax=kmax, tol=tol, nonlinear_tol=1e-30, nonlinear_maxiter=iterations,
+                               inner_kmax=kmax, inner_tol=1e-4, **kw)
+        u = torch.zeros(neq, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        info = fem.Process_NS(M, A, Re, u, opts, fx=fx, fy=fy)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t1
+        states[solver] = u
+        out[solver] = {"ms_per_picard_iteration": round(1e3 * el / max(info.iterations, 1), 3),
+                       "picard_iterations": info.iterations,
+                       "linear_iterations": [int(v) for v in list(info.iters)[:info.iterations]],
+                       "linear_status": info.status}
+        print(f"# ns {solver}: {out[solver]}", file=sys.stderr, flush=True)
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms_per_picard_iteration"] / out["mixed"]["ms_per_picard_iteration"], 3)
+    out["mixed_rel_err_vs_fp64"] = float(torch.linalg.norm(states["mixed"] - states["fp64"]) /
+                                         torch.linalg.norm(states["fp64"]))
+    del M, A
+    return out
+
+
 def serving_run(count, rate=100.0, nmin=2000, nmax=50000, seed=20261202):
     """The serving workload (steady solves on Delaunay meshes of 2k-50k points, Poisson arrivals)
     through the library's request loop, and the same requests through a sequential plain-FP64 GPU
@@ -248,7 +290,8 @@ def serving_run(count, rate=100.0, nmin=2000, nmax=50000, seed=20261202):
                       "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
                       "max_true_rel_residual": float(max(i.true_rel_residual for i in infos)),
                       "converged": int(sum(i.status == 0 for i in infos)), "batch_size": batch,
-                      "wg_threads": wg, "persistent_launch_requests": int(sum(i.wg_threads > 0 for i in infos)),
+                      "wg_threads": wg, "requests_in_batches": int(sum(i.batch_size > 1 for i in infos)),
+                      "persistent_launch_requests": int(sum(i.wg_threads > 0 for i in infos)),
                       "mean_prep_ms": round(1e3 * float(np.mean([i.prep_s for i in infos])), 3),
                       "mean_setup_ms": round(1e3 * float(np.mean([i.setup_s for i in infos])), 3),
                       "mean_solve_ms": round(1e3 * float(np.mean([i.solve_s for i in infos])), 3)},
@@ -261,6 +304,28 @@ def serving_run(count, rate=100.0, nmin=2000, nmax=50000, seed=20261202):
                                "mean_service_ms": round(1e3 * float(service.mean()), 3)},
            "max_rel_diff_vs_fp64": diff}
     print(f"# serving: {out['server']}", file=sys.stderr, flush=True)
+    out["small_requests"] = serving_small_run(3 * count, seed=seed + 1)
+    return out
+
+
+def serving_small_run(count, rate=1000.0, nmin=200, nmax=3000, seed=20261203):
+    """Small requests (Delaunay meshes of 200-3 000 points) at a high arrival rate, where the server
+    packs consecutive requests into one block-diagonal system and takes the persistent launch (a
+    workgroup per system) for batches whose systems all have at most 3 072 rows."""
+    from parallel_fem_codes_amd import serving
+    reqs = serving.make_requests(count, seed, nmin, nmax, rate)
+    sols, infos, wall, batch, wg = serving.run_server(reqs, serving.server_opts(small_solver="auto"), warmup=8)
+    lat = np.array([i.t_end - i.t_submit for i in infos])
+    iso = serving.run_isolated(reqs, serving.server_opts(small_solver="auto"))
+    out = {"requests": count, "arrival_rate_per_s": rate, "points": [nmin, nmax], "small_solver": "auto",
+           "solves_per_s": round(count / wall, 3), "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+           "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+           "isolated_p99_ms": round(1e3 * float(np.percentile(iso, 99)), 3),
+           "max_true_rel_residual": float(max(i.true_rel_residual for i in infos)),
+           "converged": int(sum(i.status == 0 for i in infos)), "batch_size": batch, "wg_threads": wg,
+           "requests_in_batches": int(sum(i.batch_size > 1 for i in infos)),
+           "persistent_launch_requests": int(sum(i.wg_threads > 0 for i in infos))}
+    print(f"# serving (small requests): {out}", file=sys.stderr, flush=True)
     return out
 
 
@@ -281,6 +346,8 @@ def main():
     ap.add_argument("--nonlinear-tol", type=float, default=1e-4, help="multicorrection stop (StopByNorm)")
     ap.add_argument("--euler-steps", type=int, default=3,
                     help="pseudo-time steps of the 801x401 Euler workload run after the timed region (0: skip)")
+    ap.add_argument("--ns-iterations", type=int, default=3,
+                    help="Picard iterations of the 513x513 cavity run after the timed region (0: skip)")
     ap.add_argument("--serving-requests", type=int, default=100,
                     help="requests of the serving workload run after the timed region (0: skip)")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -511,6 +578,10 @@ def main():
         euler = euler_run(fem, torch, dev, timed, timed_cold, steps=a.euler_steps)
     del flush, flush_sum
 
+    ns = None
+    if a.ns_iterations > 0:
+        ns = ns_run(fem, torch, dev, iterations=a.ns_iterations)
+
     time_int = None
     if a.time_steps > 0:
         time_int = time_integration(fem, torch, M, A, pb, P, dt, alpha, kmax, tol, a)
@@ -591,6 +662,7 @@ def main():
         "roofline": roofline,
         "time_integration": time_int,
         "euler": euler,
+        "navier_stokes": ns,
         "serving": serving_out,
         "cpu_baseline": cpu,
     }

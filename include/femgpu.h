@@ -181,8 +181,8 @@ int femgpu_ilu0_factor_host(femgpu_csr *A);
 int femgpu_ilu0_apply(femgpu_csr *A, const double *d_p, double *d_z);
 /* FP64 ILU(0) factors by `sweeps` fixed-point sweeps of the parallel ILU(0) iteration (Chow and
  * Patel, SIAM J. Sci. Comput. 37(2), 2015) from the current values; no exact factorisation. Needs
- * rows and strict triangles of at most 16 entries. No reference counterpart: the FP64 form of the
- * mixed solver's preconditioner (a precision-only comparison with it). */
+ * rows and strict triangles of at most 16 entries; 1 <= sweeps <= 1024. No reference counterpart:
+ * the FP64 form of the mixed solver's preconditioner (a precision-only comparison with it). */
 int femgpu_ilu0_factor_parallel(femgpu_csr *A, int sweeps);
 /* How the FP64 ILU(0) preconditioner of A is applied (femgpu_ilu0_apply, and pgmres / pcg with A as
  * preconditioner): 0 = the exact level-scheduled sweeps on the factors of femgpu_ilu0_factor (the

@@ -40,7 +40,8 @@ EU_SRC  := $(EU)/Build_M_K_F_SUPG.c $(EU)/eval_U_dU.c $(EU)/Ax_Ay_calculations.c
            $(CF)/Solvers_and_Preconditioners/pgmres.c $(CF)/MatrixVector_Operations/EBE/ebemvNDOF4.c \
            $(CF)/BLAS_Operations/daxpy.c $(CF)/BLAS_Operations/dcopy.c $(CF)/BLAS_Operations/dscal.c
 
-all: $(OUT)/st_harness $(OUT)/st_harness_ofast $(OUT)/ns_harness $(OUT)/eu_harness $(OUT)/meshGenerator
+all: $(OUT)/st_harness $(OUT)/st_harness_ofast $(OUT)/st_harness_femgpu $(OUT)/ns_harness $(OUT)/eu_harness \
+     $(OUT)/meshGenerator
 
 # The structured mesh generator is a self-contained C program.
 $(OUT)/meshGenerator: $(REF)/Mesh_Generator/Structured_Mesh/meshGenerator.c
@@ -62,6 +63,23 @@ $(OUT)/st_harness_ofast: $(HERE)ref_harness_st.c $(ST_SRC) $(CF_SRC)
 	mkdir -p $(OUT)
 	gcc -Ofast -march=native -w -DSSTranspEquation2D -I$(ST) -I$(MPI)/include -o $@ $(HERE)ref_harness_st.c \
 	    $(ST_SRC) $(CF_SRC) -L$(MPI)/lib -Wl,-rpath,$(MPI)/lib -lmpi -lm \
+	    -Wl,--unresolved-symbols=ignore-in-object-files
+
+# The same harness with the reference-side binding of the library: the reference's assembly and
+# structure code around parallel_fem_codes_amd/compat/femgpu_compat.c and libfemgpu.so, which
+# replace ebemv, csrmv, ILUp_precond(_setup), NO_precond_setup and pgmres (tests/test_gpu_shim.py
+# runs it on the GPU).
+LIBDIR  := $(abspath $(HERE)../parallel_fem_codes_amd)
+ROCM    ?= /opt/rocm
+SHIM_CF := $(filter-out %/ebemv.c %/csrmv.c %/ILUp_precond.c %/ILUp_precond_setup.c %/pgmres.c \
+             %/NO_precond_setup.c,$(CF_SRC))
+$(OUT)/st_harness_femgpu: $(HERE)ref_harness_st.c $(ST_SRC) $(SHIM_CF) $(HERE)../parallel_fem_codes_amd/compat/femgpu_compat.c \
+                         $(LIBDIR)/libfemgpu.so
+	mkdir -p $(OUT)
+	gcc $(CFLAGS) -DFEMGPU_SHIM -fgnu89-inline -I$(CF) -I$(HERE)../include -I$(ROCM)/include -D__HIP_PLATFORM_AMD__ \
+	    -o $@ $(HERE)ref_harness_st.c $(ST_SRC) $(SHIM_CF) $(HERE)../parallel_fem_codes_amd/compat/femgpu_compat.c \
+	    -Wl,-rpath-link,/usr/lib/x86_64-linux-gnu -L$(LIBDIR) -Wl,-rpath,$(LIBDIR) -lfemgpu -L$(ROCM)/lib \
+	    -Wl,-rpath,$(ROCM)/lib -lamdhip64 -L$(MPI)/lib -Wl,-rpath,$(MPI)/lib -lmpi -lm \
 	    -Wl,--unresolved-symbols=ignore-in-object-files
 
 $(OUT)/ns_harness: $(HERE)ref_harness_ns.c $(NS_SRC) $(NS_CF)

@@ -1,8 +1,8 @@
 /*
  * ref_harness_st.c -- one-rank driver around the UNMODIFIED reference hot-path sources of
  * 01_SS_Transport_Equation (compiled in place from the reference tree by oracle/ref.mk).
- * Test infrastructure: it writes golden vectors for tests/golden; it is never shipped or run
- * on a GPU machine.
+ * Test infrastructure: it writes golden vectors for tests/golden and is never shipped. Its shim
+ * build (below) is the one form that runs on a GPU machine: the test of the reference-side binding.
  *
  * It plays the role of Preprocess.c:127-357 and Process.c:19-86 for NPROC = 1 (the real
  * drivers refuse fewer than two ranks, Preprocess.c:50-53) and supplies:
@@ -17,6 +17,9 @@
  *
  * Input (binary, little endian): int32 header[16], float64 params[16], then x[nn], y[nn],
  * type[nn] (int32), ien[3*nel] (int32), ubc[nn], probe[nn] (float64).
+ * Built with -DFEMGPU_SHIM and linked with parallel_fem_codes_amd/compat/femgpu_compat.c and
+ * libfemgpu.so in place of the reference's ebemv, csrmv, ILUp_precond(_setup), NO_precond_setup
+ * and pgmres objects, it runs the reference's own assembly around the library's device solver.
  * Output: a stream of records {char name[32]; int32 kind (0 int32, 1 float64); int64 count;
  * payload}. The record "timers" holds the wall seconds spent in Build_K_F_SUPG, precond_setup and
  * pgmres, summed over the nonlinear iterations (clock_gettime, CLOCK_MONOTONIC).
@@ -192,6 +195,7 @@ int main(int argc, char **argv)
         timers[1] += now() - t0;
         if (it == 1) {
             rec("MinvF", 1, neq, F);
+#ifndef FEMGPU_SHIM   /* with the library's shim the factors live on the device, not in MD->ILUp */
             if (!ebe) {
                 SparILU *lu = MD->ILUp;
                 long nl = 0, nu = 0;
@@ -205,6 +209,7 @@ int main(int argc, char **argv)
                 rec("ILU_L", 1, nl, Lv); rec("ILU_U", 1, nu, Uv); rec("ILU_D", 1, neq, lu->D);
                 free(Lv); free(Uv);
             }
+#endif
         }
         const int before = P->iterations;
         t0 = now();

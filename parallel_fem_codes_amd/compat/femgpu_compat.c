@@ -25,9 +25,24 @@
  * element matrices and F are uploaded, processed on the device and written back, so the driver's
  * host arrays stay the reference's state.
  *
+ * Matrix values cross to the device when they change: every driver calls precond_setup after each
+ * assembly (ST Process.c:60, Predictor_Old.c:95, NS Process.c:37), so the setup entry points --
+ * ILUp_precond_setup, the block-diagonal setups and NO_precond_setup, which this shim also
+ * provides -- mark the values stale, and the next product, preconditioner or solve uploads them
+ * once. A product called before any setup uploads them on first use.
+ *
+ * Solver selection: the drivers pick pgmres by name (setSolver.c:6-15 accepts "GMRES" only), so the
+ * mixed-precision solver is selected from the environment: FEMGPU_SOLVER=mixed makes pgmres run
+ * femgpu_gmres_mixed (FP32 inner GMRES with kmax = KrylovBasisVectorsQuantity inside FP64
+ * refinement, stopped at a true relative residual ||F - A x|| <= SolverTolerance ||F||) and makes
+ * the setups leave F unpreconditioned (the mixed solver takes the raw right-hand side; the drivers
+ * use F only as the solver's input). CSR systems use the parallel ILU(0) with Jacobi sweeps
+ * (FEMGPU_MIXED_PREC=exact: the FP32 copy of the exact ILU(0)); EBE systems the FP32 copy of their
+ * block-diagonal preconditioner.
+ *
  * One rank only (the GPU holds the whole mesh): NPROC != 1 stops with a message. Failures print the
  * library's message and exit(1), as the reference does on fatal errors (mycalloc.c:10-13).
- * ILU(p) with p > 0 and preconditioners other than NO/ILU0 are not provided by the library.
+ * ILU(p) with p > 0 and preconditioners other than NO/ILU0/BlockDiag are not provided.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -58,6 +73,9 @@ static struct {
     double *dA;                        /* element matrices on the device (block diagonal, scaling) */
     const MatrixDataType *csr_owner;   /* the MatrixData whose pattern csr holds */
     int factored;                      /* csr holds ILU(0) factors of the current AA */
+    int stale;                         /* the driver's matrix values changed since the last upload */
+    int mixed;                         /* FEMGPU_SOLVER=mixed */
+    int mixed_exact;                   /* FEMGPU_MIXED_PREC=exact */
     double *dP, *dQ;                   /* NEQ + 1 doubles each */
     int NEQ;
 } G;
@@ -85,6 +103,10 @@ static void compat_init(ParametersType *Parameters)
     }
     CALL(femgpu_create(0, NULL, &G.ctx));
     G.NEQ = Parameters->NEQ;
+    G.stale = 1;
+    const char *solver = getenv("FEMGPU_SOLVER"), *prec = getenv("FEMGPU_MIXED_PREC");
+    G.mixed = solver && strcmp(solver, "mixed") == 0;
+    G.mixed_exact = prec && strcmp(prec, "exact") == 0;
     if (hipMalloc((void **)&G.dP, sizeof(double) * (G.NEQ + 1)) != hipSuccess ||
         hipMalloc((void **)&G.dQ, sizeof(double) * (G.NEQ + 1)) != hipSuccess) {
         printf("libfemgpu: device allocation failed\n");
@@ -104,8 +126,8 @@ static void down(double *h, const double *d, int n)
         fail("download");
 }
 
-/* The CSR operator of MatrixData (pattern uploaded on first use), with its current values. */
-static femgpu_csr *csr_of(ParametersType *Parameters, MatrixDataType *MatrixData, int upload_values)
+/* The CSR operator of MatrixData (pattern uploaded on first use), its values uploaded when stale. */
+static femgpu_csr *csr_of(ParametersType *Parameters, MatrixDataType *MatrixData)
 {
     if (!G.csr || G.csr_owner != MatrixData) {
         if (G.csr)
@@ -113,21 +135,42 @@ static femgpu_csr *csr_of(ParametersType *Parameters, MatrixDataType *MatrixData
         CALL(femgpu_csr_create(G.ctx, Parameters->neq, Parameters->nnzero, MatrixData->IA, MatrixData->JA, &G.csr));
         G.csr_owner = MatrixData;
         G.factored = 0;
+        G.stale = 1;
     }
-    if (upload_values) {
+    if (G.stale) {
         CALL(femgpu_csr_set_values(G.csr, MatrixData->AA, 0));
         G.factored = 0;
+        G.stale = 0;
     }
     return G.csr;
 }
 
-/* The EBE operator (element lists on first use) with the current element matrices. */
+/* The EBE operator (element lists on first use), its element matrices uploaded when stale. */
 static femgpu_ebe *ebe_of(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs)
 {
-    if (!G.ebe)
+    if (!G.ebe) {
         CALL(femgpu_ebe_create(G.ctx, Parameters->nel, COMPAT_S, Parameters->NEQ, FemStructs->lmaux, &G.ebe));
-    CALL(femgpu_ebe_set_values(G.ebe, MatrixData->Aaux, 0));
+        G.stale = 1;
+    }
+    if (G.stale) {
+        CALL(femgpu_ebe_set_values(G.ebe, MatrixData->Aaux, 0));
+        G.stale = 0;
+    }
     return G.ebe;
+}
+
+/* NO_precond_setup (NO_precond_setup.c:3): nothing to set up; the matrix values the driver has just
+ * assembled are marked for upload. */
+int NO_precond_setup(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs, int tag,
+                     double *F)
+{
+    (void)MatrixData;
+    (void)FemStructs;
+    (void)tag;
+    (void)F;
+    compat_init(Parameters);
+    G.stale = 1;
+    return 0;
 }
 
 #if COMPAT_S > 3
@@ -171,12 +214,16 @@ static double *elements_up(ParametersType *Parameters, MatrixDataType *MatrixDat
     return G.dA;
 }
 
-/* BlockDiag(DOF3)_precond_EBE_setup: blocks of the current element matrices, F := D^-1 F. */
+/* BlockDiag(DOF3)_precond_EBE_setup: blocks of the current element matrices, F := D^-1 F (left raw
+ * for the mixed solver). */
 static int bdiag_setup(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType *FemStructs, double *F)
 {
     compat_init(Parameters);
+    G.stale = 1;
     femgpu_blockdiag *B = bdiag_of(Parameters, FemStructs);
     CALL(femgpu_blockdiag_setup(B, elements_up(Parameters, MatrixData)));
+    if (G.mixed)
+        return 0;
     F[Parameters->NEQ] = 0.0;
     up(G.dP, F, Parameters->neq);
     CALL(femgpu_blockdiag_apply(B, G.dP, G.dP));
@@ -265,7 +312,7 @@ int csrmv(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsType
 {
     (void)FemStructs;
     compat_init(Parameters);
-    femgpu_csr *A = csr_of(Parameters, MatrixData, 1);
+    femgpu_csr *A = csr_of(Parameters, MatrixData);
     up(G.dP, P, Parameters->neq);
     CALL(femgpu_csrmv(A, G.dP, G.dQ));
     down(Q, G.dQ, Parameters->neq);
@@ -298,7 +345,14 @@ int ILUp_precond_setup(ParametersType *Parameters, MatrixDataType *MatrixData, F
         printf("libfemgpu: only ILU(0) is provided (Preconditioner = %s)\n", Parameters->Preconditioner);
         exit(1);
     }
-    femgpu_csr *A = csr_of(Parameters, MatrixData, 1);
+    G.stale = 1;
+    femgpu_csr *A = csr_of(Parameters, MatrixData);
+    if (G.mixed) {   /* the mixed solver factors in FP32 itself (or uses the exact factors) and takes F raw */
+        if (G.mixed_exact)
+            CALL(femgpu_ilu0_factor(A));
+        G.factored = 1;
+        return 0;
+    }
     CALL(femgpu_ilu0_factor(A));
     G.factored = 1;
     up(G.dP, F, Parameters->neq);
@@ -312,9 +366,9 @@ int ILUp_precond(ParametersType *Parameters, MatrixDataType *MatrixData, FemStru
 {
     (void)FemStructs;
     compat_init(Parameters);
-    femgpu_csr *A = csr_of(Parameters, MatrixData, 0);
-    if (!G.factored)
-        fail("ILUp_precond before ILUp_precond_setup");
+    femgpu_csr *A = csr_of(Parameters, MatrixData);
+    if (!G.factored || G.mixed)
+        fail("ILUp_precond before ILUp_precond_setup (or with FEMGPU_SOLVER=mixed, which keeps no FP64 factors)");
     up(G.dP, p, Parameters->neq);
     CALL(femgpu_ilu0_apply(A, G.dP, G.dQ));
     down(z, G.dQ, Parameters->neq);
@@ -328,7 +382,7 @@ int pgmres(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsTyp
     femgpu_solve_info info;
     compat_init(Parameters);
     if (FemFunctions->precond == ILUp_precond) {
-        S.ilu = csr_of(Parameters, MatrixData, 0);
+        S.ilu = csr_of(Parameters, MatrixData);
         if (!G.factored)
             fail("pgmres: ILU(0) preconditioner used before ILUp_precond_setup");
     }
@@ -344,7 +398,7 @@ int pgmres(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsTyp
         exit(1);
     }
     if (FemFunctions->mv == csrmv)
-        S.csr = csr_of(Parameters, MatrixData, S.ilu == NULL);   /* factored values are current */
+        S.csr = csr_of(Parameters, MatrixData);
 #if COMPAT_S == 3
     else if (FemFunctions->mv == ebemv)
 #elif COMPAT_S == 9
@@ -359,8 +413,26 @@ int pgmres(ParametersType *Parameters, MatrixDataType *MatrixData, FemStructsTyp
         exit(1);
     }
     up(G.dP, B, Parameters->neq);
-    CALL(femgpu_pgmres(G.ctx, &S, Parameters->KrylovBasisVectorsQuantity, Parameters->LinearMaxIter,
-                       Parameters->SolverTolerance, G.dP, G.dQ, &info));
+    if (G.mixed) {
+        femgpu_mixed_opts mo;
+        memset(&mo, 0, sizeof(mo));
+        mo.inner_kmax = Parameters->KrylovBasisVectorsQuantity;
+        mo.inner_cycles = 1;
+        mo.inner_tol = 1e-4;
+        mo.max_refinements = 200;
+        if (S.csr && !S.ilu) {
+            printf("libfemgpu: FEMGPU_SOLVER=mixed needs a preconditioner (ILU0 for CSR, BlockDiag for EBE)\n");
+            exit(1);
+        }
+        if (S.ilu) {
+            mo.sweep_mode = G.mixed_exact ? 0 : 2;
+            mo.factor_sweeps = G.mixed_exact ? 0 : 2;
+        }
+        CALL(femgpu_gmres_mixed(G.ctx, &S, &mo, Parameters->SolverTolerance, G.dP, G.dQ, &info));
+    } else {
+        CALL(femgpu_pgmres(G.ctx, &S, Parameters->KrylovBasisVectorsQuantity, Parameters->LinearMaxIter,
+                           Parameters->SolverTolerance, G.dP, G.dQ, &info));
+    }
     down(X, G.dQ, Parameters->neq);
     Parameters->iterations += info.iterations;
     return 0;

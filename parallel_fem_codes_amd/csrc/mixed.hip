@@ -636,7 +636,7 @@ int launch_parilu64(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
     }
     FG_CHECK(A->par_ok && A->ell_kl > 0, FEMGPU_ERR_ARG,
              "FP64 parallel ILU(0): needs the sliced-ELL plans (rows and strict triangles of at most 16 entries)");
-    FG_CHECK(sweeps >= 1 && sweeps <= 64, FEMGPU_ERR_ARG, "FP64 parallel ILU(0): %d sweeps out of range", sweeps);
+    FG_CHECK(sweeps >= 1 && sweeps <= 1024, FEMGPU_ERR_ARG, "FP64 parallel ILU(0): %d sweeps out of range", sweeps);
     const size_t tot = A->sell_col.n, n1 = static_cast<size_t>(n) + 1;
     FG_TRY(A->par64_v.alloc(tot));
     FG_TRY(A->par64_a.alloc(tot));

@@ -693,3 +693,22 @@ def obliquo_problem(mx, my, length=2.0, right=True, cfl=5.0, gamma=1.4):
     dt = cfl * h / (c + vmag)
     return dict(x=x, y=y, ien=ien, type4=t, invP=invP, neq=neq, lm=lm, presc=presc, u0=u0, dt=dt,
                 gamma=gamma, mach=vmag / c)
+
+
+def cavity_problem(m, reorder="SYMRCM"):
+    """The reference's CAVITY problem (04_SS_NavierStokes_Equations/2D/CAVITY): lid-driven cavity on a
+    structured mesh of m x m squares over the unit square, (m+1)^2 nodes. Marks of the generator
+    (walls: u and v prescribed; the centre node: p prescribed), prescribed values of
+    CAVITY_{v1,v2,p}presc.c (u = 1 on the lid y = 1, else 0; p = 0), no body force
+    (CAVITY_f{1,2}ext.c). Numbering, lm and the CSR structure (s = 9) as the steady NS driver builds
+    them. Returns a dict of host arrays."""
+    x, y, ien = structured_mesh(m, m)
+    ndof, mark, t = structured_marks("CAVITY", x, y)
+    assert ndof == 3
+    invP0, neq = number_unknowns(t, 3)
+    lm0 = Fill_LM(ien, t, invP0, neq, 3)
+    c = csr_Initialization(lm0, 9, neq, reorder, permute_lm=True, type_=t, invP_id=invP0)
+    presc = np.zeros((len(x), 3))
+    presc[np.abs(y - 1.0) <= 1e-15, 0] = 1.0
+    return dict(x=x, y=y, ien=ien, type3=t, invP=c.invP_id, presc=presc.ravel(), neq=neq, lm=c.lm, c=c,
+                fx=np.zeros(len(x)), fy=np.zeros(len(x)))
