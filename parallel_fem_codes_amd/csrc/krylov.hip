@@ -953,7 +953,8 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
             rr = r;
             d_rcur = d_rn;
         }
-        if (rn <= tol * normF || refinements >= opt->max_refinements) break;
+        // a non-finite residual (a diverged preconditioner or non-finite data) ends the solve: status 1
+        if (!std::isfinite(rn) || rn <= tol * normF || refinements >= opt->max_refinements) break;
         // inner FP32 solve of M^-1 A d = M^-1 r / ||r||
         k_scale_to_f32<<<grid * 4, 256, 0, ctx->stream>>>(n, rr, d_rcur, wA);
         FG_HIP(hipGetLastError());

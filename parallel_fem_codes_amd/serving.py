@@ -178,13 +178,14 @@ def run_server(reqs, opts=None, device=0, warmup=0):
     return sols, infos, wall, batch, wg
 
 
-def run_stream(params, opts=None, device=0, workers=6, lookahead=48, keep=(), warmup=8):
+def run_stream(params, opts=None, device=0, workers=6, lookahead=48, keep=(), warmup=8, progress=30.0):
     """The serving workload as a stream: request meshes are built by a pool of client threads ahead
     of their arrival (at most `lookahead` in flight) and submitted at their arrival times; results
     are collected as they finish. Only the requests whose index is in `keep` are retained with
     their solutions (for comparisons); of the others only the server's timings remain. Returns
     (per-request dicts in arrival order, kept {index: (request, solution)}, wall seconds, client
-    submit lag per request (seconds behind the arrival time), batch size, workgroup threads)."""
+    submit lag per request (seconds behind the arrival time), batch size, workgroup threads). A line
+    on stderr reports the requests done every `progress` seconds (0: none)."""
     import queue
     from collections import deque
     from concurrent.futures import ThreadPoolExecutor
@@ -217,9 +218,14 @@ def run_stream(params, opts=None, device=0, workers=6, lookahead=48, keep=(), wa
     th = threading.Thread(target=feed)
     th.start()
     out = [None] * count
-    for _ in range(count):
+    t_note = t0
+    for done in range(count):
         i, ticket, nn, r = q.get()
         u, info = srv.wait(ticket, nn, timeout=3600.0)
+        now = time.perf_counter()
+        if progress and now - t_note >= progress:
+            print(f"# {done + 1}/{count} requests done after {now - t0:.1f} s", file=sys.stderr, flush=True)
+            t_note = now
         out[i] = dict(n=info.n, nnz=info.nnz, latency=info.t_end - info.t_submit, prep=info.prep_s,
                       setup=info.setup_s, solve=info.solve_s, queue=info.t_start - info.t_submit - info.prep_s,
                       status=info.status, residual=info.true_rel_residual, batch=info.batch_size,

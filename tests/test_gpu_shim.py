@@ -6,8 +6,10 @@ ILUp_precond(_setup), NO_precond_setup and pgmres objects, on the reference fixt
 
 Bars: the preconditioned right-hand side M^-1 F and the operator probe A p bit-identical to the
 reference's (the device ILU(0) and products keep its operation order; values cross the ABI
-unchanged), GMRES steps within +-2 per nonlinear iteration and the final state within 1e-10 relative
-L2 of the reference's. With FEMGPU_SOLVER=mixed the solves are the mixed-precision solver's, verified
+unchanged), GMRES steps within +-2 per nonlinear iteration and the final state within 1e-9 relative
+L2 of the reference's (the bar of the device fixed-point driver's test: the GMRES reductions are
+summed in another order, and the shock-capturing fixed point carries that difference: 4e-10 on
+the st_ebe_not_cau_l8 fixture). With FEMGPU_SOLVER=mixed the solves are the mixed-precision solver's, verified
 in FP64 to the same tolerance (parallel ILU(0) with Jacobi sweeps, or with FEMGPU_MIXED_PREC=exact
 the FP32 copy of the exact ILU(0)): the state within 1e-6 of the reference's. The harness binary exists
 only where the reference tree was present at build time; without it the test is skipped."""
@@ -79,6 +81,6 @@ def test_reference_driver_through_the_shim(tmp_path, name, solver):
         its, ref_its = rec["iters"], d["ref_iters"]
         assert len(its) == len(ref_its), (its, ref_its)
         assert np.all(np.abs(its - ref_its) <= 2), (its, ref_its)
-        assert rel(rec["u"], d["ref_u"]) < 1e-10
+        assert rel(rec["u"], d["ref_u"]) < 1e-9
     else:
         assert rel(rec["u"], d["ref_u"]) < 1e-6, rel(rec["u"], d["ref_u"])

@@ -58,7 +58,7 @@ def test_structure_matches_reference(L, name):
 
 @pytest.mark.parametrize("m,right,reorder", [(200, True, "SYMRCM"), (97, False, "SYMRCM"), (60, True, "NOT")])
 def test_structure_matches_oracle_at_scale(L, oracle, m, right, reorder):
-    """Config-0 size (201x201 nodes) and others, including the transient variant that renumbers lm."""
+    """201x201 nodes and others, including the transient variant that renumbers lm."""
     x, y, ien = fem.structured_mesh(m, m, right=right)
     t = fem.boundary_types(x, y)
     invP, neq = fem.number_unknowns(t)
