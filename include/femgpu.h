@@ -248,6 +248,13 @@ int femgpu_ebemv_f32(femgpu_ebe *E, const float *d_P, float *d_Q);
 /* =========================================================== BLAS on device vectors */
 
 int femgpu_ddot(femgpu_ctx *ctx, int n, const double *d_x, const double *d_y, double *result);
+/* No reference counterpart: for callers that cannot allocate (code under hipGraph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). femgpu_ddot's reduction (the same kernels, the
+ * same result bits) with the result written to device memory *d_result on the context's stream,
+ * no host synchronisation, and its partial sums in a fixed 1024-entry (8 KiB) array per GPU inside
+ * the library instead of an allocation: it calls no allocator. Any n >= 0 fits (the reduction uses
+ * at most 1024 blocks). Calls that overlap on one GPU share that array and race. */
+int femgpu_ddot_noalloc(femgpu_ctx *ctx, int n, const double *d_x, const double *d_y, double *d_result);
 int femgpu_daxpy(femgpu_ctx *ctx, int n, double a, const double *d_x, double *d_y);
 int femgpu_dscal(femgpu_ctx *ctx, int n, double a, double *d_x);
 int femgpu_dcopy(femgpu_ctx *ctx, int n, const double *d_x, double *d_y);

@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -97,6 +98,33 @@ using namespace fg;
 extern "C" const char *femgpu_last_error(void) { return g_err; }
 extern "C" const char *femgpu_version(void) { return "femgpu 0.1 (gfx950)"; }
 
+// ------------------------------------------------------------------------------- no-allocation dot
+// Scratch of femgpu_ddot_noalloc: the per-block partial sums of the two-kernel dot product, one
+// array per GPU in the library's code object (placed when the library is loaded on that device and
+// held until the process ends), so the entry point allocates nothing. In host code `scratch` only
+// names the symbol: its device address is looked up once per device, with that device current, at
+// context creation, and only that address is passed to kernels. Every call on one GPU shares that
+// GPU's array: calls that overlap on one GPU race on it.
+static __device__ __align__(256) double scratch[kReduceBlocks];
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+// Called with `device` current.
+static int resolve_scratch(int device, double **out)
+{
+    FG_CHECK(device >= 0 && device < kMaxDevices, FEMGPU_ERR_ARG, "femgpu_create: device %d beyond %d", device,
+             kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        FG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        FG_CHECK(p, FEMGPU_ERR_HIP, "femgpu_create: no device address for the scratch array");
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return FEMGPU_OK;
+}
+
 extern "C" int femgpu_create(int device, void *stream, femgpu_ctx **out)
 {
     FG_CHECK(out, FEMGPU_ERR_ARG, "femgpu_create: null output");
@@ -112,7 +140,8 @@ extern "C" int femgpu_create(int device, void *stream, femgpu_ctx **out)
         ctx->num_cus = prop.multiProcessorCount;
         ctx->lds_limit = prop.sharedMemPerBlock;
     }
-    int rc = ctx->partials.alloc(2 * kReduceBlocks);
+    int rc = resolve_scratch(device, &ctx->noalloc_scratch);
+    if (!rc) rc = ctx->partials.alloc(2 * kReduceBlocks);
     if (!rc) rc = ctx->scalars.alloc(16);
     if (!rc) rc = ctx->err.alloc(4);
     if (!rc && hipMemset(ctx->err.p, 0, 4 * sizeof(int)) != hipSuccess) rc = FEMGPU_ERR_HIP;
@@ -470,6 +499,17 @@ extern "C" int femgpu_ddot(femgpu_ctx *ctx, int n, const double *x, const double
     FG_HIP(hipMemcpyAsync(result, ctx->scalars.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     FG_HIP(hipStreamSynchronize(ctx->stream));
     return FEMGPU_OK;
+}
+
+extern "C" int femgpu_ddot_noalloc(femgpu_ctx *ctx, int n, const double *d_x, const double *d_y, double *d_result)
+{
+    FG_CHECK(ctx && ctx->noalloc_scratch && d_x && d_y && d_result, FEMGPU_ERR_ARG,
+             "femgpu_ddot_noalloc: bad arguments");
+    FG_CHECK(n >= 0, FEMGPU_ERR_ARG, "femgpu_ddot_noalloc: n = %d", n);
+    const int grid = std::max(1, std::min(kReduceBlocks, (n + 1023) / 1024));
+    FG_CHECK(grid <= kReduceBlocks, FEMGPU_ERR_ARG, "femgpu_ddot_noalloc: %d partial sums exceed the %d-entry scratch",
+             grid, kReduceBlocks);
+    return dot64_into(ctx, n, d_x, d_y, ctx->noalloc_scratch, d_result);
 }
 
 extern "C" int femgpu_daxpy(femgpu_ctx *ctx, int n, double a, const double *x, double *y)

@@ -37,6 +37,7 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kReduceBlocks = 1024; // upper bound of blocks of the reduction kernels
+constexpr int kMaxDevices = 64;     // devices a process can hold contexts on
 
 // Typed device allocation; frees on destruction.
 template <typename T>
@@ -150,6 +151,7 @@ struct femgpu_ctx {
     int num_cus = 256;
     size_t lds_limit = 65536;      // dynamic LDS available to one workgroup
     fg::DevBuf<double> partials;   // per-block partial sums (two sets of kReduceBlocks)
+    double *noalloc_scratch = nullptr;   // this device's copy of the no-allocation entry point's array
     fg::DevBuf<double> scalars;    // small device scalars (dot results)
     fg::DevBuf<int> err;           // sticky device error flag (bounded waits)
     double *h_pinned = nullptr;    // pinned host mirror for scalar results
@@ -333,6 +335,8 @@ int launch_ebemv32(femgpu_ctx *ctx, const femgpu_ebe *E, const float *P, float *
 int launch_blockdiag_apply32(femgpu_ctx *ctx, const femgpu_blockdiag *B, const float *p, float *z, const int *skip);
 
 int dot64(femgpu_ctx *ctx, int n, const double *x, const double *y, double *d_out);
+// the same reduction with the per-block partial sums in `partials` (kReduceBlocks entries)
+int dot64_into(femgpu_ctx *ctx, int n, const double *x, const double *y, double *partials, double *d_out);
 // Element-block meshes (assembly_ns.hip): creation and the residual / CSR value gathers.
 int block_mesh_create(femgpu_ctx *ctx, int ndof, int nnodes, int nel, int neq, const double *x, const double *y,
                       const int *type, const int *invP, const double *presc, const int *ien, const int *lm,

@@ -590,7 +590,8 @@ __global__ __launch_bounds__(256) void k_dot(int n, const double *__restrict__ x
 {
     __shared__ double red[256 / kWave];
     double acc = 0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) acc += x[i] * y[i];
+    // 64-bit index: n may lie within one grid stride of INT_MAX
+    for (long i = blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) acc += x[i] * y[i];
     acc = block_sum<256>(acc, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
@@ -604,13 +605,18 @@ __global__ __launch_bounds__(256) void k_dot_finish(int nb, const double *__rest
     if (threadIdx.x == 0) *out = v;
 }
 
-int dot64(femgpu_ctx *ctx, int n, const double *x, const double *y, double *d_out)
+int dot64_into(femgpu_ctx *ctx, int n, const double *x, const double *y, double *partials, double *d_out)
 {
     const int grid = std::max(1, std::min(kReduceBlocks, (n + 1023) / 1024));
-    k_dot<<<grid, 256, 0, ctx->stream>>>(n, x, y, ctx->partials.p);
-    k_dot_finish<<<1, 256, 0, ctx->stream>>>(grid, ctx->partials.p, d_out);
+    k_dot<<<grid, 256, 0, ctx->stream>>>(n, x, y, partials);
+    k_dot_finish<<<1, 256, 0, ctx->stream>>>(grid, partials, d_out);
     FG_HIP(hipGetLastError());
     return FEMGPU_OK;
+}
+
+int dot64(femgpu_ctx *ctx, int n, const double *x, const double *y, double *d_out)
+{
+    return dot64_into(ctx, n, x, y, ctx->partials.p, d_out);
 }
 
 }  // namespace fg

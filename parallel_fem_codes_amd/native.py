@@ -171,6 +171,7 @@ _SIGS = {
     "femgpu_ebemv": (I, [vp, vp, vp]),
     "femgpu_ebemv_f32": (I, [vp, vp, vp]),
     "femgpu_ddot": (I, [vp, I, vp, vp, C.POINTER(D)]),
+    "femgpu_ddot_noalloc": (I, [vp, I, vp, vp, vp]),
     "femgpu_daxpy": (I, [vp, I, D, vp, vp]),
     "femgpu_dscal": (I, [vp, I, D, vp]),
     "femgpu_dcopy": (I, [vp, I, vp, vp]),
