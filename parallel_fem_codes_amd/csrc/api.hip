@@ -2,7 +2,6 @@
 // Solvers are in krylov.hip, assembly in assembly.hip, integer structure work in structure.cpp.
 #include <atomic>
 #include <cmath>
-#include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -12,22 +11,6 @@
 #include "device_util.h"
 
 namespace fg {
-
-static thread_local char g_err[512] = "";
-
-void set_error(const char *fmt, ...)
-{
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_err, sizeof(g_err), fmt, ap);
-    va_end(ap);
-}
-
-int hip_fail(hipError_t e, const char *what, const char *file, int line)
-{
-    set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
-    return FEMGPU_ERR_HIP;
-}
 
 int check_device_error(femgpu_ctx *ctx)
 {
@@ -94,9 +77,6 @@ static int host_ilu0(int n, const std::vector<int> &IA, const std::vector<int> &
 }  // namespace fg
 
 using namespace fg;
-
-extern "C" const char *femgpu_last_error(void) { return g_err; }
-extern "C" const char *femgpu_version(void) { return "femgpu 0.1 (gfx950)"; }
 
 // ------------------------------------------------------------------------------- no-allocation dot
 // Scratch of femgpu_ddot_noalloc: the per-block partial sums of the two-kernel dot product, one

@@ -299,8 +299,12 @@ int launch_stream_factor(femgpu_ctx *ctx, femgpu_csr *A);
 // natural: rows in index order (the parallel factorisation); else the lower sweep's level schedule.
 int build_factor_plan(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag, bool natural,
                       FactorPlan &F);
+// LDS bytes one workgroup of the packed sweep needs for T-typed values.
 template <typename T>
-size_t packed_lds_bytes(const PackedSweep &P);
+inline size_t packed_lds_bytes(const PackedSweep &P)
+{
+    return sizeof(T) * (3 * static_cast<size_t>(P.R) + P.emax) + sizeof(int) * (static_cast<size_t>(P.emax) + 5 * P.R + 1);
+}
 
 // Launchers (kernels_*.hip).
 int launch_spmv64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *xs, double *xout,

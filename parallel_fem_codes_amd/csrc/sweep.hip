@@ -483,15 +483,6 @@ static PackedArgs packed_args(femgpu_ctx *ctx, PackedSweep &P, bool fp32)
     return a;
 }
 
-template <typename T>
-size_t packed_lds_bytes(const PackedSweep &P)
-{
-    return sizeof(T) * (3 * static_cast<size_t>(P.R) + P.emax) + sizeof(int) * (static_cast<size_t>(P.emax) + 5 * P.R + 1);
-}
-
-template size_t packed_lds_bytes<double>(const PackedSweep &);
-template size_t packed_lds_bytes<float>(const PackedSweep &);
-
 int packed_refresh(femgpu_ctx *ctx, femgpu_csr *A)
 {
     for (StreamSweep *P : {&A->slo, &A->sup}) {
