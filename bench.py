@@ -232,7 +232,7 @@ def euler_run(fem, torch, dev, timed, timed_cold, mx=800, my=400, steps=3, multi
     return out
 
 
-def ns_run(fem, torch, dev, m=512, Re=1000.0, iterations=3, kmax=50, lmax=40, tol=1e-8, budget_s=40.0):
+def ns_run(fem, torch, dev, m=512, Re=1000.0, iterations=2, kmax=50, lmax=4, tol=1e-8, budget_s=30.0):
     """The steady Navier-Stokes workload: the reference's CAVITY problem at Re = 1000 on (m+1)^2 nodes
     (2 m^2 triangles, u, v, p per node, 9x9 SUPG/PSPG element blocks, CSR), Picard iterations of NS
     Process.c from rest with each linear system solved by GMRES(kmax) + ILU(0) to `tol` (at most
@@ -272,6 +272,7 @@ def ns_run(fem, torch, dev, m=512, Re=1000.0, iterations=3, kmax=50, lmax=40, to
                   file=sys.stderr, flush=True)
         states[solver] = (len(per), u)
         out[solver] = {"ms_per_picard_iteration": round(1e3 * float(np.mean(per)), 3), "picard_iterations": len(per),
+                       "ms_per_arnoldi_step": round(1e3 * float(np.sum(per)) / max(sum(lin), 1), 3),
                        "linear_iterations": lin, "linear_solves_not_converged": failed}
     out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms_per_picard_iteration"] / out["mixed"]["ms_per_picard_iteration"], 3)
     (k64, u64), (kmx, umx) = states["fp64"], states["mixed"]
@@ -353,10 +354,14 @@ def main():
     ap.add_argument("--nonlinear-tol", type=float, default=1e-4, help="multicorrection stop (StopByNorm)")
     ap.add_argument("--euler-steps", type=int, default=3,
                     help="pseudo-time steps of the 801x401 Euler workload run after the timed region (0: skip)")
-    ap.add_argument("--ns-iterations", type=int, default=3,
+    ap.add_argument("--ns-iterations", type=int, default=2,
                     help="Picard iterations of the 513x513 cavity run after the timed region (0: skip)")
     ap.add_argument("--serving-requests", type=int, default=100,
                     help="requests of the serving workload run after the timed region (0: skip)")
+    ap.add_argument("--profile-step", default=None,
+                    choices=("mixed", "mixed_graphs", "mixed_exact_ilu", "fp64", "fp64_parallel"),
+                    help="run --warmup steps of this solver mode, then one step between two marker fills of a "
+                         "16-float tensor (torch fill kernels), for a kernel trace of exactly one step; then exit")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="counter runs: one mixed step, 20 launches of the roofline kernel and a 256 MiB copy")
     a = ap.parse_args()
@@ -420,7 +425,8 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"profile_kernels": "done", "copy_bytes": src.numel() * 4, "n": neq, "nnz_upper": nnz_u}))
+        print(json.dumps({"profile_kernels": "done", "copy_bytes": src.numel() * 4, "n": neq, "nnz_upper": nnz_u,
+                          "column_bytes": A.sweep_info()["jacobi_column_bytes"]}))
         return
 
     # kernel of the FP64 products (csrmv, pgmres operator, FP64 residual): CSR lanes per row or
@@ -430,6 +436,22 @@ def main():
     lanes, lane_us = A.autotune()
     print(f"# FP64 product kernel (lanes per row, 0 = sliced ELL): {lanes} (" +
           ", ".join(f"{k}: {v:.1f} us" for k, v in lane_us.items()) + ")", file=sys.stderr, flush=True)
+
+    if a.profile_step:
+        for _ in range(a.warmup):
+            step(a.profile_step)
+        marker = torch.empty(16, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        marker.fill_(1.0)
+        torch.cuda.synchronize()
+        step(a.profile_step)
+        torch.cuda.synchronize()
+        marker.fill_(2.0)
+        torch.cuda.synchronize()
+        inf = info_box[a.profile_step]
+        print(json.dumps({"profile_step": a.profile_step, "iterations": inf.iterations, "cycles": inf.cycles,
+                          "n": neq, "nnz": c.nnz}))
+        return
 
     results = {}
     for mode in ("fp64", "fp64_parallel", "mixed_exact_ilu", "mixed_graphs", "mixed"):
@@ -515,11 +537,16 @@ def main():
     t_spmv32_cold = timed_cold(lambda: A.csrmv(p32, q32))
     t_par = timed(lambda: A.ilu0_factor_parallel_f32(a.factor_sweeps), reps=5)
     t_jac = timed(lambda: A.ilu0_apply_jacobi_f32(a.jacobi, p32, z32))
-    bytes_spmv = 12 * nnz + 4 * (n + 1) + 16 * n
-    bytes_spmv32 = 8 * nnz + 4 * (n + 1) + 8 * n
+    # column bytes of the layouts the kernels read: 2 where the band lets the sliced-ELL copies store
+    # 16-bit distances from the row, else 4 (int32, as in CSR)
+    si = A.sweep_info()
+    cbs, cbj = si["sell_column_bytes"] or 4, si["jacobi_column_bytes"] or 4
+    bytes_spmv = (8 + (cbs if lanes == 0 else 4)) * nnz + 4 * (n + 1) + 16 * n
+    bytes_spmv32 = (4 + cbs) * nnz + 4 * (n + 1) + 8 * n
     bytes_ilu = 12 * (nnz - n) + 8 * (n + 1) + 8 * n + 16 * n
-    bytes_jac = a.jacobi * (8 * (nnz - n) + 44 * n)
+    bytes_jac = a.jacobi * ((4 + cbj) * (nnz - n) + 44 * n)
     kernels = {
+        "column_bytes": {"sliced_ell": cbs, "jacobi_triangles": cbj},
         "csrmv_f64": dict(us=t_spmv * 1e6, GBs=bytes_spmv / t_spmv / 1e9, lanes_per_row=lanes,
                           kernel="k_spmv_sell<double> (sliced ELL)" if lanes == 0 else f"k_spmv<double,{lanes}> (CSR)"),
         "csrmv_f64_cold": dict(us=t_spmv_cold * 1e6, GBs=bytes_spmv / t_spmv_cold / 1e9),
@@ -551,15 +578,16 @@ def main():
     del Ae_t, E
     # Roofline kernel: the upper-triangle Jacobi iteration on the FP32 factors (k_jacobi_upper_ell),
     # the largest kernel by time in the mixed step (2 launches per preconditioner application).
-    # Algorithmic bytes per launch: strict-upper entries (FP32 value + int32 column) once, and the
-    # vectors y, Dinv, z_in (gathered, counted once) and z_out: 8*nnz_U + 16*n.
+    # Algorithmic bytes per launch: strict-upper entries (FP32 value + column: 16-bit distance from
+    # the row where the band allows, else int32) once, and the vectors y, Dinv, z_in (gathered,
+    # counted once) and z_out: (4 + column bytes)*nnz_U + 16*n.
     A.ilu0_factor_parallel_f32(a.factor_sweeps)
     y32 = torch.rand(neq, dtype=torch.float32, device="cuda")
     z1, z2 = torch.empty_like(y32), torch.empty_like(y32)
     A.ilu0_jacobi_iteration_f32(True, y32, z1)
     t_up = timed(lambda: A.ilu0_jacobi_iteration_f32(True, y32, z2, z1), reps=50)
     t_up_cold = timed_cold(lambda: A.ilu0_jacobi_iteration_f32(True, y32, z2, z1), reps=50)
-    bytes_up = 8 * nnz_u + 16 * n
+    bytes_up = (4 + cbj) * nnz_u + 16 * n
     kernels["jacobi_upper_f32"] = dict(us=t_up * 1e6, GBs=bytes_up / t_up / 1e9)
     kernels["jacobi_upper_f32_cold"] = dict(us=t_up_cold * 1e6, GBs=bytes_up / t_up_cold / 1e9)
     traffic = None
@@ -567,15 +595,16 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as fh:
             tr = json.load(fh).get("k_jacobi_upper_ell", {})
-        if tr.get("n") == neq:
+        if tr.get("n") == neq and tr.get("column_bytes", 4) == cbj:
             traffic = tr.get("hbm_bytes_per_launch")
     ach = bytes_up / t_up / 1e9
     ach_cold = bytes_up / t_up_cold / 1e9
     roofline = dict(bound="hbm", kernel="k_jacobi_upper_ell (upper-triangle Jacobi iteration, FP32 factors)",
                     achieved=round(ach, 2), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 5),
-                    traffic=traffic, bytes_per_launch=bytes_up, us_per_launch=round(t_up * 1e6, 3),
-                    frac_is="back-to-back launches on resident operands, as inside the solver loop (the 40 MB "
-                            "working set fits the 256 MB last-level cache)",
+                    traffic=traffic, bytes_per_launch=bytes_up, column_bytes=cbj,
+                    us_per_launch=round(t_up * 1e6, 3),
+                    frac_is=f"back-to-back launches on resident operands, as inside the solver loop (the "
+                            f"{bytes_up / 1e6:.0f} MB working set fits the 256 MB last-level cache)",
                     achieved_cold=round(ach_cold, 2), frac_cold=round(ach_cold / HBM_PEAK_GBS, 5),
                     us_per_launch_cold=round(t_up_cold * 1e6, 3),
                     cold_is="each launch after a 1 GiB read evicts the last-level cache: operands from HBM")

@@ -12,11 +12,11 @@
  * Reference interfaces replaced (paths relative to the reference root):
  *   ebemv / ebemvNDOF3 / ebemvNDOF4 / csrmv  00_CommonFiles/MatrixVector_Operations/matvec.h:3-18
  *   pgmres                                   00_CommonFiles/Solvers_and_Preconditioners/solvers.h:14
- *   ILUp_precond / ILUp_precond_setup        00_CommonFiles/Solvers_and_Preconditioners/preconditioners.h:30,37
+ *   ILUp_precond / ILUp_precond_setup        00_CommonFiles/Solvers_and_Preconditioners/preconditioners.h:30,35
  *   ddot / daxpy / dcopy / dscal / dzero     00_CommonFiles/BLAS_Operations/ourBLAS.h:1-5
  *   Fill_LM / csr_Initialization / Permutation_of_LM
- *                                            01_SS_Transport_Equation/2D/01_CommonFiles/SSTranspEquation.h:170-221
- *   Build_K_F_SUPG / Build_M_K_R_SUPG        SSTranspEquation.h:203, 02_Transport_Equation/.../TranspEquation.h
+ *                                            01_SS_Transport_Equation/2D/01_CommonFiles/SSTranspEquation.h:174,176,225
+ *   Build_K_F_SUPG / Build_M_K_R_SUPG        SSTranspEquation.h:217, 02_Transport_Equation/2D/01_CommonFiles/Build_M_K_R_SUPG.c:6
  * The reference-signature shims that forward to this ABI live in parallel_fem_codes_amd/compat/.
  */
 #ifndef FEMGPU_H
@@ -141,6 +141,8 @@ typedef struct femgpu_sweep_info {
     int rows_per_chunk;    /* packed: row bound per chunk */
     int lds_limit;         /* LDS bytes available to one workgroup */
     int factor_kernel;     /* 0: streamed single-workgroup factorisation, 1: level-synchronous lists */
+    int jacobi_column_bytes;   /* Jacobi sliced-ELL triangles: 2 (16-bit distances from the row), 4, 0: none */
+    int sell_column_bytes;     /* sliced-ELL products: 2 (16-bit distances from the row), 4, 0: none */
 } femgpu_sweep_info;
 int femgpu_csr_sweep_info(femgpu_csr *A, femgpu_sweep_info *info);
 /* Select the sweep kernel: 0 = streamed when the plan allows it (default), 1 = packed. */

@@ -313,6 +313,8 @@ extern "C" int femgpu_csr_sweep_info(femgpu_csr *A, femgpu_sweep_info *info)
     info->rows_per_chunk = A->plo.R;
     info->lds_limit = static_cast<int>(A->ctx->lds_limit);
     info->factor_kernel = (A->factor_kernel == 0 && A->sfac.ok) ? 0 : 1;
+    info->jacobi_column_bytes = A->ell_kl > 0 ? (A->ell16 ? 2 : 4) : 0;
+    info->sell_column_bytes = A->sell_ok ? (A->sell16 ? 2 : 4) : 0;
     return FEMGPU_OK;
 }
 

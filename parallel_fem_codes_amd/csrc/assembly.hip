@@ -281,8 +281,11 @@ __global__ __launch_bounds__(256) void k_elem_transient(int nel, ElemIn in, doub
 // element order, as Build_M_K_R_SUPG.c:173-182). The sums are those of the element-array path, so
 // the results are bit-identical to it, and no element array is written or read back: elements on
 // a patch border are evaluated once per patch that touches them instead.
-constexpr int kFzRows = 128;        // rows (threads) per block
-constexpr int kFzMaxSlots = 1152;   // LDS slots per block: 1152 x 7 doubles = 63 KB
+// 64 rows per block: measured 125 us per assembly against 139 us with 128 rows and 193 us with 256
+// on the 1001 x 1001 mesh (smaller patches re-evaluate more border elements, but a block's LDS
+// shrinks with its rows and more blocks share a CU)
+constexpr int kFzRows = 64;        // rows (threads) per block
+constexpr int kFzMaxSlots = 576;   // LDS slots per block: 576 x 7 doubles = 31.5 KB
 constexpr int kFzVals = 7;
 
 template <int MAXLEN>

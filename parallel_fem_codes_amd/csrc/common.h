@@ -186,6 +186,8 @@ struct femgpu_csr {
     int sell_wmax = 0;                  // widest slice (entries per row)
     fg::DevBuf<int> sell_ptr;           // slice -> first entry (nslices + 1); slice width = difference / 64
     fg::DevBuf<int> sell_col;           // entry k of row 64s + l at sell_ptr[s] + 64k + l (padding: the row)
+    bool sell16 = false;                // the fixed-width product kernels read sell_d16 instead:
+    fg::DevBuf<short> sell_d16;         // column - row as 16 bits (padding 0), when every entry fits
     fg::DevBuf<float> sell_val;         // FP32 values in that layout (padding 0), refreshed with AA32
     fg::DevBuf<double> sell_val64;      // FP64 values in that layout, kept while spmv_lanes == 0
     std::vector<int> hsell_ptr;         // host copy of sell_ptr
@@ -215,6 +217,8 @@ struct femgpu_csr {
     int ell_kl = 0, ell_ku = 0;         // Jacobi sliced-ELL widest strict triangles (0: CSR kernels)
     fg::DevBuf<int> ell_sl, ell_su;     // slice pointers (64-row slices) of the lower / upper parts
     fg::DevBuf<int> ell_cl, ell_cu;     // columns, entry k of row i at sl[i/64] + 64k + i%64 (padding: column i)
+    bool ell16 = false;                 // the kernels read the columns as 16-bit distances from the row:
+    fg::DevBuf<unsigned short> ell_dl, ell_du;   // lower i - j, upper j - i (padding 0), when all fit
     fg::DevBuf<float> ell_vl, ell_vu;   // their FP32 factor values (padding 0), refreshed with LU32
     fg::FactorPlan fplan;           // numeric factorisation (level schedule)
     fg::FactorPlan fplan_nat;       // the same lists in row order (parallel FP32 factorisation)

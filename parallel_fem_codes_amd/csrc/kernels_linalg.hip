@@ -59,8 +59,13 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ IA,
 // k_spmv_sell_loop takes any width.
 constexpr int kSell64MaxWidth = 16;   // widest slice the FP64 sliced-ELL kernels take
 
-template <typename T, int K>
-__global__ __launch_bounds__(256) void k_spmv_sell(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+// Column of a sliced-ELL entry of row r: int columns as stored, 16-bit ones as the distance from r
+// (2 bytes per entry instead of 4 when the band allows).
+__device__ __forceinline__ int sell_colof(int c, int) { return c; }
+__device__ __forceinline__ int sell_colof(short c, int r) { return r + static_cast<int>(c); }
+
+template <typename T, int K, typename CT>
+__global__ __launch_bounds__(256) void k_spmv_sell(int n, const int *__restrict__ sptr, const CT *__restrict__ col,
                                                    const T *__restrict__ val, const T *__restrict__ x,
                                                    const double *__restrict__ xs_ptr, T *__restrict__ xout,
                                                    T *__restrict__ y, const int *__restrict__ skip)
@@ -75,7 +80,7 @@ __global__ __launch_bounds__(256) void k_spmv_sell(int n, const int *__restrict_
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int e = p0 + ((k < w ? k : 0) << 6) + l;
-        c[k] = col[e];
+        c[k] = sell_colof(col[e], rr);
         v[k] = val[e];
     }
     const T xs = xs_ptr ? static_cast<T>(*xs_ptr) : T(1);
@@ -113,8 +118,8 @@ __global__ __launch_bounds__(256) void k_spmv_sell_loop(int n, const int *__rest
 }
 
 // r = f - A x on the FP64 sliced-ELL copy (loads as in k_spmv_sell<double, K>).
-template <int K>
-__global__ __launch_bounds__(256) void k_residual_sell(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+template <int K, typename CT>
+__global__ __launch_bounds__(256) void k_residual_sell(int n, const int *__restrict__ sptr, const CT *__restrict__ col,
                                                        const double *__restrict__ val, const double *__restrict__ x,
                                                        const double *__restrict__ f, double *__restrict__ r)
 {
@@ -127,7 +132,7 @@ __global__ __launch_bounds__(256) void k_residual_sell(int n, const int *__restr
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int e = p0 + ((k < w ? k : 0) << 6) + l;
-        c[k] = col[e];
+        c[k] = sell_colof(col[e], rr);
         v[k] = val[e];
     }
     const double fr = f[rr];
@@ -202,9 +207,26 @@ int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA)
             }
         }
     }
+    // 16-bit column distances when the band allows
+    std::vector<short> d16(col.size());
+    bool fits = true;
+    for (int s = 0; s < ns && fits; ++s)
+        for (int l = 0; l < 64 && fits; ++l) {
+            const int r = std::min(64 * s + l, n - 1);
+            for (int p = sptr[s] + l; p < sptr[s + 1]; p += 64) {
+                const int d = col[p] - r;
+                fits = fits && d >= -32768 && d <= 32767;
+                d16[p] = static_cast<short>(d);
+            }
+        }
+    A->sell16 = false;
     A->hsell_ptr = sptr;
     FG_TRY(A->sell_ptr.upload(sptr.data(), sptr.size(), ctx->stream));
     FG_TRY(A->sell_col.upload(col.data(), col.size(), ctx->stream));
+    if (fits) {
+        FG_TRY(A->sell_d16.upload(d16.data(), d16.size(), ctx->stream));
+        A->sell16 = true;
+    }
     FG_TRY(A->sell_val.alloc(col.size()));
     if (A->spmv_lanes == 0 && wmax <= kSell64MaxWidth) FG_TRY(A->sell_val64.alloc(col.size()));
     FG_HIP(hipStreamSynchronize(ctx->stream));   // the host arrays end with this call
@@ -297,12 +319,20 @@ static int sell_dispatch(femgpu_ctx *ctx, const femgpu_csr *A, const T *val, con
 {
     const int n = A->n, grid = xcd_grid((n + 255) / 256);
     const int *sp = A->sell_ptr.p, *sc = A->sell_col.p;
-    if (A->sell_wmax <= 8)
-        k_spmv_sell<T, 8><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, val, x, xs, xout, y, skip);
-    else if (A->sell_wmax <= 16)
-        k_spmv_sell<T, 16><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, val, x, xs, xout, y, skip);
-    else
+    const short *sd = A->sell_d16.p;
+    if (A->sell_wmax <= 8) {
+        if (A->sell16)
+            k_spmv_sell<T, 8, short><<<grid, 256, 0, ctx->stream>>>(n, sp, sd, val, x, xs, xout, y, skip);
+        else
+            k_spmv_sell<T, 8, int><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, val, x, xs, xout, y, skip);
+    } else if (A->sell_wmax <= 16) {
+        if (A->sell16)
+            k_spmv_sell<T, 16, short><<<grid, 256, 0, ctx->stream>>>(n, sp, sd, val, x, xs, xout, y, skip);
+        else
+            k_spmv_sell<T, 16, int><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, val, x, xs, xout, y, skip);
+    } else {
         k_spmv_sell_loop<T><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, val, x, xs, xout, y, skip);
+    }
     FG_HIP(hipGetLastError());
     return FEMGPU_OK;
 }
@@ -327,11 +357,19 @@ int launch_residual64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, con
     if (sell64_active(A)) {
         const int n = A->n, grid = xcd_grid((n + 255) / 256);
         const int *sp = A->sell_ptr.p, *sc = A->sell_col.p;
+        const short *sd = A->sell_d16.p;
         const double *sv = A->sell_val64.p;
-        if (A->sell_wmax <= 8)   // sell64_active implies sell_wmax <= kSell64MaxWidth
-            k_residual_sell<8><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, sv, x, f, r);
-        else
-            k_residual_sell<kSell64MaxWidth><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, sv, x, f, r);
+        if (A->sell_wmax <= 8) {   // sell64_active implies sell_wmax <= kSell64MaxWidth
+            if (A->sell16)
+                k_residual_sell<8, short><<<grid, 256, 0, ctx->stream>>>(n, sp, sd, sv, x, f, r);
+            else
+                k_residual_sell<8, int><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, sv, x, f, r);
+        } else {
+            if (A->sell16)
+                k_residual_sell<kSell64MaxWidth, short><<<grid, 256, 0, ctx->stream>>>(n, sp, sd, sv, x, f, r);
+            else
+                k_residual_sell<kSell64MaxWidth, int><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, sv, x, f, r);
+        }
         FG_HIP(hipGetLastError());
         return FEMGPU_OK;
     }

@@ -370,9 +370,22 @@ struct EllRows {
     int w[kJacRows];
 };
 
+// Column of a stored entry of row i: int columns as stored; 16-bit distances from the row (lower
+// triangle i - j, upper j - i; padding entries 0, i.e. column i).
+template <typename CT, bool UP>
+__device__ __forceinline__ int ell_col(CT c, int i);
+template <>
+__device__ __forceinline__ int ell_col<int, false>(int c, int) { return c; }
+template <>
+__device__ __forceinline__ int ell_col<int, true>(int c, int) { return c; }
+template <>
+__device__ __forceinline__ int ell_col<unsigned short, false>(unsigned short c, int i) { return i - static_cast<int>(c); }
+template <>
+__device__ __forceinline__ int ell_col<unsigned short, true>(unsigned short c, int i) { return i + static_cast<int>(c); }
+
 // Loads the slice entries of the thread's rows; w[r] = 0 for rows past n and for empty slices.
-template <typename T, int K>
-__device__ __forceinline__ void ell_load(int n, int blk, const int *__restrict__ sp, const int *__restrict__ col,
+template <typename T, int K, typename CT, bool UP>
+__device__ __forceinline__ void ell_load(int n, int blk, const int *__restrict__ sp, const CT *__restrict__ col,
                                          const T *__restrict__ val, EllRows<T, K> &e)
 {
 #pragma unroll
@@ -385,7 +398,7 @@ __device__ __forceinline__ void ell_load(int n, int blk, const int *__restrict__
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const int p = q + ((k < w ? k : 0) << 6);
-                e.col[r][k] = col[p];
+                e.col[r][k] = ell_col<CT, UP>(col[p], ii);
                 e.val[r][k] = val[p];
             }
         } else {
@@ -399,8 +412,8 @@ __device__ __forceinline__ void ell_load(int n, int blk, const int *__restrict__
 }
 
 // y_out = b - L_s y_in (y_in == nullptr: y_in = b), sliced ELL.
-template <typename T, int K>
-__global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__restrict__ sl, const int *__restrict__ cl,
+template <typename T, int K, typename CT>
+__global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__restrict__ sl, const CT *__restrict__ cl,
                                                           const T *__restrict__ vl, const T *__restrict__ b,
                                                           const T *__restrict__ yin, T *__restrict__ yout,
                                                           const int *__restrict__ skip)
@@ -409,7 +422,7 @@ __global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__re
     const T *src = yin ? yin : b;
     const int blk = xcd_block();
     EllRows<T, K> e;
-    ell_load<T, K>(n, blk, sl, cl, vl, e);
+    ell_load<T, K, CT, false>(n, blk, sl, cl, vl, e);
     T bi[kJacRows];
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
@@ -430,8 +443,8 @@ __global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__re
 }
 
 // z_out = Dinv (y - U_s z_in) (z_in == nullptr: z_in = Dinv y), sliced ELL.
-template <typename T, int K>
-__global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, const int *__restrict__ su, const int *__restrict__ cu,
+template <typename T, int K, typename CT>
+__global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, const int *__restrict__ su, const CT *__restrict__ cu,
                                                           const T *__restrict__ vu, const T *__restrict__ Dinv,
                                                           const T *__restrict__ y, const T *__restrict__ zin,
                                                           T *__restrict__ zout, const int *__restrict__ skip)
@@ -439,7 +452,7 @@ __global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, const int *__re
     if (skip && *skip) return;
     const int blk = xcd_block();
     EllRows<T, K> e;
-    ell_load<T, K>(n, blk, su, cu, vu, e);
+    ell_load<T, K, CT, true>(n, blk, su, cu, vu, e);
     T di[kJacRows], yi[kJacRows];
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
@@ -497,10 +510,33 @@ int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *J
     }
     kl = std::max(kl, 1);
     ku = std::max(ku, 1);
+    // 16-bit distances from the row when the band allows (2 bytes per entry instead of 4)
+    std::vector<unsigned short> dl(cl.size()), du(cu.size());
+    bool fits = true;
+    for (int s = 0; s < ns && fits; ++s)
+        for (int l = 0; l < 64 && fits; ++l) {
+            const int r = std::min(64 * s + l, n - 1);
+            for (int p = sl[s] + l; p < sl[s + 1]; p += 64) {
+                const int d = r - cl[p];
+                fits = fits && d >= 0 && d <= 0xffff;
+                dl[p] = static_cast<unsigned short>(d);
+            }
+            for (int p = su[s] + l; p < su[s + 1]; p += 64) {
+                const int d = cu[p] - r;
+                fits = fits && d >= 0 && d <= 0xffff;
+                du[p] = static_cast<unsigned short>(d);
+            }
+        }
+    A->ell16 = false;
     FG_TRY(A->ell_sl.upload(sl.data(), sl.size(), ctx->stream));
     FG_TRY(A->ell_su.upload(su.data(), su.size(), ctx->stream));
     FG_TRY(A->ell_cl.upload(cl.data(), cl.size(), ctx->stream));
     FG_TRY(A->ell_cu.upload(cu.data(), cu.size(), ctx->stream));
+    if (fits) {
+        FG_TRY(A->ell_dl.upload(dl.data(), dl.size(), ctx->stream));
+        FG_TRY(A->ell_du.upload(du.data(), du.size(), ctx->stream));
+        A->ell16 = true;
+    }
     FG_TRY(A->ell_vl.alloc(cl.size()));
     FG_TRY(A->ell_vu.alloc(cu.size()));
     FG_HIP(hipStreamSynchronize(ctx->stream));   // the host arrays end with this call
@@ -593,16 +629,16 @@ int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A)
 // x = (U~ L~)^-1 b approximated with it Jacobi iterations per triangle on factors in the Jacobi
 // sliced-ELL layout (vl, vu strict triangles, dinv inverse pivots); ta, tb: two work vectors.
 // b may alias x.
-template <typename T, int K>
-static void jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, int it, const T *vl, const T *vu, const T *dinv, T *ta, T *tb,
-                       const T *b, T *x, const int *skip)
+template <typename T, int K, typename CT>
+static void jacobi_ell_cols(femgpu_ctx *ctx, femgpu_csr *A, const CT *cl, const CT *cu, int it, const T *vl,
+                            const T *vu, const T *dinv, T *ta, T *tb, const T *b, T *x, const int *skip)
 {
     const int n = A->n, grid = xcd_grid((n + 256 * kJacRows - 1) / (256 * kJacRows));
     T *t[2] = {ta, tb};
     const T *prev = nullptr;
     for (int k = 0; k < it; ++k) {
         T *out = t[k & 1];
-        k_jacobi_lower_ell<T, K><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, vl, b, prev, out, skip);
+        k_jacobi_lower_ell<T, K, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, cl, vl, b, prev, out, skip);
         prev = out;
     }
     const T *y = prev;
@@ -610,9 +646,19 @@ static void jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, int it, const T *vl, cons
     prev = nullptr;
     for (int k = 1; k <= it; ++k) {
         T *out = ((it - k) % 2 == 0) ? x : other;
-        k_jacobi_upper_ell<T, K><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, vu, dinv, y, prev, out, skip);
+        k_jacobi_upper_ell<T, K, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, cu, vu, dinv, y, prev, out, skip);
         prev = out;
     }
+}
+
+template <typename T, int K>
+static void jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, int it, const T *vl, const T *vu, const T *dinv, T *ta, T *tb,
+                       const T *b, T *x, const int *skip)
+{
+    if (A->ell16)
+        jacobi_ell_cols<T, K, unsigned short>(ctx, A, A->ell_dl.p, A->ell_du.p, it, vl, vu, dinv, ta, tb, b, x, skip);
+    else
+        jacobi_ell_cols<T, K, int>(ctx, A, A->ell_cl.p, A->ell_cu.p, it, vl, vu, dinv, ta, tb, b, x, skip);
 }
 
 // The current values of A into the sliced-ELL layout in FP64 (padding 0).
@@ -692,6 +738,34 @@ int launch_jacobi_apply64(femgpu_ctx *ctx, femgpu_csr *A, int it, const double *
     return FEMGPU_OK;
 }
 
+template <typename CT>
+static void jacobi_iteration_ell(femgpu_ctx *ctx, femgpu_csr *A, const CT *cl, const CT *cu, int w, int grid, int upper,
+                                 const float *rhs, const float *in, float *out)
+{
+    const int n = A->n;
+    if (upper) {
+        if (w <= 4)
+            k_jacobi_upper_ell<float, 4, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, cu, A->ell_vu.p, A->Dinv32.p,
+                                                                         rhs, in, out, nullptr);
+        else if (w <= 8)
+            k_jacobi_upper_ell<float, 8, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, cu, A->ell_vu.p, A->Dinv32.p,
+                                                                         rhs, in, out, nullptr);
+        else
+            k_jacobi_upper_ell<float, 16, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, cu, A->ell_vu.p, A->Dinv32.p,
+                                                                          rhs, in, out, nullptr);
+    } else {
+        if (w <= 4)
+            k_jacobi_lower_ell<float, 4, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, cl, A->ell_vl.p, rhs, in, out,
+                                                                         nullptr);
+        else if (w <= 8)
+            k_jacobi_lower_ell<float, 8, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, cl, A->ell_vl.p, rhs, in, out,
+                                                                         nullptr);
+        else
+            k_jacobi_lower_ell<float, 16, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, cl, A->ell_vl.p, rhs, in, out,
+                                                                          nullptr);
+    }
+}
+
 int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const float *rhs, const float *in,
                               float *out)
 {
@@ -700,27 +774,10 @@ int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const f
     const int grid = xcd_grid((n + 256 * kJacRows - 1) / (256 * kJacRows));
     if (A->ell_kl > 0) {
         const int w = std::max(A->ell_kl, A->ell_ku);
-        if (upper) {
-            if (w <= 4)
-                k_jacobi_upper_ell<float, 4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
-                                                                     A->Dinv32.p, rhs, in, out, nullptr);
-            else if (w <= 8)
-                k_jacobi_upper_ell<float, 8><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
-                                                                     A->Dinv32.p, rhs, in, out, nullptr);
-            else
-                k_jacobi_upper_ell<float, 16><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, A->ell_cu.p, A->ell_vu.p,
-                                                                      A->Dinv32.p, rhs, in, out, nullptr);
-        } else {
-            if (w <= 4)
-                k_jacobi_lower_ell<float, 4><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs, in,
-                                                                     out, nullptr);
-            else if (w <= 8)
-                k_jacobi_lower_ell<float, 8><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs, in,
-                                                                     out, nullptr);
-            else
-                k_jacobi_lower_ell<float, 16><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, A->ell_cl.p, A->ell_vl.p, rhs,
-                                                                      in, out, nullptr);
-        }
+        if (A->ell16)
+            jacobi_iteration_ell<unsigned short>(ctx, A, A->ell_dl.p, A->ell_du.p, w, grid, upper, rhs, in, out);
+        else
+            jacobi_iteration_ell<int>(ctx, A, A->ell_cl.p, A->ell_cu.p, w, grid, upper, rhs, in, out);
     } else {
         constexpr int LPR = 4;
         const int g4 = static_cast<int>((static_cast<long>(n) * LPR + 255) / 256);
