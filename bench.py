@@ -232,7 +232,7 @@ def euler_run(fem, torch, dev, timed, timed_cold, mx=800, my=400, steps=3, multi
     return out
 
 
-def ns_run(fem, torch, dev, m=512, Re=1000.0, iterations=2, kmax=50, lmax=4, tol=1e-8, budget_s=30.0):
+def ns_run(fem, torch, dev, m=512, Re=1000.0, iterations=2, kmax=50, lmax=20, tol=1e-8, budget_s=30.0):
     """The steady Navier-Stokes workload: the reference's CAVITY problem at Re = 1000 on (m+1)^2 nodes
     (2 m^2 triangles, u, v, p per node, 9x9 SUPG/PSPG element blocks, CSR), Picard iterations of NS
     Process.c from rest with each linear system solved by GMRES(kmax) + ILU(0) to `tol` (at most

@@ -283,13 +283,15 @@ __global__ __launch_bounds__(256) void k_elem_transient(int nel, ElemIn in, doub
 // a patch border are evaluated once per patch that touches them instead.
 // 64 rows per block: measured 125 us per assembly against 139 us with 128 rows and 193 us with 256
 // on the 1001 x 1001 mesh (smaller patches re-evaluate more border elements, but a block's LDS
-// shrinks with its rows and more blocks share a CU)
-constexpr int kFzRows = 64;        // rows (threads) per block
+// shrinks with its rows and more blocks share a CU). The element evaluation is latency-bound, so
+// each 64-row block runs 256 threads for it (126 us with 64, 109 us with 128, 103 us with 256).
+constexpr int kFzRows = 64;        // rows per block
+constexpr int kFzThreads = 256;    // threads per block: all evaluate elements, the first kFzRows gather rows
 constexpr int kFzMaxSlots = 576;   // LDS slots per block: 576 x 7 doubles = 31.5 KB
 constexpr int kFzVals = 7;
 
 template <int MAXLEN>
-__global__ __launch_bounds__(kFzRows) void k_transient_fused(int nblocks, ElemIn in, const int *__restrict__ eptr,
+__global__ __launch_bounds__(kFzThreads) void k_transient_fused(int nblocks, ElemIn in, const int *__restrict__ eptr,
                                                              const int *__restrict__ elem,
                                                              const ushort4 *__restrict__ eq,
                                                              const int *__restrict__ iptr,
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(kFzRows) void k_transient_fused(int nblocks, ElemIn
         AA[nnz] = 0.0;        // csr_assembly's dummy slot
         R[in.neq] = 0.0;      // Build_M_K_R_SUPG.c:183
     }
-    for (int le = eptr[w] + t; le < eptr[w + 1]; le += kFzRows) {
+    for (int le = eptr[w] + t; le < eptr[w + 1]; le += kFzThreads) {
         const int e = elem[le];
         const ushort4 q = eq[le];
         TransientElem T;
@@ -326,6 +328,7 @@ __global__ __launch_bounds__(kFzRows) void k_transient_fused(int nblocks, ElemIn
             }
     }
     __syncthreads();
+    if (t >= kFzRows) return;
     const int4 ri = rinfo[w * kFzRows + t];   // {row, first CSR slot, row length, incidences}
     if (ri.x < 0) return;
     double acc[MAXLEN];
@@ -818,11 +821,11 @@ extern "C" int femgpu_assemble_transport_transient(femgpu_mesh *M, const femgpu_
             const size_t lds = sizeof(double) * kFzVals * static_cast<size_t>(std::max(M->fz_slots, 1));
             const int grid = xcd_grid(M->fz_nblocks);
             if (M->fz_maxlen <= 8)
-                k_transient_fused<8><<<grid, kFzRows, lds, ctx->stream>>>(
+                k_transient_fused<8><<<grid, kFzThreads, lds, ctx->stream>>>(
                     M->fz_nblocks, in, M->fz_eptr.p, M->fz_elem.p, M->fz_eq.p, M->fz_iptr.p, M->fz_inc.p, M->fz_row.p,
                     M->nnz, d_AA, d_R);
             else
-                k_transient_fused<16><<<grid, kFzRows, lds, ctx->stream>>>(
+                k_transient_fused<16><<<grid, kFzThreads, lds, ctx->stream>>>(
                     M->fz_nblocks, in, M->fz_eptr.p, M->fz_elem.p, M->fz_eq.p, M->fz_iptr.p, M->fz_inc.p, M->fz_row.p,
                     M->nnz, d_AA, d_R);
             FG_HIP(hipGetLastError());

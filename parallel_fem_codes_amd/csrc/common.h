@@ -97,19 +97,25 @@ struct PackedSweep {
 
 // Single-workgroup streamed sweep (sweep.hip, k_sweep_stream): the level schedule cut into steps of
 // at most kStreamThreads rows (a level wider than that is split), row data padded to K dependency
-// entries per row in schedule order, and each dependency named by its slot in an LDS ring of W
-// results (schedule position mod W; slot W holds a zero for padding). The plan exists only when
-// every dependency is still in the ring when read (W >= end of the reading step - position of the
-// dependency) and the ring fits in LDS; otherwise the packed multi-workgroup sweep is used.
+// entries per row in schedule order, and each dependency named by its slot in LDS: an LDS ring of
+// W results (schedule position mod W; slot W holds a zero for padding, slot W+1 takes writes that
+// nobody reads) followed by L long-lived slots. A dependency still in the ring when read (W >= end
+// of the reading step - position of the dependency) is read from the ring; the few read later than
+// that are also written, by their own row, to a long-lived slot that no other value takes until
+// the step after its last reader. The plan exists when the rows have at most 16 dependencies and
+// everything fits in LDS; otherwise the packed multi-workgroup sweep is used.
 constexpr int kStreamThreads = 512;
 constexpr int kStreamPad = 16;     // step count padded to a multiple of this (twice the largest lookahead)
+constexpr int kStreamRingMax = 8192;    // ring entries at most (longer reaches take long-lived slots)
+constexpr int kStreamLongMax = 4096;    // long-lived slots at most
 
 struct StreamSweep {
     bool ok = false;
-    int n = 0, nsteps = 0, steps = 0, K = 0, W = 0, depth = 0;   // nsteps: padded, steps: real
+    int n = 0, nsteps = 0, steps = 0, K = 0, W = 0, L = 0, depth = 0;   // nsteps: padded, steps: real
     DevBuf<int> stepq;              // step -> first schedule position (nsteps + 2*kStreamPad + 1, tail = n)
     DevBuf<int> row_of;             // schedule position -> row (n)
-    DevBuf<unsigned short> slot;    // n*K ring slots
+    DevBuf<unsigned short> slot;    // n*K LDS slots of the dependencies
+    DevBuf<unsigned short> keep;    // L > 0: n second write slots (a long-lived slot, or W + 1)
     DevBuf<int> src;                // n*K positions in A's layout, -1 for padding (value refresh)
     DevBuf<double> val64, dinv64;   // n*K factor values / n inverse pivots (upper sweep) in schedule order
     DevBuf<float> val32, dinv32;

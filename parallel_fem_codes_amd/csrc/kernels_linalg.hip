@@ -518,8 +518,8 @@ __global__ __launch_bounds__(kFactorThreads) void k_ilu0_factor_levels(int depth
 // update list, the finished U values and pivots it reads, its own values -- so a level costs a few
 // memory round trips instead of one per update; the row's values and multipliers live in a private
 // LDS slice. Same operations in the same order as the serial factorisation, contraction off.
-// Rows beyond the compile-time bounds (MAXROW values, MAXL multipliers, MAXU updates) take a
-// sequential path with the same arithmetic.
+// Longer update lists are taken in chunks of MAXU. Rows beyond the compile-time bounds (MAXROW
+// values, MAXL multipliers) take a sequential path with the same arithmetic.
 template <int MAXROW, int MAXL, int MAXU, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_ilu0_factor_lists(int depth, const int *__restrict__ lvl_ptr,
                                                                const int4 *__restrict__ rowinfo,
@@ -539,13 +539,8 @@ __global__ __launch_bounds__(THREADS) void k_ilu0_factor_lists(int depth, const 
             const int nu = rowinfo[q + 1].w - ub;   // rowinfo has a sentinel entry after the last row
             const int i = JA[p0 + nL];
             double piv;
-            if (len <= MAXROW && nL <= MAXL && nu <= MAXU) {
-                int2 U[MAXU];
-                double Uv[MAXU];
+            if (len <= MAXROW && nL <= MAXL) {
                 int jr[MAXL];
-#pragma unroll
-                for (int u = 0; u < MAXU; ++u)
-                    if (u < nu) U[u] = upd[ub + u];
 #pragma unroll
                 for (int t = 0; t < MAXL; ++t)
                     if (t < nL) jr[t] = JA[p0 + t];
@@ -555,16 +550,25 @@ __global__ __launch_bounds__(THREADS) void k_ilu0_factor_lists(int depth, const 
 #pragma unroll
                 for (int t = 0; t < MAXL; ++t)
                     if (t < nL) dj[t * THREADS] = Dinv[jr[t]];
-#pragma unroll
-                for (int u = 0; u < MAXU; ++u)
-                    if (u < nu) Uv[u] = LU[U[u].x];
                 int sc = 0;   // L entries [0, sc) already turned into multipliers
+                // the update list in chunks of MAXU: each chunk's list entries and U values are
+                // loaded together, then applied in list order
+                for (int u0 = 0; u0 < nu; u0 += MAXU) {
+                    int2 U[MAXU];
+                    double Uv[MAXU];
 #pragma unroll
-                for (int u = 0; u < MAXU; ++u) {
-                    if (u < nu) {
-                        const int t = U[u].y >> 16, dest = U[u].y & 0xffff;
-                        for (; sc <= t; ++sc) v[sc * THREADS] = v[sc * THREADS] * dj[sc * THREADS];
-                        v[dest * THREADS] -= v[t * THREADS] * Uv[u];
+                    for (int u = 0; u < MAXU; ++u)
+                        if (u0 + u < nu) U[u] = upd[ub + u0 + u];
+#pragma unroll
+                    for (int u = 0; u < MAXU; ++u)
+                        if (u0 + u < nu) Uv[u] = LU[U[u].x];
+#pragma unroll
+                    for (int u = 0; u < MAXU; ++u) {
+                        if (u0 + u < nu) {
+                            const int t = U[u].y >> 16, dest = U[u].y & 0xffff;
+                            for (; sc <= t; ++sc) v[sc * THREADS] = v[sc * THREADS] * dj[sc * THREADS];
+                            v[dest * THREADS] -= v[t * THREADS] * Uv[u];
+                        }
                     }
                 }
                 for (; sc < nL; ++sc) v[sc * THREADS] = v[sc * THREADS] * dj[sc * THREADS];

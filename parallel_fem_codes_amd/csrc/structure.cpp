@@ -6,6 +6,7 @@
 #include <climits>
 #include <cstring>
 #include <numeric>
+#include <queue>
 #include <vector>
 
 #include "common.h"
@@ -598,7 +599,8 @@ int build_packed_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, cons
 // pattern updates that slot (or the pivot when the column is i), in U order.
 size_t stream_lds_bytes(const StreamSweep &P, size_t elem)
 {
-    return elem * (static_cast<size_t>(P.W) + 2) + sizeof(int) * (static_cast<size_t>(P.nsteps) + 2 * kStreamPad + 1);
+    return elem * (static_cast<size_t>(P.W) + 2 + P.L) +
+           sizeof(int) * (static_cast<size_t>(P.nsteps) + 2 * kStreamPad + 1);
 }
 
 int build_stream_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, const int *diag, bool upper,
@@ -622,7 +624,7 @@ int build_stream_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, cons
     }
     P.depth = depth;
     int K = 0;
-    for (int k : {4, 8})
+    for (int k : {4, 8, 16})
         if (maxdeg <= k) { K = k; break; }
     if (K == 0) return FEMGPU_OK;   // rows too long for the register-staged kernel
     // schedule: rows by (level, sweep order); levels cut into steps of at most kStreamThreads rows
@@ -637,14 +639,16 @@ int build_stream_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, cons
         }
     }
     for (int q = 0; q < n; ++q) pos[row_of[q]] = q;
-    std::vector<int> stepq(1, 0), step_beg(n), step_end(n);
+    std::vector<int> stepq(1, 0), step_beg(n), step_end(n), step_of(n);
     for (int l = 0; l < depth; ++l)
         for (int a = lptr[l]; a < lptr[l + 1]; a += kStreamThreads) {
             const int b = std::min(lptr[l + 1], a + kStreamThreads);
+            const int st = static_cast<int>(stepq.size()) - 1;
             stepq.push_back(b);
             for (int q = a; q < b; ++q) {
                 step_beg[q] = a;
                 step_end[q] = b;
+                step_of[q] = st;
             }
         }
     // kStreamPad empty steps in front (the kernel's pipeline fills on them), empty steps at the tail
@@ -660,10 +664,44 @@ int build_stream_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, cons
         for (int p = ent_begin(r); p < ent_end(r); ++p) wreq = std::max<long long>(wreq, step_end[q] - pos[JA[p]]);
     }
     int W = 64;
-    while (W < wreq && W < (1 << 15)) W <<= 1;
-    if (W < wreq) return FEMGPU_OK;
+    while (W < wreq && W < kStreamRingMax) W <<= 1;
+    // values read beyond the ring's reach: a long-lived slot from the step that writes the value to
+    // the last step that reads it (slots handed out in order of the writing step, a slot free again
+    // for a value written after its last reader's step)
+    std::vector<int> last_read(n, -1);   // by schedule position of the value: last reading step
+    for (int q = 0; q < n; ++q) {
+        const int r = row_of[q];
+        for (int p = ent_begin(r); p < ent_end(r); ++p) {
+            const int qj = pos[JA[p]];
+            if (step_end[q] - qj > W) last_read[qj] = std::max(last_read[qj], step_of[q]);
+        }
+    }
+    std::vector<int> lslot(n, -1);
+    int L = 0;
+    {
+        std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>, std::greater<>> busy;
+        std::vector<int> free_slots;
+        for (int q = 0; q < n; ++q) {   // positions in step order
+            if (last_read[q] < 0) continue;
+            while (!busy.empty() && busy.top().first < step_of[q]) {
+                free_slots.push_back(busy.top().second);
+                busy.pop();
+            }
+            int sl;
+            if (!free_slots.empty()) {
+                sl = free_slots.back();
+                free_slots.pop_back();
+            } else {
+                sl = L++;
+            }
+            lslot[q] = sl;
+            busy.push({last_read[q], sl});
+        }
+    }
+    if (L > kStreamLongMax || W + 2 + L > 0xffff) return FEMGPU_OK;
     P.K = K;
     P.W = W;
+    P.L = L;
     P.nsteps = nsteps;
     P.steps = nsteps_real;
     const size_t limit = ctx->lds_limit > 1024 ? ctx->lds_limit - 1024 : ctx->lds_limit;
@@ -680,7 +718,9 @@ int build_stream_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, cons
                 set_error("stream plan: dependency of row %d is not scheduled before it", r);
                 return FEMGPU_ERR_ARG;
             }
-            slot[static_cast<size_t>(q) * K + k] = static_cast<unsigned short>(qj & (W - 1));
+            const bool far = step_end[q] - qj > W;
+            slot[static_cast<size_t>(q) * K + k] =
+                static_cast<unsigned short>(far ? W + 2 + lslot[qj] : (qj & (W - 1)));
             src[static_cast<size_t>(q) * K + k] = p;
         }
     }
@@ -689,6 +729,12 @@ int build_stream_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, cons
     FG_TRY(P.row_of.upload(row_of.data(), n, st));
     FG_TRY(P.slot.upload(slot.data(), slot.size(), st));
     FG_TRY(P.src.upload(src.data(), src.size(), st));
+    std::vector<unsigned short> keep;
+    if (L > 0) {
+        keep.resize(n);
+        for (int q = 0; q < n; ++q) keep[q] = static_cast<unsigned short>(lslot[q] >= 0 ? W + 2 + lslot[q] : W + 1);
+        FG_TRY(P.keep.upload(keep.data(), keep.size(), st));
+    }
     FG_TRY(P.val64.alloc(slot.size()));
     FG_TRY(P.val32.alloc(slot.size()));
     FG_TRY(P.sink.alloc(kStreamThreads));
@@ -736,7 +782,7 @@ int build_stream_factor(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int
     StreamFactor &F = A->sfac;
     F.ok = false;
     const StreamSweep &P = A->slo;
-    if (!P.ok || P.K != 4 || A->n == 0) return FEMGPU_OK;
+    if (!P.ok || P.K != 4 || P.L != 0 || A->n == 0) return FEMGPU_OK;
     const size_t limit = ctx->lds_limit > 1024 ? ctx->lds_limit - 1024 : ctx->lds_limit;
     if (factor_lds_bytes(P) > limit) return FEMGPU_OK;
     const int n = A->n;

@@ -267,9 +267,10 @@ __global__ __launch_bounds__(kPackedThreads) void k_sweep_packed(PackedArgs S, c
 struct StreamArgs {
     int nsteps;          // kStreamPad empty steps, the schedule, empty steps up to a multiple of
                          // kStreamPad; stepq has nsteps + 2*kStreamPad + 1 entries
-    int W;
+    int W, L;            // ring entries, long-lived slots after them
     const int *stepq;
     const unsigned short *slot;
+    const unsigned short *keep;   // L > 0: second write slot of each position (long-lived or W + 1)
     void *sink;          // kStreamThreads results of idle threads
 };
 
@@ -279,6 +280,7 @@ struct StreamStage {
     unsigned short s[K];
     T b, d;
     int q;               // schedule position of this thread's row in the staged step, -1: idle
+    unsigned short ks;   // LONG: second write slot
 };
 
 template <int BYTES>
@@ -310,7 +312,7 @@ __device__ __forceinline__ int stream_pos(const int *sq, int s)
 }
 
 // Stage step s into g: entries, slots, pivot and right-hand side by schedule position.
-template <typename T, bool UPPER, int K>
+template <typename T, bool UPPER, int K, bool LONG>
 __device__ __forceinline__ void stream_fetch(StreamStage<T, K> &g, const StreamArgs &a, const int *sq, int s,
                                              const T *__restrict__ val, const T *__restrict__ dinv, const T *b)
 {
@@ -320,13 +322,14 @@ __device__ __forceinline__ void stream_fetch(StreamStage<T, K> &g, const StreamA
     load_bytes<K * 2>(a.slot + static_cast<size_t>(qq) * K, g.s);
     if (UPPER) g.d = dinv[qq];
     g.b = b[qq];
+    if (LONG) g.ks = a.keep[qq];
     g.q = q;
 }
 
 // UPPER: x = (b - U x) * Dinv; else x = b - L x (ILUp_precond.c:26-44, same operation order,
 // contraction off; padding entries subtract 0 * 0). b and x are in schedule order; b may alias x:
 // a row's b is read D steps before its own result is stored, and no other row reads it.
-template <typename T, bool UPPER, int K, int D>
+template <typename T, bool UPPER, int K, int D, bool LONG>
 __global__ __launch_bounds__(kStreamThreads) void k_sweep_stream(StreamArgs a, const T *__restrict__ val,
                                                                  const T *__restrict__ dinv, const T *b, T *x,
                                                                  const int *__restrict__ skip)
@@ -335,8 +338,8 @@ __global__ __launch_bounds__(kStreamThreads) void k_sweep_stream(StreamArgs a, c
     static_assert(kStreamPad % D == 0 && D <= kStreamPad, "step padding must cover the lookahead");
     if (skip && *skip) return;
     extern __shared__ unsigned char smem_raw[];
-    T *ring = reinterpret_cast<T *>(smem_raw);                  // W results, a zero, an idle slot
-    int *sq = reinterpret_cast<int *>(ring + a.W + 2);
+    T *ring = reinterpret_cast<T *>(smem_raw);   // W results, a zero, an idle slot, L long-lived slots
+    int *sq = reinterpret_cast<int *>(ring + a.W + 2 + a.L);
     const int nsq = a.nsteps + 2 * kStreamPad + 1;
     for (int i = threadIdx.x; i < nsq; i += blockDim.x) sq[i] = a.stepq[i];
     if (threadIdx.x == 0) ring[a.W] = T(0);
@@ -356,6 +359,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_sweep_stream(StreamArgs a, c
         }
         st[u].b = st[u].d = T(0);
         st[u].q = -1;
+        st[u].ks = static_cast<unsigned short>(a.W + 1);
     }
     for (int s0 = 0; s0 < a.nsteps; s0 += D) {
 #pragma unroll
@@ -370,8 +374,9 @@ __global__ __launch_bounds__(kStreamThreads) void k_sweep_stream(StreamArgs a, c
             if (UPPER) acc *= g.d;
             const bool on = g.q >= 0;
             ring[on ? (g.q & wmask) : a.W + 1] = acc;
+            if (LONG) ring[g.ks] = acc;   // read again beyond the ring's reach (else the idle slot)
             *(on ? x + g.q : sink) = acc;
-            stream_fetch<T, UPPER, K>(g, a, sq, s0 + u + D, val, dinv, b);
+            stream_fetch<T, UPPER, K, LONG>(g, a, sq, s0 + u + D, val, dinv, b);
             lds_barrier();
         }
     }
@@ -408,8 +413,10 @@ static StreamArgs stream_args(const StreamSweep &P)
     StreamArgs a;
     a.nsteps = P.nsteps;
     a.W = P.W;
+    a.L = P.L;
     a.stepq = P.stepq.p;
     a.slot = P.slot.p;
+    a.keep = P.L > 0 ? P.keep.p : nullptr;
     a.sink = P.sink.p;
     return a;
 }
@@ -420,10 +427,18 @@ static int stream_launch(femgpu_ctx *ctx, const StreamSweep &P, const T *val, co
 {
     const size_t lds = stream_lds_bytes(P, sizeof(T));
     const StreamArgs a = stream_args(P);
-    if (P.K == 4)
-        k_sweep_stream<T, UPPER, 4, 8><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
+    if (P.K == 4 && P.L == 0)
+        k_sweep_stream<T, UPPER, 4, 8, false><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
+    else if (P.K == 4)
+        k_sweep_stream<T, UPPER, 4, 8, true><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
+    else if (P.K == 8 && P.L == 0)
+        k_sweep_stream<T, UPPER, 8, 4, false><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
+    else if (P.K == 8)
+        k_sweep_stream<T, UPPER, 8, 4, true><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
+    else if (P.L == 0)
+        k_sweep_stream<T, UPPER, 16, 2, false><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
     else
-        k_sweep_stream<T, UPPER, 8, 4><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
+        k_sweep_stream<T, UPPER, 16, 2, true><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
     FG_HIP(hipGetLastError());
     return FEMGPU_OK;
 }

@@ -270,8 +270,8 @@ def test_ebemv_int32_columns_and_uncovered_unknowns(dev, torch_cuda, oracle):
 
 def synthetic_csr(n, maxdeg, seed):
     """Diagonally dominant random CSR (sorted rows, full diagonal) whose rows reach up to maxdeg
-    entries on each side of the diagonal -- longer than mesh rows, so the streamed sweep's padded
-    row format does not apply and the packed sweep is used."""
+    entries on each side of the diagonal -- longer than transport mesh rows: up to 16 the streamed
+    sweep pads rows to 16 entries, beyond that the packed sweep is used."""
     rng = np.random.default_rng(seed)
     rows = []
     for i in range(n):
@@ -299,11 +299,32 @@ def ilu_case(oracle, name):
         return steady_system(oracle, m=600, reorder="NOT")
     if name == "synthetic12":
         return synthetic_csr(20000, 12, 3)
+    if name == "synthetic20":
+        return synthetic_csr(20000, 20, 4)
+    if name.startswith("ns"):   # the Navier-Stokes cavity pattern (9x9 blocks, up to 11 entries per triangle)
+        return ns_pattern_csr(int(name[2:]), 7)
     return steady_system(oracle, d=G.load(name))
 
 
+def ns_pattern_csr(m, seed):
+    """The SYMRCM-ordered CSR pattern of the (m+1)^2-node cavity problem (u, v, p per node) with
+    diagonally dominant random values: rows of up to 11 strict entries per triangle. At m = 128 some
+    results are read more than 8 192 schedule positions after they are computed, beyond the
+    streamed sweep's largest ring, and take its long-lived LDS slots."""
+    c = fem.cavity_problem(m)["c"]
+    IA, JA = c.IA.astype(np.int32), c.JA.astype(np.int32)
+    n = len(IA) - 1
+    rng = np.random.default_rng(seed)
+    AA = rng.uniform(-1, 1, len(JA))
+    rows = np.repeat(np.arange(n), np.diff(IA))
+    diag = JA == rows
+    AA[diag] = 2.0 + np.bincount(rows, weights=np.abs(AA), minlength=n)
+    return dict(IA=IA, JA=JA, AA=np.append(AA, 0.0), neq=n)
+
+
 @pytest.mark.parametrize("kernel", ["stream", "packed"])
-@pytest.mark.parametrize("name", CSR_CASES + ["m200", "m200cau", "m600not", "synthetic12"])
+@pytest.mark.parametrize("name", CSR_CASES + ["m200", "m200cau", "m600not", "synthetic12", "synthetic20", "ns64",
+                                  "ns128"])
 def test_ilu0_factor_and_sweeps_bit_exact(dev, torch_cuda, oracle, name, kernel):
     torch = torch_cuda
     s = ilu_case(oracle, name)
@@ -311,8 +332,15 @@ def test_ilu0_factor_and_sweeps_bit_exact(dev, torch_cuda, oracle, name, kernel)
     A.set_sweep_kernel(kernel)
     A.set_factor_kernel("stream" if kernel == "stream" else "lists")
     info = A.sweep_info()
-    if kernel == "packed" or name == "synthetic12":
+    print(f"{name} ({kernel}): {info}", flush=True)
+    if kernel == "packed" or name == "synthetic20":   # synthetic20: rows beyond 16 entries
         assert info["kernel"] == 1 and info["factor_kernel"] == 1
+    elif name in ("synthetic12", "ns64", "ns128"):     # up to 16 entries per row: streamed sweep
+        assert info["kernel"] == 0 and info["entries_per_row"] == 16 and info["factor_kernel"] == 1
+        if name in ("synthetic12", "ns128"):   # some results read more than 8 192 positions later
+            assert info["long_slots"] > 0 and info["ring"] == 8192
+        else:
+            assert info["long_slots"] == 0
     else:
         assert info["kernel"] == 0 and info["entries_per_row"] == 4 and info["factor_kernel"] == 0
         if name == "m600not":
