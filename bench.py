@@ -345,7 +345,7 @@ def main():
     ap.add_argument("--m", type=int, default=1000, help="squares per side (1000 -> 1001x1001 nodes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
-    ap.add_argument("--factor-sweeps", type=int, default=2, help="parallel FP32 ILU(0) sweeps (mixed)")
+    ap.add_argument("--factor-sweeps", type=int, default=1, help="parallel FP32 ILU(0) sweeps (mixed)")
     ap.add_argument("--jacobi", type=int, default=2, help="Jacobi iterations per triangle (mixed)")
     ap.add_argument("--orth", choices=("MGS", "CGS"), default="MGS", help="inner Arnoldi Gram-Schmidt (mixed)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="length of the CPU baseline sample")
