@@ -162,25 +162,44 @@ __global__ __launch_bounds__(256) void k_parilu_init_sell(int n, const int *__re
     Dinv[i] = T(1) / own;
 }
 
-template <typename T, int MAXW>
+// One sweep. FIRST: the sweep from the initial guess (L = strict lower part of A with each column
+// divided by its diagonal entry, U = upper part, pivots = the diagonal) computed from A's values
+// directly -- the same divisions and products as k_parilu_init_sell followed by a sweep, without
+// storing the guess. ELL: the result's strict triangles are also written to the Jacobi sliced-ELL
+// arrays (vl, vu; their padding entries are zero from allocation), in place of
+// k_parilu_sell_to_ell.
+template <typename T, int MAXW, bool FIRST, bool ELL>
 __global__ __launch_bounds__(256) void k_parilu_sweep_sell(int n, const int *__restrict__ sptr,
                                                            const int *__restrict__ col, const T *__restrict__ AA,
                                                            const int *__restrict__ rinfo,
                                                            const int *__restrict__ uptr, const int *__restrict__ usrc,
                                                            const unsigned char *__restrict__ ucode,
+                                                           const int *__restrict__ dpos,
                                                            const T *__restrict__ LUo, const T *__restrict__ Dio,
-                                                           T *__restrict__ LUn, T *__restrict__ Din)
+                                                           T *__restrict__ LUn, T *__restrict__ Din,
+                                                           const int *__restrict__ sl, const int *__restrict__ su,
+                                                           T *__restrict__ vl, T *__restrict__ vu)
 {
     const int i = xcd_block() * 256 + threadIdx.x;
     if (i >= n) return;
     const int s = i >> 6, l = i & 63, p0 = sptr[s] + l, w = (sptr[s + 1] - sptr[s]) >> 6;
     const int u0 = uptr[s] + l, nu = (uptr[s + 1] - uptr[s]) >> 6;
     const int nL = rinfo[i] & 0xff;
-    T v[MAXW], lo[MAXW];
+    T v[MAXW], lo[MAXW], dcol[MAXW];
 #pragma unroll
     for (int k = 0; k < MAXW; ++k) {
         v[k] = k < w ? AA[p0 + (k << 6)] : T(0);
-        lo[k] = k < w ? LUo[p0 + (k << 6)] : T(0);
+        if (!FIRST) lo[k] = k < w ? LUo[p0 + (k << 6)] : T(0);
+    }
+    if (FIRST) {
+        // the guess's multipliers a_ij / a_jj and pivot inverses 1 / a_jj of the lower columns
+#pragma unroll
+        for (int k = 0; k < MAXW; ++k) {
+            const int e = p0 + ((k < w ? k : 0) << 6);
+            const T piv = AA[dpos[col[e]]];
+            lo[k] = k < nL ? v[k] / piv : v[k];
+            dcol[k] = T(1) / piv;
+        }
     }
     // update terms in groups of four: the group's codes, sources and factor values are loaded
     // together (positions past the row's list clamped to its first, their terms skipped), then
@@ -193,7 +212,7 @@ __global__ __launch_bounds__(256) void k_parilu_sweep_sell(int n, const int *__r
         for (int g = 0; g < G; ++g) {
             const int q = u0 + ((u + g < nu ? u + g : 0) << 6);
             code[g] = u + g < nu ? ucode[q] : 0xffu;
-            other[g] = LUo[usrc[q]];
+            other[g] = FIRST ? AA[usrc[q]] : LUo[usrc[q]];   // a strict-upper entry of an earlier row
         }
 #pragma unroll
         for (int g = 0; g < G; ++g)
@@ -209,15 +228,48 @@ __global__ __launch_bounds__(256) void k_parilu_sweep_sell(int n, const int *__r
             }
     }
     T piv = T(1);
+    const int len = rinfo[i] >> 8;
+    const int ql = ELL ? sl[s] + l : 0, qu = ELL ? su[s] + l : 0;
 #pragma unroll
     for (int k = 0; k < MAXW; ++k)
         if (k < w) {
             const int e = p0 + (k << 6);
-            const T out = k < nL ? v[k] * Dio[col[e]] : v[k];
+            const T out = k < nL ? v[k] * (FIRST ? dcol[k] : Dio[col[e]]) : v[k];
             LUn[e] = out;
             if (k == nL) piv = out;
+            if (ELL) {
+                if (k < nL) vl[ql + (k << 6)] = out;
+                else if (k > nL && k < len) vu[qu + ((k - nL - 1) << 6)] = out;
+            }
         }
     Din[i] = T(1) / piv;
+}
+
+// The FP32 and FP64 parallel factorisations: `sweeps` sweeps into the (L1, D1) / (L0, D0) ping-pong
+// ending in (La, Da); the last one also fills the Jacobi arrays when vl is set.
+template <typename T, int MAXW>
+static void parilu_sweeps(femgpu_ctx *ctx, const femgpu_csr *A, int sweeps, const T *val, T *La, T *Lb, T *Da,
+                          T *Db, T *vl, T *vu)
+{
+    const int n = A->n, grid = xcd_grid((n + 255) / 256);
+    // sweep k writes Ln and the pair swaps: sweep 1 writes La when the count is odd, so the last
+    // sweep writes La
+    T *Ln = (sweeps % 2 == 1) ? La : Lb, *Lo = (sweeps % 2 == 1) ? Lb : La;
+    T *Dn = (sweeps % 2 == 1) ? Da : Db, *Do = (sweeps % 2 == 1) ? Db : Da;
+    for (int k = 1; k <= sweeps; ++k) {
+        const bool first = k == 1, last = k == sweeps && vl;
+#define FG_PARILU_SWEEP(F, E)                                                                                         \
+    k_parilu_sweep_sell<T, MAXW, F, E><<<grid, 256, 0, ctx->stream>>>(                                                 \
+        n, A->sell_ptr.p, A->sell_col.p, val, A->par_rinfo.p, A->par_uptr.p, A->par_usrc.p, A->par_ucode.p,            \
+        A->par_dpos.p, Lo, Do, Ln, Dn, A->ell_sl.p, A->ell_su.p, vl, vu)
+        if (first && last) FG_PARILU_SWEEP(true, true);
+        else if (first) FG_PARILU_SWEEP(true, false);
+        else if (last) FG_PARILU_SWEEP(false, true);
+        else FG_PARILU_SWEEP(false, false);
+#undef FG_PARILU_SWEEP
+        std::swap(Lo, Ln);
+        std::swap(Do, Dn);
+    }
 }
 
 // The strict triangles of the SELL-layout factors into the Jacobi ELL arrays.
@@ -537,8 +589,12 @@ int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *J
         FG_TRY(A->ell_du.upload(du.data(), du.size(), ctx->stream));
         A->ell16 = true;
     }
+    // zero from allocation: the kernels that fill the values from parallel-ILU sweeps write only the
+    // real entries, so the padding must already hold zeros
     FG_TRY(A->ell_vl.alloc(cl.size()));
     FG_TRY(A->ell_vu.alloc(cu.size()));
+    FG_HIP(hipMemsetAsync(A->ell_vl.p, 0, sizeof(float) * cl.size(), ctx->stream));
+    FG_HIP(hipMemsetAsync(A->ell_vu.p, 0, sizeof(float) * cu.size(), ctx->stream));
     FG_HIP(hipStreamSynchronize(ctx->stream));   // the host arrays end with this call
     A->ell_kl = kl;
     A->ell_ku = ku;
@@ -559,34 +615,26 @@ static int parilu_sell(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
 {
     const int n = A->n, grid = xcd_grid((n + 255) / 256);
     // the final iterate must land in par_a / Dinv32
-    float *L0 = (sweeps % 2 == 0) ? A->par_a.p : A->par_b.p, *L1 = (sweeps % 2 == 0) ? A->par_b.p : A->par_a.p;
-    float *D0 = (sweeps % 2 == 0) ? A->Dinv32.p : A->Dinv32b.p, *D1 = (sweeps % 2 == 0) ? A->Dinv32b.p : A->Dinv32.p;
-    if (A->par_maxw <= 8)
-        k_parilu_init_sell<float, 8><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
-                                                             A->par_rinfo.p, A->par_dpos.p, L0, D0);
-    else
-        k_parilu_init_sell<float, 16><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
-                                                              A->par_rinfo.p, A->par_dpos.p, L0, D0);
-    for (int s = 0; s < sweeps; ++s) {
+    float *vl = A->ell_kl > 0 ? A->ell_vl.p : nullptr, *vu = A->ell_kl > 0 ? A->ell_vu.p : nullptr;
+    if (sweeps == 0) {   // the initial guess itself
         if (A->par_maxw <= 8)
-            k_parilu_sweep_sell<float, 8><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
-                                                                   A->par_rinfo.p, A->par_uptr.p, A->par_usrc.p,
-                                                                   A->par_ucode.p, L0, D0, L1, D1);
+            k_parilu_init_sell<float, 8><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
+                                                                 A->par_rinfo.p, A->par_dpos.p, A->par_a.p, A->Dinv32.p);
         else
-            k_parilu_sweep_sell<float, 16><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
-                                                                    A->par_rinfo.p, A->par_uptr.p, A->par_usrc.p,
-                                                                    A->par_ucode.p, L0, D0, L1, D1);
-        std::swap(L0, L1);
-        std::swap(D0, D1);
+            k_parilu_init_sell<float, 16><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->sell_val.p,
+                                                                  A->par_rinfo.p, A->par_dpos.p, A->par_a.p, A->Dinv32.p);
+        if (vl)
+            k_parilu_sell_to_ell<float><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->par_rinfo.p, A->par_a.p,
+                                                                A->ell_sl.p, A->ell_su.p, vl, vu);
+    } else if (A->par_maxw <= 8) {
+        parilu_sweeps<float, 8>(ctx, A, sweeps, A->sell_val.p, A->par_a.p, A->par_b.p, A->Dinv32.p, A->Dinv32b.p, vl, vu);
+    } else {
+        parilu_sweeps<float, 16>(ctx, A, sweeps, A->sell_val.p, A->par_a.p, A->par_b.p, A->Dinv32.p, A->Dinv32b.p, vl,
+                                 vu);
     }
     FG_HIP(hipGetLastError());
     A->lu32_in_sell = true;
-    if (A->ell_kl > 0) {
-        k_parilu_sell_to_ell<float><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->par_rinfo.p, A->par_a.p,
-                                                            A->ell_sl.p, A->ell_su.p, A->ell_vl.p, A->ell_vu.p);
-        FG_HIP(hipGetLastError());
-        return FEMGPU_OK;
-    }
+    if (vl) return FEMGPU_OK;
     return launch_parilu_to_csr(ctx, A);   // the CSR Jacobi kernels read LU32
 }
 
@@ -689,35 +737,22 @@ int launch_parilu64(femgpu_ctx *ctx, femgpu_csr *A, int sweeps)
     FG_TRY(A->par64_b.alloc(tot));
     FG_TRY(A->par64_d.alloc(n1));
     FG_TRY(A->par64_db.alloc(n1));
-    FG_TRY(A->ell64_vl.alloc(A->ell_cl.n));
-    FG_TRY(A->ell64_vu.alloc(A->ell_cu.n));
+    if (A->ell64_vl.n < A->ell_cl.n || A->ell64_vu.n < A->ell_cu.n) {   // new arrays: padding zeros
+        FG_TRY(A->ell64_vl.alloc(A->ell_cl.n));
+        FG_TRY(A->ell64_vu.alloc(A->ell_cu.n));
+        FG_HIP(hipMemsetAsync(A->ell64_vl.p, 0, sizeof(double) * A->ell64_vl.n, ctx->stream));
+        FG_HIP(hipMemsetAsync(A->ell64_vu.p, 0, sizeof(double) * A->ell64_vu.n, ctx->stream));
+    }
     FG_TRY(A->jac64_a.alloc(n1));
     FG_TRY(A->jac64_b.alloc(n1));
-    const int grid = xcd_grid((n + 255) / 256);
     k_sell_fill64<<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, A->IA.p, A->sell_ptr.p, A->AA.p, A->par64_v.p);
-    // the final iterate must land in par64_a / par64_d
-    double *L0 = (sweeps % 2 == 0) ? A->par64_a.p : A->par64_b.p, *L1 = (sweeps % 2 == 0) ? A->par64_b.p : A->par64_a.p;
-    double *D0 = (sweeps % 2 == 0) ? A->par64_d.p : A->par64_db.p, *D1 = (sweeps % 2 == 0) ? A->par64_db.p : A->par64_d.p;
+    // the final iterate lands in par64_a / par64_d, its strict triangles in the Jacobi arrays
     if (A->par_maxw <= 8)
-        k_parilu_init_sell<double, 8><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->par64_v.p,
-                                                                    A->par_rinfo.p, A->par_dpos.p, L0, D0);
+        parilu_sweeps<double, 8>(ctx, A, sweeps, A->par64_v.p, A->par64_a.p, A->par64_b.p, A->par64_d.p, A->par64_db.p,
+                                 A->ell64_vl.p, A->ell64_vu.p);
     else
-        k_parilu_init_sell<double, 16><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->sell_col.p, A->par64_v.p,
-                                                                     A->par_rinfo.p, A->par_dpos.p, L0, D0);
-    for (int k = 0; k < sweeps; ++k) {
-        if (A->par_maxw <= 8)
-            k_parilu_sweep_sell<double, 8><<<grid, 256, 0, ctx->stream>>>(
-                n, A->sell_ptr.p, A->sell_col.p, A->par64_v.p, A->par_rinfo.p, A->par_uptr.p, A->par_usrc.p,
-                A->par_ucode.p, L0, D0, L1, D1);
-        else
-            k_parilu_sweep_sell<double, 16><<<grid, 256, 0, ctx->stream>>>(
-                n, A->sell_ptr.p, A->sell_col.p, A->par64_v.p, A->par_rinfo.p, A->par_uptr.p, A->par_usrc.p,
-                A->par_ucode.p, L0, D0, L1, D1);
-        std::swap(L0, L1);
-        std::swap(D0, D1);
-    }
-    k_parilu_sell_to_ell<double><<<grid, 256, 0, ctx->stream>>>(n, A->sell_ptr.p, A->par_rinfo.p, A->par64_a.p,
-                                                                A->ell_sl.p, A->ell_su.p, A->ell64_vl.p, A->ell64_vu.p);
+        parilu_sweeps<double, 16>(ctx, A, sweeps, A->par64_v.p, A->par64_a.p, A->par64_b.p, A->par64_d.p,
+                                  A->par64_db.p, A->ell64_vl.p, A->ell64_vu.p);
     FG_HIP(hipGetLastError());
     A->par64_factored = true;
     return FEMGPU_OK;

@@ -608,6 +608,7 @@ int build_stream_plan(femgpu_ctx *ctx, int n, const int *IA, const int *JA, cons
 {
     P.ok = false;
     P.n = n;
+    P.K = P.W = P.L = 0;
     if (n == 0) return FEMGPU_OK;
     auto ent_begin = [&](int row) { return upper ? diag[row] + 1 : IA[row]; };
     auto ent_end = [&](int row) { return upper ? IA[row + 1] : diag[row]; };
