@@ -66,6 +66,38 @@ def test_server_results_alone_in_the_mix_and_fp64(reqs, small_solver):
     dev.close()
 
 
+def test_server_matches_oracle(reqs, oracle):
+    """The served answers against the CPU oracle's own steady pipeline on the same requests (the
+    reference's numbering, lm, SYMRCM CSR and SUPG assembly, restated): the oracle's operator and load
+    applied to each served solution leave a relative residual within the serving bar (1e-8, plus
+    the difference of summation order), and the oracle's FP64 ILU(0) + pgmres solution agrees within
+    what that residual allows on these convection-dominated systems."""
+    sub = [r for r in reqs if len(r["x"]) <= 2500][:4]
+    opts = serving.server_opts(executors=2, prep_threads=2, small_rows=5000, batch_rows=100000, max_batch=2)
+    sols, infos, _, _, _ = serving.run_server(sub, opts)
+    for r, u, info in zip(sub, sols, infos):
+        t = r["type"]
+        pb = oracle.make_problem(k=r["pb"].k, bx=tuple(r["pb"].bx), by=tuple(r["pb"].by))
+        invP0, neq = oracle.number_unknowns(t)
+        lm = oracle.fill_lm(r["ien"], t, invP0, neq)
+        c = oracle.csr_init(lm, 3, neq, reorder="SYMRCM", type_=t, invP=invP0)
+        Ke, F = oracle.build_transport_supg(r["x"], r["y"], t, c["invP"], r["ien"], r["ubc"], np.zeros(neq), pb, neq)
+        AA = oracle.csr_assemble(c["scheme"], Ke, c["nnz"])
+        U = np.zeros(neq)
+        U[c["invP"][t == 1]] = u[t == 1]
+        res = np.linalg.norm(F[:neq] - oracle.csrmv(c["IA"], c["JA"], AA, U)) / np.linalg.norm(F[:neq])
+        LU, Dinv, rc = oracle.ilu0_factor(c["IA"], c["JA"], AA)
+        assert rc == 0
+        B = oracle.ilu0_apply(c["IA"], c["JA"], LU, Dinv, F[:neq])
+        S = oracle.make_system(0, neq, IA=c["IA"], JA=c["JA"], AA=AA, ilu=(c["IA"], c["JA"], LU, Dinv))
+        Uo = oracle.pgmres(S, B, kmax=30, lmax=2000, tol=1e-12)[0][:neq]
+        print(f"request {r['index']} ({len(r['x'])} points, k {r['pb'].k:.2e}): residual in the oracle's "
+              f"system {res:.2e} (served {info.true_rel_residual:.2e}), vs the oracle's solve {rel(U, Uo):.2e}")
+        assert info.status == 0 and res <= 2e-8
+        assert rel(U, Uo) < 1e-5
+        assert np.array_equal(u[t == 0], r["ubc"][t == 0])
+
+
 def test_server_autotunes_batch_size(reqs):
     many = serving.make_requests(48, 11, 200, 1500, rate=1e4)
     opts = serving.server_opts(executors=1, prep_threads=4, small_rows=5000, batch_rows=200000, max_batch=0,
