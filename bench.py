@@ -609,6 +609,9 @@ def main():
                     achieved=round(ach, 2), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 5),
                     traffic=traffic, bytes_per_launch=bytes_up, column_bytes=cbj,
                     us_per_launch=round(t_up * 1e6, 3),
+                    # the same launch priced at the int32-column figure (8*nnz_U + 16*n) of the
+                    # previous layout: what the bytes would be without the 16-bit distances
+                    frac_int32_columns=round((8 * nnz_u + 16 * n) / t_up / 1e9 / HBM_PEAK_GBS, 5),
                     frac_is=f"back-to-back launches on resident operands, as inside the solver loop (the "
                             f"{bytes_up / 1e6:.0f} MB working set fits the 256 MB last-level cache)",
                     achieved_cold=round(ach_cold, 2), frac_cold=round(ach_cold / HBM_PEAK_GBS, 5),
