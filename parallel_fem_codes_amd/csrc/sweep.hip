@@ -421,6 +421,9 @@ static StreamArgs stream_args(const StreamSweep &P)
     return a;
 }
 
+template <typename T>
+constexpr int kD16 = sizeof(T) == 4 ? 4 : 2;
+
 template <typename T, bool UPPER>
 static int stream_launch(femgpu_ctx *ctx, const StreamSweep &P, const T *val, const T *dinv, const T *b, T *x,
                          const int *skip)
@@ -435,10 +438,12 @@ static int stream_launch(femgpu_ctx *ctx, const StreamSweep &P, const T *val, co
         k_sweep_stream<T, UPPER, 8, 4, false><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
     else if (P.K == 8)
         k_sweep_stream<T, UPPER, 8, 4, true><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
+    // 16-entry rows: 4 steps of lookahead fit the registers in FP32 (6.6 against 7.4 ms per apply on
+    // the 513 x 513 cavity), 2 in FP64
     else if (P.L == 0)
-        k_sweep_stream<T, UPPER, 16, 2, false><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
+        k_sweep_stream<T, UPPER, 16, kD16<T>, false><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
     else
-        k_sweep_stream<T, UPPER, 16, 2, true><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
+        k_sweep_stream<T, UPPER, 16, kD16<T>, true><<<1, kStreamThreads, lds, ctx->stream>>>(a, val, dinv, b, x, skip);
     FG_HIP(hipGetLastError());
     return FEMGPU_OK;
 }
