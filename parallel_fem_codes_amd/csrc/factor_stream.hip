@@ -26,7 +26,6 @@ struct FactorArgs {
     const int *stepq;
     const unsigned short *slot;   // lower sweep plan: ring slots of the L entries (4 per position)
     const uint4 *codes;
-    const int4 *meta;
     const double *aval;
     double *out, *dinv, *sink;
     int *err;
@@ -52,9 +51,8 @@ __device__ __forceinline__ void factor_fetch(FactorStage &g, const FactorArgs &A
 #pragma unroll
     for (int e = 0; e < kFactorRow; ++e) g.a[e] = A.aval[static_cast<size_t>(e) * A.n + qq];
     g.codes = A.codes[qq];
-    const int4 m = A.meta[qq];
-    g.len = m.y;
-    g.nL = m.z;
+    g.len = static_cast<int>(g.codes.x & 15u);        // bits 0-7 (the pairs of entry 0, which has none):
+    g.nL = static_cast<int>((g.codes.x >> 4) & 15u);  // the row's length and L count
     const float2 sl = *reinterpret_cast<const float2 *>(A.slot + static_cast<size_t>(qq) * 4);
     __builtin_memcpy(g.s, &sl, 8);
     g.q = q;
@@ -197,7 +195,6 @@ int launch_stream_factor(femgpu_ctx *ctx, femgpu_csr *A)
     a.stepq = P.stepq.p;
     a.slot = P.slot.p;
     a.codes = F.codes.p;
-    a.meta = F.meta.p;
     a.aval = F.aval.p;
     a.out = F.out.p;
     a.dinv = F.dinv.p;

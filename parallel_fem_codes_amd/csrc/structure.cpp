@@ -807,6 +807,8 @@ int build_stream_factor(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int
                 const int bit = 4 * (4 * e + t);
                 w[bit / 32] = (w[bit / 32] & ~(15u << (bit % 32))) | (code << (bit % 32));
             }
+        // entry 0 has no (entry, L entry) pairs: its 16 code bits carry the row's length and L count
+        w[0] = (w[0] & ~0xffu) | static_cast<unsigned>(len) | (static_cast<unsigned>(nL) << 4);
         codes[q] = make_uint4(w[0], w[1], w[2], w[3]);
         meta[q] = make_int4(p0, len, nL, i);
     }
