@@ -7,7 +7,9 @@ so these stay within seconds of host time):
     matrices equals the CSR product to 1e-13;
   * the 513 x 513 Navier-Stokes pattern (n = 785 410, rows of up to 11 strict entries per triangle,
     some results read beyond the streamed sweep's ring): ILU(0) factors and both sweeps bit-identical
-    to the oracle."""
+    to the oracle;
+  * the 801 x 401 Euler operator (640 000 12x12 elements): element matrices, residual, BlockDiag and
+    the FP64 EBE product bit-identical to the oracle."""
 import numpy as np
 import pytest
 
@@ -129,3 +131,47 @@ def test_navier_stokes_pattern_513_ilu(dev, torch_cuda, oracle):
     z = torch.empty(n, dtype=torch.float64, device="cuda")
     A.ilu0_apply(T(torch, rhs), z)
     assert np.array_equal(H(z), oracle.ilu0_apply(IA, JA, LUo, Dio, rhs))
+
+
+def test_euler_obliquo_801x401(dev, torch_cuda, oracle):
+    """The Euler workload's operator at full size (801 x 401 nodes, 640 000 elements, 4 unknowns per
+    node) on a perturbed OBLIQUO state: the 12x12 element matrices and residual bit-identical to the
+    oracle, the nodal BlockDiag preconditioner (blocks summed in element order, cofactor inverses)
+    and the FP64 EBE product bit-identical to the oracle's, the FP32 product within FP32 rounding."""
+    torch = torch_cuda
+    P = fem.obliquo_problem(800, 400)
+    neq, nn = P["neq"], len(P["x"])
+    rng = np.random.default_rng(31)
+    u = P["u0"] * (1.0 + 1e-2 * rng.uniform(-1, 1, neq))
+    du = 1e-3 * rng.standard_normal(neq)
+    prm = dict(gamma=1.4, Mach=2.0, alpha=0.5, dt=P["dt"], tolerance=1e-8, dimensionless=0, shock=1)
+    U4, dU4 = oracle.euler_nodal_state(P["type4"], P["invP"], u, du, P["presc"])
+    Ae, R = oracle.build_euler_supg(P["x"], P["y"], P["ien"], P["lm"], U4, dU4, oracle.euler_params(**prm), neq)
+    print(f"801x401 OBLIQUO: neq {neq}, {len(P['ien']) // 3} elements", flush=True)
+    M = fem.EulerMesh(dev, P["x"], P["y"], P["type4"], P["invP"], P["presc"], P["ien"], P["lm"], neq)
+    Aeg = torch.empty(len(Ae), dtype=torch.float64, device="cuda")
+    Rg = torch.empty(neq + 1, dtype=torch.float64, device="cuda")
+    M.Build_M_K_F_SUPG(fem.make_euler_problem(**prm), T(torch, u), T(torch, du), Rg, Ae=Aeg)
+    dev.sync()
+    assert np.array_equal(H(Rg), R)
+    assert np.array_equal(H(Aeg), Ae)
+    # BlockDiag: setup and apply against the oracle's
+    B = fem.BlockDiag(dev, nn, P["ien"], 4, P["type4"], P["invP"], neq)
+    B.setup(Aeg)
+    inv = oracle.blockdiag_setup(P["ien"], Ae, nn, 4)
+    ids = oracle.blockdiag_ids(P["type4"], P["invP"], 4, neq)
+    p = rng.standard_normal(neq)
+    z = torch.empty(neq, dtype=torch.float64, device="cuda")
+    B.apply(T(torch, p), z)
+    assert np.array_equal(H(z), oracle.blockdiag_apply(ids, inv, p, neq, 4))
+    # EBE products
+    E = fem.EbeMatrix(dev, P["lm"], 12, neq)
+    E.set_values(Aeg)
+    del Aeg
+    q = torch.empty(neq, dtype=torch.float64, device="cuda")
+    E.ebemv(T(torch, p), q)
+    ref = oracle.ebemv_block(P["lm"], Ae, p, neq, 12)
+    assert np.array_equal(H(q), ref)
+    q32 = torch.empty(neq, dtype=torch.float32, device="cuda")
+    E.ebemv(T(torch, p).float(), q32)
+    assert rel(H(q32).astype(np.float64), ref) < 1e-5

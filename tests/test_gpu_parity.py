@@ -1292,7 +1292,8 @@ def test_euler_predictor_old_matches_oracle(dev, torch_cuda, oracle):
     steps on the device (femgpu_eu_predictor_old_step) against the same loop over the oracle.
     FP64 path: equal multicorrection counts, Arnoldi steps within +-2 per solve, states within 1e-9
     relative L2 after every step. Mixed path (FP32 EBE + FP32 BlockDiag inner GMRES in FP64
-    refinement, each solve FP64-verified to 1e-8): states within 1e-6 of the oracle's."""
+    refinement, each solve FP64-verified to 1e-8, with MGS and with CGS in the inner solver): states
+    within 1e-6 of the oracle's."""
     torch = torch_cuda
     P = fem.obliquo_problem(24, 12)
     prm = dict(gamma=1.4, Mach=2.0, alpha=0.5, dt=P["dt"], tolerance=1e-8, dimensionless=0, shock=1)
@@ -1303,20 +1304,22 @@ def test_euler_predictor_old_matches_oracle(dev, torch_cuda, oracle):
     E = fem.EbeMatrix(dev, P["lm"], 12, P["neq"])
     B = fem.BlockDiag(dev, nn, P["ien"], 4, P["type4"], P["invP"], P["neq"])
     pb = fem.make_euler_problem(**prm)
-    for solver, bar in (("fp64", 1e-9), ("mixed", 1e-6)):
+    for leg, bar in (("fp64", 1e-9), ("mixed", 1e-6), ("mixed_cgs", 1e-6)):
+        solver = "fp64" if leg == "fp64" else "mixed"
         opts = fem.driver_opts(solver, kmax=kmax, lmax=lmax, tol=tol, nonlinear_tol=nltol, nonlinear_maxiter=maxit,
-                               inner_kmax=kmax, inner_tol=1e-4)
+                               inner_kmax=kmax, inner_tol=1e-4,
+                               orthogonalization="CGS" if leg == "mixed_cgs" else "MGS")
         u = T(torch, P["u0"])
         a = torch.zeros_like(u)
         for k in range(steps):
             info = fem.Predictor_Old_step_Euler(M, E, B, pb, u, a, opts)
             i_ref, its_ref, u_ref, a_ref = ref[k]
-            assert info.status in (0, 1), (solver, k, info.status)
+            assert info.status in (0, 1), (leg, k, info.status)
             if solver == "fp64":
                 assert info.iterations == i_ref, (k, info.iterations, i_ref)
                 for j in range(i_ref):
                     assert abs(info.iters[j] - its_ref[j]) <= 2, (k, j, info.iters[j], its_ref[j])
             e = rel(H(u), u_ref)
-            print(f"euler {solver} step {k + 1}: multicorrections {info.iterations} (oracle {i_ref}), "
+            print(f"euler {leg} step {k + 1}: multicorrections {info.iterations} (oracle {i_ref}), "
                   f"rel err u {e:.2e}")
-            assert e < bar, (solver, k, e)
+            assert e < bar, (leg, k, e)
