@@ -207,11 +207,9 @@ struct TransientElem {
     double Ae[3][3], lift[3][3], Fe[3];
 };
 
-__device__ __forceinline__ void transient_elem(const ElemIn &in, int e, TransientElem &T)
+__device__ __forceinline__ void transient_elem_nodes(const ElemIn &in, const ElemNodes &N, TransientElem &T)
 {
 #pragma clang fp contract(off)
-    ElemNodes N;
-    load_nodes(in, e, true, N);
     const double *ue = N.u, *due = N.du;
     SupgElem E;
     supg_geometry(E, N.X, N.Y, in.pb);
@@ -232,6 +230,13 @@ __device__ __forceinline__ void transient_elem(const ElemIn &in, int e, Transien
             T.Ae[a][b] = me + in.alpha_dt * E.ke[a][b];
             T.lift[a][b] = me * due[b] + E.ke[a][b] * ue[b];
         }
+}
+
+__device__ __forceinline__ void transient_elem(const ElemIn &in, int e, TransientElem &T)
+{
+    ElemNodes N;
+    load_nodes(in, e, true, N);
+    transient_elem_nodes(in, N, T);
 }
 
 // Transient element kernel: Ae (9) and, per element row a, the record {fe_a, lift_a0, lift_a1,
@@ -275,8 +280,8 @@ __global__ __launch_bounds__(256) void k_elem_transient(int nel, ElemIn in, doub
 
 // Fused transient assembly: a block of kFzRows rows (a compact patch of the mesh) first evaluates
 // every element that touches its rows, once, into LDS -- for each vertex row a of the element that
-// belongs to the block, the slot {Ae[a][0..2], Fe_a, lift[a][0..2]} -- then each thread gathers its
-// row from LDS: the CSR values of the row (each entry summed over the row's incidences in element
+// belongs to the block, the slot {Ae[a][0..2], Fe_a, lift[a][0..2]} -- then its threads gather the
+// rows from LDS: the CSR values of each row (each entry summed over the row's incidences in element
 // order, as csr_assembly.c adds them) and the residual (Fe_a, then the three coupling terms, in
 // element order, as Build_M_K_R_SUPG.c:173-182). The sums are those of the element-array path, so
 // the results are bit-identical to it, and no element array is written or read back: elements on
@@ -285,17 +290,50 @@ __global__ __launch_bounds__(256) void k_elem_transient(int nel, ElemIn in, doub
 // on the 1001 x 1001 mesh (smaller patches re-evaluate more border elements, but a block's LDS
 // shrinks with its rows and more blocks share a CU). The element evaluation is latency-bound, so
 // each 64-row block runs 256 threads for it (126 us with 64, 109 us with 128, 103 us with 256).
+// Element entries that carry their vertices' equation numbers (two dependent loads fewer per
+// element) and incidence codes loaded ahead of the evaluation: 86 us; the gather spread over all
+// four waves (four threads per row) instead of one thread per row: 84 us. 128 VGPRs hold four
+// waves per SIMD; forcing five spills registers and doubles the time (169 us).
 constexpr int kFzRows = 64;        // rows per block
-constexpr int kFzThreads = 256;    // threads per block: all evaluate elements, the first kFzRows gather rows
+constexpr int kFzThreads = 256;    // threads per block: all evaluate elements, then gather (4 per row)
 constexpr int kFzMaxSlots = 576;   // LDS slots per block: 576 x 7 doubles = 31.5 KB
 constexpr int kFzVals = 7;
+constexpr int kFzMaxInc = 16;      // incidences per row (elements per vertex) the fused kernel takes
 
-template <int MAXLEN>
+// An element of a fused-assembly block, 32 bytes: ent[0] = {v0, v1, v2, q0}, ent[1] = {q1, q2,
+// s0 | s1 << 16, s2 | 0xffff << 16} with the vertices v, their equation numbers q (-1: a Dirichlet
+// vertex) and the LDS slot s of each vertex row that belongs to the block (0xffff: none). Storing
+// the vertices' equation numbers (fixed for a mesh) lets coordinates, boundary values and state be
+// loaded together right after the entry, instead of behind connectivity, marks and numbering.
+__device__ __forceinline__ void load_nodes_fz(const ElemIn &in, int4 A, int4 B, ElemNodes &N)
+{
+    const int v[3] = {A.x, A.y, A.z}, q[3] = {A.w, B.x, B.y};
+    double uf[3], duf[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        N.v[a] = v[a];
+        N.t[a] = q[a] >= 0 ? 1 : 0;
+        N.X[a] = in.x[v[a]];
+        N.Y[a] = in.y[v[a]];
+        N.ub[a] = in.ubc[v[a]];
+        const int qq = q[a] >= 0 ? q[a] : 0;
+        uf[a] = in.u[qq];
+        duf[a] = in.du[qq];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        N.u[a] = N.t[a] == 0 ? N.ub[a] : uf[a];
+        N.du[a] = N.t[a] == 0 ? 0.0 : duf[a];
+    }
+}
+
+// ki: incidence rows per block in inc (the largest row's incidences, at most NI); the gathering
+// threads load their row record and all their incidence codes before the element evaluation, so
+// those loads overlap it instead of following the barrier one by one.
+template <int MAXLEN, int NI>
 __global__ __launch_bounds__(kFzThreads) void k_transient_fused(int nblocks, ElemIn in, const int *__restrict__ eptr,
-                                                             const int *__restrict__ elem,
-                                                             const ushort4 *__restrict__ eq,
-                                                             const int *__restrict__ iptr,
-                                                             const unsigned *__restrict__ inc,
+                                                             const int4 *__restrict__ ent,
+                                                             const unsigned *__restrict__ inc, int ki,
                                                              const int4 *__restrict__ rinfo, int nnz,
                                                              double *__restrict__ AA, double *__restrict__ R)
 {
@@ -308,12 +346,24 @@ __global__ __launch_bounds__(kFzThreads) void k_transient_fused(int nblocks, Ele
         AA[nnz] = 0.0;        // csr_assembly's dummy slot
         R[in.neq] = 0.0;      // Build_M_K_R_SUPG.c:183
     }
+    // the gathering thread's row record and incidence codes (all four waves gather: thread t takes
+    // row t % kFzRows), loaded before the element evaluation so that their latency overlaps it
+    const int row = t & (kFzRows - 1), part = t / kFzRows;
+    const int4 ri = rinfo[w * kFzRows + row];   // {row, first CSR slot, row length, incidences}
+    unsigned cc[NI];
+    {
+        const unsigned *ip = inc + static_cast<size_t>(w) * ki * kFzRows + row;
+#pragma unroll
+        for (int j = 0; j < NI; ++j) cc[j] = j < ki ? ip[j * kFzRows] : 0xffffffffu;
+    }
     for (int le = eptr[w] + t; le < eptr[w + 1]; le += kFzThreads) {
-        const int e = elem[le];
-        const ushort4 q = eq[le];
+        const int4 A = ent[2 * le], B = ent[2 * le + 1];
+        ElemNodes N;
+        load_nodes_fz(in, A, B, N);
         TransientElem T;
-        transient_elem(in, e, T);
-        const unsigned short qa[3] = {q.x, q.y, q.z};
+        transient_elem_nodes(in, N, T);
+        const unsigned qa[3] = {static_cast<unsigned>(B.z) & 0xffffu, static_cast<unsigned>(B.z) >> 16,
+                                static_cast<unsigned>(B.w) & 0xffffu};
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             if (qa[a] != 0xffffu) {
@@ -328,35 +378,42 @@ __global__ __launch_bounds__(kFzThreads) void k_transient_fused(int nblocks, Ele
             }
     }
     __syncthreads();
-    if (t >= kFzRows) return;
-    const int4 ri = rinfo[w * kFzRows + t];   // {row, first CSR slot, row length, incidences}
     if (ri.x < 0) return;
-    double acc[MAXLEN];
+    // Thread part p sums the row's CSR entries p, p + 4, ... (each over the row's incidences in
+    // element order; an element row adds to three distinct entries, so at most one of its three
+    // values matches an entry, and adding +0.0 for no match leaves a sum that starts at +0.0
+    // bit-identical) and part 3 also the residual.
+    constexpr int NE = MAXLEN / 4;
+    double acc[NE];
 #pragma unroll
-    for (int k = 0; k < MAXLEN; ++k) acc[k] = 0.0;
+    for (int e = 0; e < NE; ++e) acc[e] = 0.0;
     double r = 0.0;
-    const unsigned *ip = inc + iptr[w] + t;
-    for (int j = 0; j < ri.w; ++j) {
-        const unsigned c = ip[j * kFzRows];
-        const double *s = slot + kFzVals * (c & 0xffffu);
 #pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            const unsigned k = (c >> (16 + 4 * b)) & 15u;   // 15: a Dirichlet column
-            const double v = s[b];
+    for (int j = 0; j < NI; ++j) {
+        if (j < ri.w) {
+            const unsigned c = cc[j];
+            const double *s = slot + kFzVals * (c & 0xffffu);
+            const double s0 = s[0], s1 = s[1], s2 = s[2];
+            const unsigned k0 = (c >> 16) & 15u, k1 = (c >> 20) & 15u, k2 = (c >> 24) & 15u;   // 15: Dirichlet
 #pragma unroll
-            for (int kk = 0; kk < MAXLEN; ++kk)
-                if (static_cast<unsigned>(kk) == k) acc[kk] += v;
+            for (int e = 0; e < NE; ++e) {
+                const unsigned kk = static_cast<unsigned>(part + 4 * e);
+                const double v = k0 == kk ? s0 : k1 == kk ? s1 : k2 == kk ? s2 : 0.0;
+                acc[e] += v;
+            }
+            if (part == 3) {
+                r += s[3];
+                r -= s[4];
+                r -= s[5];
+                r -= s[6];
+            }
         }
-        r += s[3];
-        r -= s[4];
-        r -= s[5];
-        r -= s[6];
     }
     double *out = AA + ri.y;
 #pragma unroll
-    for (int kk = 0; kk < MAXLEN; ++kk)
-        if (kk < ri.z) out[kk] = acc[kk];
-    R[ri.x] = r;
+    for (int e = 0; e < NE; ++e)
+        if (part + 4 * e < ri.z) out[part + 4 * e] = acc[e];
+    if (part == 3) R[ri.x] = r;
 }
 
 // RHS gathers. rhs_src entries are e*3+a in element order.
@@ -565,12 +622,13 @@ static int build_fused_plan(femgpu_mesh *M)
                     rlast[i] = std::max(rlast[i], sl);
                 }
             }
-    int maxlen = 0;
+    int maxlen = 0, ki = 0;
     for (int i = 0; i < neq; ++i) {
-        if (rlast[i] < 0 || rptr[i + 1] - rptr[i] > kFzMaxSlots) return FEMGPU_OK;
+        if (rlast[i] < 0) return FEMGPU_OK;
         maxlen = std::max(maxlen, rlast[i] - rfirst[i] + 1);
+        ki = std::max(ki, rptr[i + 1] - rptr[i]);
     }
-    if (maxlen > 15) return FEMGPU_OK;
+    if (maxlen > 15 || ki > kFzMaxInc) return FEMGPU_OK;
     // blocks: breadth-first patches over the row graph
     std::vector<int> blk(neq, -1), order, bptr{0}, queue;
     order.reserve(neq);
@@ -603,15 +661,16 @@ static int build_fused_plan(femgpu_mesh *M)
         bptr.push_back(static_cast<int>(order.size()));
     }
     const int nb = static_cast<int>(bptr.size()) - 1;
-    std::vector<int> eptr{0}, elem, iptr{0};
-    std::vector<ushort4> eq;
-    std::vector<unsigned> inc;
+    std::vector<int> eptr{0};
+    std::vector<int4> ent;
+    // every block's incidence codes: ki rows of kFzRows (padding 0xffffffff)
+    std::vector<unsigned> inc(static_cast<size_t>(nb) * ki * kFzRows, 0xffffffffu);
     std::vector<int4> rinfo(static_cast<size_t>(nb) * kFzRows, make_int4(-1, 0, 0, 0));
     std::vector<int> qof(static_cast<size_t>(nel) * 3, -1), touched;
     int slots_max = 0;
     for (int b = 0; b < nb; ++b) {
         const int r0 = bptr[b], nr = bptr[b + 1] - r0;
-        int nslot = 0, kmax = 0;
+        int nslot = 0;
         touched.clear();
         for (int t = 0; t < nr; ++t) {
             const int i = order[r0 + t];
@@ -619,24 +678,26 @@ static int build_fused_plan(femgpu_mesh *M)
                 qof[rinc[q]] = nslot++;
                 touched.push_back(rinc[q] / 3);
             }
-            kmax = std::max(kmax, rptr[i + 1] - rptr[i]);
             rinfo[static_cast<size_t>(b) * kFzRows + t] = make_int4(i, rfirst[i], rlast[i] - rfirst[i] + 1, rptr[i + 1] - rptr[i]);
         }
         slots_max = std::max(slots_max, nslot);
         std::sort(touched.begin(), touched.end());
         touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
         for (int e : touched) {
-            elem.push_back(e);
-            unsigned short qa[3];
+            int v[3], qn[3];
+            unsigned sl[3];
             for (int a = 0; a < 3; ++a) {
+                v[a] = ien[3 * static_cast<size_t>(e) + a];
+                qn[a] = type[v[a]] == 1 ? invP[v[a]] : -1;
                 const int q = qof[3 * static_cast<size_t>(e) + a];
-                qa[a] = q >= 0 ? static_cast<unsigned short>(q) : 0xffffu;
+                sl[a] = q >= 0 ? static_cast<unsigned>(q) : 0xffffu;
             }
-            eq.push_back(make_ushort4(qa[0], qa[1], qa[2], 0));
+            ent.push_back(make_int4(v[0], v[1], v[2], qn[0]));
+            ent.push_back(make_int4(qn[1], qn[2], static_cast<int>(sl[0] | (sl[1] << 16)),
+                                    static_cast<int>(sl[2] | 0xffff0000u)));
         }
-        eptr.push_back(static_cast<int>(elem.size()));
-        const size_t base = inc.size();
-        inc.resize(base + static_cast<size_t>(kmax) * kFzRows, 0xffffffffu);
+        eptr.push_back(static_cast<int>(ent.size() / 2));
+        const size_t base = static_cast<size_t>(b) * ki * kFzRows;
         for (int t = 0; t < nr; ++t) {
             const int i = order[r0 + t];
             for (int q = rptr[i]; q < rptr[i + 1]; ++q) {
@@ -650,23 +711,22 @@ static int build_fused_plan(femgpu_mesh *M)
                 inc[base + static_cast<size_t>(q - rptr[i]) * kFzRows + t] = code;
             }
         }
-        iptr.push_back(static_cast<int>(inc.size()));
         for (int t = 0; t < nr; ++t) {   // qof back to -1 for the next block
             const int i = order[r0 + t];
             for (int q = rptr[i]; q < rptr[i + 1]; ++q) qof[rinc[q]] = -1;
         }
     }
     if (inc.empty()) inc.push_back(0xffffffffu);
+    if (ent.empty()) ent.push_back(make_int4(0, 0, 0, 0));
     FG_TRY(M->fz_eptr.upload(eptr.data(), eptr.size(), st));
-    FG_TRY(M->fz_elem.upload(elem.data(), elem.size(), st));
-    FG_TRY(M->fz_eq.upload(eq.data(), eq.size(), st));
-    FG_TRY(M->fz_iptr.upload(iptr.data(), iptr.size(), st));
+    FG_TRY(M->fz_ent.upload(ent.data(), ent.size(), st));
     FG_TRY(M->fz_inc.upload(inc.data(), inc.size(), st));
     FG_TRY(M->fz_row.upload(rinfo.data(), rinfo.size(), st));
     FG_HIP(hipStreamSynchronize(st));   // the host arrays end here
     M->fz_nblocks = nb;
     M->fz_slots = slots_max;
     M->fz_maxlen = maxlen;
+    M->fz_ki = ki;
     M->fz_state = 1;
     return FEMGPU_OK;
 }
@@ -820,14 +880,18 @@ extern "C" int femgpu_assemble_transport_transient(femgpu_mesh *M, const femgpu_
         if (M->fz_state == 1) {
             const size_t lds = sizeof(double) * kFzVals * static_cast<size_t>(std::max(M->fz_slots, 1));
             const int grid = xcd_grid(M->fz_nblocks);
-            if (M->fz_maxlen <= 8)
-                k_transient_fused<8><<<grid, kFzThreads, lds, ctx->stream>>>(
-                    M->fz_nblocks, in, M->fz_eptr.p, M->fz_elem.p, M->fz_eq.p, M->fz_iptr.p, M->fz_inc.p, M->fz_row.p,
-                    M->nnz, d_AA, d_R);
+            const int4 *ent = M->fz_ent.p, *ri = M->fz_row.p;
+            const unsigned *inc = M->fz_inc.p;
+            const int nb = M->fz_nblocks, ki = M->fz_ki;
+            if (M->fz_maxlen <= 8 && ki <= 8)
+                k_transient_fused<8, 8><<<grid, kFzThreads, lds, ctx->stream>>>(nb, in, M->fz_eptr.p, ent, inc, ki, ri,
+                                                                                M->nnz, d_AA, d_R);
+            else if (M->fz_maxlen <= 8)
+                k_transient_fused<8, 16><<<grid, kFzThreads, lds, ctx->stream>>>(nb, in, M->fz_eptr.p, ent, inc, ki, ri,
+                                                                                 M->nnz, d_AA, d_R);
             else
-                k_transient_fused<16><<<grid, kFzThreads, lds, ctx->stream>>>(
-                    M->fz_nblocks, in, M->fz_eptr.p, M->fz_elem.p, M->fz_eq.p, M->fz_iptr.p, M->fz_inc.p, M->fz_row.p,
-                    M->nnz, d_AA, d_R);
+                k_transient_fused<16, 16><<<grid, kFzThreads, lds, ctx->stream>>>(nb, in, M->fz_eptr.p, ent, inc, ki, ri,
+                                                                                  M->nnz, d_AA, d_R);
             FG_HIP(hipGetLastError());
             return FEMGPU_OK;
         }

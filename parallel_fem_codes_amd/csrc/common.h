@@ -276,13 +276,14 @@ struct femgpu_mesh {
     fg::DevBuf<int> mask;               // per element: Dirichlet-vertex bit mask
     fg::DevBuf<double> ke;              // element matrices when the caller passes none
     // Fused transient assembly (CSR values and residual with no element arrays in memory): rows in
-    // blocks of compact mesh patches; per block its elements (sorted, with the LDS slot of each
-    // vertex row in the block), per row thread its incidences (LDS slot | CSR position codes) and
-    // {row, first CSR slot, row length, incidences}. Built on first use; fz_state 0 unknown,
-    // 1 built, -1 not applicable (a row longer than 15 or a block over the LDS budget).
-    int fz_state = 0, fz_nblocks = 0, fz_slots = 0, fz_maxlen = 0;
-    fg::DevBuf<int> fz_eptr, fz_elem, fz_iptr;
-    fg::DevBuf<ushort4> fz_eq;
+    // blocks of compact mesh patches; per block its elements (sorted; two int4 each: vertices,
+    // their equation numbers and the LDS slot of each vertex row in the block), per row thread
+    // fz_ki incidence codes (LDS slot | CSR positions) and {row, first CSR slot, row length,
+    // incidences}. Built on first use; fz_state 0 unknown, 1 built, -1 not applicable (a row longer
+    // than 15 entries or with more than 16 incidences).
+    int fz_state = 0, fz_nblocks = 0, fz_slots = 0, fz_maxlen = 0, fz_ki = 0;
+    fg::DevBuf<int> fz_eptr;
+    fg::DevBuf<int4> fz_ent;
     fg::DevBuf<unsigned> fz_inc;
     fg::DevBuf<int4> fz_row;
 };
