@@ -291,13 +291,19 @@ __global__ __launch_bounds__(256) void k_elem_transient(int nel, ElemIn in, doub
 // shrinks with its rows and more blocks share a CU). The element evaluation is latency-bound, so
 // each 64-row block runs 256 threads for it (126 us with 64, 109 us with 128, 103 us with 256).
 // Element entries that carry their vertices' equation numbers (two dependent loads fewer per
-// element) and incidence codes loaded ahead of the evaluation: 86 us; the gather spread over all
-// four waves (four threads per row) instead of one thread per row: 84 us. 128 VGPRs hold four
-// waves per SIMD; forcing five spills registers and doubles the time (169 us).
-constexpr int kFzRows = 64;        // rows per block
-constexpr int kFzThreads = 256;    // threads per block: all evaluate elements, then gather (4 per row)
-constexpr int kFzMaxSlots = 576;   // LDS slots per block: 576 x 7 doubles = 31.5 KB
+// element) and incidence codes loaded ahead of the evaluation: 86 us; the gather spread over the
+// block's waves (several threads per row) instead of one thread per row: 84 us. 120 VGPRs hold
+// four waves per SIMD; forcing five spills registers and doubles the time (169 us). The element
+// arithmetic then keeps the SIMDs busy most of the time, so the patch size is set by thread
+// utilisation and border re-evaluation: a 64-row patch touches about 161 elements (1.25
+// evaluations per element), a 96-row one about 233 (1.20), which 256 threads take in one pass
+// (64 rows x 192 threads 78.7 us, 96 x 256 74.4 us, 128 x 320 95.3 us, 96 x 320 115.1 us).
+constexpr int kFzRows = 96;        // rows per block
+constexpr int kFzThreads = 256;    // threads per block: all evaluate elements, then 2 per row gather
+constexpr int kFzMaxSlots = 864;   // LDS slots per block: 864 x 7 doubles = 47.3 KB at most
 constexpr int kFzVals = 7;
+constexpr int kFzParts = kFzThreads / kFzRows;   // gathering threads per row
+static_assert(kFzParts >= 1, "a fused-assembly block needs at least one thread per row");
 constexpr int kFzMaxInc = 16;      // incidences per row (elements per vertex) the fused kernel takes
 
 // An element of a fused-assembly block, 32 bytes: ent[0] = {v0, v1, v2, q0}, ent[1] = {q1, q2,
@@ -348,7 +354,7 @@ __global__ __launch_bounds__(kFzThreads) void k_transient_fused(int nblocks, Ele
     }
     // the gathering thread's row record and incidence codes (all four waves gather: thread t takes
     // row t % kFzRows), loaded before the element evaluation so that their latency overlaps it
-    const int row = t & (kFzRows - 1), part = t / kFzRows;
+    const int row = t % kFzRows, part = t / kFzRows;
     const int4 ri = rinfo[w * kFzRows + row];   // {row, first CSR slot, row length, incidences}
     unsigned cc[NI];
     {
@@ -378,12 +384,12 @@ __global__ __launch_bounds__(kFzThreads) void k_transient_fused(int nblocks, Ele
             }
     }
     __syncthreads();
-    if (ri.x < 0) return;
-    // Thread part p sums the row's CSR entries p, p + 4, ... (each over the row's incidences in
-    // element order; an element row adds to three distinct entries, so at most one of its three
+    if (ri.x < 0 || part >= kFzParts) return;   // threads past kFzParts * kFzRows only evaluate
+    // Thread part p sums the row's CSR entries p, p + kFzParts, ... (each over the row's incidences
+    // in element order; an element row adds to three distinct entries, so at most one of its three
     // values matches an entry, and adding +0.0 for no match leaves a sum that starts at +0.0
-    // bit-identical) and part 3 also the residual.
-    constexpr int NE = MAXLEN / 4;
+    // bit-identical), the last part also the residual.
+    constexpr int NE = (MAXLEN + kFzParts - 1) / kFzParts;
     double acc[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) acc[e] = 0.0;
@@ -397,11 +403,11 @@ __global__ __launch_bounds__(kFzThreads) void k_transient_fused(int nblocks, Ele
             const unsigned k0 = (c >> 16) & 15u, k1 = (c >> 20) & 15u, k2 = (c >> 24) & 15u;   // 15: Dirichlet
 #pragma unroll
             for (int e = 0; e < NE; ++e) {
-                const unsigned kk = static_cast<unsigned>(part + 4 * e);
+                const unsigned kk = static_cast<unsigned>(part + kFzParts * e);
                 const double v = k0 == kk ? s0 : k1 == kk ? s1 : k2 == kk ? s2 : 0.0;
                 acc[e] += v;
             }
-            if (part == 3) {
+            if (part == kFzParts - 1) {
                 r += s[3];
                 r -= s[4];
                 r -= s[5];
@@ -412,8 +418,8 @@ __global__ __launch_bounds__(kFzThreads) void k_transient_fused(int nblocks, Ele
     double *out = AA + ri.y;
 #pragma unroll
     for (int e = 0; e < NE; ++e)
-        if (part + 4 * e < ri.z) out[part + 4 * e] = acc[e];
-    if (part == 3) R[ri.x] = r;
+        if (part + kFzParts * e < ri.z) out[part + kFzParts * e] = acc[e];
+    if (part == kFzParts - 1) R[ri.x] = r;
 }
 
 // RHS gathers. rhs_src entries are e*3+a in element order.
