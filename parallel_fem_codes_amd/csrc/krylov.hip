@@ -35,24 +35,43 @@ struct GDev {
 
 __device__ __forceinline__ double &Hat(const GDev &G, int r, int c) { return G.H[r * G.ld + c]; }
 
-// Givens update of column i and the convergence test (pgmres.c:118-159), one thread.
-__device__ void givens_column(const GDev &G, int i)
+constexpr int kMaxBasis = 150;   // largest GMRES basis (make_gdev)
+
+// Givens update of column i and the convergence test (pgmres.c:118-159), with h_{i+1,i} = hn.
+// Called by every thread of the block: the block stages the column, the earlier rotations and
+// e[i] in LDS with loads issued together, then one thread applies the rotations -- the operations
+// of a serial update in the same order -- and stores the results (a serial update in global memory
+// would wait for every read after a write to the same column).
+__device__ void givens_column(const GDev &G, int i, double hn)
 {
 #pragma clang fp contract(off)
-    for (int j = 0; j <= i - 1; ++j) {
-        const double h1 = G.c[j] * Hat(G, j, i) + G.s[j] * Hat(G, j + 1, i);
-        const double h2 = -G.s[j] * Hat(G, j, i) + G.c[j] * Hat(G, j + 1, i);
-        Hat(G, j, i) = h1;
-        Hat(G, j + 1, i) = h2;
+    __shared__ double col[kMaxBasis + 1], cs[kMaxBasis], sn[kMaxBasis], ei_s;
+    for (int k = threadIdx.x; k <= i; k += blockDim.x) col[k] = Hat(G, k, i);
+    for (int k = threadIdx.x; k < i; k += blockDim.x) {
+        cs[k] = G.c[k];
+        sn[k] = G.s[k];
     }
-    const double a = Hat(G, i, i), b = Hat(G, i + 1, i);
+    if (threadIdx.x == 0) ei_s = G.e[i];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    col[i + 1] = hn;
+    for (int j = 0; j <= i - 1; ++j) {
+        const double h1 = cs[j] * col[j] + sn[j] * col[j + 1];
+        const double h2 = -sn[j] * col[j] + cs[j] * col[j + 1];
+        col[j] = h1;
+        col[j + 1] = h2;
+    }
+    const double a = col[i], b = col[i + 1];
     const double r = sqrt(a * a + b * b);
-    G.c[i] = a / r;
-    G.s[i] = b / r;
+    const double ci = a / r, si = b / r;
+    G.c[i] = ci;
+    G.s[i] = si;
+    for (int k = 0; k < i; ++k) Hat(G, k, i) = col[k];
     Hat(G, i, i) = r;
     Hat(G, i + 1, i) = 0.0;
-    G.e[i + 1] = -G.s[i] * G.e[i];
-    G.e[i] = G.c[i] * G.e[i];
+    const double ei = ei_s;
+    G.e[i + 1] = -si * ei;
+    G.e[i] = ci * ei;
     const double rho = fabs(G.e[i + 1]);
     const int steps = i + 1;
     const int done = !(rho > G.scal[0]) || (steps >= G.kmax - 1);
@@ -179,13 +198,12 @@ __global__ __launch_bounds__(256) void k_mgs_finish(GDev G, int nb, int i)
     if (G.flags[0]) return;
     __shared__ double red[4];
     const double v = sum_partials(G.partials + ((i + 1) & 1) * kReduceBlocks, nb, red);
+    const double hn = sqrt(v);
     if (threadIdx.x == 0) {
-        const double hn = sqrt(v);
-        Hat(G, i + 1, i) = hn;
         G.hnorm[i + 1] = hn;
         G.invh[i + 1] = 1.0 / hn;
-        givens_column(G, i);
     }
+    givens_column(G, i, hn);
 }
 
 // Classical Gram-Schmidt for the FP32 inner solver: all coefficients h_{j,i} = <V[j], w> from one
@@ -297,13 +315,12 @@ __global__ __launch_bounds__(256) void k_cgs_finish(GDev G, int nb, int i, const
     if (G.flags[0]) return;
     __shared__ double red[4];
     const double v = sum_partials(part + (i + 1) * kCgsBlocks, nb, red);
+    const double hn = sqrt(v);
     if (threadIdx.x == 0) {
-        const double hn = sqrt(v);
-        Hat(G, i + 1, i) = hn;
         G.hnorm[i + 1] = hn;
         G.invh[i + 1] = 1.0 / hn;
-        givens_column(G, i);
     }
+    givens_column(G, i, hn);
 }
 
 // Start of a cycle from the residual vector w (already M^-1 (B - A X) or B itself):
@@ -355,6 +372,42 @@ __global__ __launch_bounds__(256) void k_cycle_start_finish(GDev G, int nb, int 
     }
 }
 
+// Back substitution H y = e over the first i + 1 columns (pgmres.c:159-170) into ys (LDS), the
+// same operations in the same order as the serial loop; called by every thread of the block. For
+// bases up to kStageBasis the block first stages the triangle and e in LDS with loads issued
+// together, so the serial loop of thread 0 does not wait on global memory per step.
+constexpr int kStageBasis = 32;
+
+__device__ void back_substitute(const GDev &G, int i, double *ys)
+{
+#pragma clang fp contract(off)
+    __shared__ double hs[kStageBasis * kStageBasis], es[kStageBasis];
+    if (i < kStageBasis) {
+        const int m = i + 1;
+        for (int q = threadIdx.x; q < m * m; q += blockDim.x) {
+            const int r = q / m, c = q - r * m;
+            if (c >= r) hs[q] = Hat(G, r, c);
+        }
+        for (int q = threadIdx.x; q < m; q += blockDim.x) es[q] = G.e[q];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            ys[i] = es[i] / hs[i * m + i];
+            for (int j = i - 1; j >= 0; --j) {
+                double soma = 0.0;
+                for (int k = j + 1; k <= i; ++k) soma = soma + hs[j * m + k] * ys[k];
+                ys[j] = (es[j] - soma) / hs[j * m + j];
+            }
+        }
+    } else if (threadIdx.x == 0) {
+        ys[i] = G.e[i] / Hat(G, i, i);
+        for (int j = i - 1; j >= 0; --j) {
+            double soma = 0.0;
+            for (int k = j + 1; k <= i; ++k) soma = soma + Hat(G, j, k) * ys[k];
+            ys[j] = (G.e[j] - soma) / Hat(G, j, j);
+        }
+    }
+}
+
 // Back substitution (pgmres.c:159-170) by every block into LDS, then X += scale * sum_j V[j] y[j]
 // in the reference's per-entry order (pgmres.c:173-175).
 template <typename T, typename TX>
@@ -362,20 +415,13 @@ __global__ __launch_bounds__(256) void k_cycle_finish(GDev G, int n, const T *__
                                                       TX *__restrict__ X, double scale, int assign)
 {
 #pragma clang fp contract(off)
-    __shared__ double ys[160];
+    __shared__ double ys[kMaxBasis + 1];
     const int i = G.flags[1] - 1;
-    if (threadIdx.x == 0 && i >= 0) {
-        ys[i] = G.e[i] / Hat(G, i, i);
-        for (int j = i - 1; j >= 0; --j) {
-            double soma = 0.0;
-            for (int k = j + 1; k <= i; ++k) soma = soma + Hat(G, j, k) * ys[k];
-            ys[j] = (G.e[j] - soma) / Hat(G, j, j);
-        }
-        if (blockIdx.x == 0)
-            for (int j = 0; j <= i; ++j) G.y[j] = ys[j];
-    }
-    __syncthreads();
     if (i < 0) return;
+    back_substitute(G, i, ys);
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        for (int j = 0; j <= i; ++j) G.y[j] = ys[j];
+    __syncthreads();
     for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
         // assign: X was not initialised (x0 = 0): the sums start from 0 instead of X
         if (scale == 1.0) {
@@ -485,18 +531,11 @@ __global__ __launch_bounds__(256) void k_cycle_finish_seg(GDev G, int n, const f
                                                           const double *__restrict__ rn)
 {
 #pragma clang fp contract(off)
-    __shared__ double ys[160];
+    __shared__ double ys[kMaxBasis + 1];
     const int i = G.flags[1] - 1;
-    if (threadIdx.x == 0 && i >= 0) {
-        ys[i] = G.e[i] / Hat(G, i, i);
-        for (int j = i - 1; j >= 0; --j) {
-            double soma = 0.0;
-            for (int k = j + 1; k <= i; ++k) soma = soma + Hat(G, j, k) * ys[k];
-            ys[j] = (G.e[j] - soma) / Hat(G, j, j);
-        }
-    }
-    __syncthreads();
     if (i < 0) return;
+    back_substitute(G, i, ys);
+    __syncthreads();
     for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
         double d = 0;
         for (int j = 0; j <= i; ++j) d = d + static_cast<double>(V[j * ldv + k]) * ys[j];
@@ -509,15 +548,24 @@ static int grid_for(int n) { return std::max(1, std::min(kReduceBlocks, (n + 102
 static int grid4_for(int n) { return std::max(1, std::min(kReduceBlocks, (n + 4095) / 4096)); }
 
 // ||x|| into d_out and the host-mapped slot of the context (slots 248..255 of the mapped doubles,
-// past the per-step residual words), then waits for the stream and returns it.
-static int norm64_sync(femgpu_ctx *ctx, int n, const double *x, double *d_out, int slot, double &host)
+// past the per-step residual words), with no wait; the slot is valid once the stream has passed
+// the launches.
+static int norm64_async(femgpu_ctx *ctx, int n, const double *x, double *d_out, int slot, volatile double *&hmap)
 {
     const int grid = grid4_for(n);
     double *dmap = reinterpret_cast<double *>(ctx->d_status + 256) + 248 + slot;
-    volatile double *hmap = reinterpret_cast<volatile double *>(ctx->h_status + 256) + 248 + slot;
+    hmap = reinterpret_cast<volatile double *>(ctx->h_status + 256) + 248 + slot;
     k_sumsq<<<grid, 256, 0, ctx->stream>>>(n, x, ctx->partials.p);
     k_norm_finish<<<1, 256, 0, ctx->stream>>>(grid, ctx->partials.p, d_out, dmap);
     FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+// The same, then waits for the stream and returns the norm.
+static int norm64_sync(femgpu_ctx *ctx, int n, const double *x, double *d_out, int slot, double &host)
+{
+    volatile double *hmap = nullptr;
+    FG_TRY(norm64_async(ctx, n, x, d_out, slot, hmap));
     FG_HIP(hipStreamSynchronize(ctx->stream));
     host = *hmap;
     return FEMGPU_OK;
@@ -558,7 +606,7 @@ static int ensure_ws(femgpu_ctx *ctx, int n, int kmax, DevBuf<T> &basis, DevBuf<
 static int make_gdev(femgpu_ctx *ctx, int kmax, GDev &G)
 {
     const int ld = kmax + 1;
-    FG_CHECK(kmax >= 2 && kmax <= 150, FEMGPU_ERR_ARG, "GMRES basis size %d outside [2, 150]", kmax);
+    FG_CHECK(kmax >= 2 && kmax <= kMaxBasis, FEMGPU_ERR_ARG, "GMRES basis size %d outside [2, %d]", kmax, kMaxBasis);
     FG_TRY(ctx->gm_small.alloc(static_cast<size_t>(ld) * ld + 6 * ld + 8));
     FG_TRY(ctx->gm_flags.alloc(8));
     double *base = ctx->gm_small.p;
@@ -924,20 +972,38 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
     double *d_rn = ctx->scalars.p, *d_fn = ctx->scalars.p + 1;
     const int grid = grid_for(n);
     if (info) memset(info, 0, sizeof(*info));
-    double normF = 0.0;
-    FG_TRY(norm64_sync(ctx, n, d_F, d_fn, 0, normF));
-    if (normF == 0.0 || opt->max_refinements == 0) {   // X = 0 is the answer (or no refinement allowed)
+    // X = 0 when it already passes: ||F|| = 0, tol >= 1, or no refinement allowed (status 1 then
+    // unless ||F|| = 0), and for non-finite ||F|| (status 1)
+    auto zero_answer = [&](double nf) -> int {
         FG_HIP(hipMemsetAsync(d_X, 0, sizeof(double) * n, ctx->stream));
+        FG_HIP(hipStreamSynchronize(ctx->stream));
         if (info) {
-            info->normb = normF;
-            info->rho = normF;
-            info->true_rel_residual = normF > 0.0 ? 1.0 : 0.0;
-            info->status = normF > 0.0 ? 1 : 0;
+            const bool ok = nf == 0.0 || (std::isfinite(nf) && tol >= 1.0);
+            info->normb = nf;
+            info->rho = nf;
+            info->true_rel_residual = nf == 0.0 ? 0.0 : (std::isfinite(nf) ? 1.0 : nf);
+            info->status = ok ? 0 : 1;
         }
         return FEMGPU_OK;
+    };
+    double normF = 0.0;
+    if (opt->max_refinements == 0 || tol >= 1.0) {
+        FG_TRY(norm64_sync(ctx, n, d_F, d_fn, 0, normF));
+        return zero_answer(normF);
     }
+    // ||F|| is not waited for: the first refinement reads it on the device, and the host reads it
+    // from the mapped slot after the first cycle's synchronisation (the cases above and ||F|| = 0 or
+    // non-finite, which the first cycle then turns into NaN work, end with X = 0)
+    volatile double *hmap_F = nullptr;
+    FG_TRY(norm64_async(ctx, n, d_F, d_fn, 0, hmap_F));
+    struct Event {
+        hipEvent_t e = nullptr;
+        ~Event() { if (e) (void)hipEventDestroy(e); }
+    } evF;
+    FG_HIP(hipEventCreateWithFlags(&evF.e, hipEventDisableTiming));
+    FG_HIP(hipEventRecord(evF.e, ctx->stream));
     int refinements = 0, inner_total = 0;
-    double rn = normF;
+    double rn = 0.0;
     for (;;) {
         // outer FP64 residual r = F - A X; on the first pass X = 0 (not yet written): r = F, ||r|| = ||F||
         const double *rr = d_F, *d_rcur = d_fn;
@@ -954,7 +1020,8 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
             d_rcur = d_rn;
         }
         // a non-finite residual (a diverged preconditioner or non-finite data) ends the solve: status 1
-        if (!std::isfinite(rn) || rn <= tol * normF || refinements >= opt->max_refinements) break;
+        if (refinements > 0 && (!std::isfinite(rn) || rn <= tol * normF || refinements >= opt->max_refinements))
+            break;
         // inner FP32 solve of M^-1 A d = M^-1 r / ||r||
         k_scale_to_f32<<<grid * 4, 256, 0, ctx->stream>>>(n, rr, d_rcur, wA);
         FG_HIP(hipGetLastError());
@@ -966,6 +1033,12 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
             FG_HIP(hipGetLastError());
             int steps = 0;
             FG_TRY(run_cycle<float>(ctx, S, G, n, V, ldv, wA, wB, kin, &steps));
+            if (refinements == 0) {   // ||F|| (run_cycle has normally waited for a later step already)
+                FG_HIP(hipEventSynchronize(evF.e));
+                normF = *hmap_F;
+                rn = normF;
+                if (normF == 0.0 || !std::isfinite(normF)) return zero_answer(normF);
+            }
             k_cycle_finish<float, double><<<grid, 256, 0, ctx->stream>>>(G, n, V, ldv, d_X, rn, refinements == 0);
             FG_HIP(hipGetLastError());
             FG_HIP(hipStreamSynchronize(ctx->stream));

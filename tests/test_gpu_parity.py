@@ -577,6 +577,31 @@ def test_mixed_precision_reaches_fp64_residual(dev, torch_cuda, oracle, m):
     assert rel(x, Xo) < 1e-4
 
 
+def test_mixed_precision_trivial_answers(dev, torch_cuda, oracle):
+    """Where X = 0 already passes (F = 0, tol >= 1) the mixed solver returns X = 0 with status 0;
+    with no refinement allowed, X = 0 and status 1; non-finite F ends with X = 0 and status 1.
+    ||F|| is read after the first inner cycle, so these cases also run the cycle on NaN data and
+    must still leave X = 0. A normal solve right after shows the context is unharmed."""
+    torch = torch_cuda
+    s = transient_system(oracle, m=60, dt=5e-3)
+    A = fem.CsrMatrix(dev, s["IA"], s["JA"])
+    A.set_values(s["AA"][:-1])
+    A.values_changed()
+    n = s["neq"]
+    F = T(torch, s["F"])
+    for kw, Fv, status in ((dict(), torch.zeros_like(F), 0), (dict(tol=1.5), F, 0),
+                           (dict(max_refinements=0), F, 1), (dict(), F * float("nan"), 1),
+                           (dict(), F * float("inf"), 1)):
+        X = torch.full((n,), 7.0, dtype=torch.float64, device="cuda")
+        info = fem.gmres_mixed(dev, A, Fv.clone(), X, **{"tol": 1e-8, **kw}, sweep_mode=2, factor_sweeps=1)
+        assert info.status == status, (kw, info.status)
+        assert torch.count_nonzero(X).item() == 0, kw
+    X = torch.zeros(n, dtype=torch.float64, device="cuda")
+    info = fem.gmres_mixed(dev, A, F, X, tol=1e-8, sweep_mode=2, factor_sweeps=1)
+    res = np.linalg.norm(s["F"] - oracle.csrmv(s["IA"], s["JA"], s["AA"], H(X))) / np.linalg.norm(s["F"])
+    assert info.status == 0 and res <= 1e-8
+
+
 # ----------------------------------------------------------------------------- FP32 preconditioner
 
 def parilu_reference(IA, JA, AA, sweeps):
