@@ -157,6 +157,7 @@ struct femgpu_ctx {
     int num_cus = 256;
     size_t lds_limit = 65536;      // dynamic LDS available to one workgroup
     fg::DevBuf<double> partials;   // per-block partial sums (two sets of kReduceBlocks)
+    fg::DevBuf<double> res_partials;   // per-block partials of the fused residual norm
     double *noalloc_scratch = nullptr;   // this device's copy of the no-allocation entry point's array
     fg::DevBuf<double> scalars;    // small device scalars (dot results)
     fg::DevBuf<int> err;           // sticky device error flag (bounded waits)
@@ -322,7 +323,11 @@ int launch_spmv64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const d
                   double *y, const int *skip);
 int launch_spmv32(femgpu_ctx *ctx, const femgpu_csr *A, const float *x, const double *xs, float *xout,
                   float *y, const int *skip);
-int launch_residual64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *f, double *r);
+// r = f - A x; sumsq (when non-null) receives per-block partials of ||r||^2, possible where
+// residual_sumsq_blocks(A) > 0 (that many partials)
+int launch_residual64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *f, double *r,
+                      double *sumsq = nullptr);
+int residual_sumsq_blocks(const femgpu_csr *A);
 int launch_sweep64(femgpu_ctx *ctx, femgpu_csr *A, const double *b, double *x, const int *skip);
 int launch_sweep32(femgpu_ctx *ctx, femgpu_csr *A, const float *b, float *x, const int *skip);
 int launch_ilu_factor(femgpu_ctx *ctx, femgpu_csr *A);

@@ -121,9 +121,11 @@ __global__ __launch_bounds__(256) void k_spmv_sell_loop(int n, const int *__rest
 template <int K, typename CT>
 __global__ __launch_bounds__(256) void k_residual_sell(int n, const int *__restrict__ sptr, const CT *__restrict__ col,
                                                        const double *__restrict__ val, const double *__restrict__ x,
-                                                       const double *__restrict__ f, double *__restrict__ r)
+                                                       const double *__restrict__ f, double *__restrict__ r,
+                                                       double *__restrict__ sumsq)
 {
-    const int row = xcd_block() * 256 + threadIdx.x;
+    __shared__ double red[4];
+    const int blk = xcd_block(), row = blk * 256 + threadIdx.x;
     const int rr = row < n ? row : n - 1;
     const int s = rr >> 6, l = rr & 63;
     const int p0 = sptr[s], w = (sptr[s + 1] - p0) >> 6;
@@ -142,7 +144,12 @@ __global__ __launch_bounds__(256) void k_residual_sell(int n, const int *__restr
         const double t = v[k] * x[c[k]];
         acc = k < w ? acc + t : acc;
     }
-    if (row < n) r[row] = fr - acc;
+    const double rv = fr - acc;
+    if (row < n) r[row] = rv;
+    if (sumsq) {   // the block's partial of ||r||^2 (a later kernel sums the blocks in order)
+        const double q = block_sum<256>(row < n ? rv * rv : 0.0, red);
+        if (threadIdx.x == 0) sumsq[blk] = q;
+    }
 }
 
 // The CSR values into the sliced-ELL layout (FP32, and FP64 when val64 is set). K > 0: slices at
@@ -351,9 +358,14 @@ int launch_spmv32(femgpu_ctx *ctx, const femgpu_csr *A, const float *x, const do
     return spmv_dispatch<float>(ctx, A, A->AA32.p, x, xs, xout, y, skip);
 }
 
-int launch_residual64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *f, double *r)
+// blocks of the fused residual launch (the XCD-rounded grid; padding blocks write zero partials)
+int residual_sumsq_blocks(const femgpu_csr *A) { return sell64_active(A) && A->n > 0 ? xcd_grid((A->n + 255) / 256) : 0; }
+
+int launch_residual64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, const double *f, double *r,
+                      double *sumsq)
 {
     if (A->n == 0) return FEMGPU_OK;
+    FG_CHECK(!sumsq || residual_sumsq_blocks(A) > 0, FEMGPU_ERR_ARG, "residual: no fused sum of squares here");
     if (sell64_active(A)) {
         const int n = A->n, grid = xcd_grid((n + 255) / 256);
         const int *sp = A->sell_ptr.p, *sc = A->sell_col.p;
@@ -361,14 +373,14 @@ int launch_residual64(femgpu_ctx *ctx, const femgpu_csr *A, const double *x, con
         const double *sv = A->sell_val64.p;
         if (A->sell_wmax <= 8) {   // sell64_active implies sell_wmax <= kSell64MaxWidth
             if (A->sell16)
-                k_residual_sell<8, short><<<grid, 256, 0, ctx->stream>>>(n, sp, sd, sv, x, f, r);
+                k_residual_sell<8, short><<<grid, 256, 0, ctx->stream>>>(n, sp, sd, sv, x, f, r, sumsq);
             else
-                k_residual_sell<8, int><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, sv, x, f, r);
+                k_residual_sell<8, int><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, sv, x, f, r, sumsq);
         } else {
             if (A->sell16)
-                k_residual_sell<kSell64MaxWidth, short><<<grid, 256, 0, ctx->stream>>>(n, sp, sd, sv, x, f, r);
+                k_residual_sell<kSell64MaxWidth, short><<<grid, 256, 0, ctx->stream>>>(n, sp, sd, sv, x, f, r, sumsq);
             else
-                k_residual_sell<kSell64MaxWidth, int><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, sv, x, f, r);
+                k_residual_sell<kSell64MaxWidth, int><<<grid, 256, 0, ctx->stream>>>(n, sp, sc, sv, x, f, r, sumsq);
         }
         FG_HIP(hipGetLastError());
         return FEMGPU_OK;

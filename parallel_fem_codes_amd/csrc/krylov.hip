@@ -35,6 +35,10 @@ struct GDev {
 
 __device__ __forceinline__ double &Hat(const GDev &G, int r, int c) { return G.H[r * G.ld + c]; }
 
+// Host-mapped doubles (G.h_rho): [0, kmax) the residual estimate after each step, kMapE0 / kMapEps
+// the cycle's starting residual and its target, 248..255 norms of the mixed solver.
+constexpr int kMapE0 = 246, kMapEps = 247;
+
 constexpr int kMaxBasis = 150;   // largest GMRES basis (make_gdev)
 
 // Givens update of column i and the convergence test (pgmres.c:118-159), with h_{i+1,i} = hn.
@@ -369,6 +373,9 @@ __global__ __launch_bounds__(256) void k_cycle_start_finish(GDev G, int nb, int 
         G.scal[1] = e0;
         G.flags[0] = 0;
         G.flags[1] = 0;
+        // the cycle's starting residual and target for the host's look-ahead
+        __hip_atomic_store(&G.h_rho[kMapE0], e0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&G.h_rho[kMapEps], G.scal[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -558,6 +565,20 @@ static int norm64_async(femgpu_ctx *ctx, int n, const double *x, double *d_out, 
     k_sumsq<<<grid, 256, 0, ctx->stream>>>(n, x, ctx->partials.p);
     k_norm_finish<<<1, 256, 0, ctx->stream>>>(grid, ctx->partials.p, d_out, dmap);
     FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
+}
+
+// ||r|| from nb per-block partials of ||r||^2 already in `partials`, into d_out and the mapped slot;
+// waits for the stream and returns it.
+static int norm64_partials_sync(femgpu_ctx *ctx, int nb, const double *partials, double *d_out, int slot,
+                                double &host)
+{
+    double *dmap = reinterpret_cast<double *>(ctx->d_status + 256) + 248 + slot;
+    volatile double *hmap = reinterpret_cast<volatile double *>(ctx->h_status + 256) + 248 + slot;
+    k_norm_finish<<<1, 256, 0, ctx->stream>>>(nb, partials, d_out, dmap);
+    FG_HIP(hipGetLastError());
+    FG_HIP(hipStreamSynchronize(ctx->stream));
+    host = *hmap;
     return FEMGPU_OK;
 }
 
@@ -774,6 +795,12 @@ static int graph_step32(femgpu_ctx *ctx, const femgpu_system *S, const GDev &G, 
     return FEMGPU_OK;
 }
 
+// The host keeps one step in flight ahead of the one it checks: step i is enqueued before step
+// i - 1's done word is read, so the device never waits for the host between steps, and the step
+// after the converged one drains (its kernels exit on the done flag). When the last contraction of
+// the residual estimate predicts that the step in flight reaches the target (rho_{i-2}^2 / rho_{i-3}
+// <= eps), the host waits for that step before enqueuing the next: the drained step's launches,
+// about 20 us, are saved, and a wrong prediction costs one host round trip.
 template <>
 int run_cycle<float>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, float *V, size_t ldv, float *wA,
                      float *wB, int kmax, int *steps_out)
@@ -781,16 +808,25 @@ int run_cycle<float>(femgpu_ctx *ctx, const femgpu_system *S, GDev &G, int n, fl
     hipEvent_t ev[2];
     FG_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     FG_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    const volatile double *hrho = reinterpret_cast<const volatile double *>(ctx->h_status + 256);
     float *wc = wA, *wn = wB;
-    int rc = FEMGPU_OK, steps = 0;
+    int rc = FEMGPU_OK, steps = 0, waited = -1;
     for (int i = 0; i < kmax - 1; ++i) {
+        if (i >= 2) {   // steps up to i - 2 are known (not done), step i - 1 is in flight
+            const double r1 = hrho[i - 2], r0 = i >= 3 ? hrho[i - 3] : hrho[kMapE0], eps = hrho[kMapEps];
+            if (r0 > 0.0 && r1 > 0.0 && r1 * (r1 / r0) <= eps) {
+                if (hipEventSynchronize(ev[(i - 1) & 1]) != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
+                waited = i - 1;
+                if (ctx->h_status[i - 1]) break;
+            }
+        }
         ctx->h_status[i] = 0;
         rc = G.graphs ? graph_step32(ctx, S, G, n, i, wA, wB, V, ldv) : step_kernels32(ctx, S, G, n, i, wc, wn, V, ldv);
         if (rc) break;
         if (hipEventRecord(ev[i & 1], ctx->stream) != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
         std::swap(wc, wn);
         steps = i + 1;
-        if (i >= 1) {
+        if (i >= 1 && waited != i - 1) {
             if (hipEventSynchronize(ev[(i - 1) & 1]) != hipSuccess) { rc = FEMGPU_ERR_HIP; break; }
             if (ctx->h_status[i - 1]) break;
         }
@@ -1008,14 +1044,21 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
         // outer FP64 residual r = F - A X; on the first pass X = 0 (not yet written): r = F, ||r|| = ||F||
         const double *rr = d_F, *d_rcur = d_fn;
         if (refinements > 0) {
-            if (csr) {
-                FG_TRY(launch_residual64(ctx, S->csr, d_X, d_F, r));
+            const int nres = csr ? residual_sumsq_blocks(S->csr) : 0;
+            if (nres > 0) {   // the residual kernel leaves the partials of ||r||^2
+                FG_TRY(ctx->res_partials.alloc(static_cast<size_t>(nres)));
+                FG_TRY(launch_residual64(ctx, S->csr, d_X, d_F, r, ctx->res_partials.p));
+                FG_TRY(norm64_partials_sync(ctx, nres, ctx->res_partials.p, d_rn, 1, rn));
             } else {
-                FG_TRY(launch_ebemv(ctx, S->ebe, d_X, r));
-                k_rsub<<<grid * 4, 256, 0, ctx->stream>>>(n, d_F, r);
-                FG_HIP(hipGetLastError());
+                if (csr) {
+                    FG_TRY(launch_residual64(ctx, S->csr, d_X, d_F, r));
+                } else {
+                    FG_TRY(launch_ebemv(ctx, S->ebe, d_X, r));
+                    k_rsub<<<grid * 4, 256, 0, ctx->stream>>>(n, d_F, r);
+                    FG_HIP(hipGetLastError());
+                }
+                FG_TRY(norm64_sync(ctx, n, r, d_rn, 1, rn));
             }
-            FG_TRY(norm64_sync(ctx, n, r, d_rn, 1, rn));
             rr = r;
             d_rcur = d_rn;
         }
