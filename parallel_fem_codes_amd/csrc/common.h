@@ -158,6 +158,7 @@ struct femgpu_ctx {
     size_t lds_limit = 65536;      // dynamic LDS available to one workgroup
     fg::DevBuf<double> partials;   // per-block partial sums (two sets of kReduceBlocks)
     fg::DevBuf<double> res_partials;   // per-block partials of the fused residual norm
+    fg::DevBuf<double> jac_partials;   // per-block partials of a product fused into the Jacobi sweeps
     double *noalloc_scratch = nullptr;   // this device's copy of the no-allocation entry point's array
     fg::DevBuf<double> scalars;    // small device scalars (dot results)
     fg::DevBuf<int> err;           // sticky device error flag (bounded waits)
@@ -339,7 +340,11 @@ int blockdiag_check(const femgpu_blockdiag *B, int n);
 int blockdiag_size_check(const femgpu_blockdiag *B, int NEQ, int nnodes, int ndof);
 int launch_blockdiag_apply(femgpu_ctx *ctx, const femgpu_blockdiag *B, const double *p, double *z, const int *skip);
 int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA, const int *diag);
-int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip);
+// dpart != nullptr: the last upper iteration also leaves jacobi_dot_blocks(A) per-block FP64 partials
+// of <dv, x> (dv == nullptr: ||x||^2) in dpart; possible where jacobi_dot_blocks(A) > 0
+int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip,
+                          const float *dv = nullptr, double *dpart = nullptr);
+int jacobi_dot_blocks(const femgpu_csr *A);
 int launch_parilu64(femgpu_ctx *ctx, femgpu_csr *A, int sweeps);
 int launch_jacobi_apply64(femgpu_ctx *ctx, femgpu_csr *A, int it, const double *b, double *x, const int *skip);
 int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA);

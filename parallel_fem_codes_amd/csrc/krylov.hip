@@ -115,10 +115,19 @@ struct Vec16<double> {
 // atomics or device-scope fences.
 
 // Sum of the nb partials in a fixed order by one block of 256 threads (valid in every thread).
+// Each thread adds its entries b, b + 256, ... in order, with four loads issued together.
 __device__ __forceinline__ double sum_partials(const double *partials, int nb, double *red)
 {
     double v = 0;
-    for (int b = threadIdx.x; b < nb; b += 256) v += partials[b];
+    int b = threadIdx.x;
+    for (; b + 3 * 256 < nb; b += 4 * 256) {
+        const double a0 = partials[b], a1 = partials[b + 256], a2 = partials[b + 512], a3 = partials[b + 768];
+        v += a0;
+        v += a1;
+        v += a2;
+        v += a3;
+    }
+    for (; b < nb; b += 256) v += partials[b];
     v = block_sum<256>(v, red);
     __shared__ double total;
     if (threadIdx.x == 0) total = v;
@@ -130,12 +139,14 @@ constexpr int kMgsVec = 4;
 
 // MGS pass j of step i on w (vector type T, accumulation in FP64); passes alternate between the two
 // partial sets, pass j writing set j & 1:
-//   uprev != null : w -= h_{j-1,i} * uprev, where every block forms h_{j-1,i} from pass j-1's
-//                   partials (the same fixed-order sum in every block) and block 0 stores it
+//   uprev != null : w -= h_{j-1,i} * uprev, where every block forms h_{j-1,i} from the nprev
+//                   partials of pass j-1 in prev (the same fixed-order sum in every block) and
+//                   block 0 stores it
 //   unext != null : partials of h_{j,i} = <unext, w>   (unext == null: partials of ||w||^2)
 template <typename T>
 __global__ __launch_bounds__(256) void k_mgs(GDev G, int n, int i, int j, T *__restrict__ w,
-                                             const T *__restrict__ uprev, const T *__restrict__ unext)
+                                             const T *__restrict__ uprev, const T *__restrict__ unext,
+                                             const double *__restrict__ prev, int nprev)
 {
     if (G.flags[0]) return;
     using V = Vec16<T>;
@@ -160,7 +171,7 @@ __global__ __launch_bounds__(256) void k_mgs(GDev G, int n, int i, int j, T *__r
     load(kfirst);
     T h = T(0);
     if (uprev) {
-        const double hp = sum_partials(G.partials + ((j - 1) & 1) * kReduceBlocks, gridDim.x, red);
+        const double hp = sum_partials(prev, nprev, red);
         if (blockIdx.x == 0 && threadIdx.x == 0) Hat(G, j - 1, i) = hp;
         h = static_cast<T>(hp);
     }
@@ -356,11 +367,12 @@ __global__ __launch_bounds__(256) void k_cycle_start(GDev G, int n, T *__restric
     if (threadIdx.x == 0) G.partials[blockIdx.x] = acc;
 }
 
-__global__ __launch_bounds__(256) void k_cycle_start_finish(GDev G, int nb, int first, double tol)
+__global__ __launch_bounds__(256) void k_cycle_start_finish(GDev G, int nb, int first, double tol,
+                                                            const double *__restrict__ part)
 {
     __shared__ double red[4];
     for (int q = threadIdx.x; q < G.ld * G.ld; q += 256) G.H[q] = 0.0;   // the cycle's Hessenberg starts at zero
-    const double v = sum_partials(G.partials, nb, red);
+    const double v = sum_partials(part, nb, red);
     if (threadIdx.x == 0) {
         const double e0 = sqrt(v);
         if (first) {   // pgmres.c:42-46: ||B|| and eps
@@ -592,26 +604,37 @@ static int norm64_sync(femgpu_ctx *ctx, int n, const double *x, double *d_out, i
     return FEMGPU_OK;
 }
 
-// The MGS passes of Arnoldi step i (j = 0..i, then the norm pass) and the column's finish.
+// The MGS passes of Arnoldi step i (j = 0..i, then the norm pass) and the column's finish. nfused > 0:
+// the kernel that produced w already left nfused partials of <V[0], w> in fpart (pass 0 is skipped).
 template <typename T>
-static void mgs_step(femgpu_ctx *ctx, const GDev &G, int n, int i, T *wn, const T *V, size_t ldv)
+static void mgs_step(femgpu_ctx *ctx, const GDev &G, int n, int i, T *wn, const T *V, size_t ldv, int nfused = 0,
+                     const double *fpart = nullptr)
 {
     const int nv = n / Vec16<T>::N;
     const int grid = std::max(1, std::min(kReduceBlocks, (nv + 256 * kMgsVec - 1) / (256 * kMgsVec)));
-    for (int j = 0; j <= i + 1; ++j) {
+    for (int j = nfused > 0 ? 1 : 0; j <= i + 1; ++j) {
         const T *uprev = j ? V + (j - 1) * ldv : nullptr;
         const T *unext = j <= i ? V + j * ldv : nullptr;
-        k_mgs<T><<<grid, 256, 0, ctx->stream>>>(G, n, i, j, wn, uprev, unext);
+        const bool from_fused = j == 1 && nfused > 0;
+        const double *prev = from_fused ? fpart : G.partials + ((j - 1) & 1) * kReduceBlocks;
+        k_mgs<T><<<grid, 256, 0, ctx->stream>>>(G, n, i, j, wn, uprev, unext, prev, from_fused ? nfused : grid);
     }
     k_mgs_finish<<<1, 256, 0, ctx->stream>>>(G, grid, i);
 }
 
+// Start of a cycle from w: ||w|| (nfused > 0: from the nfused partials of ||w||^2 that the kernel
+// producing w left in fpart), the Hessenberg reset and the first column's scalars.
 template <typename T>
-static void cycle_start(femgpu_ctx *ctx, const GDev &G, int n, T *w, const T *B, int negate, int first, double tol)
+static void cycle_start(femgpu_ctx *ctx, const GDev &G, int n, T *w, const T *B, int negate, int first, double tol,
+                        int nfused = 0, const double *fpart = nullptr)
 {
+    if (nfused > 0 && !negate) {
+        k_cycle_start_finish<<<1, 256, 0, ctx->stream>>>(G, nfused, first, tol, fpart);
+        return;
+    }
     const int grid = grid4_for(n);
     k_cycle_start<T><<<grid, 256, 0, ctx->stream>>>(G, n, w, B, negate);
-    k_cycle_start_finish<<<1, 256, 0, ctx->stream>>>(G, grid, first, tol);
+    k_cycle_start_finish<<<1, 256, 0, ctx->stream>>>(G, grid, first, tol, G.partials);
 }
 
 // ------------------------------------------------------------------------------- workspace
@@ -656,9 +679,23 @@ static int apply_op64(femgpu_ctx *ctx, const femgpu_system *S, const double *x, 
 // steps the device took in the cycle.
 // FP32 preconditioner of the inner solver: the exact level-scheduled sweeps on the FP32 copy of the
 // ILU(0) factors, or mode Jacobi iterations per triangle on the FP32 factors in A's layout.
-static int prec32(femgpu_ctx *ctx, femgpu_csr *A, int mode, const float *b, float *x, const int *skip)
+// dpart != nullptr (Jacobi mode only): the last upper iteration leaves the partials of <dv, x>
+// (dv == nullptr: ||x||^2) in dpart.
+static int prec32(femgpu_ctx *ctx, femgpu_csr *A, int mode, const float *b, float *x, const int *skip,
+                  const float *dv = nullptr, double *dpart = nullptr)
 {
-    return mode == 0 ? launch_sweep32(ctx, A, b, x, skip) : launch_jacobi_apply32(ctx, A, mode, b, x, skip);
+    if (mode == 0) {
+        FG_CHECK(!dpart, FEMGPU_ERR_ARG, "prec32: no fused product with the exact sweeps");
+        return launch_sweep32(ctx, A, b, x, skip);
+    }
+    return launch_jacobi_apply32(ctx, A, mode, b, x, skip, dv, dpart);
+}
+
+// Partials of a product the inner FP32 preconditioner can fuse (0: none): the Jacobi mode on the
+// sliced-ELL triangles.
+static int prec32_dot_blocks(const femgpu_system *S, int mode)
+{
+    return S->csr && S->ilu && mode > 0 ? jacobi_dot_blocks(S->ilu) : 0;
 }
 
 template <typename T>
@@ -708,11 +745,11 @@ __global__ void k_scale_copy(int n, const T *__restrict__ x, const double *__res
 // into xout (the basis vector). CSR: the sliced-ELL SpMV with the scaling fused, then the ILU(0)
 // sweeps; EBE: the scaled copy, the gathered element product, then the nodal block-diagonal blocks.
 static int op_prec32(femgpu_ctx *ctx, const femgpu_system *S, int prec_mode, const float *x, const double *xs,
-                     float *xout, float *w, const int *skip)
+                     float *xout, float *w, const int *skip, const float *dv = nullptr, double *dpart = nullptr)
 {
     if (S->csr) {
         FG_TRY(launch_spmv32(ctx, S->csr, x, xs, xout, w, skip));
-        return prec32(ctx, S->ilu, prec_mode, w, w, skip);
+        return prec32(ctx, S->ilu, prec_mode, w, w, skip, dv, dpart);
     }
     k_scale_copy<float><<<grid_for(S->ebe->NEQ) * 4, 256, 0, ctx->stream>>>(S->ebe->NEQ, x, xs, xout, skip);
     FG_HIP(hipGetLastError());
@@ -721,11 +758,15 @@ static int op_prec32(femgpu_ctx *ctx, const femgpu_system *S, int prec_mode, con
 }
 
 // The launches of Arnoldi step i of the FP32 inner solver: w_next = M^-1 A (v_i), v_i = w_cur / h
-// stored on the way, then the orthogonalization and the Givens update of column i.
+// stored on the way, then the orthogonalization and the Givens update of column i. With the Jacobi
+// preconditioner and MGS, the last Jacobi iteration also forms the partials of <V[0], w_next>
+// (the first MGS pass), in ctx->jac_partials.
 static int step_kernels32(femgpu_ctx *ctx, const femgpu_system *S, const GDev &G, int n, int i, float *wc,
                           float *wn, float *V, size_t ldv)
 {
-    FG_TRY(op_prec32(ctx, S, G.prec_mode, wc, G.invh + i, V + i * ldv, wn, G.flags));
+    const int nf = G.cgs ? 0 : prec32_dot_blocks(S, G.prec_mode);
+    FG_TRY(op_prec32(ctx, S, G.prec_mode, wc, G.invh + i, V + i * ldv, wn, G.flags, nf ? V : nullptr,
+                     nf ? ctx->jac_partials.p : nullptr));
     if (G.cgs) {
         const int nb = std::max(1, std::min(kCgsBlocks, (n / 4 + 255) / 256));
         k_cgs_dots<<<nb, 256, 0, ctx->stream>>>(G, n, i, wn, V, ldv, ctx->cgs_partials.p);
@@ -733,7 +774,7 @@ static int step_kernels32(femgpu_ctx *ctx, const femgpu_system *S, const GDev &G
         k_cgs_update<<<nb, 256, 0, ctx->stream>>>(G, n, i, wn, V, ldv, ctx->cgs_partials.p);
         k_cgs_finish<<<1, 256, 0, ctx->stream>>>(G, nb, i, ctx->cgs_partials.p);
     } else {
-        mgs_step<float>(ctx, G, n, i, wn, V, ldv);
+        mgs_step<float>(ctx, G, n, i, wn, V, ldv, nf, ctx->jac_partials.p);
     }
     return hipGetLastError() == hipSuccess ? FEMGPU_OK : FEMGPU_ERR_HIP;
 }
@@ -764,7 +805,7 @@ static int graph_step32(femgpu_ctx *ctx, const femgpu_system *S, const GDev &G, 
         static_cast<unsigned long long>(n), u(V), ldv, u(wA), u(wB), S->csr->serial, S->ilu->serial, u(G.H),
         u(G.flags), u(G.partials), u(G.h_status), static_cast<unsigned long long>(G.cgs), u(ctx->cgs_partials.p),
         static_cast<unsigned long long>(G.prec_mode), static_cast<unsigned long long>(G.kmax),
-        static_cast<unsigned long long>(S->ilu->sweep_kernel)};
+        static_cast<unsigned long long>(S->ilu->sweep_kernel), u(ctx->jac_partials.p)};
     if (ctx->step_key != key) {
         release_step_graphs(ctx);
         ctx->step_key = key;
@@ -1002,6 +1043,10 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
     }
     size_t ldv;
     FG_TRY(ensure_ws<float>(ctx, n, kin, ctx->gm_basis32, ctx->gm_w32, ldv));
+    // partials of the products fused into the Jacobi preconditioner (||z||^2 at a cycle's start,
+    // <V[0], w> in each Arnoldi step)
+    const int nfz = prec32_dot_blocks(S, G.prec_mode);
+    if (nfz > 0) FG_TRY(ctx->jac_partials.alloc(static_cast<size_t>(nfz)));
     FG_TRY(ctx->gm_r.alloc(static_cast<size_t>(n) + 64));
     float *V = ctx->gm_basis32.p, *wA = ctx->gm_w32.p, *wB = wA + ldv;
     double *r = ctx->gm_r.p;
@@ -1068,11 +1113,11 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
         // inner FP32 solve of M^-1 A d = M^-1 r / ||r||
         k_scale_to_f32<<<grid * 4, 256, 0, ctx->stream>>>(n, rr, d_rcur, wA);
         FG_HIP(hipGetLastError());
-        if (csr) FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wA, wA, nullptr));
+        if (csr) FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wA, wA, nullptr, nullptr, nfz ? ctx->jac_partials.p : nullptr));
         else FG_TRY(launch_blockdiag_apply32(ctx, S->bdiag, wA, wA, nullptr));
         for (int cyc = 0; cyc < std::max(1, opt->inner_cycles); ++cyc) {
             if (cyc > 0) break;   // one inner cycle per refinement step
-            cycle_start<float>(ctx, G, n, wA, nullptr, 0, 1, opt->inner_tol);
+            cycle_start<float>(ctx, G, n, wA, nullptr, 0, 1, opt->inner_tol, nfz, ctx->jac_partials.p);
             FG_HIP(hipGetLastError());
             int steps = 0;
             FG_TRY(run_cycle<float>(ctx, S, G, n, V, ldv, wA, wB, kin, &steps));
@@ -1144,6 +1189,10 @@ extern "C" int femgpu_gmres_mixed_batched(femgpu_ctx *ctx, const femgpu_system *
     }
     size_t ldv;
     FG_TRY(ensure_ws<float>(ctx, n, kin, ctx->gm_basis32, ctx->gm_w32, ldv));
+    // partials of the products fused into the Jacobi preconditioner (||z||^2 at a cycle's start,
+    // <V[0], w> in each Arnoldi step)
+    const int nfz = prec32_dot_blocks(S, G.prec_mode);
+    if (nfz > 0) FG_TRY(ctx->jac_partials.alloc(static_cast<size_t>(nfz)));
     FG_TRY(ctx->gm_r.alloc(static_cast<size_t>(n) + 64));
     FG_TRY(ctx->bt_rowoff.upload(row_off, static_cast<size_t>(nsys) + 1, ctx->stream));
     FG_TRY(ctx->bt_sys.alloc(static_cast<size_t>(n) + 1));
@@ -1185,8 +1234,8 @@ extern "C" int femgpu_gmres_mixed_batched(femgpu_ctx *ctx, const femgpu_system *
         FG_HIP(hipMemcpyAsync(d_rn, rn.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, ctx->stream));
         k_scale_to_f32_seg<<<grid * 4, 256, 0, ctx->stream>>>(n, ctx->bt_sys.p, r, d_scale, wA);
         FG_HIP(hipGetLastError());
-        FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wA, wA, nullptr));
-        cycle_start<float>(ctx, G, n, wA, nullptr, 0, 1, opt->inner_tol);
+        FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wA, wA, nullptr, nullptr, nfz ? ctx->jac_partials.p : nullptr));
+        cycle_start<float>(ctx, G, n, wA, nullptr, 0, 1, opt->inner_tol, nfz, ctx->jac_partials.p);
         FG_HIP(hipGetLastError());
         int steps = 0;
         FG_TRY(run_cycle<float>(ctx, S, G, n, V, ldv, wA, wB, kin, &steps));

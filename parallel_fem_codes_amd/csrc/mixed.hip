@@ -494,13 +494,17 @@ __global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__re
     }
 }
 
-// z_out = Dinv (y - U_s z_in) (z_in == nullptr: z_in = Dinv y), sliced ELL.
+// z_out = Dinv (y - U_s z_in) (z_in == nullptr: z_in = Dinv y), sliced ELL. dpart != nullptr: the
+// block's FP64 partial of <dv, z_out> (dv == nullptr: ||z_out||^2) into dpart[block], for a
+// following kernel to sum over the blocks in order.
 template <typename T, int K, typename CT>
 __global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, const int *__restrict__ su, const CT *__restrict__ cu,
                                                           const T *__restrict__ vu, const T *__restrict__ Dinv,
                                                           const T *__restrict__ y, const T *__restrict__ zin,
-                                                          T *__restrict__ zout, const int *__restrict__ skip)
+                                                          T *__restrict__ zout, const int *__restrict__ skip,
+                                                          const T *__restrict__ dv, double *__restrict__ dpart)
 {
+    __shared__ double red[4];
     if (skip && *skip) return;
     const int blk = xcd_block();
     EllRows<T, K> e;
@@ -512,6 +516,7 @@ __global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, const int *__re
         di[r] = Dinv[ii];
         yi[r] = y[ii];
     }
+    double dsum = 0.0;
 #pragma unroll
     for (int r = 0; r < kJacRows; ++r) {
         T acc = T(0);
@@ -522,7 +527,15 @@ __global__ __launch_bounds__(256) void k_jacobi_upper_ell(int n, const int *__re
             acc = k < e.w[r] ? acc + t : acc;
         }
         const int i = (blk * kJacRows + r) * 256 + threadIdx.x;
-        if (i < n) zout[i] = di[r] * (yi[r] - acc);
+        if (i < n) {
+            const T z = di[r] * (yi[r] - acc);
+            zout[i] = z;
+            if (dpart) dsum += static_cast<double>(dv ? dv[i] : z) * static_cast<double>(z);
+        }
+    }
+    if (dpart) {
+        dsum = block_sum<256>(dsum, red);
+        if (threadIdx.x == 0) dpart[blk] = dsum;
     }
 }
 
@@ -679,7 +692,8 @@ int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A)
 // b may alias x.
 template <typename T, int K, typename CT>
 static void jacobi_ell_cols(femgpu_ctx *ctx, femgpu_csr *A, const CT *cl, const CT *cu, int it, const T *vl,
-                            const T *vu, const T *dinv, T *ta, T *tb, const T *b, T *x, const int *skip)
+                            const T *vu, const T *dinv, T *ta, T *tb, const T *b, T *x, const int *skip,
+                            const T *dv, double *dpart)
 {
     const int n = A->n, grid = xcd_grid((n + 256 * kJacRows - 1) / (256 * kJacRows));
     T *t[2] = {ta, tb};
@@ -694,19 +708,21 @@ static void jacobi_ell_cols(femgpu_ctx *ctx, femgpu_csr *A, const CT *cl, const 
     prev = nullptr;
     for (int k = 1; k <= it; ++k) {
         T *out = ((it - k) % 2 == 0) ? x : other;
-        k_jacobi_upper_ell<T, K, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, cu, vu, dinv, y, prev, out, skip);
+        k_jacobi_upper_ell<T, K, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, cu, vu, dinv, y, prev, out, skip,
+                                                                   dv, k == it ? dpart : nullptr);
         prev = out;
     }
 }
 
 template <typename T, int K>
 static void jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, int it, const T *vl, const T *vu, const T *dinv, T *ta, T *tb,
-                       const T *b, T *x, const int *skip)
+                       const T *b, T *x, const int *skip, const T *dv = nullptr, double *dpart = nullptr)
 {
     if (A->ell16)
-        jacobi_ell_cols<T, K, unsigned short>(ctx, A, A->ell_dl.p, A->ell_du.p, it, vl, vu, dinv, ta, tb, b, x, skip);
+        jacobi_ell_cols<T, K, unsigned short>(ctx, A, A->ell_dl.p, A->ell_du.p, it, vl, vu, dinv, ta, tb, b, x, skip,
+                                              dv, dpart);
     else
-        jacobi_ell_cols<T, K, int>(ctx, A, A->ell_cl.p, A->ell_cu.p, it, vl, vu, dinv, ta, tb, b, x, skip);
+        jacobi_ell_cols<T, K, int>(ctx, A, A->ell_cl.p, A->ell_cu.p, it, vl, vu, dinv, ta, tb, b, x, skip, dv, dpart);
 }
 
 // The current values of A into the sliced-ELL layout in FP64 (padding 0).
@@ -781,13 +797,13 @@ static void jacobi_iteration_ell(femgpu_ctx *ctx, femgpu_csr *A, const CT *cl, c
     if (upper) {
         if (w <= 4)
             k_jacobi_upper_ell<float, 4, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, cu, A->ell_vu.p, A->Dinv32.p,
-                                                                         rhs, in, out, nullptr);
+                                                                         rhs, in, out, nullptr, nullptr, nullptr);
         else if (w <= 8)
             k_jacobi_upper_ell<float, 8, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, cu, A->ell_vu.p, A->Dinv32.p,
-                                                                         rhs, in, out, nullptr);
+                                                                         rhs, in, out, nullptr, nullptr, nullptr);
         else
             k_jacobi_upper_ell<float, 16, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_su.p, cu, A->ell_vu.p, A->Dinv32.p,
-                                                                          rhs, in, out, nullptr);
+                                                                          rhs, in, out, nullptr, nullptr, nullptr);
     } else {
         if (w <= 4)
             k_jacobi_lower_ell<float, 4, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, cl, A->ell_vl.p, rhs, in, out,
@@ -827,18 +843,25 @@ int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const f
     return FEMGPU_OK;
 }
 
-int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip)
+int jacobi_dot_blocks(const femgpu_csr *A)
+{
+    return A->ell_kl > 0 && A->n > 0 ? xcd_grid((A->n + 256 * kJacRows - 1) / (256 * kJacRows)) : 0;
+}
+
+int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip,
+                          const float *dv, double *dpart)
 {
     const int n = A->n;
     if (n == 0) return FEMGPU_OK;
     FG_CHECK(it >= 1 && it <= 1024, FEMGPU_ERR_ARG, "Jacobi sweeps: %d iterations out of range", it);
+    FG_CHECK(!dpart || jacobi_dot_blocks(A) > 0, FEMGPU_ERR_ARG, "Jacobi sweeps: no fused product here");
     if (A->ell_kl > 0) {
         const int w = std::max(A->ell_kl, A->ell_ku);
         const float *vl = A->ell_vl.p, *vu = A->ell_vu.p, *d = A->Dinv32.p;
         float *ta = A->jac_a.p, *tb = A->jac_b.p;
-        if (w <= 4) jacobi_ell<float, 4>(ctx, A, it, vl, vu, d, ta, tb, b, x, skip);
-        else if (w <= 8) jacobi_ell<float, 8>(ctx, A, it, vl, vu, d, ta, tb, b, x, skip);
-        else jacobi_ell<float, 16>(ctx, A, it, vl, vu, d, ta, tb, b, x, skip);
+        if (w <= 4) jacobi_ell<float, 4>(ctx, A, it, vl, vu, d, ta, tb, b, x, skip, dv, dpart);
+        else if (w <= 8) jacobi_ell<float, 8>(ctx, A, it, vl, vu, d, ta, tb, b, x, skip, dv, dpart);
+        else jacobi_ell<float, 16>(ctx, A, it, vl, vu, d, ta, tb, b, x, skip, dv, dpart);
         FG_HIP(hipGetLastError());
         return FEMGPU_OK;
     }
