@@ -1129,10 +1129,12 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
             }
             k_cycle_finish<float, double><<<grid, 256, 0, ctx->stream>>>(G, n, V, ldv, d_X, rn, refinements == 0);
             FG_HIP(hipGetLastError());
-            FG_HIP(hipStreamSynchronize(ctx->stream));
-            // only the level-scheduled sweeps (mode 0) can raise the device error word: the
-            // Jacobi-preconditioned cycle skips reading it back
-            if (csr && G.prec_mode == 0) FG_TRY(check_device_error(ctx));
+            // only the level-scheduled sweeps (mode 0) can raise the device error word; otherwise the
+            // next residual is enqueued right behind the update, and its norm's wait covers both
+            if (csr && G.prec_mode == 0) {
+                FG_HIP(hipStreamSynchronize(ctx->stream));
+                FG_TRY(check_device_error(ctx));
+            }
             // the device's step count: the first step that reported the cycle done
             int dsteps = steps;
             for (int i = 0; i < steps; ++i)
