@@ -129,15 +129,20 @@ struct StreamSweep {
 constexpr int kFactorRow = 8;    // entries per row
 constexpr int kFactorU = 4;      // U entries per row kept in the ring
 constexpr int kFactorRing = kFactorU + 1;
+// Compact factorisation rows (at most 7 entries, 3 L): index of the (entry e, L entry t) pair,
+// t < min(3, e), among the row's 15 pairs.
+constexpr int factor_pair_index(int e, int t) { return e == 1 ? 0 : e == 2 ? 1 + t : 3 + 3 * (e - 3) + t; }
 
 struct StreamFactor {
     bool ok = false;
-    DevBuf<uint4> codes;     // n: 32 codes of 4 bits, code (e, t) at bit 4*(4e+t)
+    bool compact = false;    // every row has at most 7 entries, 3 of them L and 3 U
+    DevBuf<uint4> codes;     // n: general, 32 codes of 4 bits, code (e, t) at bit 4*(4e+t); compact,
+                             // {15 2-bit U columns, slot0 | slot1 << 16, slot2 | (len | nL << 4) << 16, 0}
     DevBuf<int4> meta;       // n: {p0, len, nL, row}
     DevBuf<double> aval;     // kFactorRow*n: value e of schedule position q at e*n + q (from AA)
-    DevBuf<double> out;      // kFactorRow*n: factored rows, same layout
-    DevBuf<double> dinv;     // n: inverse pivots in schedule order
-    DevBuf<double> sink;     // idle-thread results
+    DevBuf<double> out;      // kFactorRow*(n + kStreamThreads): factored rows, planes of n + kStreamThreads
+                             // (idle threads write past n)
+    DevBuf<double> dinv;     // n + kStreamThreads: inverse pivots in schedule order
 };
 
 // Numeric ILU(0) plan: for every row in the lower sweep's level schedule, its CSR position, length,

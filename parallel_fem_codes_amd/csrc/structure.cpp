@@ -787,17 +787,25 @@ int build_stream_factor(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int
     const size_t limit = ctx->lds_limit > 1024 ? ctx->lds_limit - 1024 : ctx->lds_limit;
     if (factor_lds_bytes(P) > limit) return FEMGPU_OK;
     const int n = A->n;
+    bool compact = true;
     for (int i = 0; i < n; ++i) {
         const int len = IA[i + 1] - IA[i], nL = diag[i] - IA[i], nU = len - nL - 1;
         if (len > kFactorRow || nL > 4 || nU > kFactorU) return FEMGPU_OK;
+        compact = compact && len <= 7 && nL <= 3 && nU <= 3;
     }
     std::vector<int> row_of(n);
     FG_HIP(hipMemcpy(row_of.data(), P.row_of.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::vector<unsigned short> slot;   // compact rows carry their dependencies' ring slots
+    if (compact) {
+        slot.resize(static_cast<size_t>(n) * 4);
+        FG_HIP(hipMemcpy(slot.data(), P.slot.p, sizeof(unsigned short) * slot.size(), hipMemcpyDeviceToHost));
+    }
     std::vector<uint4> codes(n);
     std::vector<int4> meta(n);
     for (int q = 0; q < n; ++q) {
         const int i = row_of[q], p0 = IA[i], len = IA[i + 1] - p0, nL = diag[i] - p0;
         unsigned w[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+        unsigned c2 = 0xffffffffu;   // compact: 2-bit U columns of the 15 pairs (3: none)
         for (int e = 0; e < len; ++e)
             for (int t = 0; t < nL && t < e; ++t) {
                 const int j = JA[p0 + t], col = JA[p0 + e];
@@ -806,18 +814,31 @@ int build_stream_factor(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int
                     if (JA[diag[j] + 1 + u] == col) code = static_cast<unsigned>(u);
                 const int bit = 4 * (4 * e + t);
                 w[bit / 32] = (w[bit / 32] & ~(15u << (bit % 32))) | (code << (bit % 32));
+                if (compact && code < 3u) {
+                    const int b = 2 * factor_pair_index(e, t);
+                    c2 = (c2 & ~(3u << b)) | (code << b);
+                }
             }
-        // entry 0 has no (entry, L entry) pairs: its 16 code bits carry the row's length and L count
-        w[0] = (w[0] & ~0xffu) | static_cast<unsigned>(len) | (static_cast<unsigned>(nL) << 4);
-        codes[q] = make_uint4(w[0], w[1], w[2], w[3]);
+        if (compact) {
+            // ring slots of the L entries (W, the zero slot, for absent ones), length and L count
+            unsigned sl[3];
+            for (int t = 0; t < 3; ++t)
+                sl[t] = t < nL ? slot[static_cast<size_t>(q) * 4 + t] : static_cast<unsigned>(P.W);
+            codes[q] = make_uint4(c2, sl[0] | (sl[1] << 16), sl[2] | (static_cast<unsigned>(len | (nL << 4)) << 16), 0u);
+        } else {
+            // entry 0 has no (entry, L entry) pairs: its 16 code bits carry the row's length and L count
+            w[0] = (w[0] & ~0xffu) | static_cast<unsigned>(len) | (static_cast<unsigned>(nL) << 4);
+            codes[q] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
         meta[q] = make_int4(p0, len, nL, i);
     }
-    FG_TRY(F.codes.upload(codes.data(), n, ctx->stream));
+    FG_TRY(F.codes.upload(codes.data(), codes.size(), ctx->stream));
     FG_TRY(F.meta.upload(meta.data(), n, ctx->stream));
     FG_TRY(F.aval.alloc(static_cast<size_t>(n) * kFactorRow));
-    FG_TRY(F.out.alloc(static_cast<size_t>(n) * kFactorRow));
-    FG_TRY(F.dinv.alloc(n));
+    FG_TRY(F.out.alloc(static_cast<size_t>(n + kStreamThreads) * kFactorRow));
+    FG_TRY(F.dinv.alloc(static_cast<size_t>(n) + kStreamThreads));
     FG_HIP(hipStreamSynchronize(ctx->stream));
+    F.compact = compact;
     F.ok = true;
     return FEMGPU_OK;
 }
