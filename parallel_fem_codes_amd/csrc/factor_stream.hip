@@ -6,9 +6,10 @@
 // the column-match codes of its (entry, L entry) pairs, its length and L count, the ring slots of
 // its L entries -- is in schedule order and fetched into registers D steps ahead; the results are
 // written in schedule order and scattered to A's layout afterwards by a full-device kernel. Values
-// and results are stored entry-major (entry e of position q at e*n + q), so each load or store of a
-// step's entry is one contiguous run across the threads: one workgroup issues all of the kernel's
-// memory traffic, and a strided layout costs it a cache line per lane.
+// and results are stored as 16-byte pairs (entries e, e + 1 of position q at (e/2)*2n + 2q), so each
+// load or store of a step's pair is one contiguous run across the threads: one workgroup issues all
+// of the kernel's memory traffic, so its instruction count and a layout's cache lines per lane are
+// what a step costs.
 //
 // Arithmetic (ILUp_precond_setup.c:84-166, p = 0; contraction off): entries in column order; entry e
 // starts from a_e and subtracts l_t * u_{j_t, col_e} for the L entries t < e in order; an L entry
