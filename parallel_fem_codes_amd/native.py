@@ -59,7 +59,8 @@ class SweepInfo(C.Structure):
     """femgpu_sweep_info."""
     _fields_ = [(f, C.c_int) for f in ("kernel", "depth_lower", "depth_upper", "steps_lower", "steps_upper",
                                         "ring", "entries_per_row", "chunks_lower", "rows_per_chunk", "lds_limit",
-                                        "factor_kernel", "jacobi_column_bytes", "sell_column_bytes", "long_slots")]
+                                        "factor_kernel", "jacobi_column_bytes", "sell_column_bytes", "long_slots",
+                                        "factor_compact")]
 
 
 class MixedOpts(C.Structure):

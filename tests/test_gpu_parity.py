@@ -303,7 +303,46 @@ def ilu_case(oracle, name):
         return synthetic_csr(20000, 20, 4)
     if name.startswith("ns"):   # the Navier-Stokes cavity pattern (9x9 blocks, up to 11 entries per triangle)
         return ns_pattern_csr(int(name[2:]), 7)
+    if name == "m200x":
+        return with_extra_couplings(steady_system(oracle, m=200), 8)
     return steady_system(oracle, d=G.load(name))
+
+
+def with_extra_couplings(s, seed):
+    """The system with symmetric pairs of extra entries (i, j), (j, i) added about every 37 rows, j
+    a row two dependency levels after i in both triangular solves (so the level schedules keep their
+    depth), row i with 3 U entries and row j with 3 L entries: those rows get 8 entries and a fourth
+    U or L entry, so the streamed factorisation takes its general row shape instead of the compact
+    one."""
+    IA, JA, AA = s["IA"], s["JA"], s["AA"]
+    n = len(IA) - 1
+    rows = [dict(zip(JA[IA[i]:IA[i + 1]].tolist(), AA[IA[i]:IA[i + 1]].tolist())) for i in range(n)]
+    lev_l, lev_u = np.zeros(n, int), np.zeros(n, int)
+    for i in range(n):
+        lev_l[i] = 1 + max([lev_l[c] for c in rows[i] if c < i], default=0)
+    for i in range(n - 1, -1, -1):
+        lev_u[i] = 1 + max([lev_u[c] for c in rows[i] if c > i], default=0)
+    rng = np.random.default_rng(seed)
+    added = 0
+    for i in range(0, n, 37):
+        if sum(1 for c in rows[i] if c > i) != 3:
+            continue
+        cand = [j for j in range(i + 1, min(n, i + 400)) if lev_l[j] == lev_l[i] + 2 and lev_u[j] == lev_u[i] - 2
+                and j not in rows[i] and sum(1 for c in rows[j] if c < j) == 3]
+        if not cand:
+            continue
+        j = cand[0]
+        for a, b in ((i, j), (j, i)):
+            v = rng.uniform(-0.1, 0.1)
+            rows[a][b] = v
+            rows[a][a] += abs(v)
+        added += 1
+    assert added > 100
+    IA2 = np.zeros(n + 1, np.int32)
+    IA2[1:] = np.cumsum([len(r) for r in rows])
+    JA2 = np.concatenate([np.array(sorted(r), np.int32) for r in rows])
+    AA2 = np.concatenate([np.array([r[c] for c in sorted(r)]) for r in rows])
+    return dict(IA=IA2, JA=JA2, AA=np.append(AA2, 0.0), neq=n)
 
 
 def ns_pattern_csr(m, seed):
@@ -323,8 +362,8 @@ def ns_pattern_csr(m, seed):
 
 
 @pytest.mark.parametrize("kernel", ["stream", "packed"])
-@pytest.mark.parametrize("name", CSR_CASES + ["m200", "m200cau", "m600not", "synthetic12", "synthetic20", "ns64",
-                                  "ns128"])
+@pytest.mark.parametrize("name", CSR_CASES + ["m200", "m200cau", "m600not", "m200x", "synthetic12", "synthetic20",
+                                  "ns64", "ns128"])
 def test_ilu0_factor_and_sweeps_bit_exact(dev, torch_cuda, oracle, name, kernel):
     torch = torch_cuda
     s = ilu_case(oracle, name)
@@ -343,6 +382,8 @@ def test_ilu0_factor_and_sweeps_bit_exact(dev, torch_cuda, oracle, name, kernel)
             assert info["long_slots"] == 0
     else:
         assert info["kernel"] == 0 and info["entries_per_row"] == 4 and info["factor_kernel"] == 0
+        # the 7-point stencil takes the compact factorisation rows, the extra couplings the general ones
+        assert info["factor_compact"] == (0 if name == "m200x" else 1)
         if name == "m600not":
             assert info["steps_lower"] > info["depth_lower"]
     A.set_values(s["AA"][:-1])
