@@ -350,6 +350,10 @@ int build_jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *J
 int launch_jacobi_apply32(femgpu_ctx *ctx, femgpu_csr *A, int it, const float *b, float *x, const int *skip,
                           const float *dv = nullptr, double *dpart = nullptr);
 int jacobi_dot_blocks(const femgpu_csr *A);
+// The same Jacobi apply (sliced-ELL triangles only) with b = float(r / *rn) formed by its first
+// lower iteration and stored in x (the application is in place on x)
+int launch_jacobi_apply32_scaled(femgpu_ctx *ctx, femgpu_csr *A, int it, const double *r, const double *rn, float *x,
+                                 const float *dv = nullptr, double *dpart = nullptr);
 int launch_parilu64(femgpu_ctx *ctx, femgpu_csr *A, int sweeps);
 int launch_jacobi_apply64(femgpu_ctx *ctx, femgpu_csr *A, int it, const double *b, double *x, const int *skip);
 int build_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *JA);

@@ -1111,10 +1111,14 @@ extern "C" int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *S, const
         if (refinements > 0 && (!std::isfinite(rn) || rn <= tol * normF || refinements >= opt->max_refinements))
             break;
         // inner FP32 solve of M^-1 A d = M^-1 r / ||r||
-        k_scale_to_f32<<<grid * 4, 256, 0, ctx->stream>>>(n, rr, d_rcur, wA);
-        FG_HIP(hipGetLastError());
-        if (csr) FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wA, wA, nullptr, nullptr, nfz ? ctx->jac_partials.p : nullptr));
-        else FG_TRY(launch_blockdiag_apply32(ctx, S->bdiag, wA, wA, nullptr));
+        if (nfz > 0) {   // Jacobi on the sliced-ELL triangles: its first lower launch forms r / ||r|| in FP32
+            FG_TRY(launch_jacobi_apply32_scaled(ctx, S->ilu, G.prec_mode, rr, d_rcur, wA, nullptr, ctx->jac_partials.p));
+        } else {
+            k_scale_to_f32<<<grid * 4, 256, 0, ctx->stream>>>(n, rr, d_rcur, wA);
+            FG_HIP(hipGetLastError());
+            if (csr) FG_TRY(prec32(ctx, S->ilu, G.prec_mode, wA, wA, nullptr));
+            else FG_TRY(launch_blockdiag_apply32(ctx, S->bdiag, wA, wA, nullptr));
+        }
         for (int cyc = 0; cyc < std::max(1, opt->inner_cycles); ++cyc) {
             if (cyc > 0) break;   // one inner cycle per refinement step
             cycle_start<float>(ctx, G, n, wA, nullptr, 0, 1, opt->inner_tol, nfz, ctx->jac_partials.p);

@@ -19,6 +19,7 @@
 // the preconditioner is one fixed linear operator for the Krylov method.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -494,6 +495,43 @@ __global__ __launch_bounds__(256) void k_jacobi_lower_ell(int n, const int *__re
     }
 }
 
+// The first lower iteration from the FP64 residual: b = float(r / ||r||) (the products of the
+// scaled FP32 copy, formed here for the row and its gathers instead of by a separate kernel), y_out =
+// b - L_s b, and b stored for the next iterations.
+template <int K, typename CT>
+__global__ __launch_bounds__(256) void k_jacobi_lower_ell_scaled(int n, const int *__restrict__ sl,
+                                                                 const CT *__restrict__ cl,
+                                                                 const float *__restrict__ vl,
+                                                                 const double *__restrict__ r,
+                                                                 const double *__restrict__ rn,
+                                                                 float *__restrict__ bout, float *__restrict__ yout)
+{
+    const double inv = 1.0 / *rn;
+    const int blk = xcd_block();
+    EllRows<float, K> e;
+    ell_load<float, K, CT, false>(n, blk, sl, cl, vl, e);
+    float bi[kJacRows];
+#pragma unroll
+    for (int q = 0; q < kJacRows; ++q) {
+        const int i = (blk * kJacRows + q) * 256 + threadIdx.x;
+        bi[q] = static_cast<float>(r[i < n ? i : n - 1] * inv);
+    }
+#pragma unroll
+    for (int q = 0; q < kJacRows; ++q) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const float t = e.val[q][k] * static_cast<float>(r[e.col[q][k]] * inv);
+            acc = k < e.w[q] ? acc + t : acc;
+        }
+        const int i = (blk * kJacRows + q) * 256 + threadIdx.x;
+        if (i < n) {
+            yout[i] = bi[q] - acc;
+            bout[i] = bi[q];
+        }
+    }
+}
+
 // z_out = Dinv (y - U_s z_in) (z_in == nullptr: z_in = Dinv y), sliced ELL. dpart != nullptr: the
 // block's FP64 partial of <dv, z_out> (dv == nullptr: ||z_out||^2) into dpart[block], for a
 // following kernel to sum over the blocks in order.
@@ -690,16 +728,27 @@ int launch_factor_to_f32(femgpu_ctx *ctx, femgpu_csr *A)
 // x = (U~ L~)^-1 b approximated with it Jacobi iterations per triangle on factors in the Jacobi
 // sliced-ELL layout (vl, vu strict triangles, dinv inverse pivots); ta, tb: two work vectors.
 // b may alias x.
+// rb != nullptr (FP32): b is not given but formed by the first lower launch as float(rb / *rn) and
+// stored in bw (then b = bw for the later iterations).
 template <typename T, int K, typename CT>
 static void jacobi_ell_cols(femgpu_ctx *ctx, femgpu_csr *A, const CT *cl, const CT *cu, int it, const T *vl,
                             const T *vu, const T *dinv, T *ta, T *tb, const T *b, T *x, const int *skip,
-                            const T *dv, double *dpart)
+                            const T *dv, double *dpart, const double *rb = nullptr, const double *rn = nullptr,
+                            T *bw = nullptr)
 {
     const int n = A->n, grid = xcd_grid((n + 256 * kJacRows - 1) / (256 * kJacRows));
     T *t[2] = {ta, tb};
     const T *prev = nullptr;
     for (int k = 0; k < it; ++k) {
         T *out = t[k & 1];
+        if constexpr (std::is_same_v<T, float>) {
+            if (k == 0 && rb) {
+                k_jacobi_lower_ell_scaled<K, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, cl, vl, rb, rn, bw, out);
+                b = bw;
+                prev = out;
+                continue;
+            }
+        }
         k_jacobi_lower_ell<T, K, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, cl, vl, b, prev, out, skip);
         prev = out;
     }
@@ -716,13 +765,15 @@ static void jacobi_ell_cols(femgpu_ctx *ctx, femgpu_csr *A, const CT *cl, const 
 
 template <typename T, int K>
 static void jacobi_ell(femgpu_ctx *ctx, femgpu_csr *A, int it, const T *vl, const T *vu, const T *dinv, T *ta, T *tb,
-                       const T *b, T *x, const int *skip, const T *dv = nullptr, double *dpart = nullptr)
+                       const T *b, T *x, const int *skip, const T *dv = nullptr, double *dpart = nullptr,
+                       const double *rb = nullptr, const double *rn = nullptr, T *bw = nullptr)
 {
     if (A->ell16)
         jacobi_ell_cols<T, K, unsigned short>(ctx, A, A->ell_dl.p, A->ell_du.p, it, vl, vu, dinv, ta, tb, b, x, skip,
-                                              dv, dpart);
+                                              dv, dpart, rb, rn, bw);
     else
-        jacobi_ell_cols<T, K, int>(ctx, A, A->ell_cl.p, A->ell_cu.p, it, vl, vu, dinv, ta, tb, b, x, skip, dv, dpart);
+        jacobi_ell_cols<T, K, int>(ctx, A, A->ell_cl.p, A->ell_cu.p, it, vl, vu, dinv, ta, tb, b, x, skip, dv, dpart,
+                                   rb, rn, bw);
 }
 
 // The current values of A into the sliced-ELL layout in FP64 (padding 0).
@@ -815,6 +866,23 @@ static void jacobi_iteration_ell(femgpu_ctx *ctx, femgpu_csr *A, const CT *cl, c
             k_jacobi_lower_ell<float, 16, CT><<<grid, 256, 0, ctx->stream>>>(n, A->ell_sl.p, cl, A->ell_vl.p, rhs, in, out,
                                                                           nullptr);
     }
+}
+
+int launch_jacobi_apply32_scaled(femgpu_ctx *ctx, femgpu_csr *A, int it, const double *r, const double *rn, float *x,
+                                 const float *dv, double *dpart)
+{
+    const int n = A->n;
+    if (n == 0) return FEMGPU_OK;
+    FG_CHECK(it >= 1 && it <= 1024 && A->ell_kl > 0, FEMGPU_ERR_ARG,
+             "Jacobi sweeps from the FP64 residual: needs the sliced-ELL triangles (%d iterations)", it);
+    const int w = std::max(A->ell_kl, A->ell_ku);
+    const float *vl = A->ell_vl.p, *vu = A->ell_vu.p, *d = A->Dinv32.p;
+    float *ta = A->jac_a.p, *tb = A->jac_b.p;
+    if (w <= 4) jacobi_ell<float, 4>(ctx, A, it, vl, vu, d, ta, tb, x, x, nullptr, dv, dpart, r, rn, x);
+    else if (w <= 8) jacobi_ell<float, 8>(ctx, A, it, vl, vu, d, ta, tb, x, x, nullptr, dv, dpart, r, rn, x);
+    else jacobi_ell<float, 16>(ctx, A, it, vl, vu, d, ta, tb, x, x, nullptr, dv, dpart, r, rn, x);
+    FG_HIP(hipGetLastError());
+    return FEMGPU_OK;
 }
 
 int launch_jacobi_iteration32(femgpu_ctx *ctx, femgpu_csr *A, int upper, const float *rhs, const float *in,
