@@ -321,7 +321,9 @@ typedef struct {
  * Two system forms: sys->csr with sys->ilu (ILU(0), factored unless opt->factor_sweeps > 0; the
  * inner solver uses the FP32 sliced-ELL copy and FP32 factors), or sys->ebe with sys->bdiag (set up
  * on the current element matrices; the inner solver uses the FP32 element matrices and FP32 inverse
- * blocks, sweep_mode / factor_sweeps ignored, graphs must be 0). No reference counterpart. */
+ * blocks, sweep_mode / factor_sweeps ignored, graphs must be 0). X = 0 is returned where it already
+ * passes (||F|| = 0 or tol >= 1: status 0), with max_refinements = 0 (status 1 unless ||F|| = 0)
+ * and for a non-finite F (status 1). No reference counterpart. */
 int femgpu_gmres_mixed(femgpu_ctx *ctx, const femgpu_system *sys, const femgpu_mixed_opts *opt,
                        double tol, const double *d_F, double *d_X, femgpu_solve_info *info);
 
