@@ -143,8 +143,8 @@ static int ebe_product(femgpu_ctx *ctx, const femgpu_ebe *E, const T *vals, cons
     const bool c16 = E->col16 != 0;
     switch (E->ndof) {
     case 1:
-        if (c16) gather_launch<T, short, 1, 8>(ctx, E, vals, P, Q, skip);
-        else gather_launch<T, int, 1, 8>(ctx, E, vals, P, Q, skip);
+        if (c16) gather_launch<T, short, 1, 6>(ctx, E, vals, P, Q, skip);
+        else gather_launch<T, int, 1, 6>(ctx, E, vals, P, Q, skip);
         break;
     case 3:
         if (c16) gather_launch<T, short, 3, 1>(ctx, E, vals, P, Q, skip);
