@@ -129,20 +129,22 @@ struct StreamSweep {
 constexpr int kFactorRow = 8;    // entries per row
 constexpr int kFactorU = 4;      // U entries per row kept in the ring
 constexpr int kFactorRing = kFactorU + 1;
-// Compact factorisation rows (at most 7 entries, 3 L): index of the (entry e, L entry t) pair,
-// t < min(3, e), among the row's 15 pairs.
+// Compact factorisation rows (at most 7 entries, 3 L, at fixed slots: L entries 0-2, the diagonal
+// 3, U entries 4-6): index of the (slot e, L slot t) pair, t < min(3, e), among the row's 15 pairs.
 constexpr int factor_pair_index(int e, int t) { return e == 1 ? 0 : e == 2 ? 1 + t : 3 + 3 * (e - 3) + t; }
 
 struct StreamFactor {
     bool ok = false;
-    bool compact = false;    // every row has at most 7 entries, 3 of them L and 3 U
-    DevBuf<uint4> codes;     // n: general, 32 codes of 4 bits, code (e, t) at bit 4*(4e+t); compact,
-                             // {15 2-bit U columns, slot0 | slot1 << 16, slot2 | (len | nL << 4) << 16, 0}
+    bool compact = false;    // every row has at most 7 entries, 3 of them L and 3 U, and every L
+                             // entry's row lies at most 2 047 schedule positions back
+    DevBuf<uint4> codes;     // general, n: 32 codes of 4 bits, code (e, t) at bit 4*(4e+t)
+    DevBuf<unsigned long long> rec;   // compact, n: 15 2-bit U columns of the (slot e, L slot t)
+                                      // pairs, then three 11-bit distances to the L entries' rows
     DevBuf<int4> meta;       // n: {p0, len, nL, row}
     DevBuf<double> aval;     // kFactorRow*n: value e of schedule position q at e*n + q (from AA)
-    DevBuf<double> out;      // kFactorRow*(n + kStreamThreads): factored rows, planes of n + kStreamThreads
-                             // (idle threads write past n)
-    DevBuf<double> dinv;     // n + kStreamThreads: inverse pivots in schedule order
+    DevBuf<double> out;      // factored rows in planes of n + kStreamThreads positions (idle threads
+                             // write past n): general kFactorRow planes, compact 4 (pivot, U part)
+    DevBuf<double> dinv;     // general, n + kStreamThreads: inverse pivots in schedule order
 };
 
 // Numeric ILU(0) plan: for every row in the lower sweep's level schedule, its CSR position, length,

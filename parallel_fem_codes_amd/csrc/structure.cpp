@@ -793,19 +793,21 @@ int build_stream_factor(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int
         if (len > kFactorRow || nL > 4 || nU > kFactorU) return FEMGPU_OK;
         compact = compact && len <= 7 && nL <= 3 && nU <= 3;
     }
-    std::vector<int> row_of(n);
+    std::vector<int> row_of(n), pos(n);
     FG_HIP(hipMemcpy(row_of.data(), P.row_of.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-    std::vector<unsigned short> slot;   // compact rows carry their dependencies' ring slots
-    if (compact) {
-        slot.resize(static_cast<size_t>(n) * 4);
-        FG_HIP(hipMemcpy(slot.data(), P.slot.p, sizeof(unsigned short) * slot.size(), hipMemcpyDeviceToHost));
+    for (int q = 0; q < n; ++q) pos[row_of[q]] = q;
+    // compact rows name each L entry's row by its schedule distance in 11 bits
+    for (int q = 0; q < n && compact; ++q) {
+        const int i = row_of[q];
+        for (int p = IA[i]; p < diag[i]; ++p) compact = compact && q - pos[JA[p]] < 2048;
     }
-    std::vector<uint4> codes(n);
+    std::vector<uint4> codes(compact ? 0 : n);
+    std::vector<unsigned long long> rec(compact ? n : 0);
     std::vector<int4> meta(n);
     for (int q = 0; q < n; ++q) {
         const int i = row_of[q], p0 = IA[i], len = IA[i + 1] - p0, nL = diag[i] - p0;
         unsigned w[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
-        unsigned c2 = 0xffffffffu;   // compact: 2-bit U columns of the 15 pairs (3: none)
+        unsigned long long r = 0x3fffffffULL;   // compact: 2-bit U columns of the 15 pairs (3: none)
         for (int e = 0; e < len; ++e)
             for (int t = 0; t < nL && t < e; ++t) {
                 const int j = JA[p0 + t], col = JA[p0 + e];
@@ -814,17 +816,15 @@ int build_stream_factor(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int
                     if (JA[diag[j] + 1 + u] == col) code = static_cast<unsigned>(u);
                 const int bit = 4 * (4 * e + t);
                 w[bit / 32] = (w[bit / 32] & ~(15u << (bit % 32))) | (code << (bit % 32));
-                if (compact && code < 3u) {
-                    const int b = 2 * factor_pair_index(e, t);
-                    c2 = (c2 & ~(3u << b)) | (code << b);
+                if (compact && code < 3u) {   // fixed slots: L entries at 0-2, the diagonal at 3, U at 4-6
+                    const int b = 2 * factor_pair_index(e < nL ? e : 3 + e - nL, t);
+                    r = (r & ~(3ULL << b)) | (static_cast<unsigned long long>(code) << b);
                 }
             }
         if (compact) {
-            // ring slots of the L entries (W, the zero slot, for absent ones), length and L count
-            unsigned sl[3];
-            for (int t = 0; t < 3; ++t)
-                sl[t] = t < nL ? slot[static_cast<size_t>(q) * 4 + t] : static_cast<unsigned>(P.W);
-            codes[q] = make_uint4(c2, sl[0] | (sl[1] << 16), sl[2] | (static_cast<unsigned>(len | (nL << 4)) << 16), 0u);
+            for (int t = 0; t < nL; ++t)
+                r |= static_cast<unsigned long long>(q - pos[JA[p0 + t]]) << (30 + 11 * t);
+            rec[q] = r;
         } else {
             // entry 0 has no (entry, L entry) pairs: its 16 code bits carry the row's length and L count
             w[0] = (w[0] & ~0xffu) | static_cast<unsigned>(len) | (static_cast<unsigned>(nL) << 4);
@@ -832,11 +832,15 @@ int build_stream_factor(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int
         }
         meta[q] = make_int4(p0, len, nL, i);
     }
-    FG_TRY(F.codes.upload(codes.data(), codes.size(), ctx->stream));
+    if (compact) {
+        FG_TRY(F.rec.upload(rec.data(), rec.size(), ctx->stream));
+    } else {
+        FG_TRY(F.codes.upload(codes.data(), codes.size(), ctx->stream));
+        FG_TRY(F.dinv.alloc(static_cast<size_t>(n) + kStreamThreads));
+    }
     FG_TRY(F.meta.upload(meta.data(), n, ctx->stream));
     FG_TRY(F.aval.alloc(static_cast<size_t>(n) * kFactorRow));
-    FG_TRY(F.out.alloc(static_cast<size_t>(n + kStreamThreads) * kFactorRow));
-    FG_TRY(F.dinv.alloc(static_cast<size_t>(n) + kStreamThreads));
+    FG_TRY(F.out.alloc(static_cast<size_t>(n + kStreamThreads) * (compact ? 4 : kFactorRow)));
     FG_HIP(hipStreamSynchronize(ctx->stream));
     F.compact = compact;
     F.ok = true;

@@ -2,7 +2,7 @@
 so these stay within seconds of host time):
   * the 1001 x 1001 transient transport system (n = 998 001, 2 000 000 elements): element matrices,
     residual and CSR values bit-identical to the oracle, from both the fused and the element-array
-    assembly; the mixed-precision solve meets the 1e-8 FP64 bar in the oracle's own operator and
+    assembly; the exact ILU(0) factors bit-identical to the oracle's; the mixed-precision solve meets the 1e-8 FP64 bar in the oracle's own operator and
     lies within 1e-6 of the oracle's FP64 ILU(0) + pgmres solution; the EBE product of the element
     matrices equals the CSR product to 1e-13;
   * the 513 x 513 Navier-Stokes pattern (n = 785 410, rows of up to 11 strict entries per triangle,
@@ -98,6 +98,11 @@ def test_transient_transport_1001(dev, torch_cuda, oracle):
     res = np.linalg.norm(R[:neq] - oracle.csrmv(c["IA"], c["JA"], AAo, Xh)) / np.linalg.norm(R[:neq])
     LU, Dinv, rc = oracle.ilu0_factor(c["IA"], c["JA"], AAo)
     assert rc == 0
+    # the exact ILU(0) on the device (compact fixed-slot rows): factors bit-identical to the oracle's
+    assert A.sweep_info()["factor_compact"] == 1
+    A.ilu0_factor()
+    LUg, Dig = A.ilu0_factors()
+    assert np.array_equal(LUg, LU[:-1]) and np.array_equal(Dig, Dinv)
     B = oracle.ilu0_apply(c["IA"], c["JA"], LU, Dinv, R[:neq])
     S = oracle.make_system(0, neq, IA=c["IA"], JA=c["JA"], AA=AAo, ilu=(c["IA"], c["JA"], LU, Dinv))
     Xo = oracle.pgmres(S, B, kmax=30, lmax=1000, tol=1e-12)[0][:neq]

@@ -308,6 +308,33 @@ def ilu_case(oracle, name):
     return steady_system(oracle, d=G.load(name))
 
 
+def test_ilu0_factor_zero_pivot(dev, torch_cuda, oracle):
+    """A zero pivot in the middle of the schedule (row n/2 with its L values and diagonal set to 0,
+    so its pivot is exactly 0): the streamed factorisation reports FEMGPU_ERR_ZERO_PIVOT where the
+    reference stops (ILUp_precond_setup.c:155-164) and the oracle returns an error, and the next
+    factorisation of regular values is bit-exact again."""
+    s = ilu_case(oracle, "m200")
+    IA, JA = s["IA"], s["JA"]
+    i = s["neq"] // 2
+    AA = s["AA"].copy()
+    for p in range(IA[i], IA[i + 1]):
+        if JA[p] <= i:
+            AA[p] = 0.0
+    _, _, rc = oracle.ilu0_factor(IA, JA, AA)
+    assert rc != 0
+    A = fem.CsrMatrix(dev, IA, JA)
+    assert A.sweep_info()["factor_compact"] == 1
+    A.set_values(AA[:-1])
+    with pytest.raises(native.FemGpuError) as ei:
+        A.ilu0_factor()
+    assert ei.value.code == native.ERR_ZERO_PIVOT
+    A.set_values(s["AA"][:-1])
+    A.ilu0_factor()
+    LUo, Dio, rc = oracle.ilu0_factor(IA, JA, s["AA"])
+    LU, Di = A.ilu0_factors()
+    assert rc == 0 and np.array_equal(LU, LUo[:-1]) and np.array_equal(Di, Dio)
+
+
 def with_extra_couplings(s, seed):
     """The system with symmetric pairs of extra entries (i, j), (j, i) added about every 37 rows, j
     a row two dependency levels after i in both triangular solves (so the level schedules keep their
