@@ -254,50 +254,59 @@ __global__ __launch_bounds__(kStreamThreads) void k_factor_fixed(FactorArgs A)
         for (int u = 0; u < D; ++u) {
             __builtin_amdgcn_sched_barrier(0);
             FixedStage &g = st[u];
-            const unsigned long long rec = static_cast<unsigned long long>(__double_as_longlong(g.v[3].y));
-            int gs[3];
-#pragma unroll
-            for (int t = 0; t < 3; ++t) {
-                const int d = fixed_dist(rec, t);
-                gs[t] = d ? ((g.q - d) & wmask) : A.W;
-            }
-            const double a[7] = {g.v[0].x, g.v[0].y, g.v[1].x, g.v[1].y, g.v[2].x, g.v[2].y, g.v[3].x};
-            // all ring reads first, then the elimination chain with selects; code 3 reads the
-            // slot's unused fourth U word, which the select drops
-            double dj[3], l[3], out[7], r[7][3];
-#pragma unroll
-            for (int t = 0; t < 3; ++t) dj[t] = ring[kFactorRing * gs[t] + kFactorU];
-#pragma unroll
-            for (int e = 1; e < 7; ++e)
-#pragma unroll
-                for (int t = 0; t < 3 && t < e; ++t)
-                    r[e][t] = ring[kFactorRing * gs[t] + static_cast<int>(fixed_code(rec, e, t))];
-#pragma unroll
-            for (int e = 0; e < 7; ++e) {
-                double acc = a[e];
-#pragma unroll
-                for (int t = 0; t < 3 && t < e; ++t) {
-                    const double upd = acc - l[t] * r[e][t];
-                    acc = fixed_code(rec, e, t) != 3u ? upd : acc;
-                }
-                if (e < 3) l[e] = acc * dj[e];
-                out[e] = acc;
-            }
             const bool on = g.q >= 0;
-            const double piv = out[3];
-            double dinv = 1.0 / piv;
-            if (piv == 0.0) {
-                dinv = 0.0;
-                if (on) __hip_atomic_store(A.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            const int self = on ? (g.q & wmask) : A.W + 1;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) ring[kFactorRing * self + k] = out[4 + k];
-            ring[kFactorRing * self + kFactorU] = dinv;
-            // pivot and U part as two 16-byte pairs (idle lanes of a wave with rows store past the n
-            // positions; a wave past the step's rows issues no stores)
-            const unsigned qo = on ? static_cast<unsigned>(g.q) : static_cast<unsigned>(A.n) + threadIdx.x;
+            // a wave past the step's rows (its lane 0 idle) computes, writes and stores nothing;
+            // idle lanes of a wave with rows write to slot W + 1 and store past the n positions
             if (__builtin_amdgcn_readfirstlane(on ? 1 : 0)) {
+                const unsigned long long rec = static_cast<unsigned long long>(__double_as_longlong(g.v[3].y));
+                const unsigned lo = static_cast<unsigned>(rec), hi = static_cast<unsigned>(rec >> 32);
+                // ring offsets of the L entries' slots, 32-bit
+                unsigned base[3];
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    const unsigned d = t == 0 ? (__builtin_amdgcn_alignbit(hi, lo, 30) & 0x7ffu)
+                                              : __builtin_amdgcn_ubfe(hi, 9 + kFixedDistBits * (t - 1), kFixedDistBits);
+                    const unsigned slot = d ? (static_cast<unsigned>(g.q) - d) & static_cast<unsigned>(wmask)
+                                            : static_cast<unsigned>(A.W);
+                    base[t] = kFactorRing * slot;
+                }
+                const double a[7] = {g.v[0].x, g.v[0].y, g.v[1].x, g.v[1].y, g.v[2].x, g.v[2].y, g.v[3].x};
+                // all ring reads first, then the elimination chain with selects; code 3 reads the
+                // slot's unused fourth U word, which the select drops
+                double dj[3], l[3], out[7], r[7][3];
+                unsigned code[7][3];
+#pragma unroll
+                for (int t = 0; t < 3; ++t) dj[t] = ring[base[t] + kFactorU];
+#pragma unroll
+                for (int e = 1; e < 7; ++e)
+#pragma unroll
+                    for (int t = 0; t < 3 && t < e; ++t) {
+                        code[e][t] = __builtin_amdgcn_ubfe(lo, 2 * factor_pair_index(e, t), 2);
+                        r[e][t] = ring[base[t] + code[e][t]];
+                    }
+#pragma unroll
+                for (int e = 0; e < 7; ++e) {
+                    double acc = a[e];
+#pragma unroll
+                    for (int t = 0; t < 3 && t < e; ++t) {
+                        const double upd = acc - l[t] * r[e][t];
+                        acc = code[e][t] != 3u ? upd : acc;
+                    }
+                    if (e < 3) l[e] = acc * dj[e];
+                    out[e] = acc;
+                }
+                const double piv = out[3];
+                double dinv = 1.0 / piv;
+                if (piv == 0.0) {
+                    dinv = 0.0;
+                    if (on) __hip_atomic_store(A.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                const unsigned self = kFactorRing * static_cast<unsigned>(on ? (g.q & wmask) : A.W + 1);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) ring[self + k] = out[4 + k];
+                ring[self + kFactorU] = dinv;
+                // pivot and U part as two 16-byte pairs
+                const unsigned qo = on ? static_cast<unsigned>(g.q) : static_cast<unsigned>(A.n) + threadIdx.x;
                 at32(reinterpret_cast<double2 *>(A.out), qo) = make_double2(out[3], out[4]);
                 at32(reinterpret_cast<double2 *>(A.out + 2 * static_cast<size_t>(A.ostride)), qo) =
                     make_double2(out[5], out[6]);
