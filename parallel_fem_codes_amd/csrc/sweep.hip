@@ -260,9 +260,9 @@ __global__ __launch_bounds__(kPackedThreads) void k_sweep_packed(PackedArgs S, c
 // accesses would multiply the cache lines it has to move. The row data of a step (padded factor
 // entries, ring slots, inverse pivot, right-hand side) does not depend on any result and is fetched
 // into registers D steps ahead; those loads and the result stores stay in flight across the
-// LDS-only barriers. Every thread issues the same loads and stores on every step (idle threads read
-// position 0 and store to a sink), so the compiler's wait counts see one path and wait only for the
-// step being computed.
+// LDS-only barriers. Every thread issues the same loads on every step (idle threads read position 0),
+// so the compiler's wait counts see one path and wait only for the step being computed; idle lanes
+// of a wavefront with rows store to a sink, and a wavefront without rows skips its one store.
 
 struct StreamArgs {
     int nsteps;          // kStreamPad empty steps, the schedule, empty steps up to a multiple of
@@ -375,7 +375,8 @@ __global__ __launch_bounds__(kStreamThreads) void k_sweep_stream(StreamArgs a, c
             const bool on = g.q >= 0;
             ring[on ? (g.q & wmask) : a.W + 1] = acc;
             if (LONG) ring[g.ks] = acc;   // read again beyond the ring's reach (else the idle slot)
-            *(on ? x + g.q : sink) = acc;
+            // a wave past the step's rows (its lane 0 idle) stores nothing
+            if (__builtin_amdgcn_readfirstlane(on ? 1 : 0)) *(on ? x + g.q : sink) = acc;
             stream_fetch<T, UPPER, K, LONG>(g, a, sq, s0 + u + D, val, dinv, b);
             lds_barrier();
         }
