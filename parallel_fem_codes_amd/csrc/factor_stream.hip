@@ -357,49 +357,59 @@ __global__ void k_factor_scatter(int n, int ostride, const int4 *__restrict__ me
 
 // Compact rows: the L multipliers l_e = (a_e - sum over L entries t < e with a U column of
 // l_t * u_t) * Dinv[j_e], formed from the pivots and U parts k_factor_fixed wrote, in the order
-// k_factor_fixed forms them (so bit for bit its values), then the whole row into A's layout and the
-// inverse pivot by row. One thread per schedule position.
+// k_factor_fixed forms them (so bit for bit its values), then the row into A's layout and the
+// inverse pivot by row. kFactorRow threads per schedule position, thread k writing entry k (k = 7:
+// the inverse pivot), so a row's stores are contiguous; each forms the row's multipliers itself.
 __global__ void k_factor_finish(int n, int ostride, const int4 *__restrict__ meta,
                                 const unsigned long long *__restrict__ rec, const double *__restrict__ AA,
                                 const double *__restrict__ out, double *__restrict__ LU, double *__restrict__ Dinv)
 {
 #pragma clang fp contract(off)
-    const int q = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (q >= n) return;
+    const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (k >= static_cast<size_t>(n) * kFactorRow) return;
+    const int q = static_cast<int>(k / kFactorRow), e = static_cast<int>(k % kFactorRow);
     const int4 m = meta[q];   // {first entry, length, L count, row}
     const unsigned long long rc = rec[q];
     const double2 *o0 = reinterpret_cast<const double2 *>(out);
     const double2 *o1 = reinterpret_cast<const double2 *>(out + 2 * static_cast<size_t>(ostride));
-    // every load unconditional (an absent L entry reads the row itself), so no branch splits them
-    double2 pu[3], uu[3];
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-        const int qd = t < m.z ? q - fixed_dist(rc, t) : q;
-        pu[t] = o0[qd];
-        uu[t] = o1[qd];
-    }
-    double l[3];
-#pragma unroll
-    for (int e = 0; e < 3; ++e) {
-        double acc = AA[m.x + (e < m.z ? e : 0)];
-#pragma unroll
-        for (int t = 0; t < e; ++t) {
-            const unsigned c = fixed_code(rc, e, t);
-            const double ut = c == 0u ? pu[t].y : c == 1u ? uu[t].x : uu[t].y;
-            const double upd = acc - l[t] * ut;
-            acc = c != 3u ? upd : acc;
-        }
-        const double dj = pu[e].x == 0.0 ? 0.0 : 1.0 / pu[e].x;
-        l[e] = acc * dj;
-        if (e < m.z) LU[m.x + e] = l[e];
-    }
     const double2 own0 = o0[q], own1 = o1[q];
-    LU[m.x + m.z] = own0.x;
-    const double us[3] = {own0.y, own1.x, own1.y};
+    if (e == 7) {
+        Dinv[m.w] = own0.x == 0.0 ? 0.0 : 1.0 / own0.x;
+        return;
+    }
+    if (e >= m.y) return;
+    double v;
+    if (e == m.z) {
+        v = own0.x;
+    } else if (e > m.z) {
+        const int u = e - m.z - 1;
+        v = u == 0 ? own0.y : u == 1 ? own1.x : own1.y;
+    } else {
+        // every load unconditional (an absent L entry reads the row itself), so no branch splits them
+        double2 pu[3], uu[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-        if (m.z + 1 + k < m.y) LU[m.x + m.z + 1 + k] = us[k];
-    Dinv[m.w] = own0.x == 0.0 ? 0.0 : 1.0 / own0.x;
+        for (int t = 0; t < 3; ++t) {
+            const int qd = t < m.z ? q - fixed_dist(rc, t) : q;
+            pu[t] = o0[qd];
+            uu[t] = o1[qd];
+        }
+        double l[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            double acc = AA[m.x + (f < m.z ? f : 0)];
+#pragma unroll
+            for (int t = 0; t < f; ++t) {
+                const unsigned c = fixed_code(rc, f, t);
+                const double ut = c == 0u ? pu[t].y : c == 1u ? uu[t].x : uu[t].y;
+                const double upd = acc - l[t] * ut;
+                acc = c != 3u ? upd : acc;
+            }
+            const double dj = pu[f].x == 0.0 ? 0.0 : 1.0 / pu[f].x;
+            l[f] = acc * dj;
+        }
+        v = e == 0 ? l[0] : e == 1 ? l[1] : l[2];
+    }
+    LU[m.x + e] = v;
 }
 
 int launch_stream_factor(femgpu_ctx *ctx, femgpu_csr *A)
@@ -424,8 +434,8 @@ int launch_stream_factor(femgpu_ctx *ctx, femgpu_csr *A)
     a.err = ctx->err.p;
     if (F.compact) {
         k_factor_fixed<4><<<1, kStreamThreads, factor_lds_bytes(P), ctx->stream>>>(a);
-        k_factor_finish<<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, n + kStreamThreads, F.meta.p, F.rec.p, A->AA.p,
-                                                                  F.out.p, A->LU.p, A->Dinv.p);
+        k_factor_finish<<<grid, 256, 0, ctx->stream>>>(n, n + kStreamThreads, F.meta.p, F.rec.p, A->AA.p, F.out.p,
+                                                       A->LU.p, A->Dinv.p);
     } else {
         k_factor_stream<4><<<1, kStreamThreads, factor_lds_bytes(P), ctx->stream>>>(a);
         k_factor_scatter<<<grid, 256, 0, ctx->stream>>>(n, n + kStreamThreads, F.meta.p, F.out.p, F.dinv.p, A->LU.p,
