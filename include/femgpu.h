@@ -145,7 +145,8 @@ typedef struct femgpu_sweep_info {
     int sell_column_bytes;     /* sliced-ELL products: 2 (16-bit distances from the row), 4, 0: none */
     int long_slots;            /* streamed: LDS slots for results read beyond the ring (larger of L, U) */
     int factor_compact;        /* streamed factorisation: 1 for the compact row shape (every row at most
-                                  7 entries, 3 L and 3 U), 0 for the general one */
+                                  7 entries, 3 L and 3 U), 2 for the compact shape on the pair pattern
+                                  of the structured triangle meshes, 0 for the general one */
 } femgpu_sweep_info;
 int femgpu_csr_sweep_info(femgpu_csr *A, femgpu_sweep_info *info);
 /* Select the sweep kernel: 0 = streamed when the plan allows it (default), 1 = packed. */

@@ -316,7 +316,7 @@ extern "C" int femgpu_csr_sweep_info(femgpu_csr *A, femgpu_sweep_info *info)
     info->jacobi_column_bytes = A->ell_kl > 0 ? (A->ell16 ? 2 : 4) : 0;
     info->sell_column_bytes = A->sell_ok ? (A->sell16 ? 2 : 4) : 0;
     info->long_slots = std::max(A->slo.L, A->sup.L);
-    info->factor_compact = info->factor_kernel == 0 && A->sfac.compact ? 1 : 0;
+    info->factor_compact = info->factor_kernel == 0 && A->sfac.compact ? (A->sfac.structured ? 2 : 1) : 0;
     return FEMGPU_OK;
 }
 

@@ -132,11 +132,21 @@ constexpr int kFactorRing = kFactorU + 1;
 // Compact factorisation rows (at most 7 entries, 3 L, at fixed slots: L entries 0-2, the diagonal
 // 3, U entries 4-6): index of the (slot e, L slot t) pair, t < min(3, e), among the row's 15 pairs.
 constexpr int factor_pair_index(int e, int t) { return e == 1 ? 0 : e == 2 ? 1 + t : 3 + 3 * (e - 3) + t; }
+constexpr unsigned factor_pair_bit(int e, int t) { return 1u << factor_pair_index(e, t); }
+// The pair pattern of the structured triangle meshes under SYMRCM: for every row whose L entry t is
+// present, the pairs of kStructAlways find column e in that row's U part and those of kStructNever
+// do not (the other three pairs vary). A plan whose rows all satisfy both takes the factorisation
+// variant with these pairs resolved at compile time: no select for the first, nothing for the second.
+constexpr unsigned kStructAlways = factor_pair_bit(1, 0) | factor_pair_bit(2, 0) | factor_pair_bit(3, 0) |
+                                   factor_pair_bit(3, 1) | factor_pair_bit(3, 2) | factor_pair_bit(5, 2);
+constexpr unsigned kStructNever = factor_pair_bit(2, 1) | factor_pair_bit(5, 0) | factor_pair_bit(5, 1) |
+                                  factor_pair_bit(6, 0) | factor_pair_bit(6, 1) | factor_pair_bit(6, 2);
 
 struct StreamFactor {
     bool ok = false;
     bool compact = false;    // every row has at most 7 entries, 3 of them L and 3 U, and every L
                              // entry's row lies at most 2 047 schedule positions back
+    bool structured = false; // compact, and every row fits kStructAlways / kStructNever
     DevBuf<uint4> codes;     // general, n: 32 codes of 4 bits, code (e, t) at bit 4*(4e+t)
     DevBuf<unsigned long long> rec;   // compact, n: 15 2-bit U columns of the (slot e, L slot t)
                                       // pairs, then three 11-bit distances to the L entries' rows

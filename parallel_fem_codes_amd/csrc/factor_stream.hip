@@ -233,7 +233,12 @@ __device__ __forceinline__ int fixed_dist(unsigned long long rec, int t)
     return static_cast<int>((rec >> (30 + kFixedDistBits * t)) & ((1u << kFixedDistBits) - 1u));
 }
 
-template <int D>
+// ALWAYS / NEVER: pair positions where every present L entry's row has / lacks the column of a
+// present entry e (the plan has checked every row): the first subtract without a select, the second
+// are not formed. Pairs of absent entries stay exact no-ops without the select: an absent L entry t
+// has l = +0 and reads the zero slot (+0 * +0 subtracts +0), and an absent slot e starts at +0 and
+// reads word 3 of its L entries' slots, which every row sets to +0 (+0 - (+-0) = +0).
+template <int D, unsigned ALWAYS, unsigned NEVER>
 __global__ __launch_bounds__(kStreamThreads) void k_factor_fixed(FactorArgs A)
 {
 #pragma clang fp contract(off)
@@ -281,6 +286,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_factor_fixed(FactorArgs A)
                 for (int e = 1; e < 7; ++e)
 #pragma unroll
                     for (int t = 0; t < 3 && t < e; ++t) {
+                        if ((NEVER >> factor_pair_index(e, t)) & 1u) continue;
                         code[e][t] = __builtin_amdgcn_ubfe(lo, 2 * factor_pair_index(e, t), 2);
                         r[e][t] = ring[base[t] + code[e][t]];
                     }
@@ -289,8 +295,10 @@ __global__ __launch_bounds__(kStreamThreads) void k_factor_fixed(FactorArgs A)
                     double acc = a[e];
 #pragma unroll
                     for (int t = 0; t < 3 && t < e; ++t) {
+                        const unsigned bit = 1u << factor_pair_index(e, t);
+                        if (NEVER & bit) continue;
                         const double upd = acc - l[t] * r[e][t];
-                        acc = code[e][t] != 3u ? upd : acc;
+                        acc = (ALWAYS & bit) || code[e][t] != 3u ? upd : acc;
                     }
                     if (e < 3) l[e] = acc * dj[e];
                     out[e] = acc;
@@ -302,8 +310,12 @@ __global__ __launch_bounds__(kStreamThreads) void k_factor_fixed(FactorArgs A)
                     if (on) __hip_atomic_store(A.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 const unsigned self = kFactorRing * static_cast<unsigned>(on ? (g.q & wmask) : A.W + 1);
+                // word 3 (no U entry) holds +0: a padding slot e's pair with a present L entry t
+                // reads it (code 3) and, where ALWAYS drops the select, subtracts l_t * (+0) from
+                // the padding value +0, which stays +0, so no stale LDS word reaches the results
 #pragma unroll
                 for (int k = 0; k < 3; ++k) ring[self + k] = out[4 + k];
+                ring[self + 3] = 0.0;
                 ring[self + kFactorU] = dinv;
                 // pivot and U part as two 16-byte pairs
                 const unsigned qo = on ? static_cast<unsigned>(g.q) : static_cast<unsigned>(A.n) + threadIdx.x;
@@ -433,7 +445,10 @@ int launch_stream_factor(femgpu_ctx *ctx, femgpu_csr *A)
     a.ostride = n + kStreamThreads;
     a.err = ctx->err.p;
     if (F.compact) {
-        k_factor_fixed<4><<<1, kStreamThreads, factor_lds_bytes(P), ctx->stream>>>(a);
+        if (F.structured)
+            k_factor_fixed<4, kStructAlways, kStructNever><<<1, kStreamThreads, factor_lds_bytes(P), ctx->stream>>>(a);
+        else
+            k_factor_fixed<4, 0u, 0u><<<1, kStreamThreads, factor_lds_bytes(P), ctx->stream>>>(a);
         k_factor_finish<<<grid, 256, 0, ctx->stream>>>(n, n + kStreamThreads, F.meta.p, F.rec.p, A->AA.p, F.out.p,
                                                        A->LU.p, A->Dinv.p);
     } else {

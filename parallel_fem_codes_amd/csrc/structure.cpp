@@ -804,6 +804,7 @@ int build_stream_factor(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int
     std::vector<uint4> codes(compact ? 0 : n);
     std::vector<unsigned long long> rec(compact ? n : 0);
     std::vector<int4> meta(n);
+    unsigned seen_valid = 0, seen_none = 0;   // compact: pair positions seen with / without a U column
     for (int q = 0; q < n; ++q) {
         const int i = row_of[q], p0 = IA[i], len = IA[i + 1] - p0, nL = diag[i] - p0;
         unsigned w[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
@@ -816,9 +817,14 @@ int build_stream_factor(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int
                     if (JA[diag[j] + 1 + u] == col) code = static_cast<unsigned>(u);
                 const int bit = 4 * (4 * e + t);
                 w[bit / 32] = (w[bit / 32] & ~(15u << (bit % 32))) | (code << (bit % 32));
-                if (compact && code < 3u) {   // fixed slots: L entries at 0-2, the diagonal at 3, U at 4-6
-                    const int b = 2 * factor_pair_index(e < nL ? e : 3 + e - nL, t);
-                    r = (r & ~(3ULL << b)) | (static_cast<unsigned long long>(code) << b);
+                if (compact) {   // fixed slots: L entries at 0-2, the diagonal at 3, U at 4-6
+                    const int k = factor_pair_index(e < nL ? e : 3 + e - nL, t);
+                    if (code < 3u) {
+                        r = (r & ~(3ULL << (2 * k))) | (static_cast<unsigned long long>(code) << (2 * k));
+                        seen_valid |= 1u << k;
+                    } else {
+                        seen_none |= 1u << k;
+                    }
                 }
             }
         if (compact) {
@@ -843,6 +849,7 @@ int build_stream_factor(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int
     FG_TRY(F.out.alloc(static_cast<size_t>(n + kStreamThreads) * (compact ? 4 : kFactorRow)));
     FG_HIP(hipStreamSynchronize(ctx->stream));
     F.compact = compact;
+    F.structured = compact && (seen_none & kStructAlways) == 0 && (seen_valid & kStructNever) == 0;
     F.ok = true;
     return FEMGPU_OK;
 }

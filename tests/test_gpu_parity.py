@@ -323,7 +323,7 @@ def test_ilu0_factor_zero_pivot(dev, torch_cuda, oracle):
     _, _, rc = oracle.ilu0_factor(IA, JA, AA)
     assert rc != 0
     A = fem.CsrMatrix(dev, IA, JA)
-    assert A.sweep_info()["factor_compact"] == 1
+    assert A.sweep_info()["factor_compact"] == 2
     A.set_values(AA[:-1])
     with pytest.raises(native.FemGpuError) as ei:
         A.ilu0_factor()
@@ -409,8 +409,11 @@ def test_ilu0_factor_and_sweeps_bit_exact(dev, torch_cuda, oracle, name, kernel)
             assert info["long_slots"] == 0
     else:
         assert info["kernel"] == 0 and info["entries_per_row"] == 4 and info["factor_kernel"] == 0
-        # the 7-point stencil takes the compact factorisation rows, the extra couplings the general ones
-        assert info["factor_compact"] == (0 if name == "m200x" else 1)
+        # the 7-point stencil takes the compact factorisation rows, the extra couplings the general
+        # ones; under SYMRCM the structured meshes' rows fit the compile-time pair pattern (2), in
+        # natural order they do not (1)
+        natural = name in ("m600not", "st_csr_not_lin_r16x9")
+        assert info["factor_compact"] == (0 if name == "m200x" else 1 if natural else 2)
         if name == "m600not":
             assert info["steps_lower"] > info["depth_lower"]
     A.set_values(s["AA"][:-1])
