@@ -223,6 +223,8 @@ struct femgpu_csr {
     fg::DevBuf<unsigned char> par_ucode;   // dest entry | (L entry t) << 4; 0xff: padding
     fg::DevBuf<int> par_rinfo;          // row -> nL | len << 8
     fg::DevBuf<int> par_dpos;           // row -> SELL position of its diagonal
+    fg::DevBuf<int> par_pcol;           // SELL position -> SELL position of its column's diagonal
+                                        // (padding: the row's own diagonal)
     fg::DevBuf<float> par_a, par_b;     // iterates in SELL layout
     bool lu32_in_sell = false;          // the current FP32 factors are in par_a (LU32 not written)
     fg::DevBuf<double> LU, Dinv;    // ILU(0) factors in A's layout (FP32 copies live in the packed plans)

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_parilu_sweep_sell(int n, const int *__r
                                                            const int *__restrict__ rinfo,
                                                            const int *__restrict__ uptr, const int *__restrict__ usrc,
                                                            const unsigned char *__restrict__ ucode,
-                                                           const int *__restrict__ dpos,
+                                                           const int *__restrict__ pcol,
                                                            const T *__restrict__ LUo, const T *__restrict__ Dio,
                                                            T *__restrict__ LUn, T *__restrict__ Din,
                                                            const int *__restrict__ sl, const int *__restrict__ su,
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void k_parilu_sweep_sell(int n, const int *__r
 #pragma unroll
         for (int k = 0; k < MAXW; ++k) {
             const int e = p0 + ((k < w ? k : 0) << 6);
-            const T piv = AA[dpos[col[e]]];
+            const T piv = AA[pcol[e]];
             lo[k] = k < nL ? v[k] / piv : v[k];
             dcol[k] = T(1) / piv;
         }
@@ -262,7 +262,7 @@ static void parilu_sweeps(femgpu_ctx *ctx, const femgpu_csr *A, int sweeps, cons
 #define FG_PARILU_SWEEP(F, E)                                                                                         \
     k_parilu_sweep_sell<T, MAXW, F, E><<<grid, 256, 0, ctx->stream>>>(                                                 \
         n, A->sell_ptr.p, A->sell_col.p, val, A->par_rinfo.p, A->par_uptr.p, A->par_usrc.p, A->par_ucode.p,            \
-        A->par_dpos.p, Lo, Do, Ln, Dn, A->ell_sl.p, A->ell_su.p, vl, vu)
+        A->par_pcol.p, Lo, Do, Ln, Dn, A->ell_sl.p, A->ell_su.p, vl, vu)
         if (first && last) FG_PARILU_SWEEP(true, true);
         else if (first) FG_PARILU_SWEEP(true, false);
         else if (last) FG_PARILU_SWEEP(false, true);
@@ -358,6 +358,15 @@ int build_parilu_sell(femgpu_ctx *ctx, femgpu_csr *A, const int *IA, const int *
     FG_TRY(A->par_ucode.upload(ucode.data(), ucode.size(), ctx->stream));
     FG_TRY(A->par_rinfo.upload(rinfo.data(), rinfo.size(), ctx->stream));
     FG_TRY(A->par_dpos.upload(dpos.data(), dpos.size(), ctx->stream));
+    {   // the diagonal position of each entry's column: one gather less in the first sweep
+        std::vector<int> pcol(tot, dpos[0]);
+        for (int i = 0; i < n; ++i) {
+            const int s = i >> 6, w = (sp[s + 1] - sp[s]) / 64, len = IA[i + 1] - IA[i];
+            for (int k = 0; k < w; ++k) pcol[pos(i, k)] = k < len ? dpos[JA[IA[i] + k]] : dpos[i];
+        }
+        FG_TRY(A->par_pcol.upload(pcol.data(), pcol.size(), ctx->stream));
+        FG_HIP(hipStreamSynchronize(ctx->stream));   // pcol ends here
+    }
     FG_TRY(A->par_a.alloc(tot));
     FG_TRY(A->par_b.alloc(tot));
     FG_HIP(hipStreamSynchronize(ctx->stream));   // the host arrays end with this call
