@@ -266,6 +266,34 @@ def test_ebemv_int32_columns_and_uncovered_unknowns(dev, torch_cuda, oracle):
     assert np.array_equal(H(q), oracle.ebemv_block(lm2, A, p, big, 3))
 
 
+def test_ebemv_unstructured_many_incidences(dev, torch_cuda, oracle):
+    """The 3x3 product on a Delaunay mesh of random points, where nodes have up to about ten
+    triangles: incidences are loaded six at a time, so such nodes take a second, partly clamped
+    batch. Bit-identical to the oracle's element loop; the FP32 copy within FP32 rounding."""
+    from scipy.spatial import Delaunay
+    torch = torch_cuda
+    rng = np.random.default_rng(17)
+    pts = rng.uniform(0.0, 1.0, (3000, 2))
+    ien = Delaunay(pts).simplices.astype(np.int32).ravel()
+    t = np.ones(len(pts), np.int32)
+    t[rng.choice(len(pts), 40, replace=False)] = 0     # some prescribed nodes (the Dirichlet sink)
+    invP, neq = oracle.number_unknowns(t)
+    lm = oracle.fill_lm(ien, t, invP, neq)
+    nel = len(ien) // 3
+    A = rng.standard_normal(nel * 9)
+    E = fem.EbeMatrix(dev, lm, 3, neq)
+    assert E.info.column_bytes == 2 and E.info.max_incidences > 6
+    E.set_values(A)
+    p = rng.standard_normal(neq)
+    q = torch.full((neq,), np.nan, dtype=torch.float64, device="cuda")
+    E.ebemv(T(torch, p), q)
+    ref = oracle.ebemv_block(lm, A, p, neq, 3)
+    assert np.array_equal(H(q), ref)
+    q32 = torch.empty(neq, dtype=torch.float32, device="cuda")
+    E.ebemv(T(torch, p, torch.float32), q32)
+    assert rel(H(q32).astype(np.float64), ref) < 2e-6
+
+
 # ----------------------------------------------------------------------------- ILU(0)
 
 def synthetic_csr(n, maxdeg, seed):
